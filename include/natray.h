@@ -1,0 +1,276 @@
+/* natray.h — C ABI of the MI355X (gfx950) water substep library, libnatray.so.
+ *
+ * The library replaces the GPU half of natray's `Sim` (src/sim.rs): the buffers that
+ * `Sim::new` allocates (src/sim.rs:237-315), the kernel plan (src/sim.rs:390-426) and the
+ * encode/submit/readback of `Sim::run` (src/sim.rs:585-648). The CPU half of `Sim::new`
+ * (scene build, domain layout, Params packing, src/sim.rs:208-236,428-441) stays with the
+ * caller, which hands the packed `natray_params` and the packed particle records over here.
+ *
+ * Layouts are byte-identical to the reference:
+ *   natray_params   == Params    (src/sim.rs:49-64,   shaders/sim_defs.wgsl:8-21)   192 B
+ *   natray_state    == State     (src/sim.rs:66-94,   shaders/sim_defs.wgsl:24-58)   96 B
+ *   natray_particle == Particle  (shaders/sim_defs.wgsl:61, packed at src/sim.rs:227-237) 80 B
+ *   acc / cf        == field-major grid sums and solver fields (shaders/sim_defs.wgsl:105-148)
+ *
+ * Conventions: every entry point returns 0 on success or a negative NATRAY_E* code and never
+ * unwinds; natray_last_error() gives a thread-local message for the last failure. A handle is
+ * not thread-safe; each handle owns its own HIP stream. All device memory is allocated in
+ * natray_sim_create and freed in natray_sim_destroy; stepping allocates nothing.
+ */
+#ifndef NATRAY_H
+#define NATRAY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants shared with the kernels (shaders/sim_defs.wgsl:98-157) ---------------- */
+
+#define NATRAY_NONE 0xffffffffu       /* sim_defs.wgsl:98  */
+#define NATRAY_PENDING 0xfffffffeu    /* sim_defs.wgsl:100 */
+#define NATRAY_FLT_MAX_BITS 0x7f7fffffu
+
+/* Transfer sums, ACC_FIELDS per cell, field-major (sim_defs.wgsl:110-125). */
+#define NATRAY_A_MOM(d) (2u * (d))
+#define NATRAY_A_WT(d) (2u * (d) + 1u)
+#define NATRAY_A_WC 6u
+#define NATRAY_A_PC 7u
+#define NATRAY_A_N 8u
+#define NATRAY_A_MF 9u
+#define NATRAY_A_MA 10u
+#define NATRAY_A_BU 11u
+#define NATRAY_A_SRC 14u
+#define NATRAY_A_MW 15u
+#define NATRAY_A_MV 16u
+#define NATRAY_ACC_FIELDS 19u
+
+/* Fixed-point scales (sim_defs.wgsl:129-136). */
+#define NATRAY_FP_W 16777216.0f
+#define NATRAY_FP_M 65536.0f
+#define NATRAY_FP_P 65536.0f
+#define NATRAY_FP_F 4194304.0f
+#define NATRAY_FP_A 65536.0f
+#define NATRAY_FP_Q 16777216.0f
+
+/* Solver fields, CELL_FIELDS per cell, field-major (sim_defs.wgsl:140-148; the velocity
+ * extension reuses 4..7, fluid.wgsl:604-605). */
+#define NATRAY_F_U 0u
+#define NATRAY_F_P 3u
+#define NATRAY_F_R 4u
+#define NATRAY_F_Z 5u
+#define NATRAY_F_D 6u
+#define NATRAY_F_Q 8u
+#define NATRAY_F_DIAG 9u
+#define NATRAY_F_TYPE 10u
+#define NATRAY_F_EXT 4u
+#define NATRAY_F_EXT_SET 7u
+#define NATRAY_CELL_FIELDS 11u
+
+/* Cell types stored as f32 in F_TYPE (sim_defs.wgsl:153-157). */
+#define NATRAY_GAS 0.0f
+#define NATRAY_WATER 1.0f
+#define NATRAY_SOLID 2.0f
+#define NATRAY_COVERED 3.0f
+#define NATRAY_RUBBER 4.0f
+
+/* Physical constants of the reference (src/sim.rs:17-20, src/balloon.rs:10, air.wgsl:24-31). */
+#define NATRAY_SOUND 1480.0f
+#define NATRAY_CFL 0.8f
+#define NATRAY_DT_MAX 1e-3f
+#define NATRAY_SOLVE_TOL 1e-5f
+#define NATRAY_MAX_ITERS 4096u
+#define NATRAY_MAX_BATCH 256u
+#define NATRAY_MAX_BLOCKS 65535u
+#define NATRAY_RHO_WATER 1000.0f
+#define NATRAY_SIGMA_W 0.072f
+#define NATRAY_P_FLOOR (2339.0f - 101325.0f)
+
+/* ---- layouts ----------------------------------------------------------------------------- */
+
+/* == Params (src/sim.rs:49-64, sim_defs.wgsl:8-21). */
+typedef struct natray_params {
+  float origin[4];    /* domain corner, w = cell size h */
+  uint32_t dims[4];   /* domain size in cells, w = particle count */
+  uint32_t bdims[4];  /* domain size in 4^3 blocks, w = block capacity */
+  float gravity[4];   /* xyz, w = sound speed in water */
+  float fluid[4];     /* x density, y CFL, z longest step (s), w particle spacing */
+  float wet[4];       /* x wet map half size (renderer side, unused here) */
+  uint32_t limits[4]; /* x wet res, y spray cap, z CG iteration budget, w stop time (ns) */
+  float solve[4];     /* x relative tolerance */
+  uint32_t mem[4];    /* x rubber verts, y rubber substeps, z air substeps, w bullets */
+  float rubber[4];
+  float rubber2[4];
+  float knot[4];      /* tie point xyz, w = neck radius */
+} natray_params;
+
+/* == State (src/sim.rs:66-94, sim_defs.wgsl:24-58). f32 members are stored as f32. */
+typedef struct natray_state {
+  uint32_t t_ns;
+  uint32_t dt_ns;
+  uint32_t step;
+  uint32_t n_blocks;
+  float max_speed;
+  uint32_t n_blocks_prev;
+  uint32_t overflow;
+  uint32_t solve_iters;
+  float rz;
+  float alpha;
+  float b_norm;
+  float r_norm;
+  uint32_t free_dt_ns;
+  uint32_t iters_max;
+  uint32_t unconverged;
+  float beta;
+  uint32_t spray_count;
+  float mem_dt;
+  float cap_dt;
+  float mem_dt_seen;
+  float dt_flow;
+  float air_dt;
+  float air_dt_seen;
+  uint32_t mem_ticket;
+} natray_state;
+
+/* == Particle (sim_defs.wgsl:60-61). pos.w carried; vel.w = alive (1/0); c0/c1/c2.xyz = APIC
+ * C rows of u/v/w; c0.w = carried gauge pressure (Pa); c1.w = thin-sheet age (s); c2.w = 0
+ * after a step (the reference keeps the in-block rank there during its sort; this library
+ * keeps ranks in a side array). */
+typedef struct natray_particle {
+  float pos[4];
+  float vel[4];
+  float c0[4];
+  float c1[4];
+  float c2[4];
+} natray_particle;
+
+/* Scene semantics beyond Params. The reference's only solids are the ground (y < 0) and
+ * the knot (fluid.wgsl:305-314); the synthetic benchmark scenes add box walls, a moving
+ * capsule solid, and a fixed spherical mask standing in for the membrane. */
+#define NATRAY_SCENE_GROUND 1u   /* cells below y = 0 are SOLID (fluid.wgsl:305-307)     */
+#define NATRAY_SCENE_KNOT 2u     /* the knot plug is SOLID (fluid.wgsl:309-314)          */
+#define NATRAY_SCENE_WALLS 4u    /* the outermost cell layer of the domain is SOLID       */
+#define NATRAY_SCENE_CAPSULE 8u  /* a capsule moving at constant velocity is SOLID        */
+#define NATRAY_SCENE_MASK 16u    /* non-water cells inside a sphere are COVERED           */
+#define NATRAY_SCENE_REFERENCE (NATRAY_SCENE_GROUND | NATRAY_SCENE_KNOT)
+
+#define NATRAY_PRECOND_BLOCK_SGS 0u /* fluid.wgsl:384-410 */
+#define NATRAY_PRECOND_JACOBI 1u
+
+#define NATRAY_CG_FP32 0u  /* reference: f32 throughout */
+#define NATRAY_CG_MIXED 1u /* f32 SpMV/axpy, f64 dots, norms, convergence and true-residual refresh */
+
+typedef struct natray_scene {
+  uint32_t flags;         /* NATRAY_SCENE_* */
+  uint32_t precond;       /* NATRAY_PRECOND_* */
+  uint32_t precision;     /* NATRAY_CG_* */
+  uint32_t refresh_every; /* mixed precision: recompute r = b - A p in f64 every this many iterations (0 = never) */
+  float fixed_dt;         /* > 0: every step is this long (s), instead of the CFL choice */
+  float mask_p;           /* gauge pressure (Pa) of COVERED mask cells */
+  uint32_t reserved0[2];
+  float capsule_a[4];     /* capsule axis start at t = 0 (xyz), w = radius */
+  float capsule_b[4];     /* capsule axis end at t = 0 (xyz) */
+  float capsule_v[4];     /* capsule velocity (xyz) */
+  float mask[4];          /* mask sphere centre (xyz), w = radius */
+  uint32_t reserved1[4];
+} natray_scene;
+
+/* ---- return codes --------------------------------------------------------------------- */
+
+#define NATRAY_OK 0
+#define NATRAY_E_OVERFLOW (-1) /* more blocks than the capacity (src/sim.rs:635 asserts) */
+#define NATRAY_E_NOMEM (-2)    /* allocation failed (src/gpu.rs:61-70 asserts) */
+#define NATRAY_E_HIP (-3)      /* a HIP runtime error */
+#define NATRAY_E_ARG (-4)      /* bad argument */
+#define NATRAY_E_SOLVER (-5)   /* the persistent solver lost its grid barrier */
+
+/* ---- phases of one step, in the reference's plan order (src/sim.rs:390-426) --------------- */
+
+#define NATRAY_PHASE_BEGIN 0   /* begin_step                       fluid.wgsl:31-89           */
+#define NATRAY_PHASE_BIN 1     /* clear/mark/scan/sort             fluid.wgsl:91-196          */
+#define NATRAY_PHASE_P2G 2     /* p2g                              fluid.wgsl:254-303         */
+#define NATRAY_PHASE_GRID 3    /* grid_prep (+ bullet_apply hook)  fluid.wgsl:319-356         */
+#define NATRAY_PHASE_SOLVE 4   /* solve_init/start + iterations    fluid.wgsl:445-578         */
+#define NATRAY_PHASE_PROJECT 5 /* project                          fluid.wgsl:580-598         */
+#define NATRAY_PHASE_EXTEND 6  /* extend_1, extend_2               fluid.wgsl:600-674         */
+#define NATRAY_PHASE_G2P 7     /* g2p                              fluid.wgsl:863-901         */
+#define NATRAY_N_PHASES 8
+
+/* Device buffers for zero-copy consumers (natray_sim_device_ptr). */
+#define NATRAY_BUF_PARTICLES 0
+#define NATRAY_BUF_STATE 1
+#define NATRAY_BUF_ACC 2
+#define NATRAY_BUF_CF 3
+#define NATRAY_BUF_BLOCK_LIST 4
+#define NATRAY_BUF_STREAM 5 /* the handle's hipStream_t */
+
+typedef struct natray_sim natray_sim;
+
+/* ---- entry points ----------------------------------------------------------------------- */
+
+/* Sizes of the ABI structs, in the order params, state, particle, scene. Touches no GPU. */
+void natray_abi_sizes(uint32_t out[4]);
+
+/* Replaces the GPU half of Sim::new (src/sim.rs:237-315,428-441). `particles` holds n packed
+ * 80-byte records (src/sim.rs:227-237); params->dims[3] must equal n and params->bdims[3] is
+ * the block capacity. `scene` may be NULL (reference semantics: ground and knot solid,
+ * block-SGS preconditioner, f32 CG). `init` may be NULL (the reference's first State,
+ * src/sim.rs:240-241: mem_dt = cap_dt = FLT_MAX, air_dt = 0.9 h / (3 c_air)). */
+int natray_sim_create(const natray_params* params, const float* particles, uint32_t n,
+                      const natray_scene* scene, const natray_state* init, natray_sim** out);
+void natray_sim_destroy(natray_sim* sim);
+
+/* == Sim::run (src/sim.rs:585-648): n_steps steps with a CG budget of `iters` iterations per
+ * step, stopping at `stop_ns`; reads the State back into `out` (may be NULL). Returns
+ * NATRAY_E_OVERFLOW where the reference asserts. */
+int natray_sim_run(natray_sim* sim, uint32_t n_steps, uint32_t iters, uint32_t stop_ns, natray_state* out);
+
+/* == Sim::advance (src/sim.rs:498-522), with the reference's batch sizing and CG budget
+ * adaptation (src/sim.rs:636-645). Steps taken go to *taken. The bullet handoff time t0 of
+ * the reference is 0 here (no exterior flight). */
+int natray_sim_advance(natray_sim* sim, double t_target, uint32_t budget, uint32_t* taken);
+
+/* One step with the current budget and no stop time. */
+int natray_sim_step(natray_sim* sim, natray_state* out);
+
+/* One phase of a step on the handle's stream (for parity tests). */
+int natray_sim_phase(natray_sim* sim, int phase);
+
+/* Host <-> device copies (synchronous on the handle's stream). */
+int natray_sim_read_state(natray_sim* sim, natray_state* out);
+int natray_sim_write_state(natray_sim* sim, const natray_state* in);
+int natray_sim_read_particles(natray_sim* sim, float* dst);
+int natray_sim_write_particles(natray_sim* sim, const float* src);
+int natray_sim_write_params(natray_sim* sim, const natray_params* params);
+int natray_sim_read_params(natray_sim* sim, natray_params* out);
+
+/* The grid of the blocks in use: *n_blocks, block_list[n], block_nb[27 n], block_start[n+1],
+ * acc[19][64 n] and cf[11][64 n] (field-major, compacted to the blocks in use). Any output
+ * pointer may be NULL. Call with everything NULL first to learn n_blocks. */
+int natray_sim_read_grid(natray_sim* sim, uint32_t* n_blocks, uint32_t* block_list, uint32_t* block_nb,
+                         uint32_t* block_start, int32_t* acc, float* cf);
+/* Overwrites acc and cf of the blocks in use with the same compacted layout (parity tests
+ * feed a phase from identical grid state). */
+int natray_sim_write_grid(natray_sim* sim, const int32_t* acc, const float* cf);
+/* block_ptr[S] (S = product of bdims) and sorted[N]. */
+int natray_sim_read_bins(natray_sim* sim, uint32_t* block_ptr, uint32_t* sorted);
+
+void* natray_sim_device_ptr(natray_sim* sim, int which);
+
+/* Per-phase GPU timing with HIP events on the handle's stream (the analogue of NATRAY_PROF,
+ * src/sim.rs:594-632). ms[NATRAY_N_PHASES] gets the summed milliseconds since profiling was
+ * switched on, counts[] the launches. */
+int natray_sim_set_profiling(natray_sim* sim, int on);
+int natray_sim_phase_times(natray_sim* sim, double* ms, uint32_t* counts);
+
+/* CG statistics since creation: steps solved, total iterations, total water-cell-iterations. */
+int natray_sim_cg_stats(natray_sim* sim, uint64_t out[3]);
+
+const char* natray_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NATRAY_H */

@@ -1,0 +1,76 @@
+// Device-side view of one simulation: pointers into the buffers natray_sim_create allocates.
+// Shared by the kernels (natray_kernels.hip) and the host library (natray_sim.cpp); the struct
+// is passed to every kernel by value.
+#pragma once
+#include <stdint.h>
+
+#include "natray.h"
+
+namespace natray {
+
+// A particle's own cell packed as x | y << 10 | z << 20 (domain sides up to 1024 cells), or
+// NATRAY_NONE for a dead particle or one outside the domain. The G2P kernel writes it after
+// advecting, so binning reads 4 bytes per particle instead of the 80-byte record.
+constexpr uint32_t KEY_BITS = 10;
+constexpr uint32_t KEY_MASK = (1u << KEY_BITS) - 1u;
+constexpr uint32_t MAX_DIM = 1u << KEY_BITS;
+
+// Binning scans the domain slots in tiles of this many.
+constexpr uint32_t SCAN_TILE = 1024;
+
+// Solver reductions fold blocks in chunks so there are at most this many partial sums.
+constexpr uint32_t MAX_CHUNKS = 1024;
+
+// Acc fields the transfer writes and clearing must zero. Fields 9..18 belong to the membrane,
+// air and bullet producers, which are not part of this library, and stay zero.
+constexpr uint32_t ACC_CLEARED = 9;
+
+struct Dev {
+  const natray_params* U;  // device copy of Params (the reference's uniform)
+  const natray_scene* sc;  // device copy of the scene
+  natray_state* st;        // State
+  float* P;                // particles, N x 20
+  uint32_t* key;           // N: own cell of each particle, see KEY_BITS
+  uint32_t* rank;          // N: place among the particles of its block
+  uint32_t* block_ptr;     // Spad: block index of each domain slot, or NONE
+  uint32_t* slot_count;    // Spad: particles whose own block is the slot
+  uint32_t* block_list;    // cap: slot of each block in use
+  uint32_t* block_nb;      // cap x 27: neighbour blocks (itself in the middle)
+  uint32_t* block_start;   // cap + 1: first entry of each block in `sorted`
+  uint32_t* sorted;        // N: particle indices grouped by block
+  int32_t* acc;            // 19 x capc fixed-point transfer sums
+  float* cf;               // 11 x capc solver fields
+  float* bvec;             // capc: CG right-hand side (true-residual refresh)
+  uint32_t* nbm;           // capc: bit k = face neighbour k is water, bit 6 = the cell is water
+  double* part;            // cap x 4: per-block partial sums of the solver
+  double* chunk;           // 2 x MAX_CHUNKS x 4: chunk sums of those, double-buffered
+  uint32_t* tile_sum;      // 2 x ntiles: scan tile totals (slots claimed, particles)
+  uint32_t* tile_off;      // 2 x ntiles: their exclusive prefix
+  uint32_t* bar;           // grid barrier: [0] arrivals, [1] generation, [2] abort flag
+  uint32_t* wcount;        // water cells of the current step
+  double* solver;          // double copies of rz, alpha, beta, b_norm, r_norm
+  unsigned long long* stats;  // CG: steps solved, iterations, water-cell-iterations
+  uint32_t N;
+  uint32_t S;      // domain slots (product of bdims)
+  uint32_t Spad;   // S rounded up to SCAN_TILE
+  uint32_t ntiles; // Spad / SCAN_TILE
+  uint32_t cap;    // block capacity (bdims.w)
+  uint32_t capc;   // cap x 64 cells per field
+  uint32_t fixed_ns;  // fixed step in ns, or 0
+};
+
+}  // namespace natray
+
+// Launch wrappers (natray_kernels.hip). Each returns the hipError_t of its launches as int.
+namespace natray {
+int launch_keys(const Dev& d, void* stream);
+int launch_begin(const Dev& d, void* stream);
+int launch_bin(const Dev& d, void* stream, int grid_cells);
+int launch_p2g(const Dev& d, void* stream, int grid_blocks);
+int launch_grid_prep(const Dev& d, void* stream, int grid_waves);
+int launch_solve(const Dev& d, void* stream, int grid, int mixed);
+int solve_max_grid(int mixed, int* per_cu);
+int launch_project(const Dev& d, void* stream, int grid_waves);
+int launch_extend(const Dev& d, void* stream, int grid_waves);
+int launch_g2p(const Dev& d, void* stream, int grid_blocks);
+}  // namespace natray

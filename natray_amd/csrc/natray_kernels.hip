@@ -1,0 +1,1289 @@
+// HIP kernels of the natray water substep for MI355X (gfx950, CDNA4, wave64).
+//
+// One step runs, in the reference's plan order (src/sim.rs:390-426):
+//   k_begin                       begin_step                 shaders/fluid.wgsl:31-89
+//   k_clear k_mark k_scan_* k_sort clear/mark/scan/sort      shaders/fluid.wgsl:91-196
+//   k_p2g                         p2g                        shaders/fluid.wgsl:198-303
+//   k_grid_prep                   grid_prep                  shaders/fluid.wgsl:305-356
+//   k_solve (cooperative)         solve_init..solve_beta     shaders/fluid.wgsl:381-578
+//   k_project                     project                    shaders/fluid.wgsl:580-598
+//   k_extend1 k_extend2           extend_1, extend_2         shaders/fluid.wgsl:600-674
+//   k_g2p                         g2p                        shaders/fluid.wgsl:767-901
+//
+// Mapping to the hardware:
+//   - per-cell kernels run one 64-lane wavefront per 4^3 block, lane = cell (l = 16x+4y+z);
+//     in-block reductions and the block-SGS preconditioner stay inside the wave (DPP/bpermute
+//     shuffles, no workgroup barriers);
+//   - P2G and G2P run one 256-thread workgroup per block with the 7^3 tile of 8 fields in LDS;
+//     P2G accumulates there with LDS integer atomics and flushes each non-zero tile sum with one
+//     global integer atomic, so grid sums are exact and order-independent;
+//   - block allocation is a deterministic compaction of claimed domain slots in slot order (a
+//     three-kernel scan), replacing the reference's CAS claims; particle ranks use wave-
+//     aggregated atomics;
+//   - the whole pressure solve is one persistent cooperative launch with a grid barrier, so the
+//     GPU decides when to stop iterating without host round trips.
+// Every float expression is written in the reference's order and compiled without
+// contraction, so the kernels reproduce the CPU oracle bit for bit given the same inputs.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <type_traits>
+
+#include "natray_dev.h"
+
+namespace natray {
+
+#define NONE NATRAY_NONE
+#define PENDING NATRAY_PENDING
+constexpr uint32_t TILE = 343;
+constexpr float GAS = NATRAY_GAS, WATER = NATRAY_WATER, SOLID = NATRAY_SOLID, COVERED = NATRAY_COVERED,
+                RUBBER = NATRAY_RUBBER;
+
+// ---- small helpers ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float fmin_(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmax_(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint32_t n_blocks(const Dev& d) {
+  uint32_t n = d.st->n_blocks, c = d.U->bdims[3];
+  return n < c ? n : c;
+}
+__device__ __forceinline__ float dt_of(const Dev& d) { return (float)d.st->dt_ns * 1e-9f; }
+__device__ __forceinline__ float now_of(const Dev& d) { return (float)d.st->t_ns * 1e-9f; }
+
+__device__ __forceinline__ uint32_t block_slot(const natray_params* U, int bx, int by, int bz) {
+  return ((uint32_t)bx * U->bdims[1] + (uint32_t)by) * U->bdims[2] + (uint32_t)bz;
+}
+__device__ __forceinline__ void slot_coord(const natray_params* U, uint32_t s, int& x, int& y, int& z) {
+  x = (int)(s / (U->bdims[2] * U->bdims[1]));
+  y = (int)((s / U->bdims[2]) % U->bdims[1]);
+  z = (int)(s % U->bdims[2]);
+}
+__device__ __forceinline__ void cell_coord(const Dev& d, uint32_t cell, int& x, int& y, int& z) {
+  slot_coord(d.U, d.block_list[cell >> 6], x, y, z);
+  uint32_t l = cell & 63u;
+  x = x * 4 + (int)(l >> 4);
+  y = y * 4 + (int)((l >> 2) & 3u);
+  z = z * 4 + (int)(l & 3u);
+}
+
+// to_fp / from_fp (shaders/sim_defs.wgsl:217-222): ties-to-even rounding of the clamped product.
+__device__ __forceinline__ int32_t to_fp(float x, float scale) {
+  float v = x * scale;
+  v = fmin_(fmax_(v, -2e9f), 2e9f);
+  return (int32_t)__builtin_rintf(v);
+}
+__device__ __forceinline__ float from_fp(int32_t x, float scale) { return (float)x / scale; }
+
+__device__ __forceinline__ float& cfr(const Dev& d, uint32_t f, uint32_t cell) { return d.cf[(size_t)f * d.capc + cell]; }
+__device__ __forceinline__ int32_t& accr(const Dev& d, uint32_t f, uint32_t cell) {
+  return d.acc[(size_t)f * d.capc + cell];
+}
+__device__ __forceinline__ float cell_type(const Dev& d, uint32_t cell) {
+  return cell == NONE ? GAS : cfr(d, NATRAY_F_TYPE, cell);
+}
+
+// The cell `o` away from local cell (lx, ly, lz) of the block whose 27 neighbours are `lk`
+// (shaders/sim_defs.wgsl:195-202); offsets up to 4 in any axis stay inside the 27 blocks.
+__device__ __forceinline__ uint32_t nbr(const uint32_t* lk, int lx, int ly, int lz, int ox, int oy, int oz) {
+  int qx = lx + ox, qy = ly + oy, qz = lz + oz;
+  int nx = (qx >= 4) - (qx < 0), ny = (qy >= 4) - (qy < 0), nz = (qz >= 4) - (qz < 0);
+  uint32_t b = lk[(nx + 1) * 9 + (ny + 1) * 3 + nz + 1];
+  if (b == NONE) return NONE;
+  return b * 64u + (uint32_t)(qx - nx * 4) * 16u + (uint32_t)(qy - ny * 4) * 4u + (uint32_t)(qz - nz * 4);
+}
+
+// ---- scene geometry (the reference's ground and knot, plus the synthetic scenes' solids) ------
+
+__device__ __forceinline__ void cell_centre(const Dev& d, int cx, int cy, int cz, float& px, float& py, float& pz) {
+  float h = d.U->origin[3];
+  px = d.U->origin[0] + ((float)cx + 0.5f) * h;
+  py = d.U->origin[1] + ((float)cy + 0.5f) * h;
+  pz = d.U->origin[2] + ((float)cz + 0.5f) * h;
+}
+__device__ __forceinline__ float capsule_d2(const Dev& d, float px, float py, float pz, float t, float* on, float* ab) {
+  const natray_scene* sc = d.sc;
+  float ax = sc->capsule_a[0] + sc->capsule_v[0] * t, ay = sc->capsule_a[1] + sc->capsule_v[1] * t,
+        az = sc->capsule_a[2] + sc->capsule_v[2] * t;
+  float bx = sc->capsule_b[0] - sc->capsule_a[0], by = sc->capsule_b[1] - sc->capsule_a[1],
+        bz = sc->capsule_b[2] - sc->capsule_a[2];
+  float ll = (bx * bx + by * by) + bz * bz;
+  float f = (((px - ax) * bx + (py - ay) * by) + (pz - az) * bz) / fmax_(ll, 1e-20f);
+  f = fmin_(fmax_(f, 0.0f), 1.0f);
+  float ox = ax + f * bx, oy = ay + f * by, oz = az + f * bz;
+  if (on) {
+    on[0] = ox; on[1] = oy; on[2] = oz;
+    ab[0] = bx; ab[1] = by; ab[2] = bz;
+  }
+  float dx = px - ox, dy = py - oy, dz = pz - oz;
+  return (dx * dx + dy * dy) + dz * dz;
+}
+__device__ __forceinline__ bool in_capsule(const Dev& d, int cx, int cy, int cz) {
+  float px, py, pz;
+  cell_centre(d, cx, cy, cz, px, py, pz);
+  float r = d.sc->capsule_a[3];
+  return capsule_d2(d, px, py, pz, now_of(d), nullptr, nullptr) < r * r;
+}
+__device__ __forceinline__ bool in_mask(const Dev& d, int cx, int cy, int cz) {
+  float px, py, pz;
+  cell_centre(d, cx, cy, cz, px, py, pz);
+  float dx = px - d.sc->mask[0], dy = py - d.sc->mask[1], dz = pz - d.sc->mask[2], r = d.sc->mask[3];
+  return (dx * dx + dy * dy) + dz * dz < r * r;
+}
+// is_ground / in_knot (shaders/fluid.wgsl:305-314) and the scene's walls and capsule.
+__device__ __forceinline__ bool is_solid_cell(const Dev& d, int cx, int cy, int cz) {
+  const uint32_t fl = d.sc->flags;
+  const natray_params* U = d.U;
+  float h = U->origin[3];
+  if ((fl & NATRAY_SCENE_GROUND) && U->origin[1] + ((float)cy + 0.5f) * h < 0.0f) return true;
+  if (fl & NATRAY_SCENE_KNOT) {
+    float px, py, pz;
+    cell_centre(d, cx, cy, cz, px, py, pz);
+    px -= U->knot[0];
+    py -= U->knot[1];
+    pz -= U->knot[2];
+    float r = U->knot[3] + 0.5f * h;
+    if (py > 0.0f && py < 2.0f * r && __builtin_sqrtf(px * px + pz * pz) < r) return true;
+  }
+  if ((fl & NATRAY_SCENE_WALLS) && (cx == 0 || cy == 0 || cz == 0 || cx == (int)U->dims[0] - 1 ||
+                                    cy == (int)U->dims[1] - 1 || cz == (int)U->dims[2] - 1))
+    return true;
+  if ((fl & NATRAY_SCENE_CAPSULE) && in_capsule(d, cx, cy, cz)) return true;
+  return false;
+}
+__device__ __forceinline__ float solid_u(const Dev& d, uint32_t cell, uint32_t dim) {
+  if (!(d.sc->flags & NATRAY_SCENE_CAPSULE) || cell == NONE) return 0.0f;
+  int x, y, z;
+  cell_coord(d, cell, x, y, z);
+  return in_capsule(d, x, y, z) ? d.sc->capsule_v[dim] : 0.0f;
+}
+// bounded (shaders/fluid.wgsl:360-365); a moving solid's velocity takes the place of 0.
+__device__ __forceinline__ float bounded(const Dev& d, float u, float hi, float lo, uint32_t hi_cell, uint32_t lo_cell,
+                                         uint32_t dim) {
+  if (lo == SOLID && hi == SOLID) return 0.0f;
+  if (lo == SOLID) return fmax_(u, solid_u(d, lo_cell, dim));
+  if (hi == SOLID) return fmin_(u, solid_u(d, hi_cell, dim));
+  return u;
+}
+// face_u (shaders/fluid.wgsl:368-371)
+__device__ __forceinline__ float face_u(const Dev& d, uint32_t cell, uint32_t lo, uint32_t dim) {
+  if (cell == NONE) return 0.0f;
+  return bounded(d, cfr(d, NATRAY_F_U + dim, cell), cfr(d, NATRAY_F_TYPE, cell), cell_type(d, lo), cell, lo, dim);
+}
+// pressure_of (shaders/fluid.wgsl:375-379)
+__device__ __forceinline__ float pressure_of(const Dev& d, uint32_t cell) {
+  if (cell == NONE) return 0.0f;
+  float p = cfr(d, NATRAY_F_P, cell);
+  return cfr(d, NATRAY_F_TYPE, cell) == WATER ? fmax_(p, NATRAY_P_FLOOR) : p;
+}
+
+__device__ __forceinline__ uint32_t pack_key(const Dev& d, float px, float py, float pz, bool alive) {
+  if (!alive) return NONE;
+  const natray_params* U = d.U;
+  float h = U->origin[3];
+  int cx = (int)floorf((px - U->origin[0]) / h), cy = (int)floorf((py - U->origin[1]) / h),
+      cz = (int)floorf((pz - U->origin[2]) / h);
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)U->dims[0] || cy >= (int)U->dims[1] || cz >= (int)U->dims[2]) return NONE;
+  return (uint32_t)cx | ((uint32_t)cy << KEY_BITS) | ((uint32_t)cz << (2 * KEY_BITS));
+}
+
+// ---- begin_step (shaders/fluid.wgsl:31-89) -----------------------------------------------------
+
+__global__ void k_begin(Dev d) {
+  if (threadIdx.x != 0) return;
+  natray_state* st = d.st;
+  const natray_params* U = d.U;
+  const natray_scene* sc = d.sc;
+  float vmax = st->max_speed;
+  uint32_t arrive = U->limits[3];
+  if (sc->flags & NATRAY_SCENE_CAPSULE) {
+    float vx = sc->capsule_v[0], vy = sc->capsule_v[1], vz = sc->capsule_v[2];
+    vmax = fmax_(vmax, __builtin_sqrtf((vx * vx + vy * vy) + vz * vz));
+  }
+  float mem_dt = st->mem_dt, air_dt = st->air_dt;
+  float dt_s = fmin_(U->fluid[2], st->cap_dt);
+  if (vmax > 0.0f) dt_s = fmin_(dt_s, U->fluid[1] * U->origin[3] / vmax);
+  st->dt_flow = dt_s;
+  st->mem_dt_seen = mem_dt;
+  st->air_dt_seen = air_dt;
+  dt_s = fmin_(dt_s, fmin_((float)U->mem[1] * mem_dt, (float)U->mem[2] * air_dt));
+  if (st->free_dt_ns > 0u) dt_s = fmin_(dt_s, 1.25f * (float)st->free_dt_ns * 1e-9f);
+  uint32_t free_ns = (uint32_t)(dt_s * 1e9f);
+  free_ns = free_ns < 20u ? 20u : free_ns;
+  if (d.fixed_ns) free_ns = d.fixed_ns;
+  uint32_t stop = U->limits[3];
+  bool live = st->t_ns < stop;
+  uint32_t dt_ns = 0;
+  if (live) {
+    uint32_t lim = (stop < arrive ? stop : arrive) - st->t_ns;
+    dt_ns = free_ns < lim ? free_ns : lim;
+    st->free_dt_ns = free_ns;
+    st->t_ns += dt_ns;
+    st->step += 1u;
+    st->max_speed = 0.0f;
+    st->mem_dt = FLT_MAX;
+    st->cap_dt = FLT_MAX;
+    st->air_dt = FLT_MAX;
+    st->mem_ticket = 0;
+    st->n_blocks_prev = n_blocks(d);
+    st->n_blocks = 0;
+    *d.wcount = 0;
+  }
+  st->dt_ns = dt_ns;
+}
+
+// Keys of particles written from the host.
+__global__ void k_keys(Dev d) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  const float4* p = reinterpret_cast<const float4*>(d.P) + (size_t)i * 5;
+  float4 pos = p[0], vel = p[1];
+  d.key[i] = pack_key(d, pos.x, pos.y, pos.z, !(vel.w < 0.5f));
+}
+
+// ---- binning (shaders/fluid.wgsl:91-196) ---------------------------------------------------------
+
+// clear_blocks (fluid.wgsl:91-107): zero the transfer sums of last step's cells. The slots'
+// block pointers and counts are reset by the scan itself.
+__global__ void k_clear(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  uint32_t n4 = d.st->n_blocks_prev * 16u;  // int4 per field
+  for (uint32_t f = 0; f < ACC_CLEARED; ++f) {
+    int4* a = reinterpret_cast<int4*>(d.acc + (size_t)f * d.capc);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) a[i] = make_int4(0, 0, 0, 0);
+  }
+}
+
+// mark_blocks + claim (fluid.wgsl:109-150): mark the blocks of cells c-1..c+2 as claimed and
+// take a rank in the own block. Lanes of a wave with the same own block share one atomic.
+__global__ __launch_bounds__(256) void k_mark(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  const natray_params* U = d.U;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t key = i < d.N ? d.key[i] : NONE;
+  bool active = key != NONE;
+  uint32_t own = 0;
+  if (active) {
+    int cx = (int)(key & KEY_MASK), cy = (int)((key >> KEY_BITS) & KEY_MASK), cz = (int)(key >> (2 * KEY_BITS));
+    int bmx = (int)U->bdims[0] - 1, bmy = (int)U->bdims[1] - 1, bmz = (int)U->bdims[2] - 1;
+    int lx = max(0, min((cx - 1) >> 2, bmx)), ly = max(0, min((cy - 1) >> 2, bmy)), lz = max(0, min((cz - 1) >> 2, bmz));
+    int hx = max(0, min((cx + 2) >> 2, bmx)), hy = max(0, min((cy + 2) >> 2, bmy)), hz = max(0, min((cz + 2) >> 2, bmz));
+    for (int x = lx; x <= hx; ++x)
+      for (int y = ly; y <= hy; ++y)
+        for (int z = lz; z <= hz; ++z) {
+          uint32_t s = block_slot(U, x, y, z);
+          if (d.block_ptr[s] != PENDING) d.block_ptr[s] = PENDING;
+        }
+    own = block_slot(U, cx >> 2, cy >> 2, cz >> 2);
+  }
+  const uint32_t lane = lane_id();
+  unsigned long long todo = __ballot(active);
+  uint32_t rank = 0;
+  while (todo) {
+    int leader = __ffsll((long long)todo) - 1;
+    uint32_t lslot = __shfl(own, leader);
+    unsigned long long peers = __ballot(active && own == lslot);
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(&d.slot_count[lslot], (uint32_t)__popcll(peers));
+    base = __shfl(base, leader);
+    if ((peers >> lane) & 1ull) rank = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    todo &= ~peers;
+  }
+  if (active) d.rank[i] = rank;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// scan_blocks, first pass: claimed slots and their particles per tile of SCAN_TILE slots.
+__global__ __launch_bounds__(256) void k_scan_reduce(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  __shared__ uint32_t ws[2][4];
+  const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * 4u;
+  uint4 bp = *reinterpret_cast<const uint4*>(d.block_ptr + base);
+  uint4 sc = *reinterpret_cast<const uint4*>(d.slot_count + base);
+  uint32_t f = (bp.x == PENDING) + (bp.y == PENDING) + (bp.z == PENDING) + (bp.w == PENDING);
+  uint32_t c = sc.x + sc.y + sc.z + sc.w;
+  f = wave_sum(f);
+  c = wave_sum(c);
+  if (lane_id() == 0) {
+    ws[0][threadIdx.x >> 6] = f;
+    ws[1][threadIdx.x >> 6] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    d.tile_sum[blockIdx.x] = ws[0][0] + ws[0][1] + ws[0][2] + ws[0][3];
+    d.tile_sum[d.ntiles + blockIdx.x] = ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3];
+  }
+}
+
+// Inclusive scan of two counters across a 64-lane wave.
+__device__ __forceinline__ void wave_scan2(uint32_t& a, uint32_t& b) {
+  const uint32_t lane = lane_id();
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t ta = __shfl_up(a, off), tb = __shfl_up(b, off);
+    if (lane >= (uint32_t)off) {
+      a += ta;
+      b += tb;
+    }
+  }
+}
+
+// scan_blocks, second pass: exclusive prefix of the tile totals (one workgroup), the block
+// count and the overflow flag (claim, fluid.wgsl:117-121).
+__global__ __launch_bounds__(1024) void k_scan_tiles(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  __shared__ uint32_t wsum[2][16];
+  const uint32_t nt = d.ntiles, t = threadIdx.x;
+  const uint32_t per = (nt + 1023u) / 1024u;
+  const uint32_t lo = min(t * per, nt), hi = min(lo + per, nt);
+  uint32_t fa = 0, ca = 0;
+  for (uint32_t j = lo; j < hi; ++j) {
+    fa += d.tile_sum[j];
+    ca += d.tile_sum[nt + j];
+  }
+  uint32_t fi = fa, ci = ca;
+  wave_scan2(fi, ci);
+  const uint32_t w = t >> 6;
+  if (lane_id() == 63) {
+    wsum[0][w] = fi;
+    wsum[1][w] = ci;
+  }
+  __syncthreads();
+  uint32_t fo = 0, co = 0;
+  for (uint32_t k = 0; k < w; ++k) {
+    fo += wsum[0][k];
+    co += wsum[1][k];
+  }
+  uint32_t frun = fo + fi - fa, crun = co + ci - ca;
+  for (uint32_t j = lo; j < hi; ++j) {
+    d.tile_off[j] = frun;
+    d.tile_off[nt + j] = crun;
+    frun += d.tile_sum[j];
+    crun += d.tile_sum[nt + j];
+  }
+  if (t == 1023) {
+    uint32_t total_f = fo + fi, total_c = co + ci;
+    uint32_t cap = d.U->bdims[3];
+    d.st->n_blocks = total_f;
+    if (total_f > cap) {
+      d.st->overflow = 1u;
+    } else {
+      d.block_start[total_f] = total_c;
+    }
+  }
+}
+
+// scan_blocks, third pass: block indices in slot order, block starts, and every slot's pointer
+// (NONE where unclaimed); resets the counts for the next step.
+__global__ __launch_bounds__(256) void k_scan_scatter(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  __shared__ uint32_t ws[2][4];
+  const uint32_t j = blockIdx.x, base = j * SCAN_TILE + threadIdx.x * 4u;
+  uint4 bp = *reinterpret_cast<const uint4*>(d.block_ptr + base);
+  uint4 sc = *reinterpret_cast<const uint4*>(d.slot_count + base);
+  uint32_t fl[4] = {bp.x == PENDING, bp.y == PENDING, bp.z == PENDING, bp.w == PENDING};
+  uint32_t cn[4] = {sc.x, sc.y, sc.z, sc.w};
+  uint32_t fa = fl[0] + fl[1] + fl[2] + fl[3], ca = cn[0] + cn[1] + cn[2] + cn[3];
+  uint32_t fi = fa, ci = ca;
+  wave_scan2(fi, ci);
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 63) {
+    ws[0][w] = fi;
+    ws[1][w] = ci;
+  }
+  __syncthreads();
+  uint32_t fo = d.tile_off[j], co = d.tile_off[d.ntiles + j];
+  for (uint32_t k = 0; k < w; ++k) {
+    fo += ws[0][k];
+    co += ws[1][k];
+  }
+  fo += fi - fa;
+  co += ci - ca;
+  const uint32_t cap = d.U->bdims[3];
+  uint32_t out[4];
+  for (int q = 0; q < 4; ++q) {
+    uint32_t s = base + (uint32_t)q;
+    out[q] = NONE;
+    if (fl[q]) {
+      if (fo < cap) {
+        d.block_list[fo] = s;
+        d.block_start[fo] = co;
+        out[q] = fo;
+      } else if (fo == cap) {
+        d.block_start[cap] = co;
+      }
+      ++fo;
+    }
+    co += cn[q];
+  }
+  *reinterpret_cast<uint4*>(d.block_ptr + base) = make_uint4(out[0], out[1], out[2], out[3]);
+  *reinterpret_cast<uint4*>(d.slot_count + base) = make_uint4(0, 0, 0, 0);
+}
+
+// sort_particles (fluid.wgsl:189-196)
+__global__ __launch_bounds__(256) void k_sort(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  uint32_t key = d.key[i];
+  if (key == NONE) return;
+  int cx = (int)(key & KEY_MASK), cy = (int)((key >> KEY_BITS) & KEY_MASK), cz = (int)(key >> (2 * KEY_BITS));
+  uint32_t b = d.block_ptr[block_slot(d.U, cx >> 2, cy >> 2, cz >> 2)];
+  if (b >= PENDING) return;
+  d.sorted[d.block_start[b] + d.rank[i]] = i;
+}
+
+// ---- p2g (shaders/fluid.wgsl:198-303) --------------------------------------------------------------
+
+// axis_weights (fluid.wgsl:221-223)
+__device__ __forceinline__ void axis_weights(float fx, float w[3]) {
+  w[0] = 0.5f * (1.5f - fx) * (1.5f - fx);
+  w[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f);
+  w[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
+}
+
+// tile_cell (fluid.wgsl:210-217)
+__device__ __forceinline__ uint32_t tile_cell(const uint32_t* links, uint32_t t) {
+  int rx = (int)(t / 49u) - 1, ry = (int)((t / 7u) % 7u) - 1, rz = (int)(t % 7u) - 1;
+  int nx = (rx + 4) / 4 - 1, ny = (ry + 4) / 4 - 1, nz = (rz + 4) / 4 - 1;
+  uint32_t b = links[(nx + 1) * 9 + (ny + 1) * 3 + nz + 1];
+  if (b == NONE) return NONE;
+  return b * 64u + (uint32_t)(rx - nx * 4) * 16u + (uint32_t)(ry - ny * 4) * 4u + (uint32_t)(rz - nz * 4);
+}
+
+// splat (fluid.wgsl:233-252) of one lattice (offset o = (OX, OY, OZ) / 2 cells) into the tile.
+template <int OX, int OY, int OZ>
+__device__ __forceinline__ void splat(int* tile, const float x[3], const int corner[3], uint32_t fa, float val,
+                                      const float grad[3], uint32_t fw, float scale, float h) {
+  const float o[3] = {0.5f * OX, 0.5f * OY, 0.5f * OZ};
+  int base[3];
+  float fx[3], hg[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    base[k] = (int)floorf(x[k] - o[k] - 0.5f);
+    fx[k] = x[k] - o[k] - (float)base[k];
+    hg[k] = grad[k] * h;
+  }
+  float wx[3], wy[3], wz[3];
+  axis_weights(fx[0], wx);
+  axis_weights(fx[1], wy);
+  axis_weights(fx[2], wz);
+  int t0 = (base[0] - corner[0]) * 49 + (base[1] - corner[1]) * 7 + (base[2] - corner[2]);
+  int* ta = tile + fa * TILE + t0;
+  int* tw = tile + fw * TILE + t0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float weight = wx[a] * wy[b] * wz[c];
+        float node = val + ((hg[0] * ((float)a - fx[0]) + hg[1] * ((float)b - fx[1])) + hg[2] * ((float)c - fx[2]));
+        atomicAdd(ta + a * 49 + b * 7 + c, to_fp(weight * node, scale));
+        atomicAdd(tw + a * 49 + b * 7 + c, to_fp(weight, NATRAY_FP_W));
+      }
+}
+
+// Neighbour links of block b (fluid.wgsl:261-270), also stored for the later kernels.
+__device__ __forceinline__ void block_links(const Dev& d, uint32_t b, uint32_t li, uint32_t* links) {
+  const natray_params* U = d.U;
+  int bx, by, bz;
+  slot_coord(U, d.block_list[b], bx, by, bz);
+  int qx = bx + (int)(li / 9u) - 1, qy = by + (int)((li / 3u) % 3u) - 1, qz = bz + (int)(li % 3u) - 1;
+  uint32_t idx = NONE;
+  if (qx >= 0 && qy >= 0 && qz >= 0 && qx < (int)U->bdims[0] && qy < (int)U->bdims[1] && qz < (int)U->bdims[2]) {
+    idx = d.block_ptr[block_slot(U, qx, qy, qz)];
+    if (idx >= PENDING) idx = NONE;
+  }
+  links[li] = idx;
+  d.block_nb[b * 27u + li] = idx;
+}
+
+__global__ __launch_bounds__(256) void k_p2g(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  __shared__ int tile[8 * TILE];
+  __shared__ uint32_t own[64];
+  __shared__ uint32_t links[27];
+  const uint32_t n = n_blocks(d), tid = threadIdx.x;
+  const natray_params* U = d.U;
+  const float h = U->origin[3];
+  for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
+    __syncthreads();
+    if (tid < 27) block_links(d, b, tid, links);
+    if (tid < 64) own[tid] = 0;
+    for (uint32_t t = tid; t < 8 * TILE; t += 256) tile[t] = 0;
+    const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
+    __syncthreads();
+    if (count == 0u) {
+      if (tid < 64) accr(d, NATRAY_A_N, b * 64u + tid) = 0;
+      continue;
+    }
+    int bx, by, bz;
+    slot_coord(U, d.block_list[b], bx, by, bz);
+    const int corner[3] = {bx * 4 - 1, by * 4 - 1, bz * 4 - 1};
+    for (uint32_t k = tid; k < count; k += 256) {
+      const float4* p = reinterpret_cast<const float4*>(d.P) + (size_t)d.sorted[first + k] * 5;
+      float4 pos = p[0], vel = p[1], c0 = p[2], c1 = p[3], c2 = p[4];
+      float x[3] = {(pos.x - U->origin[0]) / h, (pos.y - U->origin[1]) / h, (pos.z - U->origin[2]) / h};
+      int ox = (int)floorf(x[0]) - bx * 4, oy = (int)floorf(x[1]) - by * 4, oz = (int)floorf(x[2]) - bz * 4;
+      // A particle sorted into this block lies in it (its key was made from the same position).
+      if ((unsigned)ox > 3u || (unsigned)oy > 3u || (unsigned)oz > 3u) continue;
+      const float g0[3] = {c0.x, c0.y, c0.z}, g1[3] = {c1.x, c1.y, c1.z}, g2[3] = {c2.x, c2.y, c2.z};
+      const float zero[3] = {0.0f, 0.0f, 0.0f};
+      splat<0, 1, 1>(tile, x, corner, NATRAY_A_MOM(0), vel.x, g0, NATRAY_A_WT(0), NATRAY_FP_M, h);
+      splat<1, 0, 1>(tile, x, corner, NATRAY_A_MOM(1), vel.y, g1, NATRAY_A_WT(1), NATRAY_FP_M, h);
+      splat<1, 1, 0>(tile, x, corner, NATRAY_A_MOM(2), vel.z, g2, NATRAY_A_WT(2), NATRAY_FP_M, h);
+      splat<1, 1, 1>(tile, x, corner, NATRAY_A_PC, c0.w * 1e-6f, zero, NATRAY_A_WC, NATRAY_FP_P, h);
+      atomicAdd(&own[(uint32_t)ox * 16u + (uint32_t)oy * 4u + (uint32_t)oz], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < TILE; t += 256) {
+      uint32_t cell = tile_cell(links, t);
+      if (cell == NONE) continue;
+#pragma unroll
+      for (uint32_t f = 0; f < 8u; ++f) {
+        int v = tile[f * TILE + t];
+        if (v != 0) atomicAdd(&accr(d, f, cell), v);
+      }
+    }
+    if (tid < 64) accr(d, NATRAY_A_N, b * 64u + tid) = (int32_t)own[tid];
+  }
+}
+
+// ---- per-cell kernels: one wave per block --------------------------------------------------------
+
+// Runs body(b, lane, lx, ly, lz, links) for every block in use, 4 blocks per 256-thread
+// workgroup; `links` are the block's 27 neighbours in LDS.
+template <class Body>
+__device__ __forceinline__ void for_blocks(const Dev& d, uint32_t b_lo, uint32_t b_hi, uint32_t stride, Body body) {
+  __shared__ uint32_t lk[4][27];
+  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  const int lx = (int)(lane >> 4), ly = (int)((lane >> 2) & 3u), lz = (int)(lane & 3u);
+  for (uint32_t base = b_lo; base < b_hi; base += stride) {
+    const uint32_t b = base + w;
+    const bool act = b < b_hi;
+    __syncthreads();
+    if (act && lane < 27) lk[w][lane] = d.block_nb[b * 27u + lane];
+    __syncthreads();
+    if (act) body(b, lane, lx, ly, lz, lk[w]);
+  }
+}
+
+// grid_prep (fluid.wgsl:319-356)
+__global__ __launch_bounds__(256) void k_grid_prep(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  const float dt = dt_of(d);
+  const natray_params* U = d.U;
+  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u, [&](uint32_t b, uint32_t lane, int, int, int, const uint32_t*) {
+    const uint32_t cell = b * 64u + lane;
+    int cx, cy, cz;
+    cell_coord(d, cell, cx, cy, cz);
+    float ty = GAS;
+    if (is_solid_cell(d, cx, cy, cz)) {
+      ty = SOLID;
+    } else if (accr(d, NATRAY_A_N, cell) > 0) {
+      ty = WATER;
+    }
+    cfr(d, NATRAY_F_TYPE, cell) = ty;
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+      float w = from_fp(accr(d, NATRAY_A_WT(k), cell), NATRAY_FP_W);
+      float u = 0.0f;
+      if (w > 1e-6f) u = from_fp(accr(d, NATRAY_A_MOM(k), cell), NATRAY_FP_M) / w + dt * U->gravity[k];
+      cfr(d, NATRAY_F_U + k, cell) = u;
+    }
+    float wc = from_fp(accr(d, NATRAY_A_WC, cell), NATRAY_FP_W);
+    float ma = from_fp(accr(d, NATRAY_A_MA, cell), NATRAY_FP_A);
+    if (ty == GAS && accr(d, NATRAY_A_MW, cell) > 0) {
+      ty = RUBBER;
+      cfr(d, NATRAY_F_TYPE, cell) = ty;
+    }
+    float p = 0.0f;
+    if (ty == WATER && wc > 1e-6f) {
+      p = from_fp(accr(d, NATRAY_A_PC, cell), NATRAY_FP_P) / wc * 1e6f;
+    } else if (ty != WATER && ty != SOLID && ma > 0.0f) {
+      ty = COVERED;
+      cfr(d, NATRAY_F_TYPE, cell) = ty;
+      p = from_fp(accr(d, NATRAY_A_MF, cell), NATRAY_FP_F) / ma * 1e6f;
+    } else if (ty != WATER && ty != SOLID && (d.sc->flags & NATRAY_SCENE_MASK) && in_mask(d, cx, cy, cz)) {
+      ty = COVERED;
+      cfr(d, NATRAY_F_TYPE, cell) = ty;
+      p = d.sc->mask_p;
+    }
+    cfr(d, NATRAY_F_P, cell) = p;
+  });
+}
+
+// ---- pressure solve (fluid.wgsl:381-578) ------------------------------------------------------------
+
+// Grid barrier of the cooperative solve. Every workgroup releases its writes at agent scope,
+// one thread per workgroup counts in and waits for the generation to move, and everyone
+// acquires afterwards. A wait that runs past `kSpinLimit` polls (or sees another workgroup's
+// abort) sets the abort flag and returns, so no wave can wait forever.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ bool grid_sync(const Dev& d, uint32_t G) {
+  __shared__ uint32_t s_abort;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* count = d.bar;
+    uint32_t* gen = d.bar + 1;
+    uint32_t* abort = d.bar + 2;
+    uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t ab = 0;
+    if (arrived == G - 1u) {
+      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      uint32_t spins = 0;
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(2);
+        if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kSpinLimit) {
+          __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    ab = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_abort = ab;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_abort == 0u;
+}
+
+// block_sum's tree (fluid.wgsl:414-422) inside one wave; lane 0 ends with the sum.
+template <class R>
+__device__ __forceinline__ R wave_tree(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  return v;
+}
+
+// Block-local symmetric red-black Gauss-Seidel (fluid.wgsl:384-410), five half sweeps from
+// zero, inside one wave; or Jacobi.
+__device__ __forceinline__ float precondition(float r, float diag, bool water, int lx, int ly, int lz, uint32_t mode) {
+  if (mode == NATRAY_PRECOND_JACOBI) return water ? r / diag : 0.0f;
+  const uint32_t lane = lane_id();
+  const unsigned long long wm = __ballot(water);
+  const bool wxm = lx > 0 && ((wm >> (lane - 16u)) & 1ull), wxp = lx < 3 && ((wm >> (lane + 16u)) & 1ull);
+  const bool wym = ly > 0 && ((wm >> (lane - 4u)) & 1ull), wyp = ly < 3 && ((wm >> (lane + 4u)) & 1ull);
+  const bool wzm = lz > 0 && ((wm >> (lane - 1u)) & 1ull), wzp = lz < 3 && ((wm >> (lane + 1u)) & 1ull);
+  const int sxm = lx > 0 ? (int)lane - 16 : (int)lane, sxp = lx < 3 ? (int)lane + 16 : (int)lane;
+  const int sym = ly > 0 ? (int)lane - 4 : (int)lane, syp = ly < 3 ? (int)lane + 4 : (int)lane;
+  const int szm = lz > 0 ? (int)lane - 1 : (int)lane, szp = lz < 3 ? (int)lane + 1 : (int)lane;
+  const uint32_t colour = (uint32_t)(lx + ly + lz) & 1u;
+  float z = 0.0f;
+#pragma unroll
+  for (uint32_t k = 0; k < 5u; ++k) {
+    float a = __shfl(z, sxm), b = __shfl(z, sxp), c = __shfl(z, sym), e = __shfl(z, syp), f = __shfl(z, szm),
+          g = __shfl(z, szp);
+    if (water && colour == (k & 1u)) {
+      float s = r;
+      if (wxm) s += a;
+      if (wxp) s += b;
+      if (wym) s += c;
+      if (wyp) s += e;
+      if (wzm) s += f;
+      if (wzp) s += g;
+      z = s / diag;
+    }
+  }
+  return z;
+}
+
+template <class R>
+__device__ __forceinline__ R prod(float a, float b) {
+  return (R)a * (R)b;
+}
+
+// Fold this workgroup's chunks of block partials (components [0, nc)), K blocks per chunk, in
+// block order, into chunk buffer `buf`. Consecutive reductions use alternate buffers: a
+// workgroup that has read one reduction's chunk sums can only overwrite that buffer after the
+// next grid barrier, which every other reader has passed by then.
+template <class R>
+__device__ __forceinline__ void fold_chunks(const Dev& d, uint32_t buf, uint32_t c0, uint32_t c1, uint32_t K, uint32_t n,
+                                            int nc) {
+  __syncthreads();
+  double* chunk = d.chunk + (size_t)buf * MAX_CHUNKS * 4u;
+  for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+    const uint32_t lo = c * K, hi = min(n, lo + K);
+    for (int k = 0; k < nc; ++k) {
+      R acc = (R)0;
+      for (uint32_t b = lo; b < hi; ++b) acc += (R)d.part[b * 4u + (uint32_t)k];
+      chunk[c * 4u + (uint32_t)k] = (double)acc;
+    }
+  }
+}
+
+// total (fluid.wgsl:426-439) over the m chunk sums of buffer `buf`, component k; every
+// workgroup computes it.
+template <class R>
+__device__ __forceinline__ R total(const Dev& d, uint32_t buf, uint32_t m, uint32_t k, R* red) {
+  const uint32_t li = threadIdx.x;
+  const double* chunk = d.chunk + (size_t)buf * MAX_CHUNKS * 4u;
+  R s = (R)0;
+  for (uint32_t i = li; i < m; i += 256u) s += (R)chunk[i * 4u + k];
+  __syncthreads();
+  red[li] = s;
+  __syncthreads();
+  for (uint32_t off = 128; off > 0; off >>= 1) {
+    if (li < off) red[li] += red[li + off];
+    __syncthreads();
+  }
+  R r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ float sq(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sq(double x) { return __builtin_sqrt(x); }
+
+// converged (fluid.wgsl:441-443)
+template <class R>
+__device__ __forceinline__ bool converged(R rr, R tol, R b_norm) {
+  return sq(rr) <= tol * (b_norm < (R)1 ? (R)1 : b_norm);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_solve(Dev d) {
+  using R = typename std::conditional<MIXED, double, float>::type;
+  __shared__ R red[256];
+  if (d.st->dt_ns == 0u) return;
+  const natray_params* U = d.U;
+  const uint32_t mode = d.sc->precond;
+  const uint32_t n = n_blocks(d);
+  const uint32_t K = n ? (n + MAX_CHUNKS - 1u) / MAX_CHUNKS : 1u;
+  const uint32_t m = n ? (n + K - 1u) / K : 0u;
+  const uint32_t G = gridDim.x, g = blockIdx.x;
+  const uint32_t c0 = (uint32_t)((uint64_t)g * m / G), c1 = (uint32_t)((uint64_t)(g + 1) * m / G);
+  const uint32_t b0 = c0 * K, b1 = min(n, c1 * K);
+  const float h = U->origin[3], dt = dt_of(d);
+  const uint32_t capc = d.capc;
+  float* const cP = d.cf + (size_t)NATRAY_F_P * capc;
+  float* const cR = d.cf + (size_t)NATRAY_F_R * capc;
+  float* const cZ = d.cf + (size_t)NATRAY_F_Z * capc;
+  float* const cQ = d.cf + (size_t)NATRAY_F_Q * capc;
+  float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
+  uint32_t water_here = 0;
+
+  // solve_init (fluid.wgsl:450-491)
+  for_blocks(d, b0, b1, 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    const uint32_t cell = b * 64u + lane;
+    const bool water = cfr(d, NATRAY_F_TYPE, cell) == WATER;
+    float diag = 1.0f, r = 0.0f, bv = 0.0f;
+    uint32_t mask = 0;
+    if (water) {
+      float cs = U->gravity[3] * dt;
+      float kc = h * h / (cs * cs);
+      float div = 0.0f, off = 0.0f;
+      diag = kc;
+      bv = kc * cP[cell];
+#pragma unroll
+      for (uint32_t dim = 0; dim < 3; ++dim) {
+        const int ex = dim == 0, ey = dim == 1, ez = dim == 2;
+        const uint32_t up = nbr(lk, lx, ly, lz, ex, ey, ez), dn = nbr(lk, lx, ly, lz, -ex, -ey, -ez);
+        div += face_u(d, up, cell, dim) - face_u(d, cell, dn, dim);
+#pragma unroll
+        for (int sg = -1; sg <= 1; sg += 2) {
+          const uint32_t nb = sg < 0 ? dn : up;
+          const float nt = cell_type(d, nb);
+          if (nt == SOLID) continue;
+          diag += 1.0f;
+          if (nt == WATER) {
+            off -= cP[nb];
+            mask |= 1u << (dim * 2u + (sg > 0 ? 1u : 0u));
+          } else {
+            bv += pressure_of(d, nb);
+          }
+        }
+      }
+      div -= from_fp(accr(d, NATRAY_A_SRC, cell), NATRAY_FP_Q) / (h * h);
+      bv -= U->fluid[0] * h / dt * div;
+      r = bv - diag * cP[cell] - off;
+      cDiag[cell] = diag;
+      cR[cell] = r;
+      d.bvec[cell] = bv;
+      mask |= 64u;
+    }
+    d.nbm[cell] = mask;
+    const float z = precondition(r, diag, water, lx, ly, lz, mode);
+    if (water) cZ[cell] = z;
+    const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r)), v2 = wave_tree<R>(prod<R>(bv, bv));
+    const uint32_t nw = (uint32_t)__popcll(__ballot(water));
+    if (lane == 0) {
+      d.part[b * 4u + 0] = (double)v0;
+      d.part[b * 4u + 1] = (double)v1;
+      d.part[b * 4u + 2] = (double)v2;
+      water_here += nw;
+    }
+  });
+  if (lane_id() == 0 && water_here) atomicAdd(d.wcount, water_here);
+  fold_chunks<R>(d, 1, c0, c1, K, n, 3);
+  bool ok = grid_sync(d, G);
+
+  // solve_start (fluid.wgsl:493-506)
+  const R tol = (R)U->solve[0];
+  R rz = total<R>(d, 1, m, 0, red), rr = total<R>(d, 1, m, 1, red), bb = total<R>(d, 1, m, 2, red);
+  R beta = (R)0, alpha = (R)0;
+  const R b_norm = sq(bb);
+  R r_norm = sq(rr);
+  uint32_t iters = 0, iters_max = d.st->iters_max, unconverged = d.st->unconverged;
+  bool go = ok && rz > (R)0 && !converged<R>(rr, tol, b_norm) && U->limits[2] > 0u;
+  const bool refresh_on = MIXED && d.sc->precision == NATRAY_CG_MIXED && d.sc->refresh_every > 0u;
+
+  while (go) {
+    // solve_q (fluid.wgsl:508-534): the new direction and q = A d; neighbours' directions are
+    // rebuilt from z and the old direction.
+    const uint32_t old = iters % 2u;
+    const float* cDo = d.cf + (size_t)(NATRAY_F_D + old) * capc;
+    float* cDn = d.cf + (size_t)(NATRAY_F_D + 1u - old) * capc;
+    const float fb = (float)beta;
+    for_blocks(d, b0, b1, 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+      const uint32_t cell = b * 64u + lane;
+      const uint32_t mask = d.nbm[cell];
+      R v = (R)0;
+      if (mask & 64u) {
+        const float dn = cZ[cell] + fb * cDo[cell];
+        float q = cDiag[cell] * dn;
+        if (mask & 1u) { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 2u) { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 4u) { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 8u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 16u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 32u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; }
+        cDn[cell] = dn;
+        cQ[cell] = q;
+        v = prod<R>(dn, q);
+      }
+      v = wave_tree<R>(v);
+      if (lane == 0) d.part[b * 4u] = (double)v;
+    });
+    fold_chunks<R>(d, 0, c0, c1, K, n, 1);
+    if (!grid_sync(d, G)) break;
+    // solve_alpha (fluid.wgsl:536-541)
+    const R dq = total<R>(d, 0, m, 0, red);
+    alpha = dq > (R)0 ? rz / dq : (R)0;
+    const float fa = (float)alpha;
+    const bool refresh = refresh_on && (iters + 1u) % d.sc->refresh_every == 0u;
+    // solve_update (fluid.wgsl:543-559): p += alpha d, r -= alpha q, z = M^-1 r.
+    auto update = [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t*) {
+      const uint32_t cell = b * 64u + lane;
+      const bool water = (d.nbm[cell] & 64u) != 0u;
+      float r = 0.0f, diag = 1.0f;
+      if (water) {
+        cP[cell] += fa * cDn[cell];
+        r = cR[cell] - fa * cQ[cell];
+        cR[cell] = r;
+        diag = cDiag[cell];
+      }
+      if (!refresh) {
+        const float z = precondition(r, diag, water, lx, ly, lz, mode);
+        if (water) cZ[cell] = z;
+        const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+        if (lane == 0) {
+          d.part[b * 4u + 0] = (double)v0;
+          d.part[b * 4u + 1] = (double)v1;
+        }
+      }
+    };
+    for_blocks(d, b0, b1, 4u, update);
+    if (refresh) {
+      // True residual r = b - A p in double once every p is updated, then the preconditioner.
+      if (!grid_sync(d, G)) break;
+      for_blocks(d, b0, b1, 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+        const uint32_t cell = b * 64u + lane;
+        const uint32_t mask = d.nbm[cell];
+        const bool water = (mask & 64u) != 0u;
+        float r = 0.0f, diag = 1.0f;
+        if (water) {
+          diag = cDiag[cell];
+          double ap = (double)diag * (double)cP[cell];
+          if (mask & 1u) ap -= (double)cP[nbr(lk, lx, ly, lz, -1, 0, 0)];
+          if (mask & 2u) ap -= (double)cP[nbr(lk, lx, ly, lz, 1, 0, 0)];
+          if (mask & 4u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, -1, 0)];
+          if (mask & 8u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 1, 0)];
+          if (mask & 16u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, -1)];
+          if (mask & 32u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, 1)];
+          r = (float)((double)d.bvec[cell] - ap);
+          cR[cell] = r;
+        }
+        const float z = precondition(r, diag, water, lx, ly, lz, mode);
+        if (water) cZ[cell] = z;
+        const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+        if (lane == 0) {
+          d.part[b * 4u + 0] = (double)v0;
+          d.part[b * 4u + 1] = (double)v1;
+        }
+      });
+    }
+    fold_chunks<R>(d, 1, c0, c1, K, n, 2);
+    if (!grid_sync(d, G)) break;
+    // solve_beta (fluid.wgsl:561-578)
+    const R rz_new = total<R>(d, 1, m, 0, red), rr_new = total<R>(d, 1, m, 1, red);
+    iters += 1u;
+    r_norm = sq(rr_new);
+    beta = rz > (R)0 ? rz_new / rz : (R)0;
+    rz = rz_new;
+    const bool done = converged<R>(rr_new, tol, b_norm);
+    if (done || iters >= U->limits[2]) {
+      iters_max = iters > iters_max ? iters : iters_max;
+      unconverged += done ? 0u : 1u;
+      go = false;
+    }
+  }
+  if (g == 0 && threadIdx.x == 0) {
+    natray_state* st = d.st;
+    st->rz = (float)rz;
+    st->alpha = (float)alpha;
+    st->beta = (float)beta;
+    st->b_norm = (float)b_norm;
+    st->r_norm = (float)r_norm;
+    st->solve_iters = iters;
+    st->iters_max = iters_max;
+    st->unconverged = unconverged;
+    d.solver[0] = (double)rz;
+    d.solver[1] = (double)alpha;
+    d.solver[2] = (double)beta;
+    d.solver[3] = (double)b_norm;
+    d.solver[4] = (double)r_norm;
+    const uint32_t w = __hip_atomic_load(d.wcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    d.stats[0] += 1ull;
+    d.stats[1] += iters;
+    d.stats[2] += (unsigned long long)w * iters;
+  }
+}
+
+// ---- project, extend_1, extend_2 (fluid.wgsl:580-674) ------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_project(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  const float k = dt_of(d) / (d.U->fluid[0] * d.U->origin[3]);
+  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u,
+             [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+               const uint32_t cell = b * 64u + lane;
+               const float ty = cfr(d, NATRAY_F_TYPE, cell);
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim) {
+                 const uint32_t nb = nbr(lk, lx, ly, lz, -(dim == 0), -(dim == 1), -(dim == 2));
+                 const float nt = cell_type(d, nb);
+                 float u = bounded(d, cfr(d, NATRAY_F_U + dim, cell), ty, nt, cell, nb, dim);
+                 if (ty != SOLID && nt != SOLID && (ty == WATER || nt == WATER))
+                   u -= k * (pressure_of(d, cell) - pressure_of(d, nb));
+                 cfr(d, NATRAY_F_U + dim, cell) = u;
+               }
+             });
+}
+
+// face_set (fluid.wgsl:607-612) of the cell at offset (ox, oy, oz), for face dim.
+__device__ __forceinline__ bool face_set_at(const Dev& d, const uint32_t* lk, int lx, int ly, int lz, int ox, int oy, int oz,
+                                            uint32_t dim) {
+  const uint32_t c = nbr(lk, lx, ly, lz, ox, oy, oz);
+  if (c == NONE) return false;
+  if (cfr(d, NATRAY_F_TYPE, c) == WATER) return true;
+  return cell_type(d, nbr(lk, lx, ly, lz, ox - (dim == 0), oy - (dim == 1), oz - (dim == 2))) == WATER;
+}
+
+__global__ __launch_bounds__(256) void k_extend1(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u,
+             [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+               const uint32_t cell = b * 64u + lane;
+               uint32_t done = 0;
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim) {
+                 float u = cfr(d, NATRAY_F_U + dim, cell);
+                 if (face_set_at(d, lk, lx, ly, lz, 0, 0, 0, dim)) {
+                   done |= 1u << dim;
+                 } else {
+                   float sum = 0.0f, cnt = 0.0f;
+#pragma unroll
+                   for (int k = 0; k < 6; ++k) {
+                     const int s = (k & 1) ? 1 : -1;
+                     const int ox = (k >> 1) == 0 ? s : 0, oy = (k >> 1) == 1 ? s : 0, oz = (k >> 1) == 2 ? s : 0;
+                     if (face_set_at(d, lk, lx, ly, lz, ox, oy, oz, dim)) {
+                       sum += cfr(d, NATRAY_F_U + dim, nbr(lk, lx, ly, lz, ox, oy, oz));
+                       cnt += 1.0f;
+                     }
+                   }
+                   if (cnt > 0.0f) {
+                     u = sum / cnt;
+                     done |= 1u << dim;
+                   }
+                 }
+                 cfr(d, NATRAY_F_EXT + dim, cell) = u;
+               }
+               cfr(d, NATRAY_F_EXT_SET, cell) = (float)done;
+             });
+}
+
+__device__ __forceinline__ bool ext_set(const Dev& d, uint32_t cell, uint32_t dim) {
+  return cell != NONE && ((((uint32_t)cfr(d, NATRAY_F_EXT_SET, cell)) >> dim) & 1u) != 0u;
+}
+
+__global__ __launch_bounds__(256) void k_extend2(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u,
+             [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+               const uint32_t cell = b * 64u + lane;
+               const float ty = cfr(d, NATRAY_F_TYPE, cell);
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim) {
+                 float u = cfr(d, NATRAY_F_U + dim, cell);
+                 if (ext_set(d, cell, dim)) {
+                   u = cfr(d, NATRAY_F_EXT + dim, cell);
+                 } else {
+                   float sum = 0.0f, cnt = 0.0f;
+#pragma unroll
+                   for (int k = 0; k < 6; ++k) {
+                     const int s = (k & 1) ? 1 : -1;
+                     const uint32_t nb = nbr(lk, lx, ly, lz, (k >> 1) == 0 ? s : 0, (k >> 1) == 1 ? s : 0, (k >> 1) == 2 ? s : 0);
+                     if (ext_set(d, nb, dim)) {
+                       sum += cfr(d, NATRAY_F_EXT + dim, nb);
+                       cnt += 1.0f;
+                     }
+                   }
+                   if (cnt > 0.0f) u = sum / cnt;
+                 }
+                 const uint32_t lo = nbr(lk, lx, ly, lz, -(dim == 0), -(dim == 1), -(dim == 2));
+                 cfr(d, NATRAY_F_U + dim, cell) = bounded(d, u, ty, cell_type(d, lo), cell, lo, dim);
+               }
+             });
+}
+
+// ---- g2p (fluid.wgsl:767-901) ------------------------------------------------------------------------
+
+struct Gathered {
+  float s, g0, g1, g2, w;
+};
+
+// gather (fluid.wgsl:780-802)
+template <int OX, int OY, int OZ>
+__device__ __forceinline__ Gathered gather(const float* tg, const float X[3], const int corner[3], uint32_t fa, uint32_t fw,
+                                           float h) {
+  const float o[3] = {0.5f * OX, 0.5f * OY, 0.5f * OZ};
+  int base[3];
+  float fx[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    base[k] = (int)floorf(X[k] - o[k] - 0.5f);
+    fx[k] = X[k] - o[k] - (float)base[k];
+  }
+  float wx[3], wy[3], wz[3];
+  axis_weights(fx[0], wx);
+  axis_weights(fx[1], wy);
+  axis_weights(fx[2], wz);
+  const int t0 = (base[0] - corner[0]) * 49 + (base[1] - corner[1]) * 7 + (base[2] - corner[2]);
+  const float* ta = tg + fa * TILE + t0;
+  const float* tw = tg + fw * TILE + t0;
+  Gathered r = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float weight = wx[a] * wy[b] * wz[c];
+        const int t = a * 49 + b * 7 + c;
+        const float u = weight * ta[t];
+        r.s += u;
+        r.g0 += u * ((float)a - fx[0]);
+        r.g1 += u * ((float)b - fx[1]);
+        r.g2 += u * ((float)c - fx[2]);
+        r.w += weight * tw[t];
+      }
+  r.g0 *= h;
+  r.g1 *= h;
+  r.g2 *= h;
+  return r;
+}
+
+// carve for the moving capsule (the role of bullet.wgsl:395-424).
+__device__ __forceinline__ void carve(const Dev& d, float x[3], float v[3]) {
+  const float r = d.sc->capsule_a[3];
+  float on[3], ab[3];
+  float d2 = capsule_d2(d, x[0], x[1], x[2], now_of(d), on, ab);
+  if (d2 >= r * r) return;
+  float dd[3] = {x[0] - on[0], x[1] - on[1], x[2] - on[2]};
+  if (d2 < 1e-14f) {
+    if (ab[0] * ab[0] + ab[2] * ab[2] > 1e-20f) {
+      dd[0] = -ab[2]; dd[1] = 0.0f; dd[2] = ab[0];
+    } else {
+      dd[0] = 0.0f; dd[1] = ab[2]; dd[2] = -ab[1];
+    }
+    d2 = (dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2];
+  }
+  const float len = __builtin_sqrtf(d2);
+  const float dir[3] = {dd[0] / len, dd[1] / len, dd[2] / len};
+  x[0] = on[0] + dir[0] * r;
+  x[1] = on[1] + dir[1] * r;
+  x[2] = on[2] + dir[2] * r;
+  const float* cv = d.sc->capsule_v;
+  const float wall = (cv[0] * dir[0] + cv[1] * dir[1]) + cv[2] * dir[2];
+  const float vn = (v[0] * dir[0] + v[1] * dir[1]) + v[2] * dir[2];
+  const float push = fmax_(wall - vn, 0.0f);
+  v[0] += dir[0] * push;
+  v[1] += dir[1] * push;
+  v[2] += dir[2] * push;
+}
+
+// g2p_one (fluid.wgsl:806-861); returns the particle's speed, 0 once it is gone.
+__device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const uint32_t* links, uint32_t i, const int corner[3]) {
+  const natray_params* U = d.U;
+  const float hh = U->origin[3];
+  float4* P = reinterpret_cast<float4*>(d.P) + (size_t)i * 5;
+  const float4 pos = P[0];
+  const float c1w = P[3].w;
+  const float X[3] = {(pos.x - U->origin[0]) / hh, (pos.y - U->origin[1]) / hh, (pos.z - U->origin[2]) / hh};
+  const int own[3] = {(int)floorf(X[0]), (int)floorf(X[1]), (int)floorf(X[2])};
+  if ((unsigned)(own[0] - corner[0] - 1) > 3u || (unsigned)(own[1] - corner[1] - 1) > 3u ||
+      (unsigned)(own[2] - corner[2] - 1) > 3u)
+    return 0.0f;  // not in this block: only possible if the particle was moved behind the library's back
+  const Gathered gu = gather<0, 1, 1>(tg, X, corner, NATRAY_A_MOM(0), NATRAY_A_WT(0), hh);
+  const Gathered gv = gather<1, 0, 1>(tg, X, corner, NATRAY_A_MOM(1), NATRAY_A_WT(1), hh);
+  const Gathered gw = gather<1, 1, 0>(tg, X, corner, NATRAY_A_MOM(2), NATRAY_A_WT(2), hh);
+  const Gathered gp = gather<1, 1, 1>(tg, X, corner, NATRAY_A_PC, NATRAY_A_WC, hh);
+  float v[3] = {gu.s, gv.s, gw.s};
+  const float m_near = gu.w + gv.w + gw.w;
+  const float step = dt_of(d);
+  float x[3] = {pos.x + step * v[0], pos.y + step * v[1], pos.z + step * v[2]};
+  if (d.sc->flags & NATRAY_SCENE_CAPSULE) carve(d, x, v);
+  const float spacing = U->fluid[3];
+  float lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = U->origin[k] + 3.5f * hh;
+    hi[k] = U->origin[k] + ((float)U->dims[k] - 4.5f) * hh;
+  }
+  const bool escaped = x[0] < lo[0] || x[2] < lo[2] || x[0] > hi[0] || x[2] > hi[2] || x[1] > hi[1];
+  const float fill = m_near / 24.0f;
+  const bool on_ground = x[1] < 1.5f * spacing;
+  const float r_thin = 0.75f * spacing;
+  const float t_thin = 3.0f * __builtin_sqrtf(U->fluid[0] * r_thin * r_thin * r_thin / NATRAY_SIGMA_W);
+  const uint32_t own_cell = tile_cell(links, (uint32_t)((own[0] - corner[0]) * 49 + (own[1] - corner[1]) * 7 + (own[2] - corner[2])));
+  bool covered = own_cell != NONE && accr(d, NATRAY_A_MA, own_cell) > 0;
+  if ((d.sc->flags & NATRAY_SCENE_MASK) && in_mask(d, own[0], own[1], own[2])) covered = true;
+  const float thin_age = (fill < 0.45f && !on_ground && !covered) ? c1w + step : 0.0f;
+  if (escaped || (!covered && fill < 0.2f) || thin_age > t_thin) {
+    // break_up / ground_touch belong to the spray and wet-ground subsystems: the particle dies.
+    P[1] = make_float4(v[0], v[1], v[2], 0.0f);
+    d.key[i] = NONE;
+    return 0.0f;
+  }
+  float xx[3] = {x[0], x[1], x[2]}, vv[3] = {v[0], v[1], v[2]};
+  if (xx[1] < 0.3f * spacing) {
+    xx[1] = 0.3f * spacing;
+    vv[1] = fmax_(vv[1], 0.0f);
+  }
+  const float cs = 4.0f / (hh * hh);
+  P[0] = make_float4(xx[0], xx[1], xx[2], pos.w);
+  P[1] = make_float4(vv[0], vv[1], vv[2], 1.0f);
+  P[2] = make_float4(gu.g0 * cs, gu.g1 * cs, gu.g2 * cs, gp.w > 1e-6f ? gp.s / gp.w : 0.0f);
+  P[3] = make_float4(gv.g0 * cs, gv.g1 * cs, gv.g2 * cs, thin_age);
+  P[4] = make_float4(gw.g0 * cs, gw.g1 * cs, gw.g2 * cs, 0.0f);
+  d.key[i] = pack_key(d, xx[0], xx[1], xx[2], true);
+  return __builtin_sqrtf((vv[0] * vv[0] + vv[1] * vv[1]) + vv[2] * vv[2]);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  __shared__ float tg[8 * TILE];
+  __shared__ uint32_t links[27];
+  const uint32_t n = n_blocks(d), tid = threadIdx.x;
+  float fastest = 0.0f;
+  for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
+    __syncthreads();
+    if (tid < 27) links[tid] = d.block_nb[b * 27u + tid];
+    const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
+    __syncthreads();
+    if (count == 0u) continue;
+    for (uint32_t t = tid; t < TILE; t += 256) {
+      const uint32_t cell = tile_cell(links, t);
+      const bool water = cell != NONE && cfr(d, NATRAY_F_TYPE, cell) == WATER;
+#pragma unroll
+      for (uint32_t k = 0; k < 3; ++k) {
+        float u = 0.0f, w = 0.0f;
+        if (cell != NONE) {
+          u = cfr(d, NATRAY_F_U + k, cell);
+          w = from_fp(accr(d, NATRAY_A_WT(k), cell), NATRAY_FP_W);
+        }
+        tg[NATRAY_A_MOM(k) * TILE + t] = u;
+        tg[NATRAY_A_WT(k) * TILE + t] = w;
+      }
+      tg[NATRAY_A_PC * TILE + t] = water ? pressure_of(d, cell) : 0.0f;
+      tg[NATRAY_A_WC * TILE + t] = water ? 1.0f : 0.0f;
+    }
+    __syncthreads();
+    int bx, by, bz;
+    slot_coord(d.U, d.block_list[b], bx, by, bz);
+    const int corner[3] = {bx * 4 - 1, by * 4 - 1, bz * 4 - 1};
+    for (uint32_t k = tid; k < count; k += 256) fastest = fmax_(fastest, g2p_one(d, tg, links, d.sorted[first + k], corner));
+  }
+  for (int off = 32; off > 0; off >>= 1) fastest = fmax_(fastest, __shfl_xor(fastest, off));
+  if (lane_id() == 0 && fastest > 0.0f) atomicMax(reinterpret_cast<unsigned int*>(&d.st->max_speed), __float_as_uint(fastest));
+}
+
+// ---- launch wrappers -------------------------------------------------------------------------------
+
+static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int launch_keys(const Dev& d, void* stream) {
+  if (d.N) hipLaunchKernelGGL(k_keys, dim3((d.N + 255) / 256), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_begin(const Dev& d, void* stream) {
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_bin(const Dev& d, void* stream, int grid_cells) {
+  hipLaunchKernelGGL(k_clear, dim3(grid_cells), dim3(256), 0, S(stream), d);
+  const uint32_t gp = d.N ? (d.N + 255) / 256 : 1;
+  hipLaunchKernelGGL(k_mark, dim3(gp), dim3(256), 0, S(stream), d);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(d.ntiles), dim3(256), 0, S(stream), d);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, S(stream), d);
+  hipLaunchKernelGGL(k_scan_scatter, dim3(d.ntiles), dim3(256), 0, S(stream), d);
+  hipLaunchKernelGGL(k_sort, dim3(gp), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_p2g(const Dev& d, void* stream, int grid_blocks) {
+  hipLaunchKernelGGL(k_p2g, dim3(grid_blocks), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_grid_prep(const Dev& d, void* stream, int grid_waves) {
+  hipLaunchKernelGGL(k_grid_prep, dim3(grid_waves), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int solve_max_grid(int mixed, int* per_cu) {
+  int occ = 0;
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<true>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<false>, 256, 0);
+  *per_cu = occ;
+  return (int)e;
+}
+int launch_solve(const Dev& d, void* stream, int grid, int mixed) {
+  Dev copy = d;
+  void* args[] = {&copy};
+  hipError_t e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true>, dim3(grid), dim3(256), args, 0, S(stream))
+                       : hipLaunchCooperativeKernel((const void*)k_solve<false>, dim3(grid), dim3(256), args, 0, S(stream));
+  return (int)e;
+}
+int launch_project(const Dev& d, void* stream, int grid_waves) {
+  hipLaunchKernelGGL(k_project, dim3(grid_waves), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_extend(const Dev& d, void* stream, int grid_waves) {
+  hipLaunchKernelGGL(k_extend1, dim3(grid_waves), dim3(256), 0, S(stream), d);
+  hipLaunchKernelGGL(k_extend2, dim3(grid_waves), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_g2p(const Dev& d, void* stream, int grid_blocks) {
+  hipLaunchKernelGGL(k_g2p, dim3(grid_blocks), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+
+}  // namespace natray

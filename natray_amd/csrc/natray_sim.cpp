@@ -1,0 +1,554 @@
+// Host half of the MI355X water substep: the C ABI of include/natray.h.
+//
+// natray_sim_create allocates every device buffer once (the buffers of Sim::new,
+// src/sim.rs:237-315); stepping enqueues the kernels of one step in plan order on the
+// handle's stream (src/sim.rs:390-426) and reads the 96-byte State back once per batch
+// (src/sim.rs:623-627). natray_sim_run and natray_sim_advance keep the reference's batch sizing
+// and CG budget adaptation (src/sim.rs:498-522,636-645).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "natray.h"
+#include "natray_dev.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                                     \
+  do {                                                                                                \
+    hipError_t e_ = (x);                                                                              \
+    if (e_ != hipSuccess) return fail(NATRAY_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(x)                                                                                           \
+  do {                                                                                                         \
+    int e_ = (x);                                                                                              \
+    if (e_ != 0) return fail(NATRAY_E_HIP, std::string(#x) + ": " + hipGetErrorString((hipError_t)e_)); \
+  } while (0)
+
+constexpr double BATCH_MS = 4.0;  // src/sim.rs:24
+constexpr size_t EVENT_POOL = 8192;
+
+}  // namespace
+
+struct natray_sim {
+  natray_params U{};
+  natray_scene sc{};
+  natray::Dev D{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cus = 0;
+  int grid_cells = 0, grid_blocks = 0, grid_waves = 0, solve_grid = 0;
+  int mixed = 0;
+  std::vector<void*> allocs;
+  // Pinned staging for the small host <-> device copies of a batch.
+  natray_params* h_params = nullptr;
+  natray_state* h_state = nullptr;
+  uint32_t* h_words = nullptr;
+  // Sim::advance / Sim::run host state (src/sim.rs:187-203,480).
+  natray_state last{};
+  uint32_t iters = 64;
+  uint32_t unconverged = 0;
+  double step_ms = 0.0;
+  bool started = false;
+  double t = 0.0;
+  // Per-phase timing.
+  bool prof = false;
+  std::vector<hipEvent_t> events;
+  std::vector<int> ev_phase;
+  size_t ev_used = 0;
+  double ms[NATRAY_N_PHASES] = {};
+  uint32_t cnt[NATRAY_N_PHASES] = {};
+};
+
+namespace {
+
+template <class T>
+int dalloc(natray_sim* s, T** p, size_t count) {
+  void* q = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(NATRAY_E_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+  s->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return NATRAY_OK;
+}
+
+void free_all(natray_sim* s) {
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+  s->events.clear();
+  for (void* p : s->allocs) (void)hipFree(p);
+  s->allocs.clear();
+  if (s->h_params) (void)hipHostFree(s->h_params);
+  if (s->h_state) (void)hipHostFree(s->h_state);
+  if (s->h_words) (void)hipHostFree(s->h_words);
+  s->h_params = nullptr;
+  s->h_state = nullptr;
+  s->h_words = nullptr;
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->stream = nullptr;
+}
+
+int harvest_events(natray_sim* s) {
+  for (size_t k = 0; k < s->ev_used; ++k) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, s->events[2 * k], s->events[2 * k + 1]));
+    s->ms[s->ev_phase[k]] += ms;
+    s->cnt[s->ev_phase[k]] += 1;
+  }
+  s->ev_used = 0;
+  return NATRAY_OK;
+}
+
+int enqueue_phase(natray_sim* s, int phase) {
+  const natray::Dev& D = s->D;
+  void* st = s->stream;
+  size_t slot = 0;
+  bool timed = s->prof && s->ev_used < s->events.size() / 2;
+  if (timed) {
+    slot = s->ev_used++;
+    s->ev_phase[slot] = phase;
+    HIPCHK(hipEventRecord(s->events[2 * slot], s->stream));
+  }
+  switch (phase) {
+    case NATRAY_PHASE_BEGIN: LAUNCHCHK(natray::launch_begin(D, st)); break;
+    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(D, st, s->grid_cells)); break;
+    case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(D, st, s->grid_blocks)); break;
+    case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(D, st, s->grid_waves)); break;
+    case NATRAY_PHASE_SOLVE: LAUNCHCHK(natray::launch_solve(D, st, s->solve_grid, s->mixed)); break;
+    case NATRAY_PHASE_PROJECT: LAUNCHCHK(natray::launch_project(D, st, s->grid_waves)); break;
+    case NATRAY_PHASE_EXTEND: LAUNCHCHK(natray::launch_extend(D, st, s->grid_waves)); break;
+    case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(D, st, s->grid_blocks)); break;
+    default: return fail(NATRAY_E_ARG, "unknown phase " + std::to_string(phase));
+  }
+  if (timed) HIPCHK(hipEventRecord(s->events[2 * slot + 1], s->stream));
+  return NATRAY_OK;
+}
+
+int upload_params(natray_sim* s) {
+  *s->h_params = s->U;
+  HIPCHK(hipMemcpyAsync(const_cast<natray_params*>(s->D.U), s->h_params, sizeof(natray_params), hipMemcpyHostToDevice, s->stream));
+  return NATRAY_OK;
+}
+
+// Reads the State back and checks the solver's barrier; the stream is idle afterwards.
+int sync_state(natray_sim* s) {
+  HIPCHK(hipMemcpyAsync(s->h_state, s->D.st, sizeof(natray_state), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(s->h_words, s->D.bar, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->last = *s->h_state;
+  if (s->prof) {
+    int e = harvest_events(s);
+    if (e) return e;
+  }
+  if (s->h_words[2] != 0u) {
+    HIPCHK(hipMemsetAsync(s->D.bar, 0, 3 * sizeof(uint32_t), s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return fail(NATRAY_E_SOLVER, "the pressure solve lost its grid barrier (not all workgroups were resident)");
+  }
+  return NATRAY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void natray_abi_sizes(uint32_t out[4]) {
+  out[0] = sizeof(natray_params);
+  out[1] = sizeof(natray_state);
+  out[2] = sizeof(natray_particle);
+  out[3] = sizeof(natray_scene);
+}
+
+const char* natray_last_error(void) { return g_err.c_str(); }
+
+int natray_sim_create(const natray_params* params, const float* particles, uint32_t n, const natray_scene* scene,
+                      const natray_state* init, natray_sim** out) {
+  if (!params || !out || (n && !particles)) return fail(NATRAY_E_ARG, "null argument");
+  *out = nullptr;
+  if (params->dims[3] != n) return fail(NATRAY_E_ARG, "params.dims.w must equal the particle count");
+  for (int k = 0; k < 3; ++k) {
+    if (params->dims[k] == 0 || params->dims[k] > natray::MAX_DIM)
+      return fail(NATRAY_E_ARG, "domain sides must be 1..1024 cells");
+    if (params->bdims[k] * 4u != params->dims[k]) return fail(NATRAY_E_ARG, "bdims must be dims / 4 (dims multiples of 4)");
+  }
+  if (params->bdims[3] == 0) return fail(NATRAY_E_ARG, "block capacity bdims.w must be positive");
+  if (!(params->origin[3] > 0.0f)) return fail(NATRAY_E_ARG, "cell size origin.w must be positive");
+  uint64_t S64 = (uint64_t)params->bdims[0] * params->bdims[1] * params->bdims[2];
+  if (S64 >= 0xfffff000ull) return fail(NATRAY_E_ARG, "domain too large");
+  if ((uint64_t)params->bdims[3] * 64ull * NATRAY_ACC_FIELDS >= (1ull << 40)) return fail(NATRAY_E_ARG, "capacity too large");
+
+  natray_sim* s = new natray_sim();
+  s->U = *params;
+  if (scene) {
+    s->sc = *scene;
+  } else {
+    std::memset(&s->sc, 0, sizeof s->sc);
+    s->sc.flags = NATRAY_SCENE_REFERENCE;
+  }
+  if (s->sc.precond > NATRAY_PRECOND_JACOBI || s->sc.precision > NATRAY_CG_MIXED) {
+    delete s;
+    return fail(NATRAY_E_ARG, "unknown preconditioner or precision");
+  }
+  s->mixed = s->sc.precision == NATRAY_CG_MIXED;
+  auto bail = [&](int code) {
+    free_all(s);
+    delete s;
+    return code;
+  };
+#define TRY(x)              \
+  do {                      \
+    int e__ = (x);          \
+    if (e__) return bail(e__); \
+  } while (0)
+#define HIPTRY(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e__ = (x);                                                                           \
+    if (e__ != hipSuccess) return bail(fail(NATRAY_E_HIP, std::string(#x) + ": " + hipGetErrorString(e__))); \
+  } while (0)
+
+  HIPTRY(hipGetDevice(&s->device));
+  HIPTRY(hipDeviceGetAttribute(&s->num_cus, hipDeviceAttributeMultiprocessorCount, s->device));
+  int coop = 0;
+  HIPTRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, s->device));
+  if (!coop) return bail(fail(NATRAY_E_HIP, "the device cannot launch cooperative kernels"));
+  HIPTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPTRY(hipHostMalloc((void**)&s->h_params, sizeof(natray_params), hipHostMallocDefault));
+  HIPTRY(hipHostMalloc((void**)&s->h_state, sizeof(natray_state), hipHostMallocDefault));
+  HIPTRY(hipHostMalloc((void**)&s->h_words, 16 * sizeof(uint32_t), hipHostMallocDefault));
+
+  natray::Dev& D = s->D;
+  D.N = n;
+  D.S = (uint32_t)S64;
+  D.ntiles = (uint32_t)((S64 + natray::SCAN_TILE - 1) / natray::SCAN_TILE);
+  D.Spad = D.ntiles * natray::SCAN_TILE;
+  D.cap = params->bdims[3];
+  D.capc = D.cap * 64u;
+  D.fixed_ns = 0;
+  if (s->sc.fixed_dt > 0.0f) {
+    double ns = std::floor((double)s->sc.fixed_dt * 1e9 + 0.5);
+    D.fixed_ns = ns < 20.0 ? 20u : (ns > 4e9 ? 4000000000u : (uint32_t)ns);
+  }
+  const size_t capc = D.capc;
+  natray_params* dU = nullptr;
+  natray_scene* dSc = nullptr;
+  TRY(dalloc(s, &dU, 1));
+  TRY(dalloc(s, &dSc, 1));
+  D.U = dU;
+  D.sc = dSc;
+  TRY(dalloc(s, &D.st, 1));
+  TRY(dalloc(s, &D.P, (size_t)n * 20));
+  TRY(dalloc(s, &D.key, (size_t)n + 1));
+  TRY(dalloc(s, &D.rank, (size_t)n + 1));
+  TRY(dalloc(s, &D.block_ptr, D.Spad));
+  TRY(dalloc(s, &D.slot_count, D.Spad));
+  TRY(dalloc(s, &D.block_list, (size_t)D.cap + 1));
+  TRY(dalloc(s, &D.block_nb, (size_t)D.cap * 27));
+  TRY(dalloc(s, &D.block_start, (size_t)D.cap + 2));
+  TRY(dalloc(s, &D.sorted, (size_t)n + 1));
+  TRY(dalloc(s, &D.acc, NATRAY_ACC_FIELDS * capc));
+  TRY(dalloc(s, &D.cf, NATRAY_CELL_FIELDS * capc));
+  TRY(dalloc(s, &D.bvec, capc));
+  TRY(dalloc(s, &D.nbm, capc));
+  TRY(dalloc(s, &D.part, (size_t)D.cap * 4));
+  TRY(dalloc(s, &D.chunk, (size_t)natray::MAX_CHUNKS * 4 * 2));
+  TRY(dalloc(s, &D.tile_sum, (size_t)D.ntiles * 2));
+  TRY(dalloc(s, &D.tile_off, (size_t)D.ntiles * 2));
+  TRY(dalloc(s, &D.bar, 4));
+  TRY(dalloc(s, &D.wcount, 1));
+  TRY(dalloc(s, &D.solver, 8));
+  TRY(dalloc(s, &D.stats, 4));
+
+  HIPTRY(hipMemsetAsync(D.block_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.slot_count, 0, (size_t)D.Spad * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.block_list, 0, ((size_t)D.cap + 1) * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.block_nb, 0xff, (size_t)D.cap * 27 * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.block_start, 0, ((size_t)D.cap + 2) * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.acc, 0, NATRAY_ACC_FIELDS * capc * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.cf, 0, NATRAY_CELL_FIELDS * capc * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.bvec, 0, capc * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.nbm, 0, capc * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.part, 0, (size_t)D.cap * 4 * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.chunk, 0, (size_t)natray::MAX_CHUNKS * 4 * 8 * 2, s->stream));
+  HIPTRY(hipMemsetAsync(D.bar, 0, 4 * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.wcount, 0, 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.solver, 0, 8 * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.stats, 0, 4 * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.sorted, 0, ((size_t)n + 1) * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.rank, 0, ((size_t)n + 1) * 4, s->stream));
+  if (n) HIPTRY(hipMemcpyAsync(D.P, particles, (size_t)n * 80, hipMemcpyHostToDevice, s->stream));
+  HIPTRY(hipMemcpyAsync(dSc, &s->sc, sizeof(natray_scene), hipMemcpyHostToDevice, s->stream));
+
+  natray_state st0{};
+  if (init) {
+    st0 = *init;
+  } else {
+    // src/sim.rs:240-241
+    float c_air = std::sqrt(1.4f * 101325.0f / 1.204f);
+    st0.mem_dt = FLT_MAX;
+    st0.cap_dt = FLT_MAX;
+    st0.air_dt = 0.9f * params->origin[3] / (3.0f * c_air);
+  }
+  *s->h_state = st0;
+  HIPTRY(hipMemcpyAsync(D.st, s->h_state, sizeof(natray_state), hipMemcpyHostToDevice, s->stream));
+  TRY(upload_params(s));
+  int e = natray::launch_keys(D, s->stream);
+  if (e) return bail(fail(NATRAY_E_HIP, std::string("keys: ") + hipGetErrorString((hipError_t)e)));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  s->last = st0;
+
+  // Launch shapes: per-block kernels loop over the blocks in use, which only the GPU knows.
+  const uint32_t cap = D.cap;
+  s->grid_cells = s->num_cus * 4;
+  s->grid_blocks = (int)std::min<uint32_t>(cap, (uint32_t)s->num_cus * 8u);
+  s->grid_waves = (int)std::min<uint32_t>((cap + 3u) / 4u, (uint32_t)s->num_cus * 8u);
+  int per_cu = 0;
+  e = natray::solve_max_grid(s->mixed, &per_cu);
+  if (e || per_cu <= 0) return bail(fail(NATRAY_E_HIP, "cannot size the cooperative solve"));
+  s->solve_grid = std::min(per_cu * s->num_cus, 2 * s->num_cus);
+  *out = s;
+  return NATRAY_OK;
+#undef TRY
+#undef HIPTRY
+}
+
+void natray_sim_destroy(natray_sim* s) {
+  if (!s) return;
+  free_all(s);
+  delete s;
+}
+
+int natray_sim_phase(natray_sim* s, int phase) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  int e = enqueue_phase(s, phase);
+  if (e) return e;
+  e = sync_state(s);
+  if (e) return e;
+  return s->last.overflow ? fail(NATRAY_E_OVERFLOW, "the water needs more grid blocks than the capacity") : NATRAY_OK;
+}
+
+int natray_sim_run(natray_sim* s, uint32_t n_steps, uint32_t iters, uint32_t stop_ns, natray_state* out) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  // src/sim.rs:586-593
+  s->U.limits[2] = iters ? iters : s->iters;
+  s->U.limits[3] = stop_ns;
+  int e = upload_params(s);
+  if (e) return e;
+  s->h_words[4] = 0;
+  s->h_words[5] = 0;
+  HIPCHK(hipMemcpyAsync(&s->D.st->iters_max, &s->h_words[4], 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
+  auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t k = 0; k < n_steps; ++k)
+    for (int ph = 0; ph < NATRAY_N_PHASES; ++ph) {
+      e = enqueue_phase(s, ph);
+      if (e) return e;
+    }
+  e = sync_state(s);
+  if (e) return e;
+  if (n_steps) s->step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n_steps;
+  if (out) *out = s->last;
+  const natray_state& st = s->last;
+  if (st.overflow) return fail(NATRAY_E_OVERFLOW, "the water needs more grid blocks than the capacity");
+  // src/sim.rs:636-645
+  s->unconverged += st.unconverged;
+  if (st.unconverged > 0)
+    s->iters = std::min(s->iters * 2u, NATRAY_MAX_ITERS);
+  else if (st.iters_max > 0)
+    s->iters = std::min(std::max(st.iters_max + st.iters_max / 4u + 2u, 8u), NATRAY_MAX_ITERS);
+  return NATRAY_OK;
+}
+
+int natray_sim_advance(natray_sim* s, double t_target, uint32_t budget, uint32_t* taken_out) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  // src/sim.rs:498-522 with the handoff time t0 = 0.
+  const double t0 = 0.0;
+  if (!s->started) s->started = true;
+  double span = t_target - t0;
+  if (span > 4.0) span = 4.0;
+  if (span < 0.0) span = 0.0;
+  const uint32_t stop = (uint32_t)std::llround(span * 1e9);
+  uint32_t taken = 0;
+  while (taken < budget && s->last.t_ns < stop) {
+    const uint32_t dt = s->last.free_dt_ns > 0 ? s->last.free_dt_ns : (uint32_t)(NATRAY_DT_MAX * 1e9);
+    const size_t fit = s->step_ms > 0.0 ? (size_t)(BATCH_MS / s->step_ms) : 1;
+    size_t hi = std::min<size_t>(budget - taken, NATRAY_MAX_BATCH);
+    hi = std::min(hi, std::max<size_t>(fit, 1));
+    size_t want = (stop - s->last.t_ns + dt - 1) / dt;
+    const uint32_t nb = (uint32_t)std::max<size_t>(1, std::min(want, hi));
+    const uint32_t before = s->last.step;
+    int e = natray_sim_run(s, nb, 0, stop, nullptr);
+    if (e) return e;
+    taken += s->last.step - before;
+  }
+  s->t = t0 + s->last.t_ns * 1e-9;
+  if (taken_out) *taken_out = taken;
+  return NATRAY_OK;
+}
+
+int natray_sim_step(natray_sim* s, natray_state* out) { return natray_sim_run(s, 1, 0, 0xffffffffu, out); }
+
+int natray_sim_read_state(natray_sim* s, natray_state* out) {
+  if (!s || !out) return fail(NATRAY_E_ARG, "null argument");
+  int e = sync_state(s);
+  if (e) return e;
+  *out = s->last;
+  return NATRAY_OK;
+}
+
+int natray_sim_write_state(natray_sim* s, const natray_state* in) {
+  if (!s || !in) return fail(NATRAY_E_ARG, "null argument");
+  *s->h_state = *in;
+  HIPCHK(hipMemcpyAsync(s->D.st, s->h_state, sizeof(natray_state), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->last = *in;
+  return NATRAY_OK;
+}
+
+int natray_sim_read_particles(natray_sim* s, float* dst) {
+  if (!s || (!dst && s->D.N)) return fail(NATRAY_E_ARG, "null argument");
+  if (s->D.N) HIPCHK(hipMemcpyAsync(dst, s->D.P, (size_t)s->D.N * 80, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_write_particles(natray_sim* s, const float* src) {
+  if (!s || (!src && s->D.N)) return fail(NATRAY_E_ARG, "null argument");
+  if (s->D.N) HIPCHK(hipMemcpyAsync(s->D.P, src, (size_t)s->D.N * 80, hipMemcpyHostToDevice, s->stream));
+  LAUNCHCHK(natray::launch_keys(s->D, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_write_params(natray_sim* s, const natray_params* params) {
+  if (!s || !params) return fail(NATRAY_E_ARG, "null argument");
+  for (int k = 0; k < 4; ++k)
+    if (params->dims[k] != s->U.dims[k] || params->bdims[k] != s->U.bdims[k])
+      return fail(NATRAY_E_ARG, "the domain and capacity are fixed at creation");
+  s->U = *params;
+  int e = upload_params(s);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_read_params(natray_sim* s, natray_params* out) {
+  if (!s || !out) return fail(NATRAY_E_ARG, "null argument");
+  *out = s->U;
+  return NATRAY_OK;
+}
+
+int natray_sim_read_grid(natray_sim* s, uint32_t* n_blocks, uint32_t* block_list, uint32_t* block_nb, uint32_t* block_start,
+                         int32_t* acc, float* cf) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  int e = sync_state(s);
+  if (e) return e;
+  const uint32_t n = std::min(s->last.n_blocks, s->U.bdims[3]);
+  const size_t nc = (size_t)n * 64u, capc = s->D.capc;
+  if (n_blocks) *n_blocks = n;
+  if (block_list && n) HIPCHK(hipMemcpyAsync(block_list, s->D.block_list, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
+  if (block_nb && n) HIPCHK(hipMemcpyAsync(block_nb, s->D.block_nb, (size_t)n * 27 * 4, hipMemcpyDeviceToHost, s->stream));
+  if (block_start) HIPCHK(hipMemcpyAsync(block_start, s->D.block_start, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, s->stream));
+  if (acc && n)
+    for (uint32_t f = 0; f < NATRAY_ACC_FIELDS; ++f)
+      HIPCHK(hipMemcpyAsync(acc + f * nc, s->D.acc + f * capc, nc * 4, hipMemcpyDeviceToHost, s->stream));
+  if (cf && n)
+    for (uint32_t f = 0; f < NATRAY_CELL_FIELDS; ++f)
+      HIPCHK(hipMemcpyAsync(cf + f * nc, s->D.cf + f * capc, nc * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_write_grid(natray_sim* s, const int32_t* acc, const float* cf) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  int e = sync_state(s);
+  if (e) return e;
+  const uint32_t n = std::min(s->last.n_blocks, s->U.bdims[3]);
+  const size_t nc = (size_t)n * 64u, capc = s->D.capc;
+  if (acc && n)
+    for (uint32_t f = 0; f < NATRAY_ACC_FIELDS; ++f)
+      HIPCHK(hipMemcpyAsync(s->D.acc + f * capc, acc + f * nc, nc * 4, hipMemcpyHostToDevice, s->stream));
+  if (cf && n)
+    for (uint32_t f = 0; f < NATRAY_CELL_FIELDS; ++f)
+      HIPCHK(hipMemcpyAsync(s->D.cf + f * capc, cf + f * nc, nc * 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_read_bins(natray_sim* s, uint32_t* block_ptr, uint32_t* sorted) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  if (block_ptr) HIPCHK(hipMemcpyAsync(block_ptr, s->D.block_ptr, (size_t)s->D.S * 4, hipMemcpyDeviceToHost, s->stream));
+  if (sorted && s->D.N) HIPCHK(hipMemcpyAsync(sorted, s->D.sorted, (size_t)s->D.N * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+void* natray_sim_device_ptr(natray_sim* s, int which) {
+  if (!s) return nullptr;
+  switch (which) {
+    case NATRAY_BUF_PARTICLES: return s->D.P;
+    case NATRAY_BUF_STATE: return s->D.st;
+    case NATRAY_BUF_ACC: return s->D.acc;
+    case NATRAY_BUF_CF: return s->D.cf;
+    case NATRAY_BUF_BLOCK_LIST: return s->D.block_list;
+    case NATRAY_BUF_STREAM: return (void*)s->stream;
+    default: return nullptr;
+  }
+}
+
+int natray_sim_set_profiling(natray_sim* s, int on) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (on && s->events.empty()) {
+    s->events.resize(EVENT_POOL);
+    s->ev_phase.resize(EVENT_POOL / 2);
+    for (size_t k = 0; k < EVENT_POOL; ++k) HIPCHK(hipEventCreate(&s->events[k]));
+  }
+  s->prof = on != 0;
+  s->ev_used = 0;
+  for (int k = 0; k < NATRAY_N_PHASES; ++k) {
+    s->ms[k] = 0.0;
+    s->cnt[k] = 0;
+  }
+  return NATRAY_OK;
+}
+
+int natray_sim_phase_times(natray_sim* s, double* ms, uint32_t* counts) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->prof) {
+    int e = harvest_events(s);
+    if (e) return e;
+  }
+  for (int k = 0; k < NATRAY_N_PHASES; ++k) {
+    if (ms) ms[k] = s->ms[k];
+    if (counts) counts[k] = s->cnt[k];
+  }
+  return NATRAY_OK;
+}
+
+int natray_sim_cg_stats(natray_sim* s, uint64_t out[3]) {
+  if (!s || !out) return fail(NATRAY_E_ARG, "null argument");
+  unsigned long long h[4] = {};
+  HIPCHK(hipMemcpyAsync(h, s->D.stats, sizeof h, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  out[0] = h[0];
+  out[1] = h[1];
+  out[2] = h[2];
+  return NATRAY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,310 @@
+"""Scene construction: the reference balloon's domain, particles and Params, and the synthetic
+benchmark scenes.
+
+Reference construction follows Sim::new (src/sim.rs:208-237,428-441) and the water fill of
+balloon::build (src/balloon.rs:419-445). The balloon's inflated shape comes from the
+membrane's inflation solve (src/balloon.rs:119-248), which belongs to the membrane and is not
+part of this library; `SphereProfile` stands in for it (exact for the weightless preset, whose
+balloon is round, src/balloon.rs:497-506).
+
+The synthetic scenes are the benchmark workloads: a water sphere made of the N/8 cells nearest
+the box centre, 8 particles per cell at uniform in-cell offsets from a splitmix64 stream, in a
+unit box with solid walls; optionally a rigid capsule moving through it and a fixed spherical
+mask in place of the membrane.
+"""
+from __future__ import annotations
+
+import dataclasses as dc
+import math
+
+import numpy as np
+
+from . import abi
+from .params import SimParams
+
+# src/sim.rs:13-47
+SPRAY_CAP = 1_200_000
+WET_RES = 1024
+WET_HALF = 1.2
+FRICTION = 0.8
+
+
+@dc.dataclass
+class Scene:
+    """Everything natray_sim_create takes."""
+
+    params: abi.Params
+    particles: np.ndarray  # (N, 20) float32
+    scene: abi.Scene
+    init: abi.State | None = None
+    meta: dict = dc.field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.particles.shape[0])
+
+
+def next_multiple_of(x: int, m: int) -> int:
+    return (x + m - 1) // m * m
+
+
+# ---- reference construction ---------------------------------------------------------------
+
+
+class Rng:
+    """xorshift64 of src/balloon.rs:250-258."""
+
+    def __init__(self, seed: int = 0x9E3779B97F4A7C15):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self) -> float:
+        M = 0xFFFFFFFFFFFFFFFF
+        s = self.s
+        s ^= (s << 13) & M
+        s ^= s >> 7
+        s ^= (s << 17) & M
+        self.s = s
+        return (s >> 11) / float(1 << 53)
+
+
+# The membrane mesh is an icosphere of level 6, 10 * 4^6 + 2 vertices, and each vertex draws
+# two numbers from the generator before the water does (src/balloon.rs:357-368).
+MESH_DRAWS = 2 * (10 * 4**6 + 2)
+
+
+@dc.dataclass
+class SphereProfile:
+    """A round balloon standing in for the inflated profile (src/balloon.rs Profile)."""
+
+    radius: float
+    neck_r: float
+
+    @property
+    def z_neck(self) -> float:
+        return 2.0 * self.radius
+
+    @property
+    def z_centroid(self) -> float:
+        return self.radius
+
+    def z_top(self) -> float:
+        return 2.0 * self.radius
+
+    def r_at(self, z: float) -> float:
+        q = self.radius * self.radius - (z - self.radius) ** 2
+        return math.sqrt(q) if q > 0.0 else 0.0
+
+
+def balloon_particles(p: SimParams, profile, bottom_y: float, radius_max: float) -> tuple[np.ndarray, float]:
+    """The water fill of balloon::build (src/balloon.rs:419-445): a lattice with odd layers
+    offset by half a spacing, jittered by +-0.15 spacing, inside r(z) - spacing/2."""
+    d = p.balloon_diameter_cm * 0.01
+    spacing = d / max(p.resolution, 16)
+    rng = Rng()
+    for _ in range(MESH_DRAWS):
+        rng.next()
+    out = []
+    n = math.ceil(2.0 * radius_max / spacing) + 2
+    nz = math.ceil(profile.z_neck / spacing) + 1
+    half = int(n / 2)  # Rust integer division truncates toward zero
+    for iy in range(nz):
+        zl = (iy + 0.5) * spacing
+        if zl > profile.z_neck - spacing:
+            break
+        rmax = profile.r_at(zl) - 0.5 * spacing
+        if rmax <= 0.0:
+            continue
+        for ix in range(-half, half + 1):
+            for iz in range(-half, half + 1):
+                x = (ix + 0.5 * (iy % 2)) * spacing
+                z = (iz + 0.5 * (iy % 2)) * spacing
+                jx = (rng.next() - 0.5) * 0.3 * spacing
+                jy = (rng.next() - 0.5) * 0.3 * spacing
+                jz = (rng.next() - 0.5) * 0.3 * spacing
+                if math.sqrt(x * x + z * z) < rmax:
+                    out.append((x + jx, bottom_y + zl + jy, z + jz, 1.0))
+    return np.asarray(out, dtype=np.float32).reshape(-1, 4), spacing
+
+
+def pack_particles(xyzw: np.ndarray) -> np.ndarray:
+    """Packs particles into 80-byte records with vel.w = 1 (src/sim.rs:227-236)."""
+    P = np.zeros((xyzw.shape[0], abi.PARTICLE_FLOATS), dtype=np.float32)
+    P[:, :4] = xyzw
+    P[:, 7] = 1.0
+    return P
+
+
+def reference_scene(p: SimParams, capacity: int = abi.MAX_BLOCKS) -> Scene:
+    """Sim::new's CPU half (src/sim.rs:208-237,428-441) with SphereProfile for the shape."""
+    d = p.balloon_diameter_cm * 0.01
+    prof = SphereProfile(radius=0.5 * d, neck_r=0.05 * d)
+    bottom_y = p.balloon_height - prof.z_centroid
+    xyzw, spacing = balloon_particles(p, prof, bottom_y, prof.radius)
+    dx = 2.0 * spacing
+    top = bottom_y + prof.z_top() + 0.6
+    nxz = next_multiple_of(math.ceil(2.0 * WET_HALF / dx), 4)
+    ny = next_multiple_of(math.ceil(top / dx) + 3, 4)
+    half = np.float32(nxz * 0.5 * dx)
+    U = abi.Params()
+    U.origin[:] = [-half, np.float32(-3.0 * dx), -half, np.float32(dx)]
+    U.dims[:] = [nxz, ny, nxz, xyzw.shape[0]]
+    U.bdims[:] = [nxz // 4, ny // 4, nxz // 4, capacity]
+    U.gravity[:] = [0.0, -p.gravity, 0.0, abi.SOUND]
+    U.fluid[:] = [abi.RHO_WATER, abi.CFL, abi.DT_MAX, spacing]
+    U.wet[:] = [WET_HALF, 0.0, 0.0, 0.0]
+    U.limits[:] = [WET_RES, SPRAY_CAP, 0, 0]
+    U.solve[:] = [abi.SOLVE_TOL, 0.0, 0.0, 0.0]
+    # No rubber vertices and no bullets: the membrane and the bullets are not simulated here.
+    U.mem[:] = [0, 2, 2, 0]
+    U.rubber[:] = [0.0, 0.0, 0.0, p.rubber_thickness_mm * 1e-3]
+    U.rubber2[:] = [FRICTION, p.rubber_break_stretch, 0.0, 0.0]
+    U.knot[:] = [0.0, bottom_y + prof.z_neck, 0.0, prof.neck_r]
+    sc = abi.Scene()
+    sc.flags = abi.SCENE_REFERENCE
+    sc.precond = abi.PRECOND_BLOCK_SGS
+    sc.precision = abi.CG_FP32
+    return Scene(U, pack_particles(xyzw), sc, None, {"workload": "reference balloon", "spacing": spacing, "dx": dx})
+
+
+# ---- synthetic benchmark scenes -----------------------------------------------------------------
+
+GOLDEN = 0x9E3779B97F4A7C15
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """`count` outputs of splitmix64 seeded with `seed`, as uint64."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(GOLDEN)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, count: int) -> np.ndarray:
+    return (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def nearest_cells(side: int, n_cells: int, centre=(0.5, 0.5, 0.5)) -> np.ndarray:
+    """The n_cells cells of a side^3 unit-box grid whose centres are nearest `centre`, ties
+    broken by linear index; returned as (n, 3) int coordinates."""
+    c = np.asarray(centre) * side
+    r = (3.0 * n_cells / (4.0 * math.pi)) ** (1.0 / 3.0) + 3.0
+    lo = np.maximum(np.floor(c - r).astype(int), 0)
+    hi = np.minimum(np.ceil(c + r).astype(int) + 1, side)
+    ax = [np.arange(lo[k], hi[k]) for k in range(3)]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+    # Squared distances times 4, exactly, in integers (centres at i + 1/2).
+    cc = np.round(2.0 * c).astype(np.int64)
+    d2 = (2 * X + 1 - cc[0]) ** 2 + (2 * Y + 1 - cc[1]) ** 2 + (2 * Z + 1 - cc[2]) ** 2
+    lin = (X.astype(np.int64) * side + Y) * side + Z
+    order = np.lexsort((lin, d2))[:n_cells]
+    if len(order) < n_cells:
+        raise ValueError("sphere does not fit the box")
+    return np.stack([X[order], Y[order], Z[order]], axis=1)
+
+
+def block_order(cells: np.ndarray, side: int) -> np.ndarray:
+    """Sorts cells by 4^3 block (slot order), then by cell within the block, so particles of
+    one block sit together in memory."""
+    b = cells >> 2
+    l = cells & 3
+    bd = side // 4
+    slot = (b[:, 0] * bd + b[:, 1]) * bd + b[:, 2]
+    loc = l[:, 0] * 16 + l[:, 1] * 4 + l[:, 2]
+    return cells[np.lexsort((loc, slot))]
+
+
+def sphere_particles(side: int, n: int, seed: int, centre=(0.5, 0.5, 0.5)) -> np.ndarray:
+    """n particles, 8 per cell, in the n/8 cells nearest the centre of a unit box of side^3
+    cells, each at independent U[0,1)^3 in-cell offsets."""
+    assert n % 8 == 0
+    cells = block_order(nearest_cells(side, n // 8, centre), side)
+    u = uniform01(seed, 3 * n).reshape(n, 3)
+    base = np.repeat(cells, 8, axis=0).astype(np.float64)
+    pos = (base + u) / side
+    P = np.zeros((n, abi.PARTICLE_FLOATS), dtype=np.float32)
+    P[:, :3] = pos.astype(np.float32)
+    P[:, 3] = 1.0
+    P[:, 7] = 1.0
+    return P
+
+
+def box_params(side: int, n: int, capacity: int, gravity: float = 9.81, cfl: float = 0.5, tol: float = 1e-6,
+               iters: int = 200) -> abi.Params:
+    """Params of a unit box of side^3 cells (dx = 1/side), particle spacing dx/2."""
+    dx = np.float32(1.0 / side)
+    U = abi.Params()
+    U.origin[:] = [0.0, 0.0, 0.0, dx]
+    U.dims[:] = [side, side, side, n]
+    U.bdims[:] = [side // 4, side // 4, side // 4, capacity]
+    U.gravity[:] = [0.0, -gravity, 0.0, abi.SOUND]
+    U.fluid[:] = [abi.RHO_WATER, cfl, abi.DT_MAX, np.float32(0.5 * dx)]
+    U.wet[:] = [WET_HALF, 0.0, 0.0, 0.0]
+    U.limits[:] = [WET_RES, SPRAY_CAP, iters, 0xFFFFFFFF]
+    U.solve[:] = [tol, 0.0, 0.0, 0.0]
+    U.mem[:] = [0, 2, 2, 0]
+    U.knot[:] = [0.0, 0.0, 0.0, 0.0]
+    return U
+
+
+def no_limits_state() -> abi.State:
+    """First State with no rubber or air step limit (neither is simulated)."""
+    st = abi.State()
+    st.mem_dt = np.finfo(np.float32).max
+    st.cap_dt = np.finfo(np.float32).max
+    st.air_dt = np.finfo(np.float32).max
+    return st
+
+
+def water_sphere(side: int = 64, n: int = 32768, seed: int = 1, fixed_dt: float = 1e-4, precond: int = abi.PRECOND_JACOBI,
+                 tol: float = 1e-6, iters: int = 200, capacity: int | None = None) -> Scene:
+    """The plain water sphere: n particles in a unit box of side^3 cells, solid walls, zero
+    initial velocity, gravity 9.81, fixed dt, Jacobi CG with tolerance 1e-6 and at most 200
+    iterations. With the defaults: 32,768 particles on 64^3, dt = 1e-4."""
+    cap = capacity or min((side // 4) ** 3, abi.MAX_BLOCKS)
+    U = box_params(side, n, cap, tol=tol, iters=iters)
+    sc = abi.Scene()
+    sc.flags = abi.SCENE_WALLS
+    sc.precond = precond
+    sc.precision = abi.CG_FP32
+    sc.fixed_dt = fixed_dt
+    P = sphere_particles(side, n, seed)
+    r_eff = (3.0 * (n // 8) / (4.0 * math.pi)) ** (1.0 / 3.0) / side
+    return Scene(U, P, sc, no_limits_state(),
+                 {"workload": f"water sphere {n} particles {side}^3", "r_eff": r_eff, "dx": 1.0 / side})
+
+
+def balloon_impact(side: int = 256, n: int = 2_097_152, seed: int = 2, speed: float = 300.0, radius: float = 0.01,
+                   half_length: float = 0.015, precond: int = abi.PRECOND_BLOCK_SGS, precision: int = abi.CG_FP32,
+                   refresh_every: int = 20, tol: float = 1e-6, iters: int = 1000, capacity: int = abi.MAX_BLOCKS,
+                   cfl: float = 0.5, direction=(1.0, 0.0, 0.0), mask: bool = True) -> Scene:
+    """The balloon impact: n particles (8 per cell) in a water sphere at the centre of a unit
+    box of side^3 cells with solid walls; a rigid capsule of the given radius moving at `speed`
+    along `direction` starts two cells short of the water on the -direction side and enters it
+    as a moving solid; a fixed spherical mask at the water's radius stands in for the
+    membrane; dt from CFL `cfl`. With the defaults: 2,097,152 particles on 256^3."""
+    U = box_params(side, n, capacity, cfl=cfl, tol=tol, iters=iters)
+    P = sphere_particles(side, n, seed)
+    dx = 1.0 / side
+    r_eff = (3.0 * (n // 8) / (4.0 * math.pi)) ** (1.0 / 3.0) * dx
+    dvec = np.asarray(direction, dtype=np.float64)
+    dvec = dvec / np.linalg.norm(dvec)
+    centre = np.array([0.5, 0.5, 0.5])
+    nose = centre - dvec * (r_eff + 2.0 * dx)
+    b = nose - dvec * radius
+    a = b - dvec * (2.0 * half_length)
+    sc = abi.Scene()
+    sc.flags = abi.SCENE_WALLS | abi.SCENE_CAPSULE | (abi.SCENE_MASK if mask else 0)
+    sc.precond = precond
+    sc.precision = precision
+    sc.refresh_every = refresh_every if precision == abi.CG_MIXED else 0
+    sc.capsule_a[:] = [a[0], a[1], a[2], radius]
+    sc.capsule_b[:] = [b[0], b[1], b[2], 0.0]
+    sc.capsule_v[:] = [speed * dvec[0], speed * dvec[1], speed * dvec[2], 0.0]
+    sc.mask[:] = [0.5, 0.5, 0.5, r_eff]
+    sc.mask_p = 0.0
+    return Scene(U, P, sc, no_limits_state(),
+                 {"workload": f"balloon impact {n} particles {side}^3", "r_eff": r_eff, "dx": dx})

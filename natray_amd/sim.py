@@ -1,0 +1,154 @@
+"""`Sim`: the reference's simulation handle (src/sim.rs:149-205) over the C ABI of libnatray.
+
+    sim = Sim.from_scene(scenes.water_sphere())
+    sim.advance(t_target=5e-3, budget=1000)      # == Sim::advance (src/sim.rs:498-522)
+    state = sim.run(10, iters=200)                # == Sim::run (src/sim.rs:585-648)
+    P = sim.particles()                           # (N, 20) float32 records, the Particle layout
+
+Every call goes through libnatray.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .params import SimParams
+from .scenes import Scene, reference_scene
+
+
+class Sim:
+    def __init__(self, params: abi.Params, particles: np.ndarray, scene: abi.Scene | None = None,
+                 init: abi.State | None = None):
+        L = abi.lib()
+        P = np.ascontiguousarray(particles, dtype=np.float32)
+        assert P.ndim == 2 and P.shape[1] == abi.PARTICLE_FLOATS
+        self.n = int(P.shape[0])
+        self.params = params
+        self.scene = scene
+        h = C.c_void_p()
+        abi.check(L.natray_sim_create(C.byref(params), abi.ptr(P), self.n, C.byref(scene) if scene is not None else None,
+                                      C.byref(init) if init is not None else None, C.byref(h)))
+        self._h = h
+        self.S = int(params.bdims[0] * params.bdims[1] * params.bdims[2])
+        self.dx = float(params.origin[3])
+        self.grid_origin = tuple(float(x) for x in params.origin[:3])
+        self.dims = tuple(int(x) for x in params.dims[:3])
+
+    @classmethod
+    def from_scene(cls, s: Scene) -> "Sim":
+        return cls(s.params, s.particles, s.scene, s.init)
+
+    @classmethod
+    def new(cls, p: SimParams) -> "Sim":
+        """Sim::new(&SimParams) (src/sim.rs:208) with the round-balloon stand-in profile."""
+        return cls.from_scene(reference_scene(p))
+
+    # ---- lifetime ---------------------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            abi.lib().natray_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- stepping ---------------------------------------------------------------------------
+    def run(self, n_steps: int, iters: int = 0, stop_ns: int = 0xFFFFFFFF) -> abi.State:
+        st = abi.State()
+        abi.check(abi.lib().natray_sim_run(self._h, n_steps, iters, stop_ns, C.byref(st)))
+        return st
+
+    def advance(self, t_target: float, budget: int) -> int:
+        taken = C.c_uint32()
+        abi.check(abi.lib().natray_sim_advance(self._h, t_target, budget, C.byref(taken)))
+        return int(taken.value)
+
+    def step(self) -> abi.State:
+        st = abi.State()
+        abi.check(abi.lib().natray_sim_step(self._h, C.byref(st)))
+        return st
+
+    def phase(self, k: int) -> None:
+        abi.check(abi.lib().natray_sim_phase(self._h, k))
+
+    # ---- data -------------------------------------------------------------------------------
+    def state(self) -> abi.State:
+        st = abi.State()
+        abi.check(abi.lib().natray_sim_read_state(self._h, C.byref(st)))
+        return st
+
+    def write_state(self, st: abi.State) -> None:
+        abi.check(abi.lib().natray_sim_write_state(self._h, C.byref(st)))
+
+    def write_params(self, U: abi.Params) -> None:
+        abi.check(abi.lib().natray_sim_write_params(self._h, C.byref(U)))
+        self.params = U
+
+    def particles(self) -> np.ndarray:
+        P = np.empty((self.n, abi.PARTICLE_FLOATS), dtype=np.float32)
+        abi.check(abi.lib().natray_sim_read_particles(self._h, abi.ptr(P)))
+        return P
+
+    def write_particles(self, P: np.ndarray) -> None:
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        assert P.shape == (self.n, abi.PARTICLE_FLOATS)
+        abi.check(abi.lib().natray_sim_write_particles(self._h, abi.ptr(P)))
+
+    def n_blocks(self) -> int:
+        n = C.c_uint32()
+        abi.check(abi.lib().natray_sim_read_grid(self._h, C.byref(n), None, None, None, None, None))
+        return int(n.value)
+
+    def grid(self) -> dict:
+        """The blocks in use: block_list, block_nb, block_start, acc (19, 64n), cf (11, 64n)."""
+        n = self.n_blocks()
+        bl = np.empty(max(n, 1), np.uint32)
+        nb = np.empty(max(n, 1) * 27, np.uint32)
+        bs = np.empty(n + 1, np.uint32)
+        acc = np.empty((abi.ACC_FIELDS, max(n, 1) * 64), np.int32)
+        cf = np.empty((abi.CELL_FIELDS, max(n, 1) * 64), np.float32)
+        nn = C.c_uint32()
+        abi.check(abi.lib().natray_sim_read_grid(self._h, C.byref(nn), abi.ptr(bl), abi.ptr(nb), abi.ptr(bs),
+                                                  abi.ptr(acc), abi.ptr(cf)))
+        return {"n_blocks": n, "block_list": bl[:n], "block_nb": nb[: 27 * n].reshape(n, 27), "block_start": bs,
+                "acc": acc[:, : 64 * n], "cf": cf[:, : 64 * n]}
+
+    def write_grid(self, acc: np.ndarray | None, cf: np.ndarray | None) -> None:
+        a = None if acc is None else np.ascontiguousarray(acc, dtype=np.int32)
+        f = None if cf is None else np.ascontiguousarray(cf, dtype=np.float32)
+        abi.check(abi.lib().natray_sim_write_grid(self._h, abi.ptr(a), abi.ptr(f)))
+
+    def bins(self) -> tuple[np.ndarray, np.ndarray]:
+        bp = np.empty(self.S, np.uint32)
+        so = np.empty(max(self.n, 1), np.uint32)
+        abi.check(abi.lib().natray_sim_read_bins(self._h, abi.ptr(bp), abi.ptr(so)))
+        return bp, so[: self.n]
+
+    # ---- measurement ------------------------------------------------------------------------
+    def set_profiling(self, on: bool) -> None:
+        abi.check(abi.lib().natray_sim_set_profiling(self._h, 1 if on else 0))
+
+    def phase_times(self) -> dict:
+        ms = np.zeros(abi.N_PHASES, np.float64)
+        cnt = np.zeros(abi.N_PHASES, np.uint32)
+        abi.check(abi.lib().natray_sim_phase_times(self._h, abi.ptr(ms), abi.ptr(cnt)))
+        return {abi.PHASE_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(abi.N_PHASES)}
+
+    def cg_stats(self) -> tuple[int, int, int]:
+        out = np.zeros(3, np.uint64)
+        abi.check(abi.lib().natray_sim_cg_stats(self._h, abi.ptr(out)))
+        return int(out[0]), int(out[1]), int(out[2])
+
+    def stream_ptr(self) -> int:
+        return int(abi.lib().natray_sim_device_ptr(self._h, abi.BUF_STREAM) or 0)

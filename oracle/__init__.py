@@ -1,0 +1,165 @@
+"""CPU oracle of the natray water substep — TEST INFRASTRUCTURE ONLY.
+
+Python view of oracle/liboracle.so (natray_oracle.c, built by oracle/Makefile). Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg use it, as the checker. `Oracle` has
+the same methods as natray_amd.Sim so parity tests read the same on both sides.
+
+    o = Oracle.from_scene(scene, precision="f32")   # FP32-faithful restatement
+    o = Oracle.from_scene(scene, precision="f64")   # FP64 oracle of the error metrics
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from natray_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        for pfx in ("o32_", "o64_"):
+            sig = {
+                "create": (P, [C.POINTER(abi.Params), P, C.c_uint32, C.POINTER(abi.Scene), C.POINTER(abi.State)]),
+                "destroy": (None, [P]),
+                "phase": (C.c_int, [P, C.c_int]),
+                "run": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_uint32]),
+                "read_state": (None, [P, C.POINTER(abi.State)]),
+                "write_state": (None, [P, C.POINTER(abi.State)]),
+                "read_particles": (None, [P, P]),
+                "read_particles_f64": (None, [P, P]),
+                "write_particles": (None, [P, P]),
+                "n_blocks": (C.c_uint32, [P]),
+                "read_grid": (None, [P, P, P, P, P, P]),
+                "read_cf_f64": (None, [P, P]),
+                "write_grid": (None, [P, P, P]),
+                "read_bins": (None, [P, P, P]),
+                "solver_scalars": (None, [P, P]),
+                "set_threads": (None, [C.c_int]),
+            }
+            for name, (res, args) in sig.items():
+                fn = getattr(L, pfx + name)
+                fn.restype = res
+                fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    lib().o32_set_threads(n)
+    lib().o64_set_threads(n)
+
+
+class Oracle:
+    def __init__(self, params: abi.Params, particles: np.ndarray, scene: abi.Scene | None = None,
+                 init: abi.State | None = None, precision: str = "f32"):
+        assert precision in ("f32", "f64")
+        self.pfx = "o32_" if precision == "f32" else "o64_"
+        self._f = lambda name: getattr(lib(), self.pfx + name)
+        P = np.ascontiguousarray(particles, dtype=np.float32)
+        self.n = int(P.shape[0])
+        self.params = params
+        self.S = int(params.bdims[0] * params.bdims[1] * params.bdims[2])
+        h = self._f("create")(C.byref(params), abi.ptr(P), self.n, C.byref(scene) if scene is not None else None,
+                              C.byref(init) if init is not None else None)
+        if not h:
+            raise MemoryError("oracle allocation failed")
+        self._h = C.c_void_p(h)
+
+    @classmethod
+    def from_scene(cls, s, precision: str = "f32") -> "Oracle":
+        return cls(s.params, s.particles, s.scene, s.init, precision)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._f("destroy")(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def phase(self, k: int) -> int:
+        return int(self._f("phase")(self._h, k))
+
+    def run(self, n_steps: int, iters: int, stop_ns: int = 0xFFFFFFFF) -> abi.State:
+        self._f("run")(self._h, n_steps, iters, stop_ns)
+        return self.state()
+
+    def state(self) -> abi.State:
+        st = abi.State()
+        self._f("read_state")(self._h, C.byref(st))
+        return st
+
+    def write_state(self, st: abi.State) -> None:
+        self._f("write_state")(self._h, C.byref(st))
+
+    def particles(self) -> np.ndarray:
+        P = np.empty((self.n, abi.PARTICLE_FLOATS), np.float32)
+        self._f("read_particles")(self._h, abi.ptr(P))
+        return P
+
+    def particles_f64(self) -> np.ndarray:
+        P = np.empty((self.n, abi.PARTICLE_FLOATS), np.float64)
+        self._f("read_particles_f64")(self._h, abi.ptr(P))
+        return P
+
+    def write_particles(self, P: np.ndarray) -> None:
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        self._f("write_particles")(self._h, abi.ptr(P))
+
+    def n_blocks(self) -> int:
+        return int(self._f("n_blocks")(self._h))
+
+    def grid(self) -> dict:
+        n = self.n_blocks()
+        bl = np.empty(max(n, 1), np.uint32)
+        nb = np.empty(max(n, 1) * 27, np.uint32)
+        bs = np.empty(n + 1, np.uint32)
+        acc = np.empty((abi.ACC_FIELDS, max(n, 1) * 64), np.int32)
+        cf = np.empty((abi.CELL_FIELDS, max(n, 1) * 64), np.float32)
+        self._f("read_grid")(self._h, abi.ptr(bl), abi.ptr(nb), abi.ptr(bs), abi.ptr(acc), abi.ptr(cf))
+        return {"n_blocks": n, "block_list": bl[:n], "block_nb": nb[: 27 * n].reshape(n, 27), "block_start": bs,
+                "acc": acc[:, : 64 * n], "cf": cf[:, : 64 * n]}
+
+    def cf_f64(self) -> np.ndarray:
+        n = self.n_blocks()
+        cf = np.empty((abi.CELL_FIELDS, max(n, 1) * 64), np.float64)
+        self._f("read_cf_f64")(self._h, abi.ptr(cf))
+        return cf[:, : 64 * n]
+
+    def write_grid(self, acc: np.ndarray | None, cf: np.ndarray | None) -> None:
+        a = None if acc is None else np.ascontiguousarray(acc, dtype=np.int32)
+        f = None if cf is None else np.ascontiguousarray(cf, dtype=np.float32)
+        self._f("write_grid")(self._h, abi.ptr(a), abi.ptr(f))
+
+    def bins(self) -> tuple[np.ndarray, np.ndarray]:
+        bp = np.empty(self.S, np.uint32)
+        so = np.empty(max(self.n, 1), np.uint32)
+        self._f("read_bins")(self._h, abi.ptr(bp), abi.ptr(so))
+        return bp, so[: self.n]
+
+    def solver_scalars(self) -> dict:
+        out = np.zeros(8, np.float64)
+        self._f("solver_scalars")(self._h, abi.ptr(out))
+        keys = ["rz", "alpha", "beta", "b_norm", "r_norm", "solve_iters", "max_speed", "cg_iters"]
+        return dict(zip(keys, out.tolist()))

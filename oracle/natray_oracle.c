@@ -1,0 +1,1262 @@
+/* CPU oracle of the natray water substep — TEST INFRASTRUCTURE ONLY (see natray_oracle.h).
+ *
+ * Compiled with -DREAL=float -DPFX=o32_ (FP32-faithful) and -DREAL=double -DPFX=o64_ (FP64).
+ * Build with -ffp-contract=off and without -ffast-math: every f32 expression rounds the way
+ * the WGSL source writes it, left to right.
+ *
+ * Each function below restates one reference kernel (file:line given at the function). The
+ * deliberate, documented differences from the reference are:
+ *   - block order: allocated blocks are listed in ascending domain-slot order, not in the
+ *     order atomic claims happen to win (fluid.wgsl:109-125). The reference's order is
+ *     nondeterministic; every result that does not depend on float summation order is the
+ *     same for any order.
+ *   - ranks inside a block: particle index order (the reference's atomicAdd order is
+ *     nondeterministic; nothing downstream depends on it).
+ *   - solver reductions over blocks: per-block tree as block_sum (fluid.wgsl:414-422), then
+ *     blocks are folded left to right in fixed chunks of K = ceil(n/1024), then the chunk
+ *     sums go through the reference's total() (fluid.wgsl:426-439). For n <= 1024 this is
+ *     exactly the reference's total() over blocks in slot order.
+ *   - out-of-scope hooks are stubbed as the reference behaves with no membrane, air or
+ *     bullet: GAS pressure 0, acc fields 9..18 zero, carve/break_up/ground_touch no-ops
+ *     (death only clears vel.w).
+ *   - scene extensions (box walls, moving capsule solid, spherical mask, fixed dt, Jacobi
+ *     preconditioner, mixed-precision CG) are defined here first and the HIP kernels follow.
+ *
+ * What pins this restatement (tests/test_oracle_*.py): brute-force NumPy goldens for P2G
+ * fixed-point sums and G2P gathers on small inputs (tests/golden/), closed forms (B-spline
+ * partition of unity, vanishing first moment, APIC affine reproduction, free-fall grid
+ * velocity), binning invariants against brute force, the CG solution against a direct
+ * scipy solve of the same matrix, symmetry and definiteness of the block-SGS
+ * preconditioner probed with unit vectors, and the begin_step clock against a table.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "natray_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef REAL
+#define REAL float
+#endif
+#ifndef PFX
+#define PFX o32_
+#endif
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define F(name) CAT(PFX, name)
+
+#define IS_F64 (sizeof(REAL) == 8)
+
+static inline REAL rfloor(REAL x) { return IS_F64 ? (REAL)floor((double)x) : (REAL)floorf((float)x); }
+static inline REAL sqrt_r(REAL x) { return IS_F64 ? (REAL)sqrt((double)x) : (REAL)sqrtf((float)x); }
+static inline REAL rrint(REAL x) { return IS_F64 ? (REAL)rint((double)x) : (REAL)rintf((float)x); }
+static inline REAL rmin(REAL a, REAL b) { return b < a ? b : a; }
+static inline REAL rmax(REAL a, REAL b) { return a < b ? b : a; }
+static inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+static inline uint32_t umax(uint32_t a, uint32_t b) { return a < b ? b : a; }
+
+typedef struct { int x, y, z; } i3;
+typedef struct { REAL x, y, z; } r3;
+
+#define GAS ((REAL)0)
+#define WATER ((REAL)1)
+#define SOLID ((REAL)2)
+#define COVERED ((REAL)3)
+#define RUBBER ((REAL)4)
+#define NONE NATRAY_NONE
+#define PENDING NATRAY_PENDING
+#define TILE 343u
+
+struct osim {
+  natray_params U;
+  natray_scene sc;
+  natray_state st; /* integer members are authoritative; float members mirror the REALs below */
+  double max_speed, rz, alpha, beta, b_norm, r_norm;
+  int mixed;   /* dot products, norms and the convergence test in double */
+  int refresh; /* recompute r = b - A p every sc.refresh_every iterations */
+  uint32_t N, S, cap;
+  REAL* P;               /* N x 20 */
+  uint32_t* block_ptr;   /* S */
+  uint32_t* slot_count;  /* S */
+  uint32_t* block_list;  /* cap */
+  uint32_t* block_nb;    /* cap x 27 */
+  uint32_t* block_start; /* cap + 1 */
+  uint32_t* sorted;      /* N */
+  uint32_t* rank;        /* N */
+  int32_t* acc;          /* 19 x cap x 64 */
+  REAL* cf;              /* 11 x cap x 64 */
+  REAL* bvec;            /* cap x 64: right-hand side kept for the true-residual refresh */
+  double* part;          /* cap x 4: per-block partial sums */
+  uint64_t cg_iters;
+};
+
+/* ---- Params / State accessors (sim_defs.wgsl:159-231) -------------------------------- */
+
+static inline uint32_t cap_cells(const osim* s) { return s->U.bdims[3] * 64u; }
+static inline REAL hh_(const osim* s) { return (REAL)s->U.origin[3]; }
+/* dt() and now(): f32(ns) * 1e-9 (sim_defs.wgsl:167-173) */
+static inline REAL dt_(const osim* s) { return (REAL)s->st.dt_ns * (REAL)1e-9; }
+static inline REAL now_(const osim* s) { return (REAL)s->st.t_ns * (REAL)1e-9; }
+
+static inline uint32_t block_slot(const osim* s, i3 b) {
+  return ((uint32_t)b.x * s->U.bdims[1] + (uint32_t)b.y) * s->U.bdims[2] + (uint32_t)b.z;
+}
+static inline i3 slot_coord(const osim* s, uint32_t sl) {
+  i3 r;
+  r.x = (int)(sl / (s->U.bdims[2] * s->U.bdims[1]));
+  r.y = (int)((sl / s->U.bdims[2]) % s->U.bdims[1]);
+  r.z = (int)(sl % s->U.bdims[2]);
+  return r;
+}
+static inline i3 local_coord(uint32_t i) {
+  uint32_t l = i % 64u;
+  i3 r = {(int)(l / 16u), (int)((l / 4u) % 4u), (int)(l % 4u)};
+  return r;
+}
+static inline i3 cell_coord(const osim* s, uint32_t i) {
+  i3 b = slot_coord(s, s->block_list[i / 64u]), l = local_coord(i);
+  i3 r = {b.x * 4 + l.x, b.y * 4 + l.y, b.z * 4 + l.z};
+  return r;
+}
+/* neighbour (sim_defs.wgsl:195-202) */
+static inline uint32_t neighbour(const osim* s, uint32_t cell, int ox, int oy, int oz) {
+  i3 l = local_coord(cell);
+  int qx = l.x + ox, qy = l.y + oy, qz = l.z + oz;
+  int nx = (qx >= 4) - (qx < 0), ny = (qy >= 4) - (qy < 0), nz = (qz >= 4) - (qz < 0);
+  uint32_t b = s->block_nb[(cell / 64u) * 27u + (uint32_t)((nx + 1) * 9 + (ny + 1) * 3 + nz + 1)];
+  if (b == NONE) return NONE;
+  return b * 64u + (uint32_t)(qx - nx * 4) * 16u + (uint32_t)(qy - ny * 4) * 4u + (uint32_t)(qz - nz * 4);
+}
+static inline int32_t* accp(osim* s, uint32_t f, uint32_t cell) { return &s->acc[(size_t)f * cap_cells(s) + cell]; }
+static inline int32_t accv(const osim* s, uint32_t f, uint32_t cell) { return s->acc[(size_t)f * cap_cells(s) + cell]; }
+static inline REAL* cfp(osim* s, uint32_t f, uint32_t cell) { return &s->cf[(size_t)f * cap_cells(s) + cell]; }
+static inline REAL cfv(const osim* s, uint32_t f, uint32_t cell) { return s->cf[(size_t)f * cap_cells(s) + cell]; }
+static inline REAL cell_type(const osim* s, uint32_t cell) { return cell == NONE ? GAS : cfv(s, NATRAY_F_TYPE, cell); }
+
+/* to_fp / from_fp (sim_defs.wgsl:217-222); WGSL round() is ties-to-even. */
+static inline int32_t to_fp(REAL x, REAL scale) {
+  REAL v = x * scale;
+  v = rmin(rmax(v, (REAL)-2e9), (REAL)2e9);
+  return (int32_t)rrint(v);
+}
+static inline REAL from_fp(int32_t x, REAL scale) { return (REAL)x / scale; }
+
+/* ---- scene geometry (reference solids fluid.wgsl:305-314; extensions defined here) ----- */
+
+static inline r3 cell_centre(const osim* s, i3 c) {
+  REAL h = hh_(s);
+  r3 p = {(REAL)s->U.origin[0] + ((REAL)c.x + (REAL)0.5) * h, (REAL)s->U.origin[1] + ((REAL)c.y + (REAL)0.5) * h,
+          (REAL)s->U.origin[2] + ((REAL)c.z + (REAL)0.5) * h};
+  return p;
+}
+static inline int is_ground(const osim* s, i3 c) {
+  return (REAL)s->U.origin[1] + ((REAL)c.y + (REAL)0.5) * hh_(s) < (REAL)0;
+}
+static inline int in_knot(const osim* s, i3 c) {
+  r3 q = cell_centre(s, c);
+  REAL px = q.x - (REAL)s->U.knot[0], py = q.y - (REAL)s->U.knot[1], pz = q.z - (REAL)s->U.knot[2];
+  REAL r = (REAL)s->U.knot[3] + (REAL)0.5 * hh_(s);
+  return py > (REAL)0 && py < (REAL)2 * r && sqrt_r(px * px + pz * pz) < r;
+}
+static inline int in_walls(const osim* s, i3 c) {
+  return c.x == 0 || c.y == 0 || c.z == 0 || c.x == (int)s->U.dims[0] - 1 || c.y == (int)s->U.dims[1] - 1 ||
+         c.z == (int)s->U.dims[2] - 1;
+}
+/* Closest point on the capsule axis at time t, and the squared distance to it. */
+static inline REAL capsule_d2(const osim* s, r3 p, REAL t, r3* on_out, r3* ab_out) {
+  const natray_scene* sc = &s->sc;
+  r3 a = {(REAL)sc->capsule_a[0] + (REAL)sc->capsule_v[0] * t, (REAL)sc->capsule_a[1] + (REAL)sc->capsule_v[1] * t,
+          (REAL)sc->capsule_a[2] + (REAL)sc->capsule_v[2] * t};
+  r3 ab = {(REAL)sc->capsule_b[0] - (REAL)sc->capsule_a[0], (REAL)sc->capsule_b[1] - (REAL)sc->capsule_a[1],
+           (REAL)sc->capsule_b[2] - (REAL)sc->capsule_a[2]};
+  REAL ll = (ab.x * ab.x + ab.y * ab.y) + ab.z * ab.z;
+  REAL f = (((p.x - a.x) * ab.x + (p.y - a.y) * ab.y) + (p.z - a.z) * ab.z) / rmax(ll, (REAL)1e-20);
+  f = rmin(rmax(f, (REAL)0), (REAL)1);
+  r3 on = {a.x + f * ab.x, a.y + f * ab.y, a.z + f * ab.z};
+  r3 d = {p.x - on.x, p.y - on.y, p.z - on.z};
+  if (on_out) *on_out = on;
+  if (ab_out) *ab_out = ab;
+  return (d.x * d.x + d.y * d.y) + d.z * d.z;
+}
+static inline int in_capsule(const osim* s, i3 c) {
+  REAL r = (REAL)s->sc.capsule_a[3];
+  return capsule_d2(s, cell_centre(s, c), now_(s), NULL, NULL) < r * r;
+}
+static inline int in_mask(const osim* s, i3 c) {
+  r3 p = cell_centre(s, c);
+  REAL dx = p.x - (REAL)s->sc.mask[0], dy = p.y - (REAL)s->sc.mask[1], dz = p.z - (REAL)s->sc.mask[2];
+  REAL r = (REAL)s->sc.mask[3];
+  return (dx * dx + dy * dy) + dz * dz < r * r;
+}
+static inline int is_solid_cell(const osim* s, i3 c) {
+  uint32_t fl = s->sc.flags;
+  if ((fl & NATRAY_SCENE_GROUND) && is_ground(s, c)) return 1;
+  if ((fl & NATRAY_SCENE_KNOT) && in_knot(s, c)) return 1;
+  if ((fl & NATRAY_SCENE_WALLS) && in_walls(s, c)) return 1;
+  if ((fl & NATRAY_SCENE_CAPSULE) && in_capsule(s, c)) return 1;
+  return 0;
+}
+/* Velocity component d of the solid in `cell` (0 for the ground, knot and walls). */
+static inline REAL solid_u(const osim* s, uint32_t cell, uint32_t d) {
+  if (!(s->sc.flags & NATRAY_SCENE_CAPSULE) || cell == NONE) return (REAL)0;
+  return in_capsule(s, cell_coord(s, cell)) ? (REAL)s->sc.capsule_v[d] : (REAL)0;
+}
+
+/* bounded (fluid.wgsl:360-365), with a moving solid's velocity in place of 0. */
+static inline REAL bounded(const osim* s, REAL u, REAL hi, REAL lo, uint32_t hi_cell, uint32_t lo_cell, uint32_t d) {
+  if (lo == SOLID && hi == SOLID) return (REAL)0;
+  if (lo == SOLID) return rmax(u, solid_u(s, lo_cell, d));
+  if (hi == SOLID) return rmin(u, solid_u(s, hi_cell, d));
+  return u;
+}
+/* face_u (fluid.wgsl:368-371) */
+static inline REAL face_u(const osim* s, uint32_t cell, uint32_t lo, uint32_t d) {
+  if (cell == NONE) return (REAL)0;
+  return bounded(s, cfv(s, NATRAY_F_U + d, cell), cfv(s, NATRAY_F_TYPE, cell), cell_type(s, lo), cell, lo, d);
+}
+/* pressure_of (fluid.wgsl:375-379) */
+static inline REAL pressure_of(const osim* s, uint32_t cell) {
+  if (cell == NONE) return (REAL)0;
+  REAL p = cfv(s, NATRAY_F_P, cell);
+  return cfv(s, NATRAY_F_TYPE, cell) == WATER ? rmax(p, (REAL)NATRAY_P_FLOOR) : p;
+}
+
+static inline uint32_t n_blocks_(const osim* s) { return umin(s->st.n_blocks, s->U.bdims[3]); }
+
+static void sync_state_floats(osim* s) {
+  s->st.max_speed = (float)s->max_speed;
+  s->st.rz = (float)s->rz;
+  s->st.alpha = (float)s->alpha;
+  s->st.beta = (float)s->beta;
+  s->st.b_norm = (float)s->b_norm;
+  s->st.r_norm = (float)s->r_norm;
+}
+
+/* ---- begin_step (fluid.wgsl:31-89) --------------------------------------------------- */
+
+static void o_begin_step(osim* s) {
+  natray_state* st = &s->st;
+  REAL vmax = (REAL)s->max_speed;
+  uint32_t arrive = s->U.limits[3];
+  /* The moving capsule counts as a bullet in flight (fluid.wgsl:39-50). */
+  if (s->sc.flags & NATRAY_SCENE_CAPSULE) {
+    REAL vx = (REAL)s->sc.capsule_v[0], vy = (REAL)s->sc.capsule_v[1], vz = (REAL)s->sc.capsule_v[2];
+    vmax = rmax(vmax, sqrt_r((vx * vx + vy * vy) + vz * vz));
+  }
+  REAL mem_dt = (REAL)st->mem_dt, air_dt = (REAL)st->air_dt;
+  REAL dt_s = rmin((REAL)s->U.fluid[2], (REAL)st->cap_dt);
+  if (vmax > (REAL)0) dt_s = rmin(dt_s, (REAL)s->U.fluid[1] * hh_(s) / vmax);
+  st->dt_flow = (float)dt_s;
+  st->mem_dt_seen = (float)mem_dt;
+  st->air_dt_seen = (float)air_dt;
+  dt_s = rmin(dt_s, rmin((REAL)s->U.mem[1] * mem_dt, (REAL)s->U.mem[2] * air_dt));
+  if (st->free_dt_ns > 0u) dt_s = rmin(dt_s, (REAL)1.25 * (REAL)st->free_dt_ns * (REAL)1e-9);
+  uint32_t free_ns = umax((uint32_t)(dt_s * (REAL)1e9), 20u);
+  /* A fixed step is rounded to whole nanoseconds once, the same way in every precision. */
+  if (s->sc.fixed_dt > 0.0f) free_ns = umax((uint32_t)floor((double)s->sc.fixed_dt * 1e9 + 0.5), 20u);
+  uint32_t stop = s->U.limits[3];
+  int live = st->t_ns < stop;
+  uint32_t dt_ns = 0;
+  if (live) {
+    dt_ns = umin(free_ns, umin(stop, arrive) - st->t_ns);
+    st->free_dt_ns = free_ns;
+    st->t_ns += dt_ns;
+    st->step += 1u;
+    s->max_speed = 0.0;
+    st->mem_dt = FLT_MAX;
+    st->cap_dt = FLT_MAX;
+    st->air_dt = FLT_MAX;
+    st->mem_ticket = 0;
+    st->n_blocks_prev = n_blocks_(s);
+    st->n_blocks = 0;
+  }
+  st->dt_ns = dt_ns;
+  sync_state_floats(s);
+}
+
+/* ---- binning: clear_blocks, mark_blocks/claim, scan_blocks, sort_particles -------------- */
+
+static inline i3 own_cell(const osim* s, const REAL* p) {
+  /* own_cell (fluid.wgsl:127-129): floor((p - origin) / h) */
+  REAL h = hh_(s);
+  i3 c = {(int)rfloor((p[0] - (REAL)s->U.origin[0]) / h), (int)rfloor((p[1] - (REAL)s->U.origin[1]) / h),
+          (int)rfloor((p[2] - (REAL)s->U.origin[2]) / h)};
+  return c;
+}
+static inline int in_domain(const osim* s, i3 c) {
+  return c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < (int)s->U.dims[0] && c.y < (int)s->U.dims[1] &&
+         c.z < (int)s->U.dims[2];
+}
+static inline int floor4(int v) { return v >> 2; } /* arithmetic shift, as (c - 1) >> 2 in WGSL */
+static inline int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static void o_bin(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  const uint32_t cap = cap_cells(s);
+  /* clear_blocks (fluid.wgsl:91-107): free last step's blocks, zero their transfer sums. */
+  uint32_t nprev = s->st.n_blocks_prev;
+  for (uint32_t b = 0; b < nprev; ++b) {
+    uint32_t sl = s->block_list[b];
+    s->block_ptr[sl] = NONE;
+    s->slot_count[sl] = 0;
+  }
+  for (uint32_t f = 0; f < NATRAY_ACC_FIELDS; ++f) memset(&s->acc[(size_t)f * cap], 0, (size_t)nprev * 64u * 4u);
+  /* mark_blocks (fluid.wgsl:131-150): claim every block the stencil (cells c-1..c+2) touches,
+   * take a rank in the own block. Claims are recorded as PENDING and given indices in slot
+   * order below. */
+  int bmx = (int)s->U.bdims[0] - 1, bmy = (int)s->U.bdims[1] - 1, bmz = (int)s->U.bdims[2] - 1;
+  for (uint32_t i = 0; i < s->N; ++i) {
+    const REAL* p = &s->P[(size_t)i * 20];
+    if (p[7] < (REAL)0.5) continue;
+    i3 c = own_cell(s, p);
+    if (!in_domain(s, c)) continue; /* the reference indexes out of bounds here; such particles are not binned */
+    int lx = iclamp(floor4(c.x - 1), 0, bmx), ly = iclamp(floor4(c.y - 1), 0, bmy), lz = iclamp(floor4(c.z - 1), 0, bmz);
+    int hx = iclamp(floor4(c.x + 2), 0, bmx), hy = iclamp(floor4(c.y + 2), 0, bmy), hz = iclamp(floor4(c.z + 2), 0, bmz);
+    for (int x = lx; x <= hx; ++x)
+      for (int y = ly; y <= hy; ++y)
+        for (int z = lz; z <= hz; ++z) {
+          i3 b = {x, y, z};
+          s->block_ptr[block_slot(s, b)] = PENDING;
+        }
+    i3 ob = {c.x >> 2, c.y >> 2, c.z >> 2};
+    s->rank[i] = s->slot_count[block_slot(s, ob)]++;
+  }
+  /* Block indices in slot order; overflowing claims are dropped (claim, fluid.wgsl:117-122). */
+  uint32_t n = 0, bcap = s->U.bdims[3];
+  for (uint32_t sl = 0; sl < s->S; ++sl) {
+    if (s->block_ptr[sl] != PENDING) continue;
+    if (n >= bcap) {
+      s->st.overflow = 1u;
+      s->block_ptr[sl] = NONE;
+    } else {
+      s->block_list[n] = sl;
+      s->block_ptr[sl] = n;
+    }
+    ++n;
+  }
+  s->st.n_blocks = n;
+  /* scan_blocks (fluid.wgsl:157-187): exclusive prefix sum of the blocks' own counts. */
+  uint32_t nb = n_blocks_(s), run = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    s->block_start[b] = run;
+    run += s->slot_count[s->block_list[b]];
+  }
+  s->block_start[nb] = run;
+  /* sort_particles (fluid.wgsl:189-196) */
+  for (uint32_t i = 0; i < s->N; ++i) {
+    const REAL* p = &s->P[(size_t)i * 20];
+    if (p[7] < (REAL)0.5) continue;
+    i3 c = own_cell(s, p);
+    if (!in_domain(s, c)) continue;
+    i3 ob = {c.x >> 2, c.y >> 2, c.z >> 2};
+    uint32_t b = s->block_ptr[block_slot(s, ob)];
+    if (b >= PENDING) continue;
+    s->sorted[s->block_start[b] + s->rank[i]] = i;
+  }
+}
+
+/* ---- p2g (fluid.wgsl:198-303) ---------------------------------------------------------- */
+
+/* axis_weights (fluid.wgsl:221-223) */
+static inline void axis_weights(REAL fx, REAL w[3]) {
+  w[0] = (REAL)0.5 * ((REAL)1.5 - fx) * ((REAL)1.5 - fx);
+  w[1] = (REAL)0.75 - (fx - (REAL)1) * (fx - (REAL)1);
+  w[2] = (REAL)0.5 * (fx - (REAL)0.5) * (fx - (REAL)0.5);
+}
+
+/* splat (fluid.wgsl:233-252): one lattice of one particle into the block's tile. */
+static void splat(int32_t* tile, const osim* s, const REAL x[3], const REAL o[3], i3 corner, uint32_t fa, REAL val,
+                  const REAL grad[3], uint32_t fw, REAL scale) {
+  int base[3];
+  REAL fx[3], wx[3], wy[3], wz[3], hg[3];
+  for (int k = 0; k < 3; ++k) {
+    base[k] = (int)rfloor(x[k] - o[k] - (REAL)0.5);
+    fx[k] = x[k] - o[k] - (REAL)base[k];
+    hg[k] = grad[k] * hh_(s);
+  }
+  axis_weights(fx[0], wx);
+  axis_weights(fx[1], wy);
+  axis_weights(fx[2], wz);
+  uint32_t t0 = (uint32_t)(base[0] - corner.x) * 49u + (uint32_t)(base[1] - corner.y) * 7u + (uint32_t)(base[2] - corner.z);
+  for (uint32_t a = 0; a < 3; ++a)
+    for (uint32_t b = 0; b < 3; ++b)
+      for (uint32_t c = 0; c < 3; ++c) {
+        REAL weight = wx[a] * wy[b] * wz[c];
+        uint32_t t = t0 + a * 49u + b * 7u + c;
+        REAL node = val + ((hg[0] * ((REAL)a - fx[0]) + hg[1] * ((REAL)b - fx[1])) + hg[2] * ((REAL)c - fx[2]));
+        tile[fa * TILE + t] += to_fp(weight * node, scale);
+        tile[fw * TILE + t] += to_fp(weight, (REAL)NATRAY_FP_W);
+      }
+}
+
+/* tile_cell (fluid.wgsl:210-217) */
+static inline uint32_t tile_cell(const uint32_t links[27], uint32_t t) {
+  int rx = (int)(t / 49u) - 1, ry = (int)((t / 7u) % 7u) - 1, rz = (int)(t % 7u) - 1;
+  int nx = (rx + 4) / 4 - 1, ny = (ry + 4) / 4 - 1, nz = (rz + 4) / 4 - 1;
+  uint32_t b = links[(nx + 1) * 9 + (ny + 1) * 3 + nz + 1];
+  if (b == NONE) return NONE;
+  return b * 64u + (uint32_t)(rx - nx * 4) * 16u + (uint32_t)(ry - ny * 4) * 4u + (uint32_t)(rz - nz * 4);
+}
+
+static void o_p2g(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  uint32_t nb = n_blocks_(s);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (uint32_t b = 0; b < nb; ++b) {
+    int32_t tile[8 * TILE];
+    uint32_t own[64];
+    uint32_t links[27];
+    i3 bc = slot_coord(s, s->block_list[b]);
+    for (uint32_t li = 0; li < 27; ++li) {
+      i3 q = {bc.x + (int)(li / 9u) - 1, bc.y + (int)((li / 3u) % 3u) - 1, bc.z + (int)(li % 3u) - 1};
+      uint32_t idx = NONE;
+      if (q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < (int)s->U.bdims[0] && q.y < (int)s->U.bdims[1] &&
+          q.z < (int)s->U.bdims[2]) {
+        idx = s->block_ptr[block_slot(s, q)];
+        if (idx >= PENDING) idx = NONE;
+      }
+      links[li] = idx;
+      s->block_nb[b * 27u + li] = idx;
+    }
+    uint32_t count = s->block_start[b + 1] - s->block_start[b];
+    if (count == 0u) {
+      for (uint32_t li = 0; li < 64; ++li) *accp(s, NATRAY_A_N, b * 64u + li) = 0;
+      continue;
+    }
+    memset(tile, 0, sizeof tile);
+    memset(own, 0, sizeof own);
+    i3 corner = {bc.x * 4 - 1, bc.y * 4 - 1, bc.z * 4 - 1};
+    uint32_t first = s->block_start[b];
+    REAL h = hh_(s);
+    for (uint32_t k = 0; k < count; ++k) {
+      const REAL* p = &s->P[(size_t)s->sorted[first + k] * 20];
+      REAL x[3] = {(p[0] - (REAL)s->U.origin[0]) / h, (p[1] - (REAL)s->U.origin[1]) / h, (p[2] - (REAL)s->U.origin[2]) / h};
+      static const REAL ou[3] = {0, 0.5, 0.5}, ov[3] = {0.5, 0, 0.5}, ow[3] = {0.5, 0.5, 0}, op[3] = {0.5, 0.5, 0.5};
+      static const REAL zero[3] = {0, 0, 0};
+      splat(tile, s, x, ou, corner, NATRAY_A_MOM(0), p[4], &p[8], NATRAY_A_WT(0), (REAL)NATRAY_FP_M);
+      splat(tile, s, x, ov, corner, NATRAY_A_MOM(1), p[5], &p[12], NATRAY_A_WT(1), (REAL)NATRAY_FP_M);
+      splat(tile, s, x, ow, corner, NATRAY_A_MOM(2), p[6], &p[16], NATRAY_A_WT(2), (REAL)NATRAY_FP_M);
+      splat(tile, s, x, op, corner, NATRAY_A_PC, p[11] * (REAL)1e-6, zero, NATRAY_A_WC, (REAL)NATRAY_FP_P);
+      i3 c = own_cell(s, p);
+      own[(uint32_t)(c.x - bc.x * 4) * 16u + (uint32_t)(c.y - bc.y * 4) * 4u + (uint32_t)(c.z - bc.z * 4)] += 1u;
+    }
+    for (uint32_t t = 0; t < TILE; ++t) {
+      uint32_t cell = tile_cell(links, t);
+      if (cell == NONE) continue;
+      for (uint32_t f = 0; f < 8u; ++f) {
+        int32_t v = tile[f * TILE + t];
+        if (v != 0) {
+          int32_t* dst = accp(s, f, cell);
+#pragma omp atomic
+          *dst += v;
+        }
+      }
+    }
+    for (uint32_t li = 0; li < 64; ++li) *accp(s, NATRAY_A_N, b * 64u + li) = (int32_t)own[li];
+  }
+}
+
+/* ---- grid_prep (fluid.wgsl:319-356) ----------------------------------------------------- */
+
+static void o_grid_prep(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  uint32_t nc = n_blocks_(s) * 64u;
+  REAL dt = dt_(s);
+#pragma omp parallel for schedule(static)
+  for (uint32_t cell = 0; cell < nc; ++cell) {
+    REAL ty = GAS;
+    i3 c = cell_coord(s, cell);
+    if (is_solid_cell(s, c)) {
+      ty = SOLID;
+    } else if (accv(s, NATRAY_A_N, cell) > 0) {
+      ty = WATER;
+    }
+    *cfp(s, NATRAY_F_TYPE, cell) = ty;
+    for (uint32_t d = 0; d < 3; ++d) {
+      REAL w = from_fp(accv(s, NATRAY_A_WT(d), cell), (REAL)NATRAY_FP_W);
+      REAL u = 0;
+      if (w > (REAL)1e-6) u = from_fp(accv(s, NATRAY_A_MOM(d), cell), (REAL)NATRAY_FP_M) / w + dt * (REAL)s->U.gravity[d];
+      *cfp(s, NATRAY_F_U + d, cell) = u;
+    }
+    REAL wc = from_fp(accv(s, NATRAY_A_WC, cell), (REAL)NATRAY_FP_W);
+    REAL ma = from_fp(accv(s, NATRAY_A_MA, cell), (REAL)NATRAY_FP_A);
+    if (ty == GAS && accv(s, NATRAY_A_MW, cell) > 0) {
+      ty = RUBBER;
+      *cfp(s, NATRAY_F_TYPE, cell) = ty;
+    }
+    REAL p = 0;
+    if (ty == WATER && wc > (REAL)1e-6) {
+      p = from_fp(accv(s, NATRAY_A_PC, cell), (REAL)NATRAY_FP_P) / wc * (REAL)1e6;
+    } else if (ty != WATER && ty != SOLID && ma > (REAL)0) {
+      ty = COVERED;
+      *cfp(s, NATRAY_F_TYPE, cell) = ty;
+      p = from_fp(accv(s, NATRAY_A_MF, cell), (REAL)NATRAY_FP_F) / ma * (REAL)1e6;
+    } else if (ty != WATER && ty != SOLID && (s->sc.flags & NATRAY_SCENE_MASK) && in_mask(s, c)) {
+      /* The fixed spherical mask stands in for the membrane: the water there is held at
+       * the mask's pressure, as behind the rubber. */
+      ty = COVERED;
+      *cfp(s, NATRAY_F_TYPE, cell) = ty;
+      p = (REAL)s->sc.mask_p;
+    }
+    *cfp(s, NATRAY_F_P, cell) = p;
+  }
+}
+
+/* ---- reductions (fluid.wgsl:412-439 and the chunk fold above) ---------------------------- */
+
+/* block_sum's tree over one block's 64 values (fluid.wgsl:414-422). */
+static double tree64(double v[64], int dbl) {
+  if (dbl) {
+    for (uint32_t off = 32; off > 0; off >>= 1)
+      for (uint32_t li = 0; li < off; ++li) v[li] += v[li + off];
+    return v[0];
+  }
+  REAL r[64];
+  for (int i = 0; i < 64; ++i) r[i] = (REAL)v[i];
+  for (uint32_t off = 32; off > 0; off >>= 1)
+    for (uint32_t li = 0; li < off; ++li) r[li] += r[li + off];
+  return (double)r[0];
+}
+
+/* Sum of component k of the n block partials: fold in chunks of K = ceil(n/1024), then the
+ * reference's total() (fluid.wgsl:426-439) over the chunk sums. */
+static double total(const osim* s, uint32_t n, int k, int dbl) {
+  if (n == 0) return 0.0;
+  uint32_t K = (n + 1023u) / 1024u, m = (n + K - 1u) / K;
+  double chunk[1024];
+  double red[256];
+  if (dbl) {
+    for (uint32_t j = 0; j < m; ++j) {
+      double acc = 0;
+      for (uint32_t b = j * K; b < umin(n, j * K + K); ++b) acc += s->part[b * 4 + k];
+      chunk[j] = acc;
+    }
+    for (uint32_t li = 0; li < 256; ++li) {
+      double acc = 0;
+      for (uint32_t i = li; i < m; i += 256) acc += chunk[i];
+      red[li] = acc;
+    }
+    for (uint32_t off = 128; off > 0; off >>= 1)
+      for (uint32_t li = 0; li < off; ++li) red[li] += red[li + off];
+    return red[0];
+  }
+  REAL rc[1024], rr[256];
+  for (uint32_t j = 0; j < m; ++j) {
+    REAL acc = 0;
+    for (uint32_t b = j * K; b < umin(n, j * K + K); ++b) acc += (REAL)s->part[b * 4 + k];
+    rc[j] = acc;
+  }
+  for (uint32_t li = 0; li < 256; ++li) {
+    REAL acc = 0;
+    for (uint32_t i = li; i < m; i += 256) acc += rc[i];
+    rr[li] = acc;
+  }
+  for (uint32_t off = 128; off > 0; off >>= 1)
+    for (uint32_t li = 0; li < off; ++li) rr[li] += rr[li + off];
+  (void)chunk;
+  (void)red;
+  return (double)rr[0];
+}
+
+/* converged (fluid.wgsl:441-443) */
+static int converged(const osim* s, double rr) {
+  if (s->mixed) return sqrt(rr) <= (double)s->U.solve[0] * (s->b_norm < 1.0 ? 1.0 : s->b_norm);
+  return sqrt_r((REAL)rr) <= (REAL)s->U.solve[0] * rmax((REAL)s->b_norm, (REAL)1);
+}
+
+/* ---- precondition (fluid.wgsl:384-410) and the Jacobi alternative ------------------------- */
+
+static void precondition(const osim* s, const REAL r[64], const REAL diag[64], const int water[64], REAL z[64]) {
+  if (s->sc.precond == NATRAY_PRECOND_JACOBI) {
+    for (int li = 0; li < 64; ++li) z[li] = water[li] ? r[li] / diag[li] : (REAL)0;
+    return;
+  }
+  REAL zl[64];
+  for (int li = 0; li < 64; ++li) zl[li] = 0;
+  for (uint32_t k = 0; k < 5u; ++k) {
+    /* One colour at a time; cells of one colour read only the other colour. */
+    for (uint32_t li = 0; li < 64; ++li) {
+      uint32_t lx = li / 16u, ly = (li / 4u) % 4u, lz = li % 4u;
+      uint32_t colour = (lx + ly + lz) % 2u;
+      if (!(water[li] && colour == k % 2u)) continue;
+      REAL sum = r[li];
+      if (lx > 0 && water[li - 16]) sum += zl[li - 16];
+      if (lx < 3 && water[li + 16]) sum += zl[li + 16];
+      if (ly > 0 && water[li - 4]) sum += zl[li - 4];
+      if (ly < 3 && water[li + 4]) sum += zl[li + 4];
+      if (lz > 0 && water[li - 1]) sum += zl[li - 1];
+      if (lz < 3 && water[li + 1]) sum += zl[li + 1];
+      zl[li] = sum / diag[li];
+    }
+  }
+  for (int li = 0; li < 64; ++li) z[li] = zl[li];
+}
+
+/* product of two REALs in the reduction's precision */
+static inline double prod(const osim* s, REAL a, REAL b) { return s->mixed ? (double)a * (double)b : (double)(a * b); }
+
+/* ---- solve_init + solve_start (fluid.wgsl:445-506) ---------------------------------------- */
+
+static void o_solve_init(osim* s) {
+  uint32_t nb = n_blocks_(s);
+  REAL h = hh_(s), dt = dt_(s);
+#pragma omp parallel for schedule(static)
+  for (uint32_t blk = 0; blk < nb; ++blk) {
+    REAL r[64], diag[64], bb[64], z[64];
+    int water[64];
+    for (uint32_t li = 0; li < 64; ++li) {
+      uint32_t cell = blk * 64u + li;
+      water[li] = cfv(s, NATRAY_F_TYPE, cell) == WATER;
+      diag[li] = 1;
+      r[li] = 0;
+      bb[li] = 0;
+      if (!water[li]) continue;
+      REAL cs = (REAL)s->U.gravity[3] * dt;
+      REAL kc = h * h / (cs * cs);
+      REAL div = 0, off = 0, dg = kc, b = kc * cfv(s, NATRAY_F_P, cell);
+      for (uint32_t d = 0; d < 3; ++d) {
+        int e[3] = {0, 0, 0};
+        e[d] = 1;
+        uint32_t up = neighbour(s, cell, e[0], e[1], e[2]);
+        uint32_t dn = neighbour(s, cell, -e[0], -e[1], -e[2]);
+        div += face_u(s, up, cell, d) - face_u(s, cell, dn, d);
+        for (int sg = -1; sg <= 1; sg += 2) {
+          uint32_t nbc = neighbour(s, cell, sg * e[0], sg * e[1], sg * e[2]);
+          REAL nt = cell_type(s, nbc);
+          if (nt == SOLID) continue;
+          dg += (REAL)1;
+          if (nt == WATER) {
+            off -= cfv(s, NATRAY_F_P, nbc);
+          } else {
+            b += pressure_of(s, nbc);
+          }
+        }
+      }
+      div -= from_fp(accv(s, NATRAY_A_SRC, cell), (REAL)NATRAY_FP_Q) / (h * h);
+      b -= (REAL)s->U.fluid[0] * h / dt * div;
+      REAL rr = b - dg * cfv(s, NATRAY_F_P, cell) - off;
+      *cfp(s, NATRAY_F_DIAG, cell) = dg;
+      *cfp(s, NATRAY_F_R, cell) = rr;
+      s->bvec[cell] = b;
+      diag[li] = dg;
+      r[li] = rr;
+      bb[li] = b;
+    }
+    precondition(s, r, diag, water, z);
+    double v0[64], v1[64], v2[64];
+    for (uint32_t li = 0; li < 64; ++li) {
+      if (water[li]) *cfp(s, NATRAY_F_Z, blk * 64u + li) = z[li];
+      v0[li] = prod(s, r[li], z[li]);
+      v1[li] = prod(s, r[li], r[li]);
+      v2[li] = prod(s, bb[li], bb[li]);
+    }
+    s->part[blk * 4 + 0] = tree64(v0, s->mixed);
+    s->part[blk * 4 + 1] = tree64(v1, s->mixed);
+    s->part[blk * 4 + 2] = tree64(v2, s->mixed);
+    s->part[blk * 4 + 3] = 0;
+  }
+}
+
+/* Returns whether the iterations run. */
+static int o_solve_start(osim* s) {
+  uint32_t nb = n_blocks_(s);
+  int dbl = s->mixed;
+  double rz = total(s, nb, 0, dbl), rr = total(s, nb, 1, dbl), bb = total(s, nb, 2, dbl);
+  s->rz = rz;
+  s->beta = 0;
+  if (dbl) {
+    s->b_norm = sqrt(bb);
+    s->r_norm = sqrt(rr);
+  } else {
+    s->b_norm = (double)sqrt_r((REAL)bb);
+    s->r_norm = (double)sqrt_r((REAL)rr);
+  }
+  s->st.solve_iters = 0;
+  sync_state_floats(s);
+  return rz > 0.0 && !converged(s, rr);
+}
+
+/* ---- one CG iteration: solve_q, solve_alpha, solve_update, solve_beta (fluid.wgsl:508-578) -- */
+
+static inline REAL direction(const osim* s, uint32_t cell, uint32_t k, REAL beta) {
+  return cfv(s, NATRAY_F_Z, cell) + beta * cfv(s, NATRAY_F_D + k, cell);
+}
+
+static void o_solve_q(osim* s) {
+  uint32_t nb = n_blocks_(s);
+  uint32_t old = s->st.solve_iters % 2u;
+  REAL beta = (REAL)s->beta;
+  /* Every block reads its neighbours' old direction field and writes only the new one. */
+#pragma omp parallel for schedule(static)
+  for (uint32_t blk = 0; blk < nb; ++blk) {
+    double v[64];
+    for (uint32_t li = 0; li < 64; ++li) {
+      uint32_t cell = blk * 64u + li;
+      v[li] = 0;
+      if (cfv(s, NATRAY_F_TYPE, cell) != WATER) continue;
+      REAL dn = direction(s, cell, old, beta);
+      REAL q = cfv(s, NATRAY_F_DIAG, cell) * dn;
+      for (uint32_t d = 0; d < 3; ++d) {
+        int e[3] = {0, 0, 0};
+        e[d] = 1;
+        for (int k = -1; k <= 1; k += 2) {
+          uint32_t nbc = neighbour(s, cell, k * e[0], k * e[1], k * e[2]);
+          if (cell_type(s, nbc) == WATER) q -= direction(s, nbc, old, beta);
+        }
+      }
+      *cfp(s, NATRAY_F_D + 1u - old, cell) = dn;
+      *cfp(s, NATRAY_F_Q, cell) = q;
+      v[li] = prod(s, dn, q);
+    }
+    s->part[blk * 4 + 0] = tree64(v, s->mixed);
+  }
+}
+
+static void o_solve_alpha(osim* s) {
+  double dq = total(s, n_blocks_(s), 0, s->mixed);
+  if (s->mixed)
+    s->alpha = dq > 0.0 ? s->rz / dq : 0.0;
+  else
+    s->alpha = dq > 0.0 ? (double)((REAL)s->rz / (REAL)dq) : 0.0;
+  sync_state_floats(s);
+}
+
+/* r = b - A p at the water cells of block blk, accumulated in double (mixed precision's
+ * true-residual refresh). */
+static REAL true_residual(const osim* s, uint32_t cell) {
+  double ap = (double)cfv(s, NATRAY_F_DIAG, cell) * (double)cfv(s, NATRAY_F_P, cell);
+  for (uint32_t d = 0; d < 3; ++d) {
+    int e[3] = {0, 0, 0};
+    e[d] = 1;
+    for (int k = -1; k <= 1; k += 2) {
+      uint32_t nbc = neighbour(s, cell, k * e[0], k * e[1], k * e[2]);
+      if (cell_type(s, nbc) == WATER) ap -= (double)cfv(s, NATRAY_F_P, nbc);
+    }
+  }
+  return (REAL)((double)s->bvec[cell] - ap);
+}
+
+static void o_solve_update(osim* s) {
+  uint32_t nb = n_blocks_(s);
+  REAL a = (REAL)s->alpha;
+  uint32_t newd = NATRAY_F_D + 1u - s->st.solve_iters % 2u;
+  int refresh = s->refresh && (s->st.solve_iters + 1u) % s->sc.refresh_every == 0u;
+#pragma omp parallel for schedule(static)
+  for (uint32_t blk = 0; blk < nb; ++blk)
+    for (uint32_t li = 0; li < 64; ++li) {
+      uint32_t cell = blk * 64u + li;
+      if (cfv(s, NATRAY_F_TYPE, cell) != WATER) continue;
+      *cfp(s, NATRAY_F_P, cell) += a * cfv(s, newd, cell);
+      *cfp(s, NATRAY_F_R, cell) = cfv(s, NATRAY_F_R, cell) - a * cfv(s, NATRAY_F_Q, cell);
+    }
+  if (refresh) {
+#pragma omp parallel for schedule(static)
+    for (uint32_t blk = 0; blk < nb; ++blk)
+      for (uint32_t li = 0; li < 64; ++li) {
+        uint32_t cell = blk * 64u + li;
+        if (cfv(s, NATRAY_F_TYPE, cell) == WATER) *cfp(s, NATRAY_F_R, cell) = true_residual(s, cell);
+      }
+  }
+#pragma omp parallel for schedule(static)
+  for (uint32_t blk = 0; blk < nb; ++blk) {
+    REAL r[64], diag[64], z[64];
+    int water[64];
+    for (uint32_t li = 0; li < 64; ++li) {
+      uint32_t cell = blk * 64u + li;
+      water[li] = cfv(s, NATRAY_F_TYPE, cell) == WATER;
+      r[li] = water[li] ? cfv(s, NATRAY_F_R, cell) : (REAL)0;
+      diag[li] = water[li] ? cfv(s, NATRAY_F_DIAG, cell) : (REAL)1;
+    }
+    precondition(s, r, diag, water, z);
+    double v0[64], v1[64];
+    for (uint32_t li = 0; li < 64; ++li) {
+      if (water[li]) *cfp(s, NATRAY_F_Z, blk * 64u + li) = z[li];
+      v0[li] = prod(s, r[li], z[li]);
+      v1[li] = prod(s, r[li], r[li]);
+    }
+    s->part[blk * 4 + 0] = tree64(v0, s->mixed);
+    s->part[blk * 4 + 1] = tree64(v1, s->mixed);
+  }
+}
+
+/* Returns whether the solver keeps going. */
+static int o_solve_beta(osim* s) {
+  uint32_t nb = n_blocks_(s);
+  int dbl = s->mixed;
+  double rz = total(s, nb, 0, dbl), rr = total(s, nb, 1, dbl);
+  s->st.solve_iters += 1u;
+  if (dbl) {
+    s->r_norm = sqrt(rr);
+    s->beta = s->rz > 0.0 ? rz / s->rz : 0.0;
+  } else {
+    s->r_norm = (double)sqrt_r((REAL)rr);
+    s->beta = s->rz > 0.0 ? (double)((REAL)rz / (REAL)s->rz) : 0.0;
+  }
+  s->rz = rz;
+  int done = converged(s, rr);
+  int go = 1;
+  if (done || s->st.solve_iters >= s->U.limits[2]) {
+    s->st.iters_max = umax(s->st.iters_max, s->st.solve_iters);
+    s->st.unconverged += done ? 0u : 1u;
+    go = 0;
+  }
+  sync_state_floats(s);
+  return go;
+}
+
+static void o_solve(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  o_solve_init(s);
+  int go = o_solve_start(s) && s->U.limits[2] > 0u;
+  while (go) {
+    o_solve_q(s);
+    o_solve_alpha(s);
+    o_solve_update(s);
+    go = o_solve_beta(s);
+    s->cg_iters++;
+  }
+}
+
+/* ---- project (fluid.wgsl:580-598) ------------------------------------------------------------ */
+
+static void o_project(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  uint32_t nc = n_blocks_(s) * 64u;
+  REAL k = dt_(s) / ((REAL)s->U.fluid[0] * hh_(s));
+#pragma omp parallel for schedule(static)
+  for (uint32_t cell = 0; cell < nc; ++cell) {
+    REAL ty = cfv(s, NATRAY_F_TYPE, cell);
+    for (uint32_t d = 0; d < 3; ++d) {
+      int e[3] = {0, 0, 0};
+      e[d] = 1;
+      uint32_t nbc = neighbour(s, cell, -e[0], -e[1], -e[2]);
+      REAL nt = cell_type(s, nbc);
+      REAL u = bounded(s, cfv(s, NATRAY_F_U + d, cell), ty, nt, cell, nbc, d);
+      if (ty != SOLID && nt != SOLID && (ty == WATER || nt == WATER)) u -= k * (pressure_of(s, cell) - pressure_of(s, nbc));
+      *cfp(s, NATRAY_F_U + d, cell) = u;
+    }
+  }
+}
+
+/* ---- extend_1, extend_2 (fluid.wgsl:600-674) --------------------------------------------------- */
+
+static inline int face_set(const osim* s, uint32_t cell, uint32_t d) {
+  if (cell == NONE) return 0;
+  int e[3] = {0, 0, 0};
+  e[d] = 1;
+  return cfv(s, NATRAY_F_TYPE, cell) == WATER || cell_type(s, neighbour(s, cell, -e[0], -e[1], -e[2])) == WATER;
+}
+
+static void o_extend_1(osim* s) {
+  uint32_t nc = n_blocks_(s) * 64u;
+#pragma omp parallel for schedule(static)
+  for (uint32_t cell = 0; cell < nc; ++cell) {
+    uint32_t done = 0;
+    for (uint32_t d = 0; d < 3; ++d) {
+      REAL u = cfv(s, NATRAY_F_U + d, cell);
+      if (face_set(s, cell, d)) {
+        done |= 1u << d;
+      } else {
+        REAL sum = 0, n = 0;
+        for (int k = 0; k < 6; ++k) {
+          int e[3] = {0, 0, 0};
+          e[k / 2] = (k % 2 == 1) ? 1 : -1;
+          uint32_t nbc = neighbour(s, cell, e[0], e[1], e[2]);
+          if (face_set(s, nbc, d)) {
+            sum += cfv(s, NATRAY_F_U + d, nbc);
+            n += (REAL)1;
+          }
+        }
+        if (n > (REAL)0) {
+          u = sum / n;
+          done |= 1u << d;
+        }
+      }
+      *cfp(s, NATRAY_F_EXT + d, cell) = u;
+    }
+    *cfp(s, NATRAY_F_EXT_SET, cell) = (REAL)done;
+  }
+}
+
+static inline int ext_set(const osim* s, uint32_t cell, uint32_t d) {
+  return cell != NONE && ((((uint32_t)cfv(s, NATRAY_F_EXT_SET, cell)) >> d) & 1u) != 0u;
+}
+
+static void o_extend_2(osim* s) {
+  uint32_t nc = n_blocks_(s) * 64u;
+#pragma omp parallel for schedule(static)
+  for (uint32_t cell = 0; cell < nc; ++cell) {
+    REAL ty = cfv(s, NATRAY_F_TYPE, cell);
+    for (uint32_t d = 0; d < 3; ++d) {
+      REAL u = cfv(s, NATRAY_F_U + d, cell);
+      if (ext_set(s, cell, d)) {
+        u = cfv(s, NATRAY_F_EXT + d, cell);
+      } else {
+        REAL sum = 0, n = 0;
+        for (int k = 0; k < 6; ++k) {
+          int e[3] = {0, 0, 0};
+          e[k / 2] = (k % 2 == 1) ? 1 : -1;
+          uint32_t nbc = neighbour(s, cell, e[0], e[1], e[2]);
+          if (ext_set(s, nbc, d)) {
+            sum += cfv(s, NATRAY_F_EXT + d, nbc);
+            n += (REAL)1;
+          }
+        }
+        if (n > (REAL)0) u = sum / n;
+      }
+      int e[3] = {0, 0, 0};
+      e[d] = 1;
+      uint32_t lo = neighbour(s, cell, -e[0], -e[1], -e[2]);
+      *cfp(s, NATRAY_F_U + d, cell) = bounded(s, u, ty, cell_type(s, lo), cell, lo, d);
+    }
+  }
+}
+
+/* ---- g2p (fluid.wgsl:767-901) ------------------------------------------------------------------ */
+
+typedef struct { REAL s, g[3], w; } gathered;
+
+/* gather (fluid.wgsl:780-802) */
+static gathered gather(const osim* s, const REAL* tg, const REAL x[3], const REAL o[3], i3 corner, uint32_t fa, uint32_t fw) {
+  int base[3];
+  REAL fx[3], wx[3], wy[3], wz[3];
+  for (int k = 0; k < 3; ++k) {
+    base[k] = (int)rfloor(x[k] - o[k] - (REAL)0.5);
+    fx[k] = x[k] - o[k] - (REAL)base[k];
+  }
+  axis_weights(fx[0], wx);
+  axis_weights(fx[1], wy);
+  axis_weights(fx[2], wz);
+  uint32_t t0 = (uint32_t)(base[0] - corner.x) * 49u + (uint32_t)(base[1] - corner.y) * 7u + (uint32_t)(base[2] - corner.z);
+  gathered r = {0, {0, 0, 0}, 0};
+  for (uint32_t a = 0; a < 3; ++a)
+    for (uint32_t b = 0; b < 3; ++b)
+      for (uint32_t c = 0; c < 3; ++c) {
+        REAL weight = wx[a] * wy[b] * wz[c];
+        uint32_t t = t0 + a * 49u + b * 7u + c;
+        REAL u = weight * tg[fa * TILE + t];
+        r.s += u;
+        r.g[0] += u * ((REAL)a - fx[0]);
+        r.g[1] += u * ((REAL)b - fx[1]);
+        r.g[2] += u * ((REAL)c - fx[2]);
+        r.w += weight * tg[fw * TILE + t];
+      }
+  REAL h = hh_(s);
+  r.g[0] *= h;
+  r.g[1] *= h;
+  r.g[2] *= h;
+  return r;
+}
+
+/* carve for the moving capsule (the role of bullet.wgsl:395-424): a particle that ends a step
+ * inside the capsule goes onto its surface and takes at least the capsule's normal speed. */
+static void carve(const osim* s, REAL x[3], REAL v[3]) {
+  REAL r = (REAL)s->sc.capsule_a[3];
+  r3 p = {x[0], x[1], x[2]}, on, ab;
+  REAL d2 = capsule_d2(s, p, now_(s), &on, &ab);
+  if (d2 >= r * r) return;
+  REAL d[3] = {x[0] - on.x, x[1] - on.y, x[2] - on.z};
+  if (d2 < (REAL)1e-14) {
+    /* On the axis: push out along a fixed direction perpendicular to it. */
+    REAL ax = ab.x, ay = ab.y, az = ab.z;
+    /* cross(ab, (0, 1, 0)) or, for an axis along y, cross(ab, (1, 0, 0)) */
+    if (ax * ax + az * az > (REAL)1e-20) {
+      d[0] = -az; d[1] = 0; d[2] = ax;
+    } else {
+      d[0] = 0; d[1] = az; d[2] = -ay;
+    }
+    d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+  }
+  REAL len = sqrt_r(d2);
+  REAL dir[3] = {d[0] / len, d[1] / len, d[2] / len};
+  x[0] = on.x + dir[0] * r;
+  x[1] = on.y + dir[1] * r;
+  x[2] = on.z + dir[2] * r;
+  REAL wall = ((REAL)s->sc.capsule_v[0] * dir[0] + (REAL)s->sc.capsule_v[1] * dir[1]) + (REAL)s->sc.capsule_v[2] * dir[2];
+  REAL vn = (v[0] * dir[0] + v[1] * dir[1]) + v[2] * dir[2];
+  REAL push = rmax(wall - vn, (REAL)0);
+  v[0] += dir[0] * push;
+  v[1] += dir[1] * push;
+  v[2] += dir[2] * push;
+}
+
+/* g2p_one (fluid.wgsl:806-861). Returns the speed, 0 for a particle that died. */
+static REAL g2p_one(osim* s, const REAL* tg, uint32_t i, i3 corner, const uint32_t links[27]) {
+  REAL* p = &s->P[(size_t)i * 20];
+  REAL h = hh_(s);
+  REAL X[3] = {(p[0] - (REAL)s->U.origin[0]) / h, (p[1] - (REAL)s->U.origin[1]) / h, (p[2] - (REAL)s->U.origin[2]) / h};
+  static const REAL ou[3] = {0, 0.5, 0.5}, ov[3] = {0.5, 0, 0.5}, ow[3] = {0.5, 0.5, 0}, op[3] = {0.5, 0.5, 0.5};
+  gathered gu = gather(s, tg, X, ou, corner, NATRAY_A_MOM(0), NATRAY_A_WT(0));
+  gathered gv = gather(s, tg, X, ov, corner, NATRAY_A_MOM(1), NATRAY_A_WT(1));
+  gathered gw = gather(s, tg, X, ow, corner, NATRAY_A_MOM(2), NATRAY_A_WT(2));
+  gathered gp = gather(s, tg, X, op, corner, NATRAY_A_PC, NATRAY_A_WC);
+  REAL v[3] = {gu.s, gv.s, gw.s};
+  REAL m_near = gu.w + gv.w + gw.w;
+  REAL step = dt_(s);
+  REAL x[3] = {p[0] + step * v[0], p[1] + step * v[1], p[2] + step * v[2]};
+  if (s->sc.flags & NATRAY_SCENE_CAPSULE) carve(s, x, v);
+  REAL spacing = (REAL)s->U.fluid[3];
+  REAL lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = (REAL)s->U.origin[k] + (REAL)3.5 * h;
+    hi[k] = (REAL)s->U.origin[k] + ((REAL)s->U.dims[k] - (REAL)4.5) * h;
+  }
+  int escaped = x[0] < lo[0] || x[2] < lo[2] || x[0] > hi[0] || x[2] > hi[2] || x[1] > hi[1];
+  REAL fill = m_near / (REAL)24;
+  int on_ground = x[1] < (REAL)1.5 * spacing;
+  REAL r_thin = (REAL)0.75 * spacing;
+  REAL t_thin = (REAL)3 * sqrt_r((REAL)s->U.fluid[0] * r_thin * r_thin * r_thin / (REAL)NATRAY_SIGMA_W);
+  i3 cf_ = {(int)rfloor(X[0]), (int)rfloor(X[1]), (int)rfloor(X[2])};
+  uint32_t tix = (uint32_t)(cf_.x - corner.x) * 49u + (uint32_t)(cf_.y - corner.y) * 7u + (uint32_t)(cf_.z - corner.z);
+  uint32_t own = tile_cell(links, tix);
+  int covered = own != NONE && accv(s, NATRAY_A_MA, own) > 0;
+  if ((s->sc.flags & NATRAY_SCENE_MASK) && in_mask(s, cf_)) covered = 1;
+  REAL thin_age = (fill < (REAL)0.45 && !on_ground && !covered) ? p[15] + step : (REAL)0;
+  if (escaped || (!covered && fill < (REAL)0.2) || thin_age > t_thin) {
+    /* ground_touch / break_up are out of scope: the particle only dies. */
+    p[4] = v[0];
+    p[5] = v[1];
+    p[6] = v[2];
+    p[7] = 0;
+    return 0;
+  }
+  REAL xx[3] = {x[0], x[1], x[2]}, vv[3] = {v[0], v[1], v[2]};
+  if (xx[1] < (REAL)0.3 * spacing) {
+    xx[1] = (REAL)0.3 * spacing;
+    vv[1] = rmax(vv[1], (REAL)0);
+  }
+  REAL cs = (REAL)4 / (h * h);
+  p[0] = xx[0];
+  p[1] = xx[1];
+  p[2] = xx[2];
+  p[4] = vv[0];
+  p[5] = vv[1];
+  p[6] = vv[2];
+  p[7] = 1;
+  p[8] = gu.g[0] * cs;
+  p[9] = gu.g[1] * cs;
+  p[10] = gu.g[2] * cs;
+  p[11] = gp.w > (REAL)1e-6 ? gp.s / gp.w : (REAL)0;
+  p[12] = gv.g[0] * cs;
+  p[13] = gv.g[1] * cs;
+  p[14] = gv.g[2] * cs;
+  p[15] = thin_age;
+  p[16] = gw.g[0] * cs;
+  p[17] = gw.g[1] * cs;
+  p[18] = gw.g[2] * cs;
+  p[19] = 0;
+  return sqrt_r((vv[0] * vv[0] + vv[1] * vv[1]) + vv[2] * vv[2]);
+}
+
+static void o_g2p(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  uint32_t nb = n_blocks_(s);
+  double fastest_all = 0.0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(max : fastest_all)
+  for (uint32_t b = 0; b < nb; ++b) {
+    uint32_t count = s->block_start[b + 1] - s->block_start[b];
+    if (count == 0u) continue;
+    uint32_t links[27];
+    REAL tg[8 * TILE];
+    for (int li = 0; li < 27; ++li) links[li] = s->block_nb[b * 27u + (uint32_t)li];
+    for (uint32_t t = 0; t < TILE; ++t) {
+      uint32_t cell = tile_cell(links, t);
+      int water = cell != NONE && cfv(s, NATRAY_F_TYPE, cell) == WATER;
+      for (uint32_t d = 0; d < 3; ++d) {
+        REAL u = 0, w = 0;
+        if (cell != NONE) {
+          u = cfv(s, NATRAY_F_U + d, cell);
+          w = from_fp(accv(s, NATRAY_A_WT(d), cell), (REAL)NATRAY_FP_W);
+        }
+        tg[NATRAY_A_MOM(d) * TILE + t] = u;
+        tg[NATRAY_A_WT(d) * TILE + t] = w;
+      }
+      tg[NATRAY_A_PC * TILE + t] = water ? pressure_of(s, cell) : (REAL)0;
+      tg[NATRAY_A_WC * TILE + t] = water ? (REAL)1 : (REAL)0;
+    }
+    i3 bc = slot_coord(s, s->block_list[b]);
+    i3 corner = {bc.x * 4 - 1, bc.y * 4 - 1, bc.z * 4 - 1};
+    uint32_t first = s->block_start[b];
+    REAL fastest = 0;
+    for (uint32_t k = 0; k < count; ++k) fastest = rmax(fastest, g2p_one(s, tg, s->sorted[first + k], corner, links));
+    if ((double)fastest > fastest_all) fastest_all = (double)fastest;
+  }
+  if (fastest_all > s->max_speed) s->max_speed = fastest_all;
+  sync_state_floats(s);
+}
+
+/* ---- the plan (src/sim.rs:390-426, in-scope kernels only) ------------------------------------- */
+
+int F(phase)(osim* s, int phase) {
+  switch (phase) {
+    case NATRAY_PHASE_BEGIN: o_begin_step(s); break;
+    case NATRAY_PHASE_BIN: o_bin(s); break;
+    case NATRAY_PHASE_P2G: o_p2g(s); break;
+    case NATRAY_PHASE_GRID: o_grid_prep(s); break;
+    case NATRAY_PHASE_SOLVE: o_solve(s); break;
+    case NATRAY_PHASE_PROJECT: o_project(s); break;
+    case NATRAY_PHASE_EXTEND:
+      if (s->st.dt_ns != 0u) {
+        o_extend_1(s);
+        o_extend_2(s);
+      }
+      break;
+    case NATRAY_PHASE_G2P: o_g2p(s); break;
+    default: return NATRAY_E_ARG;
+  }
+  return s->st.overflow ? NATRAY_E_OVERFLOW : NATRAY_OK;
+}
+
+int F(run)(osim* s, uint32_t n_steps, uint32_t iters, uint32_t stop_ns) {
+  s->U.limits[2] = iters;
+  s->U.limits[3] = stop_ns;
+  s->st.iters_max = 0;
+  s->st.unconverged = 0;
+  for (uint32_t k = 0; k < n_steps; ++k)
+    for (int ph = 0; ph < NATRAY_N_PHASES; ++ph) F(phase)(s, ph);
+  return s->st.overflow ? NATRAY_E_OVERFLOW : NATRAY_OK;
+}
+
+/* ---- construction and access ---------------------------------------------------------------------- */
+
+osim* F(create)(const natray_params* params, const float* particles, uint32_t n, const natray_scene* scene,
+                const natray_state* init) {
+  osim* s = (osim*)calloc(1, sizeof(osim));
+  if (!s) return NULL;
+  s->U = *params;
+  if (scene) {
+    s->sc = *scene;
+  } else {
+    memset(&s->sc, 0, sizeof s->sc);
+    s->sc.flags = NATRAY_SCENE_REFERENCE;
+  }
+  s->mixed = IS_F64 || s->sc.precision == NATRAY_CG_MIXED;
+  s->refresh = s->sc.precision == NATRAY_CG_MIXED && s->sc.refresh_every > 0u;
+  s->N = n;
+  s->S = params->bdims[0] * params->bdims[1] * params->bdims[2];
+  s->cap = params->bdims[3];
+  size_t cells = (size_t)s->cap * 64u;
+  s->P = (REAL*)malloc((size_t)n * 20 * sizeof(REAL) + 1);
+  s->block_ptr = (uint32_t*)malloc((size_t)s->S * 4);
+  s->slot_count = (uint32_t*)calloc(s->S, 4);
+  s->block_list = (uint32_t*)calloc((size_t)s->cap + 1, 4);
+  s->block_nb = (uint32_t*)calloc((size_t)s->cap * 27 + 1, 4);
+  s->block_start = (uint32_t*)calloc((size_t)s->cap + 2, 4);
+  s->sorted = (uint32_t*)calloc((size_t)n + 1, 4);
+  s->rank = (uint32_t*)calloc((size_t)n + 1, 4);
+  s->acc = (int32_t*)calloc(NATRAY_ACC_FIELDS * cells + 1, 4);
+  s->cf = (REAL*)calloc(NATRAY_CELL_FIELDS * cells + 1, sizeof(REAL));
+  s->bvec = (REAL*)calloc(cells + 1, sizeof(REAL));
+  s->part = (double*)calloc((size_t)s->cap * 4 + 4, sizeof(double));
+  if (!s->P || !s->block_ptr || !s->slot_count || !s->block_list || !s->block_nb || !s->block_start || !s->sorted ||
+      !s->rank || !s->acc || !s->cf || !s->bvec || !s->part) {
+    F(destroy)(s);
+    return NULL;
+  }
+  for (uint32_t i = 0; i < s->S; ++i) s->block_ptr[i] = NONE;
+  for (size_t i = 0; i < (size_t)n * 20; ++i) s->P[i] = (REAL)particles[i];
+  if (init) {
+    F(write_state)(s, init);
+  } else {
+    /* src/sim.rs:240-241 */
+    memset(&s->st, 0, sizeof s->st);
+    float c_air = sqrtf(1.4f * 101325.0f / 1.204f);
+    s->st.mem_dt = FLT_MAX;
+    s->st.cap_dt = FLT_MAX;
+    s->st.air_dt = 0.9f * params->origin[3] / (3.0f * c_air);
+    s->max_speed = s->rz = s->alpha = s->beta = s->b_norm = s->r_norm = 0.0;
+  }
+  return s;
+}
+
+void F(destroy)(osim* s) {
+  if (!s) return;
+  free(s->P);
+  free(s->block_ptr);
+  free(s->slot_count);
+  free(s->block_list);
+  free(s->block_nb);
+  free(s->block_start);
+  free(s->sorted);
+  free(s->rank);
+  free(s->acc);
+  free(s->cf);
+  free(s->bvec);
+  free(s->part);
+  free(s);
+}
+
+void F(read_state)(const osim* s, natray_state* out) { *out = s->st; }
+void F(write_state)(osim* s, const natray_state* in) {
+  s->st = *in;
+  s->max_speed = in->max_speed;
+  s->rz = in->rz;
+  s->alpha = in->alpha;
+  s->beta = in->beta;
+  s->b_norm = in->b_norm;
+  s->r_norm = in->r_norm;
+}
+void F(read_particles)(const osim* s, float* dst) {
+  for (size_t i = 0; i < (size_t)s->N * 20; ++i) dst[i] = (float)s->P[i];
+}
+void F(read_particles_f64)(const osim* s, double* dst) {
+  for (size_t i = 0; i < (size_t)s->N * 20; ++i) dst[i] = (double)s->P[i];
+}
+void F(write_particles)(osim* s, const float* src) {
+  for (size_t i = 0; i < (size_t)s->N * 20; ++i) s->P[i] = (REAL)src[i];
+}
+uint32_t F(n_blocks)(const osim* s) { return n_blocks_(s); }
+void F(read_grid)(const osim* s, uint32_t* block_list, uint32_t* block_nb, uint32_t* block_start, int32_t* acc, float* cf) {
+  uint32_t n = n_blocks_(s), nc = n * 64u;
+  size_t cap = cap_cells(s);
+  if (block_list) memcpy(block_list, s->block_list, (size_t)n * 4);
+  if (block_nb) memcpy(block_nb, s->block_nb, (size_t)n * 27 * 4);
+  if (block_start) memcpy(block_start, s->block_start, ((size_t)n + 1) * 4);
+  if (acc)
+    for (uint32_t f = 0; f < NATRAY_ACC_FIELDS; ++f) memcpy(&acc[(size_t)f * nc], &s->acc[f * cap], (size_t)nc * 4);
+  if (cf)
+    for (uint32_t f = 0; f < NATRAY_CELL_FIELDS; ++f)
+      for (uint32_t c = 0; c < nc; ++c) cf[(size_t)f * nc + c] = (float)s->cf[f * cap + c];
+}
+void F(read_cf_f64)(const osim* s, double* cf) {
+  uint32_t nc = n_blocks_(s) * 64u;
+  size_t cap = cap_cells(s);
+  for (uint32_t f = 0; f < NATRAY_CELL_FIELDS; ++f)
+    for (uint32_t c = 0; c < nc; ++c) cf[(size_t)f * nc + c] = (double)s->cf[f * cap + c];
+}
+void F(write_grid)(osim* s, const int32_t* acc, const float* cf) {
+  uint32_t nc = n_blocks_(s) * 64u;
+  size_t cap = cap_cells(s);
+  if (acc)
+    for (uint32_t f = 0; f < NATRAY_ACC_FIELDS; ++f) memcpy(&s->acc[f * cap], &acc[(size_t)f * nc], (size_t)nc * 4);
+  if (cf)
+    for (uint32_t f = 0; f < NATRAY_CELL_FIELDS; ++f)
+      for (uint32_t c = 0; c < nc; ++c) s->cf[f * cap + c] = (REAL)cf[(size_t)f * nc + c];
+}
+void F(read_bins)(const osim* s, uint32_t* block_ptr, uint32_t* sorted) {
+  if (block_ptr) memcpy(block_ptr, s->block_ptr, (size_t)s->S * 4);
+  if (sorted) memcpy(sorted, s->sorted, (size_t)s->N * 4);
+}
+void F(solver_scalars)(const osim* s, double out[8]) {
+  out[0] = s->rz;
+  out[1] = s->alpha;
+  out[2] = s->beta;
+  out[3] = s->b_norm;
+  out[4] = s->r_norm;
+  out[5] = (double)s->st.solve_iters;
+  out[6] = s->max_speed;
+  out[7] = (double)s->cg_iters;
+}
+void F(set_threads)(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}

@@ -1,0 +1,140 @@
+"""GPU parity: every phase of the substep through the C ABI against the FP32 oracle.
+
+Both sides start from the same seeded scene and run phase by phase. Integer data (block
+lists, pointers, starts, the fixed-point transfer sums acc) must be bit-exact, and so must the
+float fields: the kernels evaluate every float expression in the oracle's order without
+contraction, and the solver reduces in the same fixed order on both sides. The one set that is
+compared as a multiset is `sorted` within each block (ranks come from atomics on the GPU).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from natray_amd import Sim, abi, scenes
+from natray_amd.params import SimParams
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_reference():
+    s = scenes.reference_scene(SimParams(resolution=16), capacity=4096)
+    s.params.limits[2] = 64
+    s.params.limits[3] = 0xFFFFFFFF
+    return s
+
+
+SCENES = {
+    "sphere_jacobi_32k": lambda: scenes.water_sphere(),
+    "impact_sgs_32k": lambda: scenes.balloon_impact(side=64, n=32768, seed=3, iters=400, capacity=4096),
+    "impact_mixed_32k": lambda: scenes.balloon_impact(side=64, n=32768, seed=4, iters=400, capacity=4096,
+                                                       precision=abi.CG_MIXED, refresh_every=5),
+    "reference_balloon_res16": small_reference,
+}
+
+
+def assert_same_float(name, a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, (name, a.shape, b.shape)
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        bad = np.argwhere(~same)
+        i = tuple(bad[0])
+        d = np.abs(a.astype(np.float64) - b.astype(np.float64))
+        raise AssertionError(f"{name}: {len(bad)} of {a.size} differ; first {i}: gpu {a[i]!r} oracle {b[i]!r}; max abs diff {d.max():.3e}")
+
+
+def compare_grid(tag, g, o, cf_fields=range(abi.CELL_FIELDS), acc=True):
+    assert g["n_blocks"] == o["n_blocks"], (tag, g["n_blocks"], o["n_blocks"])
+    np.testing.assert_array_equal(g["block_list"], o["block_list"], err_msg=tag)
+    np.testing.assert_array_equal(g["block_start"], o["block_start"], err_msg=tag)
+    if acc:
+        np.testing.assert_array_equal(g["acc"], o["acc"], err_msg=f"{tag} acc")
+    for f in cf_fields:
+        assert_same_float(f"{tag} cf[{f}]", g["cf"][f], o["cf"][f])
+
+
+def compare_sorted(tag, sim, orc, grid):
+    bp_g, so_g = sim.bins()
+    bp_o, so_o = orc.bins()
+    np.testing.assert_array_equal(bp_g, bp_o, err_msg=f"{tag} block_ptr")
+    bs = grid["block_start"]
+    for b in range(grid["n_blocks"]):
+        lo, hi = bs[b], bs[b + 1]
+        if not np.array_equal(np.sort(so_g[lo:hi]), np.sort(so_o[lo:hi])):
+            raise AssertionError(f"{tag}: particles of block {b} differ")
+
+
+def compare_state(tag, a, b, keys=("t_ns", "dt_ns", "step", "n_blocks", "n_blocks_prev", "overflow", "solve_iters",
+                                  "iters_max", "unconverged", "free_dt_ns")):
+    for k in keys:
+        assert getattr(a, k) == getattr(b, k), (tag, k, getattr(a, k), getattr(b, k))
+    for k in ("max_speed", "rz", "alpha", "beta", "b_norm", "r_norm", "dt_flow"):
+        assert_same_float(f"{tag} state.{k}", np.float32(getattr(a, k)), np.float32(getattr(b, k)))
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_phases_bit_exact(name):
+    s = SCENES[name]()
+    say(f"[parity] scene {name}: N={s.n} dims={tuple(s.params.dims[:3])}")
+    sim = Sim.from_scene(s)
+    orc = oracle.Oracle.from_scene(s, "f32")
+    for step in range(4):
+        for ph in range(abi.N_PHASES):
+            say(f"[parity] {name} step {step} phase {abi.PHASE_NAMES[ph]}")
+            sim.phase(ph)
+            orc.phase(ph)
+            tag = f"{name} step {step} after {abi.PHASE_NAMES[ph]}"
+            compare_state(tag, sim.state(), orc.state())
+            if ph >= abi.PHASE_BIN:
+                g, o = sim.grid(), orc.grid()
+                if ph == abi.PHASE_BIN:
+                    compare_grid(tag, g, o, cf_fields=(), acc=False)
+                    compare_sorted(tag, sim, orc, g)
+                else:
+                    compare_grid(tag, g, o)
+                    if ph >= abi.PHASE_P2G:
+                        np.testing.assert_array_equal(g["block_nb"], o["block_nb"], err_msg=tag)
+            if ph == abi.PHASE_G2P:
+                assert_same_float(f"{tag} particles", sim.particles(), orc.particles())
+    sim.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("name", ["sphere_jacobi_32k", "impact_sgs_32k"])
+def test_run_matches_oracle(name):
+    """Ten whole steps through natray_sim_run against the oracle's run."""
+    s = SCENES[name]()
+    say(f"[parity] run 10 steps of {name}")
+    sim = Sim.from_scene(s)
+    orc = oracle.Oracle.from_scene(s, "f32")
+    a = sim.run(10, iters=int(s.params.limits[2]), stop_ns=0xFFFFFFFF)
+    b = orc.run(10, iters=int(s.params.limits[2]), stop_ns=0xFFFFFFFF)
+    compare_state(name, a, b)
+    assert_same_float(f"{name} particles", sim.particles(), orc.particles())
+    sim.close()
+    orc.close()
+
+
+def test_repeat_runs_identical():
+    """Two handles stepping the same scene 30 steps end bit-identical, and equal the oracle:
+    the GPU path has no order-dependent float arithmetic and no races."""
+    s = SCENES["impact_sgs_32k"]()
+    say("[parity] determinism: 2 x 30 steps of impact_sgs_32k")
+    a = Sim.from_scene(s)
+    b = Sim.from_scene(s)
+    a.run(30, iters=400)
+    for _ in range(3):
+        b.run(10, iters=400)
+    assert_same_float("repeat particles", a.particles(), b.particles())
+    orc = oracle.Oracle.from_scene(s, "f32")
+    orc.run(30, iters=400)
+    assert_same_float("oracle particles", a.particles(), orc.particles())
+    compare_state("repeat", a.state(), orc.state())
