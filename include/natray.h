@@ -231,6 +231,15 @@ int natray_sim_run(natray_sim* sim, uint32_t n_steps, uint32_t iters, uint32_t s
  * the reference is 0 here (no exterior flight). */
 int natray_sim_advance(natray_sim* sim, double t_target, uint32_t budget, uint32_t* taken);
 
+/* The host rules of Sim::advance/run as pure functions (no GPU). natray_plan_batch: how many
+ * steps the next batch encodes (src/sim.rs:513-515): as many as the last free step length
+ * would take to reach stop_ns, at least 1, at most min(budget_left, 256, 4 ms / step_ms).
+ * natray_next_iters: the CG budget after a batch (src/sim.rs:636-645): doubled (to at most
+ * 4096) when a step ran out of iterations, else 1.25 x the most any step took + 2, within
+ * [8, 4096], else unchanged. */
+uint32_t natray_plan_batch(uint32_t stop_ns, uint32_t t_ns, uint32_t free_dt_ns, double step_ms, uint32_t budget_left);
+uint32_t natray_next_iters(uint32_t iters, uint32_t iters_max, uint32_t unconverged);
+
 /* One step with the current budget and no stop time. */
 int natray_sim_step(natray_sim* sim, natray_state* out);
 
@@ -264,8 +273,10 @@ void* natray_sim_device_ptr(natray_sim* sim, int which);
 int natray_sim_set_profiling(natray_sim* sim, int on);
 int natray_sim_phase_times(natray_sim* sim, double* ms, uint32_t* counts);
 
-/* CG statistics since creation: steps solved, total iterations, total water-cell-iterations. */
-int natray_sim_cg_stats(natray_sim* sim, uint64_t out[3]);
+/* Work counters since creation, summed over the steps that ran: [0] steps, [1] CG
+ * iterations, [2] water-cell CG iterations, [3] blocks in use, [4] water cells, [5] particles
+ * binned, [6] blocks cleared, [7] 0. They give the exact algorithmic bytes of a run. */
+int natray_sim_counters(natray_sim* sim, uint64_t out[8]);
 
 const char* natray_last_error(void);
 

@@ -49,7 +49,7 @@ struct Dev {
   uint32_t* bar;           // grid barrier: [0] arrivals, [1] generation, [2] abort flag
   uint32_t* wcount;        // water cells of the current step
   double* solver;          // double copies of rz, alpha, beta, b_norm, r_norm
-  unsigned long long* stats;  // CG: steps solved, iterations, water-cell-iterations
+  unsigned long long* stats;  // counters, see natray_sim_counters
   uint32_t N;
   uint32_t S;      // domain slots (product of bdims)
   uint32_t Spad;   // S rounded up to SCAN_TILE

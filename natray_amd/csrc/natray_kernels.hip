@@ -956,6 +956,10 @@ __global__ __launch_bounds__(256) void k_solve(Dev d) {
     d.stats[0] += 1ull;
     d.stats[1] += iters;
     d.stats[2] += (unsigned long long)w * iters;
+    d.stats[3] += n;
+    d.stats[4] += w;
+    d.stats[5] += d.block_start[n];
+    d.stats[6] += st->n_blocks_prev;
   }
 }
 

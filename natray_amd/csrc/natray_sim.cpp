@@ -271,7 +271,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &D.bar, 4));
   TRY(dalloc(s, &D.wcount, 1));
   TRY(dalloc(s, &D.solver, 8));
-  TRY(dalloc(s, &D.stats, 4));
+  TRY(dalloc(s, &D.stats, 8));
 
   HIPTRY(hipMemsetAsync(D.block_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.slot_count, 0, (size_t)D.Spad * 4, s->stream));
@@ -287,7 +287,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   HIPTRY(hipMemsetAsync(D.bar, 0, 4 * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.wcount, 0, 4, s->stream));
   HIPTRY(hipMemsetAsync(D.solver, 0, 8 * 8, s->stream));
-  HIPTRY(hipMemsetAsync(D.stats, 0, 4 * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.stats, 0, 8 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.sorted, 0, ((size_t)n + 1) * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.rank, 0, ((size_t)n + 1) * 4, s->stream));
   if (n) HIPTRY(hipMemcpyAsync(D.P, particles, (size_t)n * 80, hipMemcpyHostToDevice, s->stream));
@@ -365,11 +365,24 @@ int natray_sim_run(natray_sim* s, uint32_t n_steps, uint32_t iters, uint32_t sto
   if (st.overflow) return fail(NATRAY_E_OVERFLOW, "the water needs more grid blocks than the capacity");
   // src/sim.rs:636-645
   s->unconverged += st.unconverged;
-  if (st.unconverged > 0)
-    s->iters = std::min(s->iters * 2u, NATRAY_MAX_ITERS);
-  else if (st.iters_max > 0)
-    s->iters = std::min(std::max(st.iters_max + st.iters_max / 4u + 2u, 8u), NATRAY_MAX_ITERS);
+  s->iters = natray_next_iters(s->iters, st.iters_max, st.unconverged);
   return NATRAY_OK;
+}
+
+uint32_t natray_next_iters(uint32_t iters, uint32_t iters_max, uint32_t unconverged) {
+  if (unconverged > 0) return std::min(iters * 2u, NATRAY_MAX_ITERS);
+  if (iters_max > 0) return std::min(std::max(iters_max + iters_max / 4u + 2u, 8u), NATRAY_MAX_ITERS);
+  return iters;
+}
+
+uint32_t natray_plan_batch(uint32_t stop_ns, uint32_t t_ns, uint32_t free_dt_ns, double step_ms, uint32_t budget_left) {
+  if (t_ns >= stop_ns || budget_left == 0) return 0;
+  const uint64_t dt = free_dt_ns > 0 ? free_dt_ns : (uint64_t)(NATRAY_DT_MAX * 1e9);
+  const uint64_t fit = step_ms > 0.0 ? (uint64_t)(BATCH_MS / step_ms) : 1;
+  uint64_t hi = std::min<uint64_t>(budget_left, NATRAY_MAX_BATCH);
+  hi = std::min<uint64_t>(hi, std::max<uint64_t>(fit, 1));
+  const uint64_t want = ((uint64_t)stop_ns - t_ns + dt - 1) / dt;
+  return (uint32_t)std::max<uint64_t>(1, std::min(want, hi));
 }
 
 int natray_sim_advance(natray_sim* s, double t_target, uint32_t budget, uint32_t* taken_out) {
@@ -383,12 +396,7 @@ int natray_sim_advance(natray_sim* s, double t_target, uint32_t budget, uint32_t
   const uint32_t stop = (uint32_t)std::llround(span * 1e9);
   uint32_t taken = 0;
   while (taken < budget && s->last.t_ns < stop) {
-    const uint32_t dt = s->last.free_dt_ns > 0 ? s->last.free_dt_ns : (uint32_t)(NATRAY_DT_MAX * 1e9);
-    const size_t fit = s->step_ms > 0.0 ? (size_t)(BATCH_MS / s->step_ms) : 1;
-    size_t hi = std::min<size_t>(budget - taken, NATRAY_MAX_BATCH);
-    hi = std::min(hi, std::max<size_t>(fit, 1));
-    size_t want = (stop - s->last.t_ns + dt - 1) / dt;
-    const uint32_t nb = (uint32_t)std::max<size_t>(1, std::min(want, hi));
+    const uint32_t nb = natray_plan_batch(stop, s->last.t_ns, s->last.free_dt_ns, s->step_ms, budget - taken);
     const uint32_t before = s->last.step;
     int e = natray_sim_run(s, nb, 0, stop, nullptr);
     if (e) return e;
@@ -540,14 +548,12 @@ int natray_sim_phase_times(natray_sim* s, double* ms, uint32_t* counts) {
   return NATRAY_OK;
 }
 
-int natray_sim_cg_stats(natray_sim* s, uint64_t out[3]) {
+int natray_sim_counters(natray_sim* s, uint64_t out[8]) {
   if (!s || !out) return fail(NATRAY_E_ARG, "null argument");
-  unsigned long long h[4] = {};
+  unsigned long long h[8] = {};
   HIPCHK(hipMemcpyAsync(h, s->D.stats, sizeof h, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  out[0] = h[0];
-  out[1] = h[1];
-  out[2] = h[2];
+  for (int k = 0; k < 8; ++k) out[k] = h[k];
   return NATRAY_OK;
 }
 

@@ -145,10 +145,12 @@ class Sim:
         abi.check(abi.lib().natray_sim_phase_times(self._h, abi.ptr(ms), abi.ptr(cnt)))
         return {abi.PHASE_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(abi.N_PHASES)}
 
-    def cg_stats(self) -> tuple[int, int, int]:
-        out = np.zeros(3, np.uint64)
-        abi.check(abi.lib().natray_sim_cg_stats(self._h, abi.ptr(out)))
-        return int(out[0]), int(out[1]), int(out[2])
+    def counters(self) -> dict:
+        """Work counters since creation (natray_sim_counters)."""
+        out = np.zeros(8, np.uint64)
+        abi.check(abi.lib().natray_sim_counters(self._h, abi.ptr(out)))
+        keys = ["steps", "cg_iters", "cg_cell_iters", "blocks", "water_cells", "particles", "blocks_cleared"]
+        return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def stream_ptr(self) -> int:
         return int(abi.lib().natray_sim_device_ptr(self._h, abi.BUF_STREAM) or 0)

@@ -52,6 +52,7 @@ def lib() -> C.CDLL:
                 "write_grid": (None, [P, P, P]),
                 "read_bins": (None, [P, P, P]),
                 "solver_scalars": (None, [P, P]),
+                "precondition": (None, [P, P, P, P, C.c_uint32]),
                 "set_threads": (None, [C.c_int]),
             }
             for name, (res, args) in sig.items():
@@ -65,6 +66,17 @@ def lib() -> C.CDLL:
 def set_threads(n: int) -> None:
     lib().o32_set_threads(n)
     lib().o64_set_threads(n)
+
+
+def precondition(r: np.ndarray, diag: np.ndarray, water: np.ndarray, mode: int, precision: str = "f32") -> np.ndarray:
+    """The oracle's preconditioner applied to one block's residual r[64]."""
+    r = np.ascontiguousarray(r, np.float32)
+    diag = np.ascontiguousarray(diag, np.float32)
+    water = np.ascontiguousarray(water, np.int32)
+    z = np.empty(64, np.float32)
+    fn = lib().o32_precondition if precision == "f32" else lib().o64_precondition
+    fn(abi.ptr(r), abi.ptr(diag), abi.ptr(water), abi.ptr(z), mode)
+    return z
 
 
 class Oracle:

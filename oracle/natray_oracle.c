@@ -1253,6 +1253,22 @@ void F(solver_scalars)(const osim* s, double out[8]) {
   out[6] = s->max_speed;
   out[7] = (double)s->cg_iters;
 }
+/* The preconditioner of one block on its own, for checking it as an operator. */
+void F(precondition)(const float* r, const float* diag, const int32_t* water, float* z, uint32_t mode) {
+  osim tmp;
+  memset(&tmp, 0, sizeof tmp);
+  tmp.sc.precond = mode;
+  REAL rr[64], dd[64], zz[64];
+  int ww[64];
+  for (int i = 0; i < 64; ++i) {
+    rr[i] = (REAL)r[i];
+    dd[i] = (REAL)diag[i];
+    ww[i] = water[i] != 0;
+  }
+  precondition(&tmp, rr, dd, ww, zz);
+  for (int i = 0; i < 64; ++i) z[i] = (float)zz[i];
+}
+
 void F(set_threads)(int n) {
 #ifdef _OPENMP
   omp_set_num_threads(n > 0 ? n : 1);

@@ -38,6 +38,7 @@ typedef struct osim osim;
   void P##write_grid(osim* s, const int32_t* acc, const float* cf);                                 \
   void P##read_bins(const osim* s, uint32_t* block_ptr, uint32_t* sorted);                          \
   void P##solver_scalars(const osim* s, double out[8]);                                             \
+  void P##precondition(const float* r, const float* diag, const int32_t* water, float* z, uint32_t mode); \
   void P##set_threads(int n);
 
 ORACLE_DECL(o32_)

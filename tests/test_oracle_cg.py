@@ -1,0 +1,250 @@
+"""The oracle's pressure solve (shaders/fluid.wgsl:381-578) against an independent float64
+assembly of the same system on a dense grid and a direct scipy solve, plus the projection's
+closed form and the preconditioner as an operator.
+
+  A p = b with A = (kc + #open neighbours) I - sum over water neighbours, kc = h^2/(c dt)^2,
+  b = kc p_old + sum over open non-water neighbours of their pressure - (rho h / dt) div u*.
+
+After projection u = u* - dt/(rho h) grad p on every open face next to water, and A p = b
+gives, per water cell, (rho h / dt) div u = kc (p_old - p): the compressible relation the
+solve encodes (shaders/fluid.wgsl:3-5).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from natray_amd import abi, scenes
+from tests.helpers import cell_coords
+
+RHO, C_SOUND, P_FLOOR = 1000.0, 1480.0, 2339.0 - 101325.0
+
+
+def warm_scene(side=32, n=8 * 600, steps=3, precond=abi.PRECOND_BLOCK_SGS, precision=abi.CG_FP32, refresh=0, seed=7):
+    """A falling water ball with a side kick, after a few steps, so the solve has work."""
+    s = scenes.water_sphere(side=side, n=n, seed=seed, precond=precond, fixed_dt=2e-4)
+    s.scene.precision = precision
+    s.scene.refresh_every = refresh
+    s.particles[:, 4] = 0.7 * np.sin(7.0 * s.particles[:, 1])  # a shear flow
+    s.particles[:, 11] = 2000.0 * np.cos(5.0 * s.particles[:, 0])  # carried pressure
+    o = oracle.Oracle.from_scene(s, "f32")
+    o.run(steps, iters=500)
+    return s, o
+
+
+def dense(s, g, fields):
+    side = int(s.params.dims[0])
+    cc = cell_coords(g, s.params.bdims)
+    out = {}
+    for name, row in fields.items():
+        a = np.zeros((side, side, side), np.float64)
+        a[cc[:, 0], cc[:, 1], cc[:, 2]] = row
+        out[name] = a
+    return out
+
+
+def assemble(s, g, grid_fields):
+    """A (sparse, water cells), b, p0 in float64 from the dense grid after grid_prep."""
+    h = float(s.params.origin[3])
+    st = s._state_after_begin
+    dt = np.float32(st.dt_ns) * np.float32(1e-9)
+    D = dense(s, g, grid_fields)
+    ty, P, U = D["type"], D["p"], [D["u0"], D["u1"], D["u2"]]
+    water = ty == abi.WATER
+    idx = -np.ones(ty.shape, np.int64)
+    wc = np.argwhere(water)
+    idx[tuple(wc.T)] = np.arange(len(wc))
+    kc = h * h / (C_SOUND * float(dt)) ** 2
+    rows, cols, vals = [], [], []
+    b = np.zeros(len(wc))
+    side = ty.shape[0]
+
+    def at(a, c, default=0.0):
+        return a[tuple(c)] if np.all(c >= 0) and np.all(c < side) else default
+
+    def bounded(u, hi, lo):
+        if lo == abi.SOLID and hi == abi.SOLID:
+            return 0.0
+        if lo == abi.SOLID:
+            return max(u, 0.0)
+        if hi == abi.SOLID:
+            return min(u, 0.0)
+        return u
+
+    for k, c in enumerate(wc):
+        diag = kc
+        bb = kc * P[tuple(c)]
+        div = 0.0
+        for d in range(3):
+            e = np.zeros(3, int)
+            e[d] = 1
+            up, dn = c + e, c - e
+            u_up = bounded(at(U[d], up), at(ty, up, abi.GAS), abi.WATER)
+            u_me = bounded(U[d][tuple(c)], abi.WATER, at(ty, dn, abi.GAS))
+            div += u_up - u_me
+            for nb in (dn, up):
+                t = at(ty, nb, abi.GAS)
+                if t == abi.SOLID:
+                    continue
+                diag += 1.0
+                if t == abi.WATER:
+                    rows.append(k)
+                    cols.append(idx[tuple(nb)])
+                    vals.append(-1.0)
+                else:
+                    bb += at(P, nb)
+        rows.append(k)
+        cols.append(k)
+        vals.append(diag)
+        b[k] = bb - RHO * h / float(dt) * div
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(len(wc), len(wc)))
+    return A, b, P[tuple(wc.T)], wc, kc, float(dt)
+
+
+def solve_setup(precond=abi.PRECOND_BLOCK_SGS, precision=abi.CG_FP32, refresh=0):
+    s, o = warm_scene(precond=precond, precision=precision, refresh=refresh)
+    for ph in (abi.PHASE_BEGIN, abi.PHASE_BIN, abi.PHASE_P2G, abi.PHASE_GRID):
+        o.phase(ph)
+    s._state_after_begin = o.state()
+    g = o.grid()
+    cf = g["cf"]
+    fields = {"type": cf[abi.F_TYPE], "p": cf[abi.F_P], "u0": cf[0], "u1": cf[1], "u2": cf[2]}
+    A, b, p0, wc, kc, dt = assemble(s, g, fields)
+    return s, o, g, A, b, p0, wc, kc, dt
+
+
+def test_residual_of_the_initial_guess_matches_independent_assembly():
+    """solve_init with no iterations leaves r = b - A p_old (the particle-carried warm start,
+    shaders/fluid.wgsl:445-491): compare with the float64 assembly."""
+    s, o = warm_scene()
+    o.run(0, iters=0)  # CG budget 0: solve_init and solve_start only
+    for ph in (abi.PHASE_BEGIN, abi.PHASE_BIN, abi.PHASE_P2G, abi.PHASE_GRID):
+        o.phase(ph)
+    s._state_after_begin = o.state()
+    g = o.grid()
+    cf = g["cf"]
+    A, b, p0, wc, kc, dt = assemble(s, g, {"type": cf[abi.F_TYPE], "p": cf[abi.F_P], "u0": cf[0], "u1": cf[1],
+                                            "u2": cf[2]})
+    o.phase(abi.PHASE_SOLVE)
+    g2 = o.grid()
+    assert o.state().solve_iters == 0
+    r = water_rows(s, g2, wc, g2["cf"][abi.F_R]).astype(np.float64)
+    r_star = b - A @ p0
+    np.testing.assert_allclose(r, r_star, atol=2e-6 * np.abs(b).max() + 1e-3)
+    np.testing.assert_allclose(water_rows(s, g2, wc, g2["cf"][abi.F_DIAG]), A.diagonal(), rtol=1e-6)
+    np.testing.assert_allclose(o.state().b_norm, np.linalg.norm(b), rtol=1e-4)
+
+
+@pytest.fixture(scope="module")
+def solved():
+    s, o, g, A, b, p0, wc, kc, dt = solve_setup()
+    o.phase(abi.PHASE_SOLVE)
+    g2 = o.grid()
+    return s, o, g, g2, A, b, p0, wc, kc, dt
+
+
+def water_rows(s, g, wc, row):
+    cc = cell_coords(g, s.params.bdims)
+    index = {tuple(c): i for i, c in enumerate(cc.tolist())}
+    return np.array([row[index[tuple(c)]] for c in wc.tolist()])
+
+
+def test_diagonal_matches_independent_assembly(solved):
+    s, o, g, g2, A, b, p0, wc, kc, dt = solved
+    diag = water_rows(s, g2, wc, g2["cf"][abi.F_DIAG])
+    np.testing.assert_allclose(diag, A.diagonal(), rtol=1e-6)
+
+
+def test_solution_matches_direct_solve(solved):
+    s, o, g, g2, A, b, p0, wc, kc, dt = solved
+    st = o.state()
+    assert st.solve_iters > 5 and st.unconverged == 0
+    p = water_rows(s, g2, wc, g2["cf"][abi.F_P]).astype(np.float64)
+    p_star = spla.spsolve(A.tocsc(), b)
+    rel_res = np.linalg.norm(b - A @ p) / max(np.linalg.norm(b), 1.0)
+    assert rel_res < 5e-5, rel_res  # f32 recurrences drift a little past the 1e-6 target
+    np.testing.assert_allclose(p, p_star, rtol=0, atol=2e-4 * np.abs(p_star).max() + 1e-3)
+    # The convergence test of the reference: sqrt(rr) <= tol max(|b|, 1) (fluid.wgsl:441-443).
+    assert st.r_norm <= 1e-6 * max(st.b_norm, 1.0) * 1.0001
+    np.testing.assert_allclose(st.b_norm, np.linalg.norm(b), rtol=1e-4)
+
+
+def test_projection_closed_form(solved):
+    """After project: (rho h / dt) div u = kc (p_old - p) at every water cell whose faces
+    are all projected (no vapour clamp), to the accuracy of the solve."""
+    s, o, g, g2, A, b, p0, wc, kc, dt = solved
+    p = water_rows(s, g2, wc, g2["cf"][abi.F_P]).astype(np.float64)
+    o.phase(abi.PHASE_PROJECT)
+    g3 = o.grid()
+    h = float(s.params.origin[3])
+    D = dense(s, g3, {"u0": g3["cf"][0], "u1": g3["cf"][1], "u2": g3["cf"][2], "type": g3["cf"][abi.F_TYPE],
+                      "p": g3["cf"][abi.F_P]})
+    assert np.all(D["p"][tuple(wc.T)] > P_FLOOR)
+    U = [D["u0"], D["u1"], D["u2"]]
+    div = np.zeros(len(wc))
+    for d in range(3):
+        e = np.zeros(3, int)
+        e[d] = 1
+        div += U[d][tuple((wc + e).T)] - U[d][tuple(wc.T)]
+    lhs = RHO * h / dt * div
+    rhs = kc * (p0 - p)
+    scale = np.abs(b).max()
+    np.testing.assert_allclose(lhs, rhs, atol=5e-4 * scale)
+
+
+@pytest.mark.parametrize("precision", [abi.CG_FP32, abi.CG_MIXED])
+@pytest.mark.parametrize("precond", [abi.PRECOND_BLOCK_SGS, abi.PRECOND_JACOBI])
+def test_all_solver_variants_solve_the_same_system(precision, precond):
+    s, o, g, A, b, p0, wc, kc, dt = solve_setup(precond=precond, precision=precision, refresh=10)
+    o.phase(abi.PHASE_SOLVE)
+    g2 = o.grid()
+    p = water_rows(s, g2, wc, g2["cf"][abi.F_P]).astype(np.float64)
+    rel_res = np.linalg.norm(b - A @ p) / max(np.linalg.norm(b), 1.0)
+    assert o.state().unconverged == 0
+    assert rel_res < (2e-6 if precision == abi.CG_MIXED else 5e-5), rel_res
+
+
+def block_water(seed):
+    rng = np.random.default_rng(seed)
+    water = (rng.uniform(size=64) < 0.8).astype(np.int32)
+    diag = np.where(water, rng.uniform(3.0, 6.5, 64), 1.0).astype(np.float32)
+    return water, diag
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_block_sgs_is_symmetric_positive_definite(seed):
+    """The reference claims its preconditioner is always the same SPD operator
+    (shaders/fluid.wgsl:384-388); probe it column by column."""
+    water, diag = block_water(seed)
+    W = np.flatnonzero(water)
+    M = np.zeros((len(W), len(W)))
+    for j, cj in enumerate(W):
+        r = np.zeros(64, np.float32)
+        r[cj] = 1.0
+        z = oracle.precondition(r, diag, water, abi.PRECOND_BLOCK_SGS, "f64")
+        M[:, j] = z[W]
+        assert np.all(z[water == 0] == 0)
+    np.testing.assert_allclose(M, M.T, atol=1e-6)
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+    # It is a better preconditioner than Jacobi for the block's own Laplacian block.
+    A = np.diag(diag[W].astype(np.float64))
+    pos = {c: i for i, c in enumerate(W)}
+    for c in W:
+        x, y, z = c // 16, (c // 4) % 4, c % 4
+        for dx, dy, dz in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            q = (x + dx, y + dy, z + dz)
+            if all(0 <= v < 4 for v in q):
+                nb = q[0] * 16 + q[1] * 4 + q[2]
+                if nb in pos:
+                    A[pos[c], pos[nb]] = -1.0
+    ev_sgs = np.linalg.eigvals(M @ A).real
+    ev_jac = np.linalg.eigvals(np.diag(1.0 / diag[W]) @ A).real
+    assert ev_sgs.max() / ev_sgs.min() < ev_jac.max() / ev_jac.min()
+
+
+def test_jacobi_is_the_inverse_diagonal():
+    water, diag = block_water(4)
+    r = np.random.default_rng(1).normal(size=64).astype(np.float32)
+    z = oracle.precondition(r, diag, water, abi.PRECOND_JACOBI)
+    np.testing.assert_array_equal(z, np.where(water, r / diag, 0).astype(np.float32))
