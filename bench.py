@@ -103,6 +103,8 @@ def main() -> int:
     ap.add_argument("--iters", type=int, default=1000, help="CG iteration cap per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--write-scene", metavar="PATH",
+                    help="write the workload's scene file for natray_amd/natray_drive and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -110,9 +112,13 @@ def main() -> int:
               "which is not built.", file=sys.stderr)
         return 2
 
-    from natray_amd import Sim
+    from natray_amd import Sim, scenes
 
     scene, cfg = make_scene(args)
+    if args.write_scene:
+        scenes.write_scene_file(scene, args.write_scene)
+        print(f"wrote {args.write_scene}: {cfg['workload']}")
+        return 0
     sim = Sim.from_scene(scene)
     if args.warmup:
         sim.run(args.warmup, iters=args.iters)

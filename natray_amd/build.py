@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnatray.so")
+DRIVE = os.path.join(HERE, "natray_drive")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NATRAY_ARCH", "gfx950")
@@ -59,6 +60,13 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for obj in objs:
         os.remove(obj)
+    # The native driver a profiler wraps (natray_drive.cpp), linked against the library.
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", os.path.join(CSRC, "natray_drive.cpp"),
+           "-o", DRIVE + ".tmp", f"-L{HERE}", "-lnatray", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVE + ".tmp", DRIVE)
     return LIB
 
 

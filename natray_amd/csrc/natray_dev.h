@@ -47,7 +47,8 @@ struct Dev {
   uint32_t* tile_sum;      // 2 x ntiles: scan tile totals (slots claimed, particles)
   uint32_t* tile_off;      // 2 x ntiles: their exclusive prefix
   uint32_t* bar;           // grid barrier: [0] arrivals, [1] generation, [2] abort flag
-  uint32_t* wcount;        // water cells of the current step
+  uint32_t* wpart;         // water cells of each solve workgroup's blocks (<= 1024 workgroups)
+  float* vmax_part;        // fastest particle of each g2p workgroup (<= cap workgroups)
   double* solver;          // double copies of rz, alpha, beta, b_norm, r_norm
   unsigned long long* stats;  // counters, see natray_sim_counters
   uint32_t N;

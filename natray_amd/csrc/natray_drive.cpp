@@ -1,0 +1,80 @@
+// natray_drive: steps a scene file through the C ABI of libnatray.so, from native code.
+//
+//   natray_drive <scene file> [steps=20] [warmup=2] [cg iteration cap=1000]
+//
+// The scene file comes from `python bench.py --write-scene <path>` (natray_amd.scenes
+// write_scene_file): "NATRAYSC", u32 version 1, u32 particle count, then natray_params,
+// natray_scene, natray_state and the 80-byte particle records. The driver is what a profiler
+// wraps (`rocprofv3 ... -- natray_amd/natray_drive <file>`): a native program, so the profiler
+// sees the kernels without an interpreter in between. It prints one line with the timing.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "natray.h"
+
+static bool read_all(FILE* f, void* dst, size_t bytes) { return fread(dst, 1, bytes, f) == bytes; }
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: %s <scene file> [steps] [warmup] [iters]\n", argv[0]);
+    return 2;
+  }
+  const unsigned steps = argc > 2 ? (unsigned)atoi(argv[2]) : 20u;
+  const unsigned warmup = argc > 3 ? (unsigned)atoi(argv[3]) : 2u;
+  const unsigned iters = argc > 4 ? (unsigned)atoi(argv[4]) : 1000u;
+  FILE* f = fopen(argv[1], "rb");
+  if (!f) {
+    perror(argv[1]);
+    return 2;
+  }
+  char magic[8];
+  uint32_t version = 0, n = 0;
+  natray_params params;
+  natray_scene scene;
+  natray_state state;
+  if (!read_all(f, magic, 8) || memcmp(magic, "NATRAYSC", 8) != 0 || !read_all(f, &version, 4) || version != 1 ||
+      !read_all(f, &n, 4) || !read_all(f, &params, sizeof params) || !read_all(f, &scene, sizeof scene) ||
+      !read_all(f, &state, sizeof state)) {
+    fprintf(stderr, "%s: not a version-1 scene file\n", argv[1]);
+    fclose(f);
+    return 2;
+  }
+  std::vector<float> particles((size_t)n * 20);
+  if (!read_all(f, particles.data(), particles.size() * sizeof(float))) {
+    fprintf(stderr, "%s: truncated particle records\n", argv[1]);
+    fclose(f);
+    return 2;
+  }
+  fclose(f);
+  natray_sim* sim = nullptr;
+  int rc = natray_sim_create(&params, particles.data(), n, &scene, &state, &sim);
+  if (rc) {
+    fprintf(stderr, "natray_sim_create: %d %s\n", rc, natray_last_error());
+    return 1;
+  }
+  natray_state st;
+  if (warmup && (rc = natray_sim_run(sim, warmup, iters, 0xffffffffu, &st)) != 0) {
+    fprintf(stderr, "warmup: %d %s\n", rc, natray_last_error());
+    natray_sim_destroy(sim);
+    return 1;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  rc = natray_sim_run(sim, steps, iters, 0xffffffffu, &st);
+  auto t1 = std::chrono::steady_clock::now();
+  if (rc) {
+    fprintf(stderr, "run: %d %s\n", rc, natray_last_error());
+    natray_sim_destroy(sim);
+    return 1;
+  }
+  const double s = std::chrono::duration<double>(t1 - t0).count();
+  printf("natray_drive: %u particles, %u steps in %.3f ms (%.3f ms/step, %.4g particle-substeps/s), step %u, %u blocks, "
+         "%u CG iterations in the last step\n",
+         n, steps, s * 1e3, s * 1e3 / (steps ? steps : 1), steps ? (double)n * steps / s : 0.0, st.step, st.n_blocks,
+         st.solve_iters);
+  fflush(stdout);
+  natray_sim_destroy(sim);
+  return 0;
+}

@@ -227,7 +227,6 @@ __global__ void k_begin(Dev d) {
     st->mem_ticket = 0;
     st->n_blocks_prev = n_blocks(d);
     st->n_blocks = 0;
-    *d.wcount = 0;
   }
   st->dt_ns = dt_ns;
 }
@@ -502,18 +501,62 @@ __device__ __forceinline__ void block_links(const Dev& d, uint32_t b, uint32_t l
   d.block_nb[b * 27u + li] = idx;
 }
 
+// Splats particle i, whose own cell is local cell `l` of the block at (bx, by, bz), into the
+// tile: the four lattices of fluid.wgsl:286-289. Returns false (and adds nothing) for a
+// particle that is not in that cell, which only a position written behind the library's back
+// can cause.
+__device__ __forceinline__ bool splat_particle(const Dev& d, int* tile, uint32_t i, uint32_t l, int bx, int by, int bz,
+                                               const int corner[3], float h) {
+  const natray_params* U = d.U;
+  const float4* p = reinterpret_cast<const float4*>(d.P) + (size_t)i * 5;
+  const float4 pos = p[0], vel = p[1], c0 = p[2], c1 = p[3], c2 = p[4];
+  const float x[3] = {(pos.x - U->origin[0]) / h, (pos.y - U->origin[1]) / h, (pos.z - U->origin[2]) / h};
+  const int ox = (int)floorf(x[0]) - bx * 4, oy = (int)floorf(x[1]) - by * 4, oz = (int)floorf(x[2]) - bz * 4;
+  if ((uint32_t)ox * 16u + (uint32_t)oy * 4u + (uint32_t)oz != l || (unsigned)ox > 3u || (unsigned)oy > 3u ||
+      (unsigned)oz > 3u)
+    return false;
+  const float g0[3] = {c0.x, c0.y, c0.z}, g1[3] = {c1.x, c1.y, c1.z}, g2[3] = {c2.x, c2.y, c2.z};
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  splat<0, 1, 1>(tile, x, corner, NATRAY_A_MOM(0), vel.x, g0, NATRAY_A_WT(0), NATRAY_FP_M, h);
+  splat<1, 0, 1>(tile, x, corner, NATRAY_A_MOM(1), vel.y, g1, NATRAY_A_WT(1), NATRAY_FP_M, h);
+  splat<1, 1, 0>(tile, x, corner, NATRAY_A_MOM(2), vel.z, g2, NATRAY_A_WT(2), NATRAY_FP_M, h);
+  splat<1, 1, 1>(tile, x, corner, NATRAY_A_PC, c0.w * 1e-6f, zero, NATRAY_A_WC, NATRAY_FP_P, h);
+  return true;
+}
+
+// Local cell (16x + 4y + z) of a particle's key inside the block at (bx, by, bz), or 64 if
+// the key is not in that block.
+__device__ __forceinline__ uint32_t key_local(uint32_t key, int bx, int by, int bz) {
+  const uint32_t ox = (key & KEY_MASK) - (uint32_t)(bx * 4), oy = ((key >> KEY_BITS) & KEY_MASK) - (uint32_t)(by * 4),
+                 oz = (key >> (2 * KEY_BITS)) - (uint32_t)(bz * 4);
+  return (ox > 3u || oy > 3u || oz > 3u) ? 64u : ox * 16u + oy * 4u + oz;
+}
+
+// Particles of a block that P2G orders by cell in LDS; a block with more splats unordered.
+constexpr uint32_t P2G_MAXP = 2048;
+
+// p2g (fluid.wgsl:254-303). The block's particles are first counted and ordered by their own
+// cell in LDS; then, round r, lane l of a wave splats the r-th particle of cell l, so the 64
+// lanes of one LDS atomic instruction add into 64 different cells' stencils instead of eight
+// particles of one cell hitting the same words. The per-cell counts are A_N.
 __global__ __launch_bounds__(256) void k_p2g(Dev d) {
   if (d.st->dt_ns == 0u) return;
   __shared__ int tile[8 * TILE];
-  __shared__ uint32_t own[64];
+  __shared__ uint32_t cnt[64], start[64], fillc[64], drop[64];
+  __shared__ uint32_t order[P2G_MAXP];
   __shared__ uint32_t links[27];
-  const uint32_t n = n_blocks(d), tid = threadIdx.x;
+  __shared__ uint32_t s_maxc;
+  const uint32_t n = n_blocks(d), tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
   const natray_params* U = d.U;
   const float h = U->origin[3];
   for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
     __syncthreads();
     if (tid < 27) block_links(d, b, tid, links);
-    if (tid < 64) own[tid] = 0;
+    if (tid < 64) {
+      cnt[tid] = 0;
+      fillc[tid] = 0;
+      drop[tid] = 0;
+    }
     for (uint32_t t = tid; t < 8 * TILE; t += 256) tile[t] = 0;
     const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
     __syncthreads();
@@ -524,20 +567,39 @@ __global__ __launch_bounds__(256) void k_p2g(Dev d) {
     int bx, by, bz;
     slot_coord(U, d.block_list[b], bx, by, bz);
     const int corner[3] = {bx * 4 - 1, by * 4 - 1, bz * 4 - 1};
-    for (uint32_t k = tid; k < count; k += 256) {
-      const float4* p = reinterpret_cast<const float4*>(d.P) + (size_t)d.sorted[first + k] * 5;
-      float4 pos = p[0], vel = p[1], c0 = p[2], c1 = p[3], c2 = p[4];
-      float x[3] = {(pos.x - U->origin[0]) / h, (pos.y - U->origin[1]) / h, (pos.z - U->origin[2]) / h};
-      int ox = (int)floorf(x[0]) - bx * 4, oy = (int)floorf(x[1]) - by * 4, oz = (int)floorf(x[2]) - bz * 4;
-      // A particle sorted into this block lies in it (its key was made from the same position).
-      if ((unsigned)ox > 3u || (unsigned)oy > 3u || (unsigned)oz > 3u) continue;
-      const float g0[3] = {c0.x, c0.y, c0.z}, g1[3] = {c1.x, c1.y, c1.z}, g2[3] = {c2.x, c2.y, c2.z};
-      const float zero[3] = {0.0f, 0.0f, 0.0f};
-      splat<0, 1, 1>(tile, x, corner, NATRAY_A_MOM(0), vel.x, g0, NATRAY_A_WT(0), NATRAY_FP_M, h);
-      splat<1, 0, 1>(tile, x, corner, NATRAY_A_MOM(1), vel.y, g1, NATRAY_A_WT(1), NATRAY_FP_M, h);
-      splat<1, 1, 0>(tile, x, corner, NATRAY_A_MOM(2), vel.z, g2, NATRAY_A_WT(2), NATRAY_FP_M, h);
-      splat<1, 1, 1>(tile, x, corner, NATRAY_A_PC, c0.w * 1e-6f, zero, NATRAY_A_WC, NATRAY_FP_P, h);
-      atomicAdd(&own[(uint32_t)ox * 16u + (uint32_t)oy * 4u + (uint32_t)oz], 1u);
+    if (count <= P2G_MAXP) {
+      for (uint32_t k = tid; k < count; k += 256) {
+        const uint32_t l = key_local(d.key[d.sorted[first + k]], bx, by, bz);
+        if (l < 64u) atomicAdd(&cnt[l], 1u);
+      }
+      __syncthreads();
+      if (tid < 64) {
+        const uint32_t c = cnt[tid];
+        uint32_t incl = c, mx = c;
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t t = __shfl_up(incl, off);
+          if (lane >= (uint32_t)off) incl += t;
+        }
+        for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+        start[tid] = incl - c;
+        if (tid == 0) s_maxc = mx;
+      }
+      __syncthreads();
+      for (uint32_t k = tid; k < count; k += 256) {
+        const uint32_t i = d.sorted[first + k];
+        const uint32_t l = key_local(d.key[i], bx, by, bz);
+        if (l < 64u) order[start[l] + atomicAdd(&fillc[l], 1u)] = i;
+      }
+      __syncthreads();
+      const uint32_t maxc = s_maxc, mine = cnt[lane], base = start[lane];
+      for (uint32_t r = wv; r < maxc; r += 4u)
+        if (r < mine && !splat_particle(d, tile, order[base + r], lane, bx, by, bz, corner, h)) atomicAdd(&drop[lane], 1u);
+    } else {
+      for (uint32_t k = tid; k < count; k += 256) {
+        const uint32_t i = d.sorted[first + k];
+        const uint32_t l = key_local(d.key[i], bx, by, bz);
+        if (l < 64u && splat_particle(d, tile, i, l, bx, by, bz, corner, h)) atomicAdd(&cnt[l], 1u);
+      }
     }
     __syncthreads();
     for (uint32_t t = tid; t < TILE; t += 256) {
@@ -549,17 +611,18 @@ __global__ __launch_bounds__(256) void k_p2g(Dev d) {
         if (v != 0) atomicAdd(&accr(d, f, cell), v);
       }
     }
-    if (tid < 64) accr(d, NATRAY_A_N, b * 64u + tid) = (int32_t)own[tid];
+    if (tid < 64) accr(d, NATRAY_A_N, b * 64u + tid) = (int32_t)(cnt[tid] - drop[tid]);
   }
 }
 
 // ---- per-cell kernels: one wave per block --------------------------------------------------------
 
-// Runs body(b, lane, lx, ly, lz, links) for every block in use, 4 blocks per 256-thread
-// workgroup; `links` are the block's 27 neighbours in LDS.
-template <class Body>
+// Runs body(b, lane, lx, ly, lz, links) for the blocks b_lo, b_lo + stride, ... below b_hi,
+// WAVES blocks at a time per workgroup (one per wave); `links` are the block's 27 neighbours
+// in LDS.
+template <uint32_t WAVES = 4, class Body>
 __device__ __forceinline__ void for_blocks(const Dev& d, uint32_t b_lo, uint32_t b_hi, uint32_t stride, Body body) {
-  __shared__ uint32_t lk[4][27];
+  __shared__ uint32_t lk[WAVES][27];
   const uint32_t w = threadIdx.x >> 6, lane = lane_id();
   const int lx = (int)(lane >> 4), ly = (int)((lane >> 2) & 3u), lz = (int)(lane & 3u);
   for (uint32_t base = b_lo; base < b_hi; base += stride) {
@@ -619,41 +682,61 @@ __global__ __launch_bounds__(256) void k_grid_prep(Dev d) {
 
 // ---- pressure solve (fluid.wgsl:381-578) ------------------------------------------------------------
 
-// Grid barrier of the cooperative solve. Every workgroup releases its writes at agent scope,
-// one thread per workgroup counts in and waits for the generation to move, and everyone
-// acquires afterwards. A wait that runs past `kSpinLimit` polls (or sees another workgroup's
-// abort) sets the abort flag and returns, so no wave can wait forever.
+// Grid barrier of the cooperative solve, a two-level counter: workgroups count in on one of
+// BAR_GROUPS group counters (each on its own cache line, so arrivals do not serialise on one
+// address), the last of each group counts in on the top counter, and the last of those moves
+// the generation everyone waits on. Only thread 0 of a workgroup talks to the counters:
+// the workgroup barrier orders its threads' writes before thread 0's agent-scope release
+// fence, and the acquire fence after the wait before their reads. A wait that runs past
+// kSpinLimit polls, or sees another workgroup's abort, sets the abort flag and returns, so no
+// wave can wait forever.
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint32_t BAR_GROUPS = 16;
+// Words of Dev::bar: [0] top counter, [1] generation, [2] abort, group g's counter at
+// BAR_GROUP0 + 32 g.
+constexpr uint32_t BAR_GROUP0 = 64;
 
 __device__ __forceinline__ bool grid_sync(const Dev& d, uint32_t G) {
   __shared__ uint32_t s_abort;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t* count = d.bar;
+    uint32_t* top = d.bar;
     uint32_t* gen = d.bar + 1;
     uint32_t* abort = d.bar + 2;
-    uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t ab = 0;
-    if (arrived == G - 1u) {
-      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t ngroups = G < BAR_GROUPS ? G : BAR_GROUPS;
+    const uint32_t grp = blockIdx.x % ngroups;
+    const uint32_t members = G / ngroups + (grp < G % ngroups ? 1u : 0u);
+    uint32_t* gc = d.bar + BAR_GROUP0 + 32u * grp;
+    const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    bool last = false;
+    if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1u) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      last = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1u;
+    }
+    if (last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(gen, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       uint32_t spins = 0;
-      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        __builtin_amdgcn_s_sleep(2);
+      while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(1);
         if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kSpinLimit) {
           __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
       }
     }
-    ab = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_abort = ab;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_abort = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   return s_abort == 0u;
 }
 
@@ -727,10 +810,12 @@ template <class R>
 __device__ __forceinline__ R total(const Dev& d, uint32_t buf, uint32_t m, uint32_t k, R* red) {
   const uint32_t li = threadIdx.x;
   const double* chunk = d.chunk + (size_t)buf * MAX_CHUNKS * 4u;
-  R s = (R)0;
-  for (uint32_t i = li; i < m; i += 256u) s += (R)chunk[i * 4u + k];
   __syncthreads();
-  red[li] = s;
+  if (li < 256u) {
+    R s = (R)0;
+    for (uint32_t i = li; i < m; i += 256u) s += (R)chunk[i * 4u + k];
+    red[li] = s;
+  }
   __syncthreads();
   for (uint32_t off = 128; off > 0; off >>= 1) {
     if (li < off) red[li] += red[li + off];
@@ -750,8 +835,11 @@ __device__ __forceinline__ bool converged(R rr, R tol, R b_norm) {
   return sq(rr) <= tol * (b_norm < (R)1 ? (R)1 : b_norm);
 }
 
+constexpr uint32_t SOLVE_THREADS = 1024;
+constexpr uint32_t SOLVE_WAVES = SOLVE_THREADS / 64;
+
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_solve(Dev d) {
+__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
   using R = typename std::conditional<MIXED, double, float>::type;
   __shared__ R red[256];
   if (d.st->dt_ns == 0u) return;
@@ -773,7 +861,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d) {
   uint32_t water_here = 0;
 
   // solve_init (fluid.wgsl:450-491)
-  for_blocks(d, b0, b1, 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+  for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
     const uint32_t cell = b * 64u + lane;
     const bool water = cfr(d, NATRAY_F_TYPE, cell) == WATER;
     float diag = 1.0f, r = 0.0f, bv = 0.0f;
@@ -823,7 +911,17 @@ __global__ __launch_bounds__(256) void k_solve(Dev d) {
       water_here += nw;
     }
   });
-  if (lane_id() == 0 && water_here) atomicAdd(d.wcount, water_here);
+  {
+    // Water cells of this workgroup's blocks, for the work counters.
+    __shared__ uint32_t s_water[SOLVE_WAVES];
+    if (lane_id() == 0) s_water[threadIdx.x >> 6] = water_here;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t w = 0;
+      for (uint32_t k = 0; k < SOLVE_WAVES; ++k) w += s_water[k];
+      d.wpart[g] = w;
+    }
+  }
   fold_chunks<R>(d, 1, c0, c1, K, n, 3);
   bool ok = grid_sync(d, G);
 
@@ -844,7 +942,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d) {
     const float* cDo = d.cf + (size_t)(NATRAY_F_D + old) * capc;
     float* cDn = d.cf + (size_t)(NATRAY_F_D + 1u - old) * capc;
     const float fb = (float)beta;
-    for_blocks(d, b0, b1, 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
       const uint32_t cell = b * 64u + lane;
       const uint32_t mask = d.nbm[cell];
       R v = (R)0;
@@ -892,11 +990,11 @@ __global__ __launch_bounds__(256) void k_solve(Dev d) {
         }
       }
     };
-    for_blocks(d, b0, b1, 4u, update);
+    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, update);
     if (refresh) {
       // True residual r = b - A p in double once every p is updated, then the preconditioner.
       if (!grid_sync(d, G)) break;
-      for_blocks(d, b0, b1, 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
         const uint32_t cell = b * 64u + lane;
         const uint32_t mask = d.nbm[cell];
         const bool water = (mask & 64u) != 0u;
@@ -952,7 +1050,8 @@ __global__ __launch_bounds__(256) void k_solve(Dev d) {
     d.solver[2] = (double)beta;
     d.solver[3] = (double)b_norm;
     d.solver[4] = (double)r_norm;
-    const uint32_t w = __hip_atomic_load(d.wcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t w = 0;
+    for (uint32_t k = 0; k < G; ++k) w += d.wpart[k];
     d.stats[0] += 1ull;
     d.stats[1] += iters;
     d.stats[2] += (unsigned long long)w * iters;
@@ -1228,8 +1327,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int corner[3] = {bx * 4 - 1, by * 4 - 1, bz * 4 - 1};
     for (uint32_t k = tid; k < count; k += 256) fastest = fmax_(fastest, g2p_one(d, tg, links, d.sorted[first + k], corner));
   }
+  // The workgroup's fastest particle goes to its own slot; k_vmax takes the maximum. One
+  // atomic per wave on State.max_speed would serialise thousands of device-scope atomics on
+  // one word.
+  __shared__ float s_fast[4];
   for (int off = 32; off > 0; off >>= 1) fastest = fmax_(fastest, __shfl_xor(fastest, off));
-  if (lane_id() == 0 && fastest > 0.0f) atomicMax(reinterpret_cast<unsigned int*>(&d.st->max_speed), __float_as_uint(fastest));
+  if (lane_id() == 0) s_fast[tid >> 6] = fastest;
+  __syncthreads();
+  if (tid == 0) d.vmax_part[blockIdx.x] = fmax_(fmax_(s_fast[0], s_fast[1]), fmax_(s_fast[2], s_fast[3]));
+}
+
+// The step's fastest particle from k_g2p's per-workgroup maxima (the atomicMax of
+// fluid.wgsl:898-900 as a reduction).
+__global__ __launch_bounds__(256) void k_vmax(Dev d, uint32_t parts) {
+  if (d.st->dt_ns == 0u) return;
+  __shared__ float s[4];
+  float m = 0.0f;
+  for (uint32_t k = threadIdx.x; k < parts; k += 256) m = fmax_(m, d.vmax_part[k]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax_(m, __shfl_xor(m, off));
+  if (lane_id() == 0) s[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) d.st->max_speed = fmax_(fmax_(s[0], s[1]), fmax_(s[2], s[3]));
 }
 
 // ---- launch wrappers -------------------------------------------------------------------------------
@@ -1264,16 +1382,16 @@ int launch_grid_prep(const Dev& d, void* stream, int grid_waves) {
 }
 int solve_max_grid(int mixed, int* per_cu) {
   int occ = 0;
-  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<true>, 256, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<false>, 256, 0);
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<true>, SOLVE_THREADS, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<false>, SOLVE_THREADS, 0);
   *per_cu = occ;
   return (int)e;
 }
 int launch_solve(const Dev& d, void* stream, int grid, int mixed) {
   Dev copy = d;
   void* args[] = {&copy};
-  hipError_t e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true>, dim3(grid), dim3(256), args, 0, S(stream))
-                       : hipLaunchCooperativeKernel((const void*)k_solve<false>, dim3(grid), dim3(256), args, 0, S(stream));
+  hipError_t e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true>, dim3(grid), dim3(SOLVE_THREADS), args, 0, S(stream))
+                       : hipLaunchCooperativeKernel((const void*)k_solve<false>, dim3(grid), dim3(SOLVE_THREADS), args, 0, S(stream));
   return (int)e;
 }
 int launch_project(const Dev& d, void* stream, int grid_waves) {
@@ -1287,6 +1405,7 @@ int launch_extend(const Dev& d, void* stream, int grid_waves) {
 }
 int launch_g2p(const Dev& d, void* stream, int grid_blocks) {
   hipLaunchKernelGGL(k_g2p, dim3(grid_blocks), dim3(256), 0, S(stream), d);
+  hipLaunchKernelGGL(k_vmax, dim3(1), dim3(256), 0, S(stream), d, (uint32_t)grid_blocks);
   return (int)hipGetLastError();
 }
 

@@ -114,6 +114,16 @@ int harvest_events(natray_sim* s) {
   return NATRAY_OK;
 }
 
+// Workgroups of the cooperative solve: about four block chunks each (at most MAX_CHUNKS
+// chunks, see the solve kernel), at most one per CU, sized from the block count of the last
+// State read back. Any count is correct; fewer make the grid barrier cheaper.
+int solve_grid_for(const natray_sim* s) {
+  uint32_t n = std::min(s->last.n_blocks, s->U.bdims[3]);
+  if (n == 0) return s->solve_grid;
+  const uint32_t K = (n + natray::MAX_CHUNKS - 1) / natray::MAX_CHUNKS, m = (n + K - 1) / K;
+  return (int)std::max<uint32_t>(1, std::min<uint32_t>((m + 3) / 4, (uint32_t)s->solve_grid));
+}
+
 int enqueue_phase(natray_sim* s, int phase) {
   const natray::Dev& D = s->D;
   void* st = s->stream;
@@ -129,7 +139,7 @@ int enqueue_phase(natray_sim* s, int phase) {
     case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(D, st, s->grid_cells)); break;
     case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(D, st, s->grid_blocks)); break;
     case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(D, st, s->grid_waves)); break;
-    case NATRAY_PHASE_SOLVE: LAUNCHCHK(natray::launch_solve(D, st, s->solve_grid, s->mixed)); break;
+    case NATRAY_PHASE_SOLVE: LAUNCHCHK(natray::launch_solve(D, st, solve_grid_for(s), s->mixed)); break;
     case NATRAY_PHASE_PROJECT: LAUNCHCHK(natray::launch_project(D, st, s->grid_waves)); break;
     case NATRAY_PHASE_EXTEND: LAUNCHCHK(natray::launch_extend(D, st, s->grid_waves)); break;
     case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(D, st, s->grid_blocks)); break;
@@ -156,7 +166,7 @@ int sync_state(natray_sim* s) {
     if (e) return e;
   }
   if (s->h_words[2] != 0u) {
-    HIPCHK(hipMemsetAsync(s->D.bar, 0, 3 * sizeof(uint32_t), s->stream));
+    HIPCHK(hipMemsetAsync(s->D.bar, 0, 1024 * sizeof(uint32_t), s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return fail(NATRAY_E_SOLVER, "the pressure solve lost its grid barrier (not all workgroups were resident)");
   }
@@ -268,8 +278,9 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &D.chunk, (size_t)natray::MAX_CHUNKS * 4 * 2));
   TRY(dalloc(s, &D.tile_sum, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.tile_off, (size_t)D.ntiles * 2));
-  TRY(dalloc(s, &D.bar, 4));
-  TRY(dalloc(s, &D.wcount, 1));
+  TRY(dalloc(s, &D.bar, 1024));
+  TRY(dalloc(s, &D.wpart, 1024));
+  TRY(dalloc(s, &D.vmax_part, (size_t)D.cap));
   TRY(dalloc(s, &D.solver, 8));
   TRY(dalloc(s, &D.stats, 8));
 
@@ -284,8 +295,9 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   HIPTRY(hipMemsetAsync(D.nbm, 0, capc * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.part, 0, (size_t)D.cap * 4 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.chunk, 0, (size_t)natray::MAX_CHUNKS * 4 * 8 * 2, s->stream));
-  HIPTRY(hipMemsetAsync(D.bar, 0, 4 * 4, s->stream));
-  HIPTRY(hipMemsetAsync(D.wcount, 0, 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.bar, 0, 1024 * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.wpart, 0, 1024 * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.vmax_part, 0, (size_t)D.cap * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.solver, 0, 8 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.stats, 0, 8 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.sorted, 0, ((size_t)n + 1) * 4, s->stream));
@@ -319,7 +331,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   int per_cu = 0;
   e = natray::solve_max_grid(s->mixed, &per_cu);
   if (e || per_cu <= 0) return bail(fail(NATRAY_E_HIP, "cannot size the cooperative solve"));
-  s->solve_grid = std::min(per_cu * s->num_cus, 2 * s->num_cus);
+  s->solve_grid = std::min(per_cu * s->num_cus, s->num_cus);
   *out = s;
   return NATRAY_OK;
 #undef TRY

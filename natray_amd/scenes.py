@@ -250,6 +250,22 @@ def box_params(side: int, n: int, capacity: int, gravity: float = 9.81, cfl: flo
     return U
 
 
+def write_scene_file(s: Scene, path: str) -> None:
+    """Writes the scene for natray_drive: b"NATRAYSC", u32 version 1, u32 particle count,
+    Params, natray_scene, State, then the 80-byte particle records."""
+    import ctypes as C
+    import struct
+
+    init = s.init if s.init is not None else no_limits_state()
+    with open(path, "wb") as f:
+        f.write(b"NATRAYSC")
+        f.write(struct.pack("<II", 1, s.n))
+        f.write(bytes(C.string_at(C.addressof(s.params), C.sizeof(s.params))))
+        f.write(bytes(C.string_at(C.addressof(s.scene), C.sizeof(s.scene))))
+        f.write(bytes(C.string_at(C.addressof(init), C.sizeof(init))))
+        f.write(np.ascontiguousarray(s.particles, np.float32).tobytes())
+
+
 def no_limits_state() -> abi.State:
     """First State with no rubber or air step limit (neither is simulated)."""
     st = abi.State()
