@@ -168,7 +168,8 @@ typedef struct natray_scene {
   uint32_t refresh_every; /* mixed precision: recompute r = b - A p in f64 every this many iterations (0 = never) */
   float fixed_dt;         /* > 0: every step is this long (s), instead of the CFL choice */
   float mask_p;           /* gauge pressure (Pa) of COVERED mask cells */
-  uint32_t reserved0[2];
+  uint32_t solve_groups;  /* workgroups of the persistent pressure solve (0 = chosen from the block count) */
+  uint32_t reserved0;
   float capsule_a[4];     /* capsule axis start at t = 0 (xyz), w = radius */
   float capsule_b[4];     /* capsule axis end at t = 0 (xyz) */
   float capsule_v[4];     /* capsule velocity (xyz) */

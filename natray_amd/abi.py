@@ -115,7 +115,8 @@ class Scene(C.Structure):
         ("refresh_every", C.c_uint32),
         ("fixed_dt", C.c_float),
         ("mask_p", C.c_float),
-        ("reserved0", C.c_uint32 * 2),
+        ("solve_groups", C.c_uint32),
+        ("reserved0", C.c_uint32),
         ("capsule_a", C.c_float * 4),
         ("capsule_b", C.c_float * 4),
         ("capsule_v", C.c_float * 4),

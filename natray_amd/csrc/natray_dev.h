@@ -46,7 +46,10 @@ struct Dev {
   double* chunk;           // 2 x MAX_CHUNKS x 4: chunk sums of those, double-buffered
   uint32_t* tile_sum;      // 2 x ntiles: scan tile totals (slots claimed, particles)
   uint32_t* tile_off;      // 2 x ntiles: their exclusive prefix
-  uint32_t* bar;           // grid barrier: [0] arrivals, [1] generation, [2] abort flag
+  uint32_t* bar;           // solve control words; [2] = abort flag a stuck barrier raises
+  unsigned long long* sync_flags;  // 1024: the solve's per-workgroup arrival tokens
+  unsigned long long* sync_gen;    // [0] launch epoch, [1] last token workgroup 0 released
+  double* bcast;                   // 4: totals workgroup 0 broadcasts at a solve barrier
   uint32_t* wpart;         // water cells of each solve workgroup's blocks (<= 1024 workgroups)
   float* vmax_part;        // fastest particle of each g2p workgroup (<= cap workgroups)
   double* solver;          // double copies of rz, alpha, beta, b_norm, r_norm

@@ -682,63 +682,8 @@ __global__ __launch_bounds__(256) void k_grid_prep(Dev d) {
 
 // ---- pressure solve (fluid.wgsl:381-578) ------------------------------------------------------------
 
-// Grid barrier of the cooperative solve, a two-level counter: workgroups count in on one of
-// BAR_GROUPS group counters (each on its own cache line, so arrivals do not serialise on one
-// address), the last of each group counts in on the top counter, and the last of those moves
-// the generation everyone waits on. Only thread 0 of a workgroup talks to the counters:
-// the workgroup barrier orders its threads' writes before thread 0's agent-scope release
-// fence, and the acquire fence after the wait before their reads. A wait that runs past
-// kSpinLimit polls, or sees another workgroup's abort, sets the abort flag and returns, so no
-// wave can wait forever.
+// Polls of a solve barrier before a wave gives up and raises the abort flag (Dev::bar[2]).
 constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr uint32_t BAR_GROUPS = 16;
-// Words of Dev::bar: [0] top counter, [1] generation, [2] abort, group g's counter at
-// BAR_GROUP0 + 32 g.
-constexpr uint32_t BAR_GROUP0 = 64;
-
-__device__ __forceinline__ bool grid_sync(const Dev& d, uint32_t G) {
-  __shared__ uint32_t s_abort;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t* top = d.bar;
-    uint32_t* gen = d.bar + 1;
-    uint32_t* abort = d.bar + 2;
-    const uint32_t ngroups = G < BAR_GROUPS ? G : BAR_GROUPS;
-    const uint32_t grp = blockIdx.x % ngroups;
-    const uint32_t members = G / ngroups + (grp < G % ngroups ? 1u : 0u);
-    uint32_t* gc = d.bar + BAR_GROUP0 + 32u * grp;
-    const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    bool last = false;
-    if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1u) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      last = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1u;
-    }
-    if (last) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __hip_atomic_store(gen, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      uint32_t spins = 0;
-      while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        __builtin_amdgcn_s_sleep(1);
-        if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kSpinLimit) {
-          __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    s_abort = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  return s_abort == 0u;
-}
 
 // block_sum's tree (fluid.wgsl:414-422) inside one wave; lane 0 ends with the sum.
 template <class R>
@@ -826,6 +771,67 @@ __device__ __forceinline__ R total(const Dev& d, uint32_t buf, uint32_t m, uint3
   return r;
 }
 
+// Barrier of the cooperative solve that also reduces. Each workgroup, once its chunk sums
+// are written, releases them and stores `token` in its own flag word; workgroup 0 waits for
+// every flag, sums the chunk buffer in total()'s fixed order, and releases the nc totals with
+// the generation word; the others wait for the generation and read the totals. No two
+// workgroups write the same word, and only one reduces. Tokens are unique per launch (the
+// launch epoch in the high half), so a wait compares for equality. A wait that runs past
+// kSpinLimit polls, or sees another workgroup's abort, sets the abort flag and returns
+// false, so no wave can wait forever.
+template <class R>
+__device__ __forceinline__ bool sync_reduce(const Dev& d, uint32_t G, unsigned long long token, uint32_t buf, uint32_t m,
+                                            int nc, R* out, R* red) {
+  __shared__ uint32_t s_abort;
+  __shared__ double s_tot[4];
+  uint32_t* abort = d.bar + 2;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(&d.sync_flags[blockIdx.x], token, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (blockIdx.x == 0) {
+    for (uint32_t t = threadIdx.x; t < G; t += blockDim.x) {
+      uint32_t spins = 0;
+      while (__hip_atomic_load(&d.sync_flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != token) {
+        __builtin_amdgcn_s_sleep(1);
+        if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kSpinLimit) {
+          __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int k = 0; k < nc; ++k) out[k] = total<R>(d, buf, m, (uint32_t)k, red);
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < nc; ++k) d.bcast[k] = (double)out[k];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(&d.sync_gen[1], token, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_abort = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return s_abort == 0u;
+  }
+  if (threadIdx.x == 0) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(&d.sync_gen[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != token) {
+      __builtin_amdgcn_s_sleep(1);
+      if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kSpinLimit) {
+        __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int k = 0; k < nc; ++k) s_tot[k] = d.bcast[k];
+    s_abort = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  for (int k = 0; k < nc; ++k) out[k] = (R)s_tot[k];
+  return s_abort == 0u;
+}
+
 __device__ __forceinline__ float sq(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double sq(double x) { return __builtin_sqrt(x); }
 
@@ -837,6 +843,8 @@ __device__ __forceinline__ bool converged(R rr, R tol, R b_norm) {
 
 constexpr uint32_t SOLVE_THREADS = 1024;
 constexpr uint32_t SOLVE_WAVES = SOLVE_THREADS / 64;
+// Blocks a wave of the solve keeps in registers.
+constexpr uint32_t SOLVE_RMAX = 2;
 
 template <bool MIXED>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
@@ -859,6 +867,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
   float* const cQ = d.cf + (size_t)NATRAY_F_Q * capc;
   float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
   uint32_t water_here = 0;
+  // Barrier tokens of this launch: the epoch (moved on by workgroup 0 at the end) above a count.
+  unsigned long long token = __hip_atomic_load(&d.sync_gen[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 32;
 
   // solve_init (fluid.wgsl:450-491)
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
@@ -923,17 +933,51 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
     }
   }
   fold_chunks<R>(d, 1, c0, c1, K, n, 3);
-  bool ok = grid_sync(d, G);
+  R tt[3];
+  bool ok = sync_reduce<R>(d, G, ++token, 1, m, 3, tt, red);
 
   // solve_start (fluid.wgsl:493-506)
   const R tol = (R)U->solve[0];
-  R rz = total<R>(d, 1, m, 0, red), rr = total<R>(d, 1, m, 1, red), bb = total<R>(d, 1, m, 2, red);
+  R rz = tt[0], rr = tt[1], bb = tt[2];
   R beta = (R)0, alpha = (R)0;
   const R b_norm = sq(bb);
   R r_norm = sq(rr);
   uint32_t iters = 0, iters_max = d.st->iters_max, unconverged = d.st->unconverged;
   bool go = ok && rz > (R)0 && !converged<R>(rr, tol, b_norm) && U->limits[2] > 0u;
   const bool refresh_on = MIXED && d.sc->precision == NATRAY_CG_MIXED && d.sc->refresh_every > 0u;
+
+  // A workgroup whose blocks fit keeps them in registers for the whole solve, wave w owning
+  // blocks b0 + w + SOLVE_WAVES r (r < SOLVE_RMAX), the same blocks for_blocks gave it above:
+  // in-block neighbours come from lane shuffles and only halo values from other blocks, and
+  // the directions and z the other blocks read, go through memory. Larger workgroups run the
+  // same iteration on the cf fields in memory. Both evaluate the same expressions in the same
+  // order, and keep the same grid barriers.
+  __shared__ uint32_t lkr[SOLVE_WAVES][SOLVE_RMAX][27];
+  const bool in_regs = b1 - b0 <= SOLVE_WAVES * SOLVE_RMAX;
+  const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
+  const int lx = (int)(ln >> 4), ly = (int)((ln >> 2) & 3u), lz = (int)(ln & 3u);
+  const int sxm = lx > 0 ? (int)ln - 16 : (int)ln, sxp = lx < 3 ? (int)ln + 16 : (int)ln;
+  const int sym = ly > 0 ? (int)ln - 4 : (int)ln, syp = ly < 3 ? (int)ln + 4 : (int)ln;
+  const int szm = lz > 0 ? (int)ln - 1 : (int)ln, szp = lz < 3 ? (int)ln + 1 : (int)ln;
+  float rP[SOLVE_RMAX], rR[SOLVE_RMAX], rZ[SOLVE_RMAX], rDo[SOLVE_RMAX], rDn[SOLVE_RMAX], rQ[SOLVE_RMAX], rDg[SOLVE_RMAX];
+  uint32_t rM[SOLVE_RMAX];
+#pragma unroll
+  for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
+    rP[r] = rR[r] = rZ[r] = rDo[r] = rDn[r] = rQ[r] = 0.0f;
+    rDg[r] = 1.0f;
+    rM[r] = 0;
+    const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+    if (in_regs && b < b1) {
+      if (ln < 27) lkr[wv][r][ln] = d.block_nb[b * 27u + ln];
+      const uint32_t cell = b * 64u + ln;
+      rM[r] = d.nbm[cell];
+      rP[r] = cP[cell];
+      rR[r] = cR[cell];
+      rZ[r] = cZ[cell];
+      rDg[r] = cDiag[cell];
+      rDo[r] = d.cf[(size_t)NATRAY_F_D * capc + cell];
+    }
+  }
 
   while (go) {
     // solve_q (fluid.wgsl:508-534): the new direction and q = A d; neighbours' directions are
@@ -942,88 +986,185 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
     const float* cDo = d.cf + (size_t)(NATRAY_F_D + old) * capc;
     float* cDn = d.cf + (size_t)(NATRAY_F_D + 1u - old) * capc;
     const float fb = (float)beta;
-    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-      const uint32_t cell = b * 64u + lane;
-      const uint32_t mask = d.nbm[cell];
-      R v = (R)0;
-      if (mask & 64u) {
-        const float dn = cZ[cell] + fb * cDo[cell];
-        float q = cDiag[cell] * dn;
-        if (mask & 1u) { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 2u) { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 4u) { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 8u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 16u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 32u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; }
-        cDn[cell] = dn;
-        cQ[cell] = q;
-        v = prod<R>(dn, q);
+    if (in_regs) {
+#pragma unroll
+      for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
+        const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+        if (b >= b1) continue;
+        const uint32_t cell = b * 64u + ln, mask = rM[r];
+        const uint32_t* lk = lkr[wv][r];
+        const float dn = rZ[r] + fb * rDo[r];
+        const float vxm = __shfl(dn, sxm), vxp = __shfl(dn, sxp), vym = __shfl(dn, sym), vyp = __shfl(dn, syp),
+                    vzm = __shfl(dn, szm), vzp = __shfl(dn, szp);
+        R v = (R)0;
+        if (mask & 64u) {
+          float q = rDg[r] * dn;
+          if (mask & 1u) { if (lx > 0) q -= vxm; else { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; } }
+          if (mask & 2u) { if (lx < 3) q -= vxp; else { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; } }
+          if (mask & 4u) { if (ly > 0) q -= vym; else { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; } }
+          if (mask & 8u) { if (ly < 3) q -= vyp; else { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; } }
+          if (mask & 16u) { if (lz > 0) q -= vzm; else { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; } }
+          if (mask & 32u) { if (lz < 3) q -= vzp; else { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; } }
+          cDn[cell] = dn;
+          rDn[r] = dn;
+          rQ[r] = q;
+          v = prod<R>(dn, q);
+        }
+        v = wave_tree<R>(v);
+        if (ln == 0) d.part[b * 4u] = (double)v;
       }
-      v = wave_tree<R>(v);
-      if (lane == 0) d.part[b * 4u] = (double)v;
-    });
+    } else {
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+        const uint32_t cell = b * 64u + lane;
+        const uint32_t mask = d.nbm[cell];
+        R v = (R)0;
+        if (mask & 64u) {
+          const float dn = cZ[cell] + fb * cDo[cell];
+          float q = cDiag[cell] * dn;
+          if (mask & 1u) { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 2u) { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 4u) { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 8u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 16u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 32u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; }
+          cDn[cell] = dn;
+          cQ[cell] = q;
+          v = prod<R>(dn, q);
+        }
+        v = wave_tree<R>(v);
+        if (lane == 0) d.part[b * 4u] = (double)v;
+      });
+    }
     fold_chunks<R>(d, 0, c0, c1, K, n, 1);
-    if (!grid_sync(d, G)) break;
+    if (!sync_reduce<R>(d, G, ++token, 0, m, 1, tt, red)) break;
     // solve_alpha (fluid.wgsl:536-541)
-    const R dq = total<R>(d, 0, m, 0, red);
+    const R dq = tt[0];
     alpha = dq > (R)0 ? rz / dq : (R)0;
     const float fa = (float)alpha;
     const bool refresh = refresh_on && (iters + 1u) % d.sc->refresh_every == 0u;
     // solve_update (fluid.wgsl:543-559): p += alpha d, r -= alpha q, z = M^-1 r.
-    auto update = [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t*) {
-      const uint32_t cell = b * 64u + lane;
-      const bool water = (d.nbm[cell] & 64u) != 0u;
-      float r = 0.0f, diag = 1.0f;
-      if (water) {
-        cP[cell] += fa * cDn[cell];
-        r = cR[cell] - fa * cQ[cell];
-        cR[cell] = r;
-        diag = cDiag[cell];
-      }
-      if (!refresh) {
-        const float z = precondition(r, diag, water, lx, ly, lz, mode);
-        if (water) cZ[cell] = z;
-        const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
-        if (lane == 0) {
-          d.part[b * 4u + 0] = (double)v0;
-          d.part[b * 4u + 1] = (double)v1;
+    if (in_regs) {
+#pragma unroll
+      for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
+        const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+        if (b >= b1) continue;
+        const uint32_t cell = b * 64u + ln;
+        const bool water = (rM[r] & 64u) != 0u;
+        float rr_ = 0.0f, diag = 1.0f;
+        if (water) {
+          rP[r] += fa * rDn[r];
+          rR[r] = rR[r] - fa * rQ[r];
+          rr_ = rR[r];
+          diag = rDg[r];
+          if (refresh) cP[cell] = rP[r];
+        }
+        rDo[r] = rDn[r];
+        if (!refresh) {
+          const float z = precondition(rr_, diag, water, lx, ly, lz, mode);
+          if (water) {
+            rZ[r] = z;
+            cZ[cell] = z;
+          }
+          const R v0 = wave_tree<R>(prod<R>(rr_, z)), v1 = wave_tree<R>(prod<R>(rr_, rr_));
+          if (ln == 0) {
+            d.part[b * 4u + 0] = (double)v0;
+            d.part[b * 4u + 1] = (double)v1;
+          }
         }
       }
-    };
-    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, update);
-    if (refresh) {
-      // True residual r = b - A p in double once every p is updated, then the preconditioner.
-      if (!grid_sync(d, G)) break;
-      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    } else {
+      auto update = [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t*) {
         const uint32_t cell = b * 64u + lane;
-        const uint32_t mask = d.nbm[cell];
-        const bool water = (mask & 64u) != 0u;
+        const bool water = (d.nbm[cell] & 64u) != 0u;
         float r = 0.0f, diag = 1.0f;
         if (water) {
-          diag = cDiag[cell];
-          double ap = (double)diag * (double)cP[cell];
-          if (mask & 1u) ap -= (double)cP[nbr(lk, lx, ly, lz, -1, 0, 0)];
-          if (mask & 2u) ap -= (double)cP[nbr(lk, lx, ly, lz, 1, 0, 0)];
-          if (mask & 4u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, -1, 0)];
-          if (mask & 8u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 1, 0)];
-          if (mask & 16u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, -1)];
-          if (mask & 32u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, 1)];
-          r = (float)((double)d.bvec[cell] - ap);
+          cP[cell] += fa * cDn[cell];
+          r = cR[cell] - fa * cQ[cell];
           cR[cell] = r;
+          diag = cDiag[cell];
         }
-        const float z = precondition(r, diag, water, lx, ly, lz, mode);
-        if (water) cZ[cell] = z;
-        const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
-        if (lane == 0) {
-          d.part[b * 4u + 0] = (double)v0;
-          d.part[b * 4u + 1] = (double)v1;
+        if (!refresh) {
+          const float z = precondition(r, diag, water, lx, ly, lz, mode);
+          if (water) cZ[cell] = z;
+          const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+          if (lane == 0) {
+            d.part[b * 4u + 0] = (double)v0;
+            d.part[b * 4u + 1] = (double)v1;
+          }
         }
-      });
+      };
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, update);
+    }
+    if (refresh) {
+      // True residual r = b - A p in double once every p is updated, then the preconditioner.
+      if (!sync_reduce<R>(d, G, ++token, 0, m, 0, tt, red)) break;
+      if (in_regs) {
+#pragma unroll
+        for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
+          const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+          if (b >= b1) continue;
+          const uint32_t cell = b * 64u + ln, mask = rM[r];
+          const uint32_t* lk = lkr[wv][r];
+          const bool water = (mask & 64u) != 0u;
+          const float p = rP[r];
+          const float pxm = __shfl(p, sxm), pxp = __shfl(p, sxp), pym = __shfl(p, sym), pyp = __shfl(p, syp),
+                      pzm = __shfl(p, szm), pzp = __shfl(p, szp);
+          float rr_ = 0.0f, diag = 1.0f;
+          if (water) {
+            diag = rDg[r];
+            double ap = (double)diag * (double)p;
+            if (mask & 1u) ap -= (double)(lx > 0 ? pxm : cP[nbr(lk, lx, ly, lz, -1, 0, 0)]);
+            if (mask & 2u) ap -= (double)(lx < 3 ? pxp : cP[nbr(lk, lx, ly, lz, 1, 0, 0)]);
+            if (mask & 4u) ap -= (double)(ly > 0 ? pym : cP[nbr(lk, lx, ly, lz, 0, -1, 0)]);
+            if (mask & 8u) ap -= (double)(ly < 3 ? pyp : cP[nbr(lk, lx, ly, lz, 0, 1, 0)]);
+            if (mask & 16u) ap -= (double)(lz > 0 ? pzm : cP[nbr(lk, lx, ly, lz, 0, 0, -1)]);
+            if (mask & 32u) ap -= (double)(lz < 3 ? pzp : cP[nbr(lk, lx, ly, lz, 0, 0, 1)]);
+            rr_ = (float)((double)d.bvec[cell] - ap);
+            rR[r] = rr_;
+          }
+          const float z = precondition(rr_, diag, water, lx, ly, lz, mode);
+          if (water) {
+            rZ[r] = z;
+            cZ[cell] = z;
+          }
+          const R v0 = wave_tree<R>(prod<R>(rr_, z)), v1 = wave_tree<R>(prod<R>(rr_, rr_));
+          if (ln == 0) {
+            d.part[b * 4u + 0] = (double)v0;
+            d.part[b * 4u + 1] = (double)v1;
+          }
+        }
+      } else {
+        for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+          const uint32_t cell = b * 64u + lane;
+          const uint32_t mask = d.nbm[cell];
+          const bool water = (mask & 64u) != 0u;
+          float r = 0.0f, diag = 1.0f;
+          if (water) {
+            diag = cDiag[cell];
+            double ap = (double)diag * (double)cP[cell];
+            if (mask & 1u) ap -= (double)cP[nbr(lk, lx, ly, lz, -1, 0, 0)];
+            if (mask & 2u) ap -= (double)cP[nbr(lk, lx, ly, lz, 1, 0, 0)];
+            if (mask & 4u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, -1, 0)];
+            if (mask & 8u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 1, 0)];
+            if (mask & 16u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, -1)];
+            if (mask & 32u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, 1)];
+            r = (float)((double)d.bvec[cell] - ap);
+            cR[cell] = r;
+          }
+          const float z = precondition(r, diag, water, lx, ly, lz, mode);
+          if (water) cZ[cell] = z;
+          const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+          if (lane == 0) {
+            d.part[b * 4u + 0] = (double)v0;
+            d.part[b * 4u + 1] = (double)v1;
+          }
+        });
+      }
     }
     fold_chunks<R>(d, 1, c0, c1, K, n, 2);
-    if (!grid_sync(d, G)) break;
+    if (!sync_reduce<R>(d, G, ++token, 1, m, 2, tt, red)) break;
     // solve_beta (fluid.wgsl:561-578)
-    const R rz_new = total<R>(d, 1, m, 0, red), rr_new = total<R>(d, 1, m, 1, red);
+    const R rz_new = tt[0], rr_new = tt[1];
     iters += 1u;
     r_norm = sq(rr_new);
     beta = rz > (R)0 ? rz_new / rz : (R)0;
@@ -1035,7 +1176,21 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
       go = false;
     }
   }
+  if (in_regs) {
+    // The fields the other kernels and the host read: p, r and q of this wave's water cells.
+#pragma unroll
+    for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b < b1 && (rM[r] & 64u)) {
+        const uint32_t cell = b * 64u + ln;
+        cP[cell] = rP[r];
+        cR[cell] = rR[r];
+        if (iters > 0u) cQ[cell] = rQ[r];
+      }
+    }
+  }
   if (g == 0 && threadIdx.x == 0) {
+    __hip_atomic_store(&d.sync_gen[0], (token >> 32) + 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     natray_state* st = d.st;
     st->rz = (float)rz;
     st->alpha = (float)alpha;

@@ -118,6 +118,7 @@ int harvest_events(natray_sim* s) {
 // chunks, see the solve kernel), at most one per CU, sized from the block count of the last
 // State read back. Any count is correct; fewer make the grid barrier cheaper.
 int solve_grid_for(const natray_sim* s) {
+  if (s->sc.solve_groups) return (int)std::min<uint32_t>(s->sc.solve_groups, (uint32_t)s->solve_grid);
   uint32_t n = std::min(s->last.n_blocks, s->U.bdims[3]);
   if (n == 0) return s->solve_grid;
   const uint32_t K = (n + natray::MAX_CHUNKS - 1) / natray::MAX_CHUNKS, m = (n + K - 1) / K;
@@ -167,6 +168,8 @@ int sync_state(natray_sim* s) {
   }
   if (s->h_words[2] != 0u) {
     HIPCHK(hipMemsetAsync(s->D.bar, 0, 1024 * sizeof(uint32_t), s->stream));
+    HIPCHK(hipMemsetAsync(s->D.sync_flags, 0, 1024 * 8, s->stream));
+    HIPCHK(hipMemsetAsync(s->D.sync_gen, 0, 2 * 8, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return fail(NATRAY_E_SOLVER, "the pressure solve lost its grid barrier (not all workgroups were resident)");
   }
@@ -279,6 +282,9 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &D.tile_sum, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.tile_off, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.bar, 1024));
+  TRY(dalloc(s, &D.sync_flags, 1024));
+  TRY(dalloc(s, &D.sync_gen, 2));
+  TRY(dalloc(s, &D.bcast, 4));
   TRY(dalloc(s, &D.wpart, 1024));
   TRY(dalloc(s, &D.vmax_part, (size_t)D.cap));
   TRY(dalloc(s, &D.solver, 8));
@@ -296,6 +302,9 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   HIPTRY(hipMemsetAsync(D.part, 0, (size_t)D.cap * 4 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.chunk, 0, (size_t)natray::MAX_CHUNKS * 4 * 8 * 2, s->stream));
   HIPTRY(hipMemsetAsync(D.bar, 0, 1024 * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.sync_flags, 0, 1024 * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.sync_gen, 0, 2 * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.bcast, 0, 4 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.wpart, 0, 1024 * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.vmax_part, 0, (size_t)D.cap * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.solver, 0, 8 * 8, s->stream));

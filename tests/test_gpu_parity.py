@@ -39,6 +39,20 @@ SCENES = {
 }
 
 
+def few_solve_groups(name, groups=2):
+    """The same scene with the pressure solve on `groups` workgroups, so each holds more blocks
+    than its waves keep in registers and the solve runs on the fields in memory."""
+    def make():
+        s = SCENES[name]()
+        s.scene.solve_groups = groups
+        return s
+    return make
+
+
+SCENES["sphere_jacobi_32k_mem"] = few_solve_groups("sphere_jacobi_32k")
+SCENES["impact_mixed_32k_mem"] = few_solve_groups("impact_mixed_32k")
+
+
 def assert_same_float(name, a, b):
     a = np.asarray(a)
     b = np.asarray(b)
