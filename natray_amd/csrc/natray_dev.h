@@ -18,8 +18,11 @@ constexpr uint32_t MAX_DIM = 1u << KEY_BITS;
 // Binning scans the domain slots in tiles of this many.
 constexpr uint32_t SCAN_TILE = 1024;
 
-// Solver reductions fold blocks in chunks so there are at most this many partial sums.
-constexpr uint32_t MAX_CHUNKS = 1024;
+// Solver reductions fold blocks in chunks of K = ceil(n / MAX_CHUNKS), so there are at most
+// this many partial sums.
+constexpr uint32_t MAX_CHUNKS = 256;
+// Words of the solve's tagged reduction buffers: 2 buffers x 8 slots x MAX_CHUNKS.
+constexpr uint32_t RED_WORDS = 2u * 8u * MAX_CHUNKS;
 
 // Acc fields the transfer writes and clearing must zero. Fields 9..18 belong to the membrane,
 // air and bullet producers, which are not part of this library, and stay zero.
@@ -42,18 +45,17 @@ struct Dev {
   float* cf;               // 11 x capc solver fields
   float* bvec;             // capc: CG right-hand side (true-residual refresh)
   uint32_t* nbm;           // capc: bit k = face neighbour k is water, bit 6 = the cell is water
-  double* part;            // cap x 4: per-block partial sums of the solver
-  double* chunk;           // 2 x MAX_CHUNKS x 4: chunk sums of those, double-buffered
+  double* part;            // cap x 4: per-block partial sums of the solver (memory form)
   uint32_t* tile_sum;      // 2 x ntiles: scan tile totals (slots claimed, particles)
   uint32_t* tile_off;      // 2 x ntiles: their exclusive prefix
-  uint32_t* bar;           // solve control words; [2] = abort flag a stuck barrier raises
-  unsigned long long* sync_flags;  // 1024: the solve's per-workgroup arrival tokens
-  unsigned long long* sync_gen;    // [0] launch epoch, [1] last token workgroup 0 released
-  double* bcast;                   // 4: totals workgroup 0 broadcasts at a solve barrier
-  uint32_t* wpart;         // water cells of each solve workgroup's blocks (<= 1024 workgroups)
+  uint32_t* bar;           // solve control words: [2] abort flag a stuck wait raises,
+                           // [4] last reduction tag used
+  unsigned long long* rwords;  // RED_WORDS: tagged chunk sums of the solve's reductions
+  unsigned long long* hz;      // capc: tagged z of each water cell (register form of the solve)
   float* vmax_part;        // fastest particle of each g2p workgroup (<= cap workgroups)
   double* solver;          // double copies of rz, alpha, beta, b_norm, r_norm
   unsigned long long* stats;  // counters, see natray_sim_counters
+  const Dev* self;  // a device copy of this struct, for device functions that take it by pointer
   uint32_t N;
   uint32_t S;      // domain slots (product of bdims)
   uint32_t Spad;   // S rounded up to SCAN_TILE

@@ -681,9 +681,86 @@ __global__ __launch_bounds__(256) void k_grid_prep(Dev d) {
 }
 
 // ---- pressure solve (fluid.wgsl:381-578) ------------------------------------------------------------
+//
+// One cooperative launch runs the whole CG: solve_init, then solve_q .. solve_beta for every
+// iteration. The n blocks form m = ceil(n / K) chunks of K = ceil(n / MAX_CHUNKS) consecutive
+// blocks (both fixed by n alone, so the sums do not depend on the launch); workgroup g owns
+// the chunks [g m / G, (g + 1) m / G) and their blocks, wave w the blocks b0 + w + 16 r.
+//
+// Workgroups talk through 64-bit words that hold a 32-bit tag above a 32-bit payload, stored
+// and loaded with agent-scope relaxed atomics: a reader waits until a word carries the tag
+// it expects, and the payload beside it is then the value stored with that tag. Tags number
+// the solve's reductions over the life of the buffers (Dev::bar[4] holds the last one used),
+// and the host clears every tagged word long before the count could wrap.
+//  - A reduction: each workgroup folds its chunks and publishes the chunk sums (one word per
+//    f32, two per f64); every workgroup waits for all chunk words and sums them in total()'s
+//    order itself, so no workgroup relays for the others. Consecutive reductions alternate
+//    between two word buffers: a workgroup can only publish reduction t + 2 after it has seen
+//    every chunk of t + 1, which every workgroup publishes after it has read all of t.
+//  - Halo z: a wave that keeps its blocks in registers publishes z of its water cells with
+//    the tag of the reduction that follows; neighbours read it in the next solve_q. The other
+//    halo values a block needs (the old direction; p for the residual refresh) are rebuilt by
+//    the reader from values it already holds, with the owner's arithmetic, so they are equal.
+// When the blocks of a workgroup do not fit in registers (a launch with too few workgroups
+// for the block count) every workgroup runs the same iteration on the cf fields in memory,
+// with agent-scope release and acquire fences around each reduction. Both forms evaluate the
+// same expressions in the same order and give the same bits.
 
-// Polls of a solve barrier before a wave gives up and raises the abort flag (Dev::bar[2]).
+// Polls of a solve wait before a wave gives up and raises the abort flag (Dev::bar[2]).
 constexpr uint32_t kSpinLimit = 1u << 22;
+// Solve iterations one launch may run, whatever limits.z asks: bounds the tags a launch uses.
+constexpr uint32_t SOLVE_ITER_CAP = 1u << 24;
+
+constexpr uint32_t SOLVE_THREADS = 1024;
+constexpr uint32_t SOLVE_WAVES = SOLVE_THREADS / 64;
+// Blocks a wave of the solve keeps in registers.
+constexpr uint32_t SOLVE_RMAX = 2;
+// Words of one reduction buffer per chunk: 4 components x 2 halves.
+constexpr uint32_t RED_SLOTS = 8;
+
+__device__ __forceinline__ unsigned long long tag_word(uint32_t tag, uint32_t payload) {
+  return ((unsigned long long)tag << 32) | payload;
+}
+__device__ __forceinline__ unsigned long long load_word(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_word(unsigned long long* p, unsigned long long w) {
+  __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup barrier of the register form: orders LDS only, so global stores to other
+// workgroups' words and loads whose results are used later stay in flight across it.
+// `all` makes it a full barrier (global memory too), as the memory form needs.
+__device__ __forceinline__ void wg_sync(bool all) {
+  if (all) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  }
+}
+
+// A reduction value as 32-bit payloads.
+template <class R>
+struct Halves;
+template <>
+struct Halves<float> {
+  static constexpr int N = 1;
+  static __device__ __forceinline__ uint32_t part(float v, int) { return __float_as_uint(v); }
+  static __device__ __forceinline__ float join(uint32_t hi, uint32_t) { return __uint_as_float(hi); }
+};
+template <>
+struct Halves<double> {
+  static constexpr int N = 2;
+  static __device__ __forceinline__ uint32_t part(double v, int i) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return i ? (uint32_t)b : (uint32_t)(b >> 32);
+  }
+  static __device__ __forceinline__ double join(uint32_t hi, uint32_t lo) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  }
+};
 
 // block_sum's tree (fluid.wgsl:414-422) inside one wave; lane 0 ends with the sum.
 template <class R>
@@ -730,108 +807,6 @@ __device__ __forceinline__ R prod(float a, float b) {
   return (R)a * (R)b;
 }
 
-// Fold this workgroup's chunks of block partials (components [0, nc)), K blocks per chunk, in
-// block order, into chunk buffer `buf`. Consecutive reductions use alternate buffers: a
-// workgroup that has read one reduction's chunk sums can only overwrite that buffer after the
-// next grid barrier, which every other reader has passed by then.
-template <class R>
-__device__ __forceinline__ void fold_chunks(const Dev& d, uint32_t buf, uint32_t c0, uint32_t c1, uint32_t K, uint32_t n,
-                                            int nc) {
-  __syncthreads();
-  double* chunk = d.chunk + (size_t)buf * MAX_CHUNKS * 4u;
-  for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-    const uint32_t lo = c * K, hi = min(n, lo + K);
-    for (int k = 0; k < nc; ++k) {
-      R acc = (R)0;
-      for (uint32_t b = lo; b < hi; ++b) acc += (R)d.part[b * 4u + (uint32_t)k];
-      chunk[c * 4u + (uint32_t)k] = (double)acc;
-    }
-  }
-}
-
-// total (fluid.wgsl:426-439) over the m chunk sums of buffer `buf`, component k; every
-// workgroup computes it.
-template <class R>
-__device__ __forceinline__ R total(const Dev& d, uint32_t buf, uint32_t m, uint32_t k, R* red) {
-  const uint32_t li = threadIdx.x;
-  const double* chunk = d.chunk + (size_t)buf * MAX_CHUNKS * 4u;
-  __syncthreads();
-  if (li < 256u) {
-    R s = (R)0;
-    for (uint32_t i = li; i < m; i += 256u) s += (R)chunk[i * 4u + k];
-    red[li] = s;
-  }
-  __syncthreads();
-  for (uint32_t off = 128; off > 0; off >>= 1) {
-    if (li < off) red[li] += red[li + off];
-    __syncthreads();
-  }
-  R r = red[0];
-  __syncthreads();
-  return r;
-}
-
-// Barrier of the cooperative solve that also reduces. Each workgroup, once its chunk sums
-// are written, releases them and stores `token` in its own flag word; workgroup 0 waits for
-// every flag, sums the chunk buffer in total()'s fixed order, and releases the nc totals with
-// the generation word; the others wait for the generation and read the totals. No two
-// workgroups write the same word, and only one reduces. Tokens are unique per launch (the
-// launch epoch in the high half), so a wait compares for equality. A wait that runs past
-// kSpinLimit polls, or sees another workgroup's abort, sets the abort flag and returns
-// false, so no wave can wait forever.
-template <class R>
-__device__ __forceinline__ bool sync_reduce(const Dev& d, uint32_t G, unsigned long long token, uint32_t buf, uint32_t m,
-                                            int nc, R* out, R* red) {
-  __shared__ uint32_t s_abort;
-  __shared__ double s_tot[4];
-  uint32_t* abort = d.bar + 2;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(&d.sync_flags[blockIdx.x], token, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (blockIdx.x == 0) {
-    for (uint32_t t = threadIdx.x; t < G; t += blockDim.x) {
-      uint32_t spins = 0;
-      while (__hip_atomic_load(&d.sync_flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != token) {
-        __builtin_amdgcn_s_sleep(1);
-        if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kSpinLimit) {
-          __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int k = 0; k < nc; ++k) out[k] = total<R>(d, buf, m, (uint32_t)k, red);
-    if (threadIdx.x == 0) {
-      for (int k = 0; k < nc; ++k) d.bcast[k] = (double)out[k];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __hip_atomic_store(&d.sync_gen[1], token, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_abort = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    return s_abort == 0u;
-  }
-  if (threadIdx.x == 0) {
-    uint32_t spins = 0;
-    while (__hip_atomic_load(&d.sync_gen[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != token) {
-      __builtin_amdgcn_s_sleep(1);
-      if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kSpinLimit) {
-        __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int k = 0; k < nc; ++k) s_tot[k] = d.bcast[k];
-    s_abort = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  for (int k = 0; k < nc; ++k) out[k] = (R)s_tot[k];
-  return s_abort == 0u;
-}
-
 __device__ __forceinline__ float sq(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double sq(double x) { return __builtin_sqrt(x); }
 
@@ -841,380 +816,576 @@ __device__ __forceinline__ bool converged(R rr, R tol, R b_norm) {
   return sq(rr) <= tol * (b_norm < (R)1 ? (R)1 : b_norm);
 }
 
-constexpr uint32_t SOLVE_THREADS = 1024;
-constexpr uint32_t SOLVE_WAVES = SOLVE_THREADS / 64;
-// Blocks a wave of the solve keeps in registers.
-constexpr uint32_t SOLVE_RMAX = 2;
+// The blocks and chunks of one solve workgroup.
+struct SolveSpan {
+  uint32_t n, K, m, G;
+  uint32_t c0, c1;  // chunks of this workgroup
+  uint32_t b0, b1;  // their blocks
+  bool in_regs;     // every workgroup's blocks fit in registers (the same for all)
+};
 
-template <bool MIXED>
-__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
-  using R = typename std::conditional<MIXED, double, float>::type;
-  __shared__ R red[256];
-  if (d.st->dt_ns == 0u) return;
+// Reduction `tag` over components [0, nc) of the block partials (nc = 0: a barrier only).
+// This workgroup's block partials are in `lpart` (blocks b0.., register form) or Dev::part.
+// Every workgroup returns the same totals; false when the solve aborted.
+template <class R>
+__device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const double (*lpart)[4], R* out) {
+  constexpr int H = Halves<R>::N;
+  __shared__ R red[4][256];
+  __shared__ uint32_t s_fail;
+  const uint32_t t = threadIdx.x;
+  const int nw = nc > 0 ? nc : 1;
+  unsigned long long* words = d.rwords + (size_t)(tag & 1u) * RED_SLOTS * MAX_CHUNKS;
+  uint32_t* abort = d.bar + 2;
+  if (t == 0) s_fail = 0u;
+  wg_sync(!sp.in_regs);
+  // total()'s chunk fold, then publish: one word per chunk, component and half.
+  if (t < sp.c1 - sp.c0) {
+    const uint32_t c = sp.c0 + t, lo = c * sp.K, hi = min(sp.n, lo + sp.K);
+    if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (int k = 0; k < nw; ++k) {
+      R acc = (R)0;
+      if (k < nc)
+        for (uint32_t b = lo; b < hi; ++b) acc += (R)(sp.in_regs ? lpart[b - sp.b0][k] : d.part[b * 4u + (uint32_t)k]);
+#pragma unroll
+      for (int h = 0; h < H; ++h)
+        store_word(&words[(uint32_t)(k * 2 + h) * MAX_CHUNKS + c], tag_word(tag, Halves<R>::part(acc, h)));
+    }
+  }
+  // Every chunk's words, summed as total() does (fluid.wgsl:426-439): thread t holds chunk t,
+  // then the tree over the 256 threads.
+  if (t < 256u) {
+    R v[4] = {(R)0, (R)0, (R)0, (R)0};
+    if (t < sp.m) {
+      uint32_t pay[RED_SLOTS];
+      uint32_t have = 0, spins = 0;
+      const uint32_t full = (1u << (uint32_t)(nw * 2)) - 1u;
+      // Slots of absent halves count as seen.
+      uint32_t skip = 0;
+#pragma unroll
+      for (uint32_t s = 0; s < RED_SLOTS; ++s) {
+        pay[s] = 0u;
+        if ((s & 1u) >= (uint32_t)H) skip |= 1u << s;
+      }
+      have = skip & full;
+      while (true) {
+#pragma unroll
+        for (uint32_t s = 0; s < RED_SLOTS; ++s) {
+          if (!((full >> s) & 1u) || ((have >> s) & 1u)) continue;
+          const unsigned long long w = load_word(&words[s * MAX_CHUNKS + t]);
+          if ((uint32_t)(w >> 32) == tag) {
+            pay[s] = (uint32_t)w;
+            have |= 1u << s;
+          }
+        }
+        if (have == full) break;
+        __builtin_amdgcn_s_sleep(1);
+        if ((++spins & 255u) == 0u &&
+            (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
+          __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          s_fail = 1u;
+          break;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < nc) v[k] = (R)0 + Halves<R>::join(pay[k * 2], pay[k * 2 + 1]);
+    }
+    if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < nc) red[k][t] = v[k];
+  }
+  wg_sync(!sp.in_regs);
+  if (nc > 0) {
+    if (t < 128u) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < nc) red[k][t] += red[k][t + 128u];
+    }
+    wg_sync(!sp.in_regs);
+    if (t < 64u) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k >= nc) continue;
+        R x = red[k][t] + red[k][t + 64u];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+        if (t == 0) red[k][0] = x;
+      }
+    }
+    wg_sync(!sp.in_regs);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < nc) out[k] = red[k][0];
+  }
+  const bool ok = s_fail == 0u;
+  wg_sync(!sp.in_regs);
+  if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return ok;
+}
+
+// solve_init (fluid.wgsl:450-491) for one cell: the operator row, b, r = b - A p0 and z. Writes
+// diag, r, b, the neighbour mask and z; in register form also the tagged z (tag `ztag`).
+// Stores the block partials (r.z, r.r, b.b, water cells) through `prow` from lane 0.
+template <class R>
+__device__ __forceinline__ void init_cell(const Dev& d, uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk,
+                                          bool regs, uint32_t ztag, double* prow) {
   const natray_params* U = d.U;
-  const uint32_t mode = d.sc->precond;
-  const uint32_t n = n_blocks(d);
-  const uint32_t K = n ? (n + MAX_CHUNKS - 1u) / MAX_CHUNKS : 1u;
-  const uint32_t m = n ? (n + K - 1u) / K : 0u;
-  const uint32_t G = gridDim.x, g = blockIdx.x;
-  const uint32_t c0 = (uint32_t)((uint64_t)g * m / G), c1 = (uint32_t)((uint64_t)(g + 1) * m / G);
-  const uint32_t b0 = c0 * K, b1 = min(n, c1 * K);
   const float h = U->origin[3], dt = dt_of(d);
+  const uint32_t capc = d.capc;
+  const float* cP = d.cf + (size_t)NATRAY_F_P * capc;
+  const uint32_t cell = b * 64u + lane;
+  const bool water = cfr(d, NATRAY_F_TYPE, cell) == WATER;
+  float diag = 1.0f, r = 0.0f, bv = 0.0f;
+  uint32_t mask = 0;
+  if (water) {
+    float cs = U->gravity[3] * dt;
+    float kc = h * h / (cs * cs);
+    float div = 0.0f, off = 0.0f;
+    diag = kc;
+    bv = kc * cP[cell];
+#pragma unroll
+    for (uint32_t dim = 0; dim < 3; ++dim) {
+      const int ex = dim == 0, ey = dim == 1, ez = dim == 2;
+      const uint32_t up = nbr(lk, lx, ly, lz, ex, ey, ez), dn = nbr(lk, lx, ly, lz, -ex, -ey, -ez);
+      div += face_u(d, up, cell, dim) - face_u(d, cell, dn, dim);
+#pragma unroll
+      for (int sg = -1; sg <= 1; sg += 2) {
+        const uint32_t nb = sg < 0 ? dn : up;
+        const float nt = cell_type(d, nb);
+        if (nt == SOLID) continue;
+        diag += 1.0f;
+        if (nt == WATER) {
+          off -= cP[nb];
+          mask |= 1u << (dim * 2u + (sg > 0 ? 1u : 0u));
+        } else {
+          bv += pressure_of(d, nb);
+        }
+      }
+    }
+    div -= from_fp(accr(d, NATRAY_A_SRC, cell), NATRAY_FP_Q) / (h * h);
+    bv -= U->fluid[0] * h / dt * div;
+    r = bv - diag * cP[cell] - off;
+    d.cf[(size_t)NATRAY_F_DIAG * capc + cell] = diag;
+    d.cf[(size_t)NATRAY_F_R * capc + cell] = r;
+    d.bvec[cell] = bv;
+    mask |= 64u;
+  }
+  d.nbm[cell] = mask;
+  const float z = precondition(r, diag, water, lx, ly, lz, d.sc->precond);
+  if (water) {
+    d.cf[(size_t)NATRAY_F_Z * capc + cell] = z;
+    if (regs) store_word(&d.hz[cell], tag_word(ztag, __float_as_uint(z)));
+  }
+  const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r)), v2 = wave_tree<R>(prod<R>(bv, bv));
+  const uint32_t nw = (uint32_t)__popcll(__ballot(water));
+  if (lane == 0) {
+    prow[0] = (double)v0;
+    prow[1] = (double)v1;
+    prow[2] = (double)v2;
+    prow[3] = (double)nw;
+  }
+}
+
+// The solver scalars of a solve, kept by every workgroup alike.
+template <class R>
+struct SolveScalars {
+  R rz, alpha, beta, b_norm, r_norm;
+  uint32_t iters, iters_max, unconverged, water;
+};
+
+// solve_start (fluid.wgsl:493-506) from the init totals; returns whether to iterate.
+template <class R>
+__device__ __forceinline__ bool solve_start(const Dev& d, const R* tt, bool ok, SolveScalars<R>& s) {
+  const R tol = (R)d.U->solve[0];
+  s.rz = tt[0];
+  s.alpha = s.beta = (R)0;
+  s.b_norm = sq(tt[2]);
+  s.r_norm = sq(tt[1]);
+  s.iters = 0;
+  s.iters_max = d.st->iters_max;
+  s.unconverged = d.st->unconverged;
+  // The fourth component counts water cells (exact: at most 2^22 of them).
+  s.water = (uint32_t)tt[3];
+  return ok && s.rz > (R)0 && !converged<R>(tt[1], tol, s.b_norm) && min(d.U->limits[2], SOLVE_ITER_CAP) > 0u;
+}
+
+// solve_beta (fluid.wgsl:561-578) from the update totals; returns whether to iterate again.
+template <class R>
+__device__ __forceinline__ bool solve_beta(const Dev& d, const R* tt, SolveScalars<R>& s) {
+  const R rz_new = tt[0], rr_new = tt[1];
+  s.iters += 1u;
+  s.r_norm = sq(rr_new);
+  s.beta = s.rz > (R)0 ? rz_new / s.rz : (R)0;
+  s.rz = rz_new;
+  const bool done = converged<R>(rr_new, (R)d.U->solve[0], s.b_norm);
+  if (done || s.iters >= min(d.U->limits[2], SOLVE_ITER_CAP)) {
+    s.iters_max = s.iters > s.iters_max ? s.iters : s.iters_max;
+    s.unconverged += done ? 0u : 1u;
+    return false;
+  }
+  return true;
+}
+
+// End of a solve: workgroup 0 writes State, the solver scalars, the counters and the last tag.
+template <class R>
+__device__ __forceinline__ void solve_finish(const Dev& d, uint32_t n, uint32_t tag, const SolveScalars<R>& s) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  __hip_atomic_store(d.bar + 4, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  natray_state* st = d.st;
+  st->rz = (float)s.rz;
+  st->alpha = (float)s.alpha;
+  st->beta = (float)s.beta;
+  st->b_norm = (float)s.b_norm;
+  st->r_norm = (float)s.r_norm;
+  st->solve_iters = s.iters;
+  st->iters_max = s.iters_max;
+  st->unconverged = s.unconverged;
+  d.solver[0] = (double)s.rz;
+  d.solver[1] = (double)s.alpha;
+  d.solver[2] = (double)s.beta;
+  d.solver[3] = (double)s.b_norm;
+  d.solver[4] = (double)s.r_norm;
+  d.stats[0] += 1ull;
+  d.stats[1] += s.iters;
+  d.stats[2] += (unsigned long long)s.water * s.iters;
+  d.stats[3] += n;
+  d.stats[4] += s.water;
+  d.stats[5] += d.block_start[n];
+  d.stats[6] += st->n_blocks_prev;
+}
+
+// The memory form of the solve: the same iteration on the cf fields in memory, for launches
+// whose workgroups cannot keep their blocks in registers.
+template <bool MIXED>
+__device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t tag) {
+  using R = typename std::conditional<MIXED, double, float>::type;
+  const Dev& d = *dp;
+  const uint32_t mode = d.sc->precond;
+  const uint32_t b0 = sp.b0, b1 = sp.b1;
   const uint32_t capc = d.capc;
   float* const cP = d.cf + (size_t)NATRAY_F_P * capc;
   float* const cR = d.cf + (size_t)NATRAY_F_R * capc;
   float* const cZ = d.cf + (size_t)NATRAY_F_Z * capc;
   float* const cQ = d.cf + (size_t)NATRAY_F_Q * capc;
   float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
-  uint32_t water_here = 0;
-  // Barrier tokens of this launch: the epoch (moved on by workgroup 0 at the end) above a count.
-  unsigned long long token = __hip_atomic_load(&d.sync_gen[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 32;
-
-  // solve_init (fluid.wgsl:450-491)
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    const uint32_t cell = b * 64u + lane;
-    const bool water = cfr(d, NATRAY_F_TYPE, cell) == WATER;
-    float diag = 1.0f, r = 0.0f, bv = 0.0f;
-    uint32_t mask = 0;
-    if (water) {
-      float cs = U->gravity[3] * dt;
-      float kc = h * h / (cs * cs);
-      float div = 0.0f, off = 0.0f;
-      diag = kc;
-      bv = kc * cP[cell];
-#pragma unroll
-      for (uint32_t dim = 0; dim < 3; ++dim) {
-        const int ex = dim == 0, ey = dim == 1, ez = dim == 2;
-        const uint32_t up = nbr(lk, lx, ly, lz, ex, ey, ez), dn = nbr(lk, lx, ly, lz, -ex, -ey, -ez);
-        div += face_u(d, up, cell, dim) - face_u(d, cell, dn, dim);
-#pragma unroll
-        for (int sg = -1; sg <= 1; sg += 2) {
-          const uint32_t nb = sg < 0 ? dn : up;
-          const float nt = cell_type(d, nb);
-          if (nt == SOLID) continue;
-          diag += 1.0f;
-          if (nt == WATER) {
-            off -= cP[nb];
-            mask |= 1u << (dim * 2u + (sg > 0 ? 1u : 0u));
-          } else {
-            bv += pressure_of(d, nb);
-          }
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, false, 0u, d.part + b * 4u);
+  });
+  R tt[4];
+  SolveScalars<R> s;
+  bool go = solve_start<R>(d, tt, reduce<R>(d, sp, ++tag, 4, nullptr, tt), s);
+  const bool refresh_on = MIXED && d.sc->precision == NATRAY_CG_MIXED && d.sc->refresh_every > 0u;
+  while (go) {
+    // solve_q (fluid.wgsl:508-534): the new direction and q = A d; neighbours' directions are
+    // rebuilt from z and the old direction.
+    const uint32_t old = s.iters % 2u;
+    const float* cDo = d.cf + (size_t)(NATRAY_F_D + old) * capc;
+    float* cDn = d.cf + (size_t)(NATRAY_F_D + 1u - old) * capc;
+    const float fb = (float)s.beta;
+    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+      const uint32_t cell = b * 64u + lane;
+      const uint32_t mask = d.nbm[cell];
+      R v = (R)0;
+      if (mask & 64u) {
+        const float dn = cZ[cell] + fb * cDo[cell];
+        float q = cDiag[cell] * dn;
+        if (mask & 1u) { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 2u) { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 4u) { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 8u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 16u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; }
+        if (mask & 32u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; }
+        cDn[cell] = dn;
+        cQ[cell] = q;
+        v = prod<R>(dn, q);
+      }
+      v = wave_tree<R>(v);
+      if (lane == 0) d.part[b * 4u] = (double)v;
+    });
+    if (!reduce<R>(d, sp, ++tag, 1, nullptr, tt)) break;
+    // solve_alpha (fluid.wgsl:536-541)
+    s.alpha = tt[0] > (R)0 ? s.rz / tt[0] : (R)0;
+    const float fa = (float)s.alpha;
+    const bool refresh = refresh_on && (s.iters + 1u) % d.sc->refresh_every == 0u;
+    // solve_update (fluid.wgsl:543-559): p += alpha d, r -= alpha q, z = M^-1 r.
+    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t*) {
+      const uint32_t cell = b * 64u + lane;
+      const bool water = (d.nbm[cell] & 64u) != 0u;
+      float r = 0.0f, diag = 1.0f;
+      if (water) {
+        cP[cell] += fa * cDn[cell];
+        r = cR[cell] - fa * cQ[cell];
+        cR[cell] = r;
+        diag = cDiag[cell];
+      }
+      if (!refresh) {
+        const float z = precondition(r, diag, water, lx, ly, lz, mode);
+        if (water) cZ[cell] = z;
+        const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+        if (lane == 0) {
+          d.part[b * 4u + 0] = (double)v0;
+          d.part[b * 4u + 1] = (double)v1;
         }
       }
-      div -= from_fp(accr(d, NATRAY_A_SRC, cell), NATRAY_FP_Q) / (h * h);
-      bv -= U->fluid[0] * h / dt * div;
-      r = bv - diag * cP[cell] - off;
-      cDiag[cell] = diag;
-      cR[cell] = r;
-      d.bvec[cell] = bv;
-      mask |= 64u;
+    });
+    if (refresh) {
+      // True residual r = b - A p in double once every p is updated, then the preconditioner.
+      if (!reduce<R>(d, sp, ++tag, 0, nullptr, tt)) break;
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+        const uint32_t cell = b * 64u + lane;
+        const uint32_t mask = d.nbm[cell];
+        const bool water = (mask & 64u) != 0u;
+        float r = 0.0f, diag = 1.0f;
+        if (water) {
+          diag = cDiag[cell];
+          double ap = (double)diag * (double)cP[cell];
+          if (mask & 1u) ap -= (double)cP[nbr(lk, lx, ly, lz, -1, 0, 0)];
+          if (mask & 2u) ap -= (double)cP[nbr(lk, lx, ly, lz, 1, 0, 0)];
+          if (mask & 4u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, -1, 0)];
+          if (mask & 8u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 1, 0)];
+          if (mask & 16u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, -1)];
+          if (mask & 32u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, 1)];
+          r = (float)((double)d.bvec[cell] - ap);
+          cR[cell] = r;
+        }
+        const float z = precondition(r, diag, water, lx, ly, lz, mode);
+        if (water) cZ[cell] = z;
+        const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+        if (lane == 0) {
+          d.part[b * 4u + 0] = (double)v0;
+          d.part[b * 4u + 1] = (double)v1;
+        }
+      });
     }
-    d.nbm[cell] = mask;
-    const float z = precondition(r, diag, water, lx, ly, lz, mode);
-    if (water) cZ[cell] = z;
-    const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r)), v2 = wave_tree<R>(prod<R>(bv, bv));
-    const uint32_t nw = (uint32_t)__popcll(__ballot(water));
-    if (lane == 0) {
-      d.part[b * 4u + 0] = (double)v0;
-      d.part[b * 4u + 1] = (double)v1;
-      d.part[b * 4u + 2] = (double)v2;
-      water_here += nw;
-    }
-  });
-  {
-    // Water cells of this workgroup's blocks, for the work counters.
-    __shared__ uint32_t s_water[SOLVE_WAVES];
-    if (lane_id() == 0) s_water[threadIdx.x >> 6] = water_here;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t w = 0;
-      for (uint32_t k = 0; k < SOLVE_WAVES; ++k) w += s_water[k];
-      d.wpart[g] = w;
-    }
+    if (!reduce<R>(d, sp, ++tag, 2, nullptr, tt)) break;
+    go = solve_beta<R>(d, tt, s);
   }
-  fold_chunks<R>(d, 1, c0, c1, K, n, 3);
-  R tt[3];
-  bool ok = sync_reduce<R>(d, G, ++token, 1, m, 3, tt, red);
+  solve_finish<R>(d, sp.n, tag, s);
+}
 
-  // solve_start (fluid.wgsl:493-506)
-  const R tol = (R)U->solve[0];
-  R rz = tt[0], rr = tt[1], bb = tt[2];
-  R beta = (R)0, alpha = (R)0;
-  const R b_norm = sq(bb);
-  R r_norm = sq(rr);
-  uint32_t iters = 0, iters_max = d.st->iters_max, unconverged = d.st->unconverged;
-  bool go = ok && rz > (R)0 && !converged<R>(rr, tol, b_norm) && U->limits[2] > 0u;
+template <bool MIXED>
+__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
+  using R = typename std::conditional<MIXED, double, float>::type;
+  constexpr uint32_t RM = SOLVE_RMAX;
+  if (d.st->dt_ns == 0u) return;
+  // An earlier solve of this batch lost its barrier: the host reports it, nothing more runs.
+  if (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+  SolveSpan sp;
+  sp.n = n_blocks(d);
+  sp.K = sp.n ? (sp.n + MAX_CHUNKS - 1u) / MAX_CHUNKS : 1u;
+  sp.m = sp.n ? (sp.n + sp.K - 1u) / sp.K : 0u;
+  sp.G = gridDim.x;
+  sp.c0 = (uint32_t)((uint64_t)blockIdx.x * sp.m / sp.G);
+  sp.c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * sp.m / sp.G);
+  sp.b0 = sp.c0 * sp.K;
+  sp.b1 = min(sp.n, sp.c1 * sp.K);
+  sp.in_regs = (sp.m + sp.G - 1u) / sp.G * sp.K <= SOLVE_WAVES * RM;
+  uint32_t tag = __hip_atomic_load(d.bar + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!sp.in_regs) {
+    // Called with the device copy of Dev, which the callee reads with scalar loads.
+    solve_memory<MIXED>(d.self, sp, tag);
+    return;
+  }
+
+  // Register form. Wave wv holds blocks b0 + wv + SOLVE_WAVES r (r < RM), the blocks
+  // for_blocks gives it. Per block and lane: p, r, z, the direction d, q, diag, the neighbour
+  // mask, and for each axis the halo cell across the block face (when it is water) with its
+  // direction and p as this wave rebuilds them, and its tagged z as last loaded.
+  const uint32_t mode = d.sc->precond;
+  const uint32_t b0 = sp.b0, b1 = sp.b1;
+  const uint32_t capc = d.capc;
+  float* const cP = d.cf + (size_t)NATRAY_F_P * capc;
+  unsigned long long* const hz = d.hz;
+  __shared__ double lpart[SOLVE_WAVES * SOLVE_RMAX][4];
+  for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, true, tag + 1u, lpart[b - b0]);
+  });
+  R tt[4];
+  SolveScalars<R> s;
+  bool go = solve_start<R>(d, tt, reduce<R>(d, sp, ++tag, 4, lpart, tt), s);
   const bool refresh_on = MIXED && d.sc->precision == NATRAY_CG_MIXED && d.sc->refresh_every > 0u;
 
-  // A workgroup whose blocks fit keeps them in registers for the whole solve, wave w owning
-  // blocks b0 + w + SOLVE_WAVES r (r < SOLVE_RMAX), the same blocks for_blocks gave it above:
-  // in-block neighbours come from lane shuffles and only halo values from other blocks, and
-  // the directions and z the other blocks read, go through memory. Larger workgroups run the
-  // same iteration on the cf fields in memory. Both evaluate the same expressions in the same
-  // order, and keep the same grid barriers.
   __shared__ uint32_t lkr[SOLVE_WAVES][SOLVE_RMAX][27];
-  const bool in_regs = b1 - b0 <= SOLVE_WAVES * SOLVE_RMAX;
   const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
   const int lx = (int)(ln >> 4), ly = (int)((ln >> 2) & 3u), lz = (int)(ln & 3u);
   const int sxm = lx > 0 ? (int)ln - 16 : (int)ln, sxp = lx < 3 ? (int)ln + 16 : (int)ln;
   const int sym = ly > 0 ? (int)ln - 4 : (int)ln, syp = ly < 3 ? (int)ln + 4 : (int)ln;
   const int szm = lz > 0 ? (int)ln - 1 : (int)ln, szp = lz < 3 ? (int)ln + 1 : (int)ln;
-  float rP[SOLVE_RMAX], rR[SOLVE_RMAX], rZ[SOLVE_RMAX], rDo[SOLVE_RMAX], rDn[SOLVE_RMAX], rQ[SOLVE_RMAX], rDg[SOLVE_RMAX];
-  uint32_t rM[SOLVE_RMAX];
+  // Mask bit of the neighbour across the block face on each axis, if this lane is on one.
+  const uint32_t face_bit[3] = {lx == 0 ? 1u : (lx == 3 ? 2u : 0u), ly == 0 ? 4u : (ly == 3 ? 8u : 0u),
+                                lz == 0 ? 16u : (lz == 3 ? 32u : 0u)};
+  float rP[RM], rR[RM], rZ[RM], rD[RM], rQ[RM], rDg[RM];
+  uint32_t rM[RM];
+  float hD[RM][3], hPr[RM][3];
+  unsigned long long hW[RM][3];
+  auto halo_cell = [&](uint32_t r, int a) -> uint32_t {
+    const int sg = (face_bit[a] & 0x2Au) ? 1 : -1;
+    return nbr(lkr[wv][r], lx, ly, lz, a == 0 ? sg : 0, a == 1 ? sg : 0, a == 2 ? sg : 0);
+  };
+  auto has_halo = [&](uint32_t r, int a) -> bool { return (rM[r] & face_bit[a]) != 0u; };
 #pragma unroll
-  for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
-    rP[r] = rR[r] = rZ[r] = rDo[r] = rDn[r] = rQ[r] = 0.0f;
+  for (uint32_t r = 0; r < RM; ++r) {
+    const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+    if (b < b1 && ln < 27) lkr[wv][r][ln] = d.block_nb[b * 27u + ln];
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < RM; ++r) {
+    rP[r] = rR[r] = rZ[r] = rD[r] = rQ[r] = 0.0f;
     rDg[r] = 1.0f;
     rM[r] = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      hD[r][a] = hPr[r][a] = 0.0f;
+      hW[r][a] = 0ull;
+    }
     const uint32_t b = b0 + wv + SOLVE_WAVES * r;
-    if (in_regs && b < b1) {
-      if (ln < 27) lkr[wv][r][ln] = d.block_nb[b * 27u + ln];
+    if (b < b1) {
       const uint32_t cell = b * 64u + ln;
       rM[r] = d.nbm[cell];
       rP[r] = cP[cell];
-      rR[r] = cR[cell];
-      rZ[r] = cZ[cell];
-      rDg[r] = cDiag[cell];
-      rDo[r] = d.cf[(size_t)NATRAY_F_D * capc + cell];
+      rR[r] = d.cf[(size_t)NATRAY_F_R * capc + cell];
+      rZ[r] = d.cf[(size_t)NATRAY_F_Z * capc + cell];
+      rDg[r] = d.cf[(size_t)NATRAY_F_DIAG * capc + cell];
+      rD[r] = d.cf[(size_t)NATRAY_F_D * capc + cell];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (!has_halo(r, a)) continue;
+        const uint32_t o = halo_cell(r, a);
+        hD[r][a] = d.cf[(size_t)NATRAY_F_D * capc + o];
+        if (refresh_on) hPr[r][a] = cP[o];
+        hW[r][a] = load_word(&hz[o]);
+      }
     }
   }
 
+  bool ok = true;
   while (go) {
-    // solve_q (fluid.wgsl:508-534): the new direction and q = A d; neighbours' directions are
-    // rebuilt from z and the old direction.
-    const uint32_t old = iters % 2u;
-    const float* cDo = d.cf + (size_t)(NATRAY_F_D + old) * capc;
-    float* cDn = d.cf + (size_t)(NATRAY_F_D + 1u - old) * capc;
-    const float fb = (float)beta;
-    if (in_regs) {
+    // solve_q (fluid.wgsl:508-534): the new direction and q = A d.
+    float* cDn = d.cf + (size_t)(NATRAY_F_D + 1u - s.iters % 2u) * capc;
+    const float fb = (float)s.beta;
 #pragma unroll
-      for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
-        const uint32_t b = b0 + wv + SOLVE_WAVES * r;
-        if (b >= b1) continue;
-        const uint32_t cell = b * 64u + ln, mask = rM[r];
-        const uint32_t* lk = lkr[wv][r];
-        const float dn = rZ[r] + fb * rDo[r];
-        const float vxm = __shfl(dn, sxm), vxp = __shfl(dn, sxp), vym = __shfl(dn, sym), vyp = __shfl(dn, syp),
-                    vzm = __shfl(dn, szm), vzp = __shfl(dn, szp);
-        R v = (R)0;
-        if (mask & 64u) {
-          float q = rDg[r] * dn;
-          if (mask & 1u) { if (lx > 0) q -= vxm; else { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; } }
-          if (mask & 2u) { if (lx < 3) q -= vxp; else { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; } }
-          if (mask & 4u) { if (ly > 0) q -= vym; else { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; } }
-          if (mask & 8u) { if (ly < 3) q -= vyp; else { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; } }
-          if (mask & 16u) { if (lz > 0) q -= vzm; else { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; } }
-          if (mask & 32u) { if (lz < 3) q -= vzp; else { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; } }
-          cDn[cell] = dn;
-          rDn[r] = dn;
-          rQ[r] = q;
-          v = prod<R>(dn, q);
-        }
-        v = wave_tree<R>(v);
-        if (ln == 0) d.part[b * 4u] = (double)v;
-      }
-    } else {
-      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-        const uint32_t cell = b * 64u + lane;
-        const uint32_t mask = d.nbm[cell];
-        R v = (R)0;
-        if (mask & 64u) {
-          const float dn = cZ[cell] + fb * cDo[cell];
-          float q = cDiag[cell] * dn;
-          if (mask & 1u) { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
-          if (mask & 2u) { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
-          if (mask & 4u) { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; }
-          if (mask & 8u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; }
-          if (mask & 16u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; }
-          if (mask & 32u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; }
-          cDn[cell] = dn;
-          cQ[cell] = q;
-          v = prod<R>(dn, q);
-        }
-        v = wave_tree<R>(v);
-        if (lane == 0) d.part[b * 4u] = (double)v;
-      });
-    }
-    fold_chunks<R>(d, 0, c0, c1, K, n, 1);
-    if (!sync_reduce<R>(d, G, ++token, 0, m, 1, tt, red)) break;
-    // solve_alpha (fluid.wgsl:536-541)
-    const R dq = tt[0];
-    alpha = dq > (R)0 ? rz / dq : (R)0;
-    const float fa = (float)alpha;
-    const bool refresh = refresh_on && (iters + 1u) % d.sc->refresh_every == 0u;
-    // solve_update (fluid.wgsl:543-559): p += alpha d, r -= alpha q, z = M^-1 r.
-    if (in_regs) {
-#pragma unroll
-      for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
-        const uint32_t b = b0 + wv + SOLVE_WAVES * r;
-        if (b >= b1) continue;
-        const uint32_t cell = b * 64u + ln;
-        const bool water = (rM[r] & 64u) != 0u;
-        float rr_ = 0.0f, diag = 1.0f;
-        if (water) {
-          rP[r] += fa * rDn[r];
-          rR[r] = rR[r] - fa * rQ[r];
-          rr_ = rR[r];
-          diag = rDg[r];
-          if (refresh) cP[cell] = rP[r];
-        }
-        rDo[r] = rDn[r];
-        if (!refresh) {
-          const float z = precondition(rr_, diag, water, lx, ly, lz, mode);
-          if (water) {
-            rZ[r] = z;
-            cZ[cell] = z;
-          }
-          const R v0 = wave_tree<R>(prod<R>(rr_, z)), v1 = wave_tree<R>(prod<R>(rr_, rr_));
-          if (ln == 0) {
-            d.part[b * 4u + 0] = (double)v0;
-            d.part[b * 4u + 1] = (double)v1;
-          }
-        }
-      }
-    } else {
-      auto update = [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t*) {
-        const uint32_t cell = b * 64u + lane;
-        const bool water = (d.nbm[cell] & 64u) != 0u;
-        float r = 0.0f, diag = 1.0f;
-        if (water) {
-          cP[cell] += fa * cDn[cell];
-          r = cR[cell] - fa * cQ[cell];
-          cR[cell] = r;
-          diag = cDiag[cell];
-        }
-        if (!refresh) {
-          const float z = precondition(r, diag, water, lx, ly, lz, mode);
-          if (water) cZ[cell] = z;
-          const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
-          if (lane == 0) {
-            d.part[b * 4u + 0] = (double)v0;
-            d.part[b * 4u + 1] = (double)v1;
-          }
-        }
-      };
-      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, update);
-    }
-    if (refresh) {
-      // True residual r = b - A p in double once every p is updated, then the preconditioner.
-      if (!sync_reduce<R>(d, G, ++token, 0, m, 0, tt, red)) break;
-      if (in_regs) {
-#pragma unroll
-        for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
-          const uint32_t b = b0 + wv + SOLVE_WAVES * r;
-          if (b >= b1) continue;
-          const uint32_t cell = b * 64u + ln, mask = rM[r];
-          const uint32_t* lk = lkr[wv][r];
-          const bool water = (mask & 64u) != 0u;
-          const float p = rP[r];
-          const float pxm = __shfl(p, sxm), pxp = __shfl(p, sxp), pym = __shfl(p, sym), pyp = __shfl(p, syp),
-                      pzm = __shfl(p, szm), pzp = __shfl(p, szp);
-          float rr_ = 0.0f, diag = 1.0f;
-          if (water) {
-            diag = rDg[r];
-            double ap = (double)diag * (double)p;
-            if (mask & 1u) ap -= (double)(lx > 0 ? pxm : cP[nbr(lk, lx, ly, lz, -1, 0, 0)]);
-            if (mask & 2u) ap -= (double)(lx < 3 ? pxp : cP[nbr(lk, lx, ly, lz, 1, 0, 0)]);
-            if (mask & 4u) ap -= (double)(ly > 0 ? pym : cP[nbr(lk, lx, ly, lz, 0, -1, 0)]);
-            if (mask & 8u) ap -= (double)(ly < 3 ? pyp : cP[nbr(lk, lx, ly, lz, 0, 1, 0)]);
-            if (mask & 16u) ap -= (double)(lz > 0 ? pzm : cP[nbr(lk, lx, ly, lz, 0, 0, -1)]);
-            if (mask & 32u) ap -= (double)(lz < 3 ? pzp : cP[nbr(lk, lx, ly, lz, 0, 0, 1)]);
-            rr_ = (float)((double)d.bvec[cell] - ap);
-            rR[r] = rr_;
-          }
-          const float z = precondition(rr_, diag, water, lx, ly, lz, mode);
-          if (water) {
-            rZ[r] = z;
-            cZ[cell] = z;
-          }
-          const R v0 = wave_tree<R>(prod<R>(rr_, z)), v1 = wave_tree<R>(prod<R>(rr_, rr_));
-          if (ln == 0) {
-            d.part[b * 4u + 0] = (double)v0;
-            d.part[b * 4u + 1] = (double)v1;
-          }
-        }
-      } else {
-        for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-          const uint32_t cell = b * 64u + lane;
-          const uint32_t mask = d.nbm[cell];
-          const bool water = (mask & 64u) != 0u;
-          float r = 0.0f, diag = 1.0f;
-          if (water) {
-            diag = cDiag[cell];
-            double ap = (double)diag * (double)cP[cell];
-            if (mask & 1u) ap -= (double)cP[nbr(lk, lx, ly, lz, -1, 0, 0)];
-            if (mask & 2u) ap -= (double)cP[nbr(lk, lx, ly, lz, 1, 0, 0)];
-            if (mask & 4u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, -1, 0)];
-            if (mask & 8u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 1, 0)];
-            if (mask & 16u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, -1)];
-            if (mask & 32u) ap -= (double)cP[nbr(lk, lx, ly, lz, 0, 0, 1)];
-            r = (float)((double)d.bvec[cell] - ap);
-            cR[cell] = r;
-          }
-          const float z = precondition(r, diag, water, lx, ly, lz, mode);
-          if (water) cZ[cell] = z;
-          const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
-          if (lane == 0) {
-            d.part[b * 4u + 0] = (double)v0;
-            d.part[b * 4u + 1] = (double)v1;
-          }
-        });
-      }
-    }
-    fold_chunks<R>(d, 1, c0, c1, K, n, 2);
-    if (!sync_reduce<R>(d, G, ++token, 1, m, 2, tt, red)) break;
-    // solve_beta (fluid.wgsl:561-578)
-    const R rz_new = tt[0], rr_new = tt[1];
-    iters += 1u;
-    r_norm = sq(rr_new);
-    beta = rz > (R)0 ? rz_new / rz : (R)0;
-    rz = rz_new;
-    const bool done = converged<R>(rr_new, tol, b_norm);
-    if (done || iters >= U->limits[2]) {
-      iters_max = iters > iters_max ? iters : iters_max;
-      unconverged += done ? 0u : 1u;
-      go = false;
-    }
-  }
-  if (in_regs) {
-    // The fields the other kernels and the host read: p, r and q of this wave's water cells.
-#pragma unroll
-    for (uint32_t r = 0; r < SOLVE_RMAX; ++r) {
+    for (uint32_t r = 0; r < RM; ++r) {
       const uint32_t b = b0 + wv + SOLVE_WAVES * r;
-      if (b < b1 && (rM[r] & 64u)) {
-        const uint32_t cell = b * 64u + ln;
-        cP[cell] = rP[r];
-        cR[cell] = rR[r];
-        if (iters > 0u) cQ[cell] = rQ[r];
+      if (b >= b1) continue;
+      const uint32_t cell = b * 64u + ln, mask = rM[r];
+      // Halo directions: z across the face (tagged with the last reduction's tag), rebuilt
+      // as the owner does, z + beta d_old.
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (!has_halo(r, a)) continue;
+        uint32_t spins = 0;
+        while ((uint32_t)(hW[r][a] >> 32) != tag) {
+          __builtin_amdgcn_s_sleep(1);
+          hW[r][a] = load_word(&hz[halo_cell(r, a)]);
+          if ((++spins & 255u) == 0u &&
+              (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
+            __hip_atomic_store(d.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = false;
+            break;
+          }
+        }
+        hD[r][a] = __uint_as_float((uint32_t)hW[r][a]) + fb * hD[r][a];
+      }
+      const float dn = rZ[r] + fb * rD[r];
+      const float vxm = __shfl(dn, sxm), vxp = __shfl(dn, sxp), vym = __shfl(dn, sym), vyp = __shfl(dn, syp),
+                  vzm = __shfl(dn, szm), vzp = __shfl(dn, szp);
+      R v = (R)0;
+      if (mask & 64u) {
+        float q = rDg[r] * dn;
+        if (mask & 1u) q -= lx > 0 ? vxm : hD[r][0];
+        if (mask & 2u) q -= lx < 3 ? vxp : hD[r][0];
+        if (mask & 4u) q -= ly > 0 ? vym : hD[r][1];
+        if (mask & 8u) q -= ly < 3 ? vyp : hD[r][1];
+        if (mask & 16u) q -= lz > 0 ? vzm : hD[r][2];
+        if (mask & 32u) q -= lz < 3 ? vzp : hD[r][2];
+        cDn[cell] = dn;
+        rD[r] = dn;
+        rQ[r] = q;
+        v = prod<R>(dn, q);
+      }
+      v = wave_tree<R>(v);
+      if (ln == 0) lpart[b - b0][0] = (double)v;
+    }
+    if (!reduce<R>(d, sp, ++tag, 1, lpart, tt) || !ok) break;
+    // solve_alpha (fluid.wgsl:536-541)
+    s.alpha = tt[0] > (R)0 ? s.rz / tt[0] : (R)0;
+    const float fa = (float)s.alpha;
+    const bool refresh = refresh_on && (s.iters + 1u) % d.sc->refresh_every == 0u;
+    // solve_update (fluid.wgsl:543-559): p += alpha d, r -= alpha q, z = M^-1 r; with the
+    // refresh, r = b - A p in double instead, p of the halo rebuilt as its owner updates it.
+#pragma unroll
+    for (uint32_t r = 0; r < RM; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b >= b1) continue;
+      const uint32_t cell = b * 64u + ln, mask = rM[r];
+      const bool water = (mask & 64u) != 0u;
+      float rr = 0.0f, diag = 1.0f;
+      if (water) {
+        rP[r] += fa * rD[r];
+        rR[r] = rR[r] - fa * rQ[r];
+        rr = rR[r];
+        diag = rDg[r];
+      }
+      if (refresh_on) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          if (has_halo(r, a)) hPr[r][a] += fa * hD[r][a];
+      }
+      if (refresh) {
+        const float p = rP[r];
+        const float pxm = __shfl(p, sxm), pxp = __shfl(p, sxp), pym = __shfl(p, sym), pyp = __shfl(p, syp),
+                    pzm = __shfl(p, szm), pzp = __shfl(p, szp);
+        if (water) {
+          double ap = (double)diag * (double)p;
+          if (mask & 1u) ap -= (double)(lx > 0 ? pxm : hPr[r][0]);
+          if (mask & 2u) ap -= (double)(lx < 3 ? pxp : hPr[r][0]);
+          if (mask & 4u) ap -= (double)(ly > 0 ? pym : hPr[r][1]);
+          if (mask & 8u) ap -= (double)(ly < 3 ? pyp : hPr[r][1]);
+          if (mask & 16u) ap -= (double)(lz > 0 ? pzm : hPr[r][2]);
+          if (mask & 32u) ap -= (double)(lz < 3 ? pzp : hPr[r][2]);
+          rr = (float)((double)d.bvec[cell] - ap);
+          rR[r] = rr;
+        }
+      }
+      const float z = precondition(rr, diag, water, lx, ly, lz, mode);
+      if (water) {
+        rZ[r] = z;
+        store_word(&hz[cell], tag_word(tag + 1u, __float_as_uint(z)));
+      }
+      const R v0 = wave_tree<R>(prod<R>(rr, z)), v1 = wave_tree<R>(prod<R>(rr, rr));
+      if (ln == 0) {
+        lpart[b - b0][0] = (double)v0;
+        lpart[b - b0][1] = (double)v1;
       }
     }
+    // Halo z for the next solve_q, loaded while the reduction below completes.
+#pragma unroll
+    for (uint32_t r = 0; r < RM; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b >= b1) continue;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (has_halo(r, a)) hW[r][a] = load_word(&hz[halo_cell(r, a)]);
+    }
+    if (!reduce<R>(d, sp, ++tag, 2, lpart, tt)) break;
+    go = solve_beta<R>(d, tt, s);
   }
-  if (g == 0 && threadIdx.x == 0) {
-    __hip_atomic_store(&d.sync_gen[0], (token >> 32) + 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    natray_state* st = d.st;
-    st->rz = (float)rz;
-    st->alpha = (float)alpha;
-    st->beta = (float)beta;
-    st->b_norm = (float)b_norm;
-    st->r_norm = (float)r_norm;
-    st->solve_iters = iters;
-    st->iters_max = iters_max;
-    st->unconverged = unconverged;
-    d.solver[0] = (double)rz;
-    d.solver[1] = (double)alpha;
-    d.solver[2] = (double)beta;
-    d.solver[3] = (double)b_norm;
-    d.solver[4] = (double)r_norm;
-    uint32_t w = 0;
-    for (uint32_t k = 0; k < G; ++k) w += d.wpart[k];
-    d.stats[0] += 1ull;
-    d.stats[1] += iters;
-    d.stats[2] += (unsigned long long)w * iters;
-    d.stats[3] += n;
-    d.stats[4] += w;
-    d.stats[5] += d.block_start[n];
-    d.stats[6] += st->n_blocks_prev;
+  // The fields the other kernels and the host read: p, r, z and q of this wave's water cells
+  // (solve_q stored the directions).
+#pragma unroll
+  for (uint32_t r = 0; r < RM; ++r) {
+    const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+    if (b < b1 && (rM[r] & 64u)) {
+      const uint32_t cell = b * 64u + ln;
+      cP[cell] = rP[r];
+      d.cf[(size_t)NATRAY_F_R * capc + cell] = rR[r];
+      d.cf[(size_t)NATRAY_F_Z * capc + cell] = rZ[r];
+      if (s.iters > 0u) d.cf[(size_t)NATRAY_F_Q * capc + cell] = rQ[r];
+    }
   }
+  solve_finish<R>(d, sp.n, tag, s);
 }
 
 // ---- project, extend_1, extend_2 (fluid.wgsl:580-674) ------------------------------------------------

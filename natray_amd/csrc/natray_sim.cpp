@@ -52,6 +52,9 @@ struct natray_sim {
   int num_cus = 0;
   int grid_cells = 0, grid_blocks = 0, grid_waves = 0, solve_grid = 0;
   int mixed = 0;
+  // Upper bound of the reduction tags the solves enqueued since the tagged words were last
+  // cleared (see clear_tags).
+  uint64_t tags_used = 0;
   std::vector<void*> allocs;
   // Pinned staging for the small host <-> device copies of a batch.
   natray_params* h_params = nullptr;
@@ -114,15 +117,29 @@ int harvest_events(natray_sim* s) {
   return NATRAY_OK;
 }
 
-// Workgroups of the cooperative solve: about four block chunks each (at most MAX_CHUNKS
-// chunks, see the solve kernel), at most one per CU, sized from the block count of the last
-// State read back. Any count is correct; fewer make the grid barrier cheaper.
+// Workgroups of the cooperative solve: one per CU. The block count of a step is only known
+// on the device, so the launch does not follow it: with MAX_CHUNKS <= CUs every workgroup
+// holds at most one chunk and the solve keeps its blocks in registers up to 32 blocks per
+// chunk; a workgroup left without a chunk only takes part in the reductions. Any count gives
+// the same result.
 int solve_grid_for(const natray_sim* s) {
   if (s->sc.solve_groups) return (int)std::min<uint32_t>(s->sc.solve_groups, (uint32_t)s->solve_grid);
-  uint32_t n = std::min(s->last.n_blocks, s->U.bdims[3]);
-  if (n == 0) return s->solve_grid;
-  const uint32_t K = (n + natray::MAX_CHUNKS - 1) / natray::MAX_CHUNKS, m = (n + K - 1) / K;
-  return (int)std::max<uint32_t>(1, std::min<uint32_t>((m + 3) / 4, (uint32_t)s->solve_grid));
+  return s->solve_grid;
+}
+
+// Zeroes every tagged word of the solve and its tag counter, so tags start again from 1.
+int clear_tags(natray_sim* s) {
+  HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, s->stream));
+  HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 8, s->stream));
+  HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, s->stream));
+  s->tags_used = 0;
+  return NATRAY_OK;
+}
+
+// A solve launch uses at most one tag per reduction: one before the iterations and up to
+// three per iteration (the iteration cap is min(limits.z, 2^24) on the device).
+uint64_t solve_tag_bound(const natray_sim* s) {
+  return 3ull * std::min<uint64_t>(s->U.limits[2], 1ull << 24) + 4ull;
 }
 
 int enqueue_phase(natray_sim* s, int phase) {
@@ -140,7 +157,17 @@ int enqueue_phase(natray_sim* s, int phase) {
     case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(D, st, s->grid_cells)); break;
     case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(D, st, s->grid_blocks)); break;
     case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(D, st, s->grid_waves)); break;
-    case NATRAY_PHASE_SOLVE: LAUNCHCHK(natray::launch_solve(D, st, solve_grid_for(s), s->mixed)); break;
+    case NATRAY_PHASE_SOLVE: {
+      // Tags are 32-bit: clear the tagged words well before a tag could repeat.
+      const uint64_t bound = solve_tag_bound(s);
+      if (s->tags_used + bound >= (1ull << 31)) {
+        int e = clear_tags(s);
+        if (e) return e;
+      }
+      s->tags_used += bound;
+      LAUNCHCHK(natray::launch_solve(D, st, solve_grid_for(s), s->mixed));
+      break;
+    }
     case NATRAY_PHASE_PROJECT: LAUNCHCHK(natray::launch_project(D, st, s->grid_waves)); break;
     case NATRAY_PHASE_EXTEND: LAUNCHCHK(natray::launch_extend(D, st, s->grid_waves)); break;
     case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(D, st, s->grid_blocks)); break;
@@ -167,9 +194,10 @@ int sync_state(natray_sim* s) {
     if (e) return e;
   }
   if (s->h_words[2] != 0u) {
+    // Workgroups may have stopped at different reductions: start the tags afresh.
     HIPCHK(hipMemsetAsync(s->D.bar, 0, 1024 * sizeof(uint32_t), s->stream));
-    HIPCHK(hipMemsetAsync(s->D.sync_flags, 0, 1024 * 8, s->stream));
-    HIPCHK(hipMemsetAsync(s->D.sync_gen, 0, 2 * 8, s->stream));
+    int e = clear_tags(s);
+    if (e) return e;
     HIPCHK(hipStreamSynchronize(s->stream));
     return fail(NATRAY_E_SOLVER, "the pressure solve lost its grid barrier (not all workgroups were resident)");
   }
@@ -278,17 +306,17 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &D.bvec, capc));
   TRY(dalloc(s, &D.nbm, capc));
   TRY(dalloc(s, &D.part, (size_t)D.cap * 4));
-  TRY(dalloc(s, &D.chunk, (size_t)natray::MAX_CHUNKS * 4 * 2));
+  TRY(dalloc(s, &D.rwords, natray::RED_WORDS));
+  TRY(dalloc(s, &D.hz, capc));
   TRY(dalloc(s, &D.tile_sum, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.tile_off, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.bar, 1024));
-  TRY(dalloc(s, &D.sync_flags, 1024));
-  TRY(dalloc(s, &D.sync_gen, 2));
-  TRY(dalloc(s, &D.bcast, 4));
-  TRY(dalloc(s, &D.wpart, 1024));
   TRY(dalloc(s, &D.vmax_part, (size_t)D.cap));
   TRY(dalloc(s, &D.solver, 8));
   TRY(dalloc(s, &D.stats, 8));
+  natray::Dev* dself = nullptr;
+  TRY(dalloc(s, &dself, 1));
+  D.self = dself;
 
   HIPTRY(hipMemsetAsync(D.block_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.slot_count, 0, (size_t)D.Spad * 4, s->stream));
@@ -300,18 +328,16 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   HIPTRY(hipMemsetAsync(D.bvec, 0, capc * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.nbm, 0, capc * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.part, 0, (size_t)D.cap * 4 * 8, s->stream));
-  HIPTRY(hipMemsetAsync(D.chunk, 0, (size_t)natray::MAX_CHUNKS * 4 * 8 * 2, s->stream));
   HIPTRY(hipMemsetAsync(D.bar, 0, 1024 * 4, s->stream));
-  HIPTRY(hipMemsetAsync(D.sync_flags, 0, 1024 * 8, s->stream));
-  HIPTRY(hipMemsetAsync(D.sync_gen, 0, 2 * 8, s->stream));
-  HIPTRY(hipMemsetAsync(D.bcast, 0, 4 * 8, s->stream));
-  HIPTRY(hipMemsetAsync(D.wpart, 0, 1024 * 4, s->stream));
+  HIPTRY(hipMemsetAsync(D.rwords, 0, natray::RED_WORDS * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.hz, 0, capc * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.vmax_part, 0, (size_t)D.cap * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.solver, 0, 8 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.stats, 0, 8 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.sorted, 0, ((size_t)n + 1) * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.rank, 0, ((size_t)n + 1) * 4, s->stream));
   if (n) HIPTRY(hipMemcpyAsync(D.P, particles, (size_t)n * 80, hipMemcpyHostToDevice, s->stream));
+  HIPTRY(hipMemcpyAsync(dself, &D, sizeof(natray::Dev), hipMemcpyHostToDevice, s->stream));
   HIPTRY(hipMemcpyAsync(dSc, &s->sc, sizeof(natray_scene), hipMemcpyHostToDevice, s->stream));
 
   natray_state st0{};

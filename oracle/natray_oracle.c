@@ -13,9 +13,9 @@
  *   - ranks inside a block: particle index order (the reference's atomicAdd order is
  *     nondeterministic; nothing downstream depends on it).
  *   - solver reductions over blocks: per-block tree as block_sum (fluid.wgsl:414-422), then
- *     blocks are folded left to right in fixed chunks of K = ceil(n/1024), then the chunk
- *     sums go through the reference's total() (fluid.wgsl:426-439). For n <= 1024 this is
- *     exactly the reference's total() over blocks in slot order.
+ *     blocks are folded left to right in fixed chunks of K = ceil(n/256) (at most 256
+ *     chunks), then the chunk sums go through the reference's total() (fluid.wgsl:426-439).
+ *     For n <= 256 this is exactly the reference's total() over blocks in slot order.
  *   - out-of-scope hooks are stubbed as the reference behaves with no membrane, air or
  *     bullet: GAS pressure 0, acc fields 9..18 zero, carve/break_up/ground_touch no-ops
  *     (death only clears vel.w).
@@ -524,12 +524,13 @@ static double tree64(double v[64], int dbl) {
   return (double)r[0];
 }
 
-/* Sum of component k of the n block partials: fold in chunks of K = ceil(n/1024), then the
- * reference's total() (fluid.wgsl:426-439) over the chunk sums. */
+/* Sum of component k of the n block partials: fold in chunks of K = ceil(n/CHUNKS_MAX),
+ * then the reference's total() (fluid.wgsl:426-439) over the chunk sums. */
+#define CHUNKS_MAX 256u
 static double total(const osim* s, uint32_t n, int k, int dbl) {
   if (n == 0) return 0.0;
-  uint32_t K = (n + 1023u) / 1024u, m = (n + K - 1u) / K;
-  double chunk[1024];
+  uint32_t K = (n + CHUNKS_MAX - 1u) / CHUNKS_MAX, m = (n + K - 1u) / K;
+  double chunk[CHUNKS_MAX];
   double red[256];
   if (dbl) {
     for (uint32_t j = 0; j < m; ++j) {
@@ -546,7 +547,7 @@ static double total(const osim* s, uint32_t n, int k, int dbl) {
       for (uint32_t li = 0; li < off; ++li) red[li] += red[li + off];
     return red[0];
   }
-  REAL rc[1024], rr[256];
+  REAL rc[CHUNKS_MAX], rr[256];
   for (uint32_t j = 0; j < m; ++j) {
     REAL acc = 0;
     for (uint32_t b = j * K; b < umin(n, j * K + K); ++b) acc += (REAL)s->part[b * 4 + k];
