@@ -17,6 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnatray.so")
 DRIVE = os.path.join(HERE, "natray_drive")
+COPY = os.path.join(HERE, "natray_copy")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NATRAY_ARCH", "gfx950")
@@ -67,6 +68,12 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(DRIVE + ".tmp", DRIVE)
+    # The plain copy that calibrates the HBM traffic counters (natray_copy.hip).
+    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "natray_copy.hip"), "-o", COPY + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(COPY + ".tmp", COPY)
     return LIB
 
 
