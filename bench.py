@@ -73,6 +73,51 @@ def make_scene(args):
     return s, cfg
 
 
+def timed_run(sim, warmup: int, steps: int, iters: int):
+    """Warmup steps, then `steps` timed steps with per-phase events; wall time, counter deltas,
+    phase times and the final State."""
+    if warmup:
+        sim.run(warmup, iters=iters)
+    sim.set_profiling(True)
+    c0 = sim.counters()
+    t0 = time.perf_counter()
+    st = sim.run(steps, iters=iters)  # returns after the stream is idle
+    t1 = time.perf_counter()
+    c1 = sim.counters()
+    times = sim.phase_times()
+    sim.set_profiling(False)
+    return t1 - t0, {k: c1[k] - c0[k] for k in c0}, times, st
+
+
+def compare_precision(scene, args, plain: dict) -> dict:
+    """The same workload from the same initial state with the other CG precision (FP32 SpMV
+    and axpy, FP64 dots, norms and convergence test, FP64 true residual every 20 iterations),
+    at the same tolerance in the reference's convergence form: iterations and solve time per
+    step against the plain FP32 run."""
+    from natray_amd import Sim, abi
+
+    sc = scene.scene
+    keep = (sc.precision, sc.refresh_every)
+    sc.precision, sc.refresh_every = abi.CG_MIXED, 20
+    try:
+        sim = Sim.from_scene(scene)
+        wall, c, times, st = timed_run(sim, args.warmup, args.steps, args.iters)
+        sim.close()
+    finally:
+        sc.precision, sc.refresh_every = keep
+    steps = max(c["steps"], 1)
+    mixed = {"cg_iters_per_step": c["cg_iters"] / steps, "solve_ms_per_step": times["solve"][0] / steps,
+             "ms_per_step": wall * 1e3 / max(args.steps, 1), "unconverged_steps": st.unconverged}
+    return {
+        "tol": scene.params.solve[0],
+        "plain_fp32": plain,
+        "mixed_f32_f64": mixed,
+        "solve_speedup_mixed_over_plain": plain["solve_ms_per_step"] / mixed["solve_ms_per_step"]
+        if mixed["solve_ms_per_step"] > 0 else None,
+        "warm_start": "particle-carried pressure (the reference's p0) in both runs",
+    }
+
+
 def cpu_baseline(scene, iters: int, budget_s: float, max_steps: int) -> dict:
     """The FP32 oracle (OpenMP over the host's cores) on the same workload from the same
     initial state: one untimed step, then steps until `budget_s` of wall time or max_steps."""
@@ -102,6 +147,7 @@ def main() -> int:
     ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32")
     ap.add_argument("--iters", type=int, default=1000, help="CG iteration cap per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-compare", action="store_true", help="skip the mixed-precision comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--write-scene", metavar="PATH",
                     help="write the workload's scene file for natray_amd/natray_drive and exit")
@@ -120,18 +166,8 @@ def main() -> int:
         print(f"wrote {args.write_scene}: {cfg['workload']}")
         return 0
     sim = Sim.from_scene(scene)
-    if args.warmup:
-        sim.run(args.warmup, iters=args.iters)
-    sim.set_profiling(True)
-    c0 = sim.counters()
-    t0 = time.perf_counter()
-    st = sim.run(args.steps, iters=args.iters)  # returns after the stream is idle
-    t1 = time.perf_counter()
-    c1 = sim.counters()
-    times = sim.phase_times()
-    sim.set_profiling(False)
-    c = {k: c1[k] - c0[k] for k in c0}
-    wall = t1 - t0
+    wall, c, times, st = timed_run(sim, args.warmup, args.steps, args.iters)
+    sim.close()
     steps_done = c["steps"]
     particles = c["particles"]
     value = particles / wall if wall > 0 else 0.0
@@ -183,10 +219,14 @@ def main() -> int:
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
+    if args.workload == "impact" and args.precision == "fp32" and not args.no_compare:
+        out["precision_compare"] = compare_precision(
+            scene, args, {"cg_iters_per_step": out["cg_iters_per_step"],
+                          "solve_ms_per_step": out["phases_ms_per_step"]["solve"],
+                          "ms_per_step": out["ms_per_step"], "unconverged_steps": st.unconverged})
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.iters, args.cpu_seconds, 40)
     print(json.dumps(out), flush=True)
-    sim.close()
     return 0
 
 

@@ -21,8 +21,11 @@ constexpr uint32_t SCAN_TILE = 1024;
 // Solver reductions fold blocks in chunks of K = ceil(n / MAX_CHUNKS), so there are at most
 // this many partial sums.
 constexpr uint32_t MAX_CHUNKS = 256;
-// Words of the solve's tagged reduction buffers: 2 buffers x 8 slots x MAX_CHUNKS.
-constexpr uint32_t RED_WORDS = 2u * 8u * MAX_CHUNKS;
+// The solve's tagged chunk sums: one 256-byte record per chunk (2 buffers x 8 slots of 8
+// bytes, padded), so the 256 workgroups polling every chunk spread over many memory channels
+// instead of queueing on a few lines.
+constexpr uint32_t RED_RECORD = 32;  // words per chunk record
+constexpr uint32_t RED_WORDS = RED_RECORD * MAX_CHUNKS;
 
 // Acc fields the transfer writes and clearing must zero. Fields 9..18 belong to the membrane,
 // air and bullet producers, which are not part of this library, and stay zero.

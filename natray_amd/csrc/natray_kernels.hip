@@ -29,6 +29,7 @@
 #include <type_traits>
 
 #include "natray_dev.h"
+#include "natray_lanes.h"
 
 namespace natray {
 
@@ -68,11 +69,12 @@ __device__ __forceinline__ void cell_coord(const Dev& d, uint32_t cell, int& x, 
 }
 
 // to_fp / from_fp (shaders/sim_defs.wgsl:217-222): ties-to-even rounding of the clamped product.
+// The clamp is one v_med3_f32, which equals min(max(v, -2e9), 2e9) for every v but NaN.
 __device__ __forceinline__ int32_t to_fp(float x, float scale) {
-  float v = x * scale;
-  v = fmin_(fmax_(v, -2e9f), 2e9f);
-  return (int32_t)__builtin_rintf(v);
+  return (int32_t)__builtin_rintf(__builtin_amdgcn_fmed3f(x * scale, -2e9f, 2e9f));
 }
+// to_fp(w, FP_W) of a B-spline weight product: w lies in [0, 1], so the clamp cannot bind.
+__device__ __forceinline__ int32_t to_fp_weight(float w) { return (int32_t)__builtin_rintf(w * NATRAY_FP_W); }
 __device__ __forceinline__ float from_fp(int32_t x, float scale) { return (float)x / scale; }
 
 __device__ __forceinline__ float& cfr(const Dev& d, uint32_t f, uint32_t cell) { return d.cf[(size_t)f * d.capc + cell]; }
@@ -482,7 +484,7 @@ __device__ __forceinline__ void splat(int* tile, const float x[3], const int cor
         float weight = wx[a] * wy[b] * wz[c];
         float node = val + ((hg[0] * ((float)a - fx[0]) + hg[1] * ((float)b - fx[1])) + hg[2] * ((float)c - fx[2]));
         atomicAdd(ta + a * 49 + b * 7 + c, to_fp(weight * node, scale));
-        atomicAdd(tw + a * 49 + b * 7 + c, to_fp(weight, NATRAY_FP_W));
+        atomicAdd(tw + a * 49 + b * 7 + c, to_fp_weight(weight));
       }
 }
 
@@ -717,6 +719,10 @@ constexpr uint32_t SOLVE_WAVES = SOLVE_THREADS / 64;
 constexpr uint32_t SOLVE_RMAX = 2;
 // Words of one reduction buffer per chunk: 4 components x 2 halves.
 constexpr uint32_t RED_SLOTS = 8;
+// Word `slot` of buffer `buf` in chunk c's record.
+__device__ __forceinline__ unsigned long long* red_word(const Dev& d, uint32_t c, uint32_t buf, uint32_t slot) {
+  return d.rwords + (size_t)c * RED_RECORD + buf * RED_SLOTS + slot;
+}
 
 __device__ __forceinline__ unsigned long long tag_word(uint32_t tag, uint32_t payload) {
   return ((unsigned long long)tag << 32) | payload;
@@ -765,9 +771,7 @@ struct Halves<double> {
 // block_sum's tree (fluid.wgsl:414-422) inside one wave; lane 0 ends with the sum.
 template <class R>
 __device__ __forceinline__ R wave_tree(R v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  return v;
+  return lane_tree<R>(v);
 }
 
 // Block-local symmetric red-black Gauss-Seidel (fluid.wgsl:384-410), five half sweeps from
@@ -779,23 +783,19 @@ __device__ __forceinline__ float precondition(float r, float diag, bool water, i
   const bool wxm = lx > 0 && ((wm >> (lane - 16u)) & 1ull), wxp = lx < 3 && ((wm >> (lane + 16u)) & 1ull);
   const bool wym = ly > 0 && ((wm >> (lane - 4u)) & 1ull), wyp = ly < 3 && ((wm >> (lane + 4u)) & 1ull);
   const bool wzm = lz > 0 && ((wm >> (lane - 1u)) & 1ull), wzp = lz < 3 && ((wm >> (lane + 1u)) & 1ull);
-  const int sxm = lx > 0 ? (int)lane - 16 : (int)lane, sxp = lx < 3 ? (int)lane + 16 : (int)lane;
-  const int sym = ly > 0 ? (int)lane - 4 : (int)lane, syp = ly < 3 ? (int)lane + 4 : (int)lane;
-  const int szm = lz > 0 ? (int)lane - 1 : (int)lane, szp = lz < 3 ? (int)lane + 1 : (int)lane;
   const uint32_t colour = (uint32_t)(lx + ly + lz) & 1u;
   float z = 0.0f;
 #pragma unroll
   for (uint32_t k = 0; k < 5u; ++k) {
-    float a = __shfl(z, sxm), b = __shfl(z, sxp), c = __shfl(z, sym), e = __shfl(z, syp), f = __shfl(z, szm),
-          g = __shfl(z, szp);
+    const Faces f = block_faces(z);
     if (water && colour == (k & 1u)) {
       float s = r;
-      if (wxm) s += a;
-      if (wxp) s += b;
-      if (wym) s += c;
-      if (wyp) s += e;
-      if (wzm) s += f;
-      if (wzp) s += g;
+      if (wxm) s += f.xm;
+      if (wxp) s += f.xp;
+      if (wym) s += f.ym;
+      if (wyp) s += f.yp;
+      if (wzm) s += f.zm;
+      if (wzp) s += f.zp;
       z = s / diag;
     }
   }
@@ -834,7 +834,7 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
   __shared__ uint32_t s_fail;
   const uint32_t t = threadIdx.x;
   const int nw = nc > 0 ? nc : 1;
-  unsigned long long* words = d.rwords + (size_t)(tag & 1u) * RED_SLOTS * MAX_CHUNKS;
+  const uint32_t buf = tag & 1u;
   uint32_t* abort = d.bar + 2;
   if (t == 0) s_fail = 0u;
   wg_sync(!sp.in_regs);
@@ -848,7 +848,7 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
         for (uint32_t b = lo; b < hi; ++b) acc += (R)(sp.in_regs ? lpart[b - sp.b0][k] : d.part[b * 4u + (uint32_t)k]);
 #pragma unroll
       for (int h = 0; h < H; ++h)
-        store_word(&words[(uint32_t)(k * 2 + h) * MAX_CHUNKS + c], tag_word(tag, Halves<R>::part(acc, h)));
+        store_word(red_word(d, c, buf, (uint32_t)(k * 2 + h)), tag_word(tag, Halves<R>::part(acc, h)));
     }
   }
   // Every chunk's words, summed as total() does (fluid.wgsl:426-439): thread t holds chunk t,
@@ -871,7 +871,7 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
 #pragma unroll
         for (uint32_t s = 0; s < RED_SLOTS; ++s) {
           if (!((full >> s) & 1u) || ((have >> s) & 1u)) continue;
-          const unsigned long long w = load_word(&words[s * MAX_CHUNKS + t]);
+          const unsigned long long w = load_word(red_word(d, t, buf, s));
           if ((uint32_t)(w >> 32) == tag) {
             pay[s] = (uint32_t)w;
             have |= 1u << s;
@@ -907,9 +907,7 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (k >= nc) continue;
-        R x = red[k][t] + red[k][t + 64u];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+        const R x = lane_tree<R>(red[k][t] + red[k][t + 64u]);
         if (t == 0) red[k][0] = x;
       }
     }
@@ -1209,9 +1207,6 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
   __shared__ uint32_t lkr[SOLVE_WAVES][SOLVE_RMAX][27];
   const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
   const int lx = (int)(ln >> 4), ly = (int)((ln >> 2) & 3u), lz = (int)(ln & 3u);
-  const int sxm = lx > 0 ? (int)ln - 16 : (int)ln, sxp = lx < 3 ? (int)ln + 16 : (int)ln;
-  const int sym = ly > 0 ? (int)ln - 4 : (int)ln, syp = ly < 3 ? (int)ln + 4 : (int)ln;
-  const int szm = lz > 0 ? (int)ln - 1 : (int)ln, szp = lz < 3 ? (int)ln + 1 : (int)ln;
   // Mask bit of the neighbour across the block face on each axis, if this lane is on one.
   const uint32_t face_bit[3] = {lx == 0 ? 1u : (lx == 3 ? 2u : 0u), ly == 0 ? 4u : (ly == 3 ? 8u : 0u),
                                 lz == 0 ? 16u : (lz == 3 ? 32u : 0u)};
@@ -1289,17 +1284,16 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
         hD[r][a] = __uint_as_float((uint32_t)hW[r][a]) + fb * hD[r][a];
       }
       const float dn = rZ[r] + fb * rD[r];
-      const float vxm = __shfl(dn, sxm), vxp = __shfl(dn, sxp), vym = __shfl(dn, sym), vyp = __shfl(dn, syp),
-                  vzm = __shfl(dn, szm), vzp = __shfl(dn, szp);
+      const Faces f = block_faces(dn);
       R v = (R)0;
       if (mask & 64u) {
         float q = rDg[r] * dn;
-        if (mask & 1u) q -= lx > 0 ? vxm : hD[r][0];
-        if (mask & 2u) q -= lx < 3 ? vxp : hD[r][0];
-        if (mask & 4u) q -= ly > 0 ? vym : hD[r][1];
-        if (mask & 8u) q -= ly < 3 ? vyp : hD[r][1];
-        if (mask & 16u) q -= lz > 0 ? vzm : hD[r][2];
-        if (mask & 32u) q -= lz < 3 ? vzp : hD[r][2];
+        if (mask & 1u) q -= lx > 0 ? f.xm : hD[r][0];
+        if (mask & 2u) q -= lx < 3 ? f.xp : hD[r][0];
+        if (mask & 4u) q -= ly > 0 ? f.ym : hD[r][1];
+        if (mask & 8u) q -= ly < 3 ? f.yp : hD[r][1];
+        if (mask & 16u) q -= lz > 0 ? f.zm : hD[r][2];
+        if (mask & 32u) q -= lz < 3 ? f.zp : hD[r][2];
         cDn[cell] = dn;
         rD[r] = dn;
         rQ[r] = q;
@@ -1335,16 +1329,15 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
       }
       if (refresh) {
         const float p = rP[r];
-        const float pxm = __shfl(p, sxm), pxp = __shfl(p, sxp), pym = __shfl(p, sym), pyp = __shfl(p, syp),
-                    pzm = __shfl(p, szm), pzp = __shfl(p, szp);
+        const Faces f = block_faces(p);
         if (water) {
           double ap = (double)diag * (double)p;
-          if (mask & 1u) ap -= (double)(lx > 0 ? pxm : hPr[r][0]);
-          if (mask & 2u) ap -= (double)(lx < 3 ? pxp : hPr[r][0]);
-          if (mask & 4u) ap -= (double)(ly > 0 ? pym : hPr[r][1]);
-          if (mask & 8u) ap -= (double)(ly < 3 ? pyp : hPr[r][1]);
-          if (mask & 16u) ap -= (double)(lz > 0 ? pzm : hPr[r][2]);
-          if (mask & 32u) ap -= (double)(lz < 3 ? pzp : hPr[r][2]);
+          if (mask & 1u) ap -= (double)(lx > 0 ? f.xm : hPr[r][0]);
+          if (mask & 2u) ap -= (double)(lx < 3 ? f.xp : hPr[r][0]);
+          if (mask & 4u) ap -= (double)(ly > 0 ? f.ym : hPr[r][1]);
+          if (mask & 8u) ap -= (double)(ly < 3 ? f.yp : hPr[r][1]);
+          if (mask & 16u) ap -= (double)(lz > 0 ? f.zm : hPr[r][2]);
+          if (mask & 32u) ap -= (double)(lz < 3 ? f.zp : hPr[r][2]);
           rr = (float)((double)d.bvec[cell] - ap);
           rR[r] = rr;
         }
