@@ -73,6 +73,20 @@ def make_scene(args):
     return s, cfg
 
 
+def measured_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of `kernel` (fetch + write) from the rocprofv3 counter summary of
+    the balloon-impact workload, or None when the summary has no figure for it."""
+    path = os.path.join(ROOT, "profiles", "impact_2M_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel:
+            kv = dict(p.split("=") for p in parts[2:])
+            return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+    return None
+
+
 def timed_run(sim, warmup: int, steps: int, iters: int):
     """Warmup steps, then `steps` timed steps with per-phase events; wall time, counter deltas,
     phase times and the final State."""
@@ -214,7 +228,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            # Counter-measured HBM bytes per launch of this kernel on this workload (FP32 form);
+            # None where no profile covers it.
+            "traffic": (measured_traffic("k_solve<false>" if dominant == "solve" else kernel_names[dominant])
+                        if args.workload == "impact" and args.precision == "fp32" else None),
             "avg_launch_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },

@@ -824,11 +824,19 @@ struct SolveSpan {
   bool in_regs;     // every workgroup's blocks fit in registers (the same for all)
 };
 
+// Work for the waves of a workgroup while a reduction completes: `wait(tag)` is called once
+// per polling round and returns true once the wave has what it waits for. NoWait has nothing.
+struct NoWait {
+  __device__ __forceinline__ bool operator()(uint32_t) const { return true; }
+};
+
 // Reduction `tag` over components [0, nc) of the block partials (nc = 0: a barrier only).
 // This workgroup's block partials are in `lpart` (blocks b0.., register form) or Dev::part.
-// Every workgroup returns the same totals; false when the solve aborted.
-template <class R>
-__device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const double (*lpart)[4], R* out) {
+// Every workgroup returns the same totals; false when the solve aborted. `wait` runs in every
+// wave alongside the polling (see NoWait).
+template <class R, class Wait = NoWait>
+__device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const double (*lpart)[4], R* out,
+                       Wait wait = Wait()) {
   constexpr int H = Halves<R>::N;
   __shared__ R red[4][256];
   __shared__ uint32_t s_fail;
@@ -853,43 +861,50 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
   }
   // Every chunk's words, summed as total() does (fluid.wgsl:426-439): thread t holds chunk t,
   // then the tree over the 256 threads.
-  if (t < 256u) {
-    R v[4] = {(R)0, (R)0, (R)0, (R)0};
-    if (t < sp.m) {
-      uint32_t pay[RED_SLOTS];
-      uint32_t have = 0, spins = 0;
-      const uint32_t full = (1u << (uint32_t)(nw * 2)) - 1u;
-      // Slots of absent halves count as seen.
-      uint32_t skip = 0;
+  R v[4] = {(R)0, (R)0, (R)0, (R)0};
+  {
+    const bool poller = t < 256u && t < sp.m;
+    uint32_t pay[RED_SLOTS];
+    const uint32_t full = (1u << (uint32_t)(nw * 2)) - 1u;
+    // Slots of absent halves count as seen; a thread without a chunk has seen everything.
+    uint32_t have = full, spins = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < RED_SLOTS; ++s) {
+      pay[s] = 0u;
+      if (poller && (s & 1u) < (uint32_t)H) have &= ~(1u << s);
+    }
+    while (true) {
+      // Issue every load of the round before looking at any, so one round trip serves both
+      // the chunk words and the waves' own waits.
+      unsigned long long w[RED_SLOTS];
+#pragma unroll
+      for (uint32_t s = 0; s < RED_SLOTS; ++s)
+        w[s] = (((full >> s) & 1u) && !((have >> s) & 1u)) ? load_word(red_word(d, t, buf, s)) : 0ull;
+      const bool waited = wait(tag);
 #pragma unroll
       for (uint32_t s = 0; s < RED_SLOTS; ++s) {
-        pay[s] = 0u;
-        if ((s & 1u) >= (uint32_t)H) skip |= 1u << s;
-      }
-      have = skip & full;
-      while (true) {
-#pragma unroll
-        for (uint32_t s = 0; s < RED_SLOTS; ++s) {
-          if (!((full >> s) & 1u) || ((have >> s) & 1u)) continue;
-          const unsigned long long w = load_word(red_word(d, t, buf, s));
-          if ((uint32_t)(w >> 32) == tag) {
-            pay[s] = (uint32_t)w;
-            have |= 1u << s;
-          }
-        }
-        if (have == full) break;
-        __builtin_amdgcn_s_sleep(1);
-        if ((++spins & 255u) == 0u &&
-            (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
-          __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          s_fail = 1u;
-          break;
+        if (!((full >> s) & 1u) || ((have >> s) & 1u)) continue;
+        if ((uint32_t)(w[s] >> 32) == tag) {
+          pay[s] = (uint32_t)w[s];
+          have |= 1u << s;
         }
       }
+      if (have == full && waited) break;
+      __builtin_amdgcn_s_sleep(1);
+      if ((++spins & 255u) == 0u &&
+          (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
+        __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_fail = 1u;
+        break;
+      }
+    }
+    if (poller) {
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         if (k < nc) v[k] = (R)0 + Halves<R>::join(pay[k * 2], pay[k * 2 + 1]);
     }
+  }
+  if (t < 256u) {
     if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -1353,16 +1368,23 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
         lpart[b - b0][1] = (double)v1;
       }
     }
-    // Halo z for the next solve_q, loaded while the reduction below completes.
+    // Halo z for the next solve_q: neighbours publish it before their part of the reduction
+    // below, so every wave reloads its stale halo words while the reduction completes.
+    auto halo_poll = [&](uint32_t want) -> bool {
+      bool fresh = true;
 #pragma unroll
-    for (uint32_t r = 0; r < RM; ++r) {
-      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
-      if (b >= b1) continue;
+      for (uint32_t r = 0; r < RM; ++r) {
+        if (b0 + wv + SOLVE_WAVES * r >= b1) continue;
 #pragma unroll
-      for (int a = 0; a < 3; ++a)
-        if (has_halo(r, a)) hW[r][a] = load_word(&hz[halo_cell(r, a)]);
-    }
-    if (!reduce<R>(d, sp, ++tag, 2, lpart, tt)) break;
+        for (int a = 0; a < 3; ++a) {
+          if (!has_halo(r, a) || (uint32_t)(hW[r][a] >> 32) == want) continue;
+          hW[r][a] = load_word(&hz[halo_cell(r, a)]);
+          fresh = fresh && (uint32_t)(hW[r][a] >> 32) == want;
+        }
+      }
+      return fresh;
+    };
+    if (!reduce<R>(d, sp, ++tag, 2, lpart, tt, halo_poll)) break;
     go = solve_beta<R>(d, tt, s);
   }
   // The fields the other kernels and the host read: p, r, z and q of this wave's water cells
