@@ -51,8 +51,9 @@ struct Dev {
   double* part;            // cap x 4: per-block partial sums of the solver (memory form)
   uint32_t* tile_sum;      // 2 x ntiles: scan tile totals (slots claimed, particles)
   uint32_t* tile_off;      // 2 x ntiles: their exclusive prefix
-  uint32_t* bar;           // solve control words: [2] abort flag a stuck wait raises,
-                           // [4] last reduction tag used
+  uint32_t* bar;           // control words: [2] the solve's abort flag a stuck wait raises,
+                           // [4] its last reduction tag, [8] / [9] the next block P2G / G2P
+                           // hands out (reset by k_begin)
   unsigned long long* rwords;  // RED_WORDS: tagged chunk sums of the solve's reductions
   unsigned long long* hz;      // capc: tagged z of each water cell (register form of the solve)
   float* vmax_part;        // fastest particle of each g2p workgroup (<= cap workgroups)

@@ -231,6 +231,19 @@ __global__ void k_begin(Dev d) {
     st->n_blocks = 0;
   }
   st->dt_ns = dt_ns;
+  d.bar[8] = 0u;
+  d.bar[9] = 0u;
+}
+
+// The next block for a workgroup of a kernel that hands blocks out through counter `word` of
+// Dev::bar: blocks cost what their particles cost, so a workgroup takes a new one when it is
+// done rather than every gridDim-th. Call from all threads; returns the same block to all.
+__device__ __forceinline__ uint32_t next_block(const Dev& d, uint32_t word) {
+  __shared__ uint32_t s_next;
+  __syncthreads();
+  if (threadIdx.x == 0) s_next = atomicAdd(&d.bar[word], 1u);
+  __syncthreads();
+  return s_next;
 }
 
 // Keys of particles written from the host.
@@ -534,29 +547,33 @@ __device__ __forceinline__ uint32_t key_local(uint32_t key, int bx, int by, int 
   return (ox > 3u || oy > 3u || oz > 3u) ? 64u : ox * 16u + oy * 4u + oz;
 }
 
-// Particles of a block that P2G orders by cell in LDS; a block with more splats unordered.
+// Particles of a block that P2G orders in LDS, and the most particles one cell may hold for
+// it; a block beyond either splats its particles unordered.
 constexpr uint32_t P2G_MAXP = 2048;
+constexpr uint32_t P2G_MAXR = 64;
 
-// p2g (fluid.wgsl:254-303). The block's particles are first counted and ordered by their own
-// cell in LDS; then, round r, lane l of a wave splats the r-th particle of cell l, so the 64
-// lanes of one LDS atomic instruction add into 64 different cells' stencils instead of eight
-// particles of one cell hitting the same words. The per-cell counts are A_N.
-__global__ __launch_bounds__(256) void k_p2g(Dev d) {
+// p2g (fluid.wgsl:254-303). The block's particles are counted per own cell, then laid out in
+// LDS round-major: first the first particle of every cell that has one, then every cell's
+// second, and so on. The 256 threads splat that list in order, so the lanes of one LDS atomic
+// instruction hold particles of different cells (their stencils, not the same words), and no
+// lane idles while a crowded cell still has particles. The per-cell counts are A_N.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_p2g(Dev d) {
   if (d.st->dt_ns == 0u) return;
   __shared__ int tile[8 * TILE];
-  __shared__ uint32_t cnt[64], start[64], fillc[64], drop[64];
+  __shared__ uint32_t cnt[64], drop[64];
   __shared__ uint32_t order[P2G_MAXP];
+  __shared__ uint8_t ocell[P2G_MAXP];
+  __shared__ unsigned long long rmask[P2G_MAXR];
+  __shared__ uint32_t roff[P2G_MAXR];
   __shared__ uint32_t links[27];
   __shared__ uint32_t s_maxc;
-  const uint32_t n = n_blocks(d), tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
+  const uint32_t n = n_blocks(d), tid = threadIdx.x;
   const natray_params* U = d.U;
   const float h = U->origin[3];
-  for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
-    __syncthreads();
+  for (uint32_t b = next_block(d, 8); b < n; b = next_block(d, 8)) {
     if (tid < 27) block_links(d, b, tid, links);
     if (tid < 64) {
       cnt[tid] = 0;
-      fillc[tid] = 0;
       drop[tid] = 0;
     }
     for (uint32_t t = tid; t < 8 * TILE; t += 256) tile[t] = 0;
@@ -570,32 +587,68 @@ __global__ __launch_bounds__(256) void k_p2g(Dev d) {
     slot_coord(U, d.block_list[b], bx, by, bz);
     const int corner[3] = {bx * 4 - 1, by * 4 - 1, bz * 4 - 1};
     if (count <= P2G_MAXP) {
-      for (uint32_t k = tid; k < count; k += 256) {
-        const uint32_t l = key_local(d.key[d.sorted[first + k]], bx, by, bz);
-        if (l < 64u) atomicAdd(&cnt[l], 1u);
+      // One pass over the block's particles: each thread keeps its (at most eight) particles
+      // and their cells in registers, and the count atomic gives each its rank in its cell.
+      uint32_t pi[P2G_MAXP / 256], pl[P2G_MAXP / 256];
+#pragma unroll
+      for (uint32_t j = 0; j < P2G_MAXP / 256; ++j) {
+        const uint32_t k = tid + 256u * j;
+        pi[j] = k < count ? d.sorted[first + k] : NONE;
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < P2G_MAXP / 256; ++j) {
+        pl[j] = NONE;
+        if (pi[j] != NONE) {
+          const uint32_t l = key_local(d.key[pi[j]], bx, by, bz);
+          if (l < 64u) pl[j] = l | atomicAdd(&cnt[l], 1u) << 8;
+        }
       }
       __syncthreads();
       if (tid < 64) {
+        // Round r holds the cells with more than r particles: rmask[r] marks them, roff[r] is
+        // where the round starts in the list.
         const uint32_t c = cnt[tid];
-        uint32_t incl = c, mx = c;
-        for (int off = 1; off < 64; off <<= 1) {
-          const uint32_t t = __shfl_up(incl, off);
-          if (lane >= (uint32_t)off) incl += t;
-        }
+        uint32_t mx = c;
         for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
-        start[tid] = incl - c;
+        if (mx <= P2G_MAXR) {
+          uint32_t off = 0;
+          for (uint32_t r = 0; r < mx; ++r) {
+            const unsigned long long m = __ballot(c > r);
+            if (tid == 0) {
+              rmask[r] = m;
+              roff[r] = off;
+            }
+            off += (uint32_t)__popcll(m);
+          }
+        }
         if (tid == 0) s_maxc = mx;
       }
       __syncthreads();
-      for (uint32_t k = tid; k < count; k += 256) {
-        const uint32_t i = d.sorted[first + k];
-        const uint32_t l = key_local(d.key[i], bx, by, bz);
-        if (l < 64u) order[start[l] + atomicAdd(&fillc[l], 1u)] = i;
+      const uint32_t maxc = s_maxc;
+      if (maxc <= P2G_MAXR) {
+#pragma unroll
+        for (uint32_t j = 0; j < P2G_MAXP / 256; ++j) {
+          if (pl[j] == NONE) continue;
+          const uint32_t l = pl[j] & 255u, r = pl[j] >> 8;
+          const uint32_t pos = roff[r] + (uint32_t)__popcll(rmask[r] & ((1ull << l) - 1ull));
+          order[pos] = pi[j];
+          ocell[pos] = (uint8_t)l;
+        }
+        __syncthreads();
+        uint32_t placed = 0;
+        for (uint32_t r = 0; r < maxc; ++r) placed += (uint32_t)__popcll(rmask[r]);
+        for (uint32_t k = tid; k < placed; k += 256) {
+          const uint32_t l = ocell[k];
+          if (!splat_particle(d, tile, order[k], l, bx, by, bz, corner, h)) atomicAdd(&drop[l], 1u);
+        }
+      } else {
+        // A cell with more than P2G_MAXR particles: splat in stored order (counted above).
+        for (uint32_t k = tid; k < count; k += 256) {
+          const uint32_t i = d.sorted[first + k];
+          const uint32_t l = key_local(d.key[i], bx, by, bz);
+          if (l < 64u && !splat_particle(d, tile, i, l, bx, by, bz, corner, h)) atomicAdd(&drop[l], 1u);
+        }
       }
-      __syncthreads();
-      const uint32_t maxc = s_maxc, mine = cnt[lane], base = start[lane];
-      for (uint32_t r = wv; r < maxc; r += 4u)
-        if (r < mine && !splat_particle(d, tile, order[base + r], lane, bx, by, bz, corner, h)) atomicAdd(&drop[lane], 1u);
     } else {
       for (uint32_t k = tid; k < count; k += 256) {
         const uint32_t i = d.sorted[first + k];
@@ -1640,8 +1693,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ uint32_t links[27];
   const uint32_t n = n_blocks(d), tid = threadIdx.x;
   float fastest = 0.0f;
-  for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
-    __syncthreads();
+  for (uint32_t b = next_block(d, 9); b < n; b = next_block(d, 9)) {
     if (tid < 27) links[tid] = d.block_nb[b * 27u + tid];
     const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
     __syncthreads();
