@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -87,9 +89,12 @@ def measured_traffic(kernel: str) -> float | None:
     return None
 
 
-def timed_run(sim, warmup: int, steps: int, iters: int):
-    """Warmup steps, then `steps` timed steps with per-phase events; wall time, counter deltas,
-    phase times and the final State."""
+def timed_run(sim, warmup: int, steps: int, iters: int, graph: bool = True):
+    """Warmup steps, then `steps` timed steps with events on the library's stream; wall time,
+    counter deltas, phase times and the final State. With `graph` (the library's default)
+    the steps replay captured graphs and only the solve phase is timed; without it every
+    kernel is launched and timed on its own."""
+    sim.set_graph(graph)
     if warmup:
         sim.run(warmup, iters=iters)
     sim.set_profiling(True)
@@ -132,6 +137,111 @@ def compare_precision(scene, args, plain: dict) -> dict:
     }
 
 
+def _cells_by_slot(g: dict) -> dict:
+    return {int(slot): i for i, slot in enumerate(g["block_list"])}
+
+
+def _divergence(g: dict, U: np.ndarray) -> np.ndarray:
+    """Discrete divergence sum over dims of u(up face) - u(own face), per cell (in units of
+    velocity), from the face velocities U (3 x 64n) of the blocks in `g`; a face of a missing
+    neighbour counts as 0."""
+    n = g["n_blocks"]
+    lane = np.arange(64)
+    loc = np.stack([lane >> 4, (lane >> 2) & 3, lane & 3])
+    div = np.zeros(64 * n)
+    nb = g["block_nb"].astype(np.int64)
+    for dim in range(3):
+        up_local = loc.copy()
+        up_local[dim] += 1
+        inside = up_local[dim] < 4
+        up_local[dim] %= 4
+        up_lane = up_local[0] * 16 + up_local[1] * 4 + up_local[2]
+        link = 13 + (9, 3, 1)[dim]  # the +dim neighbour in the 27-block table
+        blk = np.repeat(np.arange(n), 64)
+        lanes = np.tile(np.arange(64), n)
+        other = nb[blk, link]
+        up_blk = np.where(inside[lanes], blk, other)
+        ok = up_blk != 0xFFFFFFFF
+        up_cell = np.where(ok, up_blk * 64 + up_lane[lanes], 0)
+        u_up = np.where(ok, U[dim][up_cell], 0.0)
+        div += u_up - U[dim]
+    return div
+
+
+def _errors(g: dict, P: np.ndarray, g_ref: dict, cf_ref: np.ndarray, P_ref: np.ndarray, h: float) -> dict:
+    """Errors of a GPU grid and particle set against the FP64 oracle's, on the cells of blocks
+    both allocated (matched by domain slot) and the particles alive in both."""
+    from natray_amd import abi
+
+    slots_ref = _cells_by_slot(g_ref)
+    common = [(i, slots_ref[int(slot)]) for i, slot in enumerate(g["block_list"]) if int(slot) in slots_ref]
+    gi = np.concatenate([np.arange(64) + 64 * i for i, _ in common])
+    ri = np.concatenate([np.arange(64) + 64 * j for _, j in common])
+    water = (g["cf"][abi.F_TYPE][gi] == abi.WATER) & (cf_ref[abi.F_TYPE][ri] == abi.WATER)
+    p_g, p_r = g["cf"][abi.F_P][gi][water].astype(np.float64), cf_ref[abi.F_P][ri][water]
+    u_g = g["cf"][abi.F_U:abi.F_U + 3][:, gi].astype(np.float64)
+    u_r = cf_ref[abi.F_U:abi.F_U + 3][:, ri]
+    div_g = _divergence(g, g["cf"][abi.F_U:abi.F_U + 3].astype(np.float64))[gi][water]
+    div_r = _divergence(g_ref, cf_ref[abi.F_U:abi.F_U + 3])[ri][water]
+    alive = (P[:, 7] == 1.0) & (P_ref[:, 7] == 1.0)
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)) if b.size else 0.0
+    return {
+        "max_pressure_error_rel": rel(p_g, p_r),
+        "max_grid_velocity_error_rel": rel(u_g, u_r),
+        "max_divergence_error_rel": rel(div_g, div_r),
+        "max_position_error_cells": float(np.abs(P[alive, 0:3].astype(np.float64) - P_ref[alive, 0:3]).max() / h)
+        if alive.any() else 0.0,
+        "alive_mismatch": int(((P[:, 7] == 1.0) != (P_ref[:, 7] == 1.0)).sum()),
+        "blocks_not_shared": int(g["n_blocks"] - len(common)),
+    }
+
+
+def accuracy_vs_fp64(scene, checkpoints, iters: int) -> dict:
+    """Config 3's report: the plain FP32 and the mixed-precision CG run from the scene's
+    initial state on the GPU, the FP64 oracle on the host; errors of the GPU runs against it
+    after each checkpoint's number of steps, at the same tolerance. Relative errors are the
+    largest difference over the largest reference magnitude."""
+    import oracle
+    from natray_amd import Sim, abi
+
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    h = float(scene.params.origin[3])
+    ref = oracle.Oracle.from_scene(scene, "f64")
+    refs, done = {}, 0
+    for k in checkpoints:
+        ref.run(k - done, iters)
+        done = k
+        sc_ref = ref.solver_scalars()
+        refs[k] = (ref.grid(), ref.cf_f64(), ref.particles_f64(),
+                   sc_ref["r_norm"] / max(sc_ref["b_norm"], 1.0), int(sc_ref["solve_iters"]))
+    ref.close()
+    out = {"tol": float(scene.params.solve[0]),
+           "criteria": "grid velocity <= 1e-5 relative and positions <= 1e-4 dx after one substep"}
+    sc = scene.scene
+    keep = (sc.precision, sc.refresh_every)
+    try:
+        for label, prec, refresh in (("plain_fp32", abi.CG_FP32, 0), ("mixed_f32_f64", abi.CG_MIXED, 20)):
+            sc.precision, sc.refresh_every = prec, refresh
+            sim = Sim.from_scene(scene)
+            done = 0
+            for k in checkpoints:
+                c0 = sim.counters()
+                st = sim.run(k - done, iters=iters)
+                c1 = sim.counters()
+                done = k
+                g_ref, cf_ref, P_ref, res_ref, it_ref = refs[k]
+                e = _errors(sim.grid(), sim.particles(), g_ref, cf_ref, P_ref, h)
+                e["cg_iters"] = int(c1["cg_iters"] - c0["cg_iters"])
+                e["cg_relative_residual_last_solve"] = st.r_norm / max(st.b_norm, 1.0)
+                e["fp64_cg_relative_residual_last_solve"] = res_ref
+                e["fp64_cg_iters_last_step"] = it_ref
+                out.setdefault(label, {})[f"after_{k}_steps"] = e
+            sim.close()
+    finally:
+        sc.precision, sc.refresh_every = keep
+    return out
+
+
 def cpu_baseline(scene, iters: int, budget_s: float, max_steps: int) -> dict:
     """The FP32 oracle (OpenMP over the host's cores) on the same workload from the same
     initial state: one untimed step, then steps until `budget_s` of wall time or max_steps."""
@@ -162,6 +272,8 @@ def main() -> int:
     ap.add_argument("--iters", type=int, default=1000, help="CG iteration cap per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-compare", action="store_true", help="skip the mixed-precision comparison run")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the error report against the FP64 oracle")
+    ap.add_argument("--accuracy-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--write-scene", metavar="PATH",
                     help="write the workload's scene file for natray_amd/natray_drive and exit")
@@ -182,21 +294,28 @@ def main() -> int:
     sim = Sim.from_scene(scene)
     wall, c, times, st = timed_run(sim, args.warmup, args.steps, args.iters)
     sim.close()
+    # The per-phase breakdown: the same steps from the same state, every kernel launched and
+    # timed on its own (the timed run above replays graphs and times the solve only).
+    sim = Sim.from_scene(scene)
+    wall_eager, _, times_eager, _ = timed_run(sim, args.warmup, args.steps, args.iters, graph=False)
+    sim.close()
     steps_done = c["steps"]
     particles = c["particles"]
     value = particles / wall if wall > 0 else 0.0
 
     pb = phase_bytes(c)
     single = ("p2g", "grid_prep", "solve", "project", "g2p")  # phases that are one kernel
-    dominant = max(single, key=lambda k: times[k][0])
-    ms, launches = times[dominant]
+    dominant = max(single, key=lambda k: times_eager[k][0])
+    # The dominant kernel's launches are timed in the graph run when it is the solve (launched
+    # between the graphs), else in the kernel-by-kernel run.
+    ms, launches = times[dominant] if times[dominant][1] > 0 else times_eager[dominant]
     avg_s = ms * 1e-3 / max(launches, 1)
     bytes_per_launch = pb[dominant] / max(launches, 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     kernel_names = {"p2g": "k_p2g", "grid_prep": "k_grid_prep", "solve": "k_solve", "project": "k_project",
                     "g2p": "k_g2p"}
     t_cg = times["solve"][0] * 1e-3
-    t_all_kernels = sum(v[0] for v in times.values()) * 1e-3
+    t_all_kernels = sum(v[0] for v in times_eager.values()) * 1e-3
     out = {
         "metric": "particle-substeps/s + CG cell-iters/s, mixed/warm vs plain FP32 at same div tol",
         "value": value,
@@ -218,9 +337,11 @@ def main() -> int:
         "water_cells_per_step": c["water_cells"] / max(steps_done, 1),
         "final_state": {"t_ns": st.t_ns, "step": st.step, "dt_ns": st.dt_ns, "unconverged": st.unconverged,
                         "max_speed": st.max_speed},
-        "phases_ms_per_step": {k: v[0] / max(steps_done, 1) for k, v in times.items()},
-        "phases_algorithmic_GBps": {k: (pb[k] / (times[k][0] * 1e-3) / 1e9 if times[k][0] > 0 else 0.0) for k in pb},
-        "kernel_time_fraction_of_wall": t_all_kernels / wall if wall > 0 else 0.0,
+        "ms_per_step_kernel_by_kernel": wall_eager * 1e3 / max(args.steps, 1),
+        "phases_ms_per_step": {k: v[0] / max(steps_done, 1) for k, v in times_eager.items()},
+        "phases_algorithmic_GBps": {k: (pb[k] / (times_eager[k][0] * 1e-3) / 1e9 if times_eager[k][0] > 0 else 0.0)
+                                    for k in pb},
+        "kernel_time_fraction_of_wall_kernel_by_kernel": t_all_kernels / wall_eager if wall_eager > 0 else 0.0,
         "roofline": {
             "kernel": kernel_names[dominant],
             "bound": "hbm",
@@ -239,10 +360,12 @@ def main() -> int:
     if args.workload == "impact" and args.precision == "fp32" and not args.no_compare:
         out["precision_compare"] = compare_precision(
             scene, args, {"cg_iters_per_step": out["cg_iters_per_step"],
-                          "solve_ms_per_step": out["phases_ms_per_step"]["solve"],
+                          "solve_ms_per_step": times["solve"][0] / max(steps_done, 1),
                           "ms_per_step": out["ms_per_step"], "unconverged_steps": st.unconverged})
+    if args.workload == "impact" and not args.no_accuracy:
+        out["accuracy_vs_fp64_oracle"] = accuracy_vs_fp64(scene, (1, args.accuracy_steps), args.iters)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, args.iters, args.cpu_seconds, 40)
+        out["cpu_baseline"] = cpu_baseline(scene, args.iters, args.cpu_seconds, 200)
     print(json.dumps(out), flush=True)
     return 0
 

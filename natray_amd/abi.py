@@ -157,6 +157,7 @@ _SIGNATURES = {
     "natray_sim_device_ptr": (C.c_void_p, [_P, C.c_int]),
     "natray_sim_set_profiling": (C.c_int, [_P, C.c_int]),
     "natray_sim_phase_times": (C.c_int, [_P, _P, _P]),
+    "natray_sim_set_graph": (C.c_int, [_P, C.c_int]),
     "natray_sim_counters": (C.c_int, [_P, _P]),
     "natray_plan_batch": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32]),
     "natray_next_iters": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),

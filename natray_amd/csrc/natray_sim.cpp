@@ -67,6 +67,10 @@ struct natray_sim {
   double step_ms = 0.0;
   bool started = false;
   double t = 0.0;
+  // Step graphs: the phases before the solve and those after it, captured on `stream`.
+  bool use_graph = true;
+  bool capturing = false;  // no timing events go into a captured graph
+  hipGraphExec_t graph_pre = nullptr, graph_post = nullptr;
   // Per-phase timing.
   bool prof = false;
   std::vector<hipEvent_t> events;
@@ -92,6 +96,9 @@ int dalloc(natray_sim* s, T** p, size_t count) {
 
 void free_all(natray_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
+  if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+  s->graph_pre = s->graph_post = nullptr;
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
   s->events.clear();
   for (void* p : s->allocs) (void)hipFree(p);
@@ -146,7 +153,7 @@ int enqueue_phase(natray_sim* s, int phase) {
   const natray::Dev& D = s->D;
   void* st = s->stream;
   size_t slot = 0;
-  bool timed = s->prof && s->ev_used < s->events.size() / 2;
+  bool timed = s->prof && !s->capturing && s->ev_used < s->events.size() / 2;
   if (timed) {
     slot = s->ev_used++;
     s->ev_phase[slot] = phase;
@@ -174,6 +181,49 @@ int enqueue_phase(natray_sim* s, int phase) {
     default: return fail(NATRAY_E_ARG, "unknown phase " + std::to_string(phase));
   }
   if (timed) HIPCHK(hipEventRecord(s->events[2 * slot + 1], s->stream));
+  return NATRAY_OK;
+}
+
+// Captures phases [lo, hi) as a graph on the handle's stream.
+int capture_phases(natray_sim* s, int lo, int hi, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int e = NATRAY_OK;
+  s->capturing = true;
+  for (int ph = lo; ph < hi && e == NATRAY_OK; ++ph) e = enqueue_phase(s, ph);
+  s->capturing = false;
+  hipError_t ec = hipStreamEndCapture(s->stream, &g);
+  if (e) {
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+  }
+  if (ec != hipSuccess) return fail(NATRAY_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  ec = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ec != hipSuccess) return fail(NATRAY_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+  return NATRAY_OK;
+}
+
+// One step: the two captured graphs around the solve, or every phase launched on its own.
+// With profiling on, the graphs' phases are not timed; the solve between them is.
+int enqueue_step(natray_sim* s) {
+  if (!s->use_graph) {
+    for (int ph = 0; ph < NATRAY_N_PHASES; ++ph) {
+      int e = enqueue_phase(s, ph);
+      if (e) return e;
+    }
+    return NATRAY_OK;
+  }
+  if (!s->graph_pre) {
+    int e = capture_phases(s, NATRAY_PHASE_BEGIN, NATRAY_PHASE_SOLVE, &s->graph_pre);
+    if (e) return e;
+    e = capture_phases(s, NATRAY_PHASE_PROJECT, NATRAY_N_PHASES, &s->graph_post);
+    if (e) return e;
+  }
+  HIPCHK(hipGraphLaunch(s->graph_pre, s->stream));
+  int e = enqueue_phase(s, NATRAY_PHASE_SOLVE);
+  if (e) return e;
+  HIPCHK(hipGraphLaunch(s->graph_post, s->stream));
   return NATRAY_OK;
 }
 
@@ -399,11 +449,10 @@ int natray_sim_run(natray_sim* s, uint32_t n_steps, uint32_t iters, uint32_t sto
   s->h_words[5] = 0;
   HIPCHK(hipMemcpyAsync(&s->D.st->iters_max, &s->h_words[4], 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
   auto t0 = std::chrono::steady_clock::now();
-  for (uint32_t k = 0; k < n_steps; ++k)
-    for (int ph = 0; ph < NATRAY_N_PHASES; ++ph) {
-      e = enqueue_phase(s, ph);
-      if (e) return e;
-    }
+  for (uint32_t k = 0; k < n_steps; ++k) {
+    e = enqueue_step(s);
+    if (e) return e;
+  }
   e = sync_state(s);
   if (e) return e;
   if (n_steps) s->step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n_steps;
@@ -578,6 +627,12 @@ int natray_sim_set_profiling(natray_sim* s, int on) {
     s->ms[k] = 0.0;
     s->cnt[k] = 0;
   }
+  return NATRAY_OK;
+}
+
+int natray_sim_set_graph(natray_sim* s, int on) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  s->use_graph = on != 0;
   return NATRAY_OK;
 }
 

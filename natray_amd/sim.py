@@ -136,6 +136,11 @@ class Sim:
         return bp, so[: self.n]
 
     # ---- measurement ------------------------------------------------------------------------
+    def set_graph(self, on: bool) -> None:
+        """Steps of run() as two replayed HIP graphs around the solve (on, the default) or kernel
+        by kernel (off); the results are the same."""
+        abi.check(abi.lib().natray_sim_set_graph(self._h, 1 if on else 0))
+
     def set_profiling(self, on: bool) -> None:
         abi.check(abi.lib().natray_sim_set_profiling(self._h, 1 if on else 0))
 
