@@ -288,6 +288,41 @@ int natray_sim_counters(natray_sim* sim, uint64_t out[8]);
 
 const char* natray_last_error(void);
 
+/* ---- the CG warm-start network ------------------------------------------------------------
+ * No counterpart in the reference: the learned initial guess of the pressure solve. A 4-layer
+ * 3D convolution net on 16^3 tiles of cells (3^3 kernels; 2 -> 16 -> 16 -> 16 -> 1 channels,
+ * ReLU between), BF16 weights and activations with FP32 accumulation on MFMA. Inputs per
+ * voxel: the solve's initial residual scaled by its tile's largest magnitude, and the cell
+ * kind (1 water, -1 solid, 0 otherwise); output: the pressure correction in the same scale.
+ * Arrays passed to these functions are host memory, tile-major, voxel v = 256 x + 16 y + z. */
+typedef struct natray_net natray_net;
+
+/* Parameters, FP32, in order: W1[27][2][16], b1[16], W2[27][16][16], b2[16], W3[27][16][16],
+ * b3[16], W4[27][16][1], b4[1] (W[tap][in][out], tap = 9 (dx + 1) + 3 (dy + 1) + dz + 1). */
+#define NATRAY_NET_PARAMS 15169
+
+/* A network with room for `max_tiles` tiles (<= 65536), He-uniform weights from `seed`. */
+int natray_net_create(uint32_t max_tiles, uint64_t seed, natray_net** out);
+void natray_net_destroy(natray_net* net);
+int natray_net_params(natray_net* net, float* out);
+int natray_net_set_params(natray_net* net, const float* in);
+/* output[tiles][4096] = the network on input[tiles][4096][2]. */
+int natray_net_forward(natray_net* net, const float* input, uint32_t tiles, float* output);
+/* One Adam step (lr; lr = 0 only computes gradients) on the masked mean squared error of the
+ * output against target[tiles][4096] over the voxels where mask[tiles][4096] is 1; *loss gets
+ * the loss before the step. */
+int natray_net_train_step(natray_net* net, const float* input, const float* target, const float* mask, uint32_t tiles,
+                          float lr, float* loss);
+/* The parameter gradients of the last train step. */
+int natray_net_grads(natray_net* net, float* out);
+/* One convolution layer alone, for gradient checks: Y = act(conv(X, W) + b) and, from dY,
+ * dX, dW and db, with X, W and the ReLU-masked dY rounded to BF16 as the network does.
+ * X[tiles][4096][cin], W[27][cin][cout], dY and Y [tiles][4096][cout]; cin is 2 or 16, cout
+ * 16 or 1, tiles <= 64. */
+int natray_net_layer_check(uint32_t cin, uint32_t cout, int relu, const float* X, const float* W, const float* b,
+                           const float* dY, uint32_t tiles, float* Y, float* dX, float* dW, float* db);
+const char* natray_net_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,7 +39,7 @@ FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["natray_kernels.hip", "natray_sim.cpp"]
+SOURCES = ["natray_kernels.hip", "natray_net.hip", "natray_sim.cpp"]
 
 
 def build(verbose: bool = False) -> str:
