@@ -323,6 +323,32 @@ int natray_net_layer_check(uint32_t cin, uint32_t cout, int relu, const float* X
                            const float* dY, uint32_t tiles, float* Y, float* dX, float* dW, float* db);
 const char* natray_net_last_error(void);
 
+/* ---- the warm start in the step ------------------------------------------------------------
+ * No counterpart in the reference, whose solve always starts from the particle-carried pressure
+ * (solve_init, shaders/fluid.wgsl:450-489). Each step, before the solve: r0 = b - A p_carried on
+ * the water cells; the 16^3 tiles holding water are listed in tile order; each listed tile's
+ * inputs are r0 / s (s = the tile's largest |r0|) and the cell kind.
+ * mode 0: off (the default). mode 1: only the inputs, kept for natray_sim_ws_samples; the solve
+ * starts as without the warm start. mode 2: `net` (up to its max_tiles tiles) maps the inputs to
+ * o, guess = p_carried + s o on the tiles' water cells (p_carried elsewhere), and the solve
+ * starts from the guess when |b - A guess|^2 < |r0|^2 (FP64 sums), else from p_carried. The
+ * right-hand side b and the tolerance are the same either way. The network must stay alive
+ * while the mode uses it. Resets the statistics of natray_sim_ws_stats. */
+int natray_sim_set_warm_start(natray_sim* sim, natray_net* net, int mode);
+/* Training samples of the last step's listed tiles (mode 1 or 2), at most max_tiles of them:
+ * inputs[tiles][4096][2] as the network got them, targets[tiles][4096] = (p_solved -
+ * p_carried) / s and masks[tiles][4096] = 1 on water cells; *tiles gets the count. Any of the
+ * three arrays may be NULL. */
+int natray_sim_ws_samples(natray_sim* sim, float* inputs, float* targets, float* masks, uint32_t max_tiles,
+                          uint32_t* tiles);
+/* Since the mode was set: [0] solves offered a guess, [1] guesses taken, [2] |b - A guess|^2
+ * and [3] |r0|^2 of the last step, [4] tiles with water in the last step (also those beyond
+ * the network's capacity), [5..7] 0. */
+int natray_sim_ws_stats(natray_sim* sim, double out[8]);
+/* The last step's guess (bdims.w * 64 cells, the grid's cell order) and whether the solve
+ * started from it. */
+int natray_sim_ws_guess(natray_sim* sim, float* guess, uint32_t* accepted);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,12 +53,27 @@ struct Dev {
   uint32_t* tile_off;      // 2 x ntiles: their exclusive prefix
   uint32_t* bar;           // control words: [2] the solve's abort flag a stuck wait raises,
                            // [4] its last reduction tag, [8] / [9] the next block P2G / G2P
-                           // hands out (reset by k_begin)
+                           // hands out, [10] warm-start tiles this step, [11] 1 when the
+                           // solve starts from the warm start's guess (reset by k_begin)
   unsigned long long* rwords;  // RED_WORDS: tagged chunk sums of the solve's reductions
   unsigned long long* hz;      // capc: tagged z of each water cell (register form of the solve)
   float* vmax_part;        // fastest particle of each g2p workgroup (<= cap workgroups)
   double* solver;          // double copies of rz, alpha, beta, b_norm, r_norm
   unsigned long long* stats;  // counters, see natray_sim_counters
+  // Warm start of the pressure solve (the learned initial guess).
+  float* guess;        // capc: the starting pressure the warm start offers
+  float* ws_r0;        // capc: b - A p of the particle-carried pressure, per water cell
+  float* ws_pcar;      // capc: the particle-carried pressure before the solve
+  uint32_t* tile_map;  // tdx x tdy x tdz: 1 + the bits of the largest |r0| of a tile's water
+                       // cells (16^3 cells per tile), 0 for a tile without water
+  uint32_t* tile_list; // ws_max: the tiles with water this step, in tile order
+  float* tile_scale;   // ws_max: each listed tile's largest |r0|
+  float* ws_in;        // ws_max x 4096 x 2: the network's inputs
+  float* ws_tgt;       // ws_max x 4096: training targets of the last solve (see natray_sim_ws_samples)
+  float* ws_mask;      // ws_max x 4096: their water mask
+  double* ws_part;     // cap x 2: per-block squared residuals of the guess and the carried pressure
+  double* ws_stat;     // 8: see natray_sim_ws_stats
+  uint32_t tdx, tdy, tdz, ws_max;
   const Dev* self;  // a device copy of this struct, for device functions that take it by pointer
   uint32_t N;
   uint32_t S;      // domain slots (product of bdims)
@@ -83,4 +98,11 @@ int solve_max_grid(int mixed, int* per_cu);
 int launch_project(const Dev& d, void* stream, int grid_waves);
 int launch_extend(const Dev& d, void* stream, int grid_waves);
 int launch_g2p(const Dev& d, void* stream, int grid_blocks);
+// Warm start, before the solve: the network's inputs for at most `cap` tiles (also into the
+// network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has run on them
+// (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a step,
+// launch_ws_samples writes the training targets of its listed tiles.
+int launch_ws_inputs(const Dev& d, void* stream, int grid_waves, uint32_t cap, void* net_in);
+int launch_ws_guess(const Dev& d, void* stream, int grid_waves, const float* pred);
+int launch_ws_samples(const Dev& d, void* stream);
 }  // namespace natray

@@ -233,6 +233,8 @@ __global__ void k_begin(Dev d) {
   st->dt_ns = dt_ns;
   d.bar[8] = 0u;
   d.bar[9] = 0u;
+  d.bar[10] = 0u;
+  d.bar[11] = 0u;
 }
 
 // The next block for a workgroup of a kernel that hands blocks out through counter `word` of
@@ -990,67 +992,88 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
   return ok;
 }
 
-// solve_init (fluid.wgsl:450-491) for one cell: the operator row, b, r = b - A p0 and z. Writes
-// diag, r, b, the neighbour mask and z; in register form also the tagged z (tag `ztag`).
-// Stores the block partials (r.z, r.r, b.b, water cells) through `prow` from lane 0.
-template <class R>
-__device__ __forceinline__ void init_cell(const Dev& d, uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk,
-                                          bool regs, uint32_t ztag, double* prow) {
+// The solve's row of one cell (solve_init, fluid.wgsl:450-489): whether it is water, the
+// diagonal, the right-hand side b (from the particle-carried pressure, always), the mask of
+// water neighbours, and the residual b - A p0 for the starting pressure p0 (the carried
+// pressure, or the warm start's guess).
+struct Row {
+  bool water;
+  float diag, bv, r;
+  uint32_t mask;
+};
+__device__ __forceinline__ Row cell_row(const Dev& d, uint32_t cell, int lx, int ly, int lz, const uint32_t* lk,
+                                        const float* p0) {
   const natray_params* U = d.U;
   const float h = U->origin[3], dt = dt_of(d);
-  const uint32_t capc = d.capc;
-  const float* cP = d.cf + (size_t)NATRAY_F_P * capc;
-  const uint32_t cell = b * 64u + lane;
-  const bool water = cfr(d, NATRAY_F_TYPE, cell) == WATER;
-  float diag = 1.0f, r = 0.0f, bv = 0.0f;
-  uint32_t mask = 0;
-  if (water) {
-    float cs = U->gravity[3] * dt;
-    float kc = h * h / (cs * cs);
-    float div = 0.0f, off = 0.0f;
-    diag = kc;
-    bv = kc * cP[cell];
+  const float* cP = d.cf + (size_t)NATRAY_F_P * d.capc;
+  Row w = {cfr(d, NATRAY_F_TYPE, cell) == WATER, 1.0f, 0.0f, 0.0f, 0u};
+  if (!w.water) return w;
+  float cs = U->gravity[3] * dt;
+  float kc = h * h / (cs * cs);
+  float div = 0.0f, off = 0.0f;
+  w.diag = kc;
+  w.bv = kc * cP[cell];
 #pragma unroll
-    for (uint32_t dim = 0; dim < 3; ++dim) {
-      const int ex = dim == 0, ey = dim == 1, ez = dim == 2;
-      const uint32_t up = nbr(lk, lx, ly, lz, ex, ey, ez), dn = nbr(lk, lx, ly, lz, -ex, -ey, -ez);
-      div += face_u(d, up, cell, dim) - face_u(d, cell, dn, dim);
+  for (uint32_t dim = 0; dim < 3; ++dim) {
+    const int ex = dim == 0, ey = dim == 1, ez = dim == 2;
+    const uint32_t up = nbr(lk, lx, ly, lz, ex, ey, ez), dn = nbr(lk, lx, ly, lz, -ex, -ey, -ez);
+    div += face_u(d, up, cell, dim) - face_u(d, cell, dn, dim);
 #pragma unroll
-      for (int sg = -1; sg <= 1; sg += 2) {
-        const uint32_t nb = sg < 0 ? dn : up;
-        const float nt = cell_type(d, nb);
-        if (nt == SOLID) continue;
-        diag += 1.0f;
-        if (nt == WATER) {
-          off -= cP[nb];
-          mask |= 1u << (dim * 2u + (sg > 0 ? 1u : 0u));
-        } else {
-          bv += pressure_of(d, nb);
-        }
+    for (int sg = -1; sg <= 1; sg += 2) {
+      const uint32_t nb = sg < 0 ? dn : up;
+      const float nt = cell_type(d, nb);
+      if (nt == SOLID) continue;
+      w.diag += 1.0f;
+      if (nt == WATER) {
+        off -= p0[nb];
+        w.mask |= 1u << (dim * 2u + (sg > 0 ? 1u : 0u));
+      } else {
+        w.bv += pressure_of(d, nb);
       }
     }
-    div -= from_fp(accr(d, NATRAY_A_SRC, cell), NATRAY_FP_Q) / (h * h);
-    bv -= U->fluid[0] * h / dt * div;
-    r = bv - diag * cP[cell] - off;
-    d.cf[(size_t)NATRAY_F_DIAG * capc + cell] = diag;
-    d.cf[(size_t)NATRAY_F_R * capc + cell] = r;
-    d.bvec[cell] = bv;
-    mask |= 64u;
   }
-  d.nbm[cell] = mask;
-  const float z = precondition(r, diag, water, lx, ly, lz, d.sc->precond);
-  if (water) {
+  div -= from_fp(accr(d, NATRAY_A_SRC, cell), NATRAY_FP_Q) / (h * h);
+  w.bv -= U->fluid[0] * h / dt * div;
+  w.r = w.bv - w.diag * p0[cell] - off;
+  w.mask |= 64u;
+  return w;
+}
+
+// solve_init (fluid.wgsl:450-491) for one cell from p0: the operator row, b, r = b - A p0 and
+// z. Writes diag, r, b, the neighbour mask and z; in register form also the tagged z (tag
+// `ztag`). Stores the block partials (r.z, r.r, b.b, water cells) through `prow` from lane 0.
+template <class R>
+__device__ __forceinline__ void init_cell(const Dev& d, uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk,
+                                          bool regs, uint32_t ztag, double* prow, const float* p0) {
+  const uint32_t capc = d.capc;
+  const uint32_t cell = b * 64u + lane;
+  const Row w = cell_row(d, cell, lx, ly, lz, lk, p0);
+  if (w.water) {
+    d.cf[(size_t)NATRAY_F_DIAG * capc + cell] = w.diag;
+    d.cf[(size_t)NATRAY_F_R * capc + cell] = w.r;
+    d.bvec[cell] = w.bv;
+  }
+  d.nbm[cell] = w.water ? w.mask : 0u;
+  const float r = w.water ? w.r : 0.0f, bv = w.water ? w.bv : 0.0f;
+  const float z = precondition(r, w.diag, w.water, lx, ly, lz, d.sc->precond);
+  if (w.water) {
     d.cf[(size_t)NATRAY_F_Z * capc + cell] = z;
     if (regs) store_word(&d.hz[cell], tag_word(ztag, __float_as_uint(z)));
   }
   const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r)), v2 = wave_tree<R>(prod<R>(bv, bv));
-  const uint32_t nw = (uint32_t)__popcll(__ballot(water));
+  const uint32_t nw = (uint32_t)__popcll(__ballot(w.water));
   if (lane == 0) {
     prow[0] = (double)v0;
     prow[1] = (double)v1;
     prow[2] = (double)v2;
     prow[3] = (double)nw;
   }
+}
+
+// The starting pressure of this solve: the warm start's guess when it was accepted this step
+// (Dev::bar[11]), else the particle-carried pressure.
+__device__ __forceinline__ const float* start_pressure(const Dev& d) {
+  return d.bar[11] ? d.guess : d.cf + (size_t)NATRAY_F_P * d.capc;
 }
 
 // The solver scalars of a solve, kept by every workgroup alike.
@@ -1135,12 +1158,19 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
   float* const cZ = d.cf + (size_t)NATRAY_F_Z * capc;
   float* const cQ = d.cf + (size_t)NATRAY_F_Q * capc;
   float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
+  const float* const p0 = start_pressure(d);
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    init_cell<R>(d, b, lane, lx, ly, lz, lk, false, 0u, d.part + b * 4u);
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, false, 0u, d.part + b * 4u, p0);
   });
   R tt[4];
   SolveScalars<R> s;
   bool go = solve_start<R>(d, tt, reduce<R>(d, sp, ++tag, 4, nullptr, tt), s);
+  if (p0 != cP) {
+    // The iterate starts at the guess; every workgroup has read the carried pressure by now.
+    for (uint32_t c = b0 * 64u + threadIdx.x; c < b1 * 64u; c += blockDim.x)
+      if (d.nbm[c] & 64u) cP[c] = p0[c];
+    __syncthreads();
+  }
   const bool refresh_on = MIXED && d.sc->precision == NATRAY_CG_MIXED && d.sc->refresh_every > 0u;
   while (go) {
     // solve_q (fluid.wgsl:508-534): the new direction and q = A d; neighbours' directions are
@@ -1262,10 +1292,11 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
   const uint32_t b0 = sp.b0, b1 = sp.b1;
   const uint32_t capc = d.capc;
   float* const cP = d.cf + (size_t)NATRAY_F_P * capc;
+  const float* const p0 = start_pressure(d);
   unsigned long long* const hz = d.hz;
   __shared__ double lpart[SOLVE_WAVES * SOLVE_RMAX][4];
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    init_cell<R>(d, b, lane, lx, ly, lz, lk, true, tag + 1u, lpart[b - b0]);
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, true, tag + 1u, lpart[b - b0], p0);
   });
   R tt[4];
   SolveScalars<R> s;
@@ -1307,7 +1338,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
     if (b < b1) {
       const uint32_t cell = b * 64u + ln;
       rM[r] = d.nbm[cell];
-      rP[r] = cP[cell];
+      rP[r] = p0[cell];
       rR[r] = d.cf[(size_t)NATRAY_F_R * capc + cell];
       rZ[r] = d.cf[(size_t)NATRAY_F_Z * capc + cell];
       rDg[r] = d.cf[(size_t)NATRAY_F_DIAG * capc + cell];
@@ -1317,7 +1348,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
         if (!has_halo(r, a)) continue;
         const uint32_t o = halo_cell(r, a);
         hD[r][a] = d.cf[(size_t)NATRAY_F_D * capc + o];
-        if (refresh_on) hPr[r][a] = cP[o];
+        if (refresh_on) hPr[r][a] = p0[o];
         hW[r][a] = load_word(&hz[o]);
       }
     }
@@ -1454,6 +1485,207 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
     }
   }
   solve_finish<R>(d, sp.n, tag, s);
+}
+
+// ---- learned warm start of the solve ---------------------------------------------------------------
+//
+// Not in the reference, whose solve always starts from the particle-carried pressure
+// (solve_init, fluid.wgsl:450-489). Per step: r0 = b - A p_carried on every water cell; the 16^3
+// tiles of cells that hold water are listed (at most a fixed number); each listed tile's inputs
+// (r0 / s with s the tile's largest |r0|, and the cell kind 1 water / -1 solid / 0 otherwise)
+// go to the network, whose output o gives guess = p_carried + s o on the tile's water cells.
+// The solve starts from the guess when |b - A guess|^2 < |r0|^2 over all water cells (both sums
+// in FP64, fixed order), else from the carried pressure as without the warm start. b is the
+// same either way, so the solve meets the same tolerance on the same system.
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// The cell of voxel v (256 x + 16 y + z) of tile t, or NONE where no block holds it.
+__device__ __forceinline__ uint32_t tile_voxel_cell(const Dev& d, uint32_t t, uint32_t v, uint32_t nb) {
+  const uint32_t tz = t % d.tdz, ty = (t / d.tdz) % d.tdy, tx = t / (d.tdz * d.tdy);
+  const uint32_t cx = tx * 16u + (v >> 8), cy = ty * 16u + ((v >> 4) & 15u), cz = tz * 16u + (v & 15u);
+  const uint32_t bx = cx >> 2, by = cy >> 2, bz = cz >> 2;
+  const natray_params* U = d.U;
+  if (bx >= U->bdims[0] || by >= U->bdims[1] || bz >= U->bdims[2]) return NONE;
+  const uint32_t b = d.block_ptr[block_slot(U, (int)bx, (int)by, (int)bz)];
+  if (b >= nb) return NONE;
+  return b * 64u + (cx & 3u) * 16u + (cy & 3u) * 4u + (cz & 3u);
+}
+
+// r0 from the carried pressure, the carried pressure kept, the guess preset to it, and the
+// tiles with water marked with their largest |r0|.
+__global__ __launch_bounds__(256) void k_ws_prepare(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  const float* cP = d.cf + (size_t)NATRAY_F_P * d.capc;
+  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    const uint32_t cell = b * 64u + lane;
+    const Row w = cell_row(d, cell, lx, ly, lz, lk, cP);
+    const float p = cP[cell];
+    d.ws_r0[cell] = w.water ? w.r : 0.0f;
+    d.ws_pcar[cell] = p;
+    d.guess[cell] = p;
+    const uint32_t m = wave_max(w.water ? __float_as_uint(__builtin_fabsf(w.r)) + 1u : 0u);
+    if (lane == 0 && m != 0u) {
+      int bx, by, bz;
+      slot_coord(d.U, d.block_list[b], bx, by, bz);
+      const uint32_t t = ((uint32_t)(bx >> 2) * d.tdy + (uint32_t)(by >> 2)) * d.tdz + (uint32_t)(bz >> 2);
+      atomicMax(&d.tile_map[t], m);
+    }
+  });
+}
+
+// One workgroup: lists the marked tiles in tile order (the first `cap`) with their scale,
+// clears the marks, and stores the listed count in bar[10].
+__global__ __launch_bounds__(1024) void k_ws_tiles(Dev d, uint32_t cap) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t base;
+  const uint32_t T = d.tdx * d.tdy * d.tdz;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  if (threadIdx.x == 0) base = 0u;
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < T; t0 += 1024u) {
+    const uint32_t t = t0 + threadIdx.x;
+    const uint32_t m = t < T ? d.tile_map[t] : 0u;
+    const unsigned long long bal = __ballot(m != 0u);
+    if (lane == 0) wsum[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    for (uint32_t w = 0; w < wave; ++w) slot += wsum[w];
+    if (t < T) {
+      if (m != 0u && slot < cap) {
+        d.tile_list[slot] = t;
+        d.tile_scale[slot] = __uint_as_float(m - 1u);
+      }
+      d.tile_map[t] = 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t add = 0;
+      for (uint32_t w = 0; w < 16u; ++w) add += wsum[w];
+      base += add;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.bar[10] = base < cap ? base : cap;
+    d.ws_stat[4] = (double)base;
+  }
+}
+
+// The network's inputs of the listed tiles: FP32 into ws_in ([slot][voxel][2]) and BF16 into
+// the network's input buffer ([slot][voxel][8], channels 2..7 zero).
+__global__ __launch_bounds__(256) void k_ws_gather(Dev d, bf16x8* __restrict__ net_in) {
+  const uint32_t nb = n_blocks(d);
+  const size_t total = (size_t)d.bar[10] * 4096u;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t slot = (uint32_t)(i >> 12), v = (uint32_t)(i & 4095u);
+    const uint32_t cell = tile_voxel_cell(d, d.tile_list[slot], v, nb);
+    float x = 0.0f, kind = 0.0f;
+    if (cell != NONE) {
+      const float ty = cfr(d, NATRAY_F_TYPE, cell);
+      const float s = d.tile_scale[slot];
+      if (ty == WATER) {
+        kind = 1.0f;
+        x = s > 0.0f ? d.ws_r0[cell] / s : 0.0f;
+      } else if (ty == SOLID) {
+        kind = -1.0f;
+      }
+    }
+    d.ws_in[2 * i] = x;
+    d.ws_in[2 * i + 1] = kind;
+    if (net_in) {
+      bf16x8 o;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) o[j] = (__bf16)0.0f;
+      o[0] = (__bf16)x;
+      o[1] = (__bf16)kind;
+      net_in[i] = o;
+    }
+  }
+}
+
+// guess = p_carried + s o on the water cells of the listed tiles (o: channel 0 of `pred`,
+// [slot][voxel][16] FP32).
+__global__ __launch_bounds__(256) void k_ws_scatter(Dev d, const float* __restrict__ pred) {
+  const uint32_t nb = n_blocks(d);
+  const size_t total = (size_t)d.bar[10] * 4096u;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t slot = (uint32_t)(i >> 12), v = (uint32_t)(i & 4095u);
+    const uint32_t cell = tile_voxel_cell(d, d.tile_list[slot], v, nb);
+    if (cell == NONE || cfr(d, NATRAY_F_TYPE, cell) != WATER) continue;
+    d.guess[cell] = d.ws_pcar[cell] + d.tile_scale[slot] * pred[i * 16u];
+  }
+}
+
+// Per block: the squared residuals of the guess and of the carried pressure (lane tree, FP64).
+__global__ __launch_bounds__(256) void k_ws_check(Dev d) {
+  if (d.st->dt_ns == 0u) return;
+  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    const uint32_t cell = b * 64u + lane;
+    const Row w = cell_row(d, cell, lx, ly, lz, lk, d.guess);
+    const double rg = w.water ? (double)w.r : 0.0, r0 = w.water ? (double)d.ws_r0[cell] : 0.0;
+    const double sg = lane_tree<double>(rg * rg), s0 = lane_tree<double>(r0 * r0);
+    if (lane == 0) {
+      d.ws_part[2 * b] = sg;
+      d.ws_part[2 * b + 1] = s0;
+    }
+  });
+}
+
+// One workgroup: sums the block residuals in a fixed order and accepts the guess (bar[11] = 1)
+// when its residual is the smaller; keeps the sums and counts in ws_stat.
+__global__ __launch_bounds__(1024) void k_ws_decide(Dev d) {
+  __shared__ double sh[2][1024];
+  if (d.st->dt_ns == 0u) return;
+  const uint32_t n = n_blocks(d);
+  double sg = 0.0, s0 = 0.0;
+  for (uint32_t b = threadIdx.x; b < n; b += 1024u) {
+    sg += d.ws_part[2 * b];
+    s0 += d.ws_part[2 * b + 1];
+  }
+  sh[0][threadIdx.x] = sg;
+  sh[1][threadIdx.x] = s0;
+  __syncthreads();
+  for (uint32_t h = 512u; h > 0u; h >>= 1) {
+    if (threadIdx.x < h) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const bool accept = sh[0][0] < sh[1][0];
+    d.bar[11] = accept ? 1u : 0u;
+    d.ws_stat[0] += 1.0;
+    d.ws_stat[1] += accept ? 1.0 : 0.0;
+    d.ws_stat[2] = sh[0][0];
+    d.ws_stat[3] = sh[1][0];
+  }
+}
+
+// Training samples of the last solve on the listed tiles: target (p - p_carried) / s and the
+// water mask, [slot][voxel].
+__global__ __launch_bounds__(256) void k_ws_samples(Dev d) {
+  const uint32_t nb = n_blocks(d);
+  const float* cP = d.cf + (size_t)NATRAY_F_P * d.capc;
+  const size_t total = (size_t)d.bar[10] * 4096u;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t slot = (uint32_t)(i >> 12), v = (uint32_t)(i & 4095u);
+    const uint32_t cell = tile_voxel_cell(d, d.tile_list[slot], v, nb);
+    const float s = d.tile_scale[slot];
+    const bool on = cell != NONE && s > 0.0f && cfr(d, NATRAY_F_TYPE, cell) == WATER;
+    d.ws_tgt[i] = on ? (cP[cell] - d.ws_pcar[cell]) / s : 0.0f;
+    d.ws_mask[i] = on ? 1.0f : 0.0f;
+  }
 }
 
 // ---- project, extend_1, extend_2 (fluid.wgsl:580-674) ------------------------------------------------
@@ -1794,6 +2026,22 @@ int launch_project(const Dev& d, void* stream, int grid_waves) {
 int launch_extend(const Dev& d, void* stream, int grid_waves) {
   hipLaunchKernelGGL(k_extend1, dim3(grid_waves), dim3(256), 0, S(stream), d);
   hipLaunchKernelGGL(k_extend2, dim3(grid_waves), dim3(256), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_ws_inputs(const Dev& d, void* stream, int grid_waves, uint32_t cap, void* net_in) {
+  hipLaunchKernelGGL(k_ws_prepare, dim3(grid_waves), dim3(256), 0, S(stream), d);
+  hipLaunchKernelGGL(k_ws_tiles, dim3(1), dim3(1024), 0, S(stream), d, cap);
+  hipLaunchKernelGGL(k_ws_gather, dim3(1024), dim3(256), 0, S(stream), d, static_cast<bf16x8*>(net_in));
+  return (int)hipGetLastError();
+}
+int launch_ws_guess(const Dev& d, void* stream, int grid_waves, const float* pred) {
+  hipLaunchKernelGGL(k_ws_scatter, dim3(1024), dim3(256), 0, S(stream), d, pred);
+  hipLaunchKernelGGL(k_ws_check, dim3(grid_waves), dim3(256), 0, S(stream), d);
+  hipLaunchKernelGGL(k_ws_decide, dim3(1), dim3(1024), 0, S(stream), d);
+  return (int)hipGetLastError();
+}
+int launch_ws_samples(const Dev& d, void* stream) {
+  hipLaunchKernelGGL(k_ws_samples, dim3(1024), dim3(256), 0, S(stream), d);
   return (int)hipGetLastError();
 }
 int launch_g2p(const Dev& d, void* stream, int grid_blocks) {

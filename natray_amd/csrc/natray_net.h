@@ -7,10 +7,11 @@
 
 namespace natray {
 namespace net {
-// Runs the network forward on `tiles` tiles already in the net's device input buffer
-// ([tiles][4096][8] BF16, channels 0..1 used); channel 0 of the last layer's FP32 output
-// ([tiles][4096][16]) is the prediction. Returns a hipError_t as int.
-int forward_device(natray_net* n, uint32_t tiles, void* stream);
+// Runs the network forward on the tiles already in the net's device input buffer
+// ([tiles][4096][8] BF16, channels 0..1 used): min(tiles, max_tiles) of them, or fewer when
+// `tiles_dev` (device memory, read by the kernels) holds a smaller count. Channel 0 of the last
+// layer's FP32 output ([tiles][4096][16]) is the prediction. Returns a NATRAY_E_* code.
+int forward_device(natray_net* n, uint32_t tiles, const uint32_t* tiles_dev, void* stream);
 // The net's device input buffer and prediction buffer, and its tile capacity.
 void* input_buffer(natray_net* n);
 const float* output_buffer(natray_net* n);

@@ -45,19 +45,24 @@ __device__ __forceinline__ int tap_voxel(int x, int y, int z, int tap) {
 
 // Y[t][v][n] = act(bias[n] + sum_k X[t][v + off(tap)][c] W[k][n]) for n < 16, with W in the
 // MFMA B layout `wb` (ksteps(cin) x 4 groups x 16 columns x 8 values, see pack kernels).
-// One workgroup of 4 waves takes 16 z-rows of one tile (4 per wave). `yf` (FP32, 16
-// channels) or `yb` (BF16, 16 channels) receives the result.
+// One workgroup of 4 waves takes 16 z-rows of one tile at a time (4 per wave), over work items
+// (tile, x) strided by the grid. `yf` (FP32, 16 channels) or `yb` (BF16, 16 channels) receives
+// the result. The tile count is `tiles`, or min(*tiles_dev, tiles) when tiles_dev is given (a
+// count only the device knows when the work is enqueued).
 __global__ __launch_bounds__(256) void k_conv(const __bf16* __restrict__ X, int cin, const bf16x8* __restrict__ wb,
                                               const float* __restrict__ bias, int relu, __bf16* __restrict__ yb,
-                                              float* __restrict__ yf, int tiles) {
+                                              float* __restrict__ yf, int tiles, const uint32_t* __restrict__ tiles_dev) {
   __shared__ bf16x8 w_lds[ksteps(16) * 64];
+  const int nt = tiles_dev ? (int)min(*tiles_dev, (uint32_t)tiles) : tiles;
+  const int items = nt * 16;
+  if ((int)blockIdx.x >= items) return;
   const int ks = ksteps(cin);
   for (int i = threadIdx.x; i < ks * 64; i += blockDim.x) w_lds[i] = wb[i];
   __syncthreads();
-  const int t = blockIdx.x / 16, rows0 = (blockIdx.x % 16) * 16;
-  if (t >= tiles) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, col = lane & 15;
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+  const int t = item / 16, rows0 = (item % 16) * 16;
   const __bf16* Xt = X + (size_t)t * VOX * cin;
   for (int r = wave; r < 16; r += 4) {
     const int row = rows0 + r;  // z-row index = 16 x + y
@@ -85,6 +90,7 @@ __global__ __launch_bounds__(256) void k_conv(const __bf16* __restrict__ X, int 
       if (yf) yf[o] = v;
       if (yb) yb[o] = (__bf16)v;
     }
+  }
   }
 }
 
@@ -313,7 +319,10 @@ int pack(natray_net* n) {
   return NATRAY_OK;
 }
 
-int forward(natray_net* n, uint32_t tiles) {
+// Workgroups of a convolution over `tiles` tiles: one per work item up to 8 per CU.
+unsigned conv_grid(uint32_t tiles) { return (unsigned)std::min<uint32_t>(tiles * 16u, 2048u); }
+
+int forward(natray_net* n, uint32_t tiles, const uint32_t* tiles_dev = nullptr) {
   if (!n->packed) {
     int e = pack(n);
     if (e) return e;
@@ -321,8 +330,8 @@ int forward(natray_net* n, uint32_t tiles) {
   const __bf16* x = n->in;
   for (int l = 0; l < 4; ++l) {
     const Layer& L = LAYERS[l];
-    hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, n->stream, x, L.cin, n->wb[l], n->p + L.b_off, L.relu,
-                       l < 3 ? n->act[l] : nullptr, l == 3 ? n->out : nullptr, (int)tiles);
+    hipLaunchKernelGGL(k_conv, dim3(conv_grid(tiles)), dim3(256), 0, n->stream, x, L.cin, n->wb[l], n->p + L.b_off, L.relu,
+                       l < 3 ? n->act[l] : nullptr, l == 3 ? n->out : nullptr, (int)tiles, tiles_dev);
     if (l < 3) x = n->act[l];
   }
   NCHK(hipGetLastError());
@@ -347,8 +356,8 @@ int backward(natray_net* n, uint32_t tiles, float inv_count) {
     hipLaunchKernelGGL(k_bgrad, dim3((unsigned)std::min<size_t>((vox + 15) / 16, 1024)), dim3(256), 0, n->stream, n->dz,
                        n->g + L.b_off, L.cout_real, vox);
     if (l > 0)
-      hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, n->stream, n->dz, 16, n->wt[l], nullptr, 0, nullptr,
-                         n->dy, (int)tiles);
+      hipLaunchKernelGGL(k_conv, dim3(conv_grid(tiles)), dim3(256), 0, n->stream, n->dz, 16, n->wt[l], nullptr, 0, nullptr,
+                         n->dy, (int)tiles, nullptr);
   }
   NCHK(hipGetLastError());
   return NATRAY_OK;
@@ -366,10 +375,10 @@ int upload_input(natray_net* n, const float* input, uint32_t tiles) {
 
 namespace natray {
 namespace net {
-int forward_device(natray_net* n, uint32_t tiles, void* stream) {
+int forward_device(natray_net* n, uint32_t tiles, const uint32_t* tiles_dev, void* stream) {
   hipStream_t keep = n->stream;
   n->stream = static_cast<hipStream_t>(stream);
-  int e = forward(n, tiles);
+  int e = forward(n, std::min(tiles, n->max_tiles), tiles_dev);
   n->stream = keep;
   return e;
 }
@@ -546,7 +555,7 @@ int natray_net_layer_check(uint32_t cin_real, uint32_t cout_real, int relu, cons
   const int cnt = ksteps(cin) * 64, cntt = ksteps(16) * 64;
   hipLaunchKernelGGL(k_pack, dim3((cnt + 255) / 256), dim3(256), 0, 0, dWd, (int)cin_real, (int)cout_real, cin, 0, wb);
   hipLaunchKernelGGL(k_pack, dim3((cntt + 255) / 256), dim3(256), 0, 0, dWd, (int)cin_real, (int)cout_real, 16, 1, wt);
-  hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, 0, xb, cin, wb, bd, relu, yb, yf, (int)tiles);
+  hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, 0, xb, cin, wb, bd, relu, yb, yf, (int)tiles, nullptr);
   hipLaunchKernelGGL(k_take, dim3(grid), dim3(256), 0, 0, yf, (int)cout_real, stage, vox);
   if (hipMemcpy(Y, stage, vox * cout_real * 4, hipMemcpyDeviceToHost)) return done(nfail(NATRAY_E_HIP, "download Y"));
   // Backward from dY.
@@ -558,7 +567,7 @@ int natray_net_layer_check(uint32_t cin_real, uint32_t cout_real, int relu, cons
                      (int)cout_real, (int)tiles);
   hipLaunchKernelGGL(k_bgrad, dim3((unsigned)std::min<size_t>((vox + 15) / 16, 1024)), dim3(256), 0, 0, dz, dbd,
                      (int)cout_real, vox);
-  hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, 0, dz, 16, wt, nullptr, 0, nullptr, dxf, (int)tiles);
+  hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, 0, dz, 16, wt, nullptr, 0, nullptr, dxf, (int)tiles, nullptr);
   hipLaunchKernelGGL(k_take, dim3(grid), dim3(256), 0, 0, dxf, (int)cin_real, dXf, vox);
   if (hipGetLastError() != hipSuccess) return done(nfail(NATRAY_E_HIP, "layer kernels"));
   if (hipMemcpy(dX, dXf, vox * cin_real * 4, hipMemcpyDeviceToHost) || hipMemcpy(dW, dWd, nw * 4, hipMemcpyDeviceToHost) ||

@@ -16,6 +16,7 @@
 
 #include "natray.h"
 #include "natray_dev.h"
+#include "natray_net.h"
 
 namespace {
 
@@ -71,6 +72,10 @@ struct natray_sim {
   bool use_graph = true;
   bool capturing = false;  // no timing events go into a captured graph
   hipGraphExec_t graph_pre = nullptr, graph_post = nullptr;
+  // Learned warm start of the solve (natray_sim_set_warm_start).
+  int ws_mode = 0;
+  natray_net* ws_net = nullptr;
+  bool ws_ready = false;  // its buffers are allocated
   // Per-phase timing.
   bool prof = false;
   std::vector<hipEvent_t> events;
@@ -149,6 +154,56 @@ uint64_t solve_tag_bound(const natray_sim* s) {
   return 3ull * std::min<uint64_t>(s->U.limits[2], 1ull << 24) + 4ull;
 }
 
+// The warm start before a solve: the network's inputs, and in mode 2 the network and the guess
+// the solve starts from when it lowers the residual.
+int enqueue_warm_start(natray_sim* s) {
+  const natray::Dev& D = s->D;
+  void* st = s->stream;
+  const bool guess = s->ws_mode == 2;
+  const uint32_t cap = guess ? std::min(D.ws_max, natray::net::max_tiles(s->ws_net)) : D.ws_max;
+  LAUNCHCHK(natray::launch_ws_inputs(D, st, s->grid_waves, cap, guess ? natray::net::input_buffer(s->ws_net) : nullptr));
+  if (!guess) return NATRAY_OK;
+  int e = natray::net::forward_device(s->ws_net, cap, D.bar + 10, st);
+  if (e) return fail(e, std::string("warm-start network: ") + natray_net_last_error());
+  LAUNCHCHK(natray::launch_ws_guess(D, st, s->grid_waves, natray::net::output_buffer(s->ws_net)));
+  return NATRAY_OK;
+}
+
+// The warm start's buffers, allocated on first use; the device copy of Dev follows and the
+// step graphs are captured again with it.
+int alloc_warm_start(natray_sim* s) {
+  natray::Dev& D = s->D;
+  D.tdx = (s->U.bdims[0] + 3u) / 4u;
+  D.tdy = (s->U.bdims[1] + 3u) / 4u;
+  D.tdz = (s->U.bdims[2] + 3u) / 4u;
+  const size_t T = (size_t)D.tdx * D.tdy * D.tdz;
+  // Tiles that can hold water: every tile of the domain, and no more than 64 per block
+  // of capacity (a block lies in one tile).
+  D.ws_max = (uint32_t)std::min<size_t>(T, D.cap);
+  const size_t capc = D.capc, W = D.ws_max;
+  int e;
+  if ((e = dalloc(s, &D.guess, capc)) || (e = dalloc(s, &D.ws_r0, capc)) || (e = dalloc(s, &D.ws_pcar, capc)) ||
+      (e = dalloc(s, &D.tile_map, T)) || (e = dalloc(s, &D.tile_list, W)) ||
+      (e = dalloc(s, &D.tile_scale, W)) || (e = dalloc(s, &D.ws_in, W * 4096 * 2)) || (e = dalloc(s, &D.ws_tgt, W * 4096)) ||
+      (e = dalloc(s, &D.ws_mask, W * 4096)) || (e = dalloc(s, &D.ws_part, (size_t)D.cap * 2)) ||
+      (e = dalloc(s, &D.ws_stat, 8)))
+    return e;
+  HIPCHK(hipMemsetAsync(D.guess, 0, capc * 4, s->stream));
+  HIPCHK(hipMemsetAsync(D.ws_r0, 0, capc * 4, s->stream));
+  HIPCHK(hipMemsetAsync(D.ws_pcar, 0, capc * 4, s->stream));
+  HIPCHK(hipMemsetAsync(D.tile_map, 0, T * 4, s->stream));
+  HIPCHK(hipMemsetAsync(D.tile_list, 0, W * 4, s->stream));
+  HIPCHK(hipMemsetAsync(D.tile_scale, 0, W * 4, s->stream));
+  HIPCHK(hipMemsetAsync(D.ws_part, 0, (size_t)D.cap * 2 * 8, s->stream));
+  HIPCHK(hipMemcpyAsync(const_cast<natray::Dev*>(D.self), &D, sizeof(natray::Dev), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
+  if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+  s->graph_pre = s->graph_post = nullptr;
+  s->ws_ready = true;
+  return NATRAY_OK;
+}
+
 int enqueue_phase(natray_sim* s, int phase) {
   const natray::Dev& D = s->D;
   void* st = s->stream;
@@ -172,6 +227,10 @@ int enqueue_phase(natray_sim* s, int phase) {
         if (e) return e;
       }
       s->tags_used += bound;
+      if (s->ws_mode) {
+        int e = enqueue_warm_start(s);
+        if (e) return e;
+      }
       LAUNCHCHK(natray::launch_solve(D, st, solve_grid_for(s), s->mixed));
       break;
     }
@@ -647,6 +706,60 @@ int natray_sim_phase_times(natray_sim* s, double* ms, uint32_t* counts) {
     if (ms) ms[k] = s->ms[k];
     if (counts) counts[k] = s->cnt[k];
   }
+  return NATRAY_OK;
+}
+
+int natray_sim_set_warm_start(natray_sim* s, natray_net* net, int mode) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  if (mode < 0 || mode > 2) return fail(NATRAY_E_ARG, "warm-start mode must be 0, 1 or 2");
+  if (mode == 2 && !net) return fail(NATRAY_E_ARG, "warm-start mode 2 needs a network");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (mode && !s->ws_ready) {
+    int e = alloc_warm_start(s);
+    if (e) return e;
+  }
+  s->ws_mode = mode;
+  s->ws_net = mode == 2 ? net : nullptr;
+  if (s->ws_ready) {
+    HIPCHK(hipMemsetAsync(s->D.ws_stat, 0, 8 * 8, s->stream));
+    HIPCHK(hipMemsetAsync(s->D.bar + 10, 0, 2 * 4, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return NATRAY_OK;
+}
+
+int natray_sim_ws_samples(natray_sim* s, float* inputs, float* targets, float* masks, uint32_t max_tiles, uint32_t* tiles) {
+  if (!s || !tiles) return fail(NATRAY_E_ARG, "null argument");
+  if (!s->ws_ready) return fail(NATRAY_E_ARG, "the warm start was never enabled");
+  LAUNCHCHK(natray::launch_ws_samples(s->D, s->stream));
+  HIPCHK(hipMemcpyAsync(s->h_words + 10, s->D.bar + 10, 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const uint32_t n = std::min(s->h_words[10], max_tiles);
+  const size_t vox = (size_t)n * 4096;
+  if (inputs && n) HIPCHK(hipMemcpyAsync(inputs, s->D.ws_in, vox * 2 * 4, hipMemcpyDeviceToHost, s->stream));
+  if (targets && n) HIPCHK(hipMemcpyAsync(targets, s->D.ws_tgt, vox * 4, hipMemcpyDeviceToHost, s->stream));
+  if (masks && n) HIPCHK(hipMemcpyAsync(masks, s->D.ws_mask, vox * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *tiles = n;
+  return NATRAY_OK;
+}
+
+int natray_sim_ws_stats(natray_sim* s, double out[8]) {
+  if (!s || !out) return fail(NATRAY_E_ARG, "null argument");
+  for (int k = 0; k < 8; ++k) out[k] = 0.0;
+  if (!s->ws_ready) return NATRAY_OK;
+  HIPCHK(hipMemcpyAsync(out, s->D.ws_stat, 8 * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_ws_guess(natray_sim* s, float* guess, uint32_t* accepted) {
+  if (!s || !guess || !accepted) return fail(NATRAY_E_ARG, "null argument");
+  if (!s->ws_ready) return fail(NATRAY_E_ARG, "the warm start was never enabled");
+  HIPCHK(hipMemcpyAsync(guess, s->D.guess, (size_t)s->D.capc * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(s->h_words + 11, s->D.bar + 11, 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *accepted = s->h_words[11];
   return NATRAY_OK;
 }
 

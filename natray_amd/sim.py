@@ -157,5 +157,37 @@ class Sim:
         keys = ["steps", "cg_iters", "cg_cell_iters", "blocks", "water_cells", "particles", "blocks_cleared"]
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
+    def set_warm_start(self, net=None, mode: int = 2) -> None:
+        """The learned warm start of the solve (natray_sim_set_warm_start): mode 0 off, 1 inputs
+        only (for ws_samples), 2 the network's guess with fallback to the carried pressure."""
+        self._ws_net = net if mode == 2 else None  # the network must outlive its use
+        h = net.handle if (net is not None and mode == 2) else None
+        abi.check(abi.lib().natray_sim_set_warm_start(self._h, h, mode))
+
+    def ws_samples(self, max_tiles: int = 1 << 16):
+        """Training samples of the last step: (inputs [T][4096][2], targets [T][4096], masks [T][4096])."""
+        n = C.c_uint32()
+        abi.check(abi.lib().natray_sim_ws_samples(self._h, None, None, None, max_tiles, C.byref(n)))
+        t = int(n.value)
+        x = np.empty((t, 4096, 2), np.float32)
+        y = np.empty((t, 4096), np.float32)
+        m = np.empty((t, 4096), np.float32)
+        abi.check(abi.lib().natray_sim_ws_samples(self._h, abi.ptr(x), abi.ptr(y), abi.ptr(m), t, C.byref(n)))
+        assert int(n.value) == t
+        return x, y, m
+
+    def ws_stats(self) -> dict:
+        out = np.zeros(8, np.float64)
+        abi.check(abi.lib().natray_sim_ws_stats(self._h, abi.ptr(out)))
+        return {"offered": int(out[0]), "accepted": int(out[1]), "guess_rr": float(out[2]), "carried_rr": float(out[3]),
+                "tiles": int(out[4])}
+
+    def ws_guess(self):
+        """(guess over bdims.w * 64 cells, whether the last solve started from it)."""
+        g = np.empty(int(self.params.bdims[3]) * 64, np.float32)
+        acc = C.c_uint32()
+        abi.check(abi.lib().natray_sim_ws_guess(self._h, abi.ptr(g), C.byref(acc)))
+        return g, bool(acc.value)
+
     def stream_ptr(self) -> int:
         return int(abi.lib().natray_sim_device_ptr(self._h, abi.BUF_STREAM) or 0)

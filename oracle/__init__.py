@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
                 "solver_scalars": (None, [P, P]),
                 "precondition": (None, [P, P, P, P, C.c_uint32]),
                 "set_threads": (None, [C.c_int]),
+                "set_guess": (None, [P, P]),
+                "residual": (None, [P, P, P]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -169,6 +171,26 @@ class Oracle:
         so = np.empty(max(self.n, 1), np.uint32)
         self._f("read_bins")(self._h, abi.ptr(bp), abi.ptr(so))
         return bp, so[: self.n]
+
+    def set_guess(self, guess: np.ndarray | None) -> None:
+        """The next solve starts from `guess` (bdims.w * 64 cells) on the water cells."""
+        g = None
+        if guess is not None:
+            g = np.ascontiguousarray(guess, dtype=np.float32)
+            assert g.size == int(self.params.bdims[3]) * 64
+        self._f("set_guess")(self._h, abi.ptr(g))
+
+    def residual(self, p: np.ndarray | None = None) -> np.ndarray:
+        """b - A p on the cells in use (0 off water) for the grid as it stands before a solve;
+        p = None is the particle-carried pressure."""
+        n = self.n_blocks()
+        r = np.zeros(max(n, 1) * 64, np.float32)
+        q = None
+        if p is not None:
+            q = np.ascontiguousarray(p, dtype=np.float32)
+            assert q.size == int(self.params.bdims[3]) * 64
+        self._f("residual")(self._h, abi.ptr(q), abi.ptr(r))
+        return r[: 64 * n]
 
     def solver_scalars(self) -> dict:
         out = np.zeros(8, np.float64)

@@ -94,6 +94,8 @@ struct osim {
   REAL* bvec;            /* cap x 64: right-hand side kept for the true-residual refresh */
   double* part;          /* cap x 4: per-block partial sums */
   uint64_t cg_iters;
+  REAL* guess;           /* cap x 64: a starting pressure for the next solve (set_guess) */
+  int use_guess;
 };
 
 /* ---- Params / State accessors (sim_defs.wgsl:159-231) -------------------------------- */
@@ -604,9 +606,44 @@ static inline double prod(const osim* s, REAL a, REAL b) { return s->mixed ? (do
 
 /* ---- solve_init + solve_start (fluid.wgsl:445-506) ---------------------------------------- */
 
+/* The solve's row of water cell `cell` (solve_init, fluid.wgsl:450-489): the diagonal and the
+ * right-hand side b, both from the particle-carried pressure, and the return value r = b - A p0
+ * for the starting pressure p0 (the carried pressure, or a warm-start guess; the reference
+ * always starts from the carried pressure). */
+static REAL solve_row(const osim* s, uint32_t cell, const REAL* p0, REAL* dg_out, REAL* b_out) {
+  REAL h = hh_(s), dt = dt_(s);
+  REAL cs = (REAL)s->U.gravity[3] * dt;
+  REAL kc = h * h / (cs * cs);
+  REAL div = 0, off = 0, dg = kc, b = kc * cfv(s, NATRAY_F_P, cell);
+  for (uint32_t d = 0; d < 3; ++d) {
+    int e[3] = {0, 0, 0};
+    e[d] = 1;
+    uint32_t up = neighbour(s, cell, e[0], e[1], e[2]);
+    uint32_t dn = neighbour(s, cell, -e[0], -e[1], -e[2]);
+    div += face_u(s, up, cell, d) - face_u(s, cell, dn, d);
+    for (int sg = -1; sg <= 1; sg += 2) {
+      uint32_t nbc = neighbour(s, cell, sg * e[0], sg * e[1], sg * e[2]);
+      REAL nt = cell_type(s, nbc);
+      if (nt == SOLID) continue;
+      dg += (REAL)1;
+      if (nt == WATER) {
+        off -= p0[nbc];
+      } else {
+        b += pressure_of(s, nbc);
+      }
+    }
+  }
+  div -= from_fp(accv(s, NATRAY_A_SRC, cell), (REAL)NATRAY_FP_Q) / (h * h);
+  b -= (REAL)s->U.fluid[0] * h / dt * div;
+  *dg_out = dg;
+  *b_out = b;
+  return b - dg * p0[cell] - off;
+}
+
 static void o_solve_init(osim* s) {
   uint32_t nb = n_blocks_(s);
-  REAL h = hh_(s), dt = dt_(s);
+  REAL* P = s->cf + (size_t)NATRAY_F_P * cap_cells(s);
+  const REAL* p0 = s->use_guess ? s->guess : P;
 #pragma omp parallel for schedule(static)
   for (uint32_t blk = 0; blk < nb; ++blk) {
     REAL r[64], diag[64], bb[64], z[64];
@@ -618,30 +655,8 @@ static void o_solve_init(osim* s) {
       r[li] = 0;
       bb[li] = 0;
       if (!water[li]) continue;
-      REAL cs = (REAL)s->U.gravity[3] * dt;
-      REAL kc = h * h / (cs * cs);
-      REAL div = 0, off = 0, dg = kc, b = kc * cfv(s, NATRAY_F_P, cell);
-      for (uint32_t d = 0; d < 3; ++d) {
-        int e[3] = {0, 0, 0};
-        e[d] = 1;
-        uint32_t up = neighbour(s, cell, e[0], e[1], e[2]);
-        uint32_t dn = neighbour(s, cell, -e[0], -e[1], -e[2]);
-        div += face_u(s, up, cell, d) - face_u(s, cell, dn, d);
-        for (int sg = -1; sg <= 1; sg += 2) {
-          uint32_t nbc = neighbour(s, cell, sg * e[0], sg * e[1], sg * e[2]);
-          REAL nt = cell_type(s, nbc);
-          if (nt == SOLID) continue;
-          dg += (REAL)1;
-          if (nt == WATER) {
-            off -= cfv(s, NATRAY_F_P, nbc);
-          } else {
-            b += pressure_of(s, nbc);
-          }
-        }
-      }
-      div -= from_fp(accv(s, NATRAY_A_SRC, cell), (REAL)NATRAY_FP_Q) / (h * h);
-      b -= (REAL)s->U.fluid[0] * h / dt * div;
-      REAL rr = b - dg * cfv(s, NATRAY_F_P, cell) - off;
+      REAL dg, b;
+      REAL rr = solve_row(s, cell, p0, &dg, &b);
       *cfp(s, NATRAY_F_DIAG, cell) = dg;
       *cfp(s, NATRAY_F_R, cell) = rr;
       s->bvec[cell] = b;
@@ -661,6 +676,11 @@ static void o_solve_init(osim* s) {
     s->part[blk * 4 + 1] = tree64(v1, s->mixed);
     s->part[blk * 4 + 2] = tree64(v2, s->mixed);
     s->part[blk * 4 + 3] = 0;
+  }
+  if (p0 != P) {
+    /* the iterate starts at the guess */
+    for (uint32_t cell = 0; cell < nb * 64u; ++cell)
+      if (cfv(s, NATRAY_F_TYPE, cell) == WATER) P[cell] = p0[cell];
   }
 }
 
@@ -814,6 +834,7 @@ static int o_solve_beta(osim* s) {
 static void o_solve(osim* s) {
   if (s->st.dt_ns == 0u) return;
   o_solve_init(s);
+  s->use_guess = 0;
   int go = o_solve_start(s) && s->U.limits[2] > 0u;
   while (go) {
     o_solve_q(s);
@@ -1190,6 +1211,7 @@ void F(destroy)(osim* s) {
   free(s->cf);
   free(s->bvec);
   free(s->part);
+  free(s->guess);
   free(s);
 }
 
@@ -1268,6 +1290,32 @@ void F(precondition)(const float* r, const float* diag, const int32_t* water, fl
   }
   precondition(&tmp, rr, dd, ww, zz);
   for (int i = 0; i < 64; ++i) z[i] = (float)zz[i];
+}
+
+void F(set_guess)(osim* s, const float* guess) {
+  s->use_guess = 0;
+  if (!guess) return;
+  size_t nc = (size_t)cap_cells(s);
+  if (!s->guess) s->guess = (REAL*)calloc(nc, sizeof(REAL));
+  if (!s->guess) return;
+  for (size_t i = 0; i < nc; ++i) s->guess[i] = (REAL)guess[i];
+  s->use_guess = 1;
+}
+
+void F(residual)(osim* s, const float* p, float* r) {
+  uint32_t nc = n_blocks_(s) * 64u;
+  REAL* P = s->cf + (size_t)NATRAY_F_P * cap_cells(s);
+  REAL* q = P;
+  if (p) {
+    q = (REAL*)malloc((size_t)cap_cells(s) * sizeof(REAL));
+    if (!q) return;
+    for (size_t i = 0; i < (size_t)cap_cells(s); ++i) q[i] = (REAL)p[i];
+  }
+  for (uint32_t cell = 0; cell < nc; ++cell) {
+    REAL dg, b;
+    r[cell] = cfv(s, NATRAY_F_TYPE, cell) == WATER ? (float)solve_row(s, cell, q, &dg, &b) : 0.0f;
+  }
+  if (q != P) free(q);
 }
 
 void F(set_threads)(int n) {

@@ -39,6 +39,8 @@ typedef struct osim osim;
   void P##read_bins(const osim* s, uint32_t* block_ptr, uint32_t* sorted);                          \
   void P##solver_scalars(const osim* s, double out[8]);                                             \
   void P##precondition(const float* r, const float* diag, const int32_t* water, float* z, uint32_t mode); \
+  void P##set_guess(osim* s, const float* guess);                                                   \
+  void P##residual(osim* s, const float* p, float* r);                                              \
   void P##set_threads(int n);
 
 ORACLE_DECL(o32_)

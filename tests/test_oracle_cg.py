@@ -248,3 +248,68 @@ def test_jacobi_is_the_inverse_diagonal():
     r = np.random.default_rng(1).normal(size=64).astype(np.float32)
     z = oracle.precondition(r, diag, water, abi.PRECOND_JACOBI)
     np.testing.assert_array_equal(z, np.where(water, r / diag, 0).astype(np.float32))
+
+
+def cells_from_water_rows(s, g, wc, base, vals):
+    """`base` (cell order) with the water rows `vals` (wc order) written in."""
+    cc = cell_coords(g, s.params.bdims)
+    index = {tuple(c): i for i, c in enumerate(cc.tolist())}
+    out = base.copy()
+    for c, v in zip(wc.tolist(), vals):
+        out[index[tuple(c)]] = v
+    return out
+
+
+def test_residual_of_any_pressure_matches_independent_assembly():
+    """b - A p, which the warm start uses to judge its guess, for the carried pressure and for an
+    arbitrary one, against the float64 assembly."""
+    s, o, g, A, b, p0, wc, kc, dt = solve_setup()
+    cap = int(s.params.bdims[3]) * 64
+    nc = 64 * g["n_blocks"]
+    tol = 2e-6 * np.abs(b).max() + 1e-3
+    np.testing.assert_allclose(water_rows(s, g, wc, o.residual()), b - A @ p0, atol=tol)
+    rng = np.random.default_rng(3)
+    p = np.zeros(cap, np.float32)
+    p[:nc] = g["cf"][abi.F_P] + (rng.standard_normal(nc) * 300.0).astype(np.float32)
+    pw = water_rows(s, g, wc, p[:nc]).astype(np.float64)
+    np.testing.assert_allclose(water_rows(s, g, wc, o.residual(p)), b - A @ pw, atol=tol)
+
+
+def test_solve_from_a_guess():
+    """A guess changes where the solve starts, not the system: the carried pressure given as a
+    guess changes no bit, and the direct solution given as a guess needs at most a couple of
+    iterations and ends at that solution."""
+    s, o, g, A, b, p0, wc, kc, dt = solve_setup()
+    s2, o2, g2, *_ = solve_setup()
+    cap = int(s.params.bdims[3]) * 64
+    nc = 64 * g["n_blocks"]
+    carried = np.zeros(cap, np.float32)
+    carried[:nc] = g["cf"][abi.F_P]
+    o2.set_guess(carried)
+    o.phase(abi.PHASE_SOLVE)
+    o2.phase(abi.PHASE_SOLVE)
+    plain_iters = o.state().solve_iters
+    assert plain_iters > 5
+    assert o2.state().solve_iters == plain_iters
+    np.testing.assert_array_equal(o2.grid()["cf"], o.grid()["cf"])
+
+    s3, o3, g3, *_ = solve_setup()
+    p_star = spla.spsolve(A.tocsc(), b)
+    guess = carried.copy()
+    guess[:nc] = cells_from_water_rows(s3, g3, wc, g3["cf"][abi.F_P], p_star.astype(np.float32))
+    o3.set_guess(guess)
+    o3.phase(abi.PHASE_SOLVE)
+    st = o3.state()
+    assert st.solve_iters <= 2, st.solve_iters
+    p = water_rows(s3, o3.grid(), wc, o3.grid()["cf"][abi.F_P]).astype(np.float64)
+    np.testing.assert_allclose(p, p_star, rtol=0, atol=2e-4 * np.abs(p_star).max() + 1e-3)
+    # The guess is used once: the next solve starts from the carried pressure again.
+    for ph in (abi.PHASE_PROJECT, abi.PHASE_EXTEND, abi.PHASE_G2P, abi.PHASE_BEGIN, abi.PHASE_BIN, abi.PHASE_P2G,
+               abi.PHASE_GRID):
+        o3.phase(ph)
+    r_carried = o3.residual()
+    o3.phase(abi.PHASE_SOLVE)
+    g4 = o3.grid()
+    water = g4["cf"][abi.F_TYPE] == abi.WATER
+    assert st.solve_iters <= 2 < o3.state().solve_iters
+    assert np.isfinite(r_carried[water]).all()
