@@ -137,6 +137,80 @@ def compare_precision(scene, args, plain: dict) -> dict:
     }
 
 
+def warm_start_report(scene, args) -> dict:
+    """The learned warm start on this workload: training tiles logged from seeded runs of the
+    same configuration (other seeds), the network trained on the GPU by its own kernels on the
+    residual its guess leaves, then the timed steps again from the same initial state with the
+    network's guess as the solve's start (plain CG to the same tolerance, the carried pressure
+    when the guess does not lower the residual), against the run without it."""
+    from natray_amd import Sim, scenes
+    from natray_amd import net as netmod
+    from natray_amd import warm_start as ws
+
+    t0 = time.perf_counter()
+    parts, seeds = [], []
+    tiles = 0
+    for seed in range(11, 11 + 8):
+        if tiles >= args.ws_tiles:
+            break
+        sc = scenes.balloon_impact(seed=seed, iters=args.iters, precision=scene.scene.precision)
+        sim = Sim.from_scene(sc)
+        d = ws.collect(sim, args.ws_collect_steps, iters=args.iters)
+        sim.close()
+        parts.append(d)
+        seeds.append(seed)
+        tiles += d["inputs"].shape[0]
+    data = {k: np.concatenate([p[k] for p in parts]) for k in ws.KEYS}
+    del parts
+    t_collect = time.perf_counter() - t0
+    T = data["inputs"].shape[0]
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(T)
+    nval = max(T // 20, 1)
+    val = ws.subset(data, np.sort(perm[:nval]))
+    train = ws.subset(data, np.sort(perm[nval:]))
+    del data
+    net = netmod.Net(max_tiles=1024, seed=1)
+    t0 = time.perf_counter()
+    losses = ws.train(net, train, args.ws_train_steps, batch=32, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    del train
+    val_ratio = ws.residual_ratio(net, val, 64)
+    del val
+
+    def run(with_net: bool) -> dict:
+        sim = Sim.from_scene(scene)
+        if with_net:
+            sim.set_warm_start(net, 2)
+        wall, c, times, st = timed_run(sim, args.warmup, args.ws_steps, args.iters)
+        stats = sim.ws_stats() if with_net else None
+        sim.close()
+        steps = max(c["steps"], 1)
+        r = {"cg_iters_per_step": c["cg_iters"] / steps, "ms_per_step": wall * 1e3 / max(args.ws_steps, 1),
+             "solve_ms_per_step": times["solve"][0] / steps, "unconverged_steps": st.unconverged}
+        if stats:
+            r["guesses_taken"] = stats["accepted"]
+            r["guesses_offered"] = stats["offered"]
+            r["tiles_last_step"] = stats["tiles"]
+            r["guess_over_carried_residual_sq_last_step"] = (stats["guess_rr"] / stats["carried_rr"]
+                                                           if stats["carried_rr"] > 0 else None)
+        return r
+
+    plain, learned = run(False), run(True)
+    return {
+        "training": {"tiles": T, "seeds": seeds, "steps_per_seed": args.ws_collect_steps,
+                     "collect_s": t_collect, "adam_steps": args.ws_train_steps, "batch_tiles": 32,
+                     "train_s": t_train, "final_loss": float(np.mean(losses[-100:])),
+                     "val_tile_residual_ratio": val_ratio},
+        "plain": plain,
+        "warm_started": learned,
+        "cg_iters_saved_per_step": plain["cg_iters_per_step"] - learned["cg_iters_per_step"],
+        "substep_time_ratio_warm_over_plain": learned["ms_per_step"] / plain["ms_per_step"],
+        "solve_time_ratio_warm_over_plain": learned["solve_ms_per_step"] / plain["solve_ms_per_step"],
+        "note": "solve time includes the warm start (inputs, network, guess check); tol and b are the same",
+    }
+
+
 def _cells_by_slot(g: dict) -> dict:
     return {int(slot): i for i, slot in enumerate(g["block_list"])}
 
@@ -275,6 +349,11 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error report against the FP64 oracle")
     ap.add_argument("--accuracy-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start report")
+    ap.add_argument("--ws-tiles", type=int, default=50000, help="training tiles to log for the warm start")
+    ap.add_argument("--ws-collect-steps", type=int, default=100, help="steps logged per training seed")
+    ap.add_argument("--ws-train-steps", type=int, default=6000)
+    ap.add_argument("--ws-steps", type=int, default=100, help="timed steps with and without the warm start")
     ap.add_argument("--write-scene", metavar="PATH",
                     help="write the workload's scene file for natray_amd/natray_drive and exit")
     args = ap.parse_args()
@@ -362,6 +441,8 @@ def main() -> int:
             scene, args, {"cg_iters_per_step": out["cg_iters_per_step"],
                           "solve_ms_per_step": times["solve"][0] / max(steps_done, 1),
                           "ms_per_step": out["ms_per_step"], "unconverged_steps": st.unconverged})
+    if args.workload == "impact" and not args.no_warm_start:
+        out["learned_warm_start"] = warm_start_report(scene, args)
     if args.workload == "impact" and not args.no_accuracy:
         out["accuracy_vs_fp64_oracle"] = accuracy_vs_fp64(scene, (1, args.accuracy_steps), args.iters)
     if not args.no_cpu:

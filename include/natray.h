@@ -313,6 +313,13 @@ int natray_net_forward(natray_net* net, const float* input, uint32_t tiles, floa
  * the loss before the step. */
 int natray_net_train_step(natray_net* net, const float* input, const float* target, const float* mask, uint32_t tiles,
                           float lr, float* loss);
+/* One Adam step on the scaled residual the guess leaves: loss = mean over water voxels of
+ * (x_v - (A o)_v)^2, x = input channel 0, o the output, (A o)_v = diag_v o_v - the sum of o
+ * over v's water neighbours inside the tile. diag[tiles][4096] is the solve's diagonal (0 off
+ * water), nbm[tiles][4096] its water-neighbour bits (bit 2 axis for the - side, 2 axis + 1 for
+ * the + side; axes x, y, z), both as natray_sim_ws_samples gives them. */
+int natray_net_train_step_residual(natray_net* net, const float* input, const float* diag, const uint32_t* nbm,
+                                   uint32_t tiles, float lr, float* loss);
 /* The parameter gradients of the last train step. */
 int natray_net_grads(natray_net* net, float* out);
 /* One convolution layer alone, for gradient checks: Y = act(conv(X, W) + b) and, from dY,
@@ -337,10 +344,12 @@ const char* natray_net_last_error(void);
 int natray_sim_set_warm_start(natray_sim* sim, natray_net* net, int mode);
 /* Training samples of the last step's listed tiles (mode 1 or 2), at most max_tiles of them:
  * inputs[tiles][4096][2] as the network got them, targets[tiles][4096] = (p_solved -
- * p_carried) / s and masks[tiles][4096] = 1 on water cells; *tiles gets the count. Any of the
- * three arrays may be NULL. */
-int natray_sim_ws_samples(natray_sim* sim, float* inputs, float* targets, float* masks, uint32_t max_tiles,
-                          uint32_t* tiles);
+ * p_carried) / s and masks[tiles][4096] = 1 on water cells (0 in a tile whose s is 0), and the
+ * solve's operator: diag[tiles][4096] (0 off those cells) and nbm[tiles][4096], the bits of the
+ * water neighbours (bit 2 axis for the - side, 2 axis + 1 for the + side). *tiles gets the
+ * count. Any of the arrays may be NULL. */
+int natray_sim_ws_samples(natray_sim* sim, float* inputs, float* targets, float* masks, float* diag, uint32_t* nbm,
+                          uint32_t max_tiles, uint32_t* tiles);
 /* Since the mode was set: [0] solves offered a guess, [1] guesses taken, [2] |b - A guess|^2
  * and [3] |r0|^2 of the last step, [4] tiles with water in the last step (also those beyond
  * the network's capacity), [5..7] 0. */

@@ -71,6 +71,8 @@ struct Dev {
   float* ws_in;        // ws_max x 4096 x 2: the network's inputs
   float* ws_tgt;       // ws_max x 4096: training targets of the last solve (see natray_sim_ws_samples)
   float* ws_mask;      // ws_max x 4096: their water mask
+  float* ws_diag;      // ws_max x 4096: the solve's diagonal there (0 where the mask is 0)
+  uint32_t* ws_nbm;    // ws_max x 4096: its water-neighbour bits
   double* ws_part;     // cap x 2: per-block squared residuals of the guess and the carried pressure
   double* ws_stat;     // 8: see natray_sim_ws_stats
   uint32_t tdx, tdy, tdz, ws_max;

@@ -1672,8 +1672,8 @@ __global__ __launch_bounds__(1024) void k_ws_decide(Dev d) {
   }
 }
 
-// Training samples of the last solve on the listed tiles: target (p - p_carried) / s and the
-// water mask, [slot][voxel].
+// Training samples of the last solve on the listed tiles, [slot][voxel]: target
+// (p - p_carried) / s, the water mask, and the operator's diagonal and water-neighbour bits.
 __global__ __launch_bounds__(256) void k_ws_samples(Dev d) {
   const uint32_t nb = n_blocks(d);
   const float* cP = d.cf + (size_t)NATRAY_F_P * d.capc;
@@ -1685,6 +1685,8 @@ __global__ __launch_bounds__(256) void k_ws_samples(Dev d) {
     const bool on = cell != NONE && s > 0.0f && cfr(d, NATRAY_F_TYPE, cell) == WATER;
     d.ws_tgt[i] = on ? (cP[cell] - d.ws_pcar[cell]) / s : 0.0f;
     d.ws_mask[i] = on ? 1.0f : 0.0f;
+    d.ws_diag[i] = on ? cfr(d, NATRAY_F_DIAG, cell) : 0.0f;
+    d.ws_nbm[i] = on ? (d.nbm[cell] & 63u) : 0u;
   }
 }
 

@@ -196,6 +196,46 @@ __global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ 
   if ((threadIdx.x & 63) == 0) atomicAdd(loss, s * inv_count);
 }
 
+// The scaled residual of a guess on each tile: e_v = x_v - (A o)_v on water voxels (diag_v > 0),
+// with (A o)_v = diag_v o_v - sum of o_u over the water neighbours u of v inside the tile (bit
+// 2 axis + 1 of nbm for the + side, 2 axis for the - side; axes x, y, z are voxel strides 256,
+// 16, 1); x is input channel 0 (FP32, [voxel][2]) and o channel 0 of `pred`. Writes e and adds
+// the mean of e^2 over the water voxels to loss[0].
+__device__ __forceinline__ float tile_stencil(const float* __restrict__ f, int fstride, size_t v, uint32_t bits) {
+  const int z = (int)(v & 15), y = (int)((v >> 4) & 15), x = (int)((v >> 8) & 15);
+  float s = 0.0f;
+  if ((bits & 1u) && x > 0) s += f[(v - 256) * fstride];
+  if ((bits & 2u) && x < 15) s += f[(v + 256) * fstride];
+  if ((bits & 4u) && y > 0) s += f[(v - 16) * fstride];
+  if ((bits & 8u) && y < 15) s += f[(v + 16) * fstride];
+  if ((bits & 16u) && z > 0) s += f[(v - 1) * fstride];
+  if ((bits & 32u) && z < 15) s += f[(v + 1) * fstride];
+  return s;
+}
+__global__ void k_res_e(const float* __restrict__ pred, const float* __restrict__ x, const float* __restrict__ diag,
+                        const uint32_t* __restrict__ nbm, float inv_count, float* __restrict__ e, float* __restrict__ loss,
+                        size_t voxels) {
+  float s = 0.0f;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < voxels; v += (size_t)gridDim.x * blockDim.x) {
+    float ev = 0.0f;
+    if (diag[v] > 0.0f) ev = x[v * 2] - (diag[v] * pred[v * 16] - tile_stencil(pred, 16, v, nbm[v]));
+    e[v] = ev;
+    s += ev * ev;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, s * inv_count);
+}
+// d loss / d o = -2 A e / count on water voxels (A is symmetric), on channel 0 of dpred.
+__global__ void k_res_grad(const float* __restrict__ e, const float* __restrict__ diag, const uint32_t* __restrict__ nbm,
+                           float inv_count, float* __restrict__ dpred, size_t voxels) {
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < voxels; v += (size_t)gridDim.x * blockDim.x) {
+    float g = 0.0f;
+    if (diag[v] > 0.0f) g = -2.0f * inv_count * (diag[v] * e[v] - tile_stencil(e, 1, v, nbm[v]));
+#pragma unroll
+    for (int n = 0; n < 16; ++n) dpred[v * 16 + n] = n == 0 ? g : 0.0f;
+  }
+}
+
 // Adam (Kingma and Ba, 2015) on n parameters with bias-corrected moments.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
                        int n, float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -267,6 +307,7 @@ struct natray_net {
   __bf16* dz = nullptr;   // [T][4096][16] the same through the ReLU, BF16
   float* target = nullptr;
   float* mask = nullptr;
+  float* resid = nullptr;  // [T][4096] residual of the residual loss
   float* stage = nullptr;  // [T][4096][2] FP32 staging of inputs and outputs
   float* loss = nullptr;
   std::vector<void*> allocs;
@@ -338,13 +379,23 @@ int forward(natray_net* n, uint32_t tiles, const uint32_t* tiles_dev = nullptr) 
   return NATRAY_OK;
 }
 
-// Gradients of the masked MSE of the last forward; the parameters' gradients go to n->g.
-int backward(natray_net* n, uint32_t tiles, float inv_count) {
+// Gradients of the loss of the last forward: the masked MSE against n->target, or with
+// `residual` the scaled residual with the operator in n->target (diagonal) and n->mask
+// (neighbour bits). The parameters' gradients go to n->g.
+int backward(natray_net* n, uint32_t tiles, float inv_count, bool residual = false) {
   const size_t vox = (size_t)tiles * VOX;
   NCHK(hipMemsetAsync(n->g, 0, NATRAY_NET_PARAMS * sizeof(float), n->stream));
   NCHK(hipMemsetAsync(n->loss, 0, sizeof(float), n->stream));
-  hipLaunchKernelGGL(k_mse, dim3(grid_for(vox)), dim3(256), 0, n->stream, n->out, n->target, n->mask, inv_count, n->dy,
-                     n->loss, vox);
+  if (!residual) {
+    hipLaunchKernelGGL(k_mse, dim3(grid_for(vox)), dim3(256), 0, n->stream, n->out, n->target, n->mask, inv_count, n->dy,
+                       n->loss, vox);
+  } else {
+    const uint32_t* bits = reinterpret_cast<const uint32_t*>(n->mask);
+    hipLaunchKernelGGL(k_res_e, dim3(grid_for(vox)), dim3(256), 0, n->stream, n->out, n->stage, n->target, bits, inv_count,
+                       n->resid, n->loss, vox);
+    hipLaunchKernelGGL(k_res_grad, dim3(grid_for(vox)), dim3(256), 0, n->stream, n->resid, n->target, bits, inv_count, n->dy,
+                       vox);
+  }
   for (int l = 3; l >= 0; --l) {
     const Layer& L = LAYERS[l];
     const __bf16* y = l < 3 ? n->act[l] : nullptr;
@@ -410,7 +461,7 @@ int natray_net_create(uint32_t max_tiles, uint64_t seed, natray_net** out) {
       (e = nalloc(n, &n->in, vox * 8)) || (e = nalloc(n, &n->act[0], vox * 16)) || (e = nalloc(n, &n->act[1], vox * 16)) ||
       (e = nalloc(n, &n->act[2], vox * 16)) || (e = nalloc(n, &n->out, vox * 16)) || (e = nalloc(n, &n->dy, vox * 16)) ||
       (e = nalloc(n, &n->dz, vox * 16)) || (e = nalloc(n, &n->target, vox)) || (e = nalloc(n, &n->mask, vox)) ||
-      (e = nalloc(n, &n->stage, vox * 2)) || (e = nalloc(n, &n->loss, 1)))
+      (e = nalloc(n, &n->stage, vox * 2)) || (e = nalloc(n, &n->loss, 1)) || (e = nalloc(n, &n->resid, vox)))
     return bail(e);
   for (int l = 0; l < 4; ++l) {
     if ((e = nalloc(n, &n->wb[l], (size_t)ksteps(LAYERS[l].cin) * 64))) return bail(e);
@@ -496,6 +547,35 @@ int natray_net_train_step(natray_net* n, const float* input, const float* target
   NCHK(hipMemcpyAsync(n->mask, mask, vox * 4, hipMemcpyHostToDevice, n->stream));
   if ((e = forward(n, tiles))) return e;
   if ((e = backward(n, tiles, count > 0.0 ? (float)(1.0 / count) : 0.0f))) return e;
+  if (lr > 0.0f) {
+    n->steps += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)n->steps), c2 = 1.0f - std::pow(b2, (float)n->steps);
+    hipLaunchKernelGGL(k_adam, dim3((NATRAY_NET_PARAMS + 255) / 256), dim3(256), 0, n->stream, n->p, n->g, n->m, n->v,
+                       NATRAY_NET_PARAMS, lr, b1, b2, 1e-8f, c1, c2);
+    NCHK(hipGetLastError());
+    n->packed = false;
+  }
+  float h = 0.0f;
+  NCHK(hipMemcpyAsync(&h, n->loss, 4, hipMemcpyDeviceToHost, n->stream));
+  NCHK(hipStreamSynchronize(n->stream));
+  if (loss) *loss = h;
+  return NATRAY_OK;
+}
+
+int natray_net_train_step_residual(natray_net* n, const float* input, const float* diag, const uint32_t* nbm,
+                                   uint32_t tiles, float lr, float* loss) {
+  if (!n || !input || !diag || !nbm) return nfail(NATRAY_E_ARG, "null argument");
+  if (tiles == 0 || tiles > n->max_tiles) return nfail(NATRAY_E_ARG, "tiles must be 1..max_tiles");
+  const size_t vox = (size_t)tiles * VOX;
+  double count = 0.0;
+  for (size_t i = 0; i < vox; ++i) count += diag[i] > 0.0f ? 1.0 : 0.0;
+  int e = upload_input(n, input, tiles);
+  if (e) return e;
+  NCHK(hipMemcpyAsync(n->target, diag, vox * 4, hipMemcpyHostToDevice, n->stream));
+  NCHK(hipMemcpyAsync(n->mask, nbm, vox * 4, hipMemcpyHostToDevice, n->stream));
+  if ((e = forward(n, tiles))) return e;
+  if ((e = backward(n, tiles, count > 0.0 ? (float)(1.0 / count) : 0.0f, true))) return e;
   if (lr > 0.0f) {
     n->steps += 1;
     const float b1 = 0.9f, b2 = 0.999f;

@@ -185,7 +185,8 @@ int alloc_warm_start(natray_sim* s) {
   if ((e = dalloc(s, &D.guess, capc)) || (e = dalloc(s, &D.ws_r0, capc)) || (e = dalloc(s, &D.ws_pcar, capc)) ||
       (e = dalloc(s, &D.tile_map, T)) || (e = dalloc(s, &D.tile_list, W)) ||
       (e = dalloc(s, &D.tile_scale, W)) || (e = dalloc(s, &D.ws_in, W * 4096 * 2)) || (e = dalloc(s, &D.ws_tgt, W * 4096)) ||
-      (e = dalloc(s, &D.ws_mask, W * 4096)) || (e = dalloc(s, &D.ws_part, (size_t)D.cap * 2)) ||
+      (e = dalloc(s, &D.ws_mask, W * 4096)) || (e = dalloc(s, &D.ws_diag, W * 4096)) ||
+      (e = dalloc(s, &D.ws_nbm, W * 4096)) || (e = dalloc(s, &D.ws_part, (size_t)D.cap * 2)) ||
       (e = dalloc(s, &D.ws_stat, 8)))
     return e;
   HIPCHK(hipMemsetAsync(D.guess, 0, capc * 4, s->stream));
@@ -728,7 +729,8 @@ int natray_sim_set_warm_start(natray_sim* s, natray_net* net, int mode) {
   return NATRAY_OK;
 }
 
-int natray_sim_ws_samples(natray_sim* s, float* inputs, float* targets, float* masks, uint32_t max_tiles, uint32_t* tiles) {
+int natray_sim_ws_samples(natray_sim* s, float* inputs, float* targets, float* masks, float* diag, uint32_t* nbm,
+                          uint32_t max_tiles, uint32_t* tiles) {
   if (!s || !tiles) return fail(NATRAY_E_ARG, "null argument");
   if (!s->ws_ready) return fail(NATRAY_E_ARG, "the warm start was never enabled");
   LAUNCHCHK(natray::launch_ws_samples(s->D, s->stream));
@@ -739,6 +741,8 @@ int natray_sim_ws_samples(natray_sim* s, float* inputs, float* targets, float* m
   if (inputs && n) HIPCHK(hipMemcpyAsync(inputs, s->D.ws_in, vox * 2 * 4, hipMemcpyDeviceToHost, s->stream));
   if (targets && n) HIPCHK(hipMemcpyAsync(targets, s->D.ws_tgt, vox * 4, hipMemcpyDeviceToHost, s->stream));
   if (masks && n) HIPCHK(hipMemcpyAsync(masks, s->D.ws_mask, vox * 4, hipMemcpyDeviceToHost, s->stream));
+  if (diag && n) HIPCHK(hipMemcpyAsync(diag, s->D.ws_diag, vox * 4, hipMemcpyDeviceToHost, s->stream));
+  if (nbm && n) HIPCHK(hipMemcpyAsync(nbm, s->D.ws_nbm, vox * 4, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   *tiles = n;
   return NATRAY_OK;

@@ -31,6 +31,7 @@ _SIGS = {
     "natray_net_forward": (C.c_int, [_P, _P, C.c_uint32, _P]),
     "natray_net_train_step": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
     "natray_net_grads": (C.c_int, [_P, _P]),
+    "natray_net_train_step_residual": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
     "natray_net_layer_check": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, _P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P]),
     "natray_net_last_error": (C.c_char_p, []),
 }
@@ -104,6 +105,16 @@ class Net:
         t = x.shape[0]
         loss = C.c_float()
         check(lib().natray_net_train_step(self._h, abi.ptr(x), abi.ptr(target), abi.ptr(mask), t, lr, C.byref(loss)))
+        return float(loss.value)
+
+    def train_step_residual(self, x: np.ndarray, diag: np.ndarray, nbm: np.ndarray, lr: float) -> float:
+        """One Adam step on the scaled residual x - A o (natray_net_train_step_residual)."""
+        x, diag = _f32(x), _f32(diag)
+        nbm = np.ascontiguousarray(nbm, dtype=np.uint32)
+        t = x.shape[0]
+        assert x.shape == (t, VOXELS, 2) and diag.shape == (t, VOXELS) and nbm.shape == (t, VOXELS)
+        loss = C.c_float()
+        check(lib().natray_net_train_step_residual(self._h, abi.ptr(x), abi.ptr(diag), abi.ptr(nbm), t, lr, C.byref(loss)))
         return float(loss.value)
 
     def grads(self) -> np.ndarray:

@@ -164,17 +164,20 @@ class Sim:
         h = net.handle if (net is not None and mode == 2) else None
         abi.check(abi.lib().natray_sim_set_warm_start(self._h, h, mode))
 
-    def ws_samples(self, max_tiles: int = 1 << 16):
-        """Training samples of the last step: (inputs [T][4096][2], targets [T][4096], masks [T][4096])."""
+    def ws_samples(self, max_tiles: int = 1 << 16) -> dict:
+        """Training samples of the last step (natray_sim_ws_samples): inputs [T][4096][2], targets,
+        masks and diag [T][4096] FP32, nbm [T][4096] uint32."""
         n = C.c_uint32()
-        abi.check(abi.lib().natray_sim_ws_samples(self._h, None, None, None, max_tiles, C.byref(n)))
+        abi.check(abi.lib().natray_sim_ws_samples(self._h, None, None, None, None, None, max_tiles, C.byref(n)))
         t = int(n.value)
-        x = np.empty((t, 4096, 2), np.float32)
-        y = np.empty((t, 4096), np.float32)
-        m = np.empty((t, 4096), np.float32)
-        abi.check(abi.lib().natray_sim_ws_samples(self._h, abi.ptr(x), abi.ptr(y), abi.ptr(m), t, C.byref(n)))
+        out = {"inputs": np.empty((t, 4096, 2), np.float32), "targets": np.empty((t, 4096), np.float32),
+               "masks": np.empty((t, 4096), np.float32), "diag": np.empty((t, 4096), np.float32),
+               "nbm": np.empty((t, 4096), np.uint32)}
+        abi.check(abi.lib().natray_sim_ws_samples(self._h, abi.ptr(out["inputs"]), abi.ptr(out["targets"]),
+                                                  abi.ptr(out["masks"]), abi.ptr(out["diag"]), abi.ptr(out["nbm"]), t,
+                                                  C.byref(n)))
         assert int(n.value) == t
-        return x, y, m
+        return out
 
     def ws_stats(self) -> dict:
         out = np.zeros(8, np.float64)

@@ -143,3 +143,70 @@ def test_training_reduces_the_loss():
         last = n.train_step(X, target, mask, lr=3e-3)
     assert last < 0.2 * first, (first, last)
     n.close()
+
+
+def random_operator(rng, T, frac=0.7, kc=7e-4):
+    """A tile operator as the solve builds it: water voxels at random, diag = kc + 6 (no solid
+    neighbours), bits for water neighbours inside the tile."""
+    water = rng.random((T, 16, 16, 16)) < frac
+    bits = np.zeros((T, 16, 16, 16), np.uint32)
+    for axis in range(3):
+        lo = [slice(None)] * 4
+        hi = [slice(None)] * 4
+        lo[axis + 1] = slice(1, None)
+        hi[axis + 1] = slice(None, -1)
+        lo, hi = tuple(lo), tuple(hi)
+        bits[lo] |= np.where(water[lo] & water[hi], np.uint32(1 << (2 * axis)), np.uint32(0))
+        bits[hi] |= np.where(water[hi] & water[lo], np.uint32(1 << (2 * axis + 1)), np.uint32(0))
+    diag = np.where(water, np.float32(kc + 6.0), np.float32(0.0)).astype(np.float32)
+    return diag.reshape(T, VOX), bits.reshape(T, VOX)
+
+
+def torch_operator(diag, bits, o):
+    T = o.shape[0]
+    o3 = o.reshape(T, 16, 16, 16)
+    b = torch.from_numpy(bits.reshape(T, 16, 16, 16).astype(np.int64))
+    s = torch.zeros_like(o3)
+    for axis in range(3):
+        n = 16 - 1
+        lo = torch.narrow(o3, axis + 1, 1, n)
+        hi = torch.narrow(o3, axis + 1, 0, n)
+        m_lo = (torch.narrow(b, axis + 1, 1, n) & (1 << (2 * axis))) != 0
+        m_hi = (torch.narrow(b, axis + 1, 0, n) & (1 << (2 * axis + 1))) != 0
+        pad_lo = [0, 0] * 3
+        pad_hi = [0, 0] * 3
+        # place contributions back at their voxel positions
+        k = 2 * (2 - axis)
+        pad_lo[k] = 1
+        pad_hi[k + 1] = 1
+        s = s + torch.nn.functional.pad(torch.where(m_lo, hi, torch.zeros_like(hi)), pad_lo)
+        s = s + torch.nn.functional.pad(torch.where(m_hi, lo, torch.zeros_like(lo)), pad_hi)
+    d = torch.from_numpy(diag.astype(np.float64)).reshape(T, 16, 16, 16)
+    return torch.where(d > 0, d * o3 - s, torch.zeros_like(o3)).reshape(T, VOX)
+
+
+@pytest.mark.gpu
+def test_residual_loss_gradient_matches_torch():
+    """The residual loss mean_water (x - A o)^2 and its gradient through the whole network
+    against torch float64 on the same BF16-rounded forward pass."""
+    say("[net] residual-loss gradient")
+    rng = np.random.default_rng(21)
+    T = 2
+    n = net.Net(max_tiles=T, seed=4)
+    X = rng.standard_normal((T, VOX, 2)).astype(np.float32)
+    diag, bits = random_operator(rng, T)
+    X[:, :, 1] = np.where(diag > 0, 1.0, 0.0)
+    loss = n.train_step_residual(X, diag, bits, lr=0.0)
+    g = n.grads()
+    out, tens = reference_net(n.params(), X)
+    x = torch.from_numpy(X[:, :, 0].astype(np.float64))
+    w = torch.from_numpy(diag > 0)
+    e = torch.where(w, x - torch_operator(diag, bits, out), torch.zeros_like(x))
+    ref_loss = (e ** 2).sum() / w.sum()
+    ref_loss.backward()
+    gr = np.concatenate([t.grad.numpy().ravel() for t in tens])
+    assert abs(loss - float(ref_loss.detach())) <= 1e-3 * float(ref_loss.detach())
+    cos = float(np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr)))
+    assert cos > 0.999, cos
+    assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 3e-2
+    n.close()

@@ -81,6 +81,31 @@ def expected_inputs(g: dict, r0: np.ndarray, bdims, cap: int):
     return listed, scale, X, cell_slot, vox
 
 
+def water_neighbour_bits(g: dict, bdims) -> np.ndarray:
+    """Per cell in use, bit 2 axis (- side) or 2 axis + 1 (+ side) for each water face
+    neighbour, from the grid's block neighbours and cell types."""
+    n = g["n_blocks"]
+    ty = g["cf"][abi.F_TYPE]
+    nb = g["block_nb"]
+    out = np.zeros(64 * n, np.uint32)
+    lane = np.arange(64)
+    loc = np.stack([lane >> 4, (lane >> 2) & 3, lane & 3], axis=1)
+    for axis in range(3):
+        for side, bit in ((-1, 2 * axis), (1, 2 * axis + 1)):
+            q = loc.copy()
+            q[:, axis] += side
+            off = np.zeros_like(q)
+            off[:, axis] = np.where(q[:, axis] < 0, -1, np.where(q[:, axis] > 3, 1, 0))
+            q = q - 4 * off
+            k = (off[:, 0] + 1) * 9 + (off[:, 1] + 1) * 3 + off[:, 2] + 1
+            blk = nb[:, k]  # [n][64]
+            cell = blk.astype(np.int64) * 64 + (q[:, 0] * 16 + q[:, 1] * 4 + q[:, 2])
+            ok = blk != 0xFFFFFFFF
+            t = np.where(ok, ty[np.where(ok, cell, 0)], abi.GAS)
+            out |= np.where(t.ravel() == abi.WATER, np.uint32(1 << bit), np.uint32(0))
+    return out
+
+
 CASES = [("impact_sgs_32k", 0.1), ("impact_sgs_32k", -0.1), ("impact_mixed_32k_mem", 0.1)]
 
 
@@ -106,7 +131,8 @@ def test_warm_start_bit_exact(name, c):
         say(f"[warm] {name} c={c} step {step} solve")
         sim.phase(abi.PHASE_SOLVE)
         tag = f"{name} c={c} step {step}"
-        X, Y, M = sim.ws_samples()
+        smp = sim.ws_samples()
+        X, Y, M = smp["inputs"], smp["targets"], smp["masks"]
         listed, scale, Xo, cell_slot, vox = expected_inputs(g, r0, bdims, 64)
         assert X.shape[0] == len(listed), (tag, X.shape, len(listed))
         assert_same_float(f"{tag} inputs", X, Xo)
@@ -141,6 +167,14 @@ def test_warm_start_bit_exact(name, c):
         Mo[cell_slot[lit], vox[lit]] = 1.0
         assert_same_float(f"{tag} targets", Y, Yo)
         assert_same_float(f"{tag} masks", M, Mo)
+        # The operator of the solve on those cells: its diagonal and water-neighbour bits.
+        Do = np.zeros_like(smp["diag"])
+        Bo = np.zeros_like(smp["nbm"])
+        Do[cell_slot[lit], vox[lit]] = g2["cf"][abi.F_DIAG][lit]
+        water_nb = water_neighbour_bits(g2, bdims)
+        Bo[cell_slot[lit], vox[lit]] = water_nb[lit]
+        assert_same_float(f"{tag} diag", smp["diag"], Do)
+        np.testing.assert_array_equal(smp["nbm"], Bo, err_msg=f"{tag} nbm")
         for ph in (abi.PHASE_PROJECT, abi.PHASE_EXTEND, abi.PHASE_G2P):
             sim.phase(ph)
             orc.phase(ph)
@@ -166,3 +200,54 @@ def test_warm_start_off_and_inputs_only_change_nothing():
     for P, st in runs[1:]:
         assert_same_float("particles", P, runs[0][0])
         compare_state("state", st, runs[0][1])
+
+
+def test_trained_network_gain():
+    """What the trained network gains over the solve without it, on a seed it was not trained
+    on: training tiles from two seeded runs of the 32k-particle impact, Adam on the residual
+    loss with the network's own GPU kernels; then 20 steps with the warm start against 20
+    without. Every guess must lower the starting residual (all are taken), by at least half in
+    squared norm, and the solve must not need more iterations than without it."""
+    from natray_amd import scenes
+    from natray_amd import warm_start as ws
+
+    say("[warm] training on the 32k impact")
+    make = lambda seed: scenes.balloon_impact(side=64, n=32768, seed=seed, iters=400, capacity=4096)
+    parts = []
+    for seed in (3, 4):
+        s = make(seed)
+        sim = Sim.from_scene(s)
+        parts.append(ws.collect(sim, 60, iters=int(s.params.limits[2])))
+        sim.close()
+    data = {k: np.concatenate([p[k] for p in parts]) for k in ws.KEYS}
+    n = netmod.Net(max_tiles=64, seed=1)
+    before = ws.residual_ratio(n, data, 64)
+    ws.train(n, data, 1500, batch=32, lr=2e-3)
+    after = ws.residual_ratio(n, data, 64)
+    say(f"[warm] tile residual ratio {before:.3f} -> {after:.4f}")
+    assert after < 0.05
+    s = make(9)
+    it = int(s.params.limits[2])
+    iters = {}
+    for mode in ("plain", "net"):
+        sim = Sim.from_scene(s)
+        sim.run(20, iters=it)
+        if mode == "net":
+            sim.set_warm_start(n, 2)
+        c0 = sim.counters()["cg_iters"]
+        ratios = []
+        for _ in range(20):
+            sim.run(1, iters=it)
+            if mode == "net":
+                st = sim.ws_stats()
+                ratios.append(st["guess_rr"] / st["carried_rr"])
+        iters[mode] = (sim.counters()["cg_iters"] - c0) / 20
+        if mode == "net":
+            st = sim.ws_stats()
+            say(f"[warm] guesses taken {st['accepted']} of {st['offered']}; residual^2 ratio mean {np.mean(ratios):.3f}")
+            assert st["accepted"] == st["offered"] == 20
+            assert np.mean(ratios) < 0.5
+        sim.close()
+    say(f"[warm] CG iterations per step: plain {iters['plain']:.2f}, warm-started {iters['net']:.2f}")
+    assert iters["net"] <= iters["plain"]
+    n.close()
