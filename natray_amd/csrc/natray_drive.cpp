@@ -1,6 +1,9 @@
 // natray_drive: steps a scene file through the C ABI of libnatray.so, from native code.
 //
-//   natray_drive <scene file> [steps=20] [warmup=2] [cg iteration cap=1000]
+//   natray_drive <scene file> [steps=20] [warmup=2] [cg iteration cap=1000] [ws]
+//
+// With `ws` the solve is warm-started by a network of seeded random weights (mode 2 of
+// natray_sim_set_warm_start): its kernels run as with trained weights, for timing them.
 //
 // The scene file comes from `python bench.py --write-scene <path>` (natray_amd.scenes
 // write_scene_file): "NATRAYSC", u32 version 1, u32 particle count, then natray_params,
@@ -25,6 +28,7 @@ int main(int argc, char** argv) {
   const unsigned steps = argc > 2 ? (unsigned)atoi(argv[2]) : 20u;
   const unsigned warmup = argc > 3 ? (unsigned)atoi(argv[3]) : 2u;
   const unsigned iters = argc > 4 ? (unsigned)atoi(argv[4]) : 1000u;
+  const bool warm = argc > 5 && strcmp(argv[5], "ws") == 0;
   FILE* f = fopen(argv[1], "rb");
   if (!f) {
     perror(argv[1]);
@@ -55,6 +59,15 @@ int main(int argc, char** argv) {
     fprintf(stderr, "natray_sim_create: %d %s\n", rc, natray_last_error());
     return 1;
   }
+  natray_net* net = nullptr;
+  if (warm) {
+    if ((rc = natray_net_create(1024, 1, &net)) != 0 || (rc = natray_sim_set_warm_start(sim, net, 2)) != 0) {
+      fprintf(stderr, "warm start: %d %s %s\n", rc, natray_last_error(), natray_net_last_error());
+      natray_sim_destroy(sim);
+      if (net) natray_net_destroy(net);
+      return 1;
+    }
+  }
   natray_state st;
   if (warmup && (rc = natray_sim_run(sim, warmup, iters, 0xffffffffu, &st)) != 0) {
     fprintf(stderr, "warmup: %d %s\n", rc, natray_last_error());
@@ -76,5 +89,6 @@ int main(int argc, char** argv) {
          st.solve_iters);
   fflush(stdout);
   natray_sim_destroy(sim);
+  if (net) natray_net_destroy(net);
   return 0;
 }

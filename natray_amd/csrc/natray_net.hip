@@ -44,53 +44,110 @@ __device__ __forceinline__ int tap_voxel(int x, int y, int z, int tap) {
 }
 
 // Y[t][v][n] = act(bias[n] + sum_k X[t][v + off(tap)][c] W[k][n]) for n < 16, with W in the
-// MFMA B layout `wb` (ksteps(cin) x 4 groups x 16 columns x 8 values, see pack kernels).
-// One workgroup of 4 waves takes 16 z-rows of one tile at a time (4 per wave), over work items
-// (tile, x) strided by the grid. `yf` (FP32, 16 channels) or `yb` (BF16, 16 channels) receives
-// the result. The tile count is `tiles`, or min(*tiles_dev, tiles) when tiles_dev is given (a
-// count only the device knows when the work is enqueued).
-__global__ __launch_bounds__(256) void k_conv(const __bf16* __restrict__ X, int cin, const bf16x8* __restrict__ wb,
+// MFMA B layout `wb` (ksteps(CIN) x 4 groups x 16 columns x 8 values, see k_pack). `yf` (FP32,
+// 16 channels) or `yb` (BF16, 16 channels) receives the result.
+//
+// A work item is XP = 2 output x-planes of one tile (32 z-rows). Its input slab, planes
+// x0 - 1 .. x0 + 2 with y and z from -1 to 16, is staged in LDS with zeros outside the tile, so
+// the A fragments are plain 16-byte LDS reads at lane-constant offsets (no bounds tests); a
+// staged voxel holds CIN channels padded to VS so that the 16 lanes of a row, 48 bytes apart
+// for CIN = 16, fall in distinct banks. The weights stay in registers (B fragments, one per
+// K step). Each wave runs two z-rows at a time, two independent MFMA chains. Work items are
+// strided over the grid; the tile count is `tiles`, or min(*tiles_dev, tiles) when tiles_dev is
+// given (a count only the device knows when the work is enqueued).
+constexpr int XP = 2;
+constexpr int SLAB = (XP + 2) * 18 * 18;
+
+template <int CIN>
+struct ConvShape {
+  static constexpr int VS = CIN == 16 ? 24 : 8;  // bf16 per staged voxel
+  static constexpr int KS = ksteps(CIN);
+};
+
+template <int CIN>
+__global__ __launch_bounds__(256) void k_conv(const __bf16* __restrict__ X, const bf16x8* __restrict__ wb,
                                               const float* __restrict__ bias, int relu, __bf16* __restrict__ yb,
                                               float* __restrict__ yf, int tiles, const uint32_t* __restrict__ tiles_dev) {
-  __shared__ bf16x8 w_lds[ksteps(16) * 64];
+  using Sh = ConvShape<CIN>;
+  constexpr int VS = Sh::VS, KS = Sh::KS, CH = CIN / 8;  // 16-byte chunks of data per staged voxel
+  constexpr int PER_THREAD = (SLAB * CH + 255) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 slab[SLAB * VS];
   const int nt = tiles_dev ? (int)min(*tiles_dev, (uint32_t)tiles) : tiles;
-  const int items = nt * 16;
+  const int items = nt * (TILE / XP);
   if ((int)blockIdx.x >= items) return;
-  const int ks = ksteps(cin);
-  for (int i = threadIdx.x; i < ks * 64; i += blockDim.x) w_lds[i] = wb[i];
-  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, col = lane & 15;
-  for (int item = blockIdx.x; item < items; item += gridDim.x) {
-  const int t = item / 16, rows0 = (item % 16) * 16;
-  const __bf16* Xt = X + (size_t)t * VOX * cin;
-  for (int r = wave; r < 16; r += 4) {
-    const int row = rows0 + r;  // z-row index = 16 x + y
-    const int x = row / TILE, y = row % TILE, z = col;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int s = 0; s < ks; ++s) {
-      const int k = s * 32 + g * 8;
-      const int tap = k / cin, c0 = k % cin;
-      bf16x8 a;
-      const int nv = tap < TAPS ? tap_voxel(x, y, z, tap) : -1;
-      if (nv >= 0) {
-        a = *reinterpret_cast<const bf16x8*>(Xt + (size_t)nv * cin + c0);
-      } else {
+  bf16x8 w[KS];
+  int off[KS];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] = (__bf16)0.0f;
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w_lds[s * 64 + lane], acc, 0, 0, 0);
-    }
-    // acc[i] is voxel z = 4 g + i of this row, output channel col.
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float v = acc[i] + (bias ? bias[col] : 0.0f);
-      if (relu) v = v > 0.0f ? v : 0.0f;
-      const size_t o = ((size_t)t * VOX + (size_t)row * TILE + 4 * g + i) * 16 + col;
-      if (yf) yf[o] = v;
-      if (yb) yb[o] = (__bf16)v;
-    }
+  for (int s = 0; s < KS; ++s) {
+    w[s] = wb[s * 64 + lane];
+    const int k = s * 32 + g * 8, tap = k / CIN, c0 = k % CIN;
+    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+    // K padding (tap 27) reads a valid slab address and is replaced by zeros below.
+    off[s] = tap < TAPS ? ((dx * 18 + dy) * 18 + dz) * VS + c0 : 0;
   }
+  // Only the last step can hold K padding (tap 27).
+  const bool last_ok = ((KS - 1) * 32 + g * 8) / CIN < TAPS;
+  bf16x8 zero;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) zero[j] = (__bf16)0.0f;
+  const float bcol = bias ? bias[col] : 0.0f;
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    const int t = item / (TILE / XP), x0 = (item % (TILE / XP)) * XP;
+    const __bf16* Xt = X + (size_t)t * VOX * CIN;
+    // Every load of the slab is issued before the first LDS store (the padding of a staged
+    // voxel beyond CIN channels is never read and stays unwritten).
+    bf16x8 val[PER_THREAD];
+#pragma unroll
+    for (int k = 0; k < PER_THREAD; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      const int v = i / CH, j = i % CH;
+      const int px = v / 324, py = (v / 18) % 18, pz = v % 18;
+      const int x = x0 - 1 + px, y = py - 1, z = pz - 1;
+      val[k] = zero;
+      if (i < SLAB * CH && (unsigned)x < TILE && (unsigned)y < TILE && (unsigned)z < TILE)
+        val[k] = *reinterpret_cast<const bf16x8*>(Xt + (size_t)((x * TILE + y) * TILE + z) * CIN + 8 * j);
+    }
+    __syncthreads();  // the previous item's reads of the slab are done
+#pragma unroll
+    for (int k = 0; k < PER_THREAD; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      if (i < SLAB * CH) *reinterpret_cast<bf16x8*>(slab + (i / CH) * VS + 8 * (i % CH)) = val[k];
+    }
+    __syncthreads();
+    for (int r = wave; r < XP * TILE; r += 8) {
+      f32x4 acc[2];
+      int base[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int rr = r + 4 * h, xo = rr / TILE, y = rr % TILE;
+        base[h] = (((xo + 1) * 18 + (y + 1)) * 18 + (col + 1)) * VS;
+        acc[h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          bf16x8 a = *reinterpret_cast<const bf16x8*>(slab + base[h] + off[s]);
+          if (s == KS - 1 && !last_ok) a = zero;
+          acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[s], acc[h], 0, 0, 0);
+        }
+      }
+      // acc[h][i] is voxel z = 4 g + i of row r + 4 h, output channel col.
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int rr = r + 4 * h, row = (x0 + rr / TILE) * TILE + rr % TILE;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float v = acc[h][i] + bcol;
+          if (relu) v = v > 0.0f ? v : 0.0f;
+          const size_t o = ((size_t)t * VOX + (size_t)row * TILE + 4 * g + i) * 16 + col;
+          if (yf) yf[o] = v;
+          if (yb) yb[o] = (__bf16)v;
+        }
+      }
+    }
   }
 }
 
@@ -360,8 +417,16 @@ int pack(natray_net* n) {
   return NATRAY_OK;
 }
 
-// Workgroups of a convolution over `tiles` tiles: one per work item up to 8 per CU.
-unsigned conv_grid(uint32_t tiles) { return (unsigned)std::min<uint32_t>(tiles * 16u, 2048u); }
+// A convolution over `tiles` tiles: one workgroup per work item (2 x-planes), up to 2 per CU
+// (the LDS slab of a 16-channel input takes 62 KB).
+void launch_conv(int cin, uint32_t tiles, hipStream_t stream, const __bf16* X, const bf16x8* wb, const float* bias, int relu,
+                 __bf16* yb, float* yf, const uint32_t* tiles_dev) {
+  const unsigned grid = (unsigned)std::min<uint32_t>(tiles * (TILE / XP), 512u);
+  if (cin == 8)
+    hipLaunchKernelGGL(k_conv<8>, dim3(grid), dim3(256), 0, stream, X, wb, bias, relu, yb, yf, (int)tiles, tiles_dev);
+  else
+    hipLaunchKernelGGL(k_conv<16>, dim3(grid), dim3(256), 0, stream, X, wb, bias, relu, yb, yf, (int)tiles, tiles_dev);
+}
 
 int forward(natray_net* n, uint32_t tiles, const uint32_t* tiles_dev = nullptr) {
   if (!n->packed) {
@@ -371,8 +436,8 @@ int forward(natray_net* n, uint32_t tiles, const uint32_t* tiles_dev = nullptr) 
   const __bf16* x = n->in;
   for (int l = 0; l < 4; ++l) {
     const Layer& L = LAYERS[l];
-    hipLaunchKernelGGL(k_conv, dim3(conv_grid(tiles)), dim3(256), 0, n->stream, x, L.cin, n->wb[l], n->p + L.b_off, L.relu,
-                       l < 3 ? n->act[l] : nullptr, l == 3 ? n->out : nullptr, (int)tiles, tiles_dev);
+    launch_conv(L.cin, tiles, n->stream, x, n->wb[l], n->p + L.b_off, L.relu, l < 3 ? n->act[l] : nullptr,
+                l == 3 ? n->out : nullptr, tiles_dev);
     if (l < 3) x = n->act[l];
   }
   NCHK(hipGetLastError());
@@ -407,8 +472,7 @@ int backward(natray_net* n, uint32_t tiles, float inv_count, bool residual = fal
     hipLaunchKernelGGL(k_bgrad, dim3((unsigned)std::min<size_t>((vox + 15) / 16, 1024)), dim3(256), 0, n->stream, n->dz,
                        n->g + L.b_off, L.cout_real, vox);
     if (l > 0)
-      hipLaunchKernelGGL(k_conv, dim3(conv_grid(tiles)), dim3(256), 0, n->stream, n->dz, 16, n->wt[l], nullptr, 0, nullptr,
-                         n->dy, (int)tiles, nullptr);
+      launch_conv(16, tiles, n->stream, n->dz, n->wt[l], nullptr, 0, nullptr, n->dy, nullptr);
   }
   NCHK(hipGetLastError());
   return NATRAY_OK;
@@ -635,7 +699,7 @@ int natray_net_layer_check(uint32_t cin_real, uint32_t cout_real, int relu, cons
   const int cnt = ksteps(cin) * 64, cntt = ksteps(16) * 64;
   hipLaunchKernelGGL(k_pack, dim3((cnt + 255) / 256), dim3(256), 0, 0, dWd, (int)cin_real, (int)cout_real, cin, 0, wb);
   hipLaunchKernelGGL(k_pack, dim3((cntt + 255) / 256), dim3(256), 0, 0, dWd, (int)cin_real, (int)cout_real, 16, 1, wt);
-  hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, 0, xb, cin, wb, bd, relu, yb, yf, (int)tiles, nullptr);
+  launch_conv(cin, tiles, 0, xb, wb, bd, relu, yb, yf, nullptr);
   hipLaunchKernelGGL(k_take, dim3(grid), dim3(256), 0, 0, yf, (int)cout_real, stage, vox);
   if (hipMemcpy(Y, stage, vox * cout_real * 4, hipMemcpyDeviceToHost)) return done(nfail(NATRAY_E_HIP, "download Y"));
   // Backward from dY.
@@ -647,7 +711,7 @@ int natray_net_layer_check(uint32_t cin_real, uint32_t cout_real, int relu, cons
                      (int)cout_real, (int)tiles);
   hipLaunchKernelGGL(k_bgrad, dim3((unsigned)std::min<size_t>((vox + 15) / 16, 1024)), dim3(256), 0, 0, dz, dbd,
                      (int)cout_real, vox);
-  hipLaunchKernelGGL(k_conv, dim3(tiles * 16), dim3(256), 0, 0, dz, 16, wt, nullptr, 0, nullptr, dxf, (int)tiles, nullptr);
+  launch_conv(16, tiles, 0, dz, wt, nullptr, 0, nullptr, dxf, nullptr);
   hipLaunchKernelGGL(k_take, dim3(grid), dim3(256), 0, 0, dxf, (int)cin_real, dXf, vox);
   if (hipGetLastError() != hipSuccess) return done(nfail(NATRAY_E_HIP, "layer kernels"));
   if (hipMemcpy(dX, dXf, vox * cin_real * 4, hipMemcpyDeviceToHost) || hipMemcpy(dW, dWd, nw * 4, hipMemcpyDeviceToHost) ||
