@@ -270,7 +270,7 @@ def _errors(g: dict, P: np.ndarray, g_ref: dict, cf_ref: np.ndarray, P_ref: np.n
     }
 
 
-def accuracy_vs_fp64(scene, checkpoints, iters: int) -> dict:
+def accuracy_vs_fp64(scene, checkpoints, iters: int, one_from: int = 20) -> dict:
     """Config 3's report: the plain FP32 and the mixed-precision CG run from the scene's
     initial state on the GPU, the FP64 oracle on the host; errors of the GPU runs against it
     after each checkpoint's number of steps, at the same tolerance. Relative errors are the
@@ -310,6 +310,23 @@ def accuracy_vs_fp64(scene, checkpoints, iters: int) -> dict:
                 e["fp64_cg_relative_residual_last_solve"] = res_ref
                 e["fp64_cg_iters_last_step"] = it_ref
                 out.setdefault(label, {})[f"after_{k}_steps"] = e
+            # One substep from identical state: the GPU's particles and State after `one_from`
+            # steps are loaded into a fresh FP64 oracle, then both take one step.
+            st = sim.run(one_from - done, iters=iters)
+            ref = oracle.Oracle(scene.params, sim.particles(), scene.scene, sim.state(), "f64")
+            ref.run(1, iters)
+            sc_ref = ref.solver_scalars()
+            g_ref, cf_ref, P_ref = ref.grid(), ref.cf_f64(), ref.particles_f64()
+            ref.close()
+            c0 = sim.counters()
+            st = sim.run(1, iters=iters)
+            c1 = sim.counters()
+            e = _errors(sim.grid(), sim.particles(), g_ref, cf_ref, P_ref, h)
+            e["cg_iters"] = int(c1["cg_iters"] - c0["cg_iters"])
+            e["fp64_cg_iters"] = int(sc_ref["solve_iters"])
+            e["cg_relative_residual"] = st.r_norm / max(st.b_norm, 1.0)
+            e["within_criteria"] = bool(e["max_grid_velocity_error_rel"] <= 1e-5 and e["max_position_error_cells"] <= 1e-4)
+            out[label][f"one_substep_from_identical_state_at_step_{one_from}"] = e
             sim.close()
     finally:
         sc.precision, sc.refresh_every = keep

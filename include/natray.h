@@ -330,6 +330,41 @@ int natray_net_layer_check(uint32_t cin, uint32_t cout, int relu, const float* X
                            const float* dY, uint32_t tiles, float* Y, float* dX, float* dW, float* db);
 const char* natray_net_last_error(void);
 
+/* ---- batches of independent scenes ----------------------------------------------------------
+ * No counterpart in the reference, which steps one Sim per scene (src/sim.rs). A batch steps
+ * many scenes, each a natray_sim of its own (particles, grid, clock, dt and CG of its own), with
+ * one launch per kernel for all of them: a workgroup map gives every workgroup a scene and its
+ * place among that scene's workgroups. Workgroups are shared among the scenes in proportion to
+ * their particle counts; the cooperative solve gives every scene workgroups of its own, which
+ * reduce among themselves. A scene gives the same bits in a batch as alone. The scenes must
+ * share the CG precision, be on one device and not use the warm start; they stay owned by the
+ * caller and must outlive the batch. A batch holds at most one scene per compute unit. */
+typedef struct natray_batch natray_batch;
+/* Launch shape of a batch: workgroups per compute unit shared among the scenes for P2G and G2P
+ * (blocks), for the one-wave-per-block kernels (waves) and for clearing (cells). */
+typedef struct {
+  uint32_t blocks_per_cu, waves_per_cu, cells_per_cu, pad;
+} natray_batch_shape;
+int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out);
+void natray_batch_destroy(natray_batch* batch);
+/* n_steps steps of every scene, with CG iteration cap `iters` (0: each scene's params.limits.z
+ * as it stands); each scene's State is read back afterwards (natray_sim_read_state). */
+int natray_batch_run(natray_batch* batch, uint32_t n_steps, uint32_t iters);
+/* Steps as two replayed graphs around the solve (on, the default) or kernel by kernel. */
+int natray_batch_set_graph(natray_batch* batch, int on);
+int natray_batch_set_shape(natray_batch* batch, const natray_batch_shape* shape);
+int natray_batch_get_shape(natray_batch* batch, natray_batch_shape* out);
+/* The autotuner: runs one untimed and `steps` timed steps (real steps of the scenes) with each
+ * candidate shape (blocks and waves per CU in {4, 8, 16}), keeps the fastest and returns it in
+ * *best, with the candidates' ms per step in ms[] (up to max_candidates) and their number. */
+int natray_batch_tune(natray_batch* batch, uint32_t steps, uint32_t iters, natray_batch_shape* best, double* ms,
+                      uint32_t max_candidates, uint32_t* n_candidates);
+/* The workgroup shares the batch uses: out[k] = scene k's part of `budget`, in proportion to
+ * weights[k] (largest remainders), at least min_each and at most max_each[k] (no bound when
+ * max_each is NULL). */
+int natray_batch_plan(const uint64_t* weights, const uint32_t* max_each, uint32_t n, uint32_t budget, uint32_t min_each,
+                      uint32_t* out);
+
 /* ---- the warm start in the step ------------------------------------------------------------
  * No counterpart in the reference, whose solve always starts from the particle-carried pressure
  * (solve_init, shaders/fluid.wgsl:450-489). Each step, before the solve: r0 = b - A p_carried on

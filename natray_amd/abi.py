@@ -163,6 +163,14 @@ _SIGNATURES = {
     "natray_sim_ws_samples": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "natray_sim_ws_stats": (C.c_int, [_P, _P]),
     "natray_sim_ws_guess": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    "natray_batch_create": (C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(_P)]),
+    "natray_batch_destroy": (None, [_P]),
+    "natray_batch_run": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "natray_batch_set_graph": (C.c_int, [_P, C.c_int]),
+    "natray_batch_set_shape": (C.c_int, [_P, _P]),
+    "natray_batch_get_shape": (C.c_int, [_P, _P]),
+    "natray_batch_tune": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "natray_batch_plan": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "natray_plan_batch": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32]),
     "natray_next_iters": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "natray_last_error": (C.c_char_p, []),

@@ -84,27 +84,57 @@ struct Dev {
   uint32_t cap;    // block capacity (bdims.w)
   uint32_t capc;   // cap x 64 cells per field
   uint32_t fixed_ns;  // fixed step in ns, or 0
+  uint32_t g2p_wgs;   // workgroups of the G2P launch (entries of vmax_part it writes)
+};
+
+// How a kernel finds its scene. One: a single scene, its Dev passed by value. Many: a batch of
+// scenes in one launch; workgroup w works for scene map[w].scene as its map[w].wg-th of
+// map[w].wgs workgroups, with that scene's Dev at scenes[map[w].scene] (device memory).
+struct WgMap {
+  uint32_t scene, wg, wgs, pad;
+};
+struct One {
+  Dev d;
+};
+struct Many {
+  const Dev* scenes;
+  const WgMap* map;
+};
+
+// The workgroup counts of a launch, by kind of kernel: one per scene; one per 256 particles;
+// one per scan tile; grid-stride over the cells to clear; blocks of P2G and G2P; one wave per
+// block (grid, project, extend, warm start); the cooperative solve.
+enum LaunchKind { LK_ONE, LK_PARTS, LK_TILES, LK_CELLS, LK_BLOCKS, LK_WAVES, LK_SOLVE, LK_KINDS };
+
+// What a launch wrapper launches on: one scene (`d`, a host copy of its Dev) or a batch
+// (`scenes` and one workgroup map per kind, in device memory), with the workgroups per kind.
+struct Target {
+  const Dev* d = nullptr;
+  const Dev* scenes = nullptr;
+  const WgMap* map[LK_KINDS] = {};
+  uint32_t grid[LK_KINDS] = {};
 };
 
 }  // namespace natray
 
-// Launch wrappers (natray_kernels.hip). Each returns the hipError_t of its launches as int.
+// Launch wrappers (natray_kernels.hip), on one scene or a batch (Target). Each returns the
+// hipError_t of its launches as int.
 namespace natray {
-int launch_keys(const Dev& d, void* stream);
-int launch_begin(const Dev& d, void* stream);
-int launch_bin(const Dev& d, void* stream, int grid_cells);
-int launch_p2g(const Dev& d, void* stream, int grid_blocks);
-int launch_grid_prep(const Dev& d, void* stream, int grid_waves);
-int launch_solve(const Dev& d, void* stream, int grid, int mixed);
+int launch_keys(const Target& t, void* stream);
+int launch_begin(const Target& t, void* stream);
+int launch_bin(const Target& t, void* stream);
+int launch_p2g(const Target& t, void* stream);
+int launch_grid_prep(const Target& t, void* stream);
+int launch_solve(const Target& t, void* stream, int mixed);
 int solve_max_grid(int mixed, int* per_cu);
-int launch_project(const Dev& d, void* stream, int grid_waves);
-int launch_extend(const Dev& d, void* stream, int grid_waves);
-int launch_g2p(const Dev& d, void* stream, int grid_blocks);
-// Warm start, before the solve: the network's inputs for at most `cap` tiles (also into the
-// network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has run on them
-// (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a step,
-// launch_ws_samples writes the training targets of its listed tiles.
-int launch_ws_inputs(const Dev& d, void* stream, int grid_waves, uint32_t cap, void* net_in);
-int launch_ws_guess(const Dev& d, void* stream, int grid_waves, const float* pred);
-int launch_ws_samples(const Dev& d, void* stream);
+int launch_project(const Target& t, void* stream);
+int launch_extend(const Target& t, void* stream);
+int launch_g2p(const Target& t, void* stream);
+// Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also
+// into the network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has
+// run on them (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a
+// step, launch_ws_samples writes the training samples of its listed tiles.
+int launch_ws_inputs(const Target& t, void* stream, uint32_t cap, void* net_in);
+int launch_ws_guess(const Target& t, void* stream, const float* pred);
+int launch_ws_samples(const Target& t, void* stream);
 }  // namespace natray

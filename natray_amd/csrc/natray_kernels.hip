@@ -39,6 +39,20 @@ constexpr uint32_t TILE = 343;
 constexpr float GAS = NATRAY_GAS, WATER = NATRAY_WATER, SOLID = NATRAY_SOLID, COVERED = NATRAY_COVERED,
                 RUBBER = NATRAY_RUBBER;
 
+// ---- launch policies ---------------------------------------------------------------------------
+
+// A kernel's scene and its workgroup among the scene's: with One (a single scene, Dev by value)
+// the launch's own; with Many (a batch of scenes in one launch) from the workgroup map.
+struct View {
+  const Dev& d;
+  uint32_t wg, wgs;
+};
+__device__ __forceinline__ View view(const One& l) { return {l.d, blockIdx.x, gridDim.x}; }
+__device__ __forceinline__ View view(const Many& l) {
+  const WgMap m = l.map[blockIdx.x];
+  return {l.scenes[m.scene], m.wg, m.wgs};
+}
+
 // ---- small helpers ---------------------------------------------------------------------------
 
 __device__ __forceinline__ float fmin_(float a, float b) { return b < a ? b : a; }
@@ -191,7 +205,10 @@ __device__ __forceinline__ uint32_t pack_key(const Dev& d, float px, float py, f
 
 // ---- begin_step (shaders/fluid.wgsl:31-89) -----------------------------------------------------
 
-__global__ void k_begin(Dev d) {
+template <class L>
+__global__ void k_begin(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (threadIdx.x != 0) return;
   natray_state* st = d.st;
   const natray_params* U = d.U;
@@ -249,8 +266,11 @@ __device__ __forceinline__ uint32_t next_block(const Dev& d, uint32_t word) {
 }
 
 // Keys of particles written from the host.
-__global__ void k_keys(Dev d) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+template <class L>
+__global__ void k_keys(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  uint32_t i = v.wg * blockDim.x + threadIdx.x;
   if (i >= d.N) return;
   const float4* p = reinterpret_cast<const float4*>(d.P) + (size_t)i * 5;
   float4 pos = p[0], vel = p[1];
@@ -261,21 +281,27 @@ __global__ void k_keys(Dev d) {
 
 // clear_blocks (fluid.wgsl:91-107): zero the transfer sums of last step's cells. The slots'
 // block pointers and counts are reset by the scan itself.
-__global__ void k_clear(Dev d) {
+template <class L>
+__global__ void k_clear(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   uint32_t n4 = d.st->n_blocks_prev * 16u;  // int4 per field
   for (uint32_t f = 0; f < ACC_CLEARED; ++f) {
     int4* a = reinterpret_cast<int4*>(d.acc + (size_t)f * d.capc);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) a[i] = make_int4(0, 0, 0, 0);
+    for (uint32_t i = v.wg * blockDim.x + threadIdx.x; i < n4; i += v.wgs * blockDim.x) a[i] = make_int4(0, 0, 0, 0);
   }
 }
 
 // mark_blocks + claim (fluid.wgsl:109-150): mark the blocks of cells c-1..c+2 as claimed and
 // take a rank in the own block. Lanes of a wave with the same own block share one atomic.
-__global__ __launch_bounds__(256) void k_mark(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_mark(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   const natray_params* U = d.U;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = v.wg * blockDim.x + threadIdx.x;
   uint32_t key = i < d.N ? d.key[i] : NONE;
   bool active = key != NONE;
   uint32_t own = 0;
@@ -314,10 +340,13 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // scan_blocks, first pass: claimed slots and their particles per tile of SCAN_TILE slots.
-__global__ __launch_bounds__(256) void k_scan_reduce(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_scan_reduce(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   __shared__ uint32_t ws[2][4];
-  const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * 4u;
+  const uint32_t base = v.wg * SCAN_TILE + threadIdx.x * 4u;
   uint4 bp = *reinterpret_cast<const uint4*>(d.block_ptr + base);
   uint4 sc = *reinterpret_cast<const uint4*>(d.slot_count + base);
   uint32_t f = (bp.x == PENDING) + (bp.y == PENDING) + (bp.z == PENDING) + (bp.w == PENDING);
@@ -330,8 +359,8 @@ __global__ __launch_bounds__(256) void k_scan_reduce(Dev d) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    d.tile_sum[blockIdx.x] = ws[0][0] + ws[0][1] + ws[0][2] + ws[0][3];
-    d.tile_sum[d.ntiles + blockIdx.x] = ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3];
+    d.tile_sum[v.wg] = ws[0][0] + ws[0][1] + ws[0][2] + ws[0][3];
+    d.tile_sum[d.ntiles + v.wg] = ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3];
   }
 }
 
@@ -349,7 +378,10 @@ __device__ __forceinline__ void wave_scan2(uint32_t& a, uint32_t& b) {
 
 // scan_blocks, second pass: exclusive prefix of the tile totals (one workgroup), the block
 // count and the overflow flag (claim, fluid.wgsl:117-121).
-__global__ __launch_bounds__(1024) void k_scan_tiles(Dev d) {
+template <class L>
+__global__ __launch_bounds__(1024) void k_scan_tiles(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   __shared__ uint32_t wsum[2][16];
   const uint32_t nt = d.ntiles, t = threadIdx.x;
@@ -394,10 +426,13 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(Dev d) {
 
 // scan_blocks, third pass: block indices in slot order, block starts, and every slot's pointer
 // (NONE where unclaimed); resets the counts for the next step.
-__global__ __launch_bounds__(256) void k_scan_scatter(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_scan_scatter(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   __shared__ uint32_t ws[2][4];
-  const uint32_t j = blockIdx.x, base = j * SCAN_TILE + threadIdx.x * 4u;
+  const uint32_t j = v.wg, base = j * SCAN_TILE + threadIdx.x * 4u;
   uint4 bp = *reinterpret_cast<const uint4*>(d.block_ptr + base);
   uint4 sc = *reinterpret_cast<const uint4*>(d.slot_count + base);
   uint32_t fl[4] = {bp.x == PENDING, bp.y == PENDING, bp.z == PENDING, bp.w == PENDING};
@@ -440,9 +475,12 @@ __global__ __launch_bounds__(256) void k_scan_scatter(Dev d) {
 }
 
 // sort_particles (fluid.wgsl:189-196)
-__global__ __launch_bounds__(256) void k_sort(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_sort(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = v.wg * blockDim.x + threadIdx.x;
   if (i >= d.N) return;
   uint32_t key = d.key[i];
   if (key == NONE) return;
@@ -559,7 +597,10 @@ constexpr uint32_t P2G_MAXR = 64;
 // second, and so on. The 256 threads splat that list in order, so the lanes of one LDS atomic
 // instruction hold particles of different cells (their stencils, not the same words), and no
 // lane idles while a crowded cell still has particles. The per-cell counts are A_N.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_p2g(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_p2g(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   __shared__ int tile[8 * TILE];
   __shared__ uint32_t cnt[64], drop[64];
@@ -693,11 +734,14 @@ __device__ __forceinline__ void for_blocks(const Dev& d, uint32_t b_lo, uint32_t
 }
 
 // grid_prep (fluid.wgsl:319-356)
-__global__ __launch_bounds__(256) void k_grid_prep(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_grid_prep(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   const float dt = dt_of(d);
   const natray_params* U = d.U;
-  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u, [&](uint32_t b, uint32_t lane, int, int, int, const uint32_t*) {
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u, [&](uint32_t b, uint32_t lane, int, int, int, const uint32_t*) {
     const uint32_t cell = b * 64u + lane;
     int cx, cy, cz;
     cell_coord(d, cell, cx, cy, cz);
@@ -874,6 +918,7 @@ __device__ __forceinline__ bool converged(R rr, R tol, R b_norm) {
 // The blocks and chunks of one solve workgroup.
 struct SolveSpan {
   uint32_t n, K, m, G;
+  uint32_t wg;      // this workgroup among the G of its scene
   uint32_t c0, c1;  // chunks of this workgroup
   uint32_t b0, b1;  // their blocks
   bool in_regs;     // every workgroup's blocks fit in registers (the same for all)
@@ -1116,10 +1161,12 @@ __device__ __forceinline__ bool solve_beta(const Dev& d, const R* tt, SolveScala
   return true;
 }
 
-// End of a solve: workgroup 0 writes State, the solver scalars, the counters and the last tag.
+// End of a solve: the scene's workgroup 0 writes State, the solver scalars, the counters and
+// the last tag.
 template <class R>
-__device__ __forceinline__ void solve_finish(const Dev& d, uint32_t n, uint32_t tag, const SolveScalars<R>& s) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__device__ __forceinline__ void solve_finish(const Dev& d, const SolveSpan& sp, uint32_t tag, const SolveScalars<R>& s) {
+  if (sp.wg != 0 || threadIdx.x != 0) return;
+  const uint32_t n = sp.n;
   __hip_atomic_store(d.bar + 4, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   natray_state* st = d.st;
   st->rz = (float)s.rz;
@@ -1257,11 +1304,13 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
     if (!reduce<R>(d, sp, ++tag, 2, nullptr, tt)) break;
     go = solve_beta<R>(d, tt, s);
   }
-  solve_finish<R>(d, sp.n, tag, s);
+  solve_finish<R>(d, sp, tag, s);
 }
 
-template <bool MIXED>
-__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
+template <bool MIXED, class L>
+__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   using R = typename std::conditional<MIXED, double, float>::type;
   constexpr uint32_t RM = SOLVE_RMAX;
   if (d.st->dt_ns == 0u) return;
@@ -1271,9 +1320,10 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
   sp.n = n_blocks(d);
   sp.K = sp.n ? (sp.n + MAX_CHUNKS - 1u) / MAX_CHUNKS : 1u;
   sp.m = sp.n ? (sp.n + sp.K - 1u) / sp.K : 0u;
-  sp.G = gridDim.x;
-  sp.c0 = (uint32_t)((uint64_t)blockIdx.x * sp.m / sp.G);
-  sp.c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * sp.m / sp.G);
+  sp.G = v.wgs;
+  sp.wg = v.wg;
+  sp.c0 = (uint32_t)((uint64_t)v.wg * sp.m / sp.G);
+  sp.c1 = (uint32_t)((uint64_t)(v.wg + 1) * sp.m / sp.G);
   sp.b0 = sp.c0 * sp.K;
   sp.b1 = min(sp.n, sp.c1 * sp.K);
   sp.in_regs = (sp.m + sp.G - 1u) / sp.G * sp.K <= SOLVE_WAVES * RM;
@@ -1484,7 +1534,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Dev d) {
       if (s.iters > 0u) d.cf[(size_t)NATRAY_F_Q * capc + cell] = rQ[r];
     }
   }
-  solve_finish<R>(d, sp.n, tag, s);
+  solve_finish<R>(d, sp, tag, s);
 }
 
 // ---- learned warm start of the solve ---------------------------------------------------------------
@@ -1523,10 +1573,13 @@ __device__ __forceinline__ uint32_t tile_voxel_cell(const Dev& d, uint32_t t, ui
 
 // r0 from the carried pressure, the carried pressure kept, the guess preset to it, and the
 // tiles with water marked with their largest |r0|.
-__global__ __launch_bounds__(256) void k_ws_prepare(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_ws_prepare(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   const float* cP = d.cf + (size_t)NATRAY_F_P * d.capc;
-  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
     const uint32_t cell = b * 64u + lane;
     const Row w = cell_row(d, cell, lx, ly, lz, lk, cP);
     const float p = cP[cell];
@@ -1545,7 +1598,10 @@ __global__ __launch_bounds__(256) void k_ws_prepare(Dev d) {
 
 // One workgroup: lists the marked tiles in tile order (the first `cap`) with their scale,
 // clears the marks, and stores the listed count in bar[10].
-__global__ __launch_bounds__(1024) void k_ws_tiles(Dev d, uint32_t cap) {
+template <class L>
+__global__ __launch_bounds__(1024) void k_ws_tiles(L l, uint32_t cap) {
+  const View v = view(l);
+  const Dev& d = v.d;
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t base;
   const uint32_t T = d.tdx * d.tdy * d.tdz;
@@ -1583,10 +1639,13 @@ __global__ __launch_bounds__(1024) void k_ws_tiles(Dev d, uint32_t cap) {
 
 // The network's inputs of the listed tiles: FP32 into ws_in ([slot][voxel][2]) and BF16 into
 // the network's input buffer ([slot][voxel][8], channels 2..7 zero).
-__global__ __launch_bounds__(256) void k_ws_gather(Dev d, bf16x8* __restrict__ net_in) {
+template <class L>
+__global__ __launch_bounds__(256) void k_ws_gather(L l, bf16x8* __restrict__ net_in) {
+  const View v = view(l);
+  const Dev& d = v.d;
   const uint32_t nb = n_blocks(d);
   const size_t total = (size_t)d.bar[10] * 4096u;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = (size_t)v.wg * blockDim.x + threadIdx.x; i < total; i += (size_t)v.wgs * blockDim.x) {
     const uint32_t slot = (uint32_t)(i >> 12), v = (uint32_t)(i & 4095u);
     const uint32_t cell = tile_voxel_cell(d, d.tile_list[slot], v, nb);
     float x = 0.0f, kind = 0.0f;
@@ -1615,10 +1674,13 @@ __global__ __launch_bounds__(256) void k_ws_gather(Dev d, bf16x8* __restrict__ n
 
 // guess = p_carried + s o on the water cells of the listed tiles (o: channel 0 of `pred`,
 // [slot][voxel][16] FP32).
-__global__ __launch_bounds__(256) void k_ws_scatter(Dev d, const float* __restrict__ pred) {
+template <class L>
+__global__ __launch_bounds__(256) void k_ws_scatter(L l, const float* __restrict__ pred) {
+  const View v = view(l);
+  const Dev& d = v.d;
   const uint32_t nb = n_blocks(d);
   const size_t total = (size_t)d.bar[10] * 4096u;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = (size_t)v.wg * blockDim.x + threadIdx.x; i < total; i += (size_t)v.wgs * blockDim.x) {
     const uint32_t slot = (uint32_t)(i >> 12), v = (uint32_t)(i & 4095u);
     const uint32_t cell = tile_voxel_cell(d, d.tile_list[slot], v, nb);
     if (cell == NONE || cfr(d, NATRAY_F_TYPE, cell) != WATER) continue;
@@ -1627,9 +1689,12 @@ __global__ __launch_bounds__(256) void k_ws_scatter(Dev d, const float* __restri
 }
 
 // Per block: the squared residuals of the guess and of the carried pressure (lane tree, FP64).
-__global__ __launch_bounds__(256) void k_ws_check(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_ws_check(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
-  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
     const uint32_t cell = b * 64u + lane;
     const Row w = cell_row(d, cell, lx, ly, lz, lk, d.guess);
     const double rg = w.water ? (double)w.r : 0.0, r0 = w.water ? (double)d.ws_r0[cell] : 0.0;
@@ -1643,7 +1708,10 @@ __global__ __launch_bounds__(256) void k_ws_check(Dev d) {
 
 // One workgroup: sums the block residuals in a fixed order and accepts the guess (bar[11] = 1)
 // when its residual is the smaller; keeps the sums and counts in ws_stat.
-__global__ __launch_bounds__(1024) void k_ws_decide(Dev d) {
+template <class L>
+__global__ __launch_bounds__(1024) void k_ws_decide(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   __shared__ double sh[2][1024];
   if (d.st->dt_ns == 0u) return;
   const uint32_t n = n_blocks(d);
@@ -1674,11 +1742,14 @@ __global__ __launch_bounds__(1024) void k_ws_decide(Dev d) {
 
 // Training samples of the last solve on the listed tiles, [slot][voxel]: target
 // (p - p_carried) / s, the water mask, and the operator's diagonal and water-neighbour bits.
-__global__ __launch_bounds__(256) void k_ws_samples(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_ws_samples(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   const uint32_t nb = n_blocks(d);
   const float* cP = d.cf + (size_t)NATRAY_F_P * d.capc;
   const size_t total = (size_t)d.bar[10] * 4096u;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = (size_t)v.wg * blockDim.x + threadIdx.x; i < total; i += (size_t)v.wgs * blockDim.x) {
     const uint32_t slot = (uint32_t)(i >> 12), v = (uint32_t)(i & 4095u);
     const uint32_t cell = tile_voxel_cell(d, d.tile_list[slot], v, nb);
     const float s = d.tile_scale[slot];
@@ -1692,10 +1763,13 @@ __global__ __launch_bounds__(256) void k_ws_samples(Dev d) {
 
 // ---- project, extend_1, extend_2 (fluid.wgsl:580-674) ------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_project(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_project(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   const float k = dt_of(d) / (d.U->fluid[0] * d.U->origin[3]);
-  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u,
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u,
              [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
                const uint32_t cell = b * 64u + lane;
                const float ty = cfr(d, NATRAY_F_TYPE, cell);
@@ -1720,9 +1794,12 @@ __device__ __forceinline__ bool face_set_at(const Dev& d, const uint32_t* lk, in
   return cell_type(d, nbr(lk, lx, ly, lz, ox - (dim == 0), oy - (dim == 1), oz - (dim == 2))) == WATER;
 }
 
-__global__ __launch_bounds__(256) void k_extend1(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_extend1(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
-  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u,
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u,
              [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
                const uint32_t cell = b * 64u + lane;
                uint32_t done = 0;
@@ -1757,9 +1834,12 @@ __device__ __forceinline__ bool ext_set(const Dev& d, uint32_t cell, uint32_t di
   return cell != NONE && ((((uint32_t)cfr(d, NATRAY_F_EXT_SET, cell)) >> dim) & 1u) != 0u;
 }
 
-__global__ __launch_bounds__(256) void k_extend2(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) void k_extend2(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
-  for_blocks(d, blockIdx.x * 4u, n_blocks(d), gridDim.x * 4u,
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u,
              [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
                const uint32_t cell = b * 64u + lane;
                const float ty = cfr(d, NATRAY_F_TYPE, cell);
@@ -1921,7 +2001,10 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
   return __builtin_sqrtf((vv[0] * vv[0] + vv[1] * vv[1]) + vv[2] * vv[2]);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(Dev d) {
+template <class L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   __shared__ float tg[8 * TILE];
   __shared__ uint32_t links[27];
@@ -1961,13 +2044,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int off = 32; off > 0; off >>= 1) fastest = fmax_(fastest, __shfl_xor(fastest, off));
   if (lane_id() == 0) s_fast[tid >> 6] = fastest;
   __syncthreads();
-  if (tid == 0) d.vmax_part[blockIdx.x] = fmax_(fmax_(s_fast[0], s_fast[1]), fmax_(s_fast[2], s_fast[3]));
+  if (tid == 0) d.vmax_part[v.wg] = fmax_(fmax_(s_fast[0], s_fast[1]), fmax_(s_fast[2], s_fast[3]));
 }
 
 // The step's fastest particle from k_g2p's per-workgroup maxima (the atomicMax of
 // fluid.wgsl:898-900 as a reduction).
-__global__ __launch_bounds__(256) void k_vmax(Dev d, uint32_t parts) {
+template <class L>
+__global__ __launch_bounds__(256) void k_vmax(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
+  const uint32_t parts = d.g2p_wgs;
   __shared__ float s[4];
   float m = 0.0f;
   for (uint32_t k = threadIdx.x; k < parts; k += 256) m = fmax_(m, d.vmax_part[k]);
@@ -1981,75 +2068,99 @@ __global__ __launch_bounds__(256) void k_vmax(Dev d, uint32_t parts) {
 
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-int launch_keys(const Dev& d, void* stream) {
-  if (d.N) hipLaunchKernelGGL(k_keys, dim3((d.N + 255) / 256), dim3(256), 0, S(stream), d);
+// Launches kernel template K on target t (one scene or a batch) with the workgroups of `kind`.
+#define NATRAY_LAUNCH(K, kind, threads, t, stream, ...)                                                      \
+  do {                                                                                                       \
+    if ((t).d)                                                                                               \
+      hipLaunchKernelGGL(K<One>, dim3((t).grid[kind]), dim3(threads), 0, stream, One{*(t).d}, ##__VA_ARGS__); \
+    else                                                                                                     \
+      hipLaunchKernelGGL(K<Many>, dim3((t).grid[kind]), dim3(threads), 0, stream, Many{(t).scenes, (t).map[kind]}, \
+                         ##__VA_ARGS__);                                                                     \
+  } while (0)
+
+int launch_keys(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_keys, LK_PARTS, 256, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_begin(const Dev& d, void* stream) {
-  hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, S(stream), d);
+int launch_begin(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_begin, LK_ONE, 64, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_bin(const Dev& d, void* stream, int grid_cells) {
-  hipLaunchKernelGGL(k_clear, dim3(grid_cells), dim3(256), 0, S(stream), d);
-  const uint32_t gp = d.N ? (d.N + 255) / 256 : 1;
-  hipLaunchKernelGGL(k_mark, dim3(gp), dim3(256), 0, S(stream), d);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(d.ntiles), dim3(256), 0, S(stream), d);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, S(stream), d);
-  hipLaunchKernelGGL(k_scan_scatter, dim3(d.ntiles), dim3(256), 0, S(stream), d);
-  hipLaunchKernelGGL(k_sort, dim3(gp), dim3(256), 0, S(stream), d);
+int launch_bin(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_clear, LK_CELLS, 256, t, S(stream));
+  NATRAY_LAUNCH(k_mark, LK_PARTS, 256, t, S(stream));
+  NATRAY_LAUNCH(k_scan_reduce, LK_TILES, 256, t, S(stream));
+  NATRAY_LAUNCH(k_scan_tiles, LK_ONE, 1024, t, S(stream));
+  NATRAY_LAUNCH(k_scan_scatter, LK_TILES, 256, t, S(stream));
+  NATRAY_LAUNCH(k_sort, LK_PARTS, 256, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_p2g(const Dev& d, void* stream, int grid_blocks) {
-  hipLaunchKernelGGL(k_p2g, dim3(grid_blocks), dim3(256), 0, S(stream), d);
+int launch_p2g(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_p2g, LK_BLOCKS, 256, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_grid_prep(const Dev& d, void* stream, int grid_waves) {
-  hipLaunchKernelGGL(k_grid_prep, dim3(grid_waves), dim3(256), 0, S(stream), d);
+int launch_grid_prep(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_grid_prep, LK_WAVES, 256, t, S(stream));
   return (int)hipGetLastError();
 }
 int solve_max_grid(int mixed, int* per_cu) {
   int occ = 0;
-  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<true>, SOLVE_THREADS, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<false>, SOLVE_THREADS, 0);
-  *per_cu = occ;
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<true, One>, SOLVE_THREADS, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<false, One>, SOLVE_THREADS, 0);
+  int occ_many = 0;
+  if (e == hipSuccess)
+    e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_many, k_solve<true, Many>, SOLVE_THREADS, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_many, k_solve<false, Many>, SOLVE_THREADS, 0);
+  *per_cu = occ < occ_many ? occ : occ_many;
   return (int)e;
 }
-int launch_solve(const Dev& d, void* stream, int grid, int mixed) {
-  Dev copy = d;
-  void* args[] = {&copy};
-  hipError_t e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true>, dim3(grid), dim3(SOLVE_THREADS), args, 0, S(stream))
-                       : hipLaunchCooperativeKernel((const void*)k_solve<false>, dim3(grid), dim3(SOLVE_THREADS), args, 0, S(stream));
+int launch_solve(const Target& t, void* stream, int mixed) {
+  hipError_t e;
+  const dim3 grid(t.grid[LK_SOLVE]), block(SOLVE_THREADS);
+  if (t.d) {
+    One one{*t.d};
+    void* args[] = {&one};
+    e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true, One>, grid, block, args, 0, S(stream))
+              : hipLaunchCooperativeKernel((const void*)k_solve<false, One>, grid, block, args, 0, S(stream));
+  } else {
+    Many many{t.scenes, t.map[LK_SOLVE]};
+    void* args[] = {&many};
+    e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true, Many>, grid, block, args, 0, S(stream))
+              : hipLaunchCooperativeKernel((const void*)k_solve<false, Many>, grid, block, args, 0, S(stream));
+  }
   return (int)e;
 }
-int launch_project(const Dev& d, void* stream, int grid_waves) {
-  hipLaunchKernelGGL(k_project, dim3(grid_waves), dim3(256), 0, S(stream), d);
+int launch_project(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_project, LK_WAVES, 256, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_extend(const Dev& d, void* stream, int grid_waves) {
-  hipLaunchKernelGGL(k_extend1, dim3(grid_waves), dim3(256), 0, S(stream), d);
-  hipLaunchKernelGGL(k_extend2, dim3(grid_waves), dim3(256), 0, S(stream), d);
+int launch_extend(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_extend1, LK_WAVES, 256, t, S(stream));
+  NATRAY_LAUNCH(k_extend2, LK_WAVES, 256, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_ws_inputs(const Dev& d, void* stream, int grid_waves, uint32_t cap, void* net_in) {
-  hipLaunchKernelGGL(k_ws_prepare, dim3(grid_waves), dim3(256), 0, S(stream), d);
-  hipLaunchKernelGGL(k_ws_tiles, dim3(1), dim3(1024), 0, S(stream), d, cap);
-  hipLaunchKernelGGL(k_ws_gather, dim3(1024), dim3(256), 0, S(stream), d, static_cast<bf16x8*>(net_in));
+int launch_ws_inputs(const Target& t, void* stream, uint32_t cap, void* net_in) {
+  NATRAY_LAUNCH(k_ws_prepare, LK_WAVES, 256, t, S(stream));
+  NATRAY_LAUNCH(k_ws_tiles, LK_ONE, 1024, t, S(stream), cap);
+  NATRAY_LAUNCH(k_ws_gather, LK_WAVES, 256, t, S(stream), static_cast<bf16x8*>(net_in));
   return (int)hipGetLastError();
 }
-int launch_ws_guess(const Dev& d, void* stream, int grid_waves, const float* pred) {
-  hipLaunchKernelGGL(k_ws_scatter, dim3(1024), dim3(256), 0, S(stream), d, pred);
-  hipLaunchKernelGGL(k_ws_check, dim3(grid_waves), dim3(256), 0, S(stream), d);
-  hipLaunchKernelGGL(k_ws_decide, dim3(1), dim3(1024), 0, S(stream), d);
+int launch_ws_guess(const Target& t, void* stream, const float* pred) {
+  NATRAY_LAUNCH(k_ws_scatter, LK_WAVES, 256, t, S(stream), pred);
+  NATRAY_LAUNCH(k_ws_check, LK_WAVES, 256, t, S(stream));
+  NATRAY_LAUNCH(k_ws_decide, LK_ONE, 1024, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_ws_samples(const Dev& d, void* stream) {
-  hipLaunchKernelGGL(k_ws_samples, dim3(1024), dim3(256), 0, S(stream), d);
+int launch_ws_samples(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_ws_samples, LK_WAVES, 256, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_g2p(const Dev& d, void* stream, int grid_blocks) {
-  hipLaunchKernelGGL(k_g2p, dim3(grid_blocks), dim3(256), 0, S(stream), d);
-  hipLaunchKernelGGL(k_vmax, dim3(1), dim3(256), 0, S(stream), d, (uint32_t)grid_blocks);
+int launch_g2p(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_g2p, LK_BLOCKS, 256, t, S(stream));
+  NATRAY_LAUNCH(k_vmax, LK_ONE, 256, t, S(stream));
   return (int)hipGetLastError();
 }
+
+#undef NATRAY_LAUNCH
 
 }  // namespace natray

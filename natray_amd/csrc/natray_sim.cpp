@@ -7,6 +7,7 @@
 // and CG budget adaptation (src/sim.rs:498-522,636-645).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -52,6 +53,7 @@ struct natray_sim {
   int device = 0;
   int num_cus = 0;
   int grid_cells = 0, grid_blocks = 0, grid_waves = 0, solve_grid = 0;
+  natray::Target tgt;  // this scene as the kernels' launch target
   int mixed = 0;
   // Upper bound of the reduction tags the solves enqueued since the tagged words were last
   // cleared (see clear_tags).
@@ -161,11 +163,11 @@ int enqueue_warm_start(natray_sim* s) {
   void* st = s->stream;
   const bool guess = s->ws_mode == 2;
   const uint32_t cap = guess ? std::min(D.ws_max, natray::net::max_tiles(s->ws_net)) : D.ws_max;
-  LAUNCHCHK(natray::launch_ws_inputs(D, st, s->grid_waves, cap, guess ? natray::net::input_buffer(s->ws_net) : nullptr));
+  LAUNCHCHK(natray::launch_ws_inputs(s->tgt, st, cap, guess ? natray::net::input_buffer(s->ws_net) : nullptr));
   if (!guess) return NATRAY_OK;
   int e = natray::net::forward_device(s->ws_net, cap, D.bar + 10, st);
   if (e) return fail(e, std::string("warm-start network: ") + natray_net_last_error());
-  LAUNCHCHK(natray::launch_ws_guess(D, st, s->grid_waves, natray::net::output_buffer(s->ws_net)));
+  LAUNCHCHK(natray::launch_ws_guess(s->tgt, st, natray::net::output_buffer(s->ws_net)));
   return NATRAY_OK;
 }
 
@@ -206,7 +208,6 @@ int alloc_warm_start(natray_sim* s) {
 }
 
 int enqueue_phase(natray_sim* s, int phase) {
-  const natray::Dev& D = s->D;
   void* st = s->stream;
   size_t slot = 0;
   bool timed = s->prof && !s->capturing && s->ev_used < s->events.size() / 2;
@@ -216,10 +217,10 @@ int enqueue_phase(natray_sim* s, int phase) {
     HIPCHK(hipEventRecord(s->events[2 * slot], s->stream));
   }
   switch (phase) {
-    case NATRAY_PHASE_BEGIN: LAUNCHCHK(natray::launch_begin(D, st)); break;
-    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(D, st, s->grid_cells)); break;
-    case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(D, st, s->grid_blocks)); break;
-    case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(D, st, s->grid_waves)); break;
+    case NATRAY_PHASE_BEGIN: LAUNCHCHK(natray::launch_begin(s->tgt, st)); break;
+    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(s->tgt, st)); break;
+    case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(s->tgt, st)); break;
+    case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(s->tgt, st)); break;
     case NATRAY_PHASE_SOLVE: {
       // Tags are 32-bit: clear the tagged words well before a tag could repeat.
       const uint64_t bound = solve_tag_bound(s);
@@ -232,12 +233,12 @@ int enqueue_phase(natray_sim* s, int phase) {
         int e = enqueue_warm_start(s);
         if (e) return e;
       }
-      LAUNCHCHK(natray::launch_solve(D, st, solve_grid_for(s), s->mixed));
+      LAUNCHCHK(natray::launch_solve(s->tgt, st, s->mixed));
       break;
     }
-    case NATRAY_PHASE_PROJECT: LAUNCHCHK(natray::launch_project(D, st, s->grid_waves)); break;
-    case NATRAY_PHASE_EXTEND: LAUNCHCHK(natray::launch_extend(D, st, s->grid_waves)); break;
-    case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(D, st, s->grid_blocks)); break;
+    case NATRAY_PHASE_PROJECT: LAUNCHCHK(natray::launch_project(s->tgt, st)); break;
+    case NATRAY_PHASE_EXTEND: LAUNCHCHK(natray::launch_extend(s->tgt, st)); break;
+    case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(s->tgt, st)); break;
     default: return fail(NATRAY_E_ARG, "unknown phase " + std::to_string(phase));
   }
   if (timed) HIPCHK(hipEventRecord(s->events[2 * slot + 1], s->stream));
@@ -428,6 +429,28 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &dself, 1));
   D.self = dself;
 
+  // Launch shapes: per-block kernels loop over the blocks in use, which only the GPU knows.
+  {
+    const uint32_t cap = D.cap;
+    s->grid_cells = s->num_cus * 4;
+    s->grid_blocks = (int)std::min<uint32_t>(cap, (uint32_t)s->num_cus * 8u);
+    s->grid_waves = (int)std::min<uint32_t>((cap + 3u) / 4u, (uint32_t)s->num_cus * 8u);
+    int per_cu = 0;
+    int e = natray::solve_max_grid(s->mixed, &per_cu);
+    if (e || per_cu <= 0) return bail(fail(NATRAY_E_HIP, "cannot size the cooperative solve"));
+    s->solve_grid = std::min(per_cu * s->num_cus, s->num_cus);
+    D.g2p_wgs = (uint32_t)s->grid_blocks;
+    natray::Target& t = s->tgt;
+    t.d = &s->D;
+    t.grid[natray::LK_ONE] = 1;
+    t.grid[natray::LK_PARTS] = n ? (n + 255u) / 256u : 1u;
+    t.grid[natray::LK_TILES] = D.ntiles;
+    t.grid[natray::LK_CELLS] = (uint32_t)s->grid_cells;
+    t.grid[natray::LK_BLOCKS] = (uint32_t)s->grid_blocks;
+    t.grid[natray::LK_WAVES] = (uint32_t)s->grid_waves;
+    t.grid[natray::LK_SOLVE] = (uint32_t)solve_grid_for(s);
+  }
+
   HIPTRY(hipMemsetAsync(D.block_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.slot_count, 0, (size_t)D.Spad * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.block_list, 0, ((size_t)D.cap + 1) * 4, s->stream));
@@ -463,20 +486,10 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   *s->h_state = st0;
   HIPTRY(hipMemcpyAsync(D.st, s->h_state, sizeof(natray_state), hipMemcpyHostToDevice, s->stream));
   TRY(upload_params(s));
-  int e = natray::launch_keys(D, s->stream);
+  int e = natray::launch_keys(s->tgt, s->stream);
   if (e) return bail(fail(NATRAY_E_HIP, std::string("keys: ") + hipGetErrorString((hipError_t)e)));
   HIPTRY(hipStreamSynchronize(s->stream));
   s->last = st0;
-
-  // Launch shapes: per-block kernels loop over the blocks in use, which only the GPU knows.
-  const uint32_t cap = D.cap;
-  s->grid_cells = s->num_cus * 4;
-  s->grid_blocks = (int)std::min<uint32_t>(cap, (uint32_t)s->num_cus * 8u);
-  s->grid_waves = (int)std::min<uint32_t>((cap + 3u) / 4u, (uint32_t)s->num_cus * 8u);
-  int per_cu = 0;
-  e = natray::solve_max_grid(s->mixed, &per_cu);
-  if (e || per_cu <= 0) return bail(fail(NATRAY_E_HIP, "cannot size the cooperative solve"));
-  s->solve_grid = std::min(per_cu * s->num_cus, s->num_cus);
   *out = s;
   return NATRAY_OK;
 #undef TRY
@@ -592,7 +605,7 @@ int natray_sim_read_particles(natray_sim* s, float* dst) {
 int natray_sim_write_particles(natray_sim* s, const float* src) {
   if (!s || (!src && s->D.N)) return fail(NATRAY_E_ARG, "null argument");
   if (s->D.N) HIPCHK(hipMemcpyAsync(s->D.P, src, (size_t)s->D.N * 80, hipMemcpyHostToDevice, s->stream));
-  LAUNCHCHK(natray::launch_keys(s->D, s->stream));
+  LAUNCHCHK(natray::launch_keys(s->tgt, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return NATRAY_OK;
 }
@@ -733,7 +746,7 @@ int natray_sim_ws_samples(natray_sim* s, float* inputs, float* targets, float* m
                           uint32_t max_tiles, uint32_t* tiles) {
   if (!s || !tiles) return fail(NATRAY_E_ARG, "null argument");
   if (!s->ws_ready) return fail(NATRAY_E_ARG, "the warm start was never enabled");
-  LAUNCHCHK(natray::launch_ws_samples(s->D, s->stream));
+  LAUNCHCHK(natray::launch_ws_samples(s->tgt, s->stream));
   HIPCHK(hipMemcpyAsync(s->h_words + 10, s->D.bar + 10, 4, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   const uint32_t n = std::min(s->h_words[10], max_tiles);
@@ -774,6 +787,371 @@ int natray_sim_counters(natray_sim* s, uint64_t out[8]) {
   HIPCHK(hipStreamSynchronize(s->stream));
   for (int k = 0; k < 8; ++k) out[k] = h[k];
   return NATRAY_OK;
+}
+
+}  // extern "C"
+
+// ---- batches of independent scenes --------------------------------------------------------------
+
+struct natray_batch {
+  std::vector<natray_sim*> sims;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  int mixed = 0;
+  int solve_budget = 0;  // co-resident workgroups of the cooperative solve
+  natray_batch_shape shape{};
+  natray::Dev* scenes = nullptr;                   // device table, one Dev per scene
+  natray::WgMap* maps[natray::LK_KINDS] = {};       // device workgroup maps, per kind
+  size_t map_cap[natray::LK_KINDS] = {};            // their capacities (entries)
+  std::vector<uint32_t> counts[natray::LK_KINDS];  // workgroups per scene, per kind
+  natray::Target tgt;
+  bool use_graph = true;
+  hipGraphExec_t graph_pre = nullptr, graph_post = nullptr;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+void batch_free(natray_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph_pre) (void)hipGraphExecDestroy(b->graph_pre);
+  if (b->graph_post) (void)hipGraphExecDestroy(b->graph_post);
+  b->graph_pre = b->graph_post = nullptr;
+  for (void* p : b->allocs) (void)hipFree(p);
+  b->allocs.clear();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
+// Shares of `budget` in proportion to `w`, each within [lo, hi[k]]: x_k = clamp(t w_k, lo, hi_k)
+// with t chosen (bisection; x is monotone in t) so that the x_k sum to the budget, then the
+// floors of x_k plus one more for the largest fractional parts (ties to the lower index). When
+// the bounds cannot meet the budget, every share sits at its bound.
+void plan_shares(const std::vector<uint64_t>& w, const std::vector<uint32_t>& hi, uint32_t budget, uint32_t lo,
+                 std::vector<uint32_t>& out) {
+  const size_t n = w.size();
+  out.assign(n, 0);
+  std::vector<double> x(n);
+  auto fill = [&](double t) {
+    double sum = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+      const double h = (double)hi[k], l = std::min((double)lo, h);
+      x[k] = std::min(h, std::max(l, t * (double)std::max<uint64_t>(w[k], 1)));
+      sum += x[k];
+    }
+    return sum;
+  };
+  double t_lo = 0.0, t_hi = 1.0;
+  if (fill(0.0) >= (double)budget) {
+    fill(0.0);
+  } else {
+    while (fill(t_hi) < (double)budget && t_hi < 1e30) t_hi *= 2.0;
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (t_lo + t_hi);
+      (fill(mid) < (double)budget ? t_lo : t_hi) = mid;
+    }
+    fill(t_hi);
+  }
+  uint64_t used = 0;
+  std::vector<std::pair<double, size_t>> frac;
+  for (size_t k = 0; k < n; ++k) {
+    out[k] = (uint32_t)std::floor(x[k] + 1e-9);
+    used += out[k];
+    if (out[k] < hi[k]) frac.push_back({x[k] - (double)out[k], k});
+  }
+  std::stable_sort(frac.begin(), frac.end(),
+                   [](const std::pair<double, size_t>& a, const std::pair<double, size_t>& b) { return a.first > b.first; });
+  for (size_t i = 0; i < frac.size() && used < budget; ++i) {
+    out[frac[i].second] += 1;
+    used += 1;
+  }
+}
+
+// Workgroup counts per scene and kind for the batch's shape, the scenes' Dev copies (with their
+// G2P workgroup counts) and the workgroup maps, uploaded; the step graphs are dropped.
+int batch_layout(natray_batch* b) {
+  const size_t n = b->sims.size();
+  std::vector<uint64_t> wpart(n);
+  std::vector<uint32_t> cap_blocks(n), cap_waves(n), unbounded(n, 0xffffffffu);
+  for (size_t k = 0; k < n; ++k) {
+    const natray_sim* s = b->sims[k];
+    wpart[k] = std::max<uint64_t>(s->D.N, 1);
+    cap_blocks[k] = s->D.cap;
+    cap_waves[k] = (s->D.cap + 3u) / 4u;
+  }
+  const uint32_t cus = (uint32_t)b->num_cus;
+  b->counts[natray::LK_ONE].assign(n, 1u);
+  auto& parts = b->counts[natray::LK_PARTS];
+  auto& tiles = b->counts[natray::LK_TILES];
+  parts.resize(n);
+  tiles.resize(n);
+  for (size_t k = 0; k < n; ++k) {
+    const natray_sim* s = b->sims[k];
+    parts[k] = s->D.N ? (s->D.N + 255u) / 256u : 1u;
+    tiles[k] = s->D.ntiles;
+  }
+  plan_shares(wpart, unbounded, std::max<uint32_t>(b->shape.cells_per_cu * cus, (uint32_t)n), 1, b->counts[natray::LK_CELLS]);
+  plan_shares(wpart, cap_blocks, std::max<uint32_t>(b->shape.blocks_per_cu * cus, (uint32_t)n), 1, b->counts[natray::LK_BLOCKS]);
+  plan_shares(wpart, cap_waves, std::max<uint32_t>(b->shape.waves_per_cu * cus, (uint32_t)n), 1, b->counts[natray::LK_WAVES]);
+  plan_shares(wpart, unbounded, (uint32_t)b->solve_budget, 1, b->counts[natray::LK_SOLVE]);
+  uint64_t solve_total = 0;
+  for (uint32_t c : b->counts[natray::LK_SOLVE]) solve_total += c;
+  if (solve_total > (uint64_t)b->solve_budget)
+    return fail(NATRAY_E_ARG, "a batch holds at most one scene per co-resident solve workgroup");
+  // Scene table.
+  std::vector<natray::Dev> table(n);
+  for (size_t k = 0; k < n; ++k) {
+    table[k] = b->sims[k]->D;
+    table[k].g2p_wgs = b->counts[natray::LK_BLOCKS][k];
+  }
+  HIPCHK(hipMemcpyAsync(b->scenes, table.data(), n * sizeof(natray::Dev), hipMemcpyHostToDevice, b->stream));
+  // Workgroup maps.
+  for (int kind = 0; kind < natray::LK_KINDS; ++kind) {
+    std::vector<natray::WgMap> m;
+    for (size_t k = 0; k < n; ++k)
+      for (uint32_t w = 0; w < b->counts[kind][k]; ++w) m.push_back({(uint32_t)k, w, b->counts[kind][k], 0u});
+    if (m.size() > b->map_cap[kind]) {
+      void* q = nullptr;
+      hipError_t e = hipMalloc(&q, m.size() * sizeof(natray::WgMap));
+      if (e != hipSuccess) return fail(NATRAY_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+      b->allocs.push_back(q);
+      b->maps[kind] = static_cast<natray::WgMap*>(q);
+      b->map_cap[kind] = m.size();
+    }
+    HIPCHK(hipMemcpyAsync(b->maps[kind], m.data(), m.size() * sizeof(natray::WgMap), hipMemcpyHostToDevice, b->stream));
+    b->tgt.map[kind] = b->maps[kind];
+    b->tgt.grid[kind] = (uint32_t)m.size();
+  }
+  b->tgt.d = nullptr;
+  b->tgt.scenes = b->scenes;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->graph_pre) (void)hipGraphExecDestroy(b->graph_pre);
+  if (b->graph_post) (void)hipGraphExecDestroy(b->graph_post);
+  b->graph_pre = b->graph_post = nullptr;
+  return NATRAY_OK;
+}
+
+int batch_phase(natray_batch* b, int phase) {
+  const natray::Target& t = b->tgt;
+  void* st = b->stream;
+  switch (phase) {
+    case NATRAY_PHASE_BEGIN: LAUNCHCHK(natray::launch_begin(t, st)); break;
+    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(t, st)); break;
+    case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(t, st)); break;
+    case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(t, st)); break;
+    case NATRAY_PHASE_SOLVE: {
+      for (natray_sim* s : b->sims) {
+        const uint64_t bound = solve_tag_bound(s);
+        if (s->tags_used + bound >= (1ull << 31)) {
+          HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, b->stream));
+          HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 8, b->stream));
+          HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, b->stream));
+          s->tags_used = 0;
+        }
+        s->tags_used += bound;
+      }
+      LAUNCHCHK(natray::launch_solve(t, st, b->mixed));
+      break;
+    }
+    case NATRAY_PHASE_PROJECT: LAUNCHCHK(natray::launch_project(t, st)); break;
+    case NATRAY_PHASE_EXTEND: LAUNCHCHK(natray::launch_extend(t, st)); break;
+    case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(t, st)); break;
+    default: return fail(NATRAY_E_ARG, "unknown phase " + std::to_string(phase));
+  }
+  return NATRAY_OK;
+}
+
+int batch_capture(natray_batch* b, int lo, int hi, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int e = NATRAY_OK;
+  for (int ph = lo; ph < hi && e == NATRAY_OK; ++ph) e = batch_phase(b, ph);
+  hipError_t ec = hipStreamEndCapture(b->stream, &g);
+  if (e) {
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+  }
+  if (ec != hipSuccess) return fail(NATRAY_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  ec = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ec != hipSuccess) return fail(NATRAY_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+  return NATRAY_OK;
+}
+
+int batch_step(natray_batch* b) {
+  if (!b->use_graph) {
+    for (int ph = 0; ph < NATRAY_N_PHASES; ++ph) {
+      int e = batch_phase(b, ph);
+      if (e) return e;
+    }
+    return NATRAY_OK;
+  }
+  if (!b->graph_pre) {
+    int e = batch_capture(b, NATRAY_PHASE_BEGIN, NATRAY_PHASE_SOLVE, &b->graph_pre);
+    if (e) return e;
+    e = batch_capture(b, NATRAY_PHASE_PROJECT, NATRAY_N_PHASES, &b->graph_post);
+    if (e) return e;
+  }
+  HIPCHK(hipGraphLaunch(b->graph_pre, b->stream));
+  int e = batch_phase(b, NATRAY_PHASE_SOLVE);
+  if (e) return e;
+  HIPCHK(hipGraphLaunch(b->graph_post, b->stream));
+  return NATRAY_OK;
+}
+
+// n_steps steps of every scene; the scenes' States are read back and checked afterwards.
+// Events t0 and t1, when given, are recorded on the batch's stream around the steps.
+int batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+  for (natray_sim* s : b->sims) {
+    s->U.limits[2] = iters ? iters : s->U.limits[2];
+    s->U.limits[3] = 0xffffffffu;
+    *s->h_params = s->U;
+    HIPCHK(hipMemcpyAsync(const_cast<natray_params*>(s->D.U), s->h_params, sizeof(natray_params), hipMemcpyHostToDevice,
+                          b->stream));
+    s->h_words[4] = 0;
+    s->h_words[5] = 0;
+    HIPCHK(hipMemcpyAsync(&s->D.st->iters_max, &s->h_words[4], 2 * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+  }
+  if (t0) HIPCHK(hipEventRecord(t0, b->stream));
+  for (uint32_t k = 0; k < n_steps; ++k) {
+    int e = batch_step(b);
+    if (e) return e;
+  }
+  if (t1) HIPCHK(hipEventRecord(t1, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (size_t k = 0; k < b->sims.size(); ++k) {
+    int e = sync_state(b->sims[k]);
+    if (e) return e;
+    if (b->sims[k]->last.overflow)
+      return fail(NATRAY_E_OVERFLOW, "scene " + std::to_string(k) + " needs more grid blocks than its capacity");
+  }
+  return NATRAY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int natray_batch_plan(const uint64_t* weights, const uint32_t* max_each, uint32_t n, uint32_t budget, uint32_t min_each,
+                      uint32_t* out) {
+  if (!weights || !out || n == 0) return fail(NATRAY_E_ARG, "null argument");
+  std::vector<uint64_t> w(weights, weights + n);
+  std::vector<uint32_t> hi(n, 0xffffffffu), shares;
+  if (max_each) hi.assign(max_each, max_each + n);
+  plan_shares(w, hi, budget, min_each, shares);
+  for (uint32_t k = 0; k < n; ++k) out[k] = shares[k];
+  return NATRAY_OK;
+}
+
+int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out) {
+  if (!sims || !out || n == 0) return fail(NATRAY_E_ARG, "a batch needs at least one scene");
+  *out = nullptr;
+  for (uint32_t k = 0; k < n; ++k) {
+    if (!sims[k]) return fail(NATRAY_E_ARG, "null scene");
+    if (sims[k]->mixed != sims[0]->mixed) return fail(NATRAY_E_ARG, "the scenes of a batch must share the CG precision");
+    if (sims[k]->device != sims[0]->device) return fail(NATRAY_E_ARG, "the scenes of a batch must be on one device");
+    if (sims[k]->ws_mode) return fail(NATRAY_E_ARG, "the warm start runs on single scenes, not in a batch");
+    for (uint32_t j = 0; j < k; ++j)
+      if (sims[j] == sims[k]) return fail(NATRAY_E_ARG, "a scene appears twice in the batch");
+  }
+  natray_batch* b = new natray_batch();
+  b->sims.assign(sims, sims + n);
+  b->num_cus = sims[0]->num_cus;
+  b->mixed = sims[0]->mixed;
+  b->solve_budget = sims[0]->solve_grid;
+  b->shape = {8u, 8u, 4u, 0u};
+  auto bail = [&](int code) {
+    batch_free(b);
+    delete b;
+    return code;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(fail(NATRAY_E_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e)));
+  void* q = nullptr;
+  e = hipMalloc(&q, (size_t)n * sizeof(natray::Dev));
+  if (e != hipSuccess) return bail(fail(NATRAY_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)));
+  b->allocs.push_back(q);
+  b->scenes = static_cast<natray::Dev*>(q);
+  // The scenes' own streams must be idle before the batch's stream works on their buffers.
+  for (uint32_t k = 0; k < n; ++k) {
+    e = hipStreamSynchronize(sims[k]->stream);
+    if (e != hipSuccess) return bail(fail(NATRAY_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e)));
+  }
+  int rc = batch_layout(b);
+  if (rc) return bail(rc);
+  *out = b;
+  return NATRAY_OK;
+}
+
+void natray_batch_destroy(natray_batch* b) {
+  if (!b) return;
+  batch_free(b);
+  delete b;
+}
+
+int natray_batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters) {
+  if (!b) return fail(NATRAY_E_ARG, "null handle");
+  for (natray_sim* s : b->sims) HIPCHK(hipStreamSynchronize(s->stream));
+  return batch_run(b, n_steps, iters);
+}
+
+int natray_batch_set_graph(natray_batch* b, int on) {
+  if (!b) return fail(NATRAY_E_ARG, "null handle");
+  b->use_graph = on != 0;
+  return NATRAY_OK;
+}
+
+int natray_batch_set_shape(natray_batch* b, const natray_batch_shape* shape) {
+  if (!b || !shape) return fail(NATRAY_E_ARG, "null argument");
+  if (shape->blocks_per_cu < 1 || shape->blocks_per_cu > 64 || shape->waves_per_cu < 1 || shape->waves_per_cu > 64 ||
+      shape->cells_per_cu < 1 || shape->cells_per_cu > 64)
+    return fail(NATRAY_E_ARG, "workgroups per CU must be 1..64");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->shape = *shape;
+  return batch_layout(b);
+}
+
+int natray_batch_get_shape(natray_batch* b, natray_batch_shape* out) {
+  if (!b || !out) return fail(NATRAY_E_ARG, "null argument");
+  *out = b->shape;
+  return NATRAY_OK;
+}
+
+int natray_batch_tune(natray_batch* b, uint32_t steps, uint32_t iters, natray_batch_shape* best, double* ms,
+                      uint32_t max_candidates, uint32_t* n_candidates) {
+  if (!b || steps == 0) return fail(NATRAY_E_ARG, "null handle or no steps");
+  const uint32_t blocks[] = {4u, 8u, 16u}, waves[] = {4u, 8u, 16u};
+  std::vector<natray_batch_shape> cand;
+  for (uint32_t bl : blocks)
+    for (uint32_t wv : waves) cand.push_back({bl, wv, 4u, 0u});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best_ms = 0.0;
+  size_t best_k = 0;
+  int rc = NATRAY_OK;
+  for (size_t k = 0; k < cand.size() && rc == NATRAY_OK; ++k) {
+    rc = natray_batch_set_shape(b, &cand[k]);
+    if (rc) break;
+    rc = batch_run(b, 1, iters);  // captures the graphs of this shape
+    if (rc) break;
+    rc = batch_run(b, steps, iters, e0, e1);
+    if (rc) break;
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    const double per = (double)t / steps;
+    if (ms && k < max_candidates) ms[k] = per;
+    if (k == 0 || per < best_ms) {
+      best_ms = per;
+      best_k = k;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (n_candidates) *n_candidates = (uint32_t)cand.size();
+  if (best) *best = cand[best_k];
+  return natray_batch_set_shape(b, &cand[best_k]);
 }
 
 }  // extern "C"

@@ -324,3 +324,34 @@ def balloon_impact(side: int = 256, n: int = 2_097_152, seed: int = 2, speed: fl
     sc.mask_p = 0.0
     return Scene(U, P, sc, no_limits_state(),
                  {"workload": f"balloon impact {n} particles {side}^3", "r_eff": r_eff, "dx": dx})
+
+
+def batch_impacts(count: int = 64, seed: int = 4, n_lo: int = 16384, n_hi: int = 524288, side_lo: int = 64,
+                  side_hi: int = 192, speed_lo: float = 200.0, speed_hi: float = 400.0, precision: int = abi.CG_MIXED,
+                  iters: int = 1000) -> list[Scene]:
+    """The batched scenes: `count` independent balloon impacts. Scene k draws from its own
+    seed (splitmix64 of `seed`, k-th value): a particle count log-uniform in [n_lo, n_hi]
+    (multiple of 8), a grid side uniform in [side_lo, side_hi] (multiple of 4), a capsule
+    speed uniform in [speed_lo, speed_hi] and a direction uniform on the sphere. FP32 state;
+    the CG precision is `precision` (mixed, refresh every 20 iterations, by default). With the
+    defaults: 64 scenes of 16k..512k particles on 64^3..192^3."""
+    seeds = splitmix64(seed, count)
+    out = []
+    for k in range(count):
+        u = uniform01(int(seeds[k]), 5)
+        n = 8 * max(1, int(round(math.exp(math.log(n_lo) + u[0] * (math.log(n_hi) - math.log(n_lo))) / 8)))
+        side = 4 * int(round((side_lo + u[1] * (side_hi - side_lo)) / 4))
+        # The water ball and the capsule's start must fit inside the walls.
+        while (3.0 * (n // 8) / (4.0 * math.pi)) ** (1.0 / 3.0) + 10 > 0.5 * side:
+            side += 4
+        speed = speed_lo + u[2] * (speed_hi - speed_lo)
+        z = 2.0 * u[3] - 1.0
+        phi = 2.0 * math.pi * u[4]
+        rho = math.sqrt(max(0.0, 1.0 - z * z))
+        direction = (rho * math.cos(phi), rho * math.sin(phi), z)
+        cap = min((side // 4) ** 3, 8 * (n // 512) + 2048)
+        s = balloon_impact(side=side, n=n, seed=int(seeds[k] & 0xFFFFFFFF), speed=speed, direction=direction,
+                           precision=precision, iters=iters, capacity=cap)
+        s.meta.update({"scene": k, "speed": speed, "direction": direction, "capacity": cap})
+        out.append(s)
+    return out

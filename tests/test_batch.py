@@ -1,0 +1,100 @@
+"""Batches of scenes on the GPU: every scene of a batch gives the same bits as the same scene
+stepped alone (which tests/test_gpu_parity.py holds bit-exact against the oracle), with the
+steps replayed as graphs or launched kernel by kernel, with the solve in register form (few
+scenes) and in memory form (many scenes sharing the solve's workgroups), for both CG
+precisions; and the autotuner's choice leaves the results unchanged."""
+import sys
+
+import numpy as np
+import pytest
+
+from natray_amd import Sim, abi, scenes
+from natray_amd.batch import Batch
+from tests.test_gpu_parity import assert_same_float, compare_state
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_set(precision):
+    specs = [(32, 4096, (1.0, 0.0, 0.0)), (48, 8192, (0.0, 1.0, 1.0)), (40, 2048, (-1.0, 0.3, 0.0)),
+             (64, 16384, (0.2, -1.0, 0.5))]
+    return [scenes.balloon_impact(side=side, n=n, seed=10 + k, speed=250.0 + 40 * k, direction=d, precision=precision,
+                                  iters=400, capacity=(side // 4) ** 3)
+            for k, (side, n, d) in enumerate(specs)]
+
+
+def many_set(count=64):
+    """Scenes with more blocks than their share of the solve's workgroups holds in registers."""
+    return [scenes.balloon_impact(side=48, n=32768, seed=100 + k, speed=300.0, precision=abi.CG_MIXED, iters=400,
+                                  direction=(1.0, 0.1 * k, 0.0), capacity=(48 // 4) ** 3)
+            for k in range(count)]
+
+
+def alone(scene_list, steps):
+    out = []
+    for s in scene_list:
+        sim = Sim.from_scene(s)
+        sim.run(steps, iters=int(s.params.limits[2]))
+        out.append((sim.particles(), sim.state()))
+        sim.close()
+    return out
+
+
+def batched(scene_list, steps, graph=True, tune=False):
+    sims = [Sim.from_scene(s) for s in scene_list]
+    b = Batch(sims)
+    b.set_graph(graph)
+    chosen = None
+    if tune:
+        chosen = b.tune(1, iters=400)
+        steps -= 2 * 9  # the tuner's steps (one untimed and one timed per candidate) are real steps
+    b.run(steps, iters=400)
+    out = [(s.particles(), s.state()) for s in sims]
+    b.close()
+    for s in sims:
+        s.close()
+    return out, chosen
+
+
+STATE_KEYS = ("t_ns", "dt_ns", "step", "n_blocks", "n_blocks_prev", "overflow", "solve_iters", "iters_max",
+              "unconverged", "free_dt_ns")
+
+
+def check_same(tag, got, want, keys=STATE_KEYS):
+    for k, ((Pg, sg), (Pw, sw)) in enumerate(zip(got, want)):
+        compare_state(f"{tag} scene {k}", sg, sw, keys=keys)
+        assert_same_float(f"{tag} scene {k} particles", Pg, Pw)
+
+
+@pytest.mark.parametrize("precision", [abi.CG_FP32, abi.CG_MIXED])
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_matches_scenes_alone(precision, graph):
+    say(f"[batch] 4 scenes, precision {precision}, graph {graph}")
+    sl = small_set(precision)
+    want = alone(sl, 6)
+    got, _ = batched(sl, 6, graph=graph)
+    check_same(f"precision {precision} graph {graph}", got, want)
+
+
+def test_batch_of_many_scenes_memory_form():
+    say("[batch] 64 scenes of 32k particles")
+    sl = many_set()
+    want = alone(sl, 3)
+    got, _ = batched(sl, 3)
+    check_same("64 scenes", got, want)
+
+
+def test_autotuned_batch_matches_scenes_alone():
+    say("[batch] autotuner")
+    sl = small_set(abi.CG_MIXED)
+    want = alone(sl, 20)
+    got, chosen = batched(sl, 20, tune=True)
+    say(f"[batch] tuner chose {chosen['best']}")
+    assert chosen["best"]["blocks_per_cu"] in (4, 8, 16) and chosen["best"]["waves_per_cu"] in (4, 8, 16)
+    # iters_max is the largest iteration count of a run call's steps; the tuner makes many calls.
+    check_same("tuned", got, want, keys=tuple(k for k in STATE_KEYS if k != "iters_max"))
