@@ -211,6 +211,60 @@ def warm_start_report(scene, args) -> dict:
     }
 
 
+def batched_report(args) -> dict:
+    """Config 4: 64 independent balloon impacts (16k..512k particles, 64^3..192^3, capsule
+    200..400 m/s in a random direction, mixed-precision CG) stepped as one batch: the
+    autotuner picks the launch shape, then timed batch steps; against the same scenes stepped
+    one after another, each on its own (its own launches and graphs)."""
+    from natray_amd import Sim, scenes
+    from natray_amd.batch import Batch
+
+    t0 = time.perf_counter()
+    sl = scenes.batch_impacts()
+    sims = [Sim.from_scene(s) for s in sl]
+    t_setup = time.perf_counter() - t0
+    b = Batch(sims)
+    tuned = b.tune(args.batch_tune_steps, iters=args.iters)
+
+    def totals():
+        c = [s.counters() for s in sims]
+        return {k: sum(x[k] for x in c) for k in c[0]}
+
+    b.run(args.warmup, iters=args.iters)
+    c0 = totals()
+    t0 = time.perf_counter()
+    b.run(args.batch_steps, iters=args.iters)
+    wall = time.perf_counter() - t0
+    c1 = totals()
+    # The same scenes one after another, each stepped alone for the same number of steps.
+    t0 = time.perf_counter()
+    for s in sims:
+        s.run(args.batch_steps, iters=args.iters)
+    wall_alone = time.perf_counter() - t0
+    c2 = totals()
+    st = [s.state() for s in sims]
+    b.close()
+    for s in sims:
+        s.close()
+    parts = c1["particles"] - c0["particles"]
+    return {
+        "workload": "64 balloon impacts, particles log-uniform in [16k, 512k], grids 64^3..192^3, capsule "
+                    "200..400 m/s in a random direction, mixed-precision CG (FP64 dots, refresh every 20), tol 1e-6",
+        "scenes": len(sl), "particles": int(sum(x.n for x in sl)), "setup_s": t_setup,
+        "autotuner": tuned,
+        "steps": args.batch_steps,
+        "ms_per_batch_step": wall * 1e3 / max(args.batch_steps, 1),
+        "particle_substeps_per_s": parts / wall if wall > 0 else 0.0,
+        "cg_iters_per_scene_step": (c1["cg_iters"] - c0["cg_iters"]) / max(c1["steps"] - c0["steps"], 1),
+        "scenes_one_after_another": {
+            "ms_per_step_of_all_scenes": wall_alone * 1e3 / max(args.batch_steps, 1),
+            "particle_substeps_per_s": (c2["particles"] - c1["particles"]) / wall_alone if wall_alone > 0 else 0.0,
+        },
+        "speedup_batch_over_one_after_another": wall_alone / wall if wall > 0 else None,
+        "unconverged_steps": int(sum(x.unconverged for x in st)),
+    }
+
+
 def _cells_by_slot(g: dict) -> dict:
     return {int(slot): i for i, slot in enumerate(g["block_list"])}
 
@@ -367,6 +421,9 @@ def main() -> int:
     ap.add_argument("--accuracy-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start report")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes report")
+    ap.add_argument("--batch-steps", type=int, default=30, help="timed steps of the 64-scene batch")
+    ap.add_argument("--batch-tune-steps", type=int, default=2, help="timed steps per autotuner candidate")
     ap.add_argument("--ws-tiles", type=int, default=50000, help="training tiles to log for the warm start")
     ap.add_argument("--ws-collect-steps", type=int, default=100, help="steps logged per training seed")
     ap.add_argument("--ws-train-steps", type=int, default=6000)
@@ -458,6 +515,8 @@ def main() -> int:
             scene, args, {"cg_iters_per_step": out["cg_iters_per_step"],
                           "solve_ms_per_step": times["solve"][0] / max(steps_done, 1),
                           "ms_per_step": out["ms_per_step"], "unconverged_steps": st.unconverged})
+    if args.workload == "impact" and not args.no_batch:
+        out["batched_scenes"] = batched_report(args)
     if args.workload == "impact" and not args.no_warm_start:
         out["learned_warm_start"] = warm_start_report(scene, args)
     if args.workload == "impact" and not args.no_accuracy:
