@@ -504,7 +504,7 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             # Counter-measured HBM bytes per launch of this kernel on this workload (FP32 form);
             # None where no profile covers it.
-            "traffic": (measured_traffic("k_solve<false>" if dominant == "solve" else kernel_names[dominant])
+            "traffic": (measured_traffic("k_solve<false,One>" if dominant == "solve" else kernel_names[dominant] + "<One>")
                         if args.workload == "impact" and args.precision == "fp32" else None),
             "avg_launch_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": bytes_per_launch,
