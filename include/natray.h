@@ -330,6 +330,24 @@ int natray_net_layer_check(uint32_t cin, uint32_t cout, int relu, const float* X
                            const float* dY, uint32_t tiles, float* Y, float* dX, float* dW, float* db);
 const char* natray_net_last_error(void);
 
+/* ---- membrane coupling ----------------------------------------------------------------------
+ * The rubber membrane's own step (Ogden forces, substeps: shaders/membrane.wgsl) is not part of
+ * this library; what the water substep runs with its vertices is. n vertices as the membrane's
+ * last substep left them: x[n][4] positions (m), v[n][4] velocities (m/s), mass[n] (0 where the
+ * knot holds a vertex), push[n][2] its push on the water (N) and its area (m^2). Each step then
+ * claims the blocks around rubber off the water (mark_rubber, membrane.wgsl:164-180), splats
+ * the rubber's area, motion and push to the grid after P2G (mem_splat, membrane.wgsl:182-216;
+ * grid_prep makes RUBBER and COVERED cells of them, fluid.wgsl:339-355) and samples the water at
+ * every vertex after the extension (mem_sample, membrane.wgsl:239-254). Sets params.mem.x = n
+ * (the reference's vertex count, src/sim.rs:428-441). Buffers grow on demand; the sampled water
+ * of the previous step, which decides which vertices are on the water, is kept while the
+ * vertex count does not grow. n = 0 leaves the membrane out. */
+int natray_sim_set_membrane(natray_sim* sim, uint32_t n, const float* x, const float* v, const float* mass,
+                            const float* push);
+/* The water sampled at each vertex by the last step: water[n][4] = face velocity (m/s) and
+ * fill (1 inside the water, 0 away from it), the reference's mem_water. */
+int natray_sim_read_membrane(natray_sim* sim, float* water);
+
 /* ---- batches of independent scenes ----------------------------------------------------------
  * No counterpart in the reference, which steps one Sim per scene (src/sim.rs). A batch steps
  * many scenes, each a natray_sim of its own (particles, grid, clock, dt and CG of its own), with

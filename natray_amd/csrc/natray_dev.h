@@ -27,8 +27,9 @@ constexpr uint32_t MAX_CHUNKS = 256;
 constexpr uint32_t RED_RECORD = 32;  // words per chunk record
 constexpr uint32_t RED_WORDS = RED_RECORD * MAX_CHUNKS;
 
-// Acc fields the transfer writes and clearing must zero. Fields 9..18 belong to the membrane,
-// air and bullet producers, which are not part of this library, and stay zero.
+// Acc fields the transfer writes and clearing must zero. Fields 9, 10 and 15..18 belong to the
+// membrane coupling and are cleared too when a membrane is set; 11..14 belong to the bullet
+// model, which is not part of this library, and stay zero.
 constexpr uint32_t ACC_CLEARED = 9;
 
 struct Dev {
@@ -76,6 +77,15 @@ struct Dev {
   double* ws_part;     // cap x 2: per-block squared residuals of the guess and the carried pressure
   double* ws_stat;     // 8: see natray_sim_ws_stats
   uint32_t tdx, tdy, tdz, ws_max;
+  // The rubber membrane's vertices as its own step left them (natray_sim_set_membrane): the
+  // coupling kernels read positions, velocities, masses and each vertex's push on the water
+  // (N) and area (m^2); mem_sample writes mem_water. mem_cap is 0 without a membrane.
+  const float4* mem_x;
+  const float4* mem_v;
+  const float* mem_mass;
+  const float2* mem_push;
+  float4* mem_water;
+  uint32_t mem_cap;
   const Dev* self;  // a device copy of this struct, for device functions that take it by pointer
   uint32_t N;
   uint32_t S;      // domain slots (product of bdims)
@@ -103,8 +113,9 @@ struct Many {
 
 // The workgroup counts of a launch, by kind of kernel: one per scene; one per 256 particles;
 // one per scan tile; grid-stride over the cells to clear; blocks of P2G and G2P; one wave per
-// block (grid, project, extend, warm start); the cooperative solve.
-enum LaunchKind { LK_ONE, LK_PARTS, LK_TILES, LK_CELLS, LK_BLOCKS, LK_WAVES, LK_SOLVE, LK_KINDS };
+// block (grid, project, extend, warm start); the cooperative solve; one per 256 membrane
+// vertices.
+enum LaunchKind { LK_ONE, LK_PARTS, LK_TILES, LK_CELLS, LK_BLOCKS, LK_WAVES, LK_SOLVE, LK_VERTS, LK_KINDS };
 
 // What a launch wrapper launches on: one scene (`d`, a host copy of its Dev) or a batch
 // (`scenes` and one workgroup map per kind, in device memory), with the workgroups per kind.
@@ -122,7 +133,7 @@ struct Target {
 namespace natray {
 int launch_keys(const Target& t, void* stream);
 int launch_begin(const Target& t, void* stream);
-int launch_bin(const Target& t, void* stream);
+int launch_bin(const Target& t, void* stream, bool rubber = false);
 int launch_p2g(const Target& t, void* stream);
 int launch_grid_prep(const Target& t, void* stream);
 int launch_solve(const Target& t, void* stream, int mixed);
@@ -130,6 +141,10 @@ int solve_max_grid(int mixed, int* per_cu);
 int launch_project(const Target& t, void* stream);
 int launch_extend(const Target& t, void* stream);
 int launch_g2p(const Target& t, void* stream);
+// Membrane coupling (one scene): the rubber's pull splatted to the grid after P2G, and the
+// water sampled at its vertices after the velocity extension.
+int launch_mem_splat(const Target& t, void* stream);
+int launch_mem_sample(const Target& t, void* stream);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also
 // into the network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has
 // run on them (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a

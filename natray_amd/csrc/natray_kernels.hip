@@ -287,7 +287,10 @@ __global__ void k_clear(L l) {
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   uint32_t n4 = d.st->n_blocks_prev * 16u;  // int4 per field
-  for (uint32_t f = 0; f < ACC_CLEARED; ++f) {
+  // The transfer fields, and with a membrane its coupling fields (A_MF, A_MA, A_MW, A_MV).
+  const uint32_t nf = ACC_CLEARED + (d.mem_cap ? 6u : 0u);
+  for (uint32_t k = 0; k < nf; ++k) {
+    const uint32_t f = k < ACC_CLEARED ? k : (k < ACC_CLEARED + 2u ? k : NATRAY_A_MW + (k - ACC_CLEARED - 2u));
     int4* a = reinterpret_cast<int4*>(d.acc + (size_t)f * d.capc);
     for (uint32_t i = v.wg * blockDim.x + threadIdx.x; i < n4; i += v.wgs * blockDim.x) a[i] = make_int4(0, 0, 0, 0);
   }
@@ -2064,6 +2067,161 @@ __global__ __launch_bounds__(256) void k_vmax(L l) {
   if (threadIdx.x == 0) d.st->max_speed = fmax_(fmax_(s[0], s[1]), fmax_(s[2], s[3]));
 }
 
+// ---- membrane coupling (membrane.wgsl:151-256, with grid_prep's RUBBER/COVERED cells) -----------
+//
+// The membrane's own step (its forces and substeps) is not part of this library: its vertices
+// come from the caller each step (natray_sim_set_membrane). These kernels are what the water
+// substep runs with them: claiming the blocks around rubber off the water (mark_rubber), the
+// rubber's pull and motion splatted to the grid (mem_splat, read by grid_prep), and the water
+// sampled at each vertex after the extension (mem_sample).
+
+constexpr float CONTACT = 0.05f;  // rubber this deep in water is on it (membrane.wgsl:38)
+
+__device__ __forceinline__ uint32_t mem_count(const Dev& d) {
+  const uint32_t n = d.U->mem[0];
+  return n < d.mem_cap ? n : d.mem_cap;
+}
+__device__ __forceinline__ bool on_water(const Dev& d, uint32_t i) {
+  return d.mem_mass[i] > 0.0f && d.mem_water[i].w >= CONTACT;
+}
+// The cell of a point, floor((p - origin) / h) per axis (own_cell, fluid.wgsl:127-129).
+__device__ __forceinline__ void point_cell(const Dev& d, float4 p, int& cx, int& cy, int& cz) {
+  const natray_params* U = d.U;
+  const float h = U->origin[3];
+  cx = (int)floorf((p.x - U->origin[0]) / h);
+  cy = (int)floorf((p.y - U->origin[1]) / h);
+  cz = (int)floorf((p.z - U->origin[2]) / h);
+}
+// The storage index of the cell at (cx, cy, cz), or NONE outside the domain or where no block
+// of this step holds it (cell_at, membrane.wgsl:156-162).
+__device__ __forceinline__ uint32_t cell_at(const Dev& d, int cx, int cy, int cz) {
+  const natray_params* U = d.U;
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)U->dims[0] || cy >= (int)U->dims[1] || cz >= (int)U->dims[2]) return NONE;
+  const uint32_t b = d.block_ptr[block_slot(U, cx >> 2, cy >> 2, cz >> 2)];
+  if (b >= n_blocks(d)) return NONE;
+  return b * 64u + (uint32_t)(cx & 3) * 16u + (uint32_t)(cy & 3) * 4u + (uint32_t)(cz & 3);
+}
+
+// mark_rubber (membrane.wgsl:164-180): the blocks of cells c-2..c+3 around rubber off the
+// water are claimed, so the air there is simulated too.
+template <class L>
+__global__ __launch_bounds__(256) void k_mark_rubber(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  if (d.st->dt_ns == 0u) return;
+  const uint32_t i = v.wg * blockDim.x + threadIdx.x;
+  if (i >= mem_count(d) || d.mem_push[i].y <= 0.0f || on_water(d, i)) return;
+  int cx, cy, cz;
+  point_cell(d, d.mem_x[i], cx, cy, cz);
+  const natray_params* U = d.U;
+  const int bmx = (int)U->bdims[0] - 1, bmy = (int)U->bdims[1] - 1, bmz = (int)U->bdims[2] - 1;
+  const int lx = max(0, min((cx - 2) >> 2, bmx)), ly = max(0, min((cy - 2) >> 2, bmy)), lz = max(0, min((cz - 2) >> 2, bmz));
+  const int hx = max(0, min((cx + 3) >> 2, bmx)), hy = max(0, min((cy + 3) >> 2, bmy)), hz = max(0, min((cz + 3) >> 2, bmz));
+  for (int x = lx; x <= hx; ++x)
+    for (int y = ly; y <= hy; ++y)
+      for (int z = lz; z <= hz; ++z) {
+        const uint32_t s = block_slot(U, x, y, z);
+        if (d.block_ptr[s] != PENDING) d.block_ptr[s] = PENDING;
+      }
+}
+
+// mem_splat (membrane.wgsl:182-216): every vertex with area adds its area (mm^2) and area
+// times velocity to its own cell (A_MW, A_MV); a vertex on the water also spreads its push
+// (A_MF) and area (A_MA) over the 27 cells around it with the particles' B-spline weights.
+template <class L>
+__global__ __launch_bounds__(256) void k_mem_splat(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  if (d.st->dt_ns == 0u) return;
+  const uint32_t i = v.wg * blockDim.x + threadIdx.x;
+  if (i >= mem_count(d) || d.mem_mass[i] == 0.0f) return;
+  const float2 pa = d.mem_push[i];
+  if (pa.y <= 0.0f) return;
+  const float4 p = d.mem_x[i];
+  int cx, cy, cz;
+  point_cell(d, p, cx, cy, cz);
+  const uint32_t own = cell_at(d, cx, cy, cz);
+  if (own != NONE) {
+    const float a = pa.y * 1e6f;
+    const float4 mv = d.mem_v[i];
+    atomicAdd(&accr(d, NATRAY_A_MW, own), to_fp(a, NATRAY_FP_A));
+    atomicAdd(&accr(d, NATRAY_A_MV + 0u, own), to_fp(a * mv.x, NATRAY_FP_A));
+    atomicAdd(&accr(d, NATRAY_A_MV + 1u, own), to_fp(a * mv.y, NATRAY_FP_A));
+    atomicAdd(&accr(d, NATRAY_A_MV + 2u, own), to_fp(a * mv.z, NATRAY_FP_A));
+  }
+  if (!on_water(d, i)) return;
+  const natray_params* U = d.U;
+  const float h = U->origin[3];
+  const float x[3] = {(p.x - U->origin[0]) / h - 0.5f, (p.y - U->origin[1]) / h - 0.5f, (p.z - U->origin[2]) / h - 0.5f};
+  int base[3];
+  float w[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float bf = floorf(x[k] - 0.5f);
+    base[k] = (int)bf;
+    axis_weights(x[k] - bf, w[k]);
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        const uint32_t cell = cell_at(d, base[0] + a, base[1] + b, base[2] + c);
+        if (cell == NONE) continue;
+        const float wt = w[0][a] * w[1][b] * w[2][c];
+        atomicAdd(&accr(d, NATRAY_A_MF, cell), to_fp(wt * pa.x, NATRAY_FP_F));
+        atomicAdd(&accr(d, NATRAY_A_MA, cell), to_fp(wt * pa.y * 1e6f, NATRAY_FP_A));
+      }
+}
+
+// sample_grid (membrane.wgsl:218-237): the B-spline sums around x (in cells, on the lattice)
+// of cell field fu weighted by transfer field fw, and of the weights.
+__device__ __forceinline__ float2 sample_grid(const Dev& d, const float x[3], uint32_t fu, uint32_t fw) {
+  int base[3];
+  float w[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float bf = floorf(x[k] - 0.5f);
+    base[k] = (int)bf;
+    axis_weights(x[k] - bf, w[k]);
+  }
+  float sx = 0.0f, sy = 0.0f;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        const uint32_t cell = cell_at(d, base[0] + a, base[1] + b, base[2] + c);
+        if (cell == NONE) continue;
+        const float wt = w[0][a] * w[1][b] * w[2][c] * from_fp(accr(d, fw, cell), NATRAY_FP_W);
+        sx = sx + wt * cfr(d, fu, cell);
+        sy = sy + wt;
+      }
+  return make_float2(sx, sy);
+}
+
+// mem_sample (membrane.wgsl:239-254): the velocity of the faces around each vertex, weighted
+// by the water they carry, and how full of water the cells around it are (a bulk cell holds 8
+// particles whose weights sum to 1 each).
+template <class L>
+__global__ __launch_bounds__(256) void k_mem_sample(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  if (d.st->dt_ns == 0u) return;
+  const uint32_t i = v.wg * blockDim.x + threadIdx.x;
+  if (i >= mem_count(d) || d.mem_mass[i] == 0.0f) return;
+  const natray_params* U = d.U;
+  const float h = U->origin[3];
+  const float4 p = d.mem_x[i];
+  const float x[3] = {(p.x - U->origin[0]) / h, (p.y - U->origin[1]) / h, (p.z - U->origin[2]) / h};
+  float vel[3];
+#pragma unroll
+  for (uint32_t dim = 0; dim < 3; ++dim) {
+    const float xs[3] = {dim == 0 ? x[0] : x[0] - 0.5f, dim == 1 ? x[1] : x[1] - 0.5f, dim == 2 ? x[2] : x[2] - 0.5f};
+    const float2 s = sample_grid(d, xs, NATRAY_F_U + dim, NATRAY_A_WT(dim));
+    vel[dim] = s.y > 1e-6f ? s.x / s.y : 0.0f;
+  }
+  const float xc[3] = {x[0] - 0.5f, x[1] - 0.5f, x[2] - 0.5f};
+  const float fill = sample_grid(d, xc, NATRAY_F_TYPE, NATRAY_A_WC).y / 8.0f;
+  d.mem_water[i] = make_float4(vel[0], vel[1], vel[2], fill);
+}
+
 // ---- launch wrappers -------------------------------------------------------------------------------
 
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -2086,9 +2244,10 @@ int launch_begin(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_begin, LK_ONE, 64, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_bin(const Target& t, void* stream) {
+int launch_bin(const Target& t, void* stream, bool rubber) {
   NATRAY_LAUNCH(k_clear, LK_CELLS, 256, t, S(stream));
   NATRAY_LAUNCH(k_mark, LK_PARTS, 256, t, S(stream));
+  if (rubber) NATRAY_LAUNCH(k_mark_rubber, LK_VERTS, 256, t, S(stream));
   NATRAY_LAUNCH(k_scan_reduce, LK_TILES, 256, t, S(stream));
   NATRAY_LAUNCH(k_scan_tiles, LK_ONE, 1024, t, S(stream));
   NATRAY_LAUNCH(k_scan_scatter, LK_TILES, 256, t, S(stream));
@@ -2158,6 +2317,15 @@ int launch_ws_samples(const Target& t, void* stream) {
 int launch_g2p(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_g2p, LK_BLOCKS, 256, t, S(stream));
   NATRAY_LAUNCH(k_vmax, LK_ONE, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+
+int launch_mem_splat(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_mem_splat, LK_VERTS, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+int launch_mem_sample(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_mem_sample, LK_VERTS, 256, t, S(stream));
   return (int)hipGetLastError();
 }
 

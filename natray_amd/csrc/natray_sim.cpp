@@ -218,8 +218,11 @@ int enqueue_phase(natray_sim* s, int phase) {
   }
   switch (phase) {
     case NATRAY_PHASE_BEGIN: LAUNCHCHK(natray::launch_begin(s->tgt, st)); break;
-    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(s->tgt, st)); break;
-    case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(s->tgt, st)); break;
+    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(s->tgt, st, s->D.mem_cap > 0)); break;
+    case NATRAY_PHASE_P2G:
+      LAUNCHCHK(natray::launch_p2g(s->tgt, st));
+      if (s->D.mem_cap) LAUNCHCHK(natray::launch_mem_splat(s->tgt, st));
+      break;
     case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(s->tgt, st)); break;
     case NATRAY_PHASE_SOLVE: {
       // Tags are 32-bit: clear the tagged words well before a tag could repeat.
@@ -237,7 +240,10 @@ int enqueue_phase(natray_sim* s, int phase) {
       break;
     }
     case NATRAY_PHASE_PROJECT: LAUNCHCHK(natray::launch_project(s->tgt, st)); break;
-    case NATRAY_PHASE_EXTEND: LAUNCHCHK(natray::launch_extend(s->tgt, st)); break;
+    case NATRAY_PHASE_EXTEND:
+      LAUNCHCHK(natray::launch_extend(s->tgt, st));
+      if (s->D.mem_cap) LAUNCHCHK(natray::launch_mem_sample(s->tgt, st));
+      break;
     case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(s->tgt, st)); break;
     default: return fail(NATRAY_E_ARG, "unknown phase " + std::to_string(phase));
   }
@@ -449,6 +455,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
     t.grid[natray::LK_BLOCKS] = (uint32_t)s->grid_blocks;
     t.grid[natray::LK_WAVES] = (uint32_t)s->grid_waves;
     t.grid[natray::LK_SOLVE] = (uint32_t)solve_grid_for(s);
+    t.grid[natray::LK_VERTS] = 1;
   }
 
   HIPTRY(hipMemsetAsync(D.block_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
@@ -761,6 +768,54 @@ int natray_sim_ws_samples(natray_sim* s, float* inputs, float* targets, float* m
   return NATRAY_OK;
 }
 
+int natray_sim_set_membrane(natray_sim* s, uint32_t n, const float* x, const float* v, const float* mass,
+                            const float* push) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  if (n && (!x || !v || !mass || !push)) return fail(NATRAY_E_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  natray::Dev& D = s->D;
+  if (n > D.mem_cap) {
+    float4 *mx = nullptr, *mv = nullptr, *mw = nullptr;
+    float* mm = nullptr;
+    float2* mp = nullptr;
+    int e;
+    if ((e = dalloc(s, &mx, n)) || (e = dalloc(s, &mv, n)) || (e = dalloc(s, &mm, n)) || (e = dalloc(s, &mp, n)) ||
+        (e = dalloc(s, &mw, n)))
+      return e;
+    HIPCHK(hipMemsetAsync(mw, 0, (size_t)n * sizeof(float4), s->stream));
+    D.mem_x = mx;
+    D.mem_v = mv;
+    D.mem_mass = mm;
+    D.mem_push = mp;
+    D.mem_water = mw;
+    D.mem_cap = n;
+    s->tgt.grid[natray::LK_VERTS] = (n + 255u) / 256u;
+    HIPCHK(hipMemcpyAsync(const_cast<natray::Dev*>(D.self), &D, sizeof(natray::Dev), hipMemcpyHostToDevice, s->stream));
+    if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
+    if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+    s->graph_pre = s->graph_post = nullptr;
+  }
+  if (n) {
+    HIPCHK(hipMemcpyAsync(const_cast<float4*>(D.mem_x), x, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(const_cast<float4*>(D.mem_v), v, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(const_cast<float*>(D.mem_mass), mass, (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(const_cast<float2*>(D.mem_push), push, (size_t)n * 8, hipMemcpyHostToDevice, s->stream));
+  }
+  s->U.mem[0] = n;
+  int e = upload_params(s);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_read_membrane(natray_sim* s, float* water) {
+  if (!s || !water) return fail(NATRAY_E_ARG, "null argument");
+  const uint32_t n = std::min(s->U.mem[0], s->D.mem_cap);
+  if (n) HIPCHK(hipMemcpyAsync(water, s->D.mem_water, (size_t)n * 16, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
 int natray_sim_ws_stats(natray_sim* s, double out[8]) {
   if (!s || !out) return fail(NATRAY_E_ARG, "null argument");
   for (int k = 0; k < 8; ++k) out[k] = 0.0;
@@ -881,6 +936,7 @@ int batch_layout(natray_batch* b) {
   }
   const uint32_t cus = (uint32_t)b->num_cus;
   b->counts[natray::LK_ONE].assign(n, 1u);
+  b->counts[natray::LK_VERTS].assign(n, 1u);
   auto& parts = b->counts[natray::LK_PARTS];
   auto& tiles = b->counts[natray::LK_TILES];
   parts.resize(n);
@@ -1051,6 +1107,7 @@ int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out)
     if (sims[k]->mixed != sims[0]->mixed) return fail(NATRAY_E_ARG, "the scenes of a batch must share the CG precision");
     if (sims[k]->device != sims[0]->device) return fail(NATRAY_E_ARG, "the scenes of a batch must be on one device");
     if (sims[k]->ws_mode) return fail(NATRAY_E_ARG, "the warm start runs on single scenes, not in a batch");
+    if (sims[k]->D.mem_cap) return fail(NATRAY_E_ARG, "the membrane coupling runs on single scenes, not in a batch");
     for (uint32_t j = 0; j < k; ++j)
       if (sims[j] == sims[k]) return fail(NATRAY_E_ARG, "a scene appears twice in the batch");
   }

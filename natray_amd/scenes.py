@@ -355,3 +355,36 @@ def batch_impacts(count: int = 64, seed: int = 4, n_lo: int = 16384, n_hi: int =
         s.meta.update({"scene": k, "speed": speed, "direction": direction, "capacity": cap})
         out.append(s)
     return out
+
+
+def membrane_shell(s: Scene, n_verts: int = 2000, gap_cells: float = 0.6, push: float = 2e-3, seed: int = 7,
+                   extra_off_water: int = 8):
+    """A stand-in for the balloon's rubber around a water sphere: n_verts vertices on a
+    Fibonacci sphere `gap_cells` cells outside the water's radius, each with mass 1e-4 kg, its
+    share of the sphere's area and a push of `push` newtons, small seeded velocities, plus
+    `extra_off_water` vertices on a ring 0.06 m further out, away from the water. Returns (x [n][4], v [n][4],
+    mass [n], push [n][2])."""
+    dx = float(s.meta["dx"])
+    r = float(s.meta["r_eff"]) + gap_cells * dx
+    k = np.arange(n_verts, dtype=np.float64) + 0.5
+    z = 1.0 - 2.0 * k / n_verts
+    phi = math.pi * (3.0 - math.sqrt(5.0)) * k
+    rho = np.sqrt(1.0 - z * z)
+    pts = np.stack([rho * np.cos(phi), rho * np.sin(phi), z], axis=1)
+    x = np.zeros((n_verts + extra_off_water, 4), np.float32)
+    x[:n_verts, :3] = 0.5 + r * pts
+    theta = 2.0 * math.pi * np.arange(extra_off_water) / max(extra_off_water, 1)
+    ring = r + 0.06
+    x[n_verts:, 0] = 0.5 + ring * np.cos(theta)
+    x[n_verts:, 1] = 0.5 + ring * np.sin(theta)
+    x[n_verts:, 2] = 0.5
+    x[:, 3] = 1.0
+    u = uniform01(seed, 3 * (n_verts + extra_off_water)).reshape(-1, 3)
+    v = np.zeros_like(x)
+    v[:, :3] = (0.05 * (2.0 * u - 1.0)).astype(np.float32)
+    mass = np.full(n_verts + extra_off_water, 1e-4, np.float32)
+    area = 4.0 * math.pi * r * r / n_verts
+    p = np.zeros((n_verts + extra_off_water, 2), np.float32)
+    p[:, 0] = push
+    p[:, 1] = area
+    return x, v, mass, p

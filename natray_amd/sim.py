@@ -157,6 +157,22 @@ class Sim:
         keys = ["steps", "cg_iters", "cg_cell_iters", "blocks", "water_cells", "particles", "blocks_cleared"]
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
+    def set_membrane(self, x: np.ndarray, v: np.ndarray, mass: np.ndarray, push: np.ndarray) -> None:
+        """The membrane's vertices for the coming steps (natray_sim_set_membrane): positions and
+        velocities [n][4], masses [n], push (N) and area (m^2) [n][2]."""
+        x, v = np.ascontiguousarray(x, np.float32), np.ascontiguousarray(v, np.float32)
+        mass, push = np.ascontiguousarray(mass, np.float32), np.ascontiguousarray(push, np.float32)
+        n = int(mass.size)
+        assert x.shape == (n, 4) and v.shape == (n, 4) and push.shape == (n, 2)
+        abi.check(abi.lib().natray_sim_set_membrane(self._h, n, abi.ptr(x), abi.ptr(v), abi.ptr(mass), abi.ptr(push)))
+        self._mem_n = n
+
+    def membrane_water(self) -> np.ndarray:
+        """The water sampled at each vertex by the last step: [n][4] velocity and fill."""
+        out = np.zeros((getattr(self, "_mem_n", 0), 4), np.float32)
+        abi.check(abi.lib().natray_sim_read_membrane(self._h, abi.ptr(out)))
+        return out
+
     def set_warm_start(self, net=None, mode: int = 2) -> None:
         """The learned warm start of the solve (natray_sim_set_warm_start): mode 0 off, 1 inputs
         only (for ws_samples), 2 the network's guess with fallback to the carried pressure."""

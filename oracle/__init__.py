@@ -56,6 +56,8 @@ def lib() -> C.CDLL:
                 "set_threads": (None, [C.c_int]),
                 "set_guess": (None, [P, P]),
                 "residual": (None, [P, P, P]),
+                "set_membrane": (C.c_int, [P, C.c_uint32, P, P, P, P]),
+                "read_membrane": (None, [P, P]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -191,6 +193,19 @@ class Oracle:
             assert q.size == int(self.params.bdims[3]) * 64
         self._f("residual")(self._h, abi.ptr(q), abi.ptr(r))
         return r[: 64 * n]
+
+    def set_membrane(self, x: np.ndarray, v: np.ndarray, mass: np.ndarray, push: np.ndarray) -> None:
+        """The membrane's vertices for the coming steps (positions and velocities [n][4], masses
+        [n], push and area [n][2])."""
+        x, v = np.ascontiguousarray(x, np.float32), np.ascontiguousarray(v, np.float32)
+        mass, push = np.ascontiguousarray(mass, np.float32), np.ascontiguousarray(push, np.float32)
+        self._mem_n = int(mass.size)
+        assert self._f("set_membrane")(self._h, self._mem_n, abi.ptr(x), abi.ptr(v), abi.ptr(mass), abi.ptr(push)) == 0
+
+    def membrane_water(self) -> np.ndarray:
+        out = np.zeros((getattr(self, "_mem_n", 0), 4), np.float32)
+        self._f("read_membrane")(self._h, abi.ptr(out))
+        return out
 
     def solver_scalars(self) -> dict:
         out = np.zeros(8, np.float64)

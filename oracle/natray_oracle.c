@@ -96,6 +96,10 @@ struct osim {
   uint64_t cg_iters;
   REAL* guess;           /* cap x 64: a starting pressure for the next solve (set_guess) */
   int use_guess;
+  /* The membrane's vertices (set_membrane): positions and velocities (4 per vertex), masses,
+   * push and area (2 per vertex); mem_water (4 per vertex) is what mem_sample leaves. */
+  uint32_t mem_cap;
+  REAL *mem_x, *mem_v, *mem_mass, *mem_push, *mem_water;
 };
 
 /* ---- Params / State accessors (sim_defs.wgsl:159-231) -------------------------------- */
@@ -298,6 +302,42 @@ static inline int in_domain(const osim* s, i3 c) {
 static inline int floor4(int v) { return v >> 2; } /* arithmetic shift, as (c - 1) >> 2 in WGSL */
 static inline int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+/* ---- membrane coupling (membrane.wgsl:151-256) ----------------------------------------- */
+
+#define CONTACT ((REAL)0.05) /* rubber this deep in water is on it (membrane.wgsl:38) */
+
+static inline uint32_t mem_count(const osim* s) { return umin(s->U.mem[0], s->mem_cap); }
+static inline int on_water(const osim* s, uint32_t i) {
+  return s->mem_mass[i] > (REAL)0 && s->mem_water[(size_t)i * 4 + 3] >= CONTACT;
+}
+/* cell_at (membrane.wgsl:156-162): the cell at c, NONE outside the domain or where no block of
+ * this step holds it. */
+static inline uint32_t cell_at(const osim* s, i3 c) {
+  if (!in_domain(s, c)) return NONE;
+  i3 b = {c.x >> 2, c.y >> 2, c.z >> 2};
+  uint32_t blk = s->block_ptr[block_slot(s, b)];
+  if (blk >= n_blocks_(s)) return NONE;
+  return blk * 64u + (uint32_t)(c.x & 3) * 16u + (uint32_t)(c.y & 3) * 4u + (uint32_t)(c.z & 3);
+}
+
+/* mark_rubber (membrane.wgsl:164-180): claim the blocks of cells c-2..c+3 around rubber off
+ * the water. */
+static void o_mark_rubber(osim* s) {
+  int bmx = (int)s->U.bdims[0] - 1, bmy = (int)s->U.bdims[1] - 1, bmz = (int)s->U.bdims[2] - 1;
+  for (uint32_t i = 0; i < mem_count(s); ++i) {
+    if (s->mem_push[(size_t)i * 2 + 1] <= (REAL)0 || on_water(s, i)) continue;
+    i3 c = own_cell(s, &s->mem_x[(size_t)i * 4]);
+    int lx = iclamp(floor4(c.x - 2), 0, bmx), ly = iclamp(floor4(c.y - 2), 0, bmy), lz = iclamp(floor4(c.z - 2), 0, bmz);
+    int hx = iclamp(floor4(c.x + 3), 0, bmx), hy = iclamp(floor4(c.y + 3), 0, bmy), hz = iclamp(floor4(c.z + 3), 0, bmz);
+    for (int x = lx; x <= hx; ++x)
+      for (int y = ly; y <= hy; ++y)
+        for (int z = lz; z <= hz; ++z) {
+          i3 b = {x, y, z};
+          s->block_ptr[block_slot(s, b)] = PENDING;
+        }
+  }
+}
+
 static void o_bin(osim* s) {
   if (s->st.dt_ns == 0u) return;
   const uint32_t cap = cap_cells(s);
@@ -329,6 +369,7 @@ static void o_bin(osim* s) {
     i3 ob = {c.x >> 2, c.y >> 2, c.z >> 2};
     s->rank[i] = s->slot_count[block_slot(s, ob)]++;
   }
+  o_mark_rubber(s);
   /* Block indices in slot order; overflowing claims are dropped (claim, fluid.wgsl:117-122). */
   uint32_t n = 0, bcap = s->U.bdims[3];
   for (uint32_t sl = 0; sl < s->S; ++sl) {
@@ -465,6 +506,95 @@ static void o_p2g(osim* s) {
 }
 
 /* ---- grid_prep (fluid.wgsl:319-356) ----------------------------------------------------- */
+
+/* mem_splat (membrane.wgsl:182-216): a vertex with area adds its area (mm^2) and area times
+ * velocity to its own cell; a vertex on the water also spreads its push and area over the 27
+ * cells around it with the particles' weights. */
+static void o_mem_splat(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  REAL h = hh_(s);
+  for (uint32_t i = 0; i < mem_count(s); ++i) {
+    if (s->mem_mass[i] == (REAL)0) continue;
+    const REAL* pa = &s->mem_push[(size_t)i * 2];
+    if (pa[1] <= (REAL)0) continue;
+    const REAL* p = &s->mem_x[(size_t)i * 4];
+    uint32_t own = cell_at(s, own_cell(s, p));
+    if (own != NONE) {
+      REAL a = pa[1] * (REAL)1e6;
+      *accp(s, NATRAY_A_MW, own) += to_fp(a, (REAL)NATRAY_FP_A);
+      for (uint32_t d = 0; d < 3; ++d) *accp(s, NATRAY_A_MV + d, own) += to_fp(a * s->mem_v[(size_t)i * 4 + d], (REAL)NATRAY_FP_A);
+    }
+    if (!on_water(s, i)) continue;
+    REAL x[3], w[3][3];
+    int base[3];
+    for (int k = 0; k < 3; ++k) {
+      x[k] = (p[k] - (REAL)s->U.origin[k]) / h - (REAL)0.5;
+      REAL bf = rfloor(x[k] - (REAL)0.5);
+      base[k] = (int)bf;
+      axis_weights(x[k] - bf, w[k]);
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          i3 cc = {base[0] + a, base[1] + b, base[2] + c};
+          uint32_t cell = cell_at(s, cc);
+          if (cell == NONE) continue;
+          REAL wt = w[0][a] * w[1][b] * w[2][c];
+          *accp(s, NATRAY_A_MF, cell) += to_fp(wt * pa[0], (REAL)NATRAY_FP_F);
+          *accp(s, NATRAY_A_MA, cell) += to_fp(wt * pa[1] * (REAL)1e6, (REAL)NATRAY_FP_A);
+        }
+  }
+}
+
+/* sample_grid (membrane.wgsl:218-237) */
+static void sample_grid(const osim* s, const REAL x[3], uint32_t fu, uint32_t fw, REAL out[2]) {
+  REAL w[3][3];
+  int base[3];
+  for (int k = 0; k < 3; ++k) {
+    REAL bf = rfloor(x[k] - (REAL)0.5);
+    base[k] = (int)bf;
+    axis_weights(x[k] - bf, w[k]);
+  }
+  REAL sx = 0, sy = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        i3 cc = {base[0] + a, base[1] + b, base[2] + c};
+        uint32_t cell = cell_at(s, cc);
+        if (cell == NONE) continue;
+        REAL wt = w[0][a] * w[1][b] * w[2][c] * from_fp(accv(s, fw, cell), (REAL)NATRAY_FP_W);
+        sx = sx + wt * cfv(s, fu, cell);
+        sy = sy + wt;
+      }
+  out[0] = sx;
+  out[1] = sy;
+}
+
+/* mem_sample (membrane.wgsl:239-254): the face velocity around each vertex, weighted by the
+ * water each face carries, and how full of water the cells around it are. */
+static void o_mem_sample(osim* s) {
+  if (s->st.dt_ns == 0u) return;
+  REAL h = hh_(s);
+  for (uint32_t i = 0; i < mem_count(s); ++i) {
+    if (s->mem_mass[i] == (REAL)0) continue;
+    const REAL* p = &s->mem_x[(size_t)i * 4];
+    REAL x[3], v[3], r[2];
+    for (int k = 0; k < 3; ++k) x[k] = (p[k] - (REAL)s->U.origin[k]) / h;
+    for (uint32_t d = 0; d < 3; ++d) {
+      REAL xs[3];
+      for (uint32_t k = 0; k < 3; ++k) xs[k] = k == d ? x[k] : x[k] - (REAL)0.5;
+      sample_grid(s, xs, NATRAY_F_U + d, NATRAY_A_WT(d), r);
+      v[d] = r[1] > (REAL)1e-6 ? r[0] / r[1] : (REAL)0;
+    }
+    REAL xc[3] = {x[0] - (REAL)0.5, x[1] - (REAL)0.5, x[2] - (REAL)0.5};
+    sample_grid(s, xc, NATRAY_F_TYPE, NATRAY_A_WC, r);
+    REAL* mw = &s->mem_water[(size_t)i * 4];
+    mw[0] = v[0];
+    mw[1] = v[1];
+    mw[2] = v[2];
+    mw[3] = r[1] / (REAL)8;
+  }
+}
 
 static void o_grid_prep(osim* s) {
   if (s->st.dt_ns == 0u) return;
@@ -1119,7 +1249,10 @@ int F(phase)(osim* s, int phase) {
   switch (phase) {
     case NATRAY_PHASE_BEGIN: o_begin_step(s); break;
     case NATRAY_PHASE_BIN: o_bin(s); break;
-    case NATRAY_PHASE_P2G: o_p2g(s); break;
+    case NATRAY_PHASE_P2G:
+      o_p2g(s);
+      o_mem_splat(s);
+      break;
     case NATRAY_PHASE_GRID: o_grid_prep(s); break;
     case NATRAY_PHASE_SOLVE: o_solve(s); break;
     case NATRAY_PHASE_PROJECT: o_project(s); break;
@@ -1128,6 +1261,7 @@ int F(phase)(osim* s, int phase) {
         o_extend_1(s);
         o_extend_2(s);
       }
+      o_mem_sample(s);
       break;
     case NATRAY_PHASE_G2P: o_g2p(s); break;
     default: return NATRAY_E_ARG;
@@ -1212,6 +1346,11 @@ void F(destroy)(osim* s) {
   free(s->bvec);
   free(s->part);
   free(s->guess);
+  free(s->mem_x);
+  free(s->mem_v);
+  free(s->mem_mass);
+  free(s->mem_push);
+  free(s->mem_water);
   free(s);
 }
 
@@ -1316,6 +1455,47 @@ void F(residual)(osim* s, const float* p, float* r) {
     r[cell] = cfv(s, NATRAY_F_TYPE, cell) == WATER ? (float)solve_row(s, cell, q, &dg, &b) : 0.0f;
   }
   if (q != P) free(q);
+}
+
+int F(set_membrane)(osim* s, uint32_t n, const float* x, const float* v, const float* mass, const float* push) {
+  if (n > s->mem_cap) {
+    REAL* nx = (REAL*)calloc((size_t)n * 4, sizeof(REAL));
+    REAL* nv = (REAL*)calloc((size_t)n * 4, sizeof(REAL));
+    REAL* nm = (REAL*)calloc((size_t)n, sizeof(REAL));
+    REAL* np = (REAL*)calloc((size_t)n * 2, sizeof(REAL));
+    REAL* nw = (REAL*)calloc((size_t)n * 4, sizeof(REAL));
+    if (!nx || !nv || !nm || !np || !nw) {
+      free(nx);
+      free(nv);
+      free(nm);
+      free(np);
+      free(nw);
+      return NATRAY_E_NOMEM;
+    }
+    free(s->mem_x);
+    free(s->mem_v);
+    free(s->mem_mass);
+    free(s->mem_push);
+    free(s->mem_water);
+    s->mem_x = nx;
+    s->mem_v = nv;
+    s->mem_mass = nm;
+    s->mem_push = np;
+    s->mem_water = nw;
+    s->mem_cap = n;
+  }
+  for (size_t k = 0; k < (size_t)n * 4; ++k) {
+    s->mem_x[k] = (REAL)x[k];
+    s->mem_v[k] = (REAL)v[k];
+  }
+  for (size_t k = 0; k < n; ++k) s->mem_mass[k] = (REAL)mass[k];
+  for (size_t k = 0; k < (size_t)n * 2; ++k) s->mem_push[k] = (REAL)push[k];
+  s->U.mem[0] = n;
+  return NATRAY_OK;
+}
+
+void F(read_membrane)(const osim* s, float* water) {
+  for (size_t k = 0; k < (size_t)mem_count(s) * 4; ++k) water[k] = (float)s->mem_water[k];
 }
 
 void F(set_threads)(int n) {

@@ -41,6 +41,9 @@ typedef struct osim osim;
   void P##precondition(const float* r, const float* diag, const int32_t* water, float* z, uint32_t mode); \
   void P##set_guess(osim* s, const float* guess);                                                   \
   void P##residual(osim* s, const float* p, float* r);                                              \
+  int P##set_membrane(osim* s, uint32_t n, const float* x, const float* v, const float* mass,        \
+                      const float* push);                                                           \
+  void P##read_membrane(const osim* s, float* water);                                               \
   void P##set_threads(int n);
 
 ORACLE_DECL(o32_)
