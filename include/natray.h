@@ -348,6 +348,12 @@ int natray_sim_set_membrane(natray_sim* sim, uint32_t n, const float* x, const f
  * fill (1 inside the water, 0 away from it), the reference's mem_water. */
 int natray_sim_read_membrane(natray_sim* sim, float* water);
 
+/* The water's part of a keyframe (pack_water, shaders/pack.wgsl:33-39 with pack_pos,
+ * shaders/keys.wgsl:21-24): words[2 N], two per particle in particle order, 21 bits per axis
+ * over an 8 m box from (-4, -0.5, -4), bit 31 of the second word set for a live particle. The
+ * keyframe places them after its 4 header words (keys.wgsl KEY_HEAD). */
+int natray_sim_pack_water(natray_sim* sim, uint32_t* words);
+
 /* ---- batches of independent scenes ----------------------------------------------------------
  * No counterpart in the reference, which steps one Sim per scene (src/sim.rs). A batch steps
  * many scenes, each a natray_sim of its own (particles, grid, clock, dt and CG of its own), with

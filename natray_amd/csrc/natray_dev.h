@@ -145,6 +145,8 @@ int launch_g2p(const Target& t, void* stream);
 // water sampled at its vertices after the velocity extension.
 int launch_mem_splat(const Target& t, void* stream);
 int launch_mem_sample(const Target& t, void* stream);
+// The keyframe words of the water: two per particle into `out` (device memory, 8 N bytes).
+int launch_pack_water(const Target& t, void* stream, void* out);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also
 // into the network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has
 // run on them (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a

@@ -2222,6 +2222,34 @@ __global__ __launch_bounds__(256) void k_mem_sample(L l) {
   d.mem_water[i] = make_float4(vel[0], vel[1], vel[2], fill);
 }
 
+// ---- keyframe pack of the water (keys.wgsl:9-37, pack.wgsl:33-39) ------------------------------
+
+// pack_pos (keys.wgsl:21-24): 21 bits per axis over an 8 m box from (-4, -0.5, -4), rounded to
+// the nearest step; x and the low 11 bits of y in the first word, the high 10 bits of y, z and
+// a flag (bit 31) in the second.
+__device__ __forceinline__ uint2 pack_pos(float px, float py, float pz, bool flag) {
+  const float lo[3] = {-4.0f, -0.5f, -4.0f}, p[3] = {px, py, pz};
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float t = fmin_(fmax_((p[k] - lo[k]) / 8.0f, 0.0f), 1.0f);
+    q[k] = (uint32_t)(t * 2097151.0f + 0.5f);
+  }
+  return make_uint2(q[0] | (q[1] << 21), (q[1] >> 11) | (q[2] << 10) | (flag ? 1u << 31 : 0u));
+}
+
+// pack_water (pack.wgsl:33-39): two words per particle, the flag for a live particle.
+template <class L>
+__global__ __launch_bounds__(256) void k_pack_water(L l, uint2* __restrict__ out) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  const uint32_t i = v.wg * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  const float4* p = reinterpret_cast<const float4*>(d.P) + (size_t)i * 5;
+  const float4 pos = p[0], vel = p[1];
+  out[i] = pack_pos(pos.x, pos.y, pos.z, vel.w > 0.5f);
+}
+
 // ---- launch wrappers -------------------------------------------------------------------------------
 
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -2326,6 +2354,11 @@ int launch_mem_splat(const Target& t, void* stream) {
 }
 int launch_mem_sample(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_mem_sample, LK_VERTS, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+
+int launch_pack_water(const Target& t, void* stream, void* out) {
+  NATRAY_LAUNCH(k_pack_water, LK_PARTS, 256, t, S(stream), static_cast<uint2*>(out));
   return (int)hipGetLastError();
 }
 

@@ -54,6 +54,7 @@ struct natray_sim {
   int num_cus = 0;
   int grid_cells = 0, grid_blocks = 0, grid_waves = 0, solve_grid = 0;
   natray::Target tgt;  // this scene as the kernels' launch target
+  uint32_t* packed = nullptr;  // 2 N words: the keyframe words of the water (natray_sim_pack_water)
   int mixed = 0;
   // Upper bound of the reduction tags the solves enqueued since the tagged words were last
   // cleared (see clear_tags).
@@ -804,6 +805,19 @@ int natray_sim_set_membrane(natray_sim* s, uint32_t n, const float* x, const flo
   s->U.mem[0] = n;
   int e = upload_params(s);
   if (e) return e;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_pack_water(natray_sim* s, uint32_t* words) {
+  if (!s || !words) return fail(NATRAY_E_ARG, "null argument");
+  if (!s->D.N) return NATRAY_OK;
+  if (!s->packed) {
+    int e = dalloc(s, &s->packed, (size_t)s->D.N * 2);
+    if (e) return e;
+  }
+  LAUNCHCHK(natray::launch_pack_water(s->tgt, s->stream, s->packed));
+  HIPCHK(hipMemcpyAsync(words, s->packed, (size_t)s->D.N * 8, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return NATRAY_OK;
 }

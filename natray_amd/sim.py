@@ -173,6 +173,12 @@ class Sim:
         abi.check(abi.lib().natray_sim_read_membrane(self._h, abi.ptr(out)))
         return out
 
+    def pack_water(self) -> np.ndarray:
+        """The water's keyframe words (natray_sim_pack_water): [N][2] uint32."""
+        out = np.zeros((self.n, 2), np.uint32)
+        abi.check(abi.lib().natray_sim_pack_water(self._h, abi.ptr(out)))
+        return out
+
     def set_warm_start(self, net=None, mode: int = 2) -> None:
         """The learned warm start of the solve (natray_sim_set_warm_start): mode 0 off, 1 inputs
         only (for ws_samples), 2 the network's guess with fallback to the carried pressure."""

@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
                 "residual": (None, [P, P, P]),
                 "set_membrane": (C.c_int, [P, C.c_uint32, P, P, P, P]),
                 "read_membrane": (None, [P, P]),
+                "pack_water": (None, [P, P]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -201,6 +202,11 @@ class Oracle:
         mass, push = np.ascontiguousarray(mass, np.float32), np.ascontiguousarray(push, np.float32)
         self._mem_n = int(mass.size)
         assert self._f("set_membrane")(self._h, self._mem_n, abi.ptr(x), abi.ptr(v), abi.ptr(mass), abi.ptr(push)) == 0
+
+    def pack_water(self) -> np.ndarray:
+        out = np.zeros((self.n, 2), np.uint32)
+        self._f("pack_water")(self._h, abi.ptr(out))
+        return out
 
     def membrane_water(self) -> np.ndarray:
         out = np.zeros((getattr(self, "_mem_n", 0), 4), np.float32)

@@ -1457,6 +1457,22 @@ void F(residual)(osim* s, const float* p, float* r) {
   if (q != P) free(q);
 }
 
+/* pack_pos (keys.wgsl:21-24) and pack_water (pack.wgsl:33-39): 21 bits per axis over the 8 m
+ * box from (-4, -0.5, -4), the live flag in bit 31 of the second word. */
+void F(pack_water)(const osim* s, uint32_t* out) {
+  const REAL lo[3] = {(REAL)-4, (REAL)-0.5, (REAL)-4};
+  for (uint32_t i = 0; i < s->N; ++i) {
+    const REAL* p = &s->P[(size_t)i * 20];
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+      REAL t = rmin(rmax((p[k] - lo[k]) / (REAL)8, (REAL)0), (REAL)1);
+      q[k] = (uint32_t)(t * (REAL)2097151 + (REAL)0.5);
+    }
+    out[2 * i] = q[0] | (q[1] << 21);
+    out[2 * i + 1] = (q[1] >> 11) | (q[2] << 10) | (p[7] > (REAL)0.5 ? 1u << 31 : 0u);
+  }
+}
+
 int F(set_membrane)(osim* s, uint32_t n, const float* x, const float* v, const float* mass, const float* push) {
   if (n > s->mem_cap) {
     REAL* nx = (REAL*)calloc((size_t)n * 4, sizeof(REAL));

@@ -44,6 +44,7 @@ typedef struct osim osim;
   int P##set_membrane(osim* s, uint32_t n, const float* x, const float* v, const float* mass,        \
                       const float* push);                                                           \
   void P##read_membrane(const osim* s, float* water);                                               \
+  void P##pack_water(const osim* s, uint32_t* out);                                                 \
   void P##set_threads(int n);
 
 ORACLE_DECL(o32_)
