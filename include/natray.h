@@ -153,7 +153,11 @@ typedef struct natray_particle {
 #define NATRAY_SCENE_WALLS 4u    /* the outermost cell layer of the domain is SOLID       */
 #define NATRAY_SCENE_CAPSULE 8u  /* a capsule moving at constant velocity is SOLID        */
 #define NATRAY_SCENE_MASK 16u    /* non-water cells inside a sphere are COVERED           */
-#define NATRAY_SCENE_REFERENCE (NATRAY_SCENE_GROUND | NATRAY_SCENE_KNOT)
+#define NATRAY_SCENE_AIR 32u     /* the air boundary: gas carried between steps, cells that
+                                    stop being water filled with gas or vapour, GAS and RUBBER
+                                    pressures from it (air.wgsl:85-148,271-277); without it
+                                    every GAS cell is at gauge 0 */
+#define NATRAY_SCENE_REFERENCE (NATRAY_SCENE_GROUND | NATRAY_SCENE_KNOT | NATRAY_SCENE_AIR)
 
 #define NATRAY_PRECOND_BLOCK_SGS 0u /* fluid.wgsl:384-410 */
 #define NATRAY_PRECOND_JACOBI 1u
@@ -347,6 +351,13 @@ int natray_sim_set_membrane(natray_sim* sim, uint32_t n, const float* x, const f
 /* The water sampled at each vertex by the last step: water[n][4] = face velocity (m/s) and
  * fill (1 inside the water, 0 away from it), the reference's mem_water. */
 int natray_sim_read_membrane(natray_sim* sim, float* water);
+
+/* With NATRAY_SCENE_AIR: each cell's gas after the last phase that wrote it, air[5][64 n] for
+ * the n blocks in use (density kg/m^3, momentum x, y, z, total energy per volume J/m^3; 0 in
+ * water and solid cells). The air's own substeps (HLL, air.wgsl) are not part of this library:
+ * the gas moves only by being carried between blocks and filled into cells that stop being
+ * water. */
+int natray_sim_read_air(natray_sim* sim, float* air);
 
 /* The water's part of a keyframe (pack_water, shaders/pack.wgsl:33-39 with pack_pos,
  * shaders/keys.wgsl:21-24): words[2 N], two per particle in particle order, 21 bits per axis

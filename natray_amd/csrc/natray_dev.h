@@ -80,6 +80,16 @@ struct Dev {
   // The rubber membrane's vertices as its own step left them (natray_sim_set_membrane): the
   // coupling kernels read positions, velocities, masses and each vertex's push on the water
   // (N) and area (m^2); mem_sample writes mem_water. mem_cap is 0 without a membrane.
+  // The air boundary (NATRAY_SCENE_AIR): each cell's gas, field-major [5][capc] (density,
+  // momentum x, y, z, total energy per volume), as the last step left it (air_x) and as
+  // carried into this step's blocks (air_tmp); the last step's block list and, per domain slot,
+  // its index in that list (stale entries are told apart by the list); blocks whose air was
+  // active (set by the air's own substeps, which are not part of this library).
+  float* air_x;
+  float* air_tmp;
+  uint32_t* prev_ptr;
+  uint32_t* block_list_prev;
+  uint32_t* block_active;
   const float4* mem_x;
   const float4* mem_v;
   const float* mem_mass;
@@ -133,7 +143,7 @@ struct Target {
 namespace natray {
 int launch_keys(const Target& t, void* stream);
 int launch_begin(const Target& t, void* stream);
-int launch_bin(const Target& t, void* stream, bool rubber = false);
+int launch_bin(const Target& t, void* stream, bool rubber = false, bool air = false);
 int launch_p2g(const Target& t, void* stream);
 int launch_grid_prep(const Target& t, void* stream);
 int launch_solve(const Target& t, void* stream, int mixed);
@@ -145,6 +155,10 @@ int launch_g2p(const Target& t, void* stream);
 // water sampled at its vertices after the velocity extension.
 int launch_mem_splat(const Target& t, void* stream);
 int launch_mem_sample(const Target& t, void* stream);
+// The air boundary (one scene): the gas carried into this step's blocks after P2G, and the
+// cells filled and the GAS and RUBBER pressures set after grid_prep.
+int launch_air_carry(const Target& t, void* stream);
+int launch_air_fill(const Target& t, void* stream);
 // The keyframe words of the water: two per particle into `out` (device memory, 8 N bytes).
 int launch_pack_water(const Target& t, void* stream, void* out);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also

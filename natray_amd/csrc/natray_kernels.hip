@@ -287,6 +287,14 @@ __global__ void k_clear(L l) {
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   uint32_t n4 = d.st->n_blocks_prev * 16u;  // int4 per field
+  // With the air boundary, the last step's block list and each slot's place in it (read by
+  // air_carry and keep_blocks; this step's list is built after this kernel).
+  if (d.air_x)
+    for (uint32_t b = v.wg * blockDim.x + threadIdx.x; b < d.st->n_blocks_prev; b += v.wgs * blockDim.x) {
+      const uint32_t s = d.block_list[b];
+      d.block_list_prev[b] = s;
+      d.prev_ptr[s] = b;
+    }
   // The transfer fields, and with a membrane its coupling fields (A_MF, A_MA, A_MW, A_MV).
   const uint32_t nf = ACC_CLEARED + (d.mem_cap ? 6u : 0u);
   for (uint32_t k = 0; k < nf; ++k) {
@@ -2222,6 +2230,112 @@ __global__ __launch_bounds__(256) void k_mem_sample(L l) {
   d.mem_water[i] = make_float4(vel[0], vel[1], vel[2], fill);
 }
 
+// ---- the air boundary (air.wgsl:85-148, 271-277; fluid.wgsl:91-107 for the block maps) -----------
+//
+// The air's own substeps (HLL fluxes) are not part of this library; what the water substep does
+// with the gas is: the gas of the last step's blocks carried into this step's (air_carry), cells
+// that stop being water given the gas next to them or vapour (air_fill), and the gas's
+// pressure handed to the solve for GAS and RUBBER cells.
+
+constexpr float GAMMA = 1.4f, P_ATM = 101325.0f, P_VAPOUR = 2339.0f, RHO_ATM = 1.204f, RHO_VAPOUR = 0.0278f;
+
+struct Gas {
+  float rho, mx, my, mz, e;
+};
+__device__ __forceinline__ Gas gas_at_rest(float rho, float p) { return {rho, 0.0f, 0.0f, 0.0f, p / (GAMMA - 1.0f)}; }
+__device__ __forceinline__ float gas_p(const Gas& g) {
+  return (GAMMA - 1.0f) * (g.e - 0.5f * ((g.mx * g.mx + g.my * g.my) + g.mz * g.mz) / g.rho);
+}
+__device__ __forceinline__ Gas read_gas(const float* air, uint32_t capc, uint32_t cell) {
+  return {air[cell], air[(size_t)capc + cell], air[2 * (size_t)capc + cell], air[3 * (size_t)capc + cell],
+          air[4 * (size_t)capc + cell]};
+}
+__device__ __forceinline__ void write_gas(float* air, uint32_t capc, uint32_t cell, const Gas& g) {
+  air[cell] = g.rho;
+  air[(size_t)capc + cell] = g.mx;
+  air[2 * (size_t)capc + cell] = g.my;
+  air[3 * (size_t)capc + cell] = g.mz;
+  air[4 * (size_t)capc + cell] = g.e;
+}
+
+// keep_blocks (air.wgsl:271-277): the blocks whose air was doing anything last step are
+// claimed again.
+template <class L>
+__global__ __launch_bounds__(256) void k_keep_blocks(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  if (d.st->dt_ns == 0u || !d.air_x) return;
+  for (uint32_t i = v.wg * blockDim.x + threadIdx.x; i < d.st->n_blocks_prev; i += v.wgs * blockDim.x)
+    if (d.block_active[i] != 0u) d.block_ptr[d.block_list_prev[i]] = PENDING;
+}
+
+// air_carry (air.wgsl:85-98): each cell of this step's blocks gets the gas it had last step, or
+// the ambient air where its block was not allocated then.
+template <class L>
+__global__ __launch_bounds__(256) void k_air_carry(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  if (d.st->dt_ns == 0u || !d.air_x) return;
+  const uint32_t n_prev = d.st->n_blocks_prev;
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u, [&](uint32_t b, uint32_t lane, int, int, int, const uint32_t*) {
+    const uint32_t s = d.block_list[b], o = d.prev_ptr[s];
+    Gas g = gas_at_rest(RHO_ATM, P_ATM);
+    if (o < n_prev && d.block_list_prev[o] == s) g = read_gas(d.air_x, d.capc, o * 64u + lane);
+    write_gas(d.air_tmp, d.capc, b * 64u + lane, g);
+    if (lane == 0) d.block_active[b] = 0u;
+  });
+}
+
+// air_fill (air.wgsl:100-148), after grid_prep: water and solid cells hold no gas; a cell
+// without gas takes the mean of its neighbours' or, with none, vapour; a GAS cell's pressure is
+// its gas's, a RUBBER cell's the lowest of its GAS neighbours' (gauge).
+template <class L>
+__global__ __launch_bounds__(256) void k_air_fill(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  if (d.st->dt_ns == 0u || !d.air_x) return;
+  const uint32_t capc = d.capc;
+  for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    const uint32_t cell = b * 64u + lane;
+    const float ty = cfr(d, NATRAY_F_TYPE, cell);
+    if (ty == WATER || ty == SOLID) {
+      write_gas(d.air_x, capc, cell, Gas{0.0f, 0.0f, 0.0f, 0.0f, 0.0f});
+      return;
+    }
+    uint32_t nbs[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int sg = (k % 2 == 1) ? 1 : -1;
+      nbs[k] = nbr(lk, lx, ly, lz, k / 2 == 0 ? sg : 0, k / 2 == 1 ? sg : 0, k / 2 == 2 ? sg : 0);
+    }
+    Gas g = read_gas(d.air_tmp, capc, cell);
+    if (g.rho <= 0.0f) {
+      Gas sum{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      float n = 0.0f;
+      for (int k = 0; k < 6; ++k) {
+        if (nbs[k] == NONE) continue;
+        const Gas q = read_gas(d.air_tmp, capc, nbs[k]);
+        if (q.rho > 0.0f) {
+          sum = Gas{sum.rho + q.rho, sum.mx + q.mx, sum.my + q.my, sum.mz + q.mz, sum.e + q.e};
+          n += 1.0f;
+        }
+      }
+      g = n > 0.0f ? Gas{sum.rho / n, sum.mx / n, sum.my / n, sum.mz / n, sum.e / n} : gas_at_rest(RHO_VAPOUR, P_VAPOUR);
+    }
+    write_gas(d.air_x, capc, cell, g);
+    if (ty == GAS) cfr(d, NATRAY_F_P, cell) = gas_p(g) - P_ATM;
+    if (ty == RUBBER) {
+      float p = 3.0e38f;
+      for (int k = 0; k < 6; ++k) {
+        if (nbs[k] == NONE || cfr(d, NATRAY_F_TYPE, nbs[k]) != GAS) continue;
+        const Gas q = read_gas(d.air_tmp, capc, nbs[k]);
+        if (q.rho > 0.0f) p = fmin_(p, gas_p(q));
+      }
+      cfr(d, NATRAY_F_P, cell) = (p < 3.0e38f ? p : gas_p(g)) - P_ATM;
+    }
+  });
+}
+
 // ---- keyframe pack of the water (keys.wgsl:9-37, pack.wgsl:33-39) ------------------------------
 
 // pack_pos (keys.wgsl:21-24): 21 bits per axis over an 8 m box from (-4, -0.5, -4), rounded to
@@ -2272,8 +2386,9 @@ int launch_begin(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_begin, LK_ONE, 64, t, S(stream));
   return (int)hipGetLastError();
 }
-int launch_bin(const Target& t, void* stream, bool rubber) {
+int launch_bin(const Target& t, void* stream, bool rubber, bool air) {
   NATRAY_LAUNCH(k_clear, LK_CELLS, 256, t, S(stream));
+  if (air) NATRAY_LAUNCH(k_keep_blocks, LK_ONE, 256, t, S(stream));
   NATRAY_LAUNCH(k_mark, LK_PARTS, 256, t, S(stream));
   if (rubber) NATRAY_LAUNCH(k_mark_rubber, LK_VERTS, 256, t, S(stream));
   NATRAY_LAUNCH(k_scan_reduce, LK_TILES, 256, t, S(stream));
@@ -2357,6 +2472,14 @@ int launch_mem_sample(const Target& t, void* stream) {
   return (int)hipGetLastError();
 }
 
+int launch_air_carry(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_air_carry, LK_WAVES, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+int launch_air_fill(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_air_fill, LK_WAVES, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
 int launch_pack_water(const Target& t, void* stream, void* out) {
   NATRAY_LAUNCH(k_pack_water, LK_PARTS, 256, t, S(stream), static_cast<uint2*>(out));
   return (int)hipGetLastError();

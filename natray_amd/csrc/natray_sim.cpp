@@ -219,12 +219,16 @@ int enqueue_phase(natray_sim* s, int phase) {
   }
   switch (phase) {
     case NATRAY_PHASE_BEGIN: LAUNCHCHK(natray::launch_begin(s->tgt, st)); break;
-    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(s->tgt, st, s->D.mem_cap > 0)); break;
+    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(s->tgt, st, s->D.mem_cap > 0, s->D.air_x != nullptr)); break;
     case NATRAY_PHASE_P2G:
       LAUNCHCHK(natray::launch_p2g(s->tgt, st));
+      if (s->D.air_x) LAUNCHCHK(natray::launch_air_carry(s->tgt, st));
       if (s->D.mem_cap) LAUNCHCHK(natray::launch_mem_splat(s->tgt, st));
       break;
-    case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(s->tgt, st)); break;
+    case NATRAY_PHASE_GRID:
+      LAUNCHCHK(natray::launch_grid_prep(s->tgt, st));
+      if (s->D.air_x) LAUNCHCHK(natray::launch_air_fill(s->tgt, st));
+      break;
     case NATRAY_PHASE_SOLVE: {
       // Tags are 32-bit: clear the tagged words well before a tag could repeat.
       const uint64_t bound = solve_tag_bound(s);
@@ -432,6 +436,18 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &D.vmax_part, (size_t)D.cap));
   TRY(dalloc(s, &D.solver, 8));
   TRY(dalloc(s, &D.stats, 8));
+  if (s->sc.flags & NATRAY_SCENE_AIR) {
+    TRY(dalloc(s, &D.air_x, 5 * capc));
+    TRY(dalloc(s, &D.air_tmp, 5 * capc));
+    TRY(dalloc(s, &D.prev_ptr, D.Spad));
+    TRY(dalloc(s, &D.block_list_prev, (size_t)D.cap + 1));
+    TRY(dalloc(s, &D.block_active, (size_t)D.cap + 1));
+    HIPTRY(hipMemsetAsync(D.air_x, 0, 5 * capc * 4, s->stream));
+    HIPTRY(hipMemsetAsync(D.air_tmp, 0, 5 * capc * 4, s->stream));
+    HIPTRY(hipMemsetAsync(D.prev_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
+    HIPTRY(hipMemsetAsync(D.block_list_prev, 0, ((size_t)D.cap + 1) * 4, s->stream));
+    HIPTRY(hipMemsetAsync(D.block_active, 0, ((size_t)D.cap + 1) * 4, s->stream));
+  }
   natray::Dev* dself = nullptr;
   TRY(dalloc(s, &dself, 1));
   D.self = dself;
@@ -809,6 +825,16 @@ int natray_sim_set_membrane(natray_sim* s, uint32_t n, const float* x, const flo
   return NATRAY_OK;
 }
 
+int natray_sim_read_air(natray_sim* s, float* air) {
+  if (!s || !air) return fail(NATRAY_E_ARG, "null argument");
+  if (!s->D.air_x) return fail(NATRAY_E_ARG, "the scene has no air boundary (NATRAY_SCENE_AIR)");
+  const size_t nc = (size_t)std::min(s->last.n_blocks, s->D.cap) * 64;
+  for (int f = 0; f < 5; ++f)
+    if (nc) HIPCHK(hipMemcpyAsync(air + f * nc, s->D.air_x + (size_t)f * s->D.capc, nc * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
 int natray_sim_pack_water(natray_sim* s, uint32_t* words) {
   if (!s || !words) return fail(NATRAY_E_ARG, "null argument");
   if (!s->D.N) return NATRAY_OK;
@@ -867,6 +893,7 @@ struct natray_batch {
   hipStream_t stream = nullptr;
   int num_cus = 0;
   int mixed = 0;
+  bool air = false;      // some scene has the air boundary (the kernels skip the others)
   int solve_budget = 0;  // co-resident workgroups of the cooperative solve
   natray_batch_shape shape{};
   natray::Dev* scenes = nullptr;                   // device table, one Dev per scene
@@ -1006,9 +1033,15 @@ int batch_phase(natray_batch* b, int phase) {
   void* st = b->stream;
   switch (phase) {
     case NATRAY_PHASE_BEGIN: LAUNCHCHK(natray::launch_begin(t, st)); break;
-    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(t, st)); break;
-    case NATRAY_PHASE_P2G: LAUNCHCHK(natray::launch_p2g(t, st)); break;
-    case NATRAY_PHASE_GRID: LAUNCHCHK(natray::launch_grid_prep(t, st)); break;
+    case NATRAY_PHASE_BIN: LAUNCHCHK(natray::launch_bin(t, st, false, b->air)); break;
+    case NATRAY_PHASE_P2G:
+      LAUNCHCHK(natray::launch_p2g(t, st));
+      if (b->air) LAUNCHCHK(natray::launch_air_carry(t, st));
+      break;
+    case NATRAY_PHASE_GRID:
+      LAUNCHCHK(natray::launch_grid_prep(t, st));
+      if (b->air) LAUNCHCHK(natray::launch_air_fill(t, st));
+      break;
     case NATRAY_PHASE_SOLVE: {
       for (natray_sim* s : b->sims) {
         const uint64_t bound = solve_tag_bound(s);
@@ -1129,6 +1162,7 @@ int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out)
   b->sims.assign(sims, sims + n);
   b->num_cus = sims[0]->num_cus;
   b->mixed = sims[0]->mixed;
+  for (uint32_t k = 0; k < n; ++k) b->air = b->air || sims[k]->D.air_x != nullptr;
   b->solve_budget = sims[0]->solve_grid;
   b->shape = {8u, 8u, 4u, 0u};
   auto bail = [&](int code) {

@@ -173,6 +173,13 @@ class Sim:
         abi.check(abi.lib().natray_sim_read_membrane(self._h, abi.ptr(out)))
         return out
 
+    def air(self) -> np.ndarray:
+        """Each cell's gas (natray_sim_read_air): [5][64 n] for the blocks in use."""
+        n = self.n_blocks()
+        out = np.zeros((5, 64 * n), np.float32)
+        abi.check(abi.lib().natray_sim_read_air(self._h, abi.ptr(out)))
+        return out
+
     def pack_water(self) -> np.ndarray:
         """The water's keyframe words (natray_sim_pack_water): [N][2] uint32."""
         out = np.zeros((self.n, 2), np.uint32)

@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
                 "set_membrane": (C.c_int, [P, C.c_uint32, P, P, P, P]),
                 "read_membrane": (None, [P, P]),
                 "pack_water": (None, [P, P]),
+                "read_air": (None, [P, P]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -202,6 +203,12 @@ class Oracle:
         mass, push = np.ascontiguousarray(mass, np.float32), np.ascontiguousarray(push, np.float32)
         self._mem_n = int(mass.size)
         assert self._f("set_membrane")(self._h, self._mem_n, abi.ptr(x), abi.ptr(v), abi.ptr(mass), abi.ptr(push)) == 0
+
+    def air(self) -> np.ndarray:
+        n = self.n_blocks()
+        out = np.zeros((5, 64 * n), np.float32)
+        self._f("read_air")(self._h, abi.ptr(out))
+        return out
 
     def pack_water(self) -> np.ndarray:
         out = np.zeros((self.n, 2), np.uint32)

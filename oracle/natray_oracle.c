@@ -100,6 +100,11 @@ struct osim {
    * push and area (2 per vertex); mem_water (4 per vertex) is what mem_sample leaves. */
   uint32_t mem_cap;
   REAL *mem_x, *mem_v, *mem_mass, *mem_push, *mem_water;
+  /* The air boundary (NATRAY_SCENE_AIR): gas per cell [5][cap x 64] after the step and as
+   * carried into the step's blocks, the last step's block list, each slot's index in it, and
+   * the blocks whose air was active. */
+  REAL *air_x, *air_tmp;
+  uint32_t *prev_ptr, *block_list_prev, *block_active;
 };
 
 /* ---- Params / State accessors (sim_defs.wgsl:159-231) -------------------------------- */
@@ -347,7 +352,15 @@ static void o_bin(osim* s) {
     uint32_t sl = s->block_list[b];
     s->block_ptr[sl] = NONE;
     s->slot_count[sl] = 0;
+    if (s->air_x) {
+      s->block_list_prev[b] = sl;
+      s->prev_ptr[sl] = b;
+    }
   }
+  /* keep_blocks (air.wgsl:271-277): the blocks whose air was active are claimed again. */
+  if (s->air_x)
+    for (uint32_t b = 0; b < nprev; ++b)
+      if (s->block_active[b]) s->block_ptr[s->block_list_prev[b]] = PENDING;
   for (uint32_t f = 0; f < NATRAY_ACC_FIELDS; ++f) memset(&s->acc[(size_t)f * cap], 0, (size_t)nprev * 64u * 4u);
   /* mark_blocks (fluid.wgsl:131-150): claim every block the stencil (cells c-1..c+2) touches,
    * take a rank in the own block. Claims are recorded as PENDING and given indices in slot
@@ -593,6 +606,101 @@ static void o_mem_sample(osim* s) {
     mw[1] = v[1];
     mw[2] = v[2];
     mw[3] = r[1] / (REAL)8;
+  }
+}
+
+/* ---- the air boundary (air.wgsl:85-148) ------------------------------------------------ */
+
+#define GAMMA ((REAL)1.4f)
+#define P_ATM ((REAL)101325.0f)
+#define P_VAPOUR ((REAL)2339.0f)
+#define RHO_ATM ((REAL)1.204f)
+#define RHO_VAPOUR ((REAL)0.0278f)
+
+typedef struct { REAL rho, mx, my, mz, e; } gas;
+static inline gas gas_at_rest(REAL rho, REAL p) { gas g = {rho, 0, 0, 0, p / (GAMMA - (REAL)1)}; return g; }
+static inline REAL gas_p(gas g) { return (GAMMA - (REAL)1) * (g.e - (REAL)0.5 * ((g.mx * g.mx + g.my * g.my) + g.mz * g.mz) / g.rho); }
+static inline gas read_gas(const REAL* air, uint32_t capc, uint32_t cell) {
+  gas g = {air[cell], air[(size_t)capc + cell], air[2 * (size_t)capc + cell], air[3 * (size_t)capc + cell], air[4 * (size_t)capc + cell]};
+  return g;
+}
+static inline void write_gas(REAL* air, uint32_t capc, uint32_t cell, gas g) {
+  air[cell] = g.rho;
+  air[(size_t)capc + cell] = g.mx;
+  air[2 * (size_t)capc + cell] = g.my;
+  air[3 * (size_t)capc + cell] = g.mz;
+  air[4 * (size_t)capc + cell] = g.e;
+}
+
+/* air_carry (air.wgsl:85-98): each cell gets its gas of last step, or the ambient air where its
+ * block was not allocated then. */
+static void o_air_carry(osim* s) {
+  if (s->st.dt_ns == 0u || !s->air_x) return;
+  uint32_t nb = n_blocks_(s), capc = cap_cells(s), nprev = s->st.n_blocks_prev;
+  for (uint32_t b = 0; b < nb; ++b) {
+    uint32_t sl = s->block_list[b], o = s->prev_ptr[sl];
+    int carried = o < nprev && s->block_list_prev[o] == sl;
+    for (uint32_t li = 0; li < 64; ++li) {
+      gas g = carried ? read_gas(s->air_x, capc, o * 64u + li) : gas_at_rest(RHO_ATM, P_ATM);
+      write_gas(s->air_tmp, capc, b * 64u + li, g);
+    }
+    s->block_active[b] = 0;
+  }
+}
+
+/* air_fill (air.wgsl:100-148): water and solid cells hold no gas; a cell without gas takes the
+ * mean of its neighbours' (in the order -x, +x, -y, +y, -z, +z) or vapour; GAS and RUBBER cells
+ * get the gas's gauge pressure, a RUBBER cell the lowest of its GAS neighbours'. */
+static void o_air_fill(osim* s) {
+  if (s->st.dt_ns == 0u || !s->air_x) return;
+  uint32_t nc = n_blocks_(s) * 64u, capc = cap_cells(s);
+  for (uint32_t cell = 0; cell < nc; ++cell) {
+    REAL ty = cfv(s, NATRAY_F_TYPE, cell);
+    if (ty == WATER || ty == SOLID) {
+      gas z = {0, 0, 0, 0, 0};
+      write_gas(s->air_x, capc, cell, z);
+      continue;
+    }
+    uint32_t nbs[6];
+    for (int k = 0; k < 6; ++k) {
+      int e[3] = {0, 0, 0};
+      e[k / 2] = (k % 2 == 1) ? 1 : -1;
+      nbs[k] = neighbour(s, cell, e[0], e[1], e[2]);
+    }
+    gas g = read_gas(s->air_tmp, capc, cell);
+    if (g.rho <= (REAL)0) {
+      gas sum = {0, 0, 0, 0, 0};
+      REAL n = 0;
+      for (int k = 0; k < 6; ++k) {
+        if (nbs[k] == NONE) continue;
+        gas q = read_gas(s->air_tmp, capc, nbs[k]);
+        if (q.rho > (REAL)0) {
+          sum.rho = sum.rho + q.rho;
+          sum.mx = sum.mx + q.mx;
+          sum.my = sum.my + q.my;
+          sum.mz = sum.mz + q.mz;
+          sum.e = sum.e + q.e;
+          n += (REAL)1;
+        }
+      }
+      if (n > (REAL)0) {
+        gas m = {sum.rho / n, sum.mx / n, sum.my / n, sum.mz / n, sum.e / n};
+        g = m;
+      } else {
+        g = gas_at_rest(RHO_VAPOUR, P_VAPOUR);
+      }
+    }
+    write_gas(s->air_x, capc, cell, g);
+    if (ty == GAS) *cfp(s, NATRAY_F_P, cell) = gas_p(g) - P_ATM;
+    if (ty == RUBBER) {
+      REAL p = (REAL)3.0e38f;
+      for (int k = 0; k < 6; ++k) {
+        if (nbs[k] == NONE || cfv(s, NATRAY_F_TYPE, nbs[k]) != GAS) continue;
+        gas q = read_gas(s->air_tmp, capc, nbs[k]);
+        if (q.rho > (REAL)0) p = rmin(p, gas_p(q));
+      }
+      *cfp(s, NATRAY_F_P, cell) = (p < (REAL)3.0e38f ? p : gas_p(g)) - P_ATM;
+    }
   }
 }
 
@@ -1251,9 +1359,13 @@ int F(phase)(osim* s, int phase) {
     case NATRAY_PHASE_BIN: o_bin(s); break;
     case NATRAY_PHASE_P2G:
       o_p2g(s);
+      o_air_carry(s);
       o_mem_splat(s);
       break;
-    case NATRAY_PHASE_GRID: o_grid_prep(s); break;
+    case NATRAY_PHASE_GRID:
+      o_grid_prep(s);
+      o_air_fill(s);
+      break;
     case NATRAY_PHASE_SOLVE: o_solve(s); break;
     case NATRAY_PHASE_PROJECT: o_project(s); break;
     case NATRAY_PHASE_EXTEND:
@@ -1316,6 +1428,18 @@ osim* F(create)(const natray_params* params, const float* particles, uint32_t n,
     return NULL;
   }
   for (uint32_t i = 0; i < s->S; ++i) s->block_ptr[i] = NONE;
+  if (s->sc.flags & NATRAY_SCENE_AIR) {
+    s->air_x = (REAL*)calloc(5 * cells + 1, sizeof(REAL));
+    s->air_tmp = (REAL*)calloc(5 * cells + 1, sizeof(REAL));
+    s->prev_ptr = (uint32_t*)malloc(((size_t)s->S + 1) * 4);
+    s->block_list_prev = (uint32_t*)calloc((size_t)s->cap + 1, 4);
+    s->block_active = (uint32_t*)calloc((size_t)s->cap + 1, 4);
+    if (!s->air_x || !s->air_tmp || !s->prev_ptr || !s->block_list_prev || !s->block_active) {
+      F(destroy)(s);
+      return NULL;
+    }
+    for (uint32_t i = 0; i < s->S; ++i) s->prev_ptr[i] = NONE;
+  }
   for (size_t i = 0; i < (size_t)n * 20; ++i) s->P[i] = (REAL)particles[i];
   if (init) {
     F(write_state)(s, init);
@@ -1351,6 +1475,11 @@ void F(destroy)(osim* s) {
   free(s->mem_mass);
   free(s->mem_push);
   free(s->mem_water);
+  free(s->air_x);
+  free(s->air_tmp);
+  free(s->prev_ptr);
+  free(s->block_list_prev);
+  free(s->block_active);
   free(s);
 }
 
@@ -1512,6 +1641,13 @@ int F(set_membrane)(osim* s, uint32_t n, const float* x, const float* v, const f
 
 void F(read_membrane)(const osim* s, float* water) {
   for (size_t k = 0; k < (size_t)mem_count(s) * 4; ++k) water[k] = (float)s->mem_water[k];
+}
+
+void F(read_air)(const osim* s, float* out) {
+  uint32_t nc = n_blocks_(s) * 64u, capc = cap_cells(s);
+  if (!s->air_x) return;
+  for (uint32_t f = 0; f < 5; ++f)
+    for (uint32_t c = 0; c < nc; ++c) out[(size_t)f * nc + c] = (float)s->air_x[(size_t)f * capc + c];
 }
 
 void F(set_threads)(int n) {

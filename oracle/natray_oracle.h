@@ -45,6 +45,7 @@ typedef struct osim osim;
                       const float* push);                                                           \
   void P##read_membrane(const osim* s, float* water);                                               \
   void P##pack_water(const osim* s, uint32_t* out);                                                 \
+  void P##read_air(const osim* s, float* out);                                                      \
   void P##set_threads(int n);
 
 ORACLE_DECL(o32_)
