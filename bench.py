@@ -236,6 +236,14 @@ def batched_report(args) -> dict:
     b.run(args.batch_steps, iters=args.iters)
     wall = time.perf_counter() - t0
     c1 = totals()
+    # The per-phase breakdown: kernel-by-kernel batch steps with events around each phase.
+    b.set_graph(False)
+    b.set_profiling(True)
+    b.run(args.batch_steps, iters=args.iters)
+    times = b.phase_times()
+    b.set_profiling(False)
+    b.set_graph(True)
+    c1p = totals()
     # The same scenes one after another, each stepped alone for the same number of steps.
     t0 = time.perf_counter()
     for s in sims:
@@ -256,9 +264,10 @@ def batched_report(args) -> dict:
         "ms_per_batch_step": wall * 1e3 / max(args.batch_steps, 1),
         "particle_substeps_per_s": parts / wall if wall > 0 else 0.0,
         "cg_iters_per_scene_step": (c1["cg_iters"] - c0["cg_iters"]) / max(c1["steps"] - c0["steps"], 1),
+        "phases_ms_per_batch_step_kernel_by_kernel": {k: v[0] / max(v[1], 1) for k, v in times.items()},
         "scenes_one_after_another": {
             "ms_per_step_of_all_scenes": wall_alone * 1e3 / max(args.batch_steps, 1),
-            "particle_substeps_per_s": (c2["particles"] - c1["particles"]) / wall_alone if wall_alone > 0 else 0.0,
+            "particle_substeps_per_s": (c2["particles"] - c1p["particles"]) / wall_alone if wall_alone > 0 else 0.0,
         },
         "speedup_batch_over_one_after_another": wall_alone / wall if wall > 0 else None,
         "unconverged_steps": int(sum(x.unconverged for x in st)),

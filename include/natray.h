@@ -383,11 +383,17 @@ typedef struct {
 int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out);
 void natray_batch_destroy(natray_batch* batch);
 /* n_steps steps of every scene, with CG iteration cap `iters` (0: each scene's params.limits.z
- * as it stands); each scene's State is read back afterwards (natray_sim_read_state). */
+ * as it stands); each scene's State is read back afterwards (natray_sim_read_state). Once every
+ * scene has stepped, the solve's workgroups are shared by each scene's blocks times its last
+ * CG iteration count rather than by particles. */
 int natray_batch_run(natray_batch* batch, uint32_t n_steps, uint32_t iters);
 /* Steps as two replayed graphs around the solve (on, the default) or kernel by kernel. */
 int natray_batch_set_graph(natray_batch* batch, int on);
 int natray_batch_set_shape(natray_batch* batch, const natray_batch_shape* shape);
+/* Per-phase GPU time of the batch's kernel-by-kernel steps (graphs off), summed over the runs
+ * since profiling was switched on, as natray_sim_phase_times. */
+int natray_batch_set_profiling(natray_batch* batch, int on);
+int natray_batch_phase_times(natray_batch* batch, double* ms, uint32_t* counts);
 int natray_batch_get_shape(natray_batch* batch, natray_batch_shape* out);
 /* The autotuner: runs one untimed and `steps` timed steps (real steps of the scenes) with each
  * candidate shape (blocks and waves per CU in {4, 8, 16}), keeps the fastest and returns it in

@@ -172,6 +172,8 @@ _SIGNATURES = {
     "natray_batch_run": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "natray_batch_set_graph": (C.c_int, [_P, C.c_int]),
     "natray_batch_set_shape": (C.c_int, [_P, _P]),
+    "natray_batch_set_profiling": (C.c_int, [_P, C.c_int]),
+    "natray_batch_phase_times": (C.c_int, [_P, _P, _P]),
     "natray_batch_get_shape": (C.c_int, [_P, _P]),
     "natray_batch_tune": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "natray_batch_plan": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),

@@ -56,6 +56,16 @@ class Batch:
     def set_graph(self, on: bool) -> None:
         abi.check(abi.lib().natray_batch_set_graph(self._h, 1 if on else 0))
 
+    def set_profiling(self, on: bool) -> None:
+        abi.check(abi.lib().natray_batch_set_profiling(self._h, 1 if on else 0))
+
+    def phase_times(self) -> dict:
+        """{phase: (ms, launches)} of the kernel-by-kernel steps since profiling was switched on."""
+        ms = np.zeros(abi.N_PHASES, np.float64)
+        cnt = np.zeros(abi.N_PHASES, np.uint32)
+        abi.check(abi.lib().natray_batch_phase_times(self._h, abi.ptr(ms), abi.ptr(cnt)))
+        return {abi.PHASE_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(abi.N_PHASES)}
+
     def shape(self) -> dict:
         s = Shape()
         abi.check(abi.lib().natray_batch_get_shape(self._h, C.byref(s)))

@@ -904,11 +904,21 @@ struct natray_batch {
   bool use_graph = true;
   hipGraphExec_t graph_pre = nullptr, graph_post = nullptr;
   std::vector<void*> allocs;
+  // Per-phase timing (natray_batch_phase_times): events around each phase of kernel-by-kernel
+  // steps, summed when the run ends.
+  bool prof = false;
+  std::vector<hipEvent_t> events;
+  std::vector<int> ev_phase;
+  size_t ev_used = 0;
+  double ms[NATRAY_N_PHASES] = {};
+  uint32_t cnt[NATRAY_N_PHASES] = {};
 };
 
 namespace {
 
 void batch_free(natray_batch* b) {
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  b->events.clear();
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->graph_pre) (void)hipGraphExecDestroy(b->graph_pre);
   if (b->graph_post) (void)hipGraphExecDestroy(b->graph_post);
@@ -1028,7 +1038,49 @@ int batch_layout(natray_batch* b) {
   return NATRAY_OK;
 }
 
+int batch_phase_launch(natray_batch* b, int phase);
+
+// The solve's workgroups shared by the work each scene's last solve had, blocks times CG
+// iterations (the slowest scene sets the batch's solve time), once every scene has run a
+// step; the solve is launched outside the step graphs, so only its map changes.
+int batch_plan_solve(natray_batch* b) {
+  const size_t n = b->sims.size();
+  std::vector<uint64_t> w(n);
+  for (size_t k = 0; k < n; ++k) {
+    const natray_state& st = b->sims[k]->last;
+    if (st.step == 0 || st.n_blocks == 0) return NATRAY_OK;
+    w[k] = (uint64_t)std::min(st.n_blocks, b->sims[k]->D.cap) * (uint64_t)(st.solve_iters + 1u);
+  }
+  std::vector<uint32_t> shares;
+  plan_shares(w, std::vector<uint32_t>(n, 0xffffffffu), (uint32_t)b->solve_budget, 1, shares);
+  if (shares == b->counts[natray::LK_SOLVE]) return NATRAY_OK;
+  std::vector<natray::WgMap> m;
+  for (size_t k = 0; k < n; ++k)
+    for (uint32_t g = 0; g < shares[k]; ++g) m.push_back({(uint32_t)k, g, shares[k], 0u});
+  if (m.size() > b->map_cap[natray::LK_SOLVE]) return NATRAY_OK;  // keeps the current plan
+  HIPCHK(hipMemcpyAsync(b->maps[natray::LK_SOLVE], m.data(), m.size() * sizeof(natray::WgMap), hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->counts[natray::LK_SOLVE] = shares;
+  b->tgt.grid[natray::LK_SOLVE] = (uint32_t)m.size();
+  return NATRAY_OK;
+}
+
 int batch_phase(natray_batch* b, int phase) {
+  const bool timed = b->prof && !b->use_graph && b->ev_used < b->events.size() / 2;
+  size_t slot = 0;
+  if (timed) {
+    slot = b->ev_used++;
+    b->ev_phase[slot] = phase;
+    HIPCHK(hipEventRecord(b->events[2 * slot], b->stream));
+  }
+  int e = batch_phase_launch(b, phase);
+  if (e) return e;
+  if (timed) HIPCHK(hipEventRecord(b->events[2 * slot + 1], b->stream));
+  return NATRAY_OK;
+}
+
+int batch_phase_launch(natray_batch* b, int phase) {
   const natray::Target& t = b->tgt;
   void* st = b->stream;
   switch (phase) {
@@ -1105,6 +1157,8 @@ int batch_step(natray_batch* b) {
 // n_steps steps of every scene; the scenes' States are read back and checked afterwards.
 // Events t0 and t1, when given, are recorded on the batch's stream around the steps.
 int batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+  int rc = batch_plan_solve(b);
+  if (rc) return rc;
   for (natray_sim* s : b->sims) {
     s->U.limits[2] = iters ? iters : s->U.limits[2];
     s->U.limits[3] = 0xffffffffu;
@@ -1122,6 +1176,13 @@ int batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters, hipEvent_t t0 =
   }
   if (t1) HIPCHK(hipEventRecord(t1, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  for (size_t k = 0; k < b->ev_used; ++k) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, b->events[2 * k], b->events[2 * k + 1]));
+    b->ms[b->ev_phase[k]] += ms;
+    b->cnt[b->ev_phase[k]] += 1;
+  }
+  b->ev_used = 0;
   for (size_t k = 0; k < b->sims.size(); ++k) {
     int e = sync_state(b->sims[k]);
     if (e) return e;
@@ -1203,6 +1264,32 @@ int natray_batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters) {
 int natray_batch_set_graph(natray_batch* b, int on) {
   if (!b) return fail(NATRAY_E_ARG, "null handle");
   b->use_graph = on != 0;
+  return NATRAY_OK;
+}
+
+int natray_batch_set_profiling(natray_batch* b, int on) {
+  if (!b) return fail(NATRAY_E_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (on && b->events.empty()) {
+    b->events.resize(EVENT_POOL);
+    b->ev_phase.resize(EVENT_POOL / 2);
+    for (size_t k = 0; k < EVENT_POOL; ++k) HIPCHK(hipEventCreate(&b->events[k]));
+  }
+  b->prof = on != 0;
+  b->ev_used = 0;
+  for (int k = 0; k < NATRAY_N_PHASES; ++k) {
+    b->ms[k] = 0.0;
+    b->cnt[k] = 0;
+  }
+  return NATRAY_OK;
+}
+
+int natray_batch_phase_times(natray_batch* b, double* ms, uint32_t* counts) {
+  if (!b) return fail(NATRAY_E_ARG, "null handle");
+  for (int k = 0; k < NATRAY_N_PHASES; ++k) {
+    if (ms) ms[k] = b->ms[k];
+    if (counts) counts[k] = b->cnt[k];
+  }
   return NATRAY_OK;
 }
 
