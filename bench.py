@@ -432,7 +432,7 @@ def main() -> int:
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start report")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes report")
     ap.add_argument("--batch-steps", type=int, default=30, help="timed steps of the 64-scene batch")
-    ap.add_argument("--batch-tune-steps", type=int, default=2, help="timed steps per autotuner candidate")
+    ap.add_argument("--batch-tune-steps", type=int, default=4, help="autotuner rounds (one timed step per candidate each)")
     ap.add_argument("--ws-tiles", type=int, default=50000, help="training tiles to log for the warm start")
     ap.add_argument("--ws-collect-steps", type=int, default=100, help="steps logged per training seed")
     ap.add_argument("--ws-train-steps", type=int, default=6000)

@@ -395,9 +395,12 @@ int natray_batch_set_shape(natray_batch* batch, const natray_batch_shape* shape)
 int natray_batch_set_profiling(natray_batch* batch, int on);
 int natray_batch_phase_times(natray_batch* batch, double* ms, uint32_t* counts);
 int natray_batch_get_shape(natray_batch* batch, natray_batch_shape* out);
-/* The autotuner: runs one untimed and `steps` timed steps (real steps of the scenes) with each
- * candidate shape (blocks and waves per CU in {4, 8, 16}), keeps the fastest and returns it in
- * *best, with the candidates' ms per step in ms[] (up to max_candidates) and their number. */
+/* The autotuner: times each candidate shape (blocks and waves per CU in {4, 8, 16}) on real
+ * steps of the scenes, one step per candidate per round, the rounds visiting the candidates
+ * forwards and backwards in turn so that the scenes' change over the steps weighs all alike
+ * (`steps` rounds, rounded up to even; the scenes advance by rounds x candidates steps). Keeps
+ * the fastest and returns it in *best, with the candidates' mean ms per step in ms[] (up to
+ * max_candidates) and their number. */
 int natray_batch_tune(natray_batch* batch, uint32_t steps, uint32_t iters, natray_batch_shape* best, double* ms,
                       uint32_t max_candidates, uint32_t* n_candidates);
 /* The workgroup shares the batch uses: out[k] = scene k's part of `budget`, in proportion to

@@ -1133,6 +1133,14 @@ int batch_capture(natray_batch* b, int lo, int hi, hipGraphExec_t* out) {
   return NATRAY_OK;
 }
 
+// Captures the step graphs of the current shape if they are missing (recording runs nothing).
+int batch_graphs(natray_batch* b) {
+  if (!b->use_graph || b->graph_pre) return NATRAY_OK;
+  int e = batch_capture(b, NATRAY_PHASE_BEGIN, NATRAY_PHASE_SOLVE, &b->graph_pre);
+  if (e) return e;
+  return batch_capture(b, NATRAY_PHASE_PROJECT, NATRAY_N_PHASES, &b->graph_post);
+}
+
 int batch_step(natray_batch* b) {
   if (!b->use_graph) {
     for (int ph = 0; ph < NATRAY_N_PHASES; ++ph) {
@@ -1141,14 +1149,10 @@ int batch_step(natray_batch* b) {
     }
     return NATRAY_OK;
   }
-  if (!b->graph_pre) {
-    int e = batch_capture(b, NATRAY_PHASE_BEGIN, NATRAY_PHASE_SOLVE, &b->graph_pre);
-    if (e) return e;
-    e = batch_capture(b, NATRAY_PHASE_PROJECT, NATRAY_N_PHASES, &b->graph_post);
-    if (e) return e;
-  }
+  int e = batch_graphs(b);
+  if (e) return e;
   HIPCHK(hipGraphLaunch(b->graph_pre, b->stream));
-  int e = batch_phase(b, NATRAY_PHASE_SOLVE);
+  e = batch_phase(b, NATRAY_PHASE_SOLVE);
   if (e) return e;
   HIPCHK(hipGraphLaunch(b->graph_post, b->stream));
   return NATRAY_OK;
@@ -1319,28 +1323,36 @@ int natray_batch_tune(natray_batch* b, uint32_t steps, uint32_t iters, natray_ba
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  double best_ms = 0.0;
-  size_t best_k = 0;
+  // The scenes change from step to step (an impact's water spreads and its steps get dearer),
+  // so the candidates are timed one step at a time in rounds that visit them forwards, then
+  // backwards: over each pair of rounds a drift linear in the step count adds the same time to
+  // every candidate. `steps` is rounded up to an even number of rounds. A candidate's graphs
+  // are captured before its timed step, which is the only step it runs per round.
+  const uint32_t rounds = (steps + 1u) & ~1u;
+  std::vector<double> total(cand.size(), 0.0);
   int rc = NATRAY_OK;
-  for (size_t k = 0; k < cand.size() && rc == NATRAY_OK; ++k) {
-    rc = natray_batch_set_shape(b, &cand[k]);
-    if (rc) break;
-    rc = batch_run(b, 1, iters);  // captures the graphs of this shape
-    if (rc) break;
-    rc = batch_run(b, steps, iters, e0, e1);
-    if (rc) break;
-    float t = 0.0f;
-    (void)hipEventElapsedTime(&t, e0, e1);
-    const double per = (double)t / steps;
-    if (ms && k < max_candidates) ms[k] = per;
-    if (k == 0 || per < best_ms) {
-      best_ms = per;
-      best_k = k;
+  for (uint32_t r = 0; r < rounds && rc == NATRAY_OK; ++r) {
+    for (size_t i = 0; i < cand.size() && rc == NATRAY_OK; ++i) {
+      const size_t k = (r & 1u) ? cand.size() - 1 - i : i;
+      rc = natray_batch_set_shape(b, &cand[k]);
+      if (rc) break;
+      rc = batch_graphs(b);
+      if (rc) break;
+      rc = batch_run(b, 1, iters, e0, e1);
+      if (rc) break;
+      float t = 0.0f;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      total[k] += (double)t;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
+  size_t best_k = 0;
+  for (size_t k = 0; k < cand.size(); ++k) {
+    if (ms && k < max_candidates) ms[k] = total[k] / rounds;
+    if (total[k] < total[best_k]) best_k = k;
+  }
   if (n_candidates) *n_candidates = (uint32_t)cand.size();
   if (best) *best = cand[best_k];
   return natray_batch_set_shape(b, &cand[best_k]);

@@ -52,7 +52,7 @@ def batched(scene_list, steps, graph=True, tune=False):
     chosen = None
     if tune:
         chosen = b.tune(1, iters=400)
-        steps -= 2 * 9  # the tuner's steps (one untimed and one timed per candidate) are real steps
+        steps -= 2 * 9  # the tuner's steps (two rounds of one step per candidate) are real steps
     b.run(steps, iters=400)
     out = [(s.particles(), s.state()) for s in sims]
     b.close()
