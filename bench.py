@@ -109,32 +109,35 @@ def timed_run(sim, warmup: int, steps: int, iters: int, graph: bool = True):
 
 
 def compare_precision(scene, args, plain: dict) -> dict:
-    """The same workload from the same initial state with the other CG precision (FP32 SpMV
-    and axpy, FP64 dots, norms and convergence test, FP64 true residual every 20 iterations),
-    at the same tolerance in the reference's convergence form: iterations and solve time per
-    step against the plain FP32 run."""
+    """The same workload from the same initial state with the other CG forms, at the same
+    tolerance in the reference's convergence form: mixed precision (FP32 SpMV and axpy, FP64
+    dots, norms and convergence test, FP64 true residual every 20 iterations) and the pipelined
+    CG with that precision (one reduction per iteration, residual replacement every 10
+    iterations while the residual is large). Iterations and solve time per step against the
+    plain FP32 run."""
     from natray_amd import Sim, abi
 
     sc = scene.scene
     keep = (sc.precision, sc.refresh_every)
-    sc.precision, sc.refresh_every = abi.CG_MIXED, 20
+    out = {"tol": scene.params.solve[0], "plain_fp32": plain}
     try:
-        sim = Sim.from_scene(scene)
-        wall, c, times, st = timed_run(sim, args.warmup, args.steps, args.iters)
-        sim.close()
+        for label, prec, refresh in (("mixed_f32_f64", abi.CG_MIXED, 20), ("pipelined_f32_f64", abi.CG_PIPELINED, 10)):
+            sc.precision, sc.refresh_every = prec, refresh
+            sim = Sim.from_scene(scene)
+            wall, c, times, st = timed_run(sim, args.warmup, args.steps, args.iters)
+            sim.close()
+            steps = max(c["steps"], 1)
+            out[label] = {"cg_iters_per_step": c["cg_iters"] / steps, "solve_ms_per_step": times["solve"][0] / steps,
+                          "ms_per_step": wall * 1e3 / max(args.steps, 1), "unconverged_steps": st.unconverged,
+                          "refresh_every": refresh}
     finally:
         sc.precision, sc.refresh_every = keep
-    steps = max(c["steps"], 1)
-    mixed = {"cg_iters_per_step": c["cg_iters"] / steps, "solve_ms_per_step": times["solve"][0] / steps,
-             "ms_per_step": wall * 1e3 / max(args.steps, 1), "unconverged_steps": st.unconverged}
-    return {
-        "tol": scene.params.solve[0],
-        "plain_fp32": plain,
-        "mixed_f32_f64": mixed,
-        "solve_speedup_mixed_over_plain": plain["solve_ms_per_step"] / mixed["solve_ms_per_step"]
-        if mixed["solve_ms_per_step"] > 0 else None,
-        "warm_start": "particle-carried pressure (the reference's p0) in both runs",
-    }
+    for label, key in (("mixed_f32_f64", "solve_speedup_mixed_over_plain"),
+                       ("pipelined_f32_f64", "solve_speedup_pipelined_over_plain")):
+        m = out[label]["solve_ms_per_step"]
+        out[key] = plain["solve_ms_per_step"] / m if m > 0 else None
+    out["warm_start"] = "particle-carried pressure (the reference's p0) in every run"
+    return out
 
 
 def warm_start_report(scene, args) -> dict:
@@ -334,7 +337,7 @@ def _errors(g: dict, P: np.ndarray, g_ref: dict, cf_ref: np.ndarray, P_ref: np.n
 
 
 def accuracy_vs_fp64(scene, checkpoints, iters: int, one_from: int = 20) -> dict:
-    """Config 3's report: the plain FP32 and the mixed-precision CG run from the scene's
+    """Config 3's report: the plain FP32, the mixed-precision and the pipelined CG run from the scene's
     initial state on the GPU, the FP64 oracle on the host; errors of the GPU runs against it
     after each checkpoint's number of steps, at the same tolerance. Relative errors are the
     largest difference over the largest reference magnitude."""
@@ -357,7 +360,8 @@ def accuracy_vs_fp64(scene, checkpoints, iters: int, one_from: int = 20) -> dict
     sc = scene.scene
     keep = (sc.precision, sc.refresh_every)
     try:
-        for label, prec, refresh in (("plain_fp32", abi.CG_FP32, 0), ("mixed_f32_f64", abi.CG_MIXED, 20)):
+        for label, prec, refresh in (("plain_fp32", abi.CG_FP32, 0), ("mixed_f32_f64", abi.CG_MIXED, 20),
+                                     ("pipelined_f32_f64", abi.CG_PIPELINED, 10)):
             sc.precision, sc.refresh_every = prec, refresh
             sim = Sim.from_scene(scene)
             done = 0

@@ -164,12 +164,17 @@ typedef struct natray_particle {
 
 #define NATRAY_CG_FP32 0u  /* reference: f32 throughout */
 #define NATRAY_CG_MIXED 1u /* f32 SpMV/axpy, f64 dots, norms, convergence and true-residual refresh */
+/* Pipelined preconditioned CG (Ghysels and Vanroose, Parallel Computing 40, 2014) with the
+ * precision of NATRAY_CG_MIXED: one reduction per iteration instead of two, and residual
+ * replacement (Cools et al., SIAM J. Sci. Comput. 40, 2018) every refresh_every iterations.
+ * Not in the reference; the same system and tolerance test as the other two. */
+#define NATRAY_CG_PIPELINED 2u
 
 typedef struct natray_scene {
   uint32_t flags;         /* NATRAY_SCENE_* */
   uint32_t precond;       /* NATRAY_PRECOND_* */
   uint32_t precision;     /* NATRAY_CG_* */
-  uint32_t refresh_every; /* mixed precision: recompute r = b - A p in f64 every this many iterations (0 = never) */
+  uint32_t refresh_every; /* mixed and pipelined: recompute r = b - A p in f64 every this many iterations (0 = never) */
   float fixed_dt;         /* > 0: every step is this long (s), instead of the CFL choice */
   float mask_p;           /* gauge pressure (Pa) of COVERED mask cells */
   uint32_t solve_groups;  /* workgroups of the persistent pressure solve (0 = chosen from the block count) */

@@ -33,7 +33,7 @@ RHO_WATER = 1000.0
 SCENE_GROUND, SCENE_KNOT, SCENE_WALLS, SCENE_CAPSULE, SCENE_MASK, SCENE_AIR = 1, 2, 4, 8, 16, 32
 SCENE_REFERENCE = SCENE_GROUND | SCENE_KNOT | SCENE_AIR
 PRECOND_BLOCK_SGS, PRECOND_JACOBI = 0, 1
-CG_FP32, CG_MIXED = 0, 1
+CG_FP32, CG_MIXED, CG_PIPELINED = 0, 1, 2
 
 OK, E_OVERFLOW, E_NOMEM, E_HIP, E_ARG, E_SOLVER = 0, -1, -2, -3, -4, -5
 

@@ -57,7 +57,10 @@ struct Dev {
                            // hands out, [10] warm-start tiles this step, [11] 1 when the
                            // solve starts from the warm start's guess (reset by k_begin)
   unsigned long long* rwords;  // RED_WORDS: tagged chunk sums of the solve's reductions
-  unsigned long long* hz;      // capc: tagged z of each water cell (register form of the solve)
+  unsigned long long* hz;      // 2 x capc: tagged halo words of each water cell (register forms of
+                               // the solve; the pipelined form alternates between the two halves)
+  float* pcg;                  // 4 x capc: z, q and two m buffers of the pipelined solve's memory
+                               // form (NATRAY_CG_PIPELINED only; else null)
   float* vmax_part;        // fastest particle of each g2p workgroup (<= cap workgroups)
   double* solver;          // double copies of rz, alpha, beta, b_norm, r_norm
   unsigned long long* stats;  // counters, see natray_sim_counters
@@ -146,8 +149,9 @@ int launch_begin(const Target& t, void* stream);
 int launch_bin(const Target& t, void* stream, bool rubber = false, bool air = false);
 int launch_p2g(const Target& t, void* stream);
 int launch_grid_prep(const Target& t, void* stream);
-int launch_solve(const Target& t, void* stream, int mixed);
-int solve_max_grid(int mixed, int* per_cu);
+// The solve of precision `kind` (NATRAY_CG_FP32, _MIXED or _PIPELINED).
+int launch_solve(const Target& t, void* stream, int kind);
+int solve_max_grid(int kind, int* per_cu);
 int launch_project(const Target& t, void* stream);
 int launch_extend(const Target& t, void* stream);
 int launch_g2p(const Target& t, void* stream);

@@ -818,6 +818,9 @@ __global__ __launch_bounds__(256) void k_grid_prep(L l) {
 // with agent-scope release and acquire fences around each reduction. Both forms evaluate the
 // same expressions in the same order and give the same bits.
 
+// The pipelined solve replaces its residual only while |r| is above this many times the
+// tolerance test's bound.
+constexpr double PIPE_REPLACE = 1000.0;
 // Polls of a solve wait before a wave gives up and raises the abort flag (Dev::bar[2]).
 constexpr uint32_t kSpinLimit = 1u << 22;
 // Solve iterations one launch may run, whatever limits.z asks: bounds the tags a launch uses.
@@ -957,18 +960,18 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
   uint32_t* abort = d.bar + 2;
   if (t == 0) s_fail = 0u;
   wg_sync(!sp.in_regs);
-  // total()'s chunk fold, then publish: one word per chunk, component and half.
-  if (t < sp.c1 - sp.c0) {
-    const uint32_t c = sp.c0 + t, lo = c * sp.K, hi = min(sp.n, lo + sp.K);
+  // total()'s chunk fold, then publish: one word per chunk, component and half. One thread
+  // per chunk and component, each summing its component over the chunk's blocks in order.
+  if (t < (sp.c1 - sp.c0) * (uint32_t)nw) {
+    const uint32_t k = t % (uint32_t)nw, c = sp.c0 + t / (uint32_t)nw, lo = c * sp.K, hi = min(sp.n, lo + sp.K);
     if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    for (int k = 0; k < nw; ++k) {
-      R acc = (R)0;
-      if (k < nc)
-        for (uint32_t b = lo; b < hi; ++b) acc += (R)(sp.in_regs ? lpart[b - sp.b0][k] : d.part[b * 4u + (uint32_t)k]);
-#pragma unroll
-      for (int h = 0; h < H; ++h)
-        store_word(red_word(d, c, buf, (uint32_t)(k * 2 + h)), tag_word(tag, Halves<R>::part(acc, h)));
+    R acc = (R)0;
+    if ((int)k < nc) {
+#pragma unroll 8
+      for (uint32_t b = lo; b < hi; ++b) acc += (R)(sp.in_regs ? lpart[b - sp.b0][k] : d.part[b * 4u + k]);
     }
+#pragma unroll
+    for (int h = 0; h < H; ++h) store_word(red_word(d, c, buf, k * 2u + (uint32_t)h), tag_word(tag, Halves<R>::part(acc, h)));
   }
   // Every chunk's words, summed as total() does (fluid.wgsl:426-439): thread t holds chunk t,
   // then the tree over the 256 threads.
@@ -984,17 +987,23 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
       pay[s] = 0u;
       if (poller && (s & 1u) < (uint32_t)H) have &= ~(1u << s);
     }
+    // A chunk's words are stored together (the last slot's with the last component), so until
+    // the last slot's word has come, only that one is polled: every workgroup polls every
+    // chunk, and polling all of a chunk's words each round multiplies the uncached loads that
+    // compete with the stores being waited for. Each word's own tag still decides whether it
+    // has come.
+    const uint32_t last = (uint32_t)((nw - 1) * 2 + (H - 1));
     while (true) {
       // Issue every load of the round before looking at any, so one round trip serves both
       // the chunk words and the waves' own waits.
+      const uint32_t want = ((have >> last) & 1u) ? full & ~have : full & ~have & (1u << last);
       unsigned long long w[RED_SLOTS];
 #pragma unroll
-      for (uint32_t s = 0; s < RED_SLOTS; ++s)
-        w[s] = (((full >> s) & 1u) && !((have >> s) & 1u)) ? load_word(red_word(d, t, buf, s)) : 0ull;
+      for (uint32_t s = 0; s < RED_SLOTS; ++s) w[s] = ((want >> s) & 1u) ? load_word(red_word(d, t, buf, s)) : 0ull;
       const bool waited = wait(tag);
 #pragma unroll
       for (uint32_t s = 0; s < RED_SLOTS; ++s) {
-        if (!((full >> s) & 1u) || ((have >> s) & 1u)) continue;
+        if (!((want >> s) & 1u)) continue;
         if ((uint32_t)(w[s] >> 32) == tag) {
           pay[s] = (uint32_t)w[s];
           have |= 1u << s;
@@ -1096,11 +1105,12 @@ __device__ __forceinline__ Row cell_row(const Dev& d, uint32_t cell, int lx, int
 }
 
 // solve_init (fluid.wgsl:450-491) for one cell from p0: the operator row, b, r = b - A p0 and
-// z. Writes diag, r, b, the neighbour mask and z; in register form also the tagged z (tag
-// `ztag`). Stores the block partials (r.z, r.r, b.b, water cells) through `prow` from lane 0.
+// z. Writes diag, r, b, the neighbour mask and z; in register form also z as a tagged word
+// (tag `ztag`) into zw. Stores the block partials (r.z, r.r, b.b, water cells) through `prow`
+// from lane 0.
 template <class R>
 __device__ __forceinline__ void init_cell(const Dev& d, uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk,
-                                          bool regs, uint32_t ztag, double* prow, const float* p0) {
+                                          unsigned long long* zw, uint32_t ztag, double* prow, const float* p0) {
   const uint32_t capc = d.capc;
   const uint32_t cell = b * 64u + lane;
   const Row w = cell_row(d, cell, lx, ly, lz, lk, p0);
@@ -1114,7 +1124,7 @@ __device__ __forceinline__ void init_cell(const Dev& d, uint32_t b, uint32_t lan
   const float z = precondition(r, w.diag, w.water, lx, ly, lz, d.sc->precond);
   if (w.water) {
     d.cf[(size_t)NATRAY_F_Z * capc + cell] = z;
-    if (regs) store_word(&d.hz[cell], tag_word(ztag, __float_as_uint(z)));
+    if (zw) store_word(&zw[cell], tag_word(ztag, __float_as_uint(z)));
   }
   const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r)), v2 = wave_tree<R>(prod<R>(bv, bv));
   const uint32_t nw = (uint32_t)__popcll(__ballot(w.water));
@@ -1218,7 +1228,7 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
   float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
   const float* const p0 = start_pressure(d);
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    init_cell<R>(d, b, lane, lx, ly, lz, lk, false, 0u, d.part + b * 4u, p0);
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, nullptr, 0u, d.part + b * 4u, p0);
   });
   R tt[4];
   SolveScalars<R> s;
@@ -1318,6 +1328,23 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
   solve_finish<R>(d, sp, tag, s);
 }
 
+// This workgroup's chunks and blocks in its scene's solve, and whether every workgroup keeps
+// its blocks in registers.
+__device__ __forceinline__ SolveSpan solve_span(const View& v) {
+  SolveSpan sp;
+  sp.n = n_blocks(v.d);
+  sp.K = sp.n ? (sp.n + MAX_CHUNKS - 1u) / MAX_CHUNKS : 1u;
+  sp.m = sp.n ? (sp.n + sp.K - 1u) / sp.K : 0u;
+  sp.G = v.wgs;
+  sp.wg = v.wg;
+  sp.c0 = (uint32_t)((uint64_t)v.wg * sp.m / sp.G);
+  sp.c1 = (uint32_t)((uint64_t)(v.wg + 1) * sp.m / sp.G);
+  sp.b0 = sp.c0 * sp.K;
+  sp.b1 = min(sp.n, sp.c1 * sp.K);
+  sp.in_regs = (sp.m + sp.G - 1u) / sp.G * sp.K <= SOLVE_WAVES * SOLVE_RMAX;
+  return sp;
+}
+
 template <bool MIXED, class L>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
   const View v = view(l);
@@ -1327,17 +1354,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
   if (d.st->dt_ns == 0u) return;
   // An earlier solve of this batch lost its barrier: the host reports it, nothing more runs.
   if (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-  SolveSpan sp;
-  sp.n = n_blocks(d);
-  sp.K = sp.n ? (sp.n + MAX_CHUNKS - 1u) / MAX_CHUNKS : 1u;
-  sp.m = sp.n ? (sp.n + sp.K - 1u) / sp.K : 0u;
-  sp.G = v.wgs;
-  sp.wg = v.wg;
-  sp.c0 = (uint32_t)((uint64_t)v.wg * sp.m / sp.G);
-  sp.c1 = (uint32_t)((uint64_t)(v.wg + 1) * sp.m / sp.G);
-  sp.b0 = sp.c0 * sp.K;
-  sp.b1 = min(sp.n, sp.c1 * sp.K);
-  sp.in_regs = (sp.m + sp.G - 1u) / sp.G * sp.K <= SOLVE_WAVES * RM;
+  const SolveSpan sp = solve_span(v);
   uint32_t tag = __hip_atomic_load(d.bar + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!sp.in_regs) {
     // Called with the device copy of Dev, which the callee reads with scalar loads.
@@ -1357,7 +1374,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
   unsigned long long* const hz = d.hz;
   __shared__ double lpart[SOLVE_WAVES * SOLVE_RMAX][4];
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    init_cell<R>(d, b, lane, lx, ly, lz, lk, true, tag + 1u, lpart[b - b0], p0);
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, d.hz, tag + 1u, lpart[b - b0], p0);
   });
   R tt[4];
   SolveScalars<R> s;
@@ -1546,6 +1563,499 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
     }
   }
   solve_finish<R>(d, sp, tag, s);
+}
+
+// ---- pipelined CG (NATRAY_CG_PIPELINED; not in the reference) ---------------------------------------
+//
+// Ghysels and Vanroose's pipelined preconditioned CG (Parallel Computing 40, 2014, Algorithm 3):
+// the Krylov iterates of the solve above in exact arithmetic, with ONE reduction per iteration,
+// of (r, u), (w, u) and (r, r), where u = M^-1 r and w = A u are carried by recurrences. Per
+// iteration: m = M^-1 w, n = A m, then
+//   beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+//   z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p,
+//   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z
+// with gamma = (r, u), delta = (w, u). Every refresh_every iterations the residual is replaced
+// (Cools et al., SIAM J. Sci. Comput. 40, 2018): r = b - A x in double, u = M^-1 r, w = A u,
+// s = A p, q = M^-1 s, z = A q. Dots, norms and the tolerance test are in double, vectors in
+// f32. The iteration stops when (r, r) meets the tolerance (tested from the second reduction
+// on; the first one's (r, r) is the start's) or after limits.z updates. When the solve ends,
+// x is in P, r in R, u in Z, s in Q, p in D and w in D + 1.
+//
+// A replacement brings the recurrences' r, u, w, s, q and z back to what x and p give, which
+// upsets the recurrences when r has already fallen to the rounding gap between the recurred
+// and the true residual: so a replacement is made only while |r| > PIPE_REPLACE tol max(|b|, 1)
+// (the tolerance test's right-hand side times PIPE_REPLACE), where the gap is still small
+// against r; the gap that builds up afterwards starts from residuals that small.
+//
+// Register form: m of the water cells on block faces is published as tagged words and read by
+// the neighbours while the iteration's reduction completes, so the halo exchange costs no wait
+// of its own. A replacement exchanges x, u, p and q in turn. Halo words are numbered 8 t + k
+// (t the iteration's reduction tag; k = 0 for m, 1..4 for x, u, p, q of a replacement; u0 at
+// the start has 8 t + 5 with t the start's reduction) and alternate between two word buffers:
+// m of reduction t and u and q of its replacement in buffer t & 1, x and p in the other, u0
+// in the start's. A wave rewrites a buffer only after a reduction or an exchange that every
+// neighbour joins after it has read the buffer's last words.
+
+// The pipelined iteration's scalars from the reduction's (r, u), (w, u), (r, r); returns
+// whether to update (false once (r, r) meets the tolerance or limits.z updates ran).
+__device__ __forceinline__ bool pipe_scalars(const Dev& d, const double* tt, SolveScalars<double>& s) {
+  const double g = tt[0], dl = tt[1], rr = tt[2];
+  s.r_norm = sq(rr);
+  if (s.iters > 0u) {
+    const bool done = converged<double>(rr, (double)d.U->solve[0], s.b_norm);
+    if (done || s.iters >= min(d.U->limits[2], SOLVE_ITER_CAP)) {
+      s.iters_max = s.iters > s.iters_max ? s.iters : s.iters_max;
+      s.unconverged += done ? 0u : 1u;
+      s.rz = g;
+      return false;
+    }
+  }
+  const double beta = (s.iters > 0u && s.rz > 0.0) ? g / s.rz : 0.0;
+  const double den = (s.iters > 0u && s.alpha != 0.0) ? dl - beta * g / s.alpha : dl;
+  s.alpha = den > 0.0 ? g / den : 0.0;
+  s.beta = beta;
+  s.rz = g;
+  return true;
+}
+
+// (A v) at this lane's cell in register form: diag v minus the water neighbours' v in the order
+// -x, +x, -y, +y, -z, +z, those across a block face from h (one per axis). Every lane of the
+// wave calls it (block_faces is a wave operation); the result is meaningful on water cells.
+__device__ __forceinline__ float apply_row(float v, float diag, uint32_t mask, const float h[3], int lx, int ly, int lz) {
+  const Faces f = block_faces(v);
+  float q = diag * v;
+  if (mask & 1u) q -= lx > 0 ? f.xm : h[0];
+  if (mask & 2u) q -= lx < 3 ? f.xp : h[0];
+  if (mask & 4u) q -= ly > 0 ? f.ym : h[1];
+  if (mask & 8u) q -= ly < 3 ? f.yp : h[1];
+  if (mask & 16u) q -= lz > 0 ? f.zm : h[2];
+  if (mask & 32u) q -= lz < 3 ? f.zp : h[2];
+  return q;
+}
+
+// The same row on a field in memory.
+__device__ __forceinline__ float mem_row(const float* v, float diag, uint32_t mask, uint32_t cell, const uint32_t* lk, int lx,
+                                         int ly, int lz) {
+  float q = diag * v[cell];
+  if (mask & 1u) q -= v[nbr(lk, lx, ly, lz, -1, 0, 0)];
+  if (mask & 2u) q -= v[nbr(lk, lx, ly, lz, 1, 0, 0)];
+  if (mask & 4u) q -= v[nbr(lk, lx, ly, lz, 0, -1, 0)];
+  if (mask & 8u) q -= v[nbr(lk, lx, ly, lz, 0, 1, 0)];
+  if (mask & 16u) q -= v[nbr(lk, lx, ly, lz, 0, 0, -1)];
+  if (mask & 32u) q -= v[nbr(lk, lx, ly, lz, 0, 0, 1)];
+  return q;
+}
+
+// Waits until the word at p carries `want` (w holds the last load). Gives up like the
+// reductions' waits: returns false, with the abort flag raised, after kSpinLimit polls or
+// once another wave has raised it.
+__device__ __forceinline__ bool wait_word(const Dev& d, unsigned long long& w, const unsigned long long* p, uint32_t want) {
+  uint32_t spins = 0;
+  while ((uint32_t)(w >> 32) != want) {
+    __builtin_amdgcn_s_sleep(1);
+    w = load_word(p);
+    if ((++spins & 255u) == 0u &&
+        (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
+      __hip_atomic_store(d.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+  }
+  return true;
+}
+
+// The block tail of the memory form: m = M^-1 w into mo and the block partials (r, u), (w, u),
+// (r, r) of the values this lane holds (0 off water).
+__device__ __forceinline__ void pipe_tail_memory(const Dev& d, uint32_t b, uint32_t lane, int lx, int ly, int lz, float r,
+                                                 float u, float w, float diag, bool water, float* mo) {
+  const float m = precondition(w, diag, water, lx, ly, lz, d.sc->precond);
+  if (water) mo[b * 64u + lane] = m;
+  const double v0 = wave_tree<double>(prod<double>(r, u)), v1 = wave_tree<double>(prod<double>(w, u)),
+               v2 = wave_tree<double>(prod<double>(r, r));
+  if (lane == 0) {
+    d.part[b * 4u + 0] = v0;
+    d.part[b * 4u + 1] = v1;
+    d.part[b * 4u + 2] = v2;
+  }
+}
+
+// The memory form of the pipelined solve: every field in memory (z, q and two m buffers in
+// Dev::pcg), one reduction per iteration and three barriers more per replacement.
+__device__ __noinline__ void solve_pipe_memory(const Dev* dp, SolveSpan sp, uint32_t tag) {
+  const Dev& d = *dp;
+  const uint32_t mode = d.sc->precond;
+  const uint32_t b0 = sp.b0, b1 = sp.b1;
+  const uint32_t capc = d.capc;
+  float* const cX = d.cf + (size_t)NATRAY_F_P * capc;
+  float* const cR = d.cf + (size_t)NATRAY_F_R * capc;
+  float* const cU = d.cf + (size_t)NATRAY_F_Z * capc;
+  float* const cS = d.cf + (size_t)NATRAY_F_Q * capc;
+  float* const cP = d.cf + (size_t)NATRAY_F_D * capc;
+  float* const cW = d.cf + (size_t)(NATRAY_F_D + 1u) * capc;
+  const float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
+  float* const cZ = d.pcg;
+  float* const cQ = d.pcg + capc;
+  float* const cM[2] = {d.pcg + 2u * capc, d.pcg + 3u * capc};
+  const float* const p0 = start_pressure(d);
+  for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    init_cell<double>(d, b, lane, lx, ly, lz, lk, nullptr, 0u, d.part + b * 4u, p0);
+  });
+  double tt[4];
+  SolveScalars<double> s;
+  bool go = solve_start<double>(d, tt, reduce<double>(d, sp, ++tag, 4, nullptr, tt), s);
+  if (p0 != cX) {
+    // The iterate starts at the guess; every workgroup has read the carried pressure by now.
+    for (uint32_t c = b0 * 64u + threadIdx.x; c < b1 * 64u; c += blockDim.x)
+      if (d.nbm[c] & 64u) cX[c] = p0[c];
+    __syncthreads();
+  }
+  const bool refresh_on = d.sc->refresh_every > 0u;
+  const double replace_above = PIPE_REPLACE * (double)d.U->solve[0] * (s.b_norm < 1.0 ? 1.0 : s.b_norm);
+  if (go) {
+    // w = A u; z, q, s and p start at zero; m = M^-1 w and the first partials.
+    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+      const uint32_t cell = b * 64u + lane, mask = d.nbm[cell];
+      const bool water = (mask & 64u) != 0u;
+      float r = 0.0f, u = 0.0f, w = 0.0f, diag = 1.0f;
+      if (water) {
+        diag = cDiag[cell];
+        r = cR[cell];
+        u = cU[cell];
+        w = mem_row(cU, diag, mask, cell, lk, lx, ly, lz);
+        cW[cell] = w;
+        cZ[cell] = cQ[cell] = cS[cell] = cP[cell] = 0.0f;
+      }
+      pipe_tail_memory(d, b, lane, lx, ly, lz, r, u, w, diag, water, cM[0]);
+    });
+  }
+  while (go) {
+    if (!reduce<double>(d, sp, ++tag, 3, nullptr, tt)) break;
+    if (!pipe_scalars(d, tt, s)) break;
+    const float fa = (float)s.alpha, fb = (float)s.beta;
+    const float* const mi = cM[s.iters & 1u];
+    float* const mo = cM[(s.iters + 1u) & 1u];
+    const bool refresh = refresh_on && (s.iters + 1u) % d.sc->refresh_every == 0u && s.r_norm > replace_above;
+    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+      const uint32_t cell = b * 64u + lane, mask = d.nbm[cell];
+      const bool water = (mask & 64u) != 0u;
+      float r = 0.0f, u = 0.0f, w = 0.0f, diag = 1.0f;
+      if (water) {
+        diag = cDiag[cell];
+        const float n = mem_row(mi, diag, mask, cell, lk, lx, ly, lz);
+        const float z = n + fb * cZ[cell];
+        const float q = mi[cell] + fb * cQ[cell];
+        const float sv = cW[cell] + fb * cS[cell];
+        const float p = cU[cell] + fb * cP[cell];
+        cX[cell] = cX[cell] + fa * p;
+        r = cR[cell] - fa * sv;
+        u = cU[cell] - fa * q;
+        w = cW[cell] - fa * z;
+        cZ[cell] = z;
+        cQ[cell] = q;
+        cS[cell] = sv;
+        cP[cell] = p;
+        cR[cell] = r;
+        cU[cell] = u;
+        cW[cell] = w;
+      }
+      if (!refresh) pipe_tail_memory(d, b, lane, lx, ly, lz, r, u, w, diag, water, mo);
+    });
+    s.iters += 1u;
+    if (refresh) {
+      // r = b - A x in double, u = M^-1 r.
+      if (!reduce<double>(d, sp, ++tag, 0, nullptr, tt)) break;
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+        const uint32_t cell = b * 64u + lane, mask = d.nbm[cell];
+        const bool water = (mask & 64u) != 0u;
+        float r = 0.0f, diag = 1.0f;
+        if (water) {
+          diag = cDiag[cell];
+          double ap = (double)diag * (double)cX[cell];
+          if (mask & 1u) ap -= (double)cX[nbr(lk, lx, ly, lz, -1, 0, 0)];
+          if (mask & 2u) ap -= (double)cX[nbr(lk, lx, ly, lz, 1, 0, 0)];
+          if (mask & 4u) ap -= (double)cX[nbr(lk, lx, ly, lz, 0, -1, 0)];
+          if (mask & 8u) ap -= (double)cX[nbr(lk, lx, ly, lz, 0, 1, 0)];
+          if (mask & 16u) ap -= (double)cX[nbr(lk, lx, ly, lz, 0, 0, -1)];
+          if (mask & 32u) ap -= (double)cX[nbr(lk, lx, ly, lz, 0, 0, 1)];
+          r = (float)((double)d.bvec[cell] - ap);
+          cR[cell] = r;
+        }
+        const float u = precondition(r, diag, water, lx, ly, lz, mode);
+        if (water) cU[cell] = u;
+      });
+      // w = A u, s = A p, q = M^-1 s.
+      if (!reduce<double>(d, sp, ++tag, 0, nullptr, tt)) break;
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+        const uint32_t cell = b * 64u + lane, mask = d.nbm[cell];
+        const bool water = (mask & 64u) != 0u;
+        float sv = 0.0f, diag = 1.0f;
+        if (water) {
+          diag = cDiag[cell];
+          cW[cell] = mem_row(cU, diag, mask, cell, lk, lx, ly, lz);
+          sv = mem_row(cP, diag, mask, cell, lk, lx, ly, lz);
+          cS[cell] = sv;
+        }
+        const float q = precondition(sv, diag, water, lx, ly, lz, mode);
+        if (water) cQ[cell] = q;
+      });
+      // z = A q; m = M^-1 w and the partials.
+      if (!reduce<double>(d, sp, ++tag, 0, nullptr, tt)) break;
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+        const uint32_t cell = b * 64u + lane, mask = d.nbm[cell];
+        const bool water = (mask & 64u) != 0u;
+        float r = 0.0f, u = 0.0f, w = 0.0f, diag = 1.0f;
+        if (water) {
+          diag = cDiag[cell];
+          cZ[cell] = mem_row(cQ, diag, mask, cell, lk, lx, ly, lz);
+          r = cR[cell];
+          u = cU[cell];
+          w = cW[cell];
+        }
+        pipe_tail_memory(d, b, lane, lx, ly, lz, r, u, w, diag, water, mo);
+      });
+    }
+  }
+  solve_finish<double>(d, sp, tag, s);
+}
+
+template <class L>
+__global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
+  const View v = view(l);
+  const Dev& d = v.d;
+  constexpr uint32_t RM = SOLVE_RMAX;
+  if (d.st->dt_ns == 0u) return;
+  if (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+  const SolveSpan sp = solve_span(v);
+  uint32_t tag = __hip_atomic_load(d.bar + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!sp.in_regs) {
+    solve_pipe_memory(d.self, sp, tag);
+    return;
+  }
+
+  // Register form. Wave wv holds blocks b0 + wv + SOLVE_WAVES r (r < RM). Per block and lane:
+  // x, r, u, w, z, q, s, p, m, diag and the neighbour mask; per axis the tagged word of the
+  // halo cell across the block face (when it is water) as last loaded.
+  const uint32_t mode = d.sc->precond;
+  const uint32_t b0 = sp.b0, b1 = sp.b1;
+  const uint32_t capc = d.capc;
+  const float* const p0 = start_pressure(d);
+  unsigned long long* const hz = d.hz;
+  __shared__ double lpart[SOLVE_WAVES * SOLVE_RMAX][4];
+  const uint32_t t0 = tag + 1u;  // the start's reduction
+  unsigned long long* const hz0 = hz + (size_t)(t0 & 1u) * capc;
+  for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+    init_cell<double>(d, b, lane, lx, ly, lz, lk, hz0, 8u * t0 + 5u, lpart[b - b0], p0);
+  });
+  double tt[4];
+  SolveScalars<double> s;
+  bool go = solve_start<double>(d, tt, reduce<double>(d, sp, ++tag, 4, lpart, tt), s);
+  const bool refresh_on = d.sc->refresh_every > 0u;
+  const double replace_above = PIPE_REPLACE * (double)d.U->solve[0] * (s.b_norm < 1.0 ? 1.0 : s.b_norm);
+
+  __shared__ uint32_t lkr[SOLVE_WAVES][SOLVE_RMAX][27];
+  const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
+  const int lx = (int)(ln >> 4), ly = (int)((ln >> 2) & 3u), lz = (int)(ln & 3u);
+  const uint32_t face_bit[3] = {lx == 0 ? 1u : (lx == 3 ? 2u : 0u), ly == 0 ? 4u : (ly == 3 ? 8u : 0u),
+                                lz == 0 ? 16u : (lz == 3 ? 32u : 0u)};
+  float rX[RM], rR[RM], rU[RM], rW[RM], rZ[RM], rQ[RM], rS[RM], rP[RM], rMv[RM], rDg[RM];
+  uint32_t rM[RM];
+  unsigned long long hW[RM][3];
+  auto halo_cell = [&](uint32_t r, int a) -> uint32_t {
+    const int sg = (face_bit[a] & 0x2Au) ? 1 : -1;
+    return nbr(lkr[wv][r], lx, ly, lz, a == 0 ? sg : 0, a == 1 ? sg : 0, a == 2 ? sg : 0);
+  };
+  auto has_halo = [&](uint32_t r, int a) -> bool { return (rM[r] & face_bit[a]) != 0u; };
+#pragma unroll
+  for (uint32_t r = 0; r < RM; ++r) {
+    const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+    if (b < b1 && ln < 27) lkr[wv][r][ln] = d.block_nb[b * 27u + ln];
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < RM; ++r) {
+    rX[r] = rR[r] = rU[r] = rW[r] = rZ[r] = rQ[r] = rS[r] = rP[r] = rMv[r] = 0.0f;
+    rDg[r] = 1.0f;
+    rM[r] = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) hW[r][a] = 0ull;
+    const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+    if (b >= b1) continue;
+    const uint32_t cell = b * 64u + ln;
+    rM[r] = d.nbm[cell];
+    if (rM[r] & 64u) {
+      rX[r] = p0[cell];
+      rR[r] = d.cf[(size_t)NATRAY_F_R * capc + cell];
+      rU[r] = d.cf[(size_t)NATRAY_F_Z * capc + cell];
+      rDg[r] = d.cf[(size_t)NATRAY_F_DIAG * capc + cell];
+    }
+  }
+  bool ok = true;
+  // The halo of word buffer `buf` with tag `want` for block r: waits for each halo word and
+  // returns the payloads (0 where there is no halo cell).
+  auto halo = [&](uint32_t r, const unsigned long long* buf, uint32_t want, float h[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      h[a] = 0.0f;
+      if (!has_halo(r, a)) continue;
+      ok = wait_word(d, hW[r][a], &buf[halo_cell(r, a)], want) && ok;
+      h[a] = __uint_as_float((uint32_t)hW[r][a]);
+    }
+  };
+  // Publishes vals of this wave's water cells into buffer `buf` with tag `tg`.
+  auto publish = [&](const float* vals, unsigned long long* buf, uint32_t tg) {
+#pragma unroll
+    for (uint32_t r = 0; r < RM; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b < b1 && (rM[r] & 64u)) store_word(&buf[b * 64u + ln], tag_word(tg, __float_as_uint(vals[r])));
+    }
+  };
+  if (go) {
+    // w = A u from u0 and its halo (published by init_cell with the start's reduction).
+#pragma unroll
+    for (uint32_t r = 0; r < RM; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b >= b1) continue;
+      float h[3];
+      halo(r, hz0, 8u * t0 + 5u, h);
+      const float w = apply_row(rU[r], rDg[r], rM[r], h, lx, ly, lz);
+      if (rM[r] & 64u) rW[r] = w;
+    }
+  }
+  while (go) {
+    const uint32_t T = tag + 1u;  // this iteration's reduction
+    unsigned long long* const hzT = hz + (size_t)(T & 1u) * capc;
+    // m = M^-1 w, published for the neighbours, and the block partials.
+#pragma unroll
+    for (uint32_t r = 0; r < RM; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b >= b1) continue;
+      const bool water = (rM[r] & 64u) != 0u;
+      const float m = precondition(rW[r], rDg[r], water, lx, ly, lz, mode);
+      rMv[r] = water ? m : 0.0f;
+      const double v0 = wave_tree<double>(prod<double>(rR[r], rU[r])), v1 = wave_tree<double>(prod<double>(rW[r], rU[r])),
+                   v2 = wave_tree<double>(prod<double>(rR[r], rR[r]));
+      if (ln == 0) {
+        lpart[b - b0][0] = v0;
+        lpart[b - b0][1] = v1;
+        lpart[b - b0][2] = v2;
+      }
+    }
+    publish(rMv, hzT, 8u * T);
+    // The halo m words, polled while the reduction completes.
+    auto halo_poll = [&](uint32_t) -> bool {
+      bool fresh = true;
+#pragma unroll
+      for (uint32_t r = 0; r < RM; ++r) {
+        if (b0 + wv + SOLVE_WAVES * r >= b1) continue;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (!has_halo(r, a) || (uint32_t)(hW[r][a] >> 32) == 8u * T) continue;
+          hW[r][a] = load_word(&hzT[halo_cell(r, a)]);
+          fresh = fresh && (uint32_t)(hW[r][a] >> 32) == 8u * T;
+        }
+      }
+      return fresh;
+    };
+    if (!reduce<double>(d, sp, ++tag, 3, lpart, tt, halo_poll) || !ok) break;
+    if (!pipe_scalars(d, tt, s)) break;
+    const float fa = (float)s.alpha, fb = (float)s.beta;
+    const bool refresh = refresh_on && (s.iters + 1u) % d.sc->refresh_every == 0u && s.r_norm > replace_above;
+#pragma unroll
+    for (uint32_t r = 0; r < RM; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b >= b1) continue;
+      float hm[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) hm[a] = has_halo(r, a) ? __uint_as_float((uint32_t)hW[r][a]) : 0.0f;
+      const float n = apply_row(rMv[r], rDg[r], rM[r], hm, lx, ly, lz);
+      if (rM[r] & 64u) {
+        rZ[r] = n + fb * rZ[r];
+        rQ[r] = rMv[r] + fb * rQ[r];
+        rS[r] = rW[r] + fb * rS[r];
+        rP[r] = rU[r] + fb * rP[r];
+        rX[r] = rX[r] + fa * rP[r];
+        rR[r] = rR[r] - fa * rS[r];
+        rU[r] = rU[r] - fa * rQ[r];
+        rW[r] = rW[r] - fa * rZ[r];
+      }
+    }
+    s.iters += 1u;
+    if (refresh) {
+      // Residual replacement through four halo exchanges: x (r = b - A x in double, u = M^-1 r),
+      // u (w = A u), p (s = A p, q = M^-1 s), q (z = A q); x and p in the buffer m of this
+      // iteration is not in, u and q in the other.
+      unsigned long long* const hzA = hz + (size_t)((T + 1u) & 1u) * capc;
+      publish(rX, hzA, 8u * T + 1u);
+#pragma unroll
+      for (uint32_t r = 0; r < RM; ++r) {
+        const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+        if (b >= b1) continue;
+        const uint32_t cell = b * 64u + ln, mask = rM[r];
+        const bool water = (mask & 64u) != 0u;
+        float h[3];
+        halo(r, hzA, 8u * T + 1u, h);
+        const Faces f = block_faces(rX[r]);
+        if (water) {
+          double ap = (double)rDg[r] * (double)rX[r];
+          if (mask & 1u) ap -= (double)(lx > 0 ? f.xm : h[0]);
+          if (mask & 2u) ap -= (double)(lx < 3 ? f.xp : h[0]);
+          if (mask & 4u) ap -= (double)(ly > 0 ? f.ym : h[1]);
+          if (mask & 8u) ap -= (double)(ly < 3 ? f.yp : h[1]);
+          if (mask & 16u) ap -= (double)(lz > 0 ? f.zm : h[2]);
+          if (mask & 32u) ap -= (double)(lz < 3 ? f.zp : h[2]);
+          rR[r] = (float)((double)d.bvec[cell] - ap);
+        }
+        const float u = precondition(rR[r], rDg[r], water, lx, ly, lz, mode);
+        if (water) rU[r] = u;
+      }
+      publish(rU, hzT, 8u * T + 2u);
+#pragma unroll
+      for (uint32_t r = 0; r < RM; ++r) {
+        if (b0 + wv + SOLVE_WAVES * r >= b1) continue;
+        float h[3];
+        halo(r, hzT, 8u * T + 2u, h);
+        const float w = apply_row(rU[r], rDg[r], rM[r], h, lx, ly, lz);
+        if (rM[r] & 64u) rW[r] = w;
+      }
+      publish(rP, hzA, 8u * T + 3u);
+#pragma unroll
+      for (uint32_t r = 0; r < RM; ++r) {
+        if (b0 + wv + SOLVE_WAVES * r >= b1) continue;
+        const bool water = (rM[r] & 64u) != 0u;
+        float h[3];
+        halo(r, hzA, 8u * T + 3u, h);
+        const float sv = apply_row(rP[r], rDg[r], rM[r], h, lx, ly, lz);
+        if (water) rS[r] = sv;
+        const float q = precondition(rS[r], rDg[r], water, lx, ly, lz, mode);
+        if (water) rQ[r] = q;
+      }
+      publish(rQ, hzT, 8u * T + 4u);
+#pragma unroll
+      for (uint32_t r = 0; r < RM; ++r) {
+        if (b0 + wv + SOLVE_WAVES * r >= b1) continue;
+        float h[3];
+        halo(r, hzT, 8u * T + 4u, h);
+        const float z = apply_row(rQ[r], rDg[r], rM[r], h, lx, ly, lz);
+        if (rM[r] & 64u) rZ[r] = z;
+      }
+    }
+  }
+  // The fields the other kernels and the host read.
+#pragma unroll
+  for (uint32_t r = 0; r < RM; ++r) {
+    const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+    if (b < b1 && (rM[r] & 64u)) {
+      const uint32_t cell = b * 64u + ln;
+      d.cf[(size_t)NATRAY_F_P * capc + cell] = rX[r];
+      d.cf[(size_t)NATRAY_F_R * capc + cell] = rR[r];
+      d.cf[(size_t)NATRAY_F_Z * capc + cell] = rU[r];
+      if (s.iters > 0u) {
+        d.cf[(size_t)NATRAY_F_Q * capc + cell] = rS[r];
+        d.cf[(size_t)NATRAY_F_D * capc + cell] = rP[r];
+        d.cf[(size_t)(NATRAY_F_D + 1u) * capc + cell] = rW[r];
+      }
+    }
+  }
+  solve_finish<double>(d, sp, tag, s);
 }
 
 // ---- learned warm start of the solve ---------------------------------------------------------------
@@ -2405,32 +2915,36 @@ int launch_grid_prep(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_grid_prep, LK_WAVES, 256, t, S(stream));
   return (int)hipGetLastError();
 }
-int solve_max_grid(int mixed, int* per_cu) {
-  int occ = 0;
-  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<true, One>, SOLVE_THREADS, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<false, One>, SOLVE_THREADS, 0);
-  int occ_many = 0;
-  if (e == hipSuccess)
-    e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_many, k_solve<true, Many>, SOLVE_THREADS, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_many, k_solve<false, Many>, SOLVE_THREADS, 0);
+int solve_max_grid(int kind, int* per_cu) {
+  int occ = 0, occ_many = 0;
+  hipError_t e;
+  if (kind == (int)NATRAY_CG_PIPELINED) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve_pipe<One>, SOLVE_THREADS, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_many, k_solve_pipe<Many>, SOLVE_THREADS, 0);
+  } else if (kind == (int)NATRAY_CG_MIXED) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<true, One>, SOLVE_THREADS, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_many, k_solve<true, Many>, SOLVE_THREADS, 0);
+  } else {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_solve<false, One>, SOLVE_THREADS, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_many, k_solve<false, Many>, SOLVE_THREADS, 0);
+  }
   *per_cu = occ < occ_many ? occ : occ_many;
   return (int)e;
 }
-int launch_solve(const Target& t, void* stream, int mixed) {
-  hipError_t e;
+int launch_solve(const Target& t, void* stream, int kind) {
   const dim3 grid(t.grid[LK_SOLVE]), block(SOLVE_THREADS);
-  if (t.d) {
-    One one{*t.d};
-    void* args[] = {&one};
-    e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true, One>, grid, block, args, 0, S(stream))
-              : hipLaunchCooperativeKernel((const void*)k_solve<false, One>, grid, block, args, 0, S(stream));
-  } else {
-    Many many{t.scenes, t.map[LK_SOLVE]};
-    void* args[] = {&many};
-    e = mixed ? hipLaunchCooperativeKernel((const void*)k_solve<true, Many>, grid, block, args, 0, S(stream))
-              : hipLaunchCooperativeKernel((const void*)k_solve<false, Many>, grid, block, args, 0, S(stream));
-  }
-  return (int)e;
+  One one{t.d ? *t.d : Dev{}};
+  Many many{t.scenes, t.map[LK_SOLVE]};
+  void* args_one[] = {&one};
+  void* args_many[] = {&many};
+  const void* fn;
+  if (kind == (int)NATRAY_CG_PIPELINED)
+    fn = t.d ? (const void*)k_solve_pipe<One> : (const void*)k_solve_pipe<Many>;
+  else if (kind == (int)NATRAY_CG_MIXED)
+    fn = t.d ? (const void*)k_solve<true, One> : (const void*)k_solve<true, Many>;
+  else
+    fn = t.d ? (const void*)k_solve<false, One> : (const void*)k_solve<false, Many>;
+  return (int)hipLaunchCooperativeKernel(fn, grid, block, t.d ? args_one : args_many, 0, S(stream));
 }
 int launch_project(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_project, LK_WAVES, 256, t, S(stream));

@@ -55,7 +55,7 @@ struct natray_sim {
   int grid_cells = 0, grid_blocks = 0, grid_waves = 0, solve_grid = 0;
   natray::Target tgt;  // this scene as the kernels' launch target
   uint32_t* packed = nullptr;  // 2 N words: the keyframe words of the water (natray_sim_pack_water)
-  int mixed = 0;
+  int mixed = 0;  // the CG precision (NATRAY_CG_*)
   // Upper bound of the reduction tags the solves enqueued since the tagged words were last
   // cleared (see clear_tags).
   uint64_t tags_used = 0;
@@ -145,16 +145,19 @@ int solve_grid_for(const natray_sim* s) {
 // Zeroes every tagged word of the solve and its tag counter, so tags start again from 1.
 int clear_tags(natray_sim* s) {
   HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, s->stream));
-  HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 8, s->stream));
+  HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 16, s->stream));
   HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, s->stream));
   s->tags_used = 0;
   return NATRAY_OK;
 }
 
 // A solve launch uses at most one tag per reduction: one before the iterations and up to
-// three per iteration (the iteration cap is min(limits.z, 2^24) on the device).
+// four per iteration (the iteration cap is min(limits.z, 2^24) on the device). The pipelined
+// form numbers its halo words 8 t + k from reduction tag t, so tags stay below 2^28 (the
+// words' 32-bit tag field then never wraps).
+constexpr uint64_t kTagLimit = 1ull << 28;
 uint64_t solve_tag_bound(const natray_sim* s) {
-  return 3ull * std::min<uint64_t>(s->U.limits[2], 1ull << 24) + 4ull;
+  return 4ull * std::min<uint64_t>(s->U.limits[2], 1ull << 24) + 4ull;
 }
 
 // The warm start before a solve: the network's inputs, and in mode 2 the network and the guess
@@ -232,7 +235,7 @@ int enqueue_phase(natray_sim* s, int phase) {
     case NATRAY_PHASE_SOLVE: {
       // Tags are 32-bit: clear the tagged words well before a tag could repeat.
       const uint64_t bound = solve_tag_bound(s);
-      if (s->tags_used + bound >= (1ull << 31)) {
+      if (s->tags_used + bound >= kTagLimit) {
         int e = clear_tags(s);
         if (e) return e;
       }
@@ -363,11 +366,11 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
     std::memset(&s->sc, 0, sizeof s->sc);
     s->sc.flags = NATRAY_SCENE_REFERENCE;
   }
-  if (s->sc.precond > NATRAY_PRECOND_JACOBI || s->sc.precision > NATRAY_CG_MIXED) {
+  if (s->sc.precond > NATRAY_PRECOND_JACOBI || s->sc.precision > NATRAY_CG_PIPELINED) {
     delete s;
     return fail(NATRAY_E_ARG, "unknown preconditioner or precision");
   }
-  s->mixed = s->sc.precision == NATRAY_CG_MIXED;
+  s->mixed = (int)s->sc.precision;
   auto bail = [&](int code) {
     free_all(s);
     delete s;
@@ -429,7 +432,8 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &D.nbm, capc));
   TRY(dalloc(s, &D.part, (size_t)D.cap * 4));
   TRY(dalloc(s, &D.rwords, natray::RED_WORDS));
-  TRY(dalloc(s, &D.hz, capc));
+  TRY(dalloc(s, &D.hz, 2 * capc));
+  if (s->sc.precision == NATRAY_CG_PIPELINED) TRY(dalloc(s, &D.pcg, 4 * capc));
   TRY(dalloc(s, &D.tile_sum, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.tile_off, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.bar, 1024));
@@ -487,7 +491,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   HIPTRY(hipMemsetAsync(D.part, 0, (size_t)D.cap * 4 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.bar, 0, 1024 * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.rwords, 0, natray::RED_WORDS * 8, s->stream));
-  HIPTRY(hipMemsetAsync(D.hz, 0, capc * 8, s->stream));
+  HIPTRY(hipMemsetAsync(D.hz, 0, capc * 16, s->stream));
   HIPTRY(hipMemsetAsync(D.vmax_part, 0, (size_t)D.cap * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.solver, 0, 8 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.stats, 0, 8 * 8, s->stream));
@@ -1097,9 +1101,9 @@ int batch_phase_launch(natray_batch* b, int phase) {
     case NATRAY_PHASE_SOLVE: {
       for (natray_sim* s : b->sims) {
         const uint64_t bound = solve_tag_bound(s);
-        if (s->tags_used + bound >= (1ull << 31)) {
+        if (s->tags_used + bound >= kTagLimit) {
           HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, b->stream));
-          HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 8, b->stream));
+          HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 16, b->stream));
           HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, b->stream));
           s->tags_used = 0;
         }

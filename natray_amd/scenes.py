@@ -316,7 +316,7 @@ def balloon_impact(side: int = 256, n: int = 2_097_152, seed: int = 2, speed: fl
     sc.flags = abi.SCENE_WALLS | abi.SCENE_CAPSULE | (abi.SCENE_MASK if mask else 0)
     sc.precond = precond
     sc.precision = precision
-    sc.refresh_every = refresh_every if precision == abi.CG_MIXED else 0
+    sc.refresh_every = refresh_every if precision != abi.CG_FP32 else 0
     sc.capsule_a[:] = [a[0], a[1], a[2], radius]
     sc.capsule_b[:] = [b[0], b[1], b[2], 0.0]
     sc.capsule_v[:] = [speed * dvec[0], speed * dvec[1], speed * dvec[2], 0.0]

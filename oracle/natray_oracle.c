@@ -96,6 +96,8 @@ struct osim {
   uint64_t cg_iters;
   REAL* guess;           /* cap x 64: a starting pressure for the next solve (set_guess) */
   int use_guess;
+  /* The pipelined CG's vectors outside cf (cap x 64 each): z, q and m (see o_solve_pipelined). */
+  REAL *pz, *pq, *pm;
   /* The membrane's vertices (set_membrane): positions and velocities (4 per vertex), masses,
    * push and area (2 per vertex); mem_water (4 per vertex) is what mem_sample leaves. */
   uint32_t mem_cap;
@@ -1069,11 +1071,166 @@ static int o_solve_beta(osim* s) {
   return go;
 }
 
+/* ---- pipelined CG (NATRAY_CG_PIPELINED; not in the reference) ---------------------------------
+ *
+ * Ghysels and Vanroose's pipelined preconditioned CG (Parallel Computing 40, 2014, Algorithm 3):
+ * the same Krylov iterates as o_solve_q .. o_solve_beta in exact arithmetic, with one
+ * reduction per iteration, of (r, u), (w, u) and (r, r), where u = M^-1 r and w = A u are
+ * carried by recurrences. Each iteration computes m = M^-1 w and n = A m, then
+ *   beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+ *   z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p,
+ *   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z,
+ * with gamma = (r, u), delta = (w, u). Every refresh_every iterations the residual is replaced
+ * (Cools et al., SIAM J. Sci. Comput. 40, 2018): r = b - A x in double, u = M^-1 r, w = A u,
+ * s = A p, q = M^-1 s, z = A q. Dots, norms and the tolerance test are in double, vectors in
+ * REAL. The fields follow the memory form of the GPU solve: x in P, r in R, u in Z, s in Q,
+ * p in D, w in D + 1; z, q and m in pz, pq, pm. The iteration stops when (r, r) meets the
+ * tolerance (tested from the second reduction on; the first is the start's) or after
+ * limits.z updates.
+ *
+ * A replacement brings r, u, w, s, q and z back to what x and p give; made once r has fallen
+ * to the rounding gap between the recurred and the true residual, it breaks the recurrences'
+ * relations (alpha's denominator above assumes them). So the residual is replaced only while
+ * |r| > PIPE_REPLACE tol max(|b|, 1): then the gap is small against r, and the gap that builds
+ * up afterwards starts from residuals that small. */
+#define PIPE_REPLACE 1000.0
+
+/* (A v) at water cell `cell`: diag v minus the water neighbours' v, in the order -x, +x, -y,
+ * +y, -z, +z (the operator of o_solve_q). */
+static REAL apply_a(const osim* s, const REAL* v, uint32_t cell) {
+  REAL q = cfv(s, NATRAY_F_DIAG, cell) * v[cell];
+  for (uint32_t d = 0; d < 3; ++d) {
+    int e[3] = {0, 0, 0};
+    e[d] = 1;
+    for (int k = -1; k <= 1; k += 2) {
+      uint32_t nbc = neighbour(s, cell, k * e[0], k * e[1], k * e[2]);
+      if (cell_type(s, nbc) == WATER) q -= v[nbc];
+    }
+  }
+  return q;
+}
+
+static inline REAL* field(osim* s, uint32_t f) { return s->cf + (size_t)f * cap_cells(s); }
+
+/* M^-1 of field `src` over block blk into dst (water cells only). */
+static void block_precondition(osim* s, uint32_t blk, const REAL* src, REAL* dst) {
+  REAL v[64], diag[64], z[64];
+  int water[64];
+  for (uint32_t li = 0; li < 64; ++li) {
+    uint32_t cell = blk * 64u + li;
+    water[li] = cfv(s, NATRAY_F_TYPE, cell) == WATER;
+    v[li] = water[li] ? src[cell] : (REAL)0;
+    diag[li] = water[li] ? cfv(s, NATRAY_F_DIAG, cell) : (REAL)1;
+  }
+  precondition(s, v, diag, water, z);
+  for (uint32_t li = 0; li < 64; ++li)
+    if (water[li]) dst[blk * 64u + li] = z[li];
+}
+
+/* m = M^-1 w and the block partials (r, u), (w, u), (r, r) of block blk. */
+static void pipe_block_m(osim* s, uint32_t blk) {
+  const REAL *R = field(s, NATRAY_F_R), *U = field(s, NATRAY_F_Z), *W = field(s, NATRAY_F_D + 1u);
+  block_precondition(s, blk, W, s->pm);
+  double v0[64], v1[64], v2[64];
+  for (uint32_t li = 0; li < 64; ++li) {
+    uint32_t cell = blk * 64u + li;
+    int water = cfv(s, NATRAY_F_TYPE, cell) == WATER;
+    REAL r = water ? R[cell] : (REAL)0, u = water ? U[cell] : (REAL)0, w = water ? W[cell] : (REAL)0;
+    v0[li] = prod(s, r, u);
+    v1[li] = prod(s, w, u);
+    v2[li] = prod(s, r, r);
+  }
+  s->part[blk * 4 + 0] = tree64(v0, 1);
+  s->part[blk * 4 + 1] = tree64(v1, 1);
+  s->part[blk * 4 + 2] = tree64(v2, 1);
+}
+
+/* Returns whether to update: false once (r, r) meets the tolerance or limits.z updates ran. */
+static int pipe_scalars(osim* s, uint32_t i, double g, double dl, double rr) {
+  s->r_norm = sqrt(rr);
+  if (i > 0) {
+    int done = converged(s, rr);
+    if (done || i >= s->U.limits[2]) {
+      s->st.iters_max = umax(s->st.iters_max, i);
+      s->st.unconverged += done ? 0u : 1u;
+      s->rz = g;
+      return 0;
+    }
+  }
+  double beta = (i > 0 && s->rz > 0.0) ? g / s->rz : 0.0;
+  double den = (i > 0 && s->alpha != 0.0) ? dl - beta * g / s->alpha : dl;
+  s->alpha = den > 0.0 ? g / den : 0.0;
+  s->beta = beta;
+  s->rz = g;
+  return 1;
+}
+
+static void o_solve_pipelined(osim* s) {
+  uint32_t nb = n_blocks_(s), nc = nb * 64u;
+  REAL *X = field(s, NATRAY_F_P), *R = field(s, NATRAY_F_R), *U = field(s, NATRAY_F_Z), *S = field(s, NATRAY_F_Q);
+  REAL *Pd = field(s, NATRAY_F_D), *W = field(s, NATRAY_F_D + 1u);
+  REAL *Z = s->pz, *Q = s->pq, *M = s->pm;
+  /* w = A u; z, q, s, p start at zero. */
+#pragma omp parallel for schedule(static)
+  for (uint32_t cell = 0; cell < nc; ++cell) {
+    if (cfv(s, NATRAY_F_TYPE, cell) != WATER) continue;
+    W[cell] = apply_a(s, U, cell);
+    Z[cell] = Q[cell] = S[cell] = Pd[cell] = 0;
+  }
+  for (uint32_t i = 0;; ++i) {
+#pragma omp parallel for schedule(static)
+    for (uint32_t blk = 0; blk < nb; ++blk) pipe_block_m(s, blk);
+    double g = total(s, nb, 0, 1), dl = total(s, nb, 1, 1), rr = total(s, nb, 2, 1);
+    s->st.solve_iters = i;
+    int go = pipe_scalars(s, i, g, dl, rr);
+    sync_state_floats(s);
+    if (!go) break;
+    REAL a = (REAL)s->alpha, b = (REAL)s->beta;
+#pragma omp parallel for schedule(static)
+    for (uint32_t cell = 0; cell < nc; ++cell) {
+      if (cfv(s, NATRAY_F_TYPE, cell) != WATER) continue;
+      REAL n = apply_a(s, M, cell);
+      Z[cell] = n + b * Z[cell];
+      Q[cell] = M[cell] + b * Q[cell];
+      S[cell] = W[cell] + b * S[cell];
+      Pd[cell] = U[cell] + b * Pd[cell];
+      X[cell] = X[cell] + a * Pd[cell];
+      R[cell] = R[cell] - a * S[cell];
+      U[cell] = U[cell] - a * Q[cell];
+      W[cell] = W[cell] - a * Z[cell];
+    }
+    s->cg_iters++;
+    if (s->refresh && (i + 1u) % s->sc.refresh_every == 0u &&
+        s->r_norm > PIPE_REPLACE * (double)s->U.solve[0] * (s->b_norm < 1.0 ? 1.0 : s->b_norm)) {
+#pragma omp parallel for schedule(static)
+      for (uint32_t cell = 0; cell < nc; ++cell)
+        if (cfv(s, NATRAY_F_TYPE, cell) == WATER) R[cell] = true_residual(s, cell);
+#pragma omp parallel for schedule(static)
+      for (uint32_t blk = 0; blk < nb; ++blk) block_precondition(s, blk, R, U);
+#pragma omp parallel for schedule(static)
+      for (uint32_t cell = 0; cell < nc; ++cell) {
+        if (cfv(s, NATRAY_F_TYPE, cell) != WATER) continue;
+        W[cell] = apply_a(s, U, cell);
+        S[cell] = apply_a(s, Pd, cell);
+      }
+#pragma omp parallel for schedule(static)
+      for (uint32_t blk = 0; blk < nb; ++blk) block_precondition(s, blk, S, Q);
+#pragma omp parallel for schedule(static)
+      for (uint32_t cell = 0; cell < nc; ++cell)
+        if (cfv(s, NATRAY_F_TYPE, cell) == WATER) Z[cell] = apply_a(s, Q, cell);
+    }
+  }
+}
+
 static void o_solve(osim* s) {
   if (s->st.dt_ns == 0u) return;
   o_solve_init(s);
   s->use_guess = 0;
   int go = o_solve_start(s) && s->U.limits[2] > 0u;
+  if (s->sc.precision == NATRAY_CG_PIPELINED) {
+    if (go) o_solve_pipelined(s);
+    return;
+  }
   while (go) {
     o_solve_q(s);
     o_solve_alpha(s);
@@ -1404,8 +1561,8 @@ osim* F(create)(const natray_params* params, const float* particles, uint32_t n,
     memset(&s->sc, 0, sizeof s->sc);
     s->sc.flags = NATRAY_SCENE_REFERENCE;
   }
-  s->mixed = IS_F64 || s->sc.precision == NATRAY_CG_MIXED;
-  s->refresh = s->sc.precision == NATRAY_CG_MIXED && s->sc.refresh_every > 0u;
+  s->mixed = IS_F64 || s->sc.precision == NATRAY_CG_MIXED || s->sc.precision == NATRAY_CG_PIPELINED;
+  s->refresh = (s->sc.precision == NATRAY_CG_MIXED || s->sc.precision == NATRAY_CG_PIPELINED) && s->sc.refresh_every > 0u;
   s->N = n;
   s->S = params->bdims[0] * params->bdims[1] * params->bdims[2];
   s->cap = params->bdims[3];
@@ -1428,6 +1585,15 @@ osim* F(create)(const natray_params* params, const float* particles, uint32_t n,
     return NULL;
   }
   for (uint32_t i = 0; i < s->S; ++i) s->block_ptr[i] = NONE;
+  if (s->sc.precision == NATRAY_CG_PIPELINED) {
+    s->pz = (REAL*)calloc(cells + 1, sizeof(REAL));
+    s->pq = (REAL*)calloc(cells + 1, sizeof(REAL));
+    s->pm = (REAL*)calloc(cells + 1, sizeof(REAL));
+    if (!s->pz || !s->pq || !s->pm) {
+      F(destroy)(s);
+      return NULL;
+    }
+  }
   if (s->sc.flags & NATRAY_SCENE_AIR) {
     s->air_x = (REAL*)calloc(5 * cells + 1, sizeof(REAL));
     s->air_tmp = (REAL*)calloc(5 * cells + 1, sizeof(REAL));
@@ -1469,6 +1635,9 @@ void F(destroy)(osim* s) {
   free(s->cf);
   free(s->bvec);
   free(s->part);
+  free(s->pz);
+  free(s->pq);
+  free(s->pm);
   free(s->guess);
   free(s->mem_x);
   free(s->mem_v);

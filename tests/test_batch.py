@@ -2,7 +2,7 @@
 stepped alone (which tests/test_gpu_parity.py holds bit-exact against the oracle), with the
 steps replayed as graphs or launched kernel by kernel, with the solve in register form (few
 scenes) and in memory form (many scenes sharing the solve's workgroups), for both CG
-precisions; and the autotuner's choice leaves the results unchanged."""
+precisions and the pipelined CG; and the autotuner's choice leaves the results unchanged."""
 import sys
 
 import numpy as np
@@ -28,9 +28,9 @@ def small_set(precision):
             for k, (side, n, d) in enumerate(specs)]
 
 
-def many_set(count=64):
+def many_set(count=64, precision=abi.CG_MIXED):
     """Scenes with more blocks than their share of the solve's workgroups holds in registers."""
-    return [scenes.balloon_impact(side=48, n=32768, seed=100 + k, speed=300.0, precision=abi.CG_MIXED, iters=400,
+    return [scenes.balloon_impact(side=48, n=32768, seed=100 + k, speed=300.0, precision=precision, iters=400,
                                   direction=(1.0, 0.1 * k, 0.0), capacity=(48 // 4) ** 3)
             for k in range(count)]
 
@@ -71,7 +71,7 @@ def check_same(tag, got, want, keys=STATE_KEYS):
         assert_same_float(f"{tag} scene {k} particles", Pg, Pw)
 
 
-@pytest.mark.parametrize("precision", [abi.CG_FP32, abi.CG_MIXED])
+@pytest.mark.parametrize("precision", [abi.CG_FP32, abi.CG_MIXED, abi.CG_PIPELINED])
 @pytest.mark.parametrize("graph", [True, False])
 def test_batch_matches_scenes_alone(precision, graph):
     say(f"[batch] 4 scenes, precision {precision}, graph {graph}")
@@ -81,12 +81,13 @@ def test_batch_matches_scenes_alone(precision, graph):
     check_same(f"precision {precision} graph {graph}", got, want)
 
 
-def test_batch_of_many_scenes_memory_form():
-    say("[batch] 64 scenes of 32k particles")
-    sl = many_set()
+@pytest.mark.parametrize("precision", [abi.CG_MIXED, abi.CG_PIPELINED])
+def test_batch_of_many_scenes_memory_form(precision):
+    say(f"[batch] 64 scenes of 32k particles, precision {precision}")
+    sl = many_set(precision=precision)
     want = alone(sl, 3)
     got, _ = batched(sl, 3)
-    check_same("64 scenes", got, want)
+    check_same(f"64 scenes precision {precision}", got, want)
 
 
 def test_autotuned_batch_matches_scenes_alone():

@@ -36,6 +36,9 @@ SCENES = {
     "impact_mixed_32k": lambda: scenes.balloon_impact(side=64, n=32768, seed=4, iters=400, capacity=4096,
                                                        precision=abi.CG_MIXED, refresh_every=5),
     "reference_balloon_res16": small_reference,
+    # Replacement every 5 iterations: the first solves replace their residual once.
+    "impact_pipelined_32k": lambda: scenes.balloon_impact(side=64, n=32768, seed=5, iters=400, capacity=4096,
+                                                           precision=abi.CG_PIPELINED, refresh_every=5),
 }
 
 
@@ -51,6 +54,7 @@ def few_solve_groups(name, groups=2):
 
 SCENES["sphere_jacobi_32k_mem"] = few_solve_groups("sphere_jacobi_32k")
 SCENES["impact_mixed_32k_mem"] = few_solve_groups("impact_mixed_32k")
+SCENES["impact_pipelined_32k_mem"] = few_solve_groups("impact_pipelined_32k")
 
 
 def assert_same_float(name, a, b):
@@ -122,7 +126,7 @@ def test_phases_bit_exact(name):
     orc.close()
 
 
-@pytest.mark.parametrize("name", ["sphere_jacobi_32k", "impact_sgs_32k"])
+@pytest.mark.parametrize("name", ["sphere_jacobi_32k", "impact_sgs_32k", "impact_pipelined_32k"])
 def test_run_matches_oracle(name):
     """Ten whole steps through natray_sim_run against the oracle's run."""
     s = SCENES[name]()

@@ -193,7 +193,7 @@ def test_projection_closed_form(solved):
     np.testing.assert_allclose(lhs, rhs, atol=5e-4 * scale)
 
 
-@pytest.mark.parametrize("precision", [abi.CG_FP32, abi.CG_MIXED])
+@pytest.mark.parametrize("precision", [abi.CG_FP32, abi.CG_MIXED, abi.CG_PIPELINED])
 @pytest.mark.parametrize("precond", [abi.PRECOND_BLOCK_SGS, abi.PRECOND_JACOBI])
 def test_all_solver_variants_solve_the_same_system(precision, precond):
     s, o, g, A, b, p0, wc, kc, dt = solve_setup(precond=precond, precision=precision, refresh=10)
@@ -202,7 +202,88 @@ def test_all_solver_variants_solve_the_same_system(precision, precond):
     p = water_rows(s, g2, wc, g2["cf"][abi.F_P]).astype(np.float64)
     rel_res = np.linalg.norm(b - A @ p) / max(np.linalg.norm(b), 1.0)
     assert o.state().unconverged == 0
-    assert rel_res < (2e-6 if precision == abi.CG_MIXED else 5e-5), rel_res
+    assert rel_res < (5e-5 if precision == abi.CG_FP32 else 2e-6), rel_res
+
+
+def impact_oracle(kind, precision, refresh, seed=3):
+    s = scenes.balloon_impact(side=64, n=32768, seed=seed, iters=400, capacity=4096, precision=precision,
+                              refresh_every=refresh)
+    return oracle.Oracle.from_scene(s, kind)
+
+
+def last_direction_field(iters):
+    """The field the plain CG's last direction is in: solve_q of iteration k writes D + 1 - k % 2."""
+    return abi.F_D + 1 - (iters - 1) % 2
+
+
+@pytest.mark.parametrize("refresh", [0, 1, 5])
+def test_pipelined_cg_is_cg_in_float64(refresh):
+    """The pipelined recurrences (NATRAY_CG_PIPELINED) carry the plain CG's iterates: in float64
+    both take the same number of iterations every step and end with the same pressure x,
+    residual r, preconditioned residual u (the plain CG's z), direction p and A p (the plain
+    CG's q), to rounding. Residual replacement (every iteration, or every 5, while the residual
+    is large) changes nothing in exact arithmetic, so it must not change these either; a
+    dropped or misplaced term in any recurrence or replacement would."""
+    a = impact_oracle("f64", abi.CG_MIXED, 0)
+    b = impact_oracle("f64", abi.CG_PIPELINED, refresh)
+    for step in range(4):
+        a.run(1, 400)
+        b.run(1, 400)
+        it_a, it_b = int(a.solver_scalars()["solve_iters"]), int(b.solver_scalars()["solve_iters"])
+        assert it_a == it_b, (step, it_a, it_b)
+        if it_a == 0:
+            continue
+        ca, cb = a.cf_f64(), b.cf_f64()
+        water = ca[abi.F_TYPE] == abi.WATER
+        for fa, fb in ((abi.F_P, abi.F_P), (abi.F_R, abi.F_R), (abi.F_Z, abi.F_Z), (abi.F_Q, abi.F_Q),
+                       (last_direction_field(it_a), abi.F_D)):
+            va, vb = ca[fa][water], cb[fb][water]
+            assert np.abs(va - vb).max() <= 1e-9 * np.abs(va).max(), (step, fa, np.abs(va - vb).max(), np.abs(va).max())
+    a.close()
+    b.close()
+
+
+def true_residuals(precision, refresh, steps):
+    """Per step: the solve's iterations and |b - A p| / max(|b|, 1) of its pressure, from a twin
+    oracle stopped before the solve (the residual helper reads b from the grid before a solve)."""
+    A = impact_oracle("f32", precision, refresh)
+    B = impact_oracle("f32", precision, refresh)
+    out = []
+    for _ in range(steps):
+        for o in (A, B):
+            for ph in range(abi.PHASE_SOLVE):
+                o.phase(ph)
+        A.phase(abi.PHASE_SOLVE)
+        sc = A.solver_scalars()
+        cap = int(A.params.bdims[3]) * 64
+        x = np.zeros(cap, np.float32)
+        g = A.grid()
+        x[: g["cf"].shape[1]] = g["cf"][abi.F_P]
+        r = B.residual(x).astype(np.float64)
+        if sc["b_norm"] > 0:
+            out.append((int(sc["solve_iters"]), float(np.sqrt((r * r).sum()) / max(sc["b_norm"], 1.0))))
+        B.phase(abi.PHASE_SOLVE)
+        for o in (A, B):
+            for ph in range(abi.PHASE_SOLVE + 1, abi.N_PHASES):
+                o.phase(ph)
+    A.close()
+    B.close()
+    return out
+
+
+def test_pipelined_cg_in_float32_reaches_the_plain_tolerance():
+    """In float32 the recurred residual drifts from the true one faster in the pipelined form;
+    with residual replacement every 10 iterations (while the residual is large) the impact's
+    solves take the plain FP32 solve's iteration counts, and the true residual of the pressure
+    is no worse than the plain solve's. Without replacement it converges in as many iterations
+    but several times further from the true tolerance."""
+    plain = true_residuals(abi.CG_FP32, 0, 8)
+    pipe = true_residuals(abi.CG_PIPELINED, 10, 8)
+    bare = true_residuals(abi.CG_PIPELINED, 0, 8)
+    assert [k for k, _ in pipe] == [k for k, _ in plain]
+    worst_plain = max(r for _, r in plain)
+    assert max(r for _, r in pipe) <= 1.05 * max(worst_plain, 1e-6), (pipe, plain)
+    assert max(r for _, r in bare) > 3.0 * worst_plain
 
 
 def block_water(seed):
