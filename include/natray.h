@@ -157,7 +157,12 @@ typedef struct natray_particle {
                                     stop being water filled with gas or vapour, GAS and RUBBER
                                     pressures from it (air.wgsl:85-148,271-277); without it
                                     every GAS cell is at gauge 0 */
-#define NATRAY_SCENE_REFERENCE (NATRAY_SCENE_GROUND | NATRAY_SCENE_KNOT | NATRAY_SCENE_AIR)
+/* Spray and wet ground: a particle that leaves the resolved flow breaks up into drops
+ * (break_up, fluid.wgsl:720-765) or, on the ground, wets it (ground_touch, sim_defs.wgsl:233-240),
+ * and the drops fly until they land (spray_update, fluid.wgsl:686-711). Without it a leaving
+ * particle only dies. */
+#define NATRAY_SCENE_SPRAY 64u
+#define NATRAY_SCENE_REFERENCE (NATRAY_SCENE_GROUND | NATRAY_SCENE_KNOT | NATRAY_SCENE_AIR | NATRAY_SCENE_SPRAY)
 
 #define NATRAY_PRECOND_BLOCK_SGS 0u /* fluid.wgsl:384-410 */
 #define NATRAY_PRECOND_JACOBI 1u
@@ -356,6 +361,20 @@ int natray_sim_set_membrane(natray_sim* sim, uint32_t n, const float* x, const f
 /* The water sampled at each vertex by the last step: water[n][4] = face velocity (m/s) and
  * fill (1 inside the water, 0 away from it), the reference's mem_water. */
 int natray_sim_read_membrane(natray_sim* sim, float* water);
+
+/* With NATRAY_SCENE_SPRAY: the spray records as break_up and spray_update left them, the
+ * `spray` buffer of the reference (src/sim.rs:252, Spray at sim_defs.wgsl:62) and `landed`
+ * (src/sim.rs:254). Copies the first min(count, cap, max_records) records: spray[k][12] =
+ * position (m) and radius (m, 0 once gone), velocity (m/s) and weight (drops in the parcel),
+ * 0 0 0 and the time it was made (s); landed[k][4] = where it landed (x, z), the radius of its
+ * drops' volume and when (zero until it lands). *count = State.spray_count, which counts every
+ * slot asked for, also past the cap (as the reference's). The slots follow the order in which
+ * particles broke up, which is not fixed: compare records as a set. Either array may be null. */
+int natray_sim_read_spray(natray_sim* sim, uint32_t max_records, float* spray, float* landed, uint32_t* count);
+/* With NATRAY_SCENE_SPRAY: the wet map, `touch` of the reference (src/sim.rs:253, written by
+ * ground_touch, sim_defs.wgsl:233-240), limits.x^2 words, row z, column x: the f32 bits of the
+ * first time (s) water touched each ground cell, 0xffffffff where none has. */
+int natray_sim_read_wet(natray_sim* sim, uint32_t* touch);
 
 /* With NATRAY_SCENE_AIR: each cell's gas after the last phase that wrote it, air[5][64 n] for
  * the n blocks in use (density kg/m^3, momentum x, y, z, total energy per volume J/m^3; 0 in

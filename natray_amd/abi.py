@@ -30,8 +30,8 @@ SOUND, CFL, DT_MAX, SOLVE_TOL = 1480.0, 0.8, 1e-3, 1e-5
 MAX_ITERS, MAX_BATCH, MAX_BLOCKS = 4096, 256, 65535
 RHO_WATER = 1000.0
 
-SCENE_GROUND, SCENE_KNOT, SCENE_WALLS, SCENE_CAPSULE, SCENE_MASK, SCENE_AIR = 1, 2, 4, 8, 16, 32
-SCENE_REFERENCE = SCENE_GROUND | SCENE_KNOT | SCENE_AIR
+SCENE_GROUND, SCENE_KNOT, SCENE_WALLS, SCENE_CAPSULE, SCENE_MASK, SCENE_AIR, SCENE_SPRAY = 1, 2, 4, 8, 16, 32, 64
+SCENE_REFERENCE = SCENE_GROUND | SCENE_KNOT | SCENE_AIR | SCENE_SPRAY
 PRECOND_BLOCK_SGS, PRECOND_JACOBI = 0, 1
 CG_FP32, CG_MIXED, CG_PIPELINED = 0, 1, 2
 
@@ -163,6 +163,8 @@ _SIGNATURES = {
     "natray_sim_set_membrane": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "natray_sim_pack_water": (C.c_int, [_P, _P]),
     "natray_sim_read_air": (C.c_int, [_P, _P]),
+    "natray_sim_read_spray": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
+    "natray_sim_read_wet": (C.c_int, [_P, _P]),
     "natray_sim_read_membrane": (C.c_int, [_P, _P]),
     "natray_sim_ws_samples": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "natray_sim_ws_stats": (C.c_int, [_P, _P]),

@@ -93,6 +93,14 @@ struct Dev {
   uint32_t* prev_ptr;
   uint32_t* block_list_prev;
   uint32_t* block_active;
+  // Spray and wet ground (NATRAY_SCENE_SPRAY): spray_cap records of 12 floats (position and
+  // radius, velocity and weight, 0 0 0 and the time made), where each landed (4 floats), and the
+  // wet map (wet_res^2 words: f32 bits of the first time water touched each ground cell,
+  // 0xffffffff untouched). Null without the flag.
+  float* spray;
+  float* landed;
+  uint32_t* touch;
+  uint32_t spray_cap, wet_res;
   const float4* mem_x;
   const float4* mem_v;
   const float* mem_mass;
@@ -163,6 +171,8 @@ int launch_mem_sample(const Target& t, void* stream);
 // cells filled and the GAS and RUBBER pressures set after grid_prep.
 int launch_air_carry(const Target& t, void* stream);
 int launch_air_fill(const Target& t, void* stream);
+// Spray (one scene): the drops fly, evaporate and land; before G2P, which makes new ones.
+int launch_spray_update(const Target& t, void* stream);
 // The keyframe words of the water: two per particle into `out` (device memory, 8 N bytes).
 int launch_pack_water(const Target& t, void* stream, void* out);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also

@@ -30,6 +30,7 @@
 
 #include "natray_dev.h"
 #include "natray_lanes.h"
+#include "natray_math.h"
 
 namespace natray {
 
@@ -2464,6 +2465,141 @@ __device__ __forceinline__ void carve(const Dev& d, float x[3], float v[3]) {
   v[2] += dir[2] * push;
 }
 
+// ---- spray and wet ground (fluid.wgsl:676-765, sim_defs.wgsl:233-240; NATRAY_SCENE_SPRAY) --------
+//
+// A particle that leaves the resolved flow becomes spray (break_up) or, on the ground, wets it
+// (ground_touch); the drops fly through still air until they land (spray_update, before G2P).
+// exp, log, cos and pow come from natray_math.h, which the CPU oracle uses too, so the records
+// are the oracle's bit for bit; their slots come from an atomic add, so their order is not.
+
+constexpr float RHO_AIR = 1.2f;          // sim_defs.wgsl:101
+constexpr float MU_AIR = 1.8e-5f;        // sim_defs.wgsl:102
+constexpr float EVAPORATION = 1.3e-10f;  // fluid.wgsl:684
+constexpr float WE_CRIT = 12.0f;         // fluid.wgsl:718
+constexpr float PI_F = 3.14159265358979f;  // common.wgsl:1
+constexpr uint32_t BREAKUP_DROPS = 4;    // fluid.wgsl:719
+
+// pcg, hash3, rnd, rnd_gauss (common.wgsl:3-22)
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+  const uint32_t t = v * 747796405u + 2891336453u;
+  const uint32_t w = ((t >> ((t >> 28u) + 4u)) ^ t) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) { return pcg(a ^ pcg(b ^ pcg(c))); }
+__device__ __forceinline__ float rnd(uint32_t& seed) {
+  seed = pcg(seed);
+  return (float)(seed >> 8u) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ float rnd_gauss(uint32_t& seed) {
+  const float u1 = fmax_(rnd(seed), 1e-7f);
+  const float u2 = rnd(seed);
+  return __builtin_sqrtf(-2.0f * nm_logf(u1)) * nm_cosf(2.0f * PI_F * u2);
+}
+
+// ground_touch (sim_defs.wgsl:233-240); the cell index is clamped to the map, which matters
+// only for map sides that are not powers of two.
+__device__ __forceinline__ void ground_touch(const Dev& d, float px, float pz) {
+  const float hw = d.U->wet[0];
+  const uint32_t res = d.wet_res;
+  const float ux = (px + hw) / (2.0f * hw), uz = (pz + hw) / (2.0f * hw);
+  if (ux < 0.0f || uz < 0.0f || ux >= 1.0f || uz >= 1.0f || res == 0u) return;
+  const uint32_t cx = min((uint32_t)(ux * (float)res), res - 1u), cz = min((uint32_t)(uz * (float)res), res - 1u);
+  atomicMin(&d.touch[(size_t)cz * res + cx], __float_as_uint(now_of(d)));
+}
+
+// break_up (fluid.wgsl:720-765): four drops and a parcel of finer drops, in five consecutive
+// spray slots.
+__device__ __noinline__ void break_up(const Dev& d, float x0, float x1, float x2, float v0, float v1, float v2, uint32_t id) {
+  const uint32_t cap = d.spray_cap;
+  const uint32_t first = atomicAdd(&d.st->spray_count, BREAKUP_DROPS + 1u);
+  if (first >= cap) return;
+  uint32_t seed = hash3(d.st->step, id, 0xb4eau);
+  const float hs = d.U->fluid[3];
+  const float vol = hs * hs * hs;
+  const float speed = __builtin_sqrtf((v0 * v0 + v1 * v1) + v2 * v2);
+  // breakup_diameter (fluid.wgsl:713-716)
+  const float bd = (WE_CRIT * NATRAY_SIGMA_W) / (RHO_AIR * fmax_(speed * speed, 1e-4f));
+  const float d_med = fmin_(0.55f * hs, 0.5f * bd);
+  float dd[BREAKUP_DROPS];
+  float used = 0.0f;
+#pragma unroll
+  for (uint32_t k = 0; k < BREAKUP_DROPS; ++k) {
+    dd[k] = fmin_(fmax_(d_med * nm_expf(0.5f * rnd_gauss(seed)), 2e-5f), 1.24f * hs);
+    used += 0.5236f * dd[k] * dd[k] * dd[k];
+  }
+  const float df = 0.5f * d_med;
+  const bool fine = df < 1e-4f;
+  const float fit = nm_powf(vol / used, 1.0f / 3.0f);
+  const float scale = fine ? fmin_(1.0f, fit) : fit;
+  const float tnow = now_of(d);
+  const float x[3] = {x0, x1, x2}, v[3] = {v0, v1, v2};
+#pragma unroll
+  for (uint32_t k = 0; k < BREAKUP_DROPS; ++k) {
+    const uint32_t slot = first + k;
+    if (slot >= cap) return;
+    const float dk = dd[k] * scale;
+    const float spread = __builtin_sqrtf(NATRAY_SIGMA_W / (d.U->fluid[0] * dk)) + 0.05f * speed;
+    const float g0 = rnd_gauss(seed);
+    const float g1 = rnd_gauss(seed);
+    const float g2 = rnd_gauss(seed);
+    const float w0 = g0 + 1e-6f, w1 = g1 + 1e-6f, w2 = g2 + 1e-6f;
+    const float len = __builtin_sqrtf((w0 * w0 + w1 * w1) + w2 * w2);
+    const float u[3] = {w0 / len, w1 / len, w2 / len};
+    const float o0 = rnd(seed);
+    const float o1 = rnd(seed);
+    const float o2 = rnd(seed);
+    const float rv = rnd(seed);
+    float4* rec = reinterpret_cast<float4*>(d.spray) + (size_t)slot * 3;
+    rec[0] = make_float4(x[0] + (o0 - 0.5f) * hs, x[1] + (o1 - 0.5f) * hs, x[2] + (o2 - 0.5f) * hs, 0.5f * dk);
+    rec[1] = make_float4(v[0] + u[0] * spread * rv, v[1] + u[1] * spread * rv, v[2] + u[2] * spread * rv, 1.0f);
+    rec[2] = make_float4(0.0f, 0.0f, 0.0f, tnow);
+  }
+  const uint32_t slot = first + BREAKUP_DROPS;
+  if (slot >= cap) return;
+  const float n = (vol - used * scale * scale * scale) / (0.5236f * df * df * df);
+  float4* rec = reinterpret_cast<float4*>(d.spray) + (size_t)slot * 3;
+  rec[0] = make_float4(x[0], x[1], x[2], (fine && n >= 2.0f) ? 0.5f * df : 0.0f);
+  rec[1] = make_float4(v[0], v[1], v[2], fmax_(n, 2.0f));
+  rec[2] = make_float4(0.0f, 0.0f, 0.0f, tnow);
+}
+
+// drag_cd (fluid.wgsl:676-679)
+__device__ __forceinline__ float drag_cd(float re) {
+  if (re < 1e-4f) return 24.0f / 1e-4f;
+  return 24.0f / re * (1.0f + 0.15f * nm_powf(re, 0.687f)) + 0.42f / (1.0f + 42500.0f * nm_powf(re, -1.16f));
+}
+
+// spray_update (fluid.wgsl:686-711), grid-stride over the records made so far.
+template <class L>
+__global__ __launch_bounds__(256) void k_spray_update(L l) {
+  const View vw = view(l);
+  const Dev& d = vw.d;
+  if (d.st->dt_ns == 0u || !d.spray) return;
+  const uint32_t n = min(d.st->spray_count, d.spray_cap);
+  const float step = dt_of(d), tnow = now_of(d);
+  const float g[3] = {d.U->gravity[0], d.U->gravity[1], d.U->gravity[2]};
+  for (uint32_t i = vw.wg * 256u + threadIdx.x; i < n; i += vw.wgs * 256u) {
+    float4* rec = reinterpret_cast<float4*>(d.spray) + (size_t)i * 3;
+    const float4 pos = rec[0], vel = rec[1];
+    float r = pos.w;
+    if (r <= 0.0f) continue;
+    const float sp = __builtin_sqrtf((vel.x * vel.x + vel.y * vel.y) + vel.z * vel.z);
+    const float re = 2.0f * r * sp * RHO_AIR / MU_AIR;
+    const float k = 3.0f * RHO_AIR * drag_cd(re) * sp / (8.0f * d.U->fluid[0] * r);
+    const float den = 1.0f + k * step;
+    const float vn[3] = {(vel.x + step * g[0]) / den, (vel.y + step * g[1]) / den, (vel.z + step * g[2]) / den};
+    r = __builtin_sqrtf(fmax_(r * r - EVAPORATION * step, 0.0f));
+    if (r < 2.5e-6f) r = 0.0f;
+    const float p[3] = {pos.x + step * vn[0], pos.y + step * vn[1], pos.z + step * vn[2]};
+    if (p[1] <= r) {
+      reinterpret_cast<float4*>(d.landed)[i] = make_float4(p[0], p[2], r * nm_powf(vel.w, 1.0f / 3.0f), tnow);
+      r = 0.0f;
+    }
+    rec[0] = make_float4(p[0], p[1], p[2], r);
+    rec[1] = make_float4(vn[0], vn[1], vn[2], vel.w);
+  }
+}
+
 // g2p_one (fluid.wgsl:806-861); returns the particle's speed, 0 once it is gone.
 __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const uint32_t* links, uint32_t i, const int corner[3]) {
   const natray_params* U = d.U;
@@ -2502,7 +2638,13 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
   if ((d.sc->flags & NATRAY_SCENE_MASK) && in_mask(d, own[0], own[1], own[2])) covered = true;
   const float thin_age = (fill < 0.45f && !on_ground && !covered) ? c1w + step : 0.0f;
   if (escaped || (!covered && fill < 0.2f) || thin_age > t_thin) {
-    // break_up / ground_touch belong to the spray and wet-ground subsystems: the particle dies.
+    if (d.spray) {
+      // Too thin for the grid: on the ground it soaks in, elsewhere it breaks up into spray.
+      if (on_ground)
+        ground_touch(d, x[0], x[2]);
+      else
+        break_up(d, x[0], x[1], x[2], v[0], v[1], v[2], i);
+    }
     P[1] = make_float4(v[0], v[1], v[2], 0.0f);
     d.key[i] = NONE;
     return 0.0f;
@@ -2512,6 +2654,7 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
     xx[1] = 0.3f * spacing;
     vv[1] = fmax_(vv[1], 0.0f);
   }
+  if (d.spray && xx[1] < 0.7f * spacing) ground_touch(d, xx[0], xx[2]);
   const float cs = 4.0f / (hh * hh);
   P[0] = make_float4(xx[0], xx[1], xx[2], pos.w);
   P[1] = make_float4(vv[0], vv[1], vv[2], 1.0f);
@@ -2992,6 +3135,10 @@ int launch_air_carry(const Target& t, void* stream) {
 }
 int launch_air_fill(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_air_fill, LK_WAVES, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+int launch_spray_update(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_spray_update, LK_CELLS, 256, t, S(stream));
   return (int)hipGetLastError();
 }
 int launch_pack_water(const Target& t, void* stream, void* out) {

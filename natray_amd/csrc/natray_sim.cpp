@@ -252,7 +252,11 @@ int enqueue_phase(natray_sim* s, int phase) {
       LAUNCHCHK(natray::launch_extend(s->tgt, st));
       if (s->D.mem_cap) LAUNCHCHK(natray::launch_mem_sample(s->tgt, st));
       break;
-    case NATRAY_PHASE_G2P: LAUNCHCHK(natray::launch_g2p(s->tgt, st)); break;
+    case NATRAY_PHASE_G2P:
+      // spray_update runs before g2p (src/sim.rs:425), so drops made in a step move in the next.
+      if (s->D.spray) LAUNCHCHK(natray::launch_spray_update(s->tgt, st));
+      LAUNCHCHK(natray::launch_g2p(s->tgt, st));
+      break;
     default: return fail(NATRAY_E_ARG, "unknown phase " + std::to_string(phase));
   }
   if (timed) HIPCHK(hipEventRecord(s->events[2 * slot + 1], s->stream));
@@ -451,6 +455,18 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
     HIPTRY(hipMemsetAsync(D.prev_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
     HIPTRY(hipMemsetAsync(D.block_list_prev, 0, ((size_t)D.cap + 1) * 4, s->stream));
     HIPTRY(hipMemsetAsync(D.block_active, 0, ((size_t)D.cap + 1) * 4, s->stream));
+  }
+  if (s->sc.flags & NATRAY_SCENE_SPRAY) {
+    // src/sim.rs:252-254: spray and landed start zeroed, the wet map at u32::MAX.
+    D.spray_cap = s->U.limits[1];
+    D.wet_res = s->U.limits[0];
+    const size_t ww = (size_t)D.wet_res * D.wet_res;
+    TRY(dalloc(s, &D.spray, (size_t)D.spray_cap * 12));
+    TRY(dalloc(s, &D.landed, (size_t)D.spray_cap * 4));
+    TRY(dalloc(s, &D.touch, ww));
+    HIPTRY(hipMemsetAsync(D.spray, 0, (size_t)D.spray_cap * 48, s->stream));
+    HIPTRY(hipMemsetAsync(D.landed, 0, (size_t)D.spray_cap * 16, s->stream));
+    HIPTRY(hipMemsetAsync(D.touch, 0xff, ww * 4, s->stream));
   }
   natray::Dev* dself = nullptr;
   TRY(dalloc(s, &dself, 1));
@@ -835,6 +851,29 @@ int natray_sim_read_air(natray_sim* s, float* air) {
   const size_t nc = (size_t)std::min(s->last.n_blocks, s->D.cap) * 64;
   for (int f = 0; f < 5; ++f)
     if (nc) HIPCHK(hipMemcpyAsync(air + f * nc, s->D.air_x + (size_t)f * s->D.capc, nc * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_read_spray(natray_sim* s, uint32_t max_records, float* spray, float* landed, uint32_t* count) {
+  if (!s || !count) return fail(NATRAY_E_ARG, "null argument");
+  if (!s->D.spray) return fail(NATRAY_E_ARG, "the scene has no spray (NATRAY_SCENE_SPRAY)");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  int e = sync_state(s);
+  if (e) return e;
+  *count = s->last.spray_count;
+  const size_t n = std::min(std::min(s->last.spray_count, s->D.spray_cap), max_records);
+  if (n && spray) HIPCHK(hipMemcpyAsync(spray, s->D.spray, n * 48, hipMemcpyDeviceToHost, s->stream));
+  if (n && landed) HIPCHK(hipMemcpyAsync(landed, s->D.landed, n * 16, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_read_wet(natray_sim* s, uint32_t* touch) {
+  if (!s || !touch) return fail(NATRAY_E_ARG, "null argument");
+  if (!s->D.touch) return fail(NATRAY_E_ARG, "the scene has no wet map (NATRAY_SCENE_SPRAY)");
+  const size_t ww = (size_t)s->D.wet_res * s->D.wet_res;
+  if (ww) HIPCHK(hipMemcpyAsync(touch, s->D.touch, ww * 4, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return NATRAY_OK;
 }
@@ -1224,6 +1263,7 @@ int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out)
     if (sims[k]->device != sims[0]->device) return fail(NATRAY_E_ARG, "the scenes of a batch must be on one device");
     if (sims[k]->ws_mode) return fail(NATRAY_E_ARG, "the warm start runs on single scenes, not in a batch");
     if (sims[k]->D.mem_cap) return fail(NATRAY_E_ARG, "the membrane coupling runs on single scenes, not in a batch");
+    if (sims[k]->D.spray) return fail(NATRAY_E_ARG, "the spray runs on single scenes, not in a batch");
     for (uint32_t j = 0; j < k; ++j)
       if (sims[j] == sims[k]) return fail(NATRAY_E_ARG, "a scene appears twice in the batch");
   }

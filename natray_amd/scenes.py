@@ -293,6 +293,39 @@ def water_sphere(side: int = 64, n: int = 32768, seed: int = 1, fixed_dt: float 
                  {"workload": f"water sphere {n} particles {side}^3", "r_eff": r_eff, "dx": 1.0 / side})
 
 
+def breakup_scene(side: int = 32, n_ball: int = 8 * 400, n_slow: int = 48, n_fast: int = 32, n_ground: int = 32,
+                  seed: int = 9, fixed_dt: float = 1e-3, spray_cap: int = 4096) -> Scene:
+    """Spray and wet ground (NATRAY_SCENE_SPRAY): a water ball in a solid-walled box plus lone
+    particles, which leave the resolved flow at the first G2P (a lone particle's own weights
+    give it a fill near 0.04): `n_slow` 0.10-0.18 m up falling at 15-25 m/s (drops of ~0.5 mm
+    that land within ~10 steps), `n_fast` high up at 40-60 m/s sideways (the fine
+    spray branch, a parcel of drops, with strong drag), and `n_ground` below 1.5 spacings
+    (they wet the ground instead). Fixed dt; a small spray capacity."""
+    base = water_sphere(side=side, n=n_ball, seed=seed, fixed_dt=fixed_dt)
+    U, sc = base.params, base.scene
+    dx = 1.0 / side
+    spacing = 0.5 * dx
+    rng = np.random.default_rng(seed)
+    groups = []
+    for count, ylo, yhi, vlo, vhi in ((n_slow, 0.10, 0.18, (-2.0, -25.0, -2.0), (2.0, -15.0, 2.0)),
+                                      (n_fast, 0.3, 0.6, (40.0, -1.0, -5.0), (60.0, 1.0, 5.0)),
+                                      (n_ground, 0.2 * spacing, 1.2 * spacing, (-0.5, -0.5, -0.5), (0.5, 0.0, 0.5))):
+        P = np.zeros((count, abi.PARTICLE_FLOATS), np.float32)
+        P[:, 0] = rng.uniform(0.25, 0.6, count)
+        P[:, 1] = rng.uniform(ylo, yhi, count)
+        P[:, 2] = rng.uniform(0.25, 0.75, count)
+        P[:, 3] = 1.0
+        P[:, 4:7] = rng.uniform(vlo, vhi, (count, 3))
+        P[:, 7] = 1.0
+        groups.append(P)
+    P = np.concatenate([base.particles] + groups)
+    U.dims[3] = P.shape[0]
+    U.limits[1] = spray_cap
+    sc.flags |= abi.SCENE_SPRAY
+    return Scene(U, P, sc, no_limits_state(),
+                 {"workload": f"breakup: water ball {n_ball} + {n_slow}/{n_fast}/{n_ground} lone particles {side}^3", "dx": dx})
+
+
 def balloon_impact(side: int = 256, n: int = 2_097_152, seed: int = 2, speed: float = 300.0, radius: float = 0.01,
                    half_length: float = 0.015, precond: int = abi.PRECOND_BLOCK_SGS, precision: int = abi.CG_FP32,
                    refresh_every: int = 20, tol: float = 1e-6, iters: int = 1000, capacity: int = abi.MAX_BLOCKS,
@@ -328,12 +361,12 @@ def balloon_impact(side: int = 256, n: int = 2_097_152, seed: int = 2, speed: fl
 
 def batch_impacts(count: int = 64, seed: int = 4, n_lo: int = 16384, n_hi: int = 524288, side_lo: int = 64,
                   side_hi: int = 192, speed_lo: float = 200.0, speed_hi: float = 400.0, precision: int = abi.CG_MIXED,
-                  iters: int = 1000) -> list[Scene]:
+                  iters: int = 1000, refresh_every: int = 20) -> list[Scene]:
     """The batched scenes: `count` independent balloon impacts. Scene k draws from its own
     seed (splitmix64 of `seed`, k-th value): a particle count log-uniform in [n_lo, n_hi]
     (multiple of 8), a grid side uniform in [side_lo, side_hi] (multiple of 4), a capsule
     speed uniform in [speed_lo, speed_hi] and a direction uniform on the sphere. FP32 state;
-    the CG precision is `precision` (mixed, refresh every 20 iterations, by default). With the
+    the CG precision is `precision` (mixed, refresh every `refresh_every` = 20 iterations, by default). With the
     defaults: 64 scenes of 16k..512k particles on 64^3..192^3."""
     seeds = splitmix64(seed, count)
     out = []
@@ -351,7 +384,7 @@ def batch_impacts(count: int = 64, seed: int = 4, n_lo: int = 16384, n_hi: int =
         direction = (rho * math.cos(phi), rho * math.sin(phi), z)
         cap = min((side // 4) ** 3, 8 * (n // 512) + 2048)
         s = balloon_impact(side=side, n=n, seed=int(seeds[k] & 0xFFFFFFFF), speed=speed, direction=direction,
-                           precision=precision, iters=iters, capacity=cap)
+                           precision=precision, iters=iters, capacity=cap, refresh_every=refresh_every)
         s.meta.update({"scene": k, "speed": speed, "direction": direction, "capacity": cap})
         out.append(s)
     return out

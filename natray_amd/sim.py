@@ -180,6 +180,23 @@ class Sim:
         abi.check(abi.lib().natray_sim_read_air(self._h, abi.ptr(out)))
         return out
 
+    def spray(self, max_records: int = 1 << 20) -> dict:
+        """The spray (natray_sim_read_spray): count = State.spray_count, the first
+        min(count, cap, max_records) records [k][12] and where each landed [k][4]."""
+        rec = np.zeros((max_records, 12), np.float32)
+        landed = np.zeros((max_records, 4), np.float32)
+        cnt = C.c_uint32()
+        abi.check(abi.lib().natray_sim_read_spray(self._h, max_records, abi.ptr(rec), abi.ptr(landed), C.byref(cnt)))
+        n = min(cnt.value, int(self.params.limits[1]), max_records)
+        return {"count": cnt.value, "records": rec[:n], "landed": landed[:n]}
+
+    def wet(self) -> np.ndarray:
+        """The wet map (natray_sim_read_wet): [limits.x][limits.x] words."""
+        res = int(self.params.limits[0])
+        out = np.zeros((res, res), np.uint32)
+        abi.check(abi.lib().natray_sim_read_wet(self._h, abi.ptr(out)))
+        return out
+
     def pack_water(self) -> np.ndarray:
         """The water's keyframe words (natray_sim_pack_water): [N][2] uint32."""
         out = np.zeros((self.n, 2), np.uint32)

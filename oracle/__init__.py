@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
                 "read_membrane": (None, [P, P]),
                 "pack_water": (None, [P, P]),
                 "read_air": (None, [P, P]),
+                "read_spray": (C.c_uint32, [P, C.c_uint32, P, P]),
+                "read_wet": (None, [P, P]),
+                "elementary": (None, [C.c_int, P, P, P, C.c_uint32]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -83,6 +86,15 @@ def precondition(r: np.ndarray, diag: np.ndarray, water: np.ndarray, mode: int, 
     fn = lib().o32_precondition if precision == "f32" else lib().o64_precondition
     fn(abi.ptr(r), abi.ptr(diag), abi.ptr(water), abi.ptr(z), mode)
     return z
+
+
+def elementary(which: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    """The spray's exp, log, cos or pow(a, b) in float32, as the kernels evaluate them."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(a if b is None else b, np.float32)
+    out = np.empty_like(a)
+    lib().o32_elementary({"exp": 0, "log": 1, "cos": 2, "pow": 3}[which], abi.ptr(a), abi.ptr(b), abi.ptr(out), a.size)
+    return out
 
 
 class Oracle:
@@ -208,6 +220,21 @@ class Oracle:
         n = self.n_blocks()
         out = np.zeros((5, 64 * n), np.float32)
         self._f("read_air")(self._h, abi.ptr(out))
+        return out
+
+    def spray(self, max_records: int = 1 << 20) -> dict:
+        """The spray's records (NATRAY_SCENE_SPRAY): count (State.spray_count), the first
+        min(count, cap, max_records) records [k][12] and where each landed [k][4], as doubles."""
+        rec = np.zeros((max_records, 12), np.float64)
+        landed = np.zeros((max_records, 4), np.float64)
+        n = int(self._f("read_spray")(self._h, max_records, abi.ptr(rec), abi.ptr(landed)))
+        return {"count": int(self.state().spray_count), "records": rec[:n], "landed": landed[:n]}
+
+    def wet(self) -> np.ndarray:
+        """The wet map (limits.x^2 words, f32 bits of the first touch; 0xffffffff untouched)."""
+        res = int(self.params.limits[0])
+        out = np.zeros((res, res), np.uint32)
+        self._f("read_wet")(self._h, abi.ptr(out))
         return out
 
     def pack_water(self) -> np.ndarray:

@@ -36,6 +36,9 @@
 #include <string.h>
 
 #include "natray_oracle.h"
+/* The spray's exp, log, cos and pow: the kernels' own functions, so both give the same bits;
+ * they are pinned against the C library by tests/test_oracle_spray.py. */
+#include "natray_math.h"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -106,6 +109,12 @@ struct osim {
    * carried into the step's blocks, the last step's block list, each slot's index in it, and
    * the blocks whose air was active. */
   REAL *air_x, *air_tmp;
+  /* Spray and wet ground (NATRAY_SCENE_SPRAY): spray_cap records of 12 (position and radius,
+   * velocity and weight, 0 0 0 and the time made), where each landed (4), and the wet map
+   * (wet_res^2 f32 bits of the first time water touched each ground cell). */
+  REAL *spray, *landed;
+  uint32_t* touch;
+  uint32_t spray_cap, wet_res;
   uint32_t *prev_ptr, *block_list_prev, *block_active;
 };
 
@@ -1404,6 +1413,171 @@ static void carve(const osim* s, REAL x[3], REAL v[3]) {
 }
 
 /* g2p_one (fluid.wgsl:806-861). Returns the speed, 0 for a particle that died. */
+/* ---- spray and wet ground (fluid.wgsl:676-765, sim_defs.wgsl:233-240; NATRAY_SCENE_SPRAY) ----- */
+
+#define RHO_AIR 1.2          /* sim_defs.wgsl:101 */
+#define MU_AIR 1.8e-5        /* sim_defs.wgsl:102 */
+#define EVAPORATION 1.3e-10  /* fluid.wgsl:684 */
+#define WE_CRIT 12.0         /* fluid.wgsl:718 */
+#define PI_C 3.14159265358979 /* common.wgsl:1 */
+#define BREAKUP_DROPS 4u     /* fluid.wgsl:719 */
+
+static inline REAL r_exp(REAL x) { return IS_F64 ? (REAL)nm_exp_d((double)x) : (REAL)nm_expf((float)x); }
+static inline REAL r_log(REAL x) { return IS_F64 ? (REAL)nm_log_d((double)x) : (REAL)nm_logf((float)x); }
+static inline REAL r_cos(REAL x) { return IS_F64 ? (REAL)nm_cos_d((double)x) : (REAL)nm_cosf((float)x); }
+static inline REAL r_pow(REAL x, REAL y) {
+  if (!IS_F64) return (REAL)nm_powf((float)x, (float)y);
+  return x == 0 ? (y > 0 ? (REAL)0 : (REAL)HUGE_VAL) : (REAL)nm_exp_d((double)y * nm_log_d((double)x));
+}
+
+/* pcg, hash3, rnd, rnd_gauss (common.wgsl:3-22) */
+static inline uint32_t pcg(uint32_t v) {
+  uint32_t t = v * 747796405u + 2891336453u;
+  uint32_t w = ((t >> ((t >> 28u) + 4u)) ^ t) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+static inline uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) { return pcg(a ^ pcg(b ^ pcg(c))); }
+static inline REAL rnd(uint32_t* seed) {
+  *seed = pcg(*seed);
+  return (REAL)(*seed >> 8u) * (REAL)(1.0 / 16777216.0);
+}
+static inline REAL rnd_gauss(uint32_t* seed) {
+  REAL u1 = rmax(rnd(seed), (REAL)1e-7);
+  REAL u2 = rnd(seed);
+  return sqrt_r((REAL)-2 * r_log(u1)) * r_cos((REAL)2 * (REAL)PI_C * u2);
+}
+
+/* now() (sim_defs.wgsl:171-173) */
+static inline REAL now_r(const osim* s) { return (REAL)s->st.t_ns * (REAL)1e-9; }
+
+/* ground_touch (sim_defs.wgsl:233-240): the first time water touched the ground cell under p,
+ * as the f32 bits of now() (an atomic minimum). The cell index is clamped to the map, which
+ * only matters for map sides that are not powers of two. */
+static void ground_touch(osim* s, REAL px, REAL pz) {
+  REAL hw = (REAL)s->U.wet[0];
+  uint32_t res = s->wet_res;
+  REAL ux = (px + hw) / ((REAL)2 * hw), uz = (pz + hw) / ((REAL)2 * hw);
+  if (ux < (REAL)0 || uz < (REAL)0 || ux >= (REAL)1 || uz >= (REAL)1 || res == 0u) return;
+  uint32_t cx = umin((uint32_t)(ux * (REAL)res), res - 1u), cz = umin((uint32_t)(uz * (REAL)res), res - 1u);
+  float t = (float)now_r(s);
+  uint32_t bits;
+  memcpy(&bits, &t, 4);
+  uint32_t* w = &s->touch[(size_t)cz * res + cx];
+  uint32_t cur = __atomic_load_n(w, __ATOMIC_RELAXED);
+  while (bits < cur && !__atomic_compare_exchange_n(w, &cur, bits, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+}
+
+/* break_up (fluid.wgsl:720-765): a particle that leaves the resolved flow becomes four drops
+ * and a parcel of finer drops, in five consecutive spray slots taken with an atomic add. */
+static void break_up(osim* s, const REAL x[3], const REAL v[3], uint32_t id) {
+  uint32_t first = __atomic_fetch_add(&s->st.spray_count, BREAKUP_DROPS + 1u, __ATOMIC_RELAXED);
+  uint32_t cap = s->spray_cap;
+  if (first >= cap) return;
+  uint32_t seed = hash3(s->st.step, id, 0xb4eau);
+  REAL hs = (REAL)s->U.fluid[3];
+  REAL vol = hs * hs * hs;
+  REAL speed = sqrt_r((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+  /* breakup_diameter (fluid.wgsl:713-716) */
+  REAL bd = ((REAL)WE_CRIT * (REAL)NATRAY_SIGMA_W) / ((REAL)RHO_AIR * rmax(speed * speed, (REAL)1e-4));
+  REAL d_med = rmin((REAL)0.55 * hs, (REAL)0.5 * bd);
+  REAL dd[BREAKUP_DROPS];
+  REAL used = 0;
+  for (uint32_t k = 0; k < BREAKUP_DROPS; ++k) {
+    dd[k] = rmin(rmax(d_med * r_exp((REAL)0.5 * rnd_gauss(&seed)), (REAL)2e-5), (REAL)1.24 * hs);
+    used += (REAL)0.5236 * dd[k] * dd[k] * dd[k];
+  }
+  REAL df = (REAL)0.5 * d_med;
+  int fine = df < (REAL)1e-4;
+  REAL fit = r_pow(vol / used, (REAL)1 / (REAL)3);
+  REAL scale = fine ? rmin((REAL)1, fit) : fit;
+  REAL tnow = now_r(s);
+  for (uint32_t k = 0; k < BREAKUP_DROPS; ++k) {
+    uint32_t slot = first + k;
+    if (slot >= cap) return;
+    REAL dk = dd[k] * scale;
+    REAL spread = sqrt_r((REAL)NATRAY_SIGMA_W / ((REAL)s->U.fluid[0] * dk)) + (REAL)0.05 * speed;
+    REAL g0 = rnd_gauss(&seed);
+    REAL g1 = rnd_gauss(&seed);
+    REAL g2 = rnd_gauss(&seed);
+    REAL w0 = g0 + (REAL)1e-6, w1 = g1 + (REAL)1e-6, w2 = g2 + (REAL)1e-6;
+    REAL len = sqrt_r((w0 * w0 + w1 * w1) + w2 * w2);
+    REAL u[3] = {w0 / len, w1 / len, w2 / len};
+    REAL o0 = rnd(&seed);
+    REAL o1 = rnd(&seed);
+    REAL o2 = rnd(&seed);
+    REAL rv = rnd(&seed);
+    REAL* rec = s->spray + (size_t)slot * 12;
+    rec[0] = x[0] + (o0 - (REAL)0.5) * hs;
+    rec[1] = x[1] + (o1 - (REAL)0.5) * hs;
+    rec[2] = x[2] + (o2 - (REAL)0.5) * hs;
+    rec[3] = (REAL)0.5 * dk;
+    for (int c = 0; c < 3; ++c) rec[4 + c] = v[c] + u[c] * spread * rv;
+    rec[7] = 1;
+    rec[8] = rec[9] = rec[10] = 0;
+    rec[11] = tnow;
+  }
+  uint32_t slot = first + BREAKUP_DROPS;
+  if (slot >= cap) return;
+  REAL n = (vol - used * scale * scale * scale) / ((REAL)0.5236 * df * df * df);
+  REAL* rec = s->spray + (size_t)slot * 12;
+  rec[0] = x[0];
+  rec[1] = x[1];
+  rec[2] = x[2];
+  rec[3] = (fine && n >= (REAL)2) ? (REAL)0.5 * df : (REAL)0;
+  rec[4] = v[0];
+  rec[5] = v[1];
+  rec[6] = v[2];
+  rec[7] = rmax(n, (REAL)2);
+  rec[8] = rec[9] = rec[10] = 0;
+  rec[11] = tnow;
+}
+
+/* drag_cd (fluid.wgsl:676-679) */
+static REAL drag_cd(REAL re) {
+  if (re < (REAL)1e-4) return (REAL)24 / (REAL)1e-4;
+  return (REAL)24 / re * ((REAL)1 + (REAL)0.15 * r_pow(re, (REAL)0.687)) +
+         (REAL)0.42 / ((REAL)1 + (REAL)42500 * r_pow(re, (REAL)-1.16));
+}
+
+/* spray_update (fluid.wgsl:686-711): drops fly through still air (implicit drag), evaporate,
+ * and land. Runs before g2p, so drops made in a step start moving in the next. */
+static void o_spray_update(osim* s) {
+  if (s->st.dt_ns == 0u || !s->spray) return;
+  uint32_t n = umin(s->st.spray_count, s->spray_cap);
+  REAL step = dt_(s), tnow = now_r(s);
+  REAL g[3] = {(REAL)s->U.gravity[0], (REAL)s->U.gravity[1], (REAL)s->U.gravity[2]};
+#pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < n; ++i) {
+    REAL* rec = s->spray + (size_t)i * 12;
+    REAL r = rec[3];
+    if (r <= (REAL)0) continue;
+    REAL sp = sqrt_r((rec[4] * rec[4] + rec[5] * rec[5]) + rec[6] * rec[6]);
+    REAL re = (REAL)2 * r * sp * (REAL)RHO_AIR / (REAL)MU_AIR;
+    REAL k = (REAL)3 * (REAL)RHO_AIR * drag_cd(re) * sp / ((REAL)8 * (REAL)s->U.fluid[0] * r);
+    REAL den = (REAL)1 + k * step;
+    REAL vn[3];
+    for (int c = 0; c < 3; ++c) vn[c] = (rec[4 + c] + step * g[c]) / den;
+    r = sqrt_r(rmax(r * r - (REAL)EVAPORATION * step, (REAL)0));
+    if (r < (REAL)2.5e-6) r = 0;
+    REAL pos[3];
+    for (int c = 0; c < 3; ++c) pos[c] = rec[c] + step * vn[c];
+    if (pos[1] <= r) {
+      REAL* l = s->landed + (size_t)i * 4;
+      l[0] = pos[0];
+      l[1] = pos[2];
+      l[2] = r * r_pow(rec[7], (REAL)1 / (REAL)3);
+      l[3] = tnow;
+      r = 0;
+    }
+    for (int c = 0; c < 3; ++c) {
+      rec[c] = pos[c];
+      rec[4 + c] = vn[c];
+    }
+    rec[3] = r;
+  }
+}
+
 static REAL g2p_one(osim* s, const REAL* tg, uint32_t i, i3 corner, const uint32_t links[27]) {
   REAL* p = &s->P[(size_t)i * 20];
   REAL h = hh_(s);
@@ -1436,7 +1610,13 @@ static REAL g2p_one(osim* s, const REAL* tg, uint32_t i, i3 corner, const uint32
   if ((s->sc.flags & NATRAY_SCENE_MASK) && in_mask(s, cf_)) covered = 1;
   REAL thin_age = (fill < (REAL)0.45 && !on_ground && !covered) ? p[15] + step : (REAL)0;
   if (escaped || (!covered && fill < (REAL)0.2) || thin_age > t_thin) {
-    /* ground_touch / break_up are out of scope: the particle only dies. */
+    if (s->spray) {
+      /* Too thin for the grid: on the ground it soaks in, elsewhere it breaks up into spray. */
+      if (on_ground)
+        ground_touch(s, x[0], x[2]);
+      else
+        break_up(s, x, v, i);
+    }
     p[4] = v[0];
     p[5] = v[1];
     p[6] = v[2];
@@ -1448,6 +1628,7 @@ static REAL g2p_one(osim* s, const REAL* tg, uint32_t i, i3 corner, const uint32
     xx[1] = (REAL)0.3 * spacing;
     vv[1] = rmax(vv[1], (REAL)0);
   }
+  if (s->spray && xx[1] < (REAL)0.7 * spacing) ground_touch(s, xx[0], xx[2]);
   REAL cs = (REAL)4 / (h * h);
   p[0] = xx[0];
   p[1] = xx[1];
@@ -1532,7 +1713,10 @@ int F(phase)(osim* s, int phase) {
       }
       o_mem_sample(s);
       break;
-    case NATRAY_PHASE_G2P: o_g2p(s); break;
+    case NATRAY_PHASE_G2P:
+      o_spray_update(s);
+      o_g2p(s);
+      break;
     default: return NATRAY_E_ARG;
   }
   return s->st.overflow ? NATRAY_E_OVERFLOW : NATRAY_OK;
@@ -1594,6 +1778,20 @@ osim* F(create)(const natray_params* params, const float* particles, uint32_t n,
       return NULL;
     }
   }
+  if (s->sc.flags & NATRAY_SCENE_SPRAY) {
+    /* src/sim.rs:252-254: spray and landed start zeroed, the wet map at u32::MAX. */
+    s->spray_cap = params->limits[1];
+    s->wet_res = params->limits[0];
+    size_t ww = (size_t)s->wet_res * s->wet_res;
+    s->spray = (REAL*)calloc((size_t)s->spray_cap * 12 + 1, sizeof(REAL));
+    s->landed = (REAL*)calloc((size_t)s->spray_cap * 4 + 1, sizeof(REAL));
+    s->touch = (uint32_t*)malloc((ww + 1) * 4);
+    if (!s->spray || !s->landed || !s->touch) {
+      F(destroy)(s);
+      return NULL;
+    }
+    memset(s->touch, 0xff, (ww + 1) * 4);
+  }
   if (s->sc.flags & NATRAY_SCENE_AIR) {
     s->air_x = (REAL*)calloc(5 * cells + 1, sizeof(REAL));
     s->air_tmp = (REAL*)calloc(5 * cells + 1, sizeof(REAL));
@@ -1645,6 +1843,9 @@ void F(destroy)(osim* s) {
   free(s->mem_push);
   free(s->mem_water);
   free(s->air_x);
+  free(s->spray);
+  free(s->landed);
+  free(s->touch);
   free(s->air_tmp);
   free(s->prev_ptr);
   free(s->block_list_prev);
@@ -1810,6 +2011,26 @@ int F(set_membrane)(osim* s, uint32_t n, const float* x, const float* v, const f
 
 void F(read_membrane)(const osim* s, float* water) {
   for (size_t k = 0; k < (size_t)mem_count(s) * 4; ++k) water[k] = (float)s->mem_water[k];
+}
+
+/* The spray's elementary functions on n floats (which: 0 exp, 1 log, 2 cos, 3 pow(a, b)), for
+ * the tests that pin them against the C library. */
+void F(elementary)(int which, const float* a, const float* b, float* out, uint32_t n) {
+  for (uint32_t k = 0; k < n; ++k)
+    out[k] = which == 0 ? nm_expf(a[k]) : which == 1 ? nm_logf(a[k]) : which == 2 ? nm_cosf(a[k]) : nm_powf(a[k], b[k]);
+}
+
+uint32_t F(read_spray)(const osim* s, uint32_t max_records, double* spray, double* landed) {
+  if (!s->spray) return 0;
+  uint32_t n = umin(umin(s->st.spray_count, s->spray_cap), max_records);
+  for (size_t k = 0; k < (size_t)n * 12; ++k) spray[k] = (double)s->spray[k];
+  for (size_t k = 0; k < (size_t)n * 4; ++k) landed[k] = (double)s->landed[k];
+  return n;
+}
+
+void F(read_wet)(const osim* s, uint32_t* touch) {
+  if (!s->touch) return;
+  memcpy(touch, s->touch, (size_t)s->wet_res * s->wet_res * 4);
 }
 
 void F(read_air)(const osim* s, float* out) {

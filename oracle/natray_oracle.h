@@ -46,6 +46,9 @@ typedef struct osim osim;
   void P##read_membrane(const osim* s, float* water);                                               \
   void P##pack_water(const osim* s, uint32_t* out);                                                 \
   void P##read_air(const osim* s, float* out);                                                      \
+  uint32_t P##read_spray(const osim* s, uint32_t max_records, double* spray, double* landed);       \
+  void P##read_wet(const osim* s, uint32_t* touch);                                                 \
+  void P##elementary(int which, const float* a, const float* b, float* out, uint32_t n);           \
   void P##set_threads(int n);
 
 ORACLE_DECL(o32_)
