@@ -953,13 +953,17 @@ template <class R, class Wait = NoWait>
 __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const double (*lpart)[4], R* out,
                        Wait wait = Wait()) {
   constexpr int H = Halves<R>::N;
+  // The chunk sums, then the totals and the abort flag, the last two per tag parity: a wave can
+  // leave once it has read them, and they are rewritten two reductions later, after barriers
+  // every wave has passed.
   __shared__ R red[4][256];
-  __shared__ uint32_t s_fail;
+  __shared__ R red_out[2][4];
+  __shared__ uint32_t s_fail[2];
   const uint32_t t = threadIdx.x;
   const int nw = nc > 0 ? nc : 1;
   const uint32_t buf = tag & 1u;
   uint32_t* abort = d.bar + 2;
-  if (t == 0) s_fail = 0u;
+  if (t == 0) s_fail[buf] = 0u;
   wg_sync(!sp.in_regs);
   // total()'s chunk fold, then publish: one word per chunk, component and half. One thread
   // per chunk and component, each summing its component over the chunk's blocks in order.
@@ -1015,7 +1019,7 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
       if ((++spins & 255u) == 0u &&
           (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
         __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_fail = 1u;
+        s_fail[buf] = 1u;
         break;
       }
     }
@@ -1033,27 +1037,22 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
   }
   wg_sync(!sp.in_regs);
   if (nc > 0) {
-    if (t < 128u) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (k < nc) red[k][t] += red[k][t + 128u];
-    }
-    wg_sync(!sp.in_regs);
+    // total()'s tree in one wave: lane l sums (c_l + c_l+128) + (c_l+64 + c_l+192), the tree's
+    // first two levels, then the 64 lanes.
     if (t < 64u) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (k >= nc) continue;
-        const R x = lane_tree<R>(red[k][t] + red[k][t + 64u]);
-        if (t == 0) red[k][0] = x;
+        const R x = lane_tree<R>((red[k][t] + red[k][t + 128u]) + (red[k][t + 64u] + red[k][t + 192u]));
+        if (t == 0) red_out[buf][k] = x;
       }
     }
     wg_sync(!sp.in_regs);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (k < nc) out[k] = red[k][0];
+      if (k < nc) out[k] = red_out[buf][k];
   }
-  const bool ok = s_fail == 0u;
-  wg_sync(!sp.in_regs);
+  const bool ok = s_fail[buf] == 0u;
   if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   return ok;
 }
