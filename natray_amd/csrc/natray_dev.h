@@ -55,7 +55,8 @@ struct Dev {
   uint32_t* bar;           // control words: [2] the solve's abort flag a stuck wait raises,
                            // [4] its last reduction tag, [8] / [9] the next block P2G / G2P
                            // hands out, [10] warm-start tiles this step, [11] 1 when the
-                           // solve starts from the warm start's guess (reset by k_begin)
+                           // solve starts from the warm start's guess, [12] particles that
+                           // broke up this step (reset by k_begin)
   unsigned long long* rwords;  // RED_WORDS: tagged chunk sums of the solve's reductions
   unsigned long long* hz;      // 2 x capc: tagged halo words of each water cell (register forms of
                                // the solve; the pipelined form alternates between the two halves)
@@ -99,6 +100,8 @@ struct Dev {
   // 0xffffffff untouched). Null without the flag.
   float* spray;
   float* landed;
+  float* breakups;  // N x 8: the particles G2P let break up this step (index bits, x, v, 0),
+                    // bar[12] of them, which k_break_up turns into spray
   uint32_t* touch;
   uint32_t spray_cap, wet_res;
   const float4* mem_x;
@@ -173,6 +176,8 @@ int launch_air_carry(const Target& t, void* stream);
 int launch_air_fill(const Target& t, void* stream);
 // Spray (one scene): the drops fly, evaporate and land; before G2P, which makes new ones.
 int launch_spray_update(const Target& t, void* stream);
+// Spray (one scene): the particles G2P let break up become drops; after G2P.
+int launch_break_up(const Target& t, void* stream);
 // The keyframe words of the water: two per particle into `out` (device memory, 8 N bytes).
 int launch_pack_water(const Target& t, void* stream, void* out);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also

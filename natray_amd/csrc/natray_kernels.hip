@@ -253,6 +253,7 @@ __global__ void k_begin(L l) {
   d.bar[9] = 0u;
   d.bar[10] = 0u;
   d.bar[11] = 0u;
+  d.bar[12] = 0u;
 }
 
 // The next block for a workgroup of a kernel that hands blocks out through counter `word` of
@@ -2507,8 +2508,9 @@ __device__ __forceinline__ void ground_touch(const Dev& d, float px, float pz) {
 }
 
 // break_up (fluid.wgsl:720-765): four drops and a parcel of finer drops, in five consecutive
-// spray slots.
-__device__ __noinline__ void break_up(const Dev& d, float x0, float x1, float x2, float v0, float v1, float v2, uint32_t id) {
+// spray slots. G2P lists the particles that break up and k_break_up runs this for them, so the
+// G2P kernel keeps no call frame of its own.
+__device__ __forceinline__ void break_up(const Dev& d, float x0, float x1, float x2, float v0, float v1, float v2, uint32_t id) {
   const uint32_t cap = d.spray_cap;
   const uint32_t first = atomicAdd(&d.st->spray_count, BREAKUP_DROPS + 1u);
   if (first >= cap) return;
@@ -2560,6 +2562,19 @@ __device__ __noinline__ void break_up(const Dev& d, float x0, float x1, float x2
   rec[0] = make_float4(x[0], x[1], x[2], (fine && n >= 2.0f) ? 0.5f * df : 0.0f);
   rec[1] = make_float4(v[0], v[1], v[2], fmax_(n, 2.0f));
   rec[2] = make_float4(0.0f, 0.0f, 0.0f, tnow);
+}
+
+template <class L>
+__global__ __launch_bounds__(256) void k_break_up(L l) {
+  const View vw = view(l);
+  const Dev& d = vw.d;
+  if (d.st->dt_ns == 0u || !d.spray) return;
+  const uint32_t n = d.bar[12];
+  for (uint32_t k = vw.wg * 256u + threadIdx.x; k < n; k += vw.wgs * 256u) {
+    const float4* e = reinterpret_cast<const float4*>(d.breakups) + (size_t)k * 2;
+    const float4 a = e[0], b = e[1];
+    break_up(d, a.y, a.z, a.w, b.x, b.y, b.z, __float_as_uint(a.x));
+  }
 }
 
 // drag_cd (fluid.wgsl:676-679)
@@ -2639,10 +2654,14 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
   if (escaped || (!covered && fill < 0.2f) || thin_age > t_thin) {
     if (d.spray) {
       // Too thin for the grid: on the ground it soaks in, elsewhere it breaks up into spray.
-      if (on_ground)
+      if (on_ground) {
         ground_touch(d, x[0], x[2]);
-      else
-        break_up(d, x[0], x[1], x[2], v[0], v[1], v[2], i);
+      } else {
+        const uint32_t k = atomicAdd(&d.bar[12], 1u);  // below N: a particle dies once a step
+        float4* e = reinterpret_cast<float4*>(d.breakups) + (size_t)k * 2;
+        e[0] = make_float4(__uint_as_float(i), x[0], x[1], x[2]);
+        e[1] = make_float4(v[0], v[1], v[2], 0.0f);
+      }
     }
     P[1] = make_float4(v[0], v[1], v[2], 0.0f);
     d.key[i] = NONE;
@@ -3138,6 +3157,10 @@ int launch_air_fill(const Target& t, void* stream) {
 }
 int launch_spray_update(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_spray_update, LK_CELLS, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+int launch_break_up(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_break_up, LK_CELLS, 256, t, S(stream));
   return (int)hipGetLastError();
 }
 int launch_pack_water(const Target& t, void* stream, void* out) {

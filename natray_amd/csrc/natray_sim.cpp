@@ -256,6 +256,7 @@ int enqueue_phase(natray_sim* s, int phase) {
       // spray_update runs before g2p (src/sim.rs:425), so drops made in a step move in the next.
       if (s->D.spray) LAUNCHCHK(natray::launch_spray_update(s->tgt, st));
       LAUNCHCHK(natray::launch_g2p(s->tgt, st));
+      if (s->D.spray) LAUNCHCHK(natray::launch_break_up(s->tgt, st));
       break;
     default: return fail(NATRAY_E_ARG, "unknown phase " + std::to_string(phase));
   }
@@ -464,6 +465,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
     TRY(dalloc(s, &D.spray, (size_t)D.spray_cap * 12));
     TRY(dalloc(s, &D.landed, (size_t)D.spray_cap * 4));
     TRY(dalloc(s, &D.touch, ww));
+    TRY(dalloc(s, &D.breakups, (size_t)std::max<uint32_t>(D.N, 1u) * 8));
     HIPTRY(hipMemsetAsync(D.spray, 0, (size_t)D.spray_cap * 48, s->stream));
     HIPTRY(hipMemsetAsync(D.landed, 0, (size_t)D.spray_cap * 16, s->stream));
     HIPTRY(hipMemsetAsync(D.touch, 0xff, ww * 4, s->stream));
