@@ -445,7 +445,7 @@ def main() -> int:
                     help="write the workload's scene file for natray_amd/natray_drive and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus}); "
               "the scene would shard by whole scenes or by Morton-ordered block ranges with halo exchange, "
               "which is not built.", file=sys.stderr)
         return 2

@@ -6,7 +6,7 @@ The simulator makes its own training set: with the warm start in mode 1 a step k
 16^3 tile with water, the network's inputs (x = initial residual / tile scale, cell kind), the
 solve's operator on the tile (diagonal, water-neighbour bits) and, after the solve, the target
 (solved - carried pressure) / tile scale. The network is trained with its own GPU kernels on
-the residual its guess leaves, x - A o: the quantity the step judges a guess by.
+the residual its guess leaves, x - A o: the quantity by which the step decides on a guess.
 """
 from __future__ import annotations
 

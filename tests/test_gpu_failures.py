@@ -1,5 +1,5 @@
 """The failure and limit paths of the substep through the C ABI, forced from the host, against
-the oracle (SURVEY's failure-detection row): block overflow is reported as NATRAY_E_OVERFLOW
+the oracle: block overflow is reported as NATRAY_E_OVERFLOW
 with the first `capacity` claimed blocks kept (fluid.wgsl:117-122, src/sim.rs:635); a CG cap
 below what the tolerance needs gives unconverged steps, bit for bit (fluid.wgsl:572-577); a
 stop time truncates the last step onto it and every step from there on is dead, changing

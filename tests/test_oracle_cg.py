@@ -342,7 +342,7 @@ def cells_from_water_rows(s, g, wc, base, vals):
 
 
 def test_residual_of_any_pressure_matches_independent_assembly():
-    """b - A p, which the warm start uses to judge its guess, for the carried pressure and for an
+    """b - A p, which the warm start uses to decide on its guess, for the carried pressure and for an
     arbitrary one, against the float64 assembly."""
     s, o, g, A, b, p0, wc, kc, dt = solve_setup()
     cap = int(s.params.bdims[3]) * 64
