@@ -323,12 +323,18 @@ __global__ __launch_bounds__(256) void k_mark(L l) {
     int bmx = (int)U->bdims[0] - 1, bmy = (int)U->bdims[1] - 1, bmz = (int)U->bdims[2] - 1;
     int lx = max(0, min((cx - 1) >> 2, bmx)), ly = max(0, min((cy - 1) >> 2, bmy)), lz = max(0, min((cz - 1) >> 2, bmz));
     int hx = max(0, min((cx + 2) >> 2, bmx)), hy = max(0, min((cy + 2) >> 2, bmy)), hz = max(0, min((cz + 2) >> 2, bmz));
-    for (int x = lx; x <= hx; ++x)
-      for (int y = ly; y <= hy; ++y)
-        for (int z = lz; z <= hz; ++z) {
-          uint32_t s = block_slot(U, x, y, z);
-          if (d.block_ptr[s] != PENDING) d.block_ptr[s] = PENDING;
-        }
+    // Cells c-1..c+2 span one or two blocks per axis. All of their slots are read before any
+    // is written, so the reads go out together instead of one round trip per slot.
+    uint32_t slot[8], cur[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int x = lx + (k >> 2), y = ly + ((k >> 1) & 1), z = lz + (k & 1);
+      slot[k] = (x <= hx && y <= hy && z <= hz) ? block_slot(U, x, y, z) : NONE;
+      cur[k] = slot[k] != NONE ? d.block_ptr[slot[k]] : PENDING;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (cur[k] != PENDING) d.block_ptr[slot[k]] = PENDING;
     own = block_slot(U, cx >> 2, cy >> 2, cz >> 2);
   }
   const uint32_t lane = lane_id();
