@@ -146,7 +146,7 @@ def warm_start_report(scene, args) -> dict:
     residual its guess leaves, then the timed steps again from the same initial state with the
     network's guess as the solve's start (plain CG to the same tolerance, the carried pressure
     when the guess does not lower the residual), against the run without it."""
-    from natray_amd import Sim, scenes
+    from natray_amd import Sim, abi, scenes
     from natray_amd import net as netmod
     from natray_amd import warm_start as ws
 
@@ -181,8 +181,15 @@ def warm_start_report(scene, args) -> dict:
     val_ratio = ws.residual_ratio(net, val, 64)
     del val
 
-    def run(with_net: bool) -> dict:
-        sim = Sim.from_scene(scene)
+    def run(with_net: bool, pipelined: bool = False) -> dict:
+        sc = scene.scene
+        keep = (sc.precision, sc.refresh_every)
+        if pipelined:
+            sc.precision, sc.refresh_every = abi.CG_PIPELINED, 10
+        try:
+            sim = Sim.from_scene(scene)
+        finally:
+            sc.precision, sc.refresh_every = keep
         if with_net:
             sim.set_warm_start(net, 2)
         wall, c, times, st = timed_run(sim, args.warmup, args.ws_steps, args.iters)
@@ -199,7 +206,7 @@ def warm_start_report(scene, args) -> dict:
                                                            if stats["carried_rr"] > 0 else None)
         return r
 
-    plain, learned = run(False), run(True)
+    plain, learned, combined = run(False), run(True), run(True, pipelined=True)
     return {
         "training": {"tiles": T, "seeds": seeds, "steps_per_seed": args.ws_collect_steps,
                      "collect_s": t_collect, "adam_steps": args.ws_train_steps, "batch_tiles": 32,
@@ -210,6 +217,9 @@ def warm_start_report(scene, args) -> dict:
         "cg_iters_saved_per_step": plain["cg_iters_per_step"] - learned["cg_iters_per_step"],
         "substep_time_ratio_warm_over_plain": learned["ms_per_step"] / plain["ms_per_step"],
         "solve_time_ratio_warm_over_plain": learned["solve_ms_per_step"] / plain["solve_ms_per_step"],
+        # Both added techniques together: the pipelined CG at mixed precision, warm-started.
+        "warm_started_pipelined_mixed": combined,
+        "solve_speedup_pipelined_mixed_warm_over_plain": plain["solve_ms_per_step"] / combined["solve_ms_per_step"],
         "note": "solve time includes the warm start (inputs, network, guess check); tol and b are the same",
     }
 

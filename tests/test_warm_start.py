@@ -106,7 +106,8 @@ def water_neighbour_bits(g: dict, bdims) -> np.ndarray:
     return out
 
 
-CASES = [("impact_sgs_32k", 0.1), ("impact_sgs_32k", -0.1), ("impact_mixed_32k_mem", 0.1)]
+CASES = [("impact_sgs_32k", 0.1), ("impact_sgs_32k", -0.1), ("impact_mixed_32k_mem", 0.1), ("impact_pipelined_32k", 0.1),
+         ("impact_pipelined_32k_mem", 0.1)]
 
 
 @pytest.mark.parametrize("name,c", CASES)
