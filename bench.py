@@ -58,8 +58,8 @@ def make_scene(args):
     from natray_amd import abi, scenes
 
     if args.workload == "impact":
-        prec = abi.CG_MIXED if args.precision == "mixed" else abi.CG_FP32
-        s = scenes.balloon_impact(precision=prec, iters=args.iters)
+        prec = {"fp32": abi.CG_FP32, "mixed": abi.CG_MIXED, "pipelined": abi.CG_PIPELINED}[args.precision]
+        s = scenes.balloon_impact(precision=prec, iters=args.iters, refresh_every=10 if args.precision == "pipelined" else 20)
         cfg = {
             "workload": "balloon impact: 2,097,152 particles, 256^3 sparse grid of 4^3 blocks, dx=1/256, "
                         "capsule r=0.01 at 300 m/s as moving solid, CFL 0.5, block-SGS CG tol 1e-6",
@@ -436,10 +436,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["impact", "sphere"], default="impact")
-    ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "mixed", "pipelined"], default="fp32")
     ap.add_argument("--iters", type=int, default=1000, help="CG iteration cap per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-compare", action="store_true", help="skip the mixed-precision comparison run")
+    ap.add_argument("--no-compare", action="store_true", help="skip the mixed-precision and pipelined comparison runs")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error report against the FP64 oracle")
     ap.add_argument("--accuracy-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

@@ -1,0 +1,40 @@
+"""bench.py's contract pieces that need no GPU: it refuses more than one GPU with a message, it
+writes the native driver's scene file for the profiles, and it finds the counter-measured HBM
+traffic of the solve in the committed profile summary."""
+import os
+import struct
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def run_bench(*args):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True, text=True,
+                          cwd=ROOT, timeout=300)
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = run_bench("--gpus", "2")
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr and r.stdout.strip() == ""
+
+
+def test_write_scene_file(tmp_path):
+    path = tmp_path / "sphere.scene"
+    r = run_bench("--workload", "sphere", "--write-scene", str(path))
+    assert r.returncode == 0, r.stderr
+    raw = path.read_bytes()
+    assert raw[:8] == b"NATRAYSC"
+    version, n = struct.unpack("<II", raw[8:16])
+    assert version == 1 and n == 32768
+    assert len(raw) > 16 + 192 + 96 + n * 80
+
+
+def test_solve_traffic_is_read_from_the_profile():
+    import bench
+
+    t = bench.measured_traffic("k_solve<false,One>")
+    assert t is not None and 1e8 < t < 1e9
+    assert bench.measured_traffic("no_such_kernel") is None
