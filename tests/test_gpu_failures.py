@@ -34,9 +34,10 @@ def test_overflow_is_reported():
     sim.close()
 
 
-def test_unconverged_steps_match_the_oracle():
-    print("[failures] CG capped at 2 iterations", flush=True)
-    s = SCENES["impact_sgs_32k"]()
+@pytest.mark.parametrize("name", ["impact_sgs_32k", "impact_pipelined_32k", "impact_pipelined_32k_mem"])
+def test_unconverged_steps_match_the_oracle(name):
+    print(f"[failures] CG capped at 2 iterations, {name}", flush=True)
+    s = SCENES[name]()
     sim = Sim.from_scene(s)
     orc = oracle.Oracle.from_scene(s, "f32")
     sim.run(3, iters=400)
