@@ -376,6 +376,45 @@ int natray_sim_read_spray(natray_sim* sim, uint32_t max_records, float* spray, f
  * first time (s) water touched each ground cell, 0xffffffff where none has. */
 int natray_sim_read_wet(natray_sim* sim, uint32_t* touch);
 
+/* ---- bullets -------------------------------------------------------------------------------
+ * The water's side of the reference's bullet model (shaders/bullet.wgsl): each step, after the
+ * grid is prepared, every bullet in the grid takes the pressure of the water or gas at each
+ * patch of its surface, the water takes the opposite push on its faces (bullet_step and
+ * bullet_apply, bullet.wgsl:187-386), the cavity the nose opens feeds the pressure solve as a
+ * source of volume while it is thinner than the grid (A_SRC, fluid.wgsl:481-482), and G2P moves
+ * particles out of it (carve, bullet.wgsl:390-424, fluid.wgsl:818). The bullet's clock joins
+ * begin_step's choice of dt (fluid.wgsl:36-50). Not part of this library: the flight through
+ * the air before the bullet arrives (src/flight.rs), the rubber's contact with it
+ * (rubber_contact, bullet.wgsl:519-611, part of the membrane's own substeps; its impulses and
+ * held rubber are zero here) and the renderer's pose log. */
+#define NATRAY_MAX_BULLETS 4u       /* MAX_SHOTS (src/params.rs:49) */
+#define NATRAY_BULLET_PATCHES 576u  /* MAX_PATCHES (src/bullet.rs:9) */
+#define NATRAY_BULLET_SECTIONS 32u  /* N_SEC (src/bullet.rs:10) */
+
+/* == Bullet (bullet.wgsl:16-34, GpuBullet src/bullet.rs:19-28), 112 bytes. */
+typedef struct natray_bullet {
+  float x[4];       /* centre of mass (m), w = mass (kg) */
+  float v[4];       /* velocity (m/s), w = roll angle (rad) */
+  float q[4];       /* body to world rotation (x, y, z, w); the body's +z runs base to nose */
+  float l[4];       /* angular momentum about the centre of mass, w = the pace dt allows for (m/s) */
+  float shape[4];   /* x axial moment of inertia, y transverse, z reach of the surface from the
+                       centre of mass (m), w radius (m) */
+  float reach[4];   /* centre and radius of a sphere around its thin cavity sections */
+  uint32_t info[4]; /* x arrival (sim ns), y patch count, z 0 on its way / 1 in the grid / 2 gone,
+                       w sections made so far */
+} natray_bullet;
+
+/* Sets n <= NATRAY_MAX_BULLETS bullets: bullets[n] as the flight hands them over, and
+ * patches[n][NATRAY_BULLET_PATCHES][8] each bullet's surface in its body frame (position xyz
+ * and area, outward normal xyz and 0; the first info.y are used), as src/bullet.rs:97-124
+ * builds it. Clears the cavity sections. Sets params.mem.w = n (src/sim.rs:299-311). One scene
+ * only (not in a batch). n = 0 removes them. */
+int natray_sim_set_bullets(natray_sim* sim, uint32_t n, const natray_bullet* bullets, const float* patches);
+/* The bullets as the last step left them, and each one's cavity sections,
+ * sections[n][NATRAY_BULLET_SECTIONS][12] (the reference's Section, bullet.wgsl:42-50: a.xyz
+ * and when made (s), b.xyz and d(R^2)/dt, R^2 at the start). Either array may be null. */
+int natray_sim_read_bullets(natray_sim* sim, natray_bullet* bullets, float* sections);
+
 /* With NATRAY_SCENE_AIR: each cell's gas after the last phase that wrote it, air[5][64 n] for
  * the n blocks in use (density kg/m^3, momentum x, y, z, total energy per volume J/m^3; 0 in
  * water and solid cells). The air's own substeps (HLL, air.wgsl) are not part of this library:

@@ -34,6 +34,7 @@ SCENE_GROUND, SCENE_KNOT, SCENE_WALLS, SCENE_CAPSULE, SCENE_MASK, SCENE_AIR, SCE
 SCENE_REFERENCE = SCENE_GROUND | SCENE_KNOT | SCENE_AIR | SCENE_SPRAY
 PRECOND_BLOCK_SGS, PRECOND_JACOBI = 0, 1
 CG_FP32, CG_MIXED, CG_PIPELINED = 0, 1, 2
+MAX_BULLETS, BULLET_PATCHES, BULLET_SECTIONS = 4, 576, 32
 
 OK, E_OVERFLOW, E_NOMEM, E_HIP, E_ARG, E_SOLVER = 0, -1, -2, -3, -4, -5
 
@@ -105,6 +106,24 @@ class State(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Bullet(C.Structure):
+    """== Bullet (bullet.wgsl:16-34, GpuBullet src/bullet.rs:19-28), 112 bytes."""
+
+    _fields_ = [
+        ("x", C.c_float * 4),
+        ("v", C.c_float * 4),
+        ("q", C.c_float * 4),
+        ("l", C.c_float * 4),
+        ("shape", C.c_float * 4),
+        ("reach", C.c_float * 4),
+        ("info", C.c_uint32 * 4),
+    ]
+
+    def as_array(self) -> np.ndarray:
+        """The 28 words as float32 (info reinterpreted)."""
+        return np.frombuffer(bytes(self), np.float32).copy()
+
+
 class Scene(C.Structure):
     """natray_scene: scene semantics beyond Params (solids, preconditioner, precision)."""
 
@@ -165,6 +184,8 @@ _SIGNATURES = {
     "natray_sim_read_air": (C.c_int, [_P, _P]),
     "natray_sim_read_spray": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "natray_sim_read_wet": (C.c_int, [_P, _P]),
+    "natray_sim_set_bullets": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "natray_sim_read_bullets": (C.c_int, [_P, _P, _P]),
     "natray_sim_read_membrane": (C.c_int, [_P, _P]),
     "natray_sim_ws_samples": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "natray_sim_ws_stats": (C.c_int, [_P, _P]),

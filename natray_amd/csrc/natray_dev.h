@@ -27,10 +27,6 @@ constexpr uint32_t MAX_CHUNKS = 256;
 constexpr uint32_t RED_RECORD = 32;  // words per chunk record
 constexpr uint32_t RED_WORDS = RED_RECORD * MAX_CHUNKS;
 
-// Acc fields the transfer writes and clearing must zero. Fields 9, 10 and 15..18 belong to the
-// membrane coupling and are cleared too when a membrane is set; 11..14 belong to the bullet
-// model, which is not part of this library, and stay zero.
-constexpr uint32_t ACC_CLEARED = 9;
 
 struct Dev {
   const natray_params* U;  // device copy of Params (the reference's uniform)
@@ -110,6 +106,12 @@ struct Dev {
   const float2* mem_push;
   float4* mem_water;
   uint32_t mem_cap;
+  // Bullets (natray_sim_set_bullets): n_bullets of them, their patches ([n][576][8]) and cavity
+  // sections ([n][32][12]); null and 0 without bullets.
+  natray_bullet* bullets;
+  const float* patches;
+  float* sections;
+  uint32_t n_bullets;
   const Dev* self;  // a device copy of this struct, for device functions that take it by pointer
   uint32_t N;
   uint32_t S;      // domain slots (product of bdims)
@@ -178,6 +180,8 @@ int launch_air_fill(const Target& t, void* stream);
 int launch_spray_update(const Target& t, void* stream);
 // Spray (one scene): the particles G2P let break up become drops; after G2P.
 int launch_break_up(const Target& t, void* stream);
+// Bullets (one scene): their step and the water's push (after air_fill, before the solve).
+int launch_bullets(const Target& t, void* stream);
 // The keyframe words of the water: two per particle into `out` (device memory, 8 N bytes).
 int launch_pack_water(const Target& t, void* stream, void* out);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also

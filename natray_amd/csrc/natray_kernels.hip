@@ -220,6 +220,18 @@ __global__ void k_begin(L l) {
     float vx = sc->capsule_v[0], vy = sc->capsule_v[1], vz = sc->capsule_v[2];
     vmax = fmax_(vmax, __builtin_sqrtf((vx * vx + vy * vy) + vz * vz));
   }
+  // A bullet's own pace counts from the step it joins in, which starts exactly when it arrives.
+  for (uint32_t k = 0; k < d.n_bullets; ++k) {
+    const natray_bullet& b = d.bullets[k];
+    if (b.info[2] == 1u) {
+      vmax = fmax_(vmax, b.l[3]);
+    } else if (b.info[2] == 0u) {
+      if (b.info[0] > st->t_ns)
+        arrive = arrive < b.info[0] ? arrive : b.info[0];
+      else
+        vmax = fmax_(vmax, __builtin_sqrtf((b.v[0] * b.v[0] + b.v[1] * b.v[1]) + b.v[2] * b.v[2]));
+    }
+  }
   float mem_dt = st->mem_dt, air_dt = st->air_dt;
   float dt_s = fmin_(U->fluid[2], st->cap_dt);
   if (vmax > 0.0f) dt_s = fmin_(dt_s, U->fluid[1] * U->origin[3] / vmax);
@@ -297,10 +309,12 @@ __global__ void k_clear(L l) {
       d.block_list_prev[b] = s;
       d.prev_ptr[s] = b;
     }
-  // The transfer fields, and with a membrane its coupling fields (A_MF, A_MA, A_MW, A_MV).
-  const uint32_t nf = ACC_CLEARED + (d.mem_cap ? 6u : 0u);
-  for (uint32_t k = 0; k < nf; ++k) {
-    const uint32_t f = k < ACC_CLEARED ? k : (k < ACC_CLEARED + 2u ? k : NATRAY_A_MW + (k - ACC_CLEARED - 2u));
+  // The transfer fields, with a membrane its coupling fields (A_MF, A_MA, A_MW, A_MV) and with
+  // bullets theirs (A_BU, A_SRC).
+  for (uint32_t f = 0; f < NATRAY_ACC_FIELDS; ++f) {
+    const bool mem_f = f == NATRAY_A_MF || f == NATRAY_A_MA || f >= NATRAY_A_MW;
+    const bool bullet_f = f >= NATRAY_A_BU && f <= NATRAY_A_SRC;
+    if ((mem_f && !d.mem_cap) || (bullet_f && !d.n_bullets)) continue;
     int4* a = reinterpret_cast<int4*>(d.acc + (size_t)f * d.capc);
     for (uint32_t i = v.wg * blockDim.x + threadIdx.x; i < n4; i += v.wgs * blockDim.x) a[i] = make_int4(0, 0, 0, 0);
   }
@@ -2620,7 +2634,88 @@ __global__ __launch_bounds__(256) void k_spray_update(L l) {
   }
 }
 
+// ---- bullets: vectors, cavity sections and carve (bullet.wgsl:85-177, 388-424) --------------------
+//
+// Every vector expression is written out in the order the WGSL source evaluates it (left to
+// right, no contraction), so that the CPU restatement gives the same bits.
+
+struct b3 {
+  float x, y, z;
+};
+__device__ __forceinline__ b3 operator+(b3 a, b3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ b3 operator-(b3 a, b3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ b3 operator*(b3 a, float k) { return {a.x * k, a.y * k, a.z * k}; }
+__device__ __forceinline__ b3 operator/(b3 a, float k) { return {a.x / k, a.y / k, a.z / k}; }
+__device__ __forceinline__ b3 adds(b3 a, float k) { return {a.x + k, a.y + k, a.z + k}; }
+__device__ __forceinline__ float dot3(b3 a, b3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ b3 cross3(b3 a, b3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len3(b3 a) { return __builtin_sqrtf(dot3(a, a)); }
+__device__ __forceinline__ b3 norm3(b3 a) { return a / len3(a); }
+__device__ __forceinline__ b3 xyz(const float* v) { return {v[0], v[1], v[2]}; }
+__device__ __forceinline__ b3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+// qrot and qmul (bullet.wgsl:91-98); quaternions are (x, y, z, w).
+__device__ __forceinline__ b3 qrot(float4 q, b3 v) {
+  const b3 qv = xyz(q);
+  const b3 t = cross3(qv, v) * 2.0f;
+  return (v + t * q.w) + cross3(qv, t);
+}
+__device__ __forceinline__ float4 qmul(float4 a, float4 b) {
+  const b3 r = (xyz(b) * a.w + xyz(a) * b.w) + cross3(xyz(a), xyz(b));
+  return make_float4(r.x, r.y, r.z, a.w * b.w - dot3(xyz(a), xyz(b)));
+}
+
+constexpr float R_HAND = 1.5f;  // a section this wide is the grid's (bullet.wgsl:85)
+
+// Section k of bullet bi: a.xyz and when made, b.xyz and d(R^2)/dt, R^2 at the start
+// (bullet.wgsl:42-50).
+__device__ __forceinline__ const float4* section_of(const Dev& d, uint32_t bi, uint32_t k) {
+  return reinterpret_cast<const float4*>(d.sections) + ((size_t)bi * NATRAY_BULLET_SECTIONS + k) * 3;
+}
+// section_r2 (bullet.wgsl:152-154)
+__device__ __forceinline__ float section_r2(const float4* s, float t) { return s[2].x + s[1].w * fmax_(t - s[0].w, 0.0f); }
+__device__ __forceinline__ float hand2_of(float hh) { return R_HAND * R_HAND * hh * hh; }
+
+// carve (bullet.wgsl:388-424): a particle inside a cavity section still thinner than the grid
+// goes onto its wall and takes at least the wall's speed.
+__device__ __forceinline__ void carve_cavity(const Dev& d, float x[3], float v[3], uint32_t id) {
+  const float t = now_of(d);
+  const float hand2 = hand2_of(d.U->origin[3]);
+  b3 X = xyz(x), V = xyz(v);
+  for (uint32_t bi = 0; bi < d.n_bullets; ++bi) {
+    const float* reach = d.bullets[bi].reach;
+    if (reach[3] <= 0.0f || len3(X - xyz(reach)) > reach[3]) continue;
+    for (uint32_t k = 0; k < NATRAY_BULLET_SECTIONS; ++k) {
+      const float4* s = section_of(d, bi, k);
+      const float r2 = section_r2(s, t);
+      if (s[1].w <= 0.0f || r2 >= hand2) continue;
+      const b3 a = xyz(s[0]), ab = xyz(s[1]) - a;
+      const float f = dot3(X - a, ab) / fmax_(dot3(ab, ab), 1e-20f);
+      if (f < 0.0f || f > 1.0f) continue;
+      const b3 on = a + ab * f;
+      b3 dd = X - on;
+      if (dot3(dd, dd) >= r2) continue;
+      if (dot3(dd, dd) < 1e-14f) {
+        uint32_t seed = hash3(id, d.st->step, k);
+        const float g0 = rnd_gauss(seed);
+        const float g1 = rnd_gauss(seed);
+        const float g2 = rnd_gauss(seed);
+        dd = cross3(adds(ab, 1e-9f), b3{g0, g1, g2});
+      }
+      const b3 dir = norm3(dd);
+      const float r = __builtin_sqrtf(r2);
+      X = on + dir * r;
+      const float wall = s[1].w / (2.0f * r);
+      V = V + dir * fmax_(wall - dot3(V, dir), 0.0f);
+    }
+  }
+  x[0] = X.x, x[1] = X.y, x[2] = X.z;
+  v[0] = V.x, v[1] = V.y, v[2] = V.z;
+}
+
 // g2p_one (fluid.wgsl:806-861); returns the particle's speed, 0 once it is gone.
+template <bool CAV>
 __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const uint32_t* links, uint32_t i, const int corner[3]) {
   const natray_params* U = d.U;
   const float hh = U->origin[3];
@@ -2640,6 +2735,7 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
   const float m_near = gu.w + gv.w + gw.w;
   const float step = dt_of(d);
   float x[3] = {pos.x + step * v[0], pos.y + step * v[1], pos.z + step * v[2]};
+  if (CAV) carve_cavity(d, x, v, i);
   if (d.sc->flags & NATRAY_SCENE_CAPSULE) carve(d, x, v);
   const float spacing = U->fluid[3];
   float lo[3], hi[3];
@@ -2689,7 +2785,7 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
   return __builtin_sqrtf((vv[0] * vv[0] + vv[1] * vv[1]) + vv[2] * vv[2]);
 }
 
-template <class L>
+template <class L, bool CAV = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(L l) {
   const View v = view(l);
   const Dev& d = v.d;
@@ -2723,7 +2819,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     int bx, by, bz;
     slot_coord(d.U, d.block_list[b], bx, by, bz);
     const int corner[3] = {bx * 4 - 1, by * 4 - 1, bz * 4 - 1};
-    for (uint32_t k = tid; k < count; k += 256) fastest = fmax_(fastest, g2p_one(d, tg, links, d.sorted[first + k], corner));
+    for (uint32_t k = tid; k < count; k += 256) fastest = fmax_(fastest, g2p_one<CAV>(d, tg, links, d.sorted[first + k], corner));
   }
   // The workgroup's fastest particle goes to its own slot; k_vmax takes the maximum. One
   // atomic per wave on State.max_speed would serialise thousands of device-scope atomics on
@@ -2905,6 +3001,301 @@ __global__ __launch_bounds__(256) void k_mem_sample(L l) {
   const float xc[3] = {x[0] - 0.5f, x[1] - 0.5f, x[2] - 0.5f};
   const float fill = sample_grid(d, xc, NATRAY_F_TYPE, NATRAY_A_WC).y / 8.0f;
   d.mem_water[i] = make_float4(vel[0], vel[1], vel[2], fill);
+}
+
+// ---- bullets: their step and the water's push (bullet.wgsl:100-386) ------------------------------
+//
+// One workgroup of 256 per bullet (the reference's own shape): each thread takes patches k,
+// k + 256, ..., the workgroup sums their forces in the reference's tree, and thread 0 steps the
+// body and records the cavity section the nose opened. The rubber's impulses and held rubber
+// (bullet_hit, written by rubber_contact in the membrane's substeps) are zero here.
+
+constexpr float LOGVINOVICH_K = 2.0f, CP_WAKE = -0.4f, CF_SKIN = 0.003f;
+
+// cp_patch (bullet.wgsl:100-111)
+__device__ __forceinline__ float cp_patch(float s) {
+  if (s < 0.0f) return CP_WAKE * s * s;
+  const float beta = nm_asinf(fmin_(s, 1.0f));
+  if (beta < PI_F / 12.0f) return 2.13f * s * s;
+  const float x = beta / PI_F - 0.25f;
+  return 0.5f + 1.81f * x - 2.0f * x * x;
+}
+
+// sample_pressure (bullet.wgsl:113-134): B-spline sums of gauge pressure and weight around x (in
+// cells, on the lattice of cell centres) over the cells that are not solid (nor water, for
+// gas_only).
+__device__ __forceinline__ float2 sample_pressure(const Dev& d, const float x[3], bool gas_only) {
+  int base[3];
+  float w[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float bf = floorf(x[k] - 0.5f);
+    base[k] = (int)bf;
+    axis_weights(x[k] - bf, w[k]);
+  }
+  float sx = 0.0f, sy = 0.0f;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        const uint32_t cell = cell_at(d, base[0] + a, base[1] + b, base[2] + c);
+        if (cell == NONE) continue;
+        const float ty = cfr(d, NATRAY_F_TYPE, cell);
+        if (ty == SOLID || (gas_only && ty == WATER)) continue;
+        const float wt = w[0][a] * w[1][b] * w[2][c];
+        sx = sx + wt * cfr(d, NATRAY_F_P, cell);
+        sy = sy + wt;
+      }
+  return make_float2(sx, sy);
+}
+
+// push_water (bullet.wgsl:136-150): j (N s) along axis dim to the water faces around X (cells).
+__device__ __forceinline__ void push_water(const Dev& d, const float X[3], uint32_t dim, float j, float sw) {
+  if (sw <= 1e-6f || j == 0.0f) return;
+  const natray_params* U = d.U;
+  int base[3];
+  float w[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float lx = X[k] - ((uint32_t)k == dim ? 0.0f : 0.5f);
+    const float bf = floorf(lx - 0.5f);
+    base[k] = (int)bf;
+    axis_weights(lx - bf, w[k]);
+  }
+  const float hs = U->fluid[3];
+  const float kk = j / (U->fluid[0] * hs * hs * hs * sw);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        const uint32_t cell = cell_at(d, base[0] + a, base[1] + b, base[2] + c);
+        if (cell == NONE || accr(d, NATRAY_A_WT(dim), cell) <= 0) continue;
+        atomicAdd(&accr(d, NATRAY_A_BU + dim, cell), to_fp(kk * w[0][a] * w[1][b] * w[2][c], NATRAY_FP_M));
+      }
+}
+
+// in_cavity (bullet.wgsl:156-170)
+__device__ __forceinline__ bool in_cavity(const Dev& d, uint32_t bi, b3 p, float t) {
+  const float hand2 = hand2_of(d.U->origin[3]);
+  for (uint32_t k = 0; k < NATRAY_BULLET_SECTIONS; ++k) {
+    const float4* s = section_of(d, bi, k);
+    if (s[1].w <= 0.0f || section_r2(s, t) >= hand2) continue;
+    const b3 a = xyz(s[0]), ab = xyz(s[1]) - a;
+    const float f = dot3(p - a, ab) / fmax_(dot3(ab, ab), 1e-20f);
+    if (f < 0.0f || f > 1.0f) continue;
+    const b3 dd = p - (a + ab * f);
+    if (dot3(dd, dd) < section_r2(s, t)) return true;
+  }
+  return false;
+}
+
+// bullet_step (bullet.wgsl:179-377)
+template <class L>
+__global__ __launch_bounds__(256) void k_bullet_step(L l) {
+  const View vw = view(l);
+  const Dev& d = vw.d;
+  if (d.st->dt_ns == 0u) return;
+  const natray_params* U = d.U;
+  const uint32_t bi = vw.wg, li = threadIdx.x;
+  natray_bullet* B = d.bullets + bi;
+  const natray_bullet b = *B;
+  const float step = dt_of(d), t_end = now_of(d), hh = U->origin[3];
+  const b3 origin = xyz(U->origin), bx = xyz(b.x), bv = xyz(b.v);
+  __shared__ uint32_t s_state;
+  __shared__ float s_pf;
+  __shared__ float4 red_f[256], red_t[256], red_c[256];
+  if (li == 0) {
+    uint32_t state = b.info[2];
+    // It joins at the first step that starts once it has arrived.
+    if (state == 0u && d.st->t_ns - d.st->dt_ns >= b.info[0]) state = 1u;
+    s_state = state;
+    const b3 ahead = bx + norm3(adds(bv, 1e-9f)) * (b.shape[2] + 3.0f * hh);
+    const b3 xa = (ahead - origin) / hh;
+    const float xs[3] = {xa.x - 0.5f, xa.y - 0.5f, xa.z - 0.5f};
+    const float2 sp = sample_pressure(d, xs, false);
+    s_pf = sp.y > 1e-6f ? sp.x / sp.y : 0.0f;
+  }
+  __syncthreads();
+  const uint32_t state = s_state;
+  if (state != 1u) {
+    if (li == 0) B->info[2] = state;
+    return;
+  }
+  const float pf = s_pf;
+  const float4 q = make_float4(b.q[0], b.q[1], b.q[2], b.q[3]);
+  // The spin, from the angular momentum and the inertia in the body frame.
+  const b3 lb = qrot(make_float4(-q.x, -q.y, -q.z, q.w), xyz(b.l));
+  const b3 w = qrot(q, b3{lb.x / b.shape[1], lb.y / b.shape[1], lb.z / b.shape[0]});
+  const float rho = U->fluid[0];
+  b3 f = {0.0f, 0.0f, 0.0f}, tq = {0.0f, 0.0f, 0.0f}, cen = {0.0f, 0.0f, 0.0f};
+  float cav_x = 0.0f, cav_y = 0.0f, cen_w = 3.0e38f;
+  const b3 vhat = norm3(adds(bv, 1e-9f));
+  const float4* pts = reinterpret_cast<const float4*>(d.patches) + (size_t)bi * NATRAY_BULLET_PATCHES * 2;
+  for (uint32_t k = li; k < b.info[1]; k += 256u) {
+    const float4 pp = pts[2 * k], pn = pts[2 * k + 1];
+    const b3 r = qrot(q, xyz(pp));
+    const b3 p = bx + r;
+    const b3 n = qrot(q, xyz(pn));
+    const float area = pp.w;
+    const b3 Xv = (p - origin) / hh;
+    const float X[3] = {Xv.x, Xv.y, Xv.z};
+    float uw[3], sw[3];
+#pragma unroll
+    for (uint32_t dim = 0; dim < 3; ++dim) {
+      const float xs[3] = {X[0] - (dim == 0 ? 0.0f : 0.5f), X[1] - (dim == 1 ? 0.0f : 0.5f), X[2] - (dim == 2 ? 0.0f : 0.5f)};
+      const float2 sg = sample_grid(d, xs, NATRAY_F_U + dim, NATRAY_A_WT(dim));
+      uw[dim] = sg.y > 1e-6f ? sg.x / sg.y : 0.0f;
+      sw[dim] = sg.y;
+    }
+    const float xc[3] = {X[0] - 0.5f, X[1] - 0.5f, X[2] - 0.5f};
+    float fill = fmin_(fmax_(sample_grid(d, xc, NATRAY_F_TYPE, NATRAY_A_WC).y / 8.0f, 0.0f), 1.0f);
+    if (in_cavity(d, bi, p, t_end - step)) fill = 0.0f;
+    const float2 gs = sample_pressure(d, xc, true);
+    const float p_gas = gs.y > 1e-6f ? gs.x / gs.y : 0.0f;
+    const b3 vp = bv + cross3(w, r);
+    const b3 rel = vp - b3{uw[0], uw[1], uw[2]};
+    const float sp = len3(rel);
+    const float wn = dot3(rel, n);
+    const float qd = 0.5f * rho * sp * sp;
+    const float p_water = fmax_(NATRAY_P_FLOOR, pf + qd * cp_patch(wn / fmax_(sp, 1e-9f)));
+    // The wet share of the patch: pressure, and skin friction along the surface.
+    b3 fw = n * (-fill * p_water * area);
+    const b3 rt = rel - n * wn;
+    const float rtl = len3(rt);
+    if (rtl > 1e-9f) fw = fw - (rt * (fill * CF_SKIN * 0.5f * rho * rtl)) * area;
+    const b3 fp = fw - n * ((1.0f - fill) * fmax_(p_gas, NATRAY_P_FLOOR) * area);
+    f = f + fp;
+    tq = tq + cross3(r, fp);
+    push_water(d, X, 0, -fw.x * step, sw[0]);
+    push_water(d, X, 1, -fw.y * step, sw[1]);
+    push_water(d, X, 2, -fw.z * step, sw[2]);
+    if (wn > 0.0f && fill > 0.0f) {
+      const float face = fill * area * fmax_(dot3(n, vhat), 0.0f);
+      cav_x = cav_x + -dot3(fw, vhat);
+      cav_y = cav_y + face;
+      cen = cen + p * face;
+      cen_w = fmin_(cen_w, dot3(r, vhat));
+    }
+  }
+  red_f[li] = make_float4(f.x, f.y, f.z, cav_x);
+  red_t[li] = make_float4(tq.x, tq.y, tq.z, cav_y);
+  red_c[li] = make_float4(cen.x, cen.y, cen.z, cen_w);
+  __syncthreads();
+  for (uint32_t off = 128u; off > 0u; off >>= 1u) {
+    if (li < off) {
+      const float4 a = red_f[li], c = red_f[li + off];
+      red_f[li] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + c.w);
+      const float4 e = red_t[li], g = red_t[li + off];
+      red_t[li] = make_float4(e.x + g.x, e.y + g.y, e.z + g.z, e.w + g.w);
+      const float4 m = red_c[li], o = red_c[li + off];
+      red_c[li] = make_float4(m.x + o.x, m.y + o.y, m.z + o.z, fmin_(m.w, o.w));
+    }
+    __syncthreads();
+  }
+  const float hand2 = hand2_of(hh);
+  if (li == 0) {
+    const float4 rf = red_f[0], rt4 = red_t[0], rc = red_c[0];
+    const float mass = b.x[3];
+    const b3 hit = {0.0f, 0.0f, 0.0f}, hit_l = {0.0f, 0.0f, 0.0f};
+    const b3 force = xyz(rf) + xyz(U->gravity) * mass;
+    const float drag = rf.w, face = rt4.w;
+    const b3 v = bv + (force * step + hit) / mass;
+    const b3 x = bx + v * step;
+    const b3 lnew = (xyz(b.l) + xyz(rt4) * step) + hit_l;
+    // A symmetric top turns about its angular momentum at |L| / Iy and about its own axis at
+    // the rest of its spin; taking the two exactly keeps the spin and the cone whatever the step.
+    const b3 axis0 = qrot(q, b3{0.0f, 0.0f, 1.0f});
+    const float spin_extra = dot3(lnew, axis0) * (1.0f / b.shape[0] - 1.0f / b.shape[1]) * step;
+    const float cone = len3(lnew) / b.shape[1] * step;
+    float4 qn = qmul(q, make_float4(0.0f, 0.0f, nm_sinf(0.5f * spin_extra), nm_cosf(0.5f * spin_extra)));
+    if (cone > 1e-12f) {
+      const b3 ax = norm3(lnew) * nm_sinf(0.5f * cone);
+      qn = qmul(make_float4(ax.x, ax.y, ax.z, nm_cosf(0.5f * cone)), qn);
+    }
+    const float ql = __builtin_sqrtf(((qn.x * qn.x + qn.y * qn.y) + qn.z * qn.z) + qn.w * qn.w);
+    qn = make_float4(qn.x / ql, qn.y / ql, qn.z / ql, qn.w / ql);
+    const float roll = b.v[3] + dot3(lnew, axis0) / b.shape[0] * step;
+    uint32_t info[4] = {b.info[0], b.info[1], 1u, b.info[3]};
+    const float xa[3] = {x.x, x.y, x.z};
+    for (int k = 0; k < 3; ++k)
+      if (xa[k] < U->origin[k] || xa[k] > U->origin[k] + (float)U->dims[k] * hh) info[2] = 2u;
+    // Out of reach of any water it can go a block per step: it cannot miss one on the way,
+    // since the blocks around it are checked every step.
+    const b3 xc = (x - origin) / hh;
+    const int cb[3] = {(int)floorf(xc.x) >> 2, (int)floorf(xc.y) >> 2, (int)floorf(xc.z) >> 2};
+    bool near = false;
+    for (int k = 0; k < 27; ++k) {
+      const int nb[3] = {cb[0] + k / 9 - 1, cb[1] + (k / 3) % 3 - 1, cb[2] + k % 3 - 1};
+      if (nb[0] >= 0 && nb[1] >= 0 && nb[2] >= 0 && nb[0] < (int)U->bdims[0] && nb[1] < (int)U->bdims[1] &&
+          nb[2] < (int)U->bdims[2] && d.block_ptr[block_slot(U, nb[0], nb[1], nb[2])] < n_blocks(d))
+        near = true;
+    }
+    const float pace = near ? len3(v) : len3(v) * U->fluid[1] / 4.0f;
+    float4* secs = reinterpret_cast<float4*>(d.sections) + (size_t)bi * NATRAY_BULLET_SECTIONS * 3;
+    if (drag > 0.0f && face > 0.0f) {
+      // The face's centre, moved back along the path to where the flow leaves it.
+      const b3 mid = xyz(rc) / face;
+      const b3 a = mid + vhat * (rc.w - dot3(mid - bx, vhat));
+      const b3 e = a + v * step;
+      float4* sn = secs + (size_t)(info[3] % NATRAY_BULLET_SECTIONS) * 3;
+      sn[0] = make_float4(a.x, a.y, a.z, t_end - 0.5f * step);
+      sn[1] = make_float4(e.x, e.y, e.z, 2.0f * __builtin_sqrtf(drag / (PI_F * rho * LOGVINOVICH_K)));
+      sn[2] = make_float4(face / PI_F, 0.0f, 0.0f, 0.0f);
+      info[3] += 1u;
+    }
+    float reach = 0.0f;
+    for (uint32_t k = 0; k < NATRAY_BULLET_SECTIONS; ++k) {
+      const float4* sk = secs + (size_t)k * 3;
+      if (sk[1].w <= 0.0f || section_r2(sk, t_end) >= hand2) continue;
+      reach = fmax_(reach, fmax_(len3(xyz(sk[0]) - x), len3(xyz(sk[1]) - x)));
+    }
+    B->x[0] = x.x, B->x[1] = x.y, B->x[2] = x.z;
+    B->v[0] = v.x, B->v[1] = v.y, B->v[2] = v.z, B->v[3] = roll;
+    B->q[0] = qn.x, B->q[1] = qn.y, B->q[2] = qn.z, B->q[3] = qn.w;
+    B->l[0] = lnew.x, B->l[1] = lnew.y, B->l[2] = lnew.z, B->l[3] = pace;
+    B->reach[0] = x.x, B->reach[1] = x.y, B->reach[2] = x.z, B->reach[3] = reach > 0.0f ? reach + R_HAND * hh : 0.0f;
+    for (int k = 0; k < 4; ++k) B->info[k] = info[k];
+  }
+  __syncthreads();
+  // Sections thinner than the grid push the water out of the way as they open.
+  if (li < NATRAY_BULLET_SECTIONS) {
+    const float4* sk = section_of(d, bi, li);
+    const float r2 = section_r2(sk, t_end);
+    if (sk[1].w > 0.0f && r2 < hand2) {
+      const b3 a = xyz(sk[0]), e = xyz(sk[1]);
+      const float qv = PI_F * sk[1].w * len3(e - a);
+      const b3 m = ((a + e) * 0.5f - origin) / hh;
+      const float x[3] = {m.x - 0.5f, m.y - 0.5f, m.z - 0.5f};
+      int base[3];
+      float wt[3][3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float bf = floorf(x[k] - 0.5f);
+        base[k] = (int)bf;
+        axis_weights(x[k] - bf, wt[k]);
+      }
+      for (int a3 = 0; a3 < 3; ++a3)
+        for (int b3i = 0; b3i < 3; ++b3i)
+          for (int c3 = 0; c3 < 3; ++c3) {
+            const uint32_t cell = cell_at(d, base[0] + a3, base[1] + b3i, base[2] + c3);
+            if (cell == NONE) continue;
+            atomicAdd(&accr(d, NATRAY_A_SRC, cell), to_fp(qv * wt[0][a3] * wt[1][b3i] * wt[2][c3], NATRAY_FP_Q));
+          }
+    }
+  }
+}
+
+// bullet_apply (bullet.wgsl:379-386): the bullets' pushes onto the water's faces.
+template <class L>
+__global__ __launch_bounds__(256) void k_bullet_apply(L l) {
+  const View vw = view(l);
+  const Dev& d = vw.d;
+  if (d.st->dt_ns == 0u) return;
+  const uint32_t cells = n_blocks(d) * 64u;
+  for (uint32_t cell = vw.wg * blockDim.x + threadIdx.x; cell < cells; cell += vw.wgs * blockDim.x)
+#pragma unroll
+    for (uint32_t dim = 0; dim < 3; ++dim) {
+      const int32_t du = accr(d, NATRAY_A_BU + dim, cell);
+      if (du != 0) cfr(d, NATRAY_F_U + dim, cell) += from_fp(du, NATRAY_FP_M);
+    }
 }
 
 // ---- the air boundary (air.wgsl:85-148, 271-277; fluid.wgsl:91-107 for the block maps) -----------
@@ -3139,7 +3530,10 @@ int launch_ws_samples(const Target& t, void* stream) {
   return (int)hipGetLastError();
 }
 int launch_g2p(const Target& t, void* stream) {
-  NATRAY_LAUNCH(k_g2p, LK_BLOCKS, 256, t, S(stream));
+  if (t.d && t.d->n_bullets)
+    hipLaunchKernelGGL((k_g2p<One, true>), dim3(t.grid[LK_BLOCKS]), dim3(256), 0, S(stream), One{*t.d});
+  else
+    NATRAY_LAUNCH(k_g2p, LK_BLOCKS, 256, t, S(stream));
   NATRAY_LAUNCH(k_vmax, LK_ONE, 256, t, S(stream));
   return (int)hipGetLastError();
 }
@@ -3167,6 +3561,12 @@ int launch_spray_update(const Target& t, void* stream) {
 }
 int launch_break_up(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_break_up, LK_CELLS, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+int launch_bullets(const Target& t, void* stream) {
+  if (!t.d || !t.d->n_bullets) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_bullet_step<One>, dim3(t.d->n_bullets), dim3(256), 0, S(stream), One{*t.d});
+  NATRAY_LAUNCH(k_bullet_apply, LK_CELLS, 256, t, S(stream));
   return (int)hipGetLastError();
 }
 int launch_pack_water(const Target& t, void* stream, void* out) {

@@ -1,15 +1,18 @@
-/* Elementary functions for the spray (exp, log, cos, pow), shared by the kernels and the CPU
- * oracle so that both give the same bits. WGSL leaves the accuracy of its exp, log, cos and pow
- * to the adapter (several ulp), so any implementation within a few ulp is as faithful to the
- * reference as the adapter's own.
+/* Elementary functions for the spray (exp, log, cos, pow) and the bullet (sin, cos, asin),
+ * shared by the kernels and the CPU oracle so that both give the same bits. WGSL leaves the
+ * accuracy of these functions to the adapter (several ulp), so any implementation within a few
+ * ulp is as faithful to the reference as the adapter's own.
  *
  * Each function is evaluated in double and rounded once to float:
  *  - exp: x = k ln2 + r with |r| <= ln2 / 2 (ln2 split in a high part exact in the product and
  *    a low part), exp(r) by its Taylor series to r^13 / 13!, scaled by 2^k;
  *  - log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m - 1) / (m + 1),
  *    |s| <= 0.172, by the atanh series to s^21 / 21, plus e ln2;
- *  - cos: x = q pi/2 + t with |t| <= pi/4 (pi/2 split high and low), then cos t or +-sin t by
- *    their Taylor series to t^18 / 18! and t^17 / 17!;
+ *  - cos and sin: x = q pi/2 + t with |t| <= pi/4 (pi/2 split high and low), then +-cos t or
+ *    +-sin t by their Taylor series to t^18 / 18! and t^17 / 17!;
+ *  - asin: for |x| <= 1/2 the series sum of (2n)! / (4^n n!^2 (2n+1)) x^(2n+1) to n = 29 (the
+ *    terms fall by x^2 <= 1/4 each); above, asin x = pi/2 - 2 asin(sqrt((1 - x) / 2)), whose
+ *    argument is at most 1/2 (1 - x is exact there);
  *  - pow(x, y) = exp(y log x) for x > 0, in double.
  * The double results are within ~1e-15 relative of the true values, so the float results are
  * the correctly rounded ones except when the true value lies within that of a rounding
@@ -33,6 +36,7 @@
 #define NM_PIO2_HI 1.57079632673412561417e+00 /* 0x3ff921fb54400000: q * NM_PIO2_HI is exact */
 #define NM_PIO2_LO 6.07710050650619224932e-11
 #define NM_2_PI 6.36619772367581382433e-01
+#define NM_PIO2 1.57079632679489655800e+00
 
 NM_FN double nm_exp_d(double x) {
   if (x != x) return x;
@@ -83,7 +87,8 @@ NM_FN double nm_log_d(double x) {
   return (double)e * NM_LN2 + 2.0 * s * p;
 }
 
-NM_FN double nm_cos_d(double x) {
+/* cos x for shift 0, sin x = cos(x - pi/2) for shift 1. */
+NM_FN double nm_trig_d(double x, int shift) {
   if (x != x || x == HUGE_VAL || x == -HUGE_VAL) return NAN;
   const double q = rint(x * NM_2_PI);
   const double t = (x - q * NM_PIO2_HI) - q * NM_PIO2_LO;
@@ -110,13 +115,34 @@ NM_FN double nm_cos_d(double x) {
   s = -1.0 / 6.0 + t2 * s;
   s = t + t * (t2 * s);
   const long long iq = (long long)q;
-  const int quad = (int)(((iq % 4) + 4) % 4);
+  const int quad = (int)((((iq - shift) % 4) + 4) % 4);
   return quad == 0 ? c : (quad == 1 ? -s : (quad == 2 ? -c : s));
+}
+NM_FN double nm_cos_d(double x) { return nm_trig_d(x, 0); }
+NM_FN double nm_sin_d(double x) { return nm_trig_d(x, 1); }
+
+/* The asin series for 0 <= x <= 1/2. */
+NM_FN double nm_asin_series(double x) {
+  const double x2 = x * x;
+  double a = x, sum = x;
+  for (int n = 0; n < 29; ++n) {
+    a = a * x2 * (double)(2 * n + 1) / (double)(2 * n + 2);
+    sum = sum + a / (double)(2 * n + 3);
+  }
+  return sum;
+}
+NM_FN double nm_asin_d(double x) {
+  if (x != x || x > 1.0 || x < -1.0) return NAN;
+  const double ax = x < 0.0 ? -x : x;
+  const double r = ax <= 0.5 ? nm_asin_series(ax) : NM_PIO2 - 2.0 * nm_asin_series(sqrt((1.0 - ax) * 0.5));
+  return x < 0.0 ? -r : r;
 }
 
 NM_FN float nm_expf(float x) { return (float)nm_exp_d((double)x); }
 NM_FN float nm_logf(float x) { return (float)nm_log_d((double)x); }
 NM_FN float nm_cosf(float x) { return (float)nm_cos_d((double)x); }
+NM_FN float nm_sinf(float x) { return (float)nm_sin_d((double)x); }
+NM_FN float nm_asinf(float x) { return (float)nm_asin_d((double)x); }
 /* pow(x, y) for x > 0 (the spray's only use); 0 for x == 0 and y > 0. */
 NM_FN float nm_powf(float x, float y) {
   if (x == 0.0f) return y > 0.0f ? 0.0f : (float)HUGE_VAL;

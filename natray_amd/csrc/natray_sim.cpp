@@ -231,6 +231,8 @@ int enqueue_phase(natray_sim* s, int phase) {
     case NATRAY_PHASE_GRID:
       LAUNCHCHK(natray::launch_grid_prep(s->tgt, st));
       if (s->D.air_x) LAUNCHCHK(natray::launch_air_fill(s->tgt, st));
+      // bullet_step and bullet_apply come after air_fill and before solve_init (src/sim.rs:406-409).
+      if (s->D.n_bullets) LAUNCHCHK(natray::launch_bullets(s->tgt, st));
       break;
     case NATRAY_PHASE_SOLVE: {
       // Tags are 32-bit: clear the tagged words well before a tag could repeat.
@@ -847,6 +849,55 @@ int natray_sim_set_membrane(natray_sim* s, uint32_t n, const float* x, const flo
   return NATRAY_OK;
 }
 
+int natray_sim_set_bullets(natray_sim* s, uint32_t n, const natray_bullet* bullets, const float* patches) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  if (n > NATRAY_MAX_BULLETS) return fail(NATRAY_E_ARG, "at most NATRAY_MAX_BULLETS bullets");
+  if (n && (!bullets || !patches)) return fail(NATRAY_E_ARG, "null argument");
+  for (uint32_t k = 0; k < n; ++k)
+    if (bullets[k].info[1] > NATRAY_BULLET_PATCHES) return fail(NATRAY_E_ARG, "a bullet has more than NATRAY_BULLET_PATCHES patches");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  natray::Dev& D = s->D;
+  if (n && !D.bullets) {
+    int e;
+    float* pt = nullptr;
+    if ((e = dalloc(s, &D.bullets, NATRAY_MAX_BULLETS)) || (e = dalloc(s, &pt, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 8)) ||
+        (e = dalloc(s, &D.sections, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 12)))
+      return e;
+    D.patches = pt;
+  }
+  if (n) {
+    HIPCHK(hipMemcpyAsync(D.bullets, bullets, (size_t)n * sizeof(natray_bullet), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(const_cast<float*>(D.patches), patches, (size_t)n * NATRAY_BULLET_PATCHES * 32,
+                          hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemsetAsync(D.sections, 0, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 48, s->stream));
+  }
+  // Without bullets nothing clears their acc fields any more, and the solve reads A_SRC.
+  if (!n && D.n_bullets)
+    HIPCHK(hipMemsetAsync(D.acc + (size_t)NATRAY_A_BU * D.capc, 0, (size_t)4 * D.capc * 4, s->stream));
+  // The bullet count is a launch argument (G2P's carve, the bullets' grid): the step graphs
+  // are captured again.
+  D.n_bullets = n;
+  HIPCHK(hipMemcpyAsync(const_cast<natray::Dev*>(D.self), &D, sizeof(natray::Dev), hipMemcpyHostToDevice, s->stream));
+  if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
+  if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+  s->graph_pre = s->graph_post = nullptr;
+  s->U.mem[3] = n;
+  int e = upload_params(s);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_read_bullets(natray_sim* s, natray_bullet* bullets, float* sections) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  const uint32_t n = s->D.n_bullets;
+  if (n && bullets) HIPCHK(hipMemcpyAsync(bullets, s->D.bullets, (size_t)n * sizeof(natray_bullet), hipMemcpyDeviceToHost, s->stream));
+  if (n && sections)
+    HIPCHK(hipMemcpyAsync(sections, s->D.sections, (size_t)n * NATRAY_BULLET_SECTIONS * 48, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
 int natray_sim_read_air(natray_sim* s, float* air) {
   if (!s || !air) return fail(NATRAY_E_ARG, "null argument");
   if (!s->D.air_x) return fail(NATRAY_E_ARG, "the scene has no air boundary (NATRAY_SCENE_AIR)");
@@ -1266,6 +1317,7 @@ int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out)
     if (sims[k]->ws_mode) return fail(NATRAY_E_ARG, "the warm start runs on single scenes, not in a batch");
     if (sims[k]->D.mem_cap) return fail(NATRAY_E_ARG, "the membrane coupling runs on single scenes, not in a batch");
     if (sims[k]->D.spray) return fail(NATRAY_E_ARG, "the spray runs on single scenes, not in a batch");
+    if (sims[k]->D.n_bullets) return fail(NATRAY_E_ARG, "bullets run on single scenes, not in a batch");
     for (uint32_t j = 0; j < k; ++j)
       if (sims[j] == sims[k]) return fail(NATRAY_E_ARG, "a scene appears twice in the batch");
   }

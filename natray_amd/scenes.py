@@ -421,3 +421,83 @@ def membrane_shell(s: Scene, n_verts: int = 2000, gap_cells: float = 0.6, push: 
     p[:, 0] = push
     p[:, 1] = area
     return x, v, mass, p
+
+
+def bullet_surface(length: float, radius: float, nose: float, u_cg: float, slices: int = 32, sectors: int = 16):
+    """A bullet's outline and surface in its body frame, as src/bullet.rs:58-124 cuts them: the
+    outline (a cylinder of `radius` with a cone of length `nose` at the front, base at u = 0,
+    tip at u = length) resampled at `slices` equal steps of arc length, each band between two
+    outline points cut into `sectors` patches at the band's mean radius, and a disk of sectors
+    at the flat base (centroids two thirds out). Returns (patches [576][8] float32: position
+    relative to the centre of mass at u_cg and area, outward normal and 0; count; hull, the
+    farthest surface point from the centre of mass)."""
+    def profile(u):
+        return radius if u <= length - nose else radius * (length - u) / nose
+    fine = np.linspace(0.0, length, 4001)
+    line = np.array([[u, profile(u)] for u in fine])
+    arc = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(line[:, 0]), np.diff(line[:, 1])))])
+    target = arc[-1] * np.arange(slices + 1) / slices
+    pts = np.stack([np.interp(target, arc, line[:, 0]), np.interp(target, arc, line[:, 1])], axis=1)
+    rows = []
+
+    def ring(u, r, radial, axial, area):
+        for j in range(sectors):
+            phi = (j + 0.5) * 2.0 * math.pi / sectors
+            c, sn = math.cos(phi), math.sin(phi)
+            rows.append([r * c, r * sn, u - u_cg, area / sectors, radial * c, radial * sn, axial, 0.0])
+
+    for (ua, ra), (ub, rb) in zip(pts[:-1], pts[1:]):
+        seg = math.hypot(ub - ua, rb - ra)
+        if seg <= 0.0:
+            continue
+        r = 0.5 * (ra + rb)
+        ring(0.5 * (ua + ub), r, (ub - ua) / seg, -(rb - ra) / seg, 2.0 * math.pi * r * seg)
+    ring(0.0, 2.0 / 3.0 * radius, 0.0, -1.0, math.pi * radius * radius)
+    assert len(rows) <= abi.BULLET_PATCHES
+    out = np.zeros((abi.BULLET_PATCHES, 8), np.float32)
+    out[: len(rows)] = np.array(rows, np.float32)
+    hull = float(np.max(np.hypot(pts[:, 0] - u_cg, pts[:, 1])))
+    return out, len(rows), hull
+
+
+def rotation_arc(n) -> tuple:
+    """The shortest rotation taking +z to unit vector n, as (x, y, z, w)."""
+    n = np.asarray(n, np.float64) / np.linalg.norm(n)
+    c = np.cross([0.0, 0.0, 1.0], n)
+    q = np.array([c[0], c[1], c[2], 1.0 + n[2]])
+    return tuple(q / np.linalg.norm(q))
+
+
+def bullet_shot(at=(0.25, 0.5, 0.5), direction=(1.0, 0.0, 0.0), speed: float = 300.0, arrive_ns: int = 0,
+                mass: float = 0.0042, radius: float = 0.0045, length: float = 0.02, nose: float = 0.008,
+                spin_rev_s: float = 3000.0, yaw: float = 0.02):
+    """One bullet as the flight hands it over (the role of src/bullet.rs:127-144 with a flight
+    of our own): at `at` moving along `direction` at `speed`, nose first, arriving at `arrive_ns`,
+    spinning at `spin_rev_s` about its axis, with a transverse angular momentum of `yaw` times
+    the spin's (so the axis cones). A solid of revolution of `mass`: moments of inertia of a
+    cylinder of `radius` and `length`. Returns (abi.Bullet, patches [576][8])."""
+    patches, count, hull = bullet_surface(length, radius, nose, 0.45 * length)
+    ix = 0.5 * mass * radius * radius
+    iy = mass * (3.0 * radius * radius + length * length) / 12.0
+    n = np.asarray(direction, np.float64) / np.linalg.norm(direction)
+    q = rotation_arc(n)
+    side = np.cross(n, [0.0, 1.0, 0.0])
+    side = side / np.linalg.norm(side) if np.linalg.norm(side) > 0 else np.array([1.0, 0.0, 0.0])
+    spin = ix * 2.0 * math.pi * spin_rev_s
+    h = spin * n + yaw * spin * side
+    b = abi.Bullet()
+    b.x[:] = [at[0], at[1], at[2], mass]
+    b.v[:] = [speed * n[0], speed * n[1], speed * n[2], 0.0]
+    b.q[:] = list(q)
+    b.l[:] = [h[0], h[1], h[2], speed]
+    b.shape[:] = [ix, iy, hull, radius]
+    b.info[:] = [arrive_ns, count, 0, 0]
+    return b, patches
+
+
+def bullet_scene(side: int = 64, n: int = 32768, seed: int = 1) -> Scene:
+    """A water sphere (32,768 particles on 64^3, solid walls, Jacobi CG) whose step follows the
+    CFL and the bullets' pace (no fixed dt), for bullet_shot's bullets."""
+    s = water_sphere(side=side, n=n, seed=seed, fixed_dt=0.0)
+    s.meta["workload"] = f"bullet into a water sphere {n} particles {side}^3"
+    return s

@@ -197,6 +197,24 @@ class Sim:
         abi.check(abi.lib().natray_sim_read_wet(self._h, abi.ptr(out)))
         return out
 
+    def set_bullets(self, bullets: list, patches: np.ndarray) -> None:
+        """Bullets for the coming steps (natray_sim_set_bullets): abi.Bullet records and their
+        patches [n][576][8] (position and area, normal and 0, body frame)."""
+        n = len(bullets)
+        arr = (abi.Bullet * max(n, 1))(*bullets)
+        pt = np.ascontiguousarray(patches, np.float32)
+        assert pt.shape == (n, abi.BULLET_PATCHES, 8)
+        abi.check(abi.lib().natray_sim_set_bullets(self._h, n, C.cast(arr, C.c_void_p), abi.ptr(pt)))
+        self._bullets_n = n
+
+    def bullets(self) -> tuple[list, np.ndarray]:
+        """The bullets as the last step left them and their cavity sections [n][32][12]."""
+        n = getattr(self, "_bullets_n", 0)
+        arr = (abi.Bullet * max(n, 1))()
+        sec = np.zeros((n, abi.BULLET_SECTIONS, 12), np.float32)
+        abi.check(abi.lib().natray_sim_read_bullets(self._h, C.cast(arr, C.c_void_p), abi.ptr(sec)))
+        return list(arr)[:n], sec
+
     def pack_water(self) -> np.ndarray:
         """The water's keyframe words (natray_sim_pack_water): [N][2] uint32."""
         out = np.zeros((self.n, 2), np.uint32)

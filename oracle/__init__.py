@@ -63,6 +63,8 @@ def lib() -> C.CDLL:
                 "read_spray": (C.c_uint32, [P, C.c_uint32, P, P]),
                 "read_wet": (None, [P, P]),
                 "elementary": (None, [C.c_int, P, P, P, C.c_uint32]),
+                "set_bullets": (C.c_int, [P, C.c_uint32, P, P]),
+                "read_bullets": (None, [P, P, P]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -89,11 +91,12 @@ def precondition(r: np.ndarray, diag: np.ndarray, water: np.ndarray, mode: int, 
 
 
 def elementary(which: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
-    """The spray's exp, log, cos or pow(a, b) in float32, as the kernels evaluate them."""
+    """The spray's exp, log, cos or pow(a, b), or the bullet's sin or asin, in float32, as the
+    kernels evaluate them."""
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(a if b is None else b, np.float32)
     out = np.empty_like(a)
-    lib().o32_elementary({"exp": 0, "log": 1, "cos": 2, "pow": 3}[which], abi.ptr(a), abi.ptr(b), abi.ptr(out), a.size)
+    lib().o32_elementary({"exp": 0, "log": 1, "cos": 2, "pow": 3, "sin": 4, "asin": 5}[which], abi.ptr(a), abi.ptr(b), abi.ptr(out), a.size)
     return out
 
 
@@ -236,6 +239,23 @@ class Oracle:
         out = np.zeros((res, res), np.uint32)
         self._f("read_wet")(self._h, abi.ptr(out))
         return out
+
+    def set_bullets(self, bullets: list, patches: np.ndarray) -> None:
+        """Bullets for the coming steps: abi.Bullet records and patches [n][576][8]."""
+        n = len(bullets)
+        arr = (abi.Bullet * max(n, 1))(*bullets)
+        pt = np.ascontiguousarray(patches, np.float32)
+        assert pt.shape == (n, abi.BULLET_PATCHES, 8)
+        assert self._f("set_bullets")(self._h, n, C.cast(arr, C.c_void_p), abi.ptr(pt)) == 0
+        self._bullets_n = n
+
+    def bullets(self) -> tuple[list, np.ndarray]:
+        """The bullets (as float32 records) and their cavity sections [n][32][12] as doubles."""
+        n = getattr(self, "_bullets_n", 0)
+        arr = (abi.Bullet * max(n, 1))()
+        sec = np.zeros((n, abi.BULLET_SECTIONS, 12), np.float64)
+        self._f("read_bullets")(self._h, C.cast(arr, C.c_void_p), abi.ptr(sec))
+        return list(arr)[:n], sec
 
     def pack_water(self) -> np.ndarray:
         out = np.zeros((self.n, 2), np.uint32)

@@ -116,6 +116,15 @@ struct osim {
   uint32_t* touch;
   uint32_t spray_cap, wet_res;
   uint32_t *prev_ptr, *block_list_prev, *block_active;
+  /* Bullets (set_bullets): n_bullets of them; their patches [n][576][8] and cavity sections
+   * [n][32][12] (bullet.wgsl:16-50). */
+  struct obullet {
+    REAL x[4], v[4], q[4], l[4], shape[4], reach[4];
+    uint32_t info[4];
+  } bullets[NATRAY_MAX_BULLETS];
+  REAL* patches;
+  REAL* sections;
+  uint32_t n_bullets;
 };
 
 /* ---- Params / State accessors (sim_defs.wgsl:159-231) -------------------------------- */
@@ -270,6 +279,18 @@ static void o_begin_step(osim* s) {
   if (s->sc.flags & NATRAY_SCENE_CAPSULE) {
     REAL vx = (REAL)s->sc.capsule_v[0], vy = (REAL)s->sc.capsule_v[1], vz = (REAL)s->sc.capsule_v[2];
     vmax = rmax(vmax, sqrt_r((vx * vx + vy * vy) + vz * vz));
+  }
+  /* A bullet's own pace counts from the step it joins in (fluid.wgsl:36-50). */
+  for (uint32_t k = 0; k < s->n_bullets; ++k) {
+    const struct obullet* b = &s->bullets[k];
+    if (b->info[2] == 1u) {
+      vmax = rmax(vmax, b->l[3]);
+    } else if (b->info[2] == 0u) {
+      if (b->info[0] > st->t_ns)
+        arrive = umin(arrive, b->info[0]);
+      else
+        vmax = rmax(vmax, sqrt_r((b->v[0] * b->v[0] + b->v[1] * b->v[1]) + b->v[2] * b->v[2]));
+    }
   }
   REAL mem_dt = (REAL)st->mem_dt, air_dt = (REAL)st->air_dt;
   REAL dt_s = rmin((REAL)s->U.fluid[2], (REAL)st->cap_dt);
@@ -1540,6 +1561,341 @@ static REAL drag_cd(REAL re) {
          (REAL)0.42 / ((REAL)1 + (REAL)42500 * r_pow(re, (REAL)-1.16));
 }
 
+/* ---- bullets (bullet.wgsl:85-424): the water's side of the reference's bullet model ---------------
+ *
+ * One bullet at a time, as a workgroup of 256 does it: "thread" li sums patches li, li + 256,
+ * ... in order, the 256 partial sums go through the reference's tree (bullet.wgsl:290-298), and
+ * the body is stepped from the root. The rubber's impulses (bullet_hit) are zero: rubber_contact
+ * belongs to the membrane's own substeps. Every vector expression follows the WGSL order. */
+
+#define B_SEC NATRAY_BULLET_SECTIONS
+#define R_HAND ((REAL)1.5)
+#define LOGVINOVICH_K ((REAL)2)
+#define CP_WAKE ((REAL)-0.4)
+#define CF_SKIN ((REAL)0.003)
+
+static inline r3 v3(REAL x, REAL y, REAL z) { r3 r = {x, y, z}; return r; }
+static inline r3 vxyz(const REAL* a) { return v3(a[0], a[1], a[2]); }
+static inline r3 vadd(r3 a, r3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline r3 vsub(r3 a, r3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline r3 vmul(r3 a, REAL k) { return v3(a.x * k, a.y * k, a.z * k); }
+static inline r3 vdiv(r3 a, REAL k) { return v3(a.x / k, a.y / k, a.z / k); }
+static inline r3 vadds(r3 a, REAL k) { return v3(a.x + k, a.y + k, a.z + k); }
+static inline REAL vdot(r3 a, r3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline r3 vcross(r3 a, r3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline REAL vlen(r3 a) { return sqrt_r(vdot(a, a)); }
+static inline r3 vnorm(r3 a) { return vdiv(a, vlen(a)); }
+/* qrot, qmul (bullet.wgsl:91-98); q = (x, y, z, w) */
+static inline r3 qrot(const REAL q[4], r3 v) {
+  r3 qv = vxyz(q);
+  r3 t = vmul(vcross(qv, v), (REAL)2);
+  return vadd(vadd(v, vmul(t, q[3])), vcross(qv, t));
+}
+static inline void qmul(const REAL a[4], const REAL b[4], REAL out[4]) {
+  r3 r = vadd(vadd(vmul(vxyz(b), a[3]), vmul(vxyz(a), b[3])), vcross(vxyz(a), vxyz(b)));
+  REAL w = a[3] * b[3] - vdot(vxyz(a), vxyz(b));
+  out[0] = r.x, out[1] = r.y, out[2] = r.z, out[3] = w;
+}
+static inline REAL r_sin(REAL x) { return IS_F64 ? (REAL)nm_sin_d((double)x) : (REAL)nm_sinf((float)x); }
+static inline REAL r_asin(REAL x) { return IS_F64 ? (REAL)nm_asin_d((double)x) : (REAL)nm_asinf((float)x); }
+static inline REAL* section_of(const osim* s, uint32_t bi, uint32_t k) { return &s->sections[((size_t)bi * B_SEC + k) * 12]; }
+/* section_r2 (bullet.wgsl:152-154): c.x + b.w max(t - a.w, 0) */
+static inline REAL section_r2(const REAL* sc, REAL t) { return sc[8] + sc[7] * rmax(t - sc[3], (REAL)0); }
+static inline REAL hand2_of(REAL hh) { return R_HAND * R_HAND * hh * hh; }
+
+/* cp_patch (bullet.wgsl:100-111) */
+static REAL cp_patch(REAL sn) {
+  if (sn < (REAL)0) return CP_WAKE * sn * sn;
+  REAL beta = r_asin(rmin(sn, (REAL)1));
+  if (beta < (REAL)PI_C / (REAL)12) return (REAL)2.13 * sn * sn;
+  REAL x = beta / (REAL)PI_C - (REAL)0.25;
+  return (REAL)0.5 + (REAL)1.81 * x - (REAL)2 * x * x;
+}
+
+/* sample_pressure (bullet.wgsl:113-134) */
+static void sample_pressure(const osim* s, const REAL x[3], int gas_only, REAL out[2]) {
+  REAL w[3][3];
+  int base[3];
+  for (int k = 0; k < 3; ++k) {
+    REAL bf = rfloor(x[k] - (REAL)0.5);
+    base[k] = (int)bf;
+    axis_weights(x[k] - bf, w[k]);
+  }
+  REAL sx = 0, sy = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        i3 cc = {base[0] + a, base[1] + b, base[2] + c};
+        uint32_t cell = cell_at(s, cc);
+        if (cell == NONE) continue;
+        REAL ty = cfv(s, NATRAY_F_TYPE, cell);
+        if (ty == SOLID || (gas_only && ty == WATER)) continue;
+        REAL wt = w[0][a] * w[1][b] * w[2][c];
+        sx = sx + wt * cfv(s, NATRAY_F_P, cell);
+        sy = sy + wt;
+      }
+  out[0] = sx;
+  out[1] = sy;
+}
+
+/* push_water (bullet.wgsl:136-150) */
+static void push_water(osim* s, const REAL X[3], uint32_t dim, REAL j, REAL sw) {
+  if (sw <= (REAL)1e-6 || j == (REAL)0) return;
+  REAL w[3][3];
+  int base[3];
+  for (int k = 0; k < 3; ++k) {
+    REAL lx = X[k] - ((uint32_t)k == dim ? (REAL)0 : (REAL)0.5);
+    REAL bf = rfloor(lx - (REAL)0.5);
+    base[k] = (int)bf;
+    axis_weights(lx - bf, w[k]);
+  }
+  REAL hs = (REAL)s->U.fluid[3];
+  REAL kk = j / ((REAL)s->U.fluid[0] * hs * hs * hs * sw);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        i3 cc = {base[0] + a, base[1] + b, base[2] + c};
+        uint32_t cell = cell_at(s, cc);
+        if (cell == NONE || accv(s, NATRAY_A_WT(dim), cell) <= 0) continue;
+        *accp(s, NATRAY_A_BU + dim, cell) += to_fp(kk * w[0][a] * w[1][b] * w[2][c], (REAL)NATRAY_FP_M);
+      }
+}
+
+/* in_cavity (bullet.wgsl:156-170) */
+static int in_cavity(const osim* s, uint32_t bi, r3 p, REAL t) {
+  REAL hand2 = hand2_of(hh_(s));
+  for (uint32_t k = 0; k < B_SEC; ++k) {
+    const REAL* sc = section_of(s, bi, k);
+    if (sc[7] <= (REAL)0 || section_r2(sc, t) >= hand2) continue;
+    r3 a = vxyz(sc), ab = vsub(vxyz(sc + 4), a);
+    REAL f = vdot(vsub(p, a), ab) / rmax(vdot(ab, ab), (REAL)1e-20);
+    if (f < (REAL)0 || f > (REAL)1) continue;
+    r3 d = vsub(p, vadd(a, vmul(ab, f)));
+    if (vdot(d, d) < section_r2(sc, t)) return 1;
+  }
+  return 0;
+}
+
+/* bullet_step (bullet.wgsl:179-377) for bullet bi */
+static void bullet_step_one(osim* s, uint32_t bi) {
+  struct obullet* B = &s->bullets[bi];
+  const struct obullet b = *B;
+  const REAL step = dt_(s), t_end = now_(s), hh = hh_(s);
+  const r3 origin = v3((REAL)s->U.origin[0], (REAL)s->U.origin[1], (REAL)s->U.origin[2]);
+  const r3 bx = vxyz(b.x), bv = vxyz(b.v);
+  uint32_t state = b.info[2];
+  if (state == 0u && s->st.t_ns - s->st.dt_ns >= b.info[0]) state = 1u;
+  REAL pf;
+  {
+    r3 ahead = vadd(bx, vmul(vnorm(vadds(bv, (REAL)1e-9)), b.shape[2] + (REAL)3 * hh));
+    r3 xa = vdiv(vsub(ahead, origin), hh);
+    REAL xs[3] = {xa.x - (REAL)0.5, xa.y - (REAL)0.5, xa.z - (REAL)0.5}, sp[2];
+    sample_pressure(s, xs, 0, sp);
+    pf = sp[1] > (REAL)1e-6 ? sp[0] / sp[1] : (REAL)0;
+  }
+  if (state != 1u) {
+    B->info[2] = state;
+    return;
+  }
+  const REAL conj[4] = {-b.q[0], -b.q[1], -b.q[2], b.q[3]};
+  const r3 lb = qrot(conj, vxyz(b.l));
+  const r3 w = qrot(b.q, v3(lb.x / b.shape[1], lb.y / b.shape[1], lb.z / b.shape[0]));
+  const REAL rho = (REAL)s->U.fluid[0];
+  const r3 vhat = vnorm(vadds(bv, (REAL)1e-9));
+  REAL red_f[256][4], red_t[256][4], red_c[256][4];
+  for (uint32_t li = 0; li < 256; ++li) {
+    r3 f = v3(0, 0, 0), tq = v3(0, 0, 0), cen = v3(0, 0, 0);
+    REAL cav_x = 0, cav_y = 0, cen_w = (REAL)3.0e38;
+    for (uint32_t k = li; k < b.info[1]; k += 256u) {
+      const REAL* pt = &s->patches[((size_t)bi * NATRAY_BULLET_PATCHES + k) * 8];
+      r3 r = qrot(b.q, vxyz(pt));
+      r3 p = vadd(bx, r);
+      r3 n = qrot(b.q, vxyz(pt + 4));
+      REAL area = pt[3];
+      r3 Xv = vdiv(vsub(p, origin), hh);
+      REAL X[3] = {Xv.x, Xv.y, Xv.z}, uw[3], sw[3], sg[2];
+      for (uint32_t dim = 0; dim < 3; ++dim) {
+        REAL xs[3] = {X[0] - (dim == 0 ? (REAL)0 : (REAL)0.5), X[1] - (dim == 1 ? (REAL)0 : (REAL)0.5),
+                      X[2] - (dim == 2 ? (REAL)0 : (REAL)0.5)};
+        sample_grid(s, xs, NATRAY_F_U + dim, NATRAY_A_WT(dim), sg);
+        uw[dim] = sg[1] > (REAL)1e-6 ? sg[0] / sg[1] : (REAL)0;
+        sw[dim] = sg[1];
+      }
+      REAL xc[3] = {X[0] - (REAL)0.5, X[1] - (REAL)0.5, X[2] - (REAL)0.5};
+      sample_grid(s, xc, NATRAY_F_TYPE, NATRAY_A_WC, sg);
+      REAL fill = rmin(rmax(sg[1] / (REAL)8, (REAL)0), (REAL)1);
+      if (in_cavity(s, bi, p, t_end - step)) fill = 0;
+      REAL gs[2];
+      sample_pressure(s, xc, 1, gs);
+      REAL p_gas = gs[1] > (REAL)1e-6 ? gs[0] / gs[1] : (REAL)0;
+      r3 vp = vadd(bv, vcross(w, r));
+      r3 rel = vsub(vp, v3(uw[0], uw[1], uw[2]));
+      REAL sp = vlen(rel);
+      REAL wn = vdot(rel, n);
+      REAL qd = (REAL)0.5 * rho * sp * sp;
+      REAL p_water = rmax((REAL)NATRAY_P_FLOOR, pf + qd * cp_patch(wn / rmax(sp, (REAL)1e-9)));
+      r3 fw = vmul(n, -fill * p_water * area);
+      r3 rt = vsub(rel, vmul(n, wn));
+      REAL rtl = vlen(rt);
+      if (rtl > (REAL)1e-9) fw = vsub(fw, vmul(vmul(rt, fill * CF_SKIN * (REAL)0.5 * rho * rtl), area));
+      r3 fp = vsub(fw, vmul(n, ((REAL)1 - fill) * rmax(p_gas, (REAL)NATRAY_P_FLOOR) * area));
+      f = vadd(f, fp);
+      tq = vadd(tq, vcross(r, fp));
+      push_water(s, X, 0, -fw.x * step, sw[0]);
+      push_water(s, X, 1, -fw.y * step, sw[1]);
+      push_water(s, X, 2, -fw.z * step, sw[2]);
+      if (wn > (REAL)0 && fill > (REAL)0) {
+        REAL face = fill * area * rmax(vdot(n, vhat), (REAL)0);
+        cav_x = cav_x + -vdot(fw, vhat);
+        cav_y = cav_y + face;
+        cen = vadd(cen, vmul(p, face));
+        cen_w = rmin(cen_w, vdot(r, vhat));
+      }
+    }
+    red_f[li][0] = f.x, red_f[li][1] = f.y, red_f[li][2] = f.z, red_f[li][3] = cav_x;
+    red_t[li][0] = tq.x, red_t[li][1] = tq.y, red_t[li][2] = tq.z, red_t[li][3] = cav_y;
+    red_c[li][0] = cen.x, red_c[li][1] = cen.y, red_c[li][2] = cen.z, red_c[li][3] = cen_w;
+  }
+  for (uint32_t off = 128u; off > 0u; off >>= 1u)
+    for (uint32_t li = 0; li < off; ++li)
+      for (int c = 0; c < 4; ++c) {
+        red_f[li][c] = red_f[li][c] + red_f[li + off][c];
+        red_t[li][c] = red_t[li][c] + red_t[li + off][c];
+        red_c[li][c] = c < 3 ? red_c[li][c] + red_c[li + off][c] : rmin(red_c[li][c], red_c[li + off][c]);
+      }
+  const REAL hand2 = hand2_of(hh);
+  {
+    const REAL mass = b.x[3];
+    const r3 hit = v3(0, 0, 0), hit_l = v3(0, 0, 0);
+    const r3 force = vadd(vxyz(red_f[0]), vmul(v3((REAL)s->U.gravity[0], (REAL)s->U.gravity[1], (REAL)s->U.gravity[2]), mass));
+    const REAL drag = red_f[0][3], face = red_t[0][3];
+    const r3 v = vadd(bv, vdiv(vadd(vmul(force, step), hit), mass));
+    const r3 x = vadd(bx, vmul(v, step));
+    const r3 lnew = vadd(vadd(vxyz(b.l), vmul(vxyz(red_t[0]), step)), hit_l);
+    const r3 axis0 = qrot(b.q, v3(0, 0, 1));
+    const REAL spin_extra = vdot(lnew, axis0) * ((REAL)1 / b.shape[0] - (REAL)1 / b.shape[1]) * step;
+    const REAL cone = vlen(lnew) / b.shape[1] * step;
+    REAL qs[4] = {0, 0, r_sin((REAL)0.5 * spin_extra), r_cos((REAL)0.5 * spin_extra)}, qn[4];
+    qmul(b.q, qs, qn);
+    if (cone > (REAL)1e-12) {
+      r3 ax = vmul(vnorm(lnew), r_sin((REAL)0.5 * cone));
+      REAL qa[4] = {ax.x, ax.y, ax.z, r_cos((REAL)0.5 * cone)}, t[4];
+      qmul(qa, qn, t);
+      memcpy(qn, t, sizeof t);
+    }
+    const REAL ql = sqrt_r(((qn[0] * qn[0] + qn[1] * qn[1]) + qn[2] * qn[2]) + qn[3] * qn[3]);
+    for (int k = 0; k < 4; ++k) qn[k] = qn[k] / ql;
+    const REAL roll = b.v[3] + vdot(lnew, axis0) / b.shape[0] * step;
+    uint32_t info[4] = {b.info[0], b.info[1], 1u, b.info[3]};
+    const REAL xa[3] = {x.x, x.y, x.z};
+    for (int k = 0; k < 3; ++k)
+      if (xa[k] < (REAL)s->U.origin[k] || xa[k] > (REAL)s->U.origin[k] + (REAL)s->U.dims[k] * hh) info[2] = 2u;
+    const r3 xc = vdiv(vsub(x, origin), hh);
+    const int cb[3] = {(int)rfloor(xc.x) >> 2, (int)rfloor(xc.y) >> 2, (int)rfloor(xc.z) >> 2};
+    int near = 0;
+    for (int k = 0; k < 27; ++k) {
+      i3 nb = {cb[0] + k / 9 - 1, cb[1] + (k / 3) % 3 - 1, cb[2] + k % 3 - 1};
+      if (nb.x >= 0 && nb.y >= 0 && nb.z >= 0 && nb.x < (int)s->U.bdims[0] && nb.y < (int)s->U.bdims[1] &&
+          nb.z < (int)s->U.bdims[2] && s->block_ptr[block_slot(s, nb)] < n_blocks_(s))
+        near = 1;
+    }
+    const REAL pace = near ? vlen(v) : vlen(v) * (REAL)s->U.fluid[1] / (REAL)4;
+    if (drag > (REAL)0 && face > (REAL)0) {
+      const r3 mid = vdiv(vxyz(red_c[0]), face);
+      const r3 a = vadd(mid, vmul(vhat, red_c[0][3] - vdot(vsub(mid, bx), vhat)));
+      const r3 e = vadd(a, vmul(v, step));
+      REAL* sn = section_of(s, bi, info[3] % B_SEC);
+      sn[0] = a.x, sn[1] = a.y, sn[2] = a.z, sn[3] = t_end - (REAL)0.5 * step;
+      sn[4] = e.x, sn[5] = e.y, sn[6] = e.z, sn[7] = (REAL)2 * sqrt_r(drag / ((REAL)PI_C * rho * LOGVINOVICH_K));
+      sn[8] = face / (REAL)PI_C, sn[9] = 0, sn[10] = 0, sn[11] = 0;
+      info[3] += 1u;
+    }
+    REAL reach = 0;
+    for (uint32_t k = 0; k < B_SEC; ++k) {
+      const REAL* sk = section_of(s, bi, k);
+      if (sk[7] <= (REAL)0 || section_r2(sk, t_end) >= hand2) continue;
+      reach = rmax(reach, rmax(vlen(vsub(vxyz(sk), x)), vlen(vsub(vxyz(sk + 4), x))));
+    }
+    B->x[0] = x.x, B->x[1] = x.y, B->x[2] = x.z;
+    B->v[0] = v.x, B->v[1] = v.y, B->v[2] = v.z, B->v[3] = roll;
+    for (int k = 0; k < 4; ++k) B->q[k] = qn[k];
+    B->l[0] = lnew.x, B->l[1] = lnew.y, B->l[2] = lnew.z, B->l[3] = pace;
+    B->reach[0] = x.x, B->reach[1] = x.y, B->reach[2] = x.z, B->reach[3] = reach > (REAL)0 ? reach + R_HAND * hh : (REAL)0;
+    for (int k = 0; k < 4; ++k) B->info[k] = info[k];
+  }
+  /* Sections thinner than the grid push the water out of the way as they open. */
+  for (uint32_t li = 0; li < B_SEC; ++li) {
+    const REAL* sk = section_of(s, bi, li);
+    REAL r2 = section_r2(sk, t_end);
+    if (!(sk[7] > (REAL)0 && r2 < hand2)) continue;
+    r3 a = vxyz(sk), e = vxyz(sk + 4);
+    REAL qv = (REAL)PI_C * sk[7] * vlen(vsub(e, a));
+    r3 m = vdiv(vsub(vmul(vadd(a, e), (REAL)0.5), origin), hh);
+    REAL x[3] = {m.x - (REAL)0.5, m.y - (REAL)0.5, m.z - (REAL)0.5}, wt[3][3];
+    int base[3];
+    for (int k = 0; k < 3; ++k) {
+      REAL bf = rfloor(x[k] - (REAL)0.5);
+      base[k] = (int)bf;
+      axis_weights(x[k] - bf, wt[k]);
+    }
+    for (int a3 = 0; a3 < 3; ++a3)
+      for (int b3 = 0; b3 < 3; ++b3)
+        for (int c3 = 0; c3 < 3; ++c3) {
+          i3 cc = {base[0] + a3, base[1] + b3, base[2] + c3};
+          uint32_t cell = cell_at(s, cc);
+          if (cell == NONE) continue;
+          *accp(s, NATRAY_A_SRC, cell) += to_fp(qv * wt[0][a3] * wt[1][b3] * wt[2][c3], (REAL)NATRAY_FP_Q);
+        }
+  }
+}
+
+/* bullet_step for every bullet, then bullet_apply (bullet.wgsl:379-386). */
+static void o_bullets(osim* s) {
+  if (s->st.dt_ns == 0u || !s->n_bullets) return;
+  for (uint32_t bi = 0; bi < s->n_bullets; ++bi) bullet_step_one(s, bi);
+  uint32_t nc = n_blocks_(s) * 64u;
+  for (uint32_t cell = 0; cell < nc; ++cell)
+    for (uint32_t d = 0; d < 3; ++d) {
+      int32_t du = accv(s, NATRAY_A_BU + d, cell);
+      if (du != 0) *cfp(s, NATRAY_F_U + d, cell) += from_fp(du, (REAL)NATRAY_FP_M);
+    }
+}
+
+/* carve (bullet.wgsl:388-424) */
+static void carve_cavity(const osim* s, REAL x[3], REAL v[3], uint32_t id) {
+  const REAL t = now_(s), hand2 = hand2_of(hh_(s));
+  r3 X = vxyz(x), V = vxyz(v);
+  for (uint32_t bi = 0; bi < s->n_bullets; ++bi) {
+    const REAL* reach = s->bullets[bi].reach;
+    if (reach[3] <= (REAL)0 || vlen(vsub(X, vxyz(reach))) > reach[3]) continue;
+    for (uint32_t k = 0; k < B_SEC; ++k) {
+      const REAL* sc = section_of(s, bi, k);
+      REAL r2 = section_r2(sc, t);
+      if (sc[7] <= (REAL)0 || r2 >= hand2) continue;
+      r3 a = vxyz(sc), ab = vsub(vxyz(sc + 4), a);
+      REAL f = vdot(vsub(X, a), ab) / rmax(vdot(ab, ab), (REAL)1e-20);
+      if (f < (REAL)0 || f > (REAL)1) continue;
+      r3 on = vadd(a, vmul(ab, f));
+      r3 d = vsub(X, on);
+      if (vdot(d, d) >= r2) continue;
+      if (vdot(d, d) < (REAL)1e-14) {
+        uint32_t seed = hash3(id, s->st.step, k);
+        REAL g0 = rnd_gauss(&seed);
+        REAL g1 = rnd_gauss(&seed);
+        REAL g2 = rnd_gauss(&seed);
+        d = vcross(vadds(ab, (REAL)1e-9), v3(g0, g1, g2));
+      }
+      r3 dir = vnorm(d);
+      REAL r = sqrt_r(r2);
+      X = vadd(on, vmul(dir, r));
+      REAL wall = sc[7] / ((REAL)2 * r);
+      V = vadd(V, vmul(dir, rmax(wall - vdot(V, dir), (REAL)0)));
+    }
+  }
+  x[0] = X.x, x[1] = X.y, x[2] = X.z;
+  v[0] = V.x, v[1] = V.y, v[2] = V.z;
+}
+
 /* spray_update (fluid.wgsl:686-711): drops fly through still air (implicit drag), evaporate,
  * and land. Runs before g2p, so drops made in a step start moving in the next. */
 static void o_spray_update(osim* s) {
@@ -1591,6 +1947,7 @@ static REAL g2p_one(osim* s, const REAL* tg, uint32_t i, i3 corner, const uint32
   REAL m_near = gu.w + gv.w + gw.w;
   REAL step = dt_(s);
   REAL x[3] = {p[0] + step * v[0], p[1] + step * v[1], p[2] + step * v[2]};
+  if (s->n_bullets) carve_cavity(s, x, v, i);
   if (s->sc.flags & NATRAY_SCENE_CAPSULE) carve(s, x, v);
   REAL spacing = (REAL)s->U.fluid[3];
   REAL lo[3], hi[3];
@@ -1703,6 +2060,7 @@ int F(phase)(osim* s, int phase) {
     case NATRAY_PHASE_GRID:
       o_grid_prep(s);
       o_air_fill(s);
+      o_bullets(s);
       break;
     case NATRAY_PHASE_SOLVE: o_solve(s); break;
     case NATRAY_PHASE_PROJECT: o_project(s); break;
@@ -1846,6 +2204,8 @@ void F(destroy)(osim* s) {
   free(s->spray);
   free(s->landed);
   free(s->touch);
+  free(s->patches);
+  free(s->sections);
   free(s->air_tmp);
   free(s->prev_ptr);
   free(s->block_list_prev);
@@ -2013,11 +2373,12 @@ void F(read_membrane)(const osim* s, float* water) {
   for (size_t k = 0; k < (size_t)mem_count(s) * 4; ++k) water[k] = (float)s->mem_water[k];
 }
 
-/* The spray's elementary functions on n floats (which: 0 exp, 1 log, 2 cos, 3 pow(a, b)), for
- * the tests that pin them against the C library. */
+/* The spray's and the bullet's elementary functions on n floats (which: 0 exp, 1 log, 2 cos,
+ * 3 pow(a, b), 4 sin, 5 asin), for the tests that pin them against the C library. */
 void F(elementary)(int which, const float* a, const float* b, float* out, uint32_t n) {
   for (uint32_t k = 0; k < n; ++k)
-    out[k] = which == 0 ? nm_expf(a[k]) : which == 1 ? nm_logf(a[k]) : which == 2 ? nm_cosf(a[k]) : nm_powf(a[k], b[k]);
+    out[k] = which == 0 ? nm_expf(a[k]) : which == 1 ? nm_logf(a[k]) : which == 2 ? nm_cosf(a[k])
+           : which == 3 ? nm_powf(a[k], b[k]) : which == 4 ? nm_sinf(a[k]) : nm_asinf(a[k]);
 }
 
 uint32_t F(read_spray)(const osim* s, uint32_t max_records, double* spray, double* landed) {
@@ -2026,6 +2387,45 @@ uint32_t F(read_spray)(const osim* s, uint32_t max_records, double* spray, doubl
   for (size_t k = 0; k < (size_t)n * 12; ++k) spray[k] = (double)s->spray[k];
   for (size_t k = 0; k < (size_t)n * 4; ++k) landed[k] = (double)s->landed[k];
   return n;
+}
+
+int F(set_bullets)(osim* s, uint32_t n, const natray_bullet* bullets, const float* patches) {
+  if (n > NATRAY_MAX_BULLETS) return NATRAY_E_ARG;
+  if (n && !s->patches) {
+    s->patches = (REAL*)calloc((size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 8, sizeof(REAL));
+    s->sections = (REAL*)calloc((size_t)NATRAY_MAX_BULLETS * B_SEC * 12, sizeof(REAL));
+    if (!s->patches || !s->sections) return NATRAY_E_NOMEM;
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    const natray_bullet* b = &bullets[k];
+    struct obullet* o = &s->bullets[k];
+    for (int c = 0; c < 4; ++c) {
+      o->x[c] = (REAL)b->x[c], o->v[c] = (REAL)b->v[c], o->q[c] = (REAL)b->q[c], o->l[c] = (REAL)b->l[c];
+      o->shape[c] = (REAL)b->shape[c], o->reach[c] = (REAL)b->reach[c], o->info[c] = b->info[c];
+    }
+  }
+  if (n) {
+    for (size_t i = 0; i < (size_t)n * NATRAY_BULLET_PATCHES * 8; ++i) s->patches[i] = (REAL)patches[i];
+    memset(s->sections, 0, (size_t)NATRAY_MAX_BULLETS * B_SEC * 12 * sizeof(REAL));
+  }
+  if (!n && s->n_bullets)
+    memset(&s->acc[(size_t)NATRAY_A_BU * cap_cells(s)], 0, (size_t)4 * cap_cells(s) * 4);
+  s->n_bullets = n;
+  s->U.mem[3] = n;
+  return NATRAY_OK;
+}
+
+void F(read_bullets)(const osim* s, natray_bullet* bullets, double* sections) {
+  for (uint32_t k = 0; k < s->n_bullets; ++k) {
+    const struct obullet* o = &s->bullets[k];
+    natray_bullet* b = &bullets[k];
+    for (int c = 0; c < 4; ++c) {
+      b->x[c] = (float)o->x[c], b->v[c] = (float)o->v[c], b->q[c] = (float)o->q[c], b->l[c] = (float)o->l[c];
+      b->shape[c] = (float)o->shape[c], b->reach[c] = (float)o->reach[c], b->info[c] = o->info[c];
+    }
+  }
+  if (sections)
+    for (size_t i = 0; i < (size_t)s->n_bullets * B_SEC * 12; ++i) sections[i] = (double)s->sections[i];
 }
 
 void F(read_wet)(const osim* s, uint32_t* touch) {
