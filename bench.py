@@ -108,6 +108,39 @@ def timed_run(sim, warmup: int, steps: int, iters: int, graph: bool = True):
     return t1 - t0, {k: c1[k] - c0[k] for k in c0}, times, st
 
 
+def bullet_report(args) -> dict:
+    """The reference's bullet model in place of the capsule solid: the 2M impact without the
+    capsule, with one bullet (4.2 g, 9 mm across, 20 mm long, 300 m/s along +x, its nose two
+    cells short of the water), kernel by kernel over `--bullet-steps` steps, against the same
+    scene without the bullet. The grid phase holds grid_prep, bullet_step and bullet_apply, so
+    the difference of the two grid phases is what the bullet costs a step."""
+    from natray_amd import Sim, abi, scenes
+
+    s = scenes.balloon_impact(iters=args.iters)
+    s.scene.flags &= ~abi.SCENE_CAPSULE
+    nose = 0.5 - float(s.meta["r_eff"]) - 2.0 * float(s.meta["dx"])
+    b, patches = scenes.bullet_shot(at=(nose - 0.011, 0.5, 0.5), speed=300.0)
+    out = {"workload": f"{s.meta['workload']}, one bullet at 300 m/s instead of the capsule",
+           "steps": args.bullet_steps}
+    for label, with_bullet in (("with_bullet", True), ("without", False)):
+        sim = Sim.from_scene(s)
+        if with_bullet:
+            sim.set_bullets([b], patches[None])
+        wall, c, times, st = timed_run(sim, args.warmup, args.bullet_steps, args.iters, graph=False)
+        steps = max(c["steps"], 1)
+        out[label] = {"ms_per_step_kernel_by_kernel": wall * 1e3 / max(args.bullet_steps, 1),
+                      "grid_phase_us_per_step": times["grid_prep"][0] * 1e3 / steps,
+                      "cg_iters_per_step": c["cg_iters"] / steps, "dt_ns_last": st.dt_ns}
+        if with_bullet:
+            (B,), sec = sim.bullets()
+            out[label]["bullet"] = {"x": list(B.x[:3]), "speed": float(np.linalg.norm(list(B.v[:3]))),
+                                    "state": int(B.info[2]), "sections_made": int(B.info[3]),
+                                    "reach": float(B.reach[3])}
+        sim.close()
+    out["bullet_us_per_step"] = out["with_bullet"]["grid_phase_us_per_step"] - out["without"]["grid_phase_us_per_step"]
+    return out
+
+
 def compare_precision(scene, args, plain: dict) -> dict:
     """The same workload from the same initial state with the other CG forms, at the same
     tolerance in the reference's convergence form: mixed precision (FP32 SpMV and axpy, FP64
@@ -451,6 +484,9 @@ def main() -> int:
     ap.add_argument("--ws-collect-steps", type=int, default=100, help="steps logged per training seed")
     ap.add_argument("--ws-train-steps", type=int, default=6000)
     ap.add_argument("--ws-steps", type=int, default=100, help="timed steps with and without the warm start")
+    ap.add_argument("--no-bullets", action="store_true", help="skip the bullet report")
+    ap.add_argument("--bullet-steps", type=int, default=40, help="steps of the bullet report")
+    ap.add_argument("--bullets-only", action="store_true", help="print the bullet report alone")
     ap.add_argument("--write-scene", metavar="PATH",
                     help="write the workload's scene file for natray_amd/natray_drive and exit")
     args = ap.parse_args()
@@ -462,6 +498,9 @@ def main() -> int:
 
     from natray_amd import Sim, scenes
 
+    if args.bullets_only:
+        print(json.dumps({"bullets": bullet_report(args)}), flush=True)
+        return 0
     scene, cfg = make_scene(args)
     if args.write_scene:
         scenes.write_scene_file(scene, args.write_scene)
@@ -542,6 +581,8 @@ def main() -> int:
         out["batched_scenes"] = batched_report(args)
     if args.workload == "impact" and not args.no_warm_start:
         out["learned_warm_start"] = warm_start_report(scene, args)
+    if args.workload == "impact" and not args.no_bullets:
+        out["bullets"] = bullet_report(args)
     if args.workload == "impact" and not args.no_accuracy:
         out["accuracy_vs_fp64_oracle"] = accuracy_vs_fp64(scene, (1, args.accuracy_steps), args.iters)
     if not args.no_cpu:

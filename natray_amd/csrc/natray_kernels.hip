@@ -3021,75 +3021,210 @@ __device__ __forceinline__ float cp_patch(float s) {
   return 0.5f + 1.81f * x - 2.0f * x * x;
 }
 
-// sample_pressure (bullet.wgsl:113-134): B-spline sums of gauge pressure and weight around x (in
-// cells, on the lattice of cell centres) over the cells that are not solid (nor water, for
-// gas_only).
-__device__ __forceinline__ float2 sample_pressure(const Dev& d, const float x[3], bool gas_only) {
+// bullet_step (bullet.wgsl:179-377) in two kernels with the reference's bits. The patches are
+// latency-bound (each samples the grid at about 250 stencil cells), so k_bullet_patches gives
+// every patch a wave whose lanes 0..26 take the 27 cells of a stencil at once; the sums are then
+// taken in the sequential loop's order from the lanes (readlane), so they round as the loop
+// does. Each patch leaves a record; k_bullet_body sums the records the way the reference's
+// workgroup of 256 does (lane li takes patches li, li + 256, ... in order, then the tree) and
+// steps the body.
+
+// The 27 cells of a stencil around x (in cells, on a lattice), lane l < 27 taking cell
+// (l / 9, l / 3 % 3, l % 3): its B-spline weights and its index (NONE where there is none).
+struct Stencil {
   int base[3];
   float w[3][3];
+};
+__device__ __forceinline__ Stencil stencil_of(const float x[3]) {
+  Stencil st;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float bf = floorf(x[k] - 0.5f);
-    base[k] = (int)bf;
-    axis_weights(x[k] - bf, w[k]);
+    st.base[k] = (int)bf;
+    axis_weights(x[k] - bf, st.w[k]);
   }
+  return st;
+}
+__device__ __forceinline__ uint32_t stencil_cell(const Dev& d, const Stencil& st, uint32_t lane, int& a, int& b, int& c) {
+  a = (int)(lane / 9u), b = (int)((lane / 3u) % 3u), c = (int)(lane % 3u);
+  if (lane >= 27u) return NONE;
+  return cell_at(d, st.base[0] + a, st.base[1] + b, st.base[2] + c);
+}
+// sum over lanes 0..26 in lane order of (v, w) where `present`: the bits of the sequential
+// sx = sx + v, sy = sy + w over the stencil. Wave-uniform result.
+__device__ __forceinline__ float2 ordered_sum27(float v, float w, bool present) {
+  const unsigned long long m = __ballot(present);
   float sx = 0.0f, sy = 0.0f;
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      for (int c = 0; c < 3; ++c) {
-        const uint32_t cell = cell_at(d, base[0] + a, base[1] + b, base[2] + c);
-        if (cell == NONE) continue;
-        const float ty = cfr(d, NATRAY_F_TYPE, cell);
-        if (ty == SOLID || (gas_only && ty == WATER)) continue;
-        const float wt = w[0][a] * w[1][b] * w[2][c];
-        sx = sx + wt * cfr(d, NATRAY_F_P, cell);
-        sy = sy + wt;
-      }
+#pragma unroll
+  for (int i = 0; i < 27; ++i)
+    if ((m >> i) & 1ull) {
+      sx = sx + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+      sy = sy + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), i));
+    }
   return make_float2(sx, sy);
 }
-
-// push_water (bullet.wgsl:136-150): j (N s) along axis dim to the water faces around X (cells).
-__device__ __forceinline__ void push_water(const Dev& d, const float X[3], uint32_t dim, float j, float sw) {
+// sample_grid (membrane.wgsl:218-237) by one wave.
+__device__ __forceinline__ float2 wave_sample_grid(const Dev& d, const float x[3], uint32_t fu, uint32_t fw, uint32_t lane) {
+  const Stencil st = stencil_of(x);
+  int a, b, c;
+  const uint32_t cell = stencil_cell(d, st, lane, a, b, c);
+  float v = 0.0f, w = 0.0f;
+  if (cell != NONE) {
+    w = st.w[0][a] * st.w[1][b] * st.w[2][c] * from_fp(accr(d, fw, cell), NATRAY_FP_W);
+    v = w * cfr(d, fu, cell);
+  }
+  return ordered_sum27(v, w, cell != NONE);
+}
+// sample_pressure (bullet.wgsl:113-134) by one wave.
+__device__ __forceinline__ float2 wave_sample_pressure(const Dev& d, const float x[3], bool gas_only, uint32_t lane) {
+  const Stencil st = stencil_of(x);
+  int a, b, c;
+  const uint32_t cell = stencil_cell(d, st, lane, a, b, c);
+  bool present = false;
+  float v = 0.0f, w = 0.0f;
+  if (cell != NONE) {
+    const float ty = cfr(d, NATRAY_F_TYPE, cell);
+    present = !(ty == SOLID || (gas_only && ty == WATER));
+    if (present) {
+      w = st.w[0][a] * st.w[1][b] * st.w[2][c];
+      v = w * cfr(d, NATRAY_F_P, cell);
+    }
+  }
+  return ordered_sum27(v, w, present);
+}
+// push_water (bullet.wgsl:136-150) by one wave (integer atomics: any order gives the same sums).
+__device__ __forceinline__ void wave_push_water(const Dev& d, const float X[3], uint32_t dim, float j, float sw, uint32_t lane) {
   if (sw <= 1e-6f || j == 0.0f) return;
   const natray_params* U = d.U;
-  int base[3];
-  float w[3][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float lx = X[k] - ((uint32_t)k == dim ? 0.0f : 0.5f);
-    const float bf = floorf(lx - 0.5f);
-    base[k] = (int)bf;
-    axis_weights(lx - bf, w[k]);
-  }
+  const float lx[3] = {X[0] - (dim == 0 ? 0.0f : 0.5f), X[1] - (dim == 1 ? 0.0f : 0.5f), X[2] - (dim == 2 ? 0.0f : 0.5f)};
+  const Stencil st = stencil_of(lx);
   const float hs = U->fluid[3];
   const float kk = j / (U->fluid[0] * hs * hs * hs * sw);
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      for (int c = 0; c < 3; ++c) {
-        const uint32_t cell = cell_at(d, base[0] + a, base[1] + b, base[2] + c);
-        if (cell == NONE || accr(d, NATRAY_A_WT(dim), cell) <= 0) continue;
-        atomicAdd(&accr(d, NATRAY_A_BU + dim, cell), to_fp(kk * w[0][a] * w[1][b] * w[2][c], NATRAY_FP_M));
+  int a, b, c;
+  const uint32_t cell = stencil_cell(d, st, lane, a, b, c);
+  if (cell == NONE || accr(d, NATRAY_A_WT(dim), cell) <= 0) return;
+  atomicAdd(&accr(d, NATRAY_A_BU + dim, cell), to_fp(kk * st.w[0][a] * st.w[1][b] * st.w[2][c], NATRAY_FP_M));
+}
+// in_cavity (bullet.wgsl:156-170) by one wave: lane l < 32 tests section l.
+__device__ __forceinline__ bool wave_in_cavity(const Dev& d, uint32_t bi, b3 p, float t, uint32_t lane) {
+  bool hit = false;
+  if (lane < NATRAY_BULLET_SECTIONS) {
+    const float4* s = section_of(d, bi, lane);
+    if (!(s[1].w <= 0.0f || section_r2(s, t) >= hand2_of(d.U->origin[3]))) {
+      const b3 a = xyz(s[0]), ab = xyz(s[1]) - a;
+      const float f = dot3(p - a, ab) / fmax_(dot3(ab, ab), 1e-20f);
+      if (!(f < 0.0f || f > 1.0f)) {
+        const b3 dd = p - (a + ab * f);
+        hit = dot3(dd, dd) < section_r2(s, t);
       }
-}
-
-// in_cavity (bullet.wgsl:156-170)
-__device__ __forceinline__ bool in_cavity(const Dev& d, uint32_t bi, b3 p, float t) {
-  const float hand2 = hand2_of(d.U->origin[3]);
-  for (uint32_t k = 0; k < NATRAY_BULLET_SECTIONS; ++k) {
-    const float4* s = section_of(d, bi, k);
-    if (s[1].w <= 0.0f || section_r2(s, t) >= hand2) continue;
-    const b3 a = xyz(s[0]), ab = xyz(s[1]) - a;
-    const float f = dot3(p - a, ab) / fmax_(dot3(ab, ab), 1e-20f);
-    if (f < 0.0f || f > 1.0f) continue;
-    const b3 dd = p - (a + ab * f);
-    if (dot3(dd, dd) < section_r2(s, t)) return true;
+    }
   }
-  return false;
+  return __ballot(hit) != 0ull;
 }
 
-// bullet_step (bullet.wgsl:179-377)
+// Whether bullet b takes part in this step (it joins at the first step that starts once it has
+// arrived), as bullet_step's first lines decide.
+__device__ __forceinline__ uint32_t bullet_state(const Dev& d, const natray_bullet& b) {
+  uint32_t state = b.info[2];
+  if (state == 0u && d.st->t_ns - d.st->dt_ns >= b.info[0]) state = 1u;
+  return state;
+}
+
+// Words of a patch record: fp.xyz, 1 when the patch is on the wetted face; r x fp, its drag;
+// its face area across the path, then p times it (xyz); r . vhat.
+constexpr uint32_t PATCH_REC = 12;
+
+// The patches of bullet_step (bullet.wgsl:226-284): one wave per patch, NATRAY_MAX_BULLETS x
+// 144 workgroups of 4 waves.
 template <class L>
-__global__ __launch_bounds__(256) void k_bullet_step(L l) {
+__global__ __launch_bounds__(256) void k_bullet_patches(L l) {
+  const View vw = view(l);
+  const Dev& d = vw.d;
+  if (d.st->dt_ns == 0u) return;
+  const natray_params* U = d.U;
+  const uint32_t lane = lane_id();
+  const uint32_t per = NATRAY_BULLET_PATCHES / 4u;
+  const uint32_t bi = vw.wg / per, k = (vw.wg % per) * 4u + (threadIdx.x >> 6);
+  if (bi >= d.n_bullets) return;
+  const natray_bullet& bref = d.bullets[bi];
+  if (k >= bref.info[1] || bullet_state(d, bref) != 1u) return;
+  const float step = dt_of(d), t_end = now_of(d), hh = U->origin[3];
+  const b3 origin = xyz(U->origin), bx = xyz(bref.x), bv = xyz(bref.v);
+  // The pressure ahead of it (bullet.wgsl:194-196).
+  float pf;
+  {
+    const b3 ahead = bx + norm3(adds(bv, 1e-9f)) * (bref.shape[2] + 3.0f * hh);
+    const b3 xa = (ahead - origin) / hh;
+    const float xs[3] = {xa.x - 0.5f, xa.y - 0.5f, xa.z - 0.5f};
+    const float2 sp = wave_sample_pressure(d, xs, false, lane);
+    pf = sp.y > 1e-6f ? sp.x / sp.y : 0.0f;
+  }
+  const float4 q = make_float4(bref.q[0], bref.q[1], bref.q[2], bref.q[3]);
+  const b3 lb = qrot(make_float4(-q.x, -q.y, -q.z, q.w), xyz(bref.l));
+  const b3 w = qrot(q, b3{lb.x / bref.shape[1], lb.y / bref.shape[1], lb.z / bref.shape[0]});
+  const float rho = U->fluid[0];
+  const b3 vhat = norm3(adds(bv, 1e-9f));
+  const float4* pts = reinterpret_cast<const float4*>(d.patches) + (size_t)bi * NATRAY_BULLET_PATCHES * 2;
+  const float4 pp = pts[2 * k], pn = pts[2 * k + 1];
+  const b3 r = qrot(q, xyz(pp));
+  const b3 p = bx + r;
+  const b3 n = qrot(q, xyz(pn));
+  const float area = pp.w;
+  const b3 Xv = (p - origin) / hh;
+  const float X[3] = {Xv.x, Xv.y, Xv.z};
+  float uw[3], sw[3];
+#pragma unroll
+  for (uint32_t dim = 0; dim < 3; ++dim) {
+    const float xs[3] = {X[0] - (dim == 0 ? 0.0f : 0.5f), X[1] - (dim == 1 ? 0.0f : 0.5f), X[2] - (dim == 2 ? 0.0f : 0.5f)};
+    const float2 sg = wave_sample_grid(d, xs, NATRAY_F_U + dim, NATRAY_A_WT(dim), lane);
+    uw[dim] = sg.y > 1e-6f ? sg.x / sg.y : 0.0f;
+    sw[dim] = sg.y;
+  }
+  const float xc[3] = {X[0] - 0.5f, X[1] - 0.5f, X[2] - 0.5f};
+  float fill = fmin_(fmax_(wave_sample_grid(d, xc, NATRAY_F_TYPE, NATRAY_A_WC, lane).y / 8.0f, 0.0f), 1.0f);
+  if (wave_in_cavity(d, bi, p, t_end - step, lane)) fill = 0.0f;
+  const float2 gs = wave_sample_pressure(d, xc, true, lane);
+  const float p_gas = gs.y > 1e-6f ? gs.x / gs.y : 0.0f;
+  const b3 vp = bv + cross3(w, r);
+  const b3 rel = vp - b3{uw[0], uw[1], uw[2]};
+  const float sp = len3(rel);
+  const float wn = dot3(rel, n);
+  const float qd = 0.5f * rho * sp * sp;
+  const float p_water = fmax_(NATRAY_P_FLOOR, pf + qd * cp_patch(wn / fmax_(sp, 1e-9f)));
+  // The wet share of the patch: pressure, and skin friction along the surface.
+  b3 fw = n * (-fill * p_water * area);
+  const b3 rt = rel - n * wn;
+  const float rtl = len3(rt);
+  if (rtl > 1e-9f) fw = fw - (rt * (fill * CF_SKIN * 0.5f * rho * rtl)) * area;
+  const b3 fp = fw - n * ((1.0f - fill) * fmax_(p_gas, NATRAY_P_FLOOR) * area);
+  wave_push_water(d, X, 0, -fw.x * step, sw[0], lane);
+  wave_push_water(d, X, 1, -fw.y * step, sw[1], lane);
+  wave_push_water(d, X, 2, -fw.z * step, sw[2], lane);
+  if (lane == 0) {
+    const b3 tq = cross3(r, fp);
+    const bool wet_face = wn > 0.0f && fill > 0.0f;
+    float face = 0.0f, drag = 0.0f, rv = 0.0f;
+    b3 pface = {0.0f, 0.0f, 0.0f};
+    if (wet_face) {
+      face = fill * area * fmax_(dot3(n, vhat), 0.0f);
+      drag = -dot3(fw, vhat);
+      pface = p * face;
+      rv = dot3(r, vhat);
+    }
+    float4* rec = reinterpret_cast<float4*>(d.bullet_rec) + ((size_t)bi * NATRAY_BULLET_PATCHES + k) * (PATCH_REC / 4);
+    rec[0] = make_float4(fp.x, fp.y, fp.z, wet_face ? 1.0f : 0.0f);
+    rec[1] = make_float4(tq.x, tq.y, tq.z, drag);
+    rec[2] = make_float4(face, pface.x, pface.y, pface.z);
+    // r . vhat rides in the spare word of the next record's slot group: a 13th word.
+    d.bullet_rec[(size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * PATCH_REC + (size_t)bi * NATRAY_BULLET_PATCHES + k] = rv;
+  }
+}
+
+// The body of bullet_step (bullet.wgsl:286-377): the records summed as the reference's
+// workgroup sums its patches, the body stepped, the cavity section recorded and the thin
+// sections' volume source splatted. One workgroup of 256 per bullet.
+template <class L>
+__global__ __launch_bounds__(256) void k_bullet_body(L l) {
   const View vw = view(l);
   const Dev& d = vw.d;
   if (d.st->dt_ns == 0u) return;
@@ -3097,82 +3232,30 @@ __global__ __launch_bounds__(256) void k_bullet_step(L l) {
   const uint32_t bi = vw.wg, li = threadIdx.x;
   natray_bullet* B = d.bullets + bi;
   const natray_bullet b = *B;
-  const float step = dt_of(d), t_end = now_of(d), hh = U->origin[3];
-  const b3 origin = xyz(U->origin), bx = xyz(b.x), bv = xyz(b.v);
-  __shared__ uint32_t s_state;
-  __shared__ float s_pf;
-  __shared__ float4 red_f[256], red_t[256], red_c[256];
-  if (li == 0) {
-    uint32_t state = b.info[2];
-    // It joins at the first step that starts once it has arrived.
-    if (state == 0u && d.st->t_ns - d.st->dt_ns >= b.info[0]) state = 1u;
-    s_state = state;
-    const b3 ahead = bx + norm3(adds(bv, 1e-9f)) * (b.shape[2] + 3.0f * hh);
-    const b3 xa = (ahead - origin) / hh;
-    const float xs[3] = {xa.x - 0.5f, xa.y - 0.5f, xa.z - 0.5f};
-    const float2 sp = sample_pressure(d, xs, false);
-    s_pf = sp.y > 1e-6f ? sp.x / sp.y : 0.0f;
-  }
-  __syncthreads();
-  const uint32_t state = s_state;
+  const uint32_t state = bullet_state(d, b);
+  __syncthreads();  // every thread has read the bullet before thread 0 writes it
   if (state != 1u) {
     if (li == 0) B->info[2] = state;
     return;
   }
-  const float pf = s_pf;
-  const float4 q = make_float4(b.q[0], b.q[1], b.q[2], b.q[3]);
-  // The spin, from the angular momentum and the inertia in the body frame.
-  const b3 lb = qrot(make_float4(-q.x, -q.y, -q.z, q.w), xyz(b.l));
-  const b3 w = qrot(q, b3{lb.x / b.shape[1], lb.y / b.shape[1], lb.z / b.shape[0]});
+  const float step = dt_of(d), t_end = now_of(d), hh = U->origin[3];
+  const b3 origin = xyz(U->origin), bx = xyz(b.x), bv = xyz(b.v);
   const float rho = U->fluid[0];
+  const b3 vhat = norm3(adds(bv, 1e-9f));
+  __shared__ float4 red_f[256], red_t[256], red_c[256];
   b3 f = {0.0f, 0.0f, 0.0f}, tq = {0.0f, 0.0f, 0.0f}, cen = {0.0f, 0.0f, 0.0f};
   float cav_x = 0.0f, cav_y = 0.0f, cen_w = 3.0e38f;
-  const b3 vhat = norm3(adds(bv, 1e-9f));
-  const float4* pts = reinterpret_cast<const float4*>(d.patches) + (size_t)bi * NATRAY_BULLET_PATCHES * 2;
+  const float4* recs = reinterpret_cast<const float4*>(d.bullet_rec) + (size_t)bi * NATRAY_BULLET_PATCHES * (PATCH_REC / 4);
+  const float* rvs = d.bullet_rec + (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * PATCH_REC + (size_t)bi * NATRAY_BULLET_PATCHES;
   for (uint32_t k = li; k < b.info[1]; k += 256u) {
-    const float4 pp = pts[2 * k], pn = pts[2 * k + 1];
-    const b3 r = qrot(q, xyz(pp));
-    const b3 p = bx + r;
-    const b3 n = qrot(q, xyz(pn));
-    const float area = pp.w;
-    const b3 Xv = (p - origin) / hh;
-    const float X[3] = {Xv.x, Xv.y, Xv.z};
-    float uw[3], sw[3];
-#pragma unroll
-    for (uint32_t dim = 0; dim < 3; ++dim) {
-      const float xs[3] = {X[0] - (dim == 0 ? 0.0f : 0.5f), X[1] - (dim == 1 ? 0.0f : 0.5f), X[2] - (dim == 2 ? 0.0f : 0.5f)};
-      const float2 sg = sample_grid(d, xs, NATRAY_F_U + dim, NATRAY_A_WT(dim));
-      uw[dim] = sg.y > 1e-6f ? sg.x / sg.y : 0.0f;
-      sw[dim] = sg.y;
-    }
-    const float xc[3] = {X[0] - 0.5f, X[1] - 0.5f, X[2] - 0.5f};
-    float fill = fmin_(fmax_(sample_grid(d, xc, NATRAY_F_TYPE, NATRAY_A_WC).y / 8.0f, 0.0f), 1.0f);
-    if (in_cavity(d, bi, p, t_end - step)) fill = 0.0f;
-    const float2 gs = sample_pressure(d, xc, true);
-    const float p_gas = gs.y > 1e-6f ? gs.x / gs.y : 0.0f;
-    const b3 vp = bv + cross3(w, r);
-    const b3 rel = vp - b3{uw[0], uw[1], uw[2]};
-    const float sp = len3(rel);
-    const float wn = dot3(rel, n);
-    const float qd = 0.5f * rho * sp * sp;
-    const float p_water = fmax_(NATRAY_P_FLOOR, pf + qd * cp_patch(wn / fmax_(sp, 1e-9f)));
-    // The wet share of the patch: pressure, and skin friction along the surface.
-    b3 fw = n * (-fill * p_water * area);
-    const b3 rt = rel - n * wn;
-    const float rtl = len3(rt);
-    if (rtl > 1e-9f) fw = fw - (rt * (fill * CF_SKIN * 0.5f * rho * rtl)) * area;
-    const b3 fp = fw - n * ((1.0f - fill) * fmax_(p_gas, NATRAY_P_FLOOR) * area);
-    f = f + fp;
-    tq = tq + cross3(r, fp);
-    push_water(d, X, 0, -fw.x * step, sw[0]);
-    push_water(d, X, 1, -fw.y * step, sw[1]);
-    push_water(d, X, 2, -fw.z * step, sw[2]);
-    if (wn > 0.0f && fill > 0.0f) {
-      const float face = fill * area * fmax_(dot3(n, vhat), 0.0f);
-      cav_x = cav_x + -dot3(fw, vhat);
-      cav_y = cav_y + face;
-      cen = cen + p * face;
-      cen_w = fmin_(cen_w, dot3(r, vhat));
+    const float4 r0 = recs[3 * k], r1 = recs[3 * k + 1], r2 = recs[3 * k + 2];
+    f = f + xyz(r0);
+    tq = tq + xyz(r1);
+    if (r0.w != 0.0f) {
+      cav_x = cav_x + r1.w;
+      cav_y = cav_y + r2.x;
+      cen = cen + b3{r2.y, r2.z, r2.w};
+      cen_w = fmin_(cen_w, rvs[k]);
     }
   }
   red_f[li] = make_float4(f.x, f.y, f.z, cav_x);
@@ -3194,6 +3277,7 @@ __global__ __launch_bounds__(256) void k_bullet_step(L l) {
   if (li == 0) {
     const float4 rf = red_f[0], rt4 = red_t[0], rc = red_c[0];
     const float mass = b.x[3];
+    // The rubber's impulses (bullet_hit) are zero: rubber_contact is not part of this library.
     const b3 hit = {0.0f, 0.0f, 0.0f}, hit_l = {0.0f, 0.0f, 0.0f};
     const b3 force = xyz(rf) + xyz(U->gravity) * mass;
     const float drag = rf.w, face = rt4.w;
@@ -3202,6 +3286,7 @@ __global__ __launch_bounds__(256) void k_bullet_step(L l) {
     const b3 lnew = (xyz(b.l) + xyz(rt4) * step) + hit_l;
     // A symmetric top turns about its angular momentum at |L| / Iy and about its own axis at
     // the rest of its spin; taking the two exactly keeps the spin and the cone whatever the step.
+    const float4 q = make_float4(b.q[0], b.q[1], b.q[2], b.q[3]);
     const b3 axis0 = qrot(q, b3{0.0f, 0.0f, 1.0f});
     const float spin_extra = dot3(lnew, axis0) * (1.0f / b.shape[0] - 1.0f / b.shape[1]) * step;
     const float cone = len3(lnew) / b.shape[1] * step;
@@ -3264,20 +3349,13 @@ __global__ __launch_bounds__(256) void k_bullet_step(L l) {
       const float qv = PI_F * sk[1].w * len3(e - a);
       const b3 m = ((a + e) * 0.5f - origin) / hh;
       const float x[3] = {m.x - 0.5f, m.y - 0.5f, m.z - 0.5f};
-      int base[3];
-      float wt[3][3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float bf = floorf(x[k] - 0.5f);
-        base[k] = (int)bf;
-        axis_weights(x[k] - bf, wt[k]);
-      }
+      const Stencil st = stencil_of(x);
       for (int a3 = 0; a3 < 3; ++a3)
         for (int b3i = 0; b3i < 3; ++b3i)
           for (int c3 = 0; c3 < 3; ++c3) {
-            const uint32_t cell = cell_at(d, base[0] + a3, base[1] + b3i, base[2] + c3);
+            const uint32_t cell = cell_at(d, st.base[0] + a3, st.base[1] + b3i, st.base[2] + c3);
             if (cell == NONE) continue;
-            atomicAdd(&accr(d, NATRAY_A_SRC, cell), to_fp(qv * wt[0][a3] * wt[1][b3i] * wt[2][c3], NATRAY_FP_Q));
+            atomicAdd(&accr(d, NATRAY_A_SRC, cell), to_fp(qv * st.w[0][a3] * st.w[1][b3i] * st.w[2][c3], NATRAY_FP_Q));
           }
     }
   }
@@ -3565,7 +3643,9 @@ int launch_break_up(const Target& t, void* stream) {
 }
 int launch_bullets(const Target& t, void* stream) {
   if (!t.d || !t.d->n_bullets) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_bullet_step<One>, dim3(t.d->n_bullets), dim3(256), 0, S(stream), One{*t.d});
+  hipLaunchKernelGGL(k_bullet_patches<One>, dim3(t.d->n_bullets * (NATRAY_BULLET_PATCHES / 4u)), dim3(256), 0, S(stream),
+                     One{*t.d});
+  hipLaunchKernelGGL(k_bullet_body<One>, dim3(t.d->n_bullets), dim3(256), 0, S(stream), One{*t.d});
   NATRAY_LAUNCH(k_bullet_apply, LK_CELLS, 256, t, S(stream));
   return (int)hipGetLastError();
 }

@@ -861,7 +861,8 @@ int natray_sim_set_bullets(natray_sim* s, uint32_t n, const natray_bullet* bulle
     int e;
     float* pt = nullptr;
     if ((e = dalloc(s, &D.bullets, NATRAY_MAX_BULLETS)) || (e = dalloc(s, &pt, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 8)) ||
-        (e = dalloc(s, &D.sections, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 12)))
+        (e = dalloc(s, &D.sections, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 12)) ||
+        (e = dalloc(s, &D.bullet_rec, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 13)))
       return e;
     D.patches = pt;
   }
