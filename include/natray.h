@@ -376,6 +376,20 @@ int natray_sim_read_spray(natray_sim* sim, uint32_t max_records, float* spray, f
  * first time (s) water touched each ground cell, 0xffffffff where none has. */
 int natray_sim_read_wet(natray_sim* sim, uint32_t* touch);
 
+/* A whole keyframe of the live simulation (pack_water, pack_mem, pack_alive, index_bits,
+ * index_scan and pack_spray, shaders/pack.wgsl:33-124, in the layout of shaders/keys.wgsl:1-54):
+ * 4 header words (spray slots n = min(State.spray_count, cap), live spray slots, 0, 0); the
+ * water's 2 words per particle; the membrane's 2 words per vertex (those natray_sim_set_membrane
+ * gave); one alive bit per triangle (alive[n_tris], the membrane's own flags, may be null for
+ * n_tris = 0); then the spray's live bits, the live count before each bit word, and 3 words per
+ * live slot (packed position, radius bits) in slot order. *n_words = its size in words, which
+ * Cache sizes as key_bytes (src/cache.rs:408-416, before rounding to 256 bytes). words = null
+ * only asks the size; max_words smaller than it is NATRAY_E_ARG. spray_w[n][2] (may be null)
+ * receives each slot's weight and the time it was made, the reference's spray_w. The spray's
+ * slots follow the order in which particles broke up, which is not fixed. */
+int natray_sim_pack_keyframe(natray_sim* sim, uint32_t n_tris, const uint32_t* alive, uint32_t* words,
+                             uint32_t max_words, uint32_t* n_words, float* spray_w);
+
 /* ---- bullets -------------------------------------------------------------------------------
  * The water's side of the reference's bullet model (shaders/bullet.wgsl): each step, after the
  * grid is prepared, every bullet in the grid takes the pressure of the water or gas at each

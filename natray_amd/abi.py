@@ -184,6 +184,7 @@ _SIGNATURES = {
     "natray_sim_read_air": (C.c_int, [_P, _P]),
     "natray_sim_read_spray": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "natray_sim_read_wet": (C.c_int, [_P, _P]),
+    "natray_sim_pack_keyframe": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
     "natray_sim_set_bullets": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "natray_sim_read_bullets": (C.c_int, [_P, _P, _P]),
     "natray_sim_read_membrane": (C.c_int, [_P, _P]),

@@ -185,6 +185,18 @@ int launch_break_up(const Target& t, void* stream);
 int launch_bullets(const Target& t, void* stream);
 // The keyframe words of the water: two per particle into `out` (device memory, 8 N bytes).
 int launch_pack_water(const Target& t, void* stream, void* out);
+// The spray index of the first n slots into `index` (4 + 2 cw words, cw = bit words of the
+// spray capacity): [0] n, [1] live slots, the live bits at 4, their exclusive prefix at 4 + cw.
+int launch_spray_index(const Target& t, void* stream, uint32_t n, uint32_t cw, uint32_t* index);
+// A whole keyframe into `key` (device memory), its header words already written.
+struct KeyframeArgs {
+  void* key;
+  uint32_t n_verts, n_tris, n_spray, cw;
+  const uint32_t* alive;  // n_tris flags (device memory)
+  const uint32_t* index;  // the spray index launch_spray_index built
+  float* spray_w;         // n_spray x 2: each slot's weight and time made
+};
+int launch_pack_keyframe(const Target& t, void* stream, const KeyframeArgs& a);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also
 // into the network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has
 // run on them (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a

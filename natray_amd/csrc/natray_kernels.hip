@@ -3510,6 +3510,101 @@ __global__ __launch_bounds__(256) void k_pack_water(L l, uint2* __restrict__ out
   out[i] = pack_pos(pos.x, pos.y, pos.z, vel.w > 0.5f);
 }
 
+// ---- the rest of the keyframe (pack.wgsl:41-124; layout keys.wgsl:1-54) ----------------------------
+//
+// After the 4 header words and the water's 2 words per particle: the membrane's vertices (2
+// words each), one alive bit per triangle, then the spray: a live bit per slot, the live count
+// before each word of bits, and 3 words per live slot (position, then the radius's bits) in slot
+// order. The spray index (live bits and their exclusive prefix) is built first, into `index`:
+// [0] slots, [1] live slots, [2..3] unused, [4..4+cw) bits, [4+cw..4+2cw) prefix, where cw is the
+// bit words of the spray capacity.
+
+__host__ __device__ __forceinline__ uint32_t bit_words(uint32_t n) { return (n + 31u) / 32u; }
+
+// pack_mem (pack.wgsl:41-46)
+__global__ __launch_bounds__(256) void k_pack_mem(const float4* __restrict__ x, uint32_t n, uint2* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = x[i];
+  out[i] = pack_pos(p.x, p.y, p.z, false);
+}
+
+// pack_alive (pack.wgsl:48-58)
+__global__ __launch_bounds__(256) void k_pack_alive(const uint32_t* __restrict__ alive, uint32_t n_tris,
+                                                    uint32_t* __restrict__ out) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w * 32u >= n_tris) return;
+  uint32_t bits = 0u;
+  for (uint32_t k = 0; k < 32u; ++k) {
+    const uint32_t t = w * 32u + k;
+    if (t < n_tris && alive[t] != 0u) bits |= 1u << k;
+  }
+  out[w] = bits;
+}
+
+// index_bits (pack.wgsl:60-74): the live bits of the first n slots.
+__global__ __launch_bounds__(256) void k_spray_index_bits(const float4* __restrict__ spray, uint32_t n,
+                                                          uint32_t* __restrict__ index) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w == 0u) index[0] = n;
+  if (w >= bit_words(n)) return;
+  uint32_t bits = 0u;
+  for (uint32_t k = 0; k < 32u; ++k) {
+    const uint32_t i = w * 32u + k;
+    if (i < n && spray[(size_t)i * 3].w > 0.0f) bits |= 1u << k;
+  }
+  index[4u + w] = bits;
+}
+
+// index_scan (pack.wgsl:76-101): the exclusive prefix of the live counts over the bit words, in
+// one workgroup (integers: any order gives the same values), and the live total in index[1].
+__global__ __launch_bounds__(256) void k_spray_index_scan(uint32_t n, uint32_t cw, uint32_t* __restrict__ index) {
+  __shared__ uint32_t part[256];
+  const uint32_t li = threadIdx.x, nw = bit_words(n);
+  const uint32_t chunk = (nw + 255u) / 256u;
+  const uint32_t lo = min(li * chunk, nw), hi = min(lo + chunk, nw);
+  uint32_t sum = 0u;
+  for (uint32_t w = lo; w < hi; ++w) sum += __popc(index[4u + w]);
+  part[li] = sum;
+  __syncthreads();
+  for (uint32_t off = 1u; off < 256u; off <<= 1) {
+    const uint32_t v = li >= off ? part[li] + part[li - off] : part[li];
+    __syncthreads();
+    part[li] = v;
+    __syncthreads();
+  }
+  uint32_t run = part[li] - sum;
+  for (uint32_t w = lo; w < hi; ++w) {
+    index[4u + cw + w] = run;
+    run += __popc(index[4u + w]);
+  }
+  if (li == 255u) index[1] = part[255];
+}
+
+// pack_spray (pack.wgsl:103-124): the bits and prefix words, each slot's weight and time made
+// into spray_w, and the live slots' words.
+__global__ __launch_bounds__(256) void k_pack_spray(const float4* __restrict__ spray, uint32_t n, uint32_t cw,
+                                                    const uint32_t* __restrict__ index, uint32_t* __restrict__ key,
+                                                    uint32_t spray_word, float2* __restrict__ spray_w) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t nw = bit_words(n), w = i / 32u;
+  const uint32_t bits = index[4u + w], before = index[4u + cw + w];
+  if (i % 32u == 0u) {
+    key[spray_word + w] = bits;
+    key[spray_word + nw + w] = before;
+  }
+  const float4 pos = spray[(size_t)i * 3], vel = spray[(size_t)i * 3 + 1], air = spray[(size_t)i * 3 + 2];
+  spray_w[i] = make_float2(vel.w, air.w);
+  const uint32_t bit = 1u << (i % 32u);
+  if ((bits & bit) == 0u) return;
+  const uint32_t base = spray_word + 2u * nw + 3u * (before + (uint32_t)__popc(bits & (bit - 1u)));
+  const uint2 q = pack_pos(pos.x, pos.y, pos.z, false);
+  key[base] = q.x;
+  key[base + 1u] = q.y;
+  key[base + 2u] = __float_as_uint(pos.w);
+}
+
 // ---- launch wrappers -------------------------------------------------------------------------------
 
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -3647,6 +3742,31 @@ int launch_bullets(const Target& t, void* stream) {
                      One{*t.d});
   hipLaunchKernelGGL(k_bullet_body<One>, dim3(t.d->n_bullets), dim3(256), 0, S(stream), One{*t.d});
   NATRAY_LAUNCH(k_bullet_apply, LK_CELLS, 256, t, S(stream));
+  return (int)hipGetLastError();
+}
+int launch_pack_keyframe(const Target& t, void* stream, const KeyframeArgs& a) {
+  if (!t.d) return (int)hipErrorInvalidValue;
+  const Dev& d = *t.d;
+  uint32_t* key = static_cast<uint32_t*>(a.key);
+  NATRAY_LAUNCH(k_pack_water, LK_PARTS, 256, t, S(stream), reinterpret_cast<uint2*>(key + 4));
+  const uint32_t mem_word = 4u + 2u * d.N, alive_word = mem_word + 2u * a.n_verts;
+  if (a.n_verts)
+    hipLaunchKernelGGL(k_pack_mem, dim3((a.n_verts + 255u) / 256u), dim3(256), 0, S(stream), d.mem_x, a.n_verts,
+                       reinterpret_cast<uint2*>(key + mem_word));
+  if (a.n_tris)
+    hipLaunchKernelGGL(k_pack_alive, dim3((bit_words(a.n_tris) + 255u) / 256u), dim3(256), 0, S(stream), a.alive,
+                       a.n_tris, key + alive_word);
+  if (a.n_spray)
+    hipLaunchKernelGGL(k_pack_spray, dim3((a.n_spray + 255u) / 256u), dim3(256), 0, S(stream),
+                       reinterpret_cast<const float4*>(d.spray), a.n_spray, a.cw, a.index, key,
+                       alive_word + bit_words(a.n_tris), reinterpret_cast<float2*>(a.spray_w));
+  return (int)hipGetLastError();
+}
+int launch_spray_index(const Target& t, void* stream, uint32_t n, uint32_t cw, uint32_t* index) {
+  if (!t.d) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_spray_index_bits, dim3((bit_words(n) + 255u) / 256u + 1u), dim3(256), 0, S(stream),
+                     reinterpret_cast<const float4*>(t.d->spray), n, index);
+  hipLaunchKernelGGL(k_spray_index_scan, dim3(1), dim3(256), 0, S(stream), n, cw, index);
   return (int)hipGetLastError();
 }
 int launch_pack_water(const Target& t, void* stream, void* out) {

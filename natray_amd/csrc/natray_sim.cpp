@@ -55,6 +55,14 @@ struct natray_sim {
   int grid_cells = 0, grid_blocks = 0, grid_waves = 0, solve_grid = 0;
   natray::Target tgt;  // this scene as the kernels' launch target
   uint32_t* packed = nullptr;  // 2 N words: the keyframe words of the water (natray_sim_pack_water)
+  // natray_sim_pack_keyframe: the keyframe (kf_cap words), the spray index, the triangles' alive
+  // flags (alive_cap) and each spray slot's weight and time made.
+  uint32_t* kf = nullptr;
+  size_t kf_cap = 0;
+  uint32_t* kf_index = nullptr;
+  uint32_t* kf_alive = nullptr;
+  size_t alive_cap = 0;
+  float* kf_spray_w = nullptr;
   int mixed = 0;  // the CG precision (NATRAY_CG_*)
   // Upper bound of the reduction tags the solves enqueued since the tagged words were last
   // cleared (see clear_tags).
@@ -941,6 +949,62 @@ int natray_sim_pack_water(natray_sim* s, uint32_t* words) {
   }
   LAUNCHCHK(natray::launch_pack_water(s->tgt, s->stream, s->packed));
   HIPCHK(hipMemcpyAsync(words, s->packed, (size_t)s->D.N * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_pack_keyframe(natray_sim* s, uint32_t n_tris, const uint32_t* alive, uint32_t* words, uint32_t max_words,
+                             uint32_t* n_words, float* spray_w) {
+  if (!s || !n_words) return fail(NATRAY_E_ARG, "null argument");
+  if (n_tris && !alive) return fail(NATRAY_E_ARG, "null alive flags");
+  natray::Dev& D = s->D;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  int e = sync_state(s);
+  if (e) return e;
+  const uint32_t n_verts = std::min(s->U.mem[0], D.mem_cap);
+  const uint32_t n_spray = D.spray ? std::min(s->last.spray_count, D.spray_cap) : 0u;
+  const uint32_t cw = (D.spray_cap + 31u) / 32u;
+  uint32_t head[4] = {n_spray, 0u, 0u, 0u};
+  if (D.spray) {
+    if (!s->kf_index) {
+      if ((e = dalloc(s, &s->kf_index, 4 + 2 * (size_t)cw)) || (e = dalloc(s, &s->kf_spray_w, 2 * (size_t)std::max(D.spray_cap, 1u))))
+        return e;
+    }
+    LAUNCHCHK(natray::launch_spray_index(s->tgt, s->stream, n_spray, cw, s->kf_index));
+    HIPCHK(hipMemcpyAsync(head, s->kf_index, 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  const uint64_t total = 4ull + 2ull * ((uint64_t)D.N + n_verts) + (n_tris + 31ull) / 32ull + 2ull * ((n_spray + 31ull) / 32ull) +
+                         3ull * head[1];
+  if (total > 0xffffffffull) return fail(NATRAY_E_ARG, "keyframe larger than 2^32 words");
+  *n_words = (uint32_t)total;
+  if (!words) return NATRAY_OK;
+  if (max_words < total) return fail(NATRAY_E_ARG, "the keyframe needs " + std::to_string(total) + " words");
+  if (s->kf_cap < total) {
+    const size_t cap = std::max<size_t>(total, s->kf_cap + s->kf_cap / 2);
+    if (s->kf) {
+      s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), static_cast<void*>(s->kf)));
+      HIPCHK(hipFree(s->kf));
+      s->kf = nullptr;
+    }
+    if ((e = dalloc(s, &s->kf, cap))) return e;
+    s->kf_cap = cap;
+  }
+  if (n_tris > s->alive_cap) {
+    if (s->kf_alive) {
+      s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), static_cast<void*>(s->kf_alive)));
+      HIPCHK(hipFree(s->kf_alive));
+      s->kf_alive = nullptr;
+    }
+    if ((e = dalloc(s, &s->kf_alive, n_tris))) return e;
+    s->alive_cap = n_tris;
+  }
+  if (n_tris) HIPCHK(hipMemcpyAsync(s->kf_alive, alive, (size_t)n_tris * 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->kf, head, 16, hipMemcpyHostToDevice, s->stream));
+  natray::KeyframeArgs a{s->kf, n_verts, n_tris, n_spray, cw, s->kf_alive, s->kf_index, s->kf_spray_w};
+  LAUNCHCHK(natray::launch_pack_keyframe(s->tgt, s->stream, a));
+  HIPCHK(hipMemcpyAsync(words, s->kf, (size_t)total * 4, hipMemcpyDeviceToHost, s->stream));
+  if (spray_w && n_spray) HIPCHK(hipMemcpyAsync(spray_w, s->kf_spray_w, (size_t)n_spray * 8, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return NATRAY_OK;
 }

@@ -197,6 +197,19 @@ class Sim:
         abi.check(abi.lib().natray_sim_read_wet(self._h, abi.ptr(out)))
         return out
 
+    def pack_keyframe(self, alive: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """A whole keyframe (natray_sim_pack_keyframe): the words, and each spray slot's weight
+        and time made [n][2]. `alive` holds the membrane triangles' flags."""
+        a = None if alive is None else np.ascontiguousarray(alive, np.uint32)
+        nt = 0 if a is None else int(a.size)
+        total = C.c_uint32()
+        abi.check(abi.lib().natray_sim_pack_keyframe(self._h, nt, abi.ptr(a), None, 0, C.byref(total), None))
+        words = np.zeros(total.value, np.uint32)
+        sw = np.zeros((max(int(self.params.limits[1]), 1), 2), np.float32)
+        abi.check(abi.lib().natray_sim_pack_keyframe(self._h, nt, abi.ptr(a), abi.ptr(words), total.value,
+                                                     C.byref(total), abi.ptr(sw)))
+        return words, sw[: int(words[0])]
+
     def set_bullets(self, bullets: list, patches: np.ndarray) -> None:
         """Bullets for the coming steps (natray_sim_set_bullets): abi.Bullet records and their
         patches [n][576][8] (position and area, normal and 0, body frame)."""

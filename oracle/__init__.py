@@ -63,6 +63,7 @@ def lib() -> C.CDLL:
                 "read_spray": (C.c_uint32, [P, C.c_uint32, P, P]),
                 "read_wet": (None, [P, P]),
                 "elementary": (None, [C.c_int, P, P, P, C.c_uint32]),
+                "pack_keyframe": (C.c_uint32, [P, C.c_uint32, P, P, C.c_uint32, P]),
                 "set_bullets": (C.c_int, [P, C.c_uint32, P, P]),
                 "read_bullets": (None, [P, P, P]),
             }
@@ -261,6 +262,17 @@ class Oracle:
         out = np.zeros((self.n, 2), np.uint32)
         self._f("pack_water")(self._h, abi.ptr(out))
         return out
+
+    def pack_keyframe(self, alive: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """A whole keyframe (words) and each spray slot's weight and time made [n][2]."""
+        a = None if alive is None else np.ascontiguousarray(alive, np.uint32)
+        nt = 0 if a is None else int(a.size)
+        total = int(self._f("pack_keyframe")(self._h, nt, abi.ptr(a), None, 0, None))
+        words = np.zeros(total, np.uint32)
+        n = words.size  # at least the slot count
+        sw = np.zeros((max(int(self.state().spray_count), 1), 2), np.float32)
+        self._f("pack_keyframe")(self._h, nt, abi.ptr(a), abi.ptr(words), total, abi.ptr(sw))
+        return words, sw[: int(words[0])] if n else sw[:0]
 
     def membrane_water(self) -> np.ndarray:
         out = np.zeros((getattr(self, "_mem_n", 0), 4), np.float32)

@@ -2373,6 +2373,65 @@ void F(read_membrane)(const osim* s, float* water) {
   for (size_t k = 0; k < (size_t)mem_count(s) * 4; ++k) water[k] = (float)s->mem_water[k];
 }
 
+/* A whole keyframe (pack.wgsl:33-124, layout keys.wgsl:1-54): header (spray slots, live
+ * slots, 0, 0), the water's and the membrane's 2 words each, the triangles' alive bits, the
+ * spray's live bits, the live count before each bit word and 3 words per live slot. Returns the
+ * size in words; writes `words` (if not null and max_words is enough) and spray_w[n][2]. */
+static void pack_pos_(REAL px, REAL py, REAL pz, int flag, uint32_t out[2]) {
+  const REAL lo[3] = {(REAL)-4, (REAL)-0.5, (REAL)-4}, p[3] = {px, py, pz};
+  uint32_t q[3];
+  for (int k = 0; k < 3; ++k) {
+    REAL t = rmin(rmax((p[k] - lo[k]) / (REAL)8, (REAL)0), (REAL)1);
+    q[k] = (uint32_t)(t * (REAL)2097151 + (REAL)0.5);
+  }
+  out[0] = q[0] | (q[1] << 21);
+  out[1] = (q[1] >> 11) | (q[2] << 10) | (flag ? 1u << 31 : 0u);
+}
+uint32_t F(pack_keyframe)(const osim* s, uint32_t n_tris, const uint32_t* alive, uint32_t* words, uint32_t max_words,
+                          float* spray_w) {
+  const uint32_t n_verts = umin(s->U.mem[0], s->mem_cap);
+  const uint32_t n = s->spray ? umin(s->st.spray_count, s->spray_cap) : 0u;
+  const uint32_t nw = (n + 31u) / 32u;
+  uint32_t live = 0;
+  for (uint32_t i = 0; i < n; ++i) live += s->spray[(size_t)i * 12 + 3] > (REAL)0;
+  const uint32_t total = 4u + 2u * (s->N + n_verts) + (n_tris + 31u) / 32u + 2u * nw + 3u * live;
+  if (!words || max_words < total) return total;
+  memset(words, 0, (size_t)total * 4);
+  words[0] = n;
+  words[1] = live;
+  F(pack_water)(s, words + 4);
+  const uint32_t mem_word = 4u + 2u * s->N, alive_word = mem_word + 2u * n_verts;
+  for (uint32_t i = 0; i < n_verts; ++i) {
+    const REAL* x = &s->mem_x[(size_t)i * 4];
+    pack_pos_(x[0], x[1], x[2], 0, words + mem_word + 2u * i);
+  }
+  for (uint32_t t = 0; t < n_tris; ++t)
+    if (alive[t]) words[alive_word + t / 32u] |= 1u << (t % 32u);
+  const uint32_t spray_word = alive_word + (n_tris + 31u) / 32u;
+  uint32_t before = 0, slot = 0;
+  for (uint32_t w = 0; w < nw; ++w) {
+    uint32_t bits = 0;
+    for (uint32_t k = 0; k < 32u && w * 32u + k < n; ++k)
+      if (s->spray[(size_t)(w * 32u + k) * 12 + 3] > (REAL)0) bits |= 1u << k;
+    words[spray_word + w] = bits;
+    words[spray_word + nw + w] = before;
+    before += (uint32_t)__builtin_popcount(bits);
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    const REAL* r = &s->spray[(size_t)i * 12];
+    if (spray_w) {
+      spray_w[2 * i] = (float)r[7];
+      spray_w[2 * i + 1] = (float)r[11];
+    }
+    if (!(r[3] > (REAL)0)) continue;
+    uint32_t* e = words + spray_word + 2u * nw + 3u * slot++;
+    pack_pos_(r[0], r[1], r[2], 0, e);
+    float radius = (float)r[3];
+    memcpy(&e[2], &radius, 4);
+  }
+  return total;
+}
+
 /* The spray's and the bullet's elementary functions on n floats (which: 0 exp, 1 log, 2 cos,
  * 3 pow(a, b), 4 sin, 5 asin), for the tests that pin them against the C library. */
 void F(elementary)(int which, const float* a, const float* b, float* out, uint32_t n) {

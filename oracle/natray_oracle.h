@@ -48,6 +48,8 @@ typedef struct osim osim;
   void P##read_air(const osim* s, float* out);                                                      \
   uint32_t P##read_spray(const osim* s, uint32_t max_records, double* spray, double* landed);       \
   void P##read_wet(const osim* s, uint32_t* touch);                                                 \
+  uint32_t P##pack_keyframe(const osim* s, uint32_t n_tris, const uint32_t* alive, uint32_t* words,   \
+                            uint32_t max_words, float* spray_w);                                    \
   int P##set_bullets(osim* s, uint32_t n, const natray_bullet* bullets, const float* patches);      \
   void P##read_bullets(const osim* s, natray_bullet* bullets, double* sections);                    \
   void P##elementary(int which, const float* a, const float* b, float* out, uint32_t n);           \
