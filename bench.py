@@ -108,20 +108,29 @@ def timed_run(sim, warmup: int, steps: int, iters: int, graph: bool = True):
     return t1 - t0, {k: c1[k] - c0[k] for k in c0}, times, st
 
 
+def bullet_workload(args):
+    """The 2M impact without the capsule solid, and one bullet (4.2 g, 9 mm across, 20 mm long)
+    at 300 m/s along +x with its nose two cells short of the water: (scene, bullet, patches)."""
+    from natray_amd import abi, scenes
+
+    s = scenes.balloon_impact(iters=args.iters)
+    s.scene.flags &= ~abi.SCENE_CAPSULE
+    nose = 0.5 - float(s.meta["r_eff"]) - 2.0 * float(s.meta["dx"])
+    b, patches = scenes.bullet_shot(at=(nose - 0.011, 0.5, 0.5), speed=300.0)
+    s.meta["workload"] = f"{s.meta['workload']}, one bullet at 300 m/s instead of the capsule"
+    return s, b, patches
+
+
 def bullet_report(args) -> dict:
     """The reference's bullet model in place of the capsule solid: the 2M impact without the
     capsule, with one bullet (4.2 g, 9 mm across, 20 mm long, 300 m/s along +x, its nose two
     cells short of the water), kernel by kernel over `--bullet-steps` steps, against the same
     scene without the bullet. The grid phase holds grid_prep, bullet_step and bullet_apply, so
     the difference of the two grid phases is what the bullet costs a step."""
-    from natray_amd import Sim, abi, scenes
+    from natray_amd import Sim
 
-    s = scenes.balloon_impact(iters=args.iters)
-    s.scene.flags &= ~abi.SCENE_CAPSULE
-    nose = 0.5 - float(s.meta["r_eff"]) - 2.0 * float(s.meta["dx"])
-    b, patches = scenes.bullet_shot(at=(nose - 0.011, 0.5, 0.5), speed=300.0)
-    out = {"workload": f"{s.meta['workload']}, one bullet at 300 m/s instead of the capsule",
-           "steps": args.bullet_steps}
+    s, b, patches = bullet_workload(args)
+    out = {"workload": s.meta["workload"], "steps": args.bullet_steps}
     for label, with_bullet in (("with_bullet", True), ("without", False)):
         sim = Sim.from_scene(s)
         if with_bullet:
@@ -499,6 +508,11 @@ def main() -> int:
     from natray_amd import Sim, scenes
 
     if args.bullets_only:
+        if args.write_scene:
+            s, b, patches = bullet_workload(args)
+            scenes.write_scene_file(s, args.write_scene, bullets=([b], patches[None]))
+            print(f"wrote {args.write_scene}: {s.meta['workload']}")
+            return 0
         print(json.dumps({"bullets": bullet_report(args)}), flush=True)
         return 0
     scene, cfg = make_scene(args)

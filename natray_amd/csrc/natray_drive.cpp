@@ -7,7 +7,9 @@
 //
 // The scene file comes from `python bench.py --write-scene <path>` (natray_amd.scenes
 // write_scene_file): "NATRAYSC", u32 version 1, u32 particle count, then natray_params,
-// natray_scene, natray_state and the 80-byte particle records. The driver is what a profiler
+// natray_scene, natray_state and the 80-byte particle records, then optionally b"BULLETS\0", u32 n,
+// n natray_bullet records and their patches (n x 576 x 8 floats), which the driver hands to
+// natray_sim_set_bullets (`bench.py --bullets-only --write-scene`). The driver is what a profiler
 // wraps (`rocprofv3 ... -- natray_amd/natray_drive <file>`): a native program, so the profiler
 // sees the kernels without an interpreter in between. It prints one line with the timing.
 #include <chrono>
@@ -52,11 +54,35 @@ int main(int argc, char** argv) {
     fclose(f);
     return 2;
   }
+  // The optional bullets trailer.
+  uint32_t nb = 0;
+  std::vector<natray_bullet> bullets;
+  std::vector<float> patches;
+  char tag[8];
+  if (read_all(f, tag, 8)) {
+    if (memcmp(tag, "BULLETS", 8) != 0 || !read_all(f, &nb, 4) || nb > NATRAY_MAX_BULLETS) {
+      fprintf(stderr, "%s: bad bullets trailer\n", argv[1]);
+      fclose(f);
+      return 2;
+    }
+    bullets.resize(nb);
+    patches.resize((size_t)nb * NATRAY_BULLET_PATCHES * 8);
+    if (!read_all(f, bullets.data(), nb * sizeof(natray_bullet)) || !read_all(f, patches.data(), patches.size() * 4)) {
+      fprintf(stderr, "%s: truncated bullets\n", argv[1]);
+      fclose(f);
+      return 2;
+    }
+  }
   fclose(f);
   natray_sim* sim = nullptr;
   int rc = natray_sim_create(&params, particles.data(), n, &scene, &state, &sim);
   if (rc) {
     fprintf(stderr, "natray_sim_create: %d %s\n", rc, natray_last_error());
+    return 1;
+  }
+  if (nb && (rc = natray_sim_set_bullets(sim, nb, bullets.data(), patches.data())) != 0) {
+    fprintf(stderr, "natray_sim_set_bullets: %d %s\n", rc, natray_last_error());
+    natray_sim_destroy(sim);
     return 1;
   }
   natray_net* net = nullptr;
@@ -84,9 +110,9 @@ int main(int argc, char** argv) {
   }
   const double s = std::chrono::duration<double>(t1 - t0).count();
   printf("natray_drive: %u particles, %u steps in %.3f ms (%.3f ms/step, %.4g particle-substeps/s), step %u, %u blocks, "
-         "%u CG iterations in the last step\n",
+         "%u CG iterations in the last step, %u bullets\n",
          n, steps, s * 1e3, s * 1e3 / (steps ? steps : 1), steps ? (double)n * steps / s : 0.0, st.step, st.n_blocks,
-         st.solve_iters);
+         st.solve_iters, nb);
   fflush(stdout);
   natray_sim_destroy(sim);
   if (net) natray_net_destroy(net);

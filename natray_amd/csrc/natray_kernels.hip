@@ -3274,47 +3274,60 @@ __global__ __launch_bounds__(256) void k_bullet_body(L l) {
     __syncthreads();
   }
   const float hand2 = hand2_of(hh);
+  // Thread 0 steps the body; the neighbour-block test, the reach over the sections and the
+  // sections' source run on many lanes (an or, a max of non-negative floats and integer
+  // atomics: the same values as the sequential loops).
+  __shared__ float s_x[3];
+  __shared__ uint32_t s_reach;
+  const float4 rf = red_f[0], rt4 = red_t[0], rc = red_c[0];
+  const float mass = b.x[3];
+  // The rubber's impulses (bullet_hit) are zero: rubber_contact is not part of this library.
+  const b3 hit = {0.0f, 0.0f, 0.0f}, hit_l = {0.0f, 0.0f, 0.0f};
+  const b3 force = xyz(rf) + xyz(U->gravity) * mass;
+  const float drag = rf.w, face = rt4.w;
+  const b3 v = bv + (force * step + hit) / mass;
+  const b3 x = bx + v * step;
   if (li == 0) {
-    const float4 rf = red_f[0], rt4 = red_t[0], rc = red_c[0];
-    const float mass = b.x[3];
-    // The rubber's impulses (bullet_hit) are zero: rubber_contact is not part of this library.
-    const b3 hit = {0.0f, 0.0f, 0.0f}, hit_l = {0.0f, 0.0f, 0.0f};
-    const b3 force = xyz(rf) + xyz(U->gravity) * mass;
-    const float drag = rf.w, face = rt4.w;
-    const b3 v = bv + (force * step + hit) / mass;
-    const b3 x = bx + v * step;
-    const b3 lnew = (xyz(b.l) + xyz(rt4) * step) + hit_l;
+    s_x[0] = x.x, s_x[1] = x.y, s_x[2] = x.z;
+    s_reach = 0u;
+  }
+  // Out of reach of any water it can go a block per step: it cannot miss one on the way,
+  // since the blocks around it are checked every step.
+  bool nb_in_use = false;
+  if (li < 27u) {
+    const b3 xc = (x - origin) / hh;
+    const int k = (int)li;
+    const int nb[3] = {((int)floorf(xc.x) >> 2) + k / 9 - 1, ((int)floorf(xc.y) >> 2) + (k / 3) % 3 - 1,
+                       ((int)floorf(xc.z) >> 2) + k % 3 - 1};
+    nb_in_use = nb[0] >= 0 && nb[1] >= 0 && nb[2] >= 0 && nb[0] < (int)U->bdims[0] && nb[1] < (int)U->bdims[1] &&
+                nb[2] < (int)U->bdims[2] && d.block_ptr[block_slot(U, nb[0], nb[1], nb[2])] < n_blocks(d);
+  }
+  const bool near = __syncthreads_or(nb_in_use) != 0;
+  float4* secs = reinterpret_cast<float4*>(d.sections) + (size_t)bi * NATRAY_BULLET_SECTIONS * 3;
+  uint32_t info[4] = {b.info[0], b.info[1], 1u, b.info[3]};
+  b3 lnew = {0.0f, 0.0f, 0.0f};
+  float4 qn = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  float roll = 0.0f, pace = 0.0f;
+  if (li == 0) {
+    lnew = (xyz(b.l) + xyz(rt4) * step) + hit_l;
     // A symmetric top turns about its angular momentum at |L| / Iy and about its own axis at
     // the rest of its spin; taking the two exactly keeps the spin and the cone whatever the step.
     const float4 q = make_float4(b.q[0], b.q[1], b.q[2], b.q[3]);
     const b3 axis0 = qrot(q, b3{0.0f, 0.0f, 1.0f});
     const float spin_extra = dot3(lnew, axis0) * (1.0f / b.shape[0] - 1.0f / b.shape[1]) * step;
     const float cone = len3(lnew) / b.shape[1] * step;
-    float4 qn = qmul(q, make_float4(0.0f, 0.0f, nm_sinf(0.5f * spin_extra), nm_cosf(0.5f * spin_extra)));
+    qn = qmul(q, make_float4(0.0f, 0.0f, nm_sinf(0.5f * spin_extra), nm_cosf(0.5f * spin_extra)));
     if (cone > 1e-12f) {
       const b3 ax = norm3(lnew) * nm_sinf(0.5f * cone);
       qn = qmul(make_float4(ax.x, ax.y, ax.z, nm_cosf(0.5f * cone)), qn);
     }
     const float ql = __builtin_sqrtf(((qn.x * qn.x + qn.y * qn.y) + qn.z * qn.z) + qn.w * qn.w);
     qn = make_float4(qn.x / ql, qn.y / ql, qn.z / ql, qn.w / ql);
-    const float roll = b.v[3] + dot3(lnew, axis0) / b.shape[0] * step;
-    uint32_t info[4] = {b.info[0], b.info[1], 1u, b.info[3]};
+    roll = b.v[3] + dot3(lnew, axis0) / b.shape[0] * step;
     const float xa[3] = {x.x, x.y, x.z};
     for (int k = 0; k < 3; ++k)
       if (xa[k] < U->origin[k] || xa[k] > U->origin[k] + (float)U->dims[k] * hh) info[2] = 2u;
-    // Out of reach of any water it can go a block per step: it cannot miss one on the way,
-    // since the blocks around it are checked every step.
-    const b3 xc = (x - origin) / hh;
-    const int cb[3] = {(int)floorf(xc.x) >> 2, (int)floorf(xc.y) >> 2, (int)floorf(xc.z) >> 2};
-    bool near = false;
-    for (int k = 0; k < 27; ++k) {
-      const int nb[3] = {cb[0] + k / 9 - 1, cb[1] + (k / 3) % 3 - 1, cb[2] + k % 3 - 1};
-      if (nb[0] >= 0 && nb[1] >= 0 && nb[2] >= 0 && nb[0] < (int)U->bdims[0] && nb[1] < (int)U->bdims[1] &&
-          nb[2] < (int)U->bdims[2] && d.block_ptr[block_slot(U, nb[0], nb[1], nb[2])] < n_blocks(d))
-        near = true;
-    }
-    const float pace = near ? len3(v) : len3(v) * U->fluid[1] / 4.0f;
-    float4* secs = reinterpret_cast<float4*>(d.sections) + (size_t)bi * NATRAY_BULLET_SECTIONS * 3;
+    pace = near ? len3(v) : len3(v) * U->fluid[1] / 4.0f;
     if (drag > 0.0f && face > 0.0f) {
       // The face's centre, moved back along the path to where the flow leaves it.
       const b3 mid = xyz(rc) / face;
@@ -3326,12 +3339,18 @@ __global__ __launch_bounds__(256) void k_bullet_body(L l) {
       sn[2] = make_float4(face / PI_F, 0.0f, 0.0f, 0.0f);
       info[3] += 1u;
     }
-    float reach = 0.0f;
-    for (uint32_t k = 0; k < NATRAY_BULLET_SECTIONS; ++k) {
-      const float4* sk = secs + (size_t)k * 3;
-      if (sk[1].w <= 0.0f || section_r2(sk, t_end) >= hand2) continue;
-      reach = fmax_(reach, fmax_(len3(xyz(sk[0]) - x), len3(xyz(sk[1]) - x)));
+  }
+  __syncthreads();  // the new section is in place
+  if (li < NATRAY_BULLET_SECTIONS) {
+    const float4* sk = secs + (size_t)li * 3;
+    if (!(sk[1].w <= 0.0f || section_r2(sk, t_end) >= hand2)) {
+      const float r = fmax_(len3(xyz(sk[0]) - x), len3(xyz(sk[1]) - x));
+      atomicMax(&s_reach, __float_as_uint(r));  // r >= 0: its bits order as the floats do
     }
+  }
+  __syncthreads();
+  if (li == 0) {
+    const float reach = __uint_as_float(s_reach);
     B->x[0] = x.x, B->x[1] = x.y, B->x[2] = x.z;
     B->v[0] = v.x, B->v[1] = v.y, B->v[2] = v.z, B->v[3] = roll;
     B->q[0] = qn.x, B->q[1] = qn.y, B->q[2] = qn.z, B->q[3] = qn.w;
@@ -3339,25 +3358,22 @@ __global__ __launch_bounds__(256) void k_bullet_body(L l) {
     B->reach[0] = x.x, B->reach[1] = x.y, B->reach[2] = x.z, B->reach[3] = reach > 0.0f ? reach + R_HAND * hh : 0.0f;
     for (int k = 0; k < 4; ++k) B->info[k] = info[k];
   }
-  __syncthreads();
-  // Sections thinner than the grid push the water out of the way as they open.
-  if (li < NATRAY_BULLET_SECTIONS) {
-    const float4* sk = section_of(d, bi, li);
+  // Sections thinner than the grid push the water out of the way as they open: one (section,
+  // stencil cell) pair per thread.
+  for (uint32_t pair = li; pair < NATRAY_BULLET_SECTIONS * 27u; pair += 256u) {
+    const uint32_t k = pair / 27u, cidx = pair % 27u;
+    const float4* sk = secs + (size_t)k * 3;
     const float r2 = section_r2(sk, t_end);
-    if (sk[1].w > 0.0f && r2 < hand2) {
-      const b3 a = xyz(sk[0]), e = xyz(sk[1]);
-      const float qv = PI_F * sk[1].w * len3(e - a);
-      const b3 m = ((a + e) * 0.5f - origin) / hh;
-      const float x[3] = {m.x - 0.5f, m.y - 0.5f, m.z - 0.5f};
-      const Stencil st = stencil_of(x);
-      for (int a3 = 0; a3 < 3; ++a3)
-        for (int b3i = 0; b3i < 3; ++b3i)
-          for (int c3 = 0; c3 < 3; ++c3) {
-            const uint32_t cell = cell_at(d, st.base[0] + a3, st.base[1] + b3i, st.base[2] + c3);
-            if (cell == NONE) continue;
-            atomicAdd(&accr(d, NATRAY_A_SRC, cell), to_fp(qv * st.w[0][a3] * st.w[1][b3i] * st.w[2][c3], NATRAY_FP_Q));
-          }
-    }
+    if (!(sk[1].w > 0.0f && r2 < hand2)) continue;
+    const b3 a = xyz(sk[0]), e = xyz(sk[1]);
+    const float qv = PI_F * sk[1].w * len3(e - a);
+    const b3 m = ((a + e) * 0.5f - origin) / hh;
+    const float xs[3] = {m.x - 0.5f, m.y - 0.5f, m.z - 0.5f};
+    const Stencil st = stencil_of(xs);
+    const int a3 = (int)(cidx / 9u), b3i = (int)((cidx / 3u) % 3u), c3 = (int)(cidx % 3u);
+    const uint32_t cell = cell_at(d, st.base[0] + a3, st.base[1] + b3i, st.base[2] + c3);
+    if (cell == NONE) continue;
+    atomicAdd(&accr(d, NATRAY_A_SRC, cell), to_fp(qv * st.w[0][a3] * st.w[1][b3i] * st.w[2][c3], NATRAY_FP_Q));
   }
 }
 

@@ -250,9 +250,11 @@ def box_params(side: int, n: int, capacity: int, gravity: float = 9.81, cfl: flo
     return U
 
 
-def write_scene_file(s: Scene, path: str) -> None:
+def write_scene_file(s: Scene, path: str, bullets=None) -> None:
     """Writes the scene for natray_drive: b"NATRAYSC", u32 version 1, u32 particle count,
-    Params, natray_scene, State, then the 80-byte particle records."""
+    Params, natray_scene, State, then the 80-byte particle records. With bullets = (list of
+    abi.Bullet, patches [n][576][8]), a trailer follows: b"BULLETS\0", u32 n, the n 112-byte
+    records and the patches."""
     import ctypes as C
     import struct
 
@@ -264,6 +266,15 @@ def write_scene_file(s: Scene, path: str) -> None:
         f.write(bytes(C.string_at(C.addressof(s.scene), C.sizeof(s.scene))))
         f.write(bytes(C.string_at(C.addressof(init), C.sizeof(init))))
         f.write(np.ascontiguousarray(s.particles, np.float32).tobytes())
+        if bullets:
+            bl, pt = bullets
+            pt = np.ascontiguousarray(pt, np.float32)
+            assert pt.shape == (len(bl), abi.BULLET_PATCHES, 8)
+            f.write(b"BULLETS\0")
+            f.write(struct.pack("<I", len(bl)))
+            for b in bl:
+                f.write(bytes(b))
+            f.write(pt.tobytes())
 
 
 def no_limits_state() -> abi.State:
