@@ -1,5 +1,5 @@
 """bench.py's contract pieces that need no GPU: it refuses more than one GPU with a message, it
-writes the native driver's scene file for the profiles, and it finds the counter-measured HBM
+writes the native driver's scene file for the profiles (with its bullets trailer), and it finds the counter-measured HBM
 traffic of the solve in the committed profile summary."""
 import os
 import struct
@@ -30,6 +30,26 @@ def test_write_scene_file(tmp_path):
     version, n = struct.unpack("<II", raw[8:16])
     assert version == 1 and n == 32768
     assert len(raw) > 16 + 192 + 96 + n * 80
+
+
+def test_scene_file_carries_bullets(tmp_path):
+    import ctypes as C
+
+    import numpy as np
+
+    from natray_amd import abi, scenes
+
+    s = scenes.water_sphere(side=16, n=512)
+    b, pt = scenes.bullet_shot(arrive_ns=1234)
+    path = tmp_path / "bullet.scene"
+    scenes.write_scene_file(s, str(path), bullets=([b], pt[None]))
+    raw = path.read_bytes()
+    head = 16 + C.sizeof(abi.Params) + C.sizeof(abi.Scene) + C.sizeof(abi.State) + s.n * 80
+    assert raw[head:head + 8] == b"BULLETS\0" and struct.unpack("<I", raw[head + 8:head + 12]) == (1,)
+    rec = abi.Bullet.from_buffer_copy(raw[head + 12:head + 12 + C.sizeof(abi.Bullet)])
+    assert C.sizeof(abi.Bullet) == 112 and list(rec.info) == [1234, list(b.info)[1], 0, 0]
+    got = np.frombuffer(raw[head + 12 + 112:], np.float32).reshape(1, abi.BULLET_PATCHES, 8)
+    np.testing.assert_array_equal(got, pt[None])
 
 
 def test_solve_traffic_is_read_from_the_profile():
