@@ -3022,12 +3022,12 @@ __device__ __forceinline__ float cp_patch(float s) {
 }
 
 // bullet_step (bullet.wgsl:179-377) in two kernels with the reference's bits. The patches are
-// latency-bound (each samples the grid at about 250 stencil cells), so k_bullet_patches gives
-// every patch a wave whose lanes 0..26 take the 27 cells of a stencil at once; the sums are then
-// taken in the sequential loop's order from the lanes (readlane), so they round as the loop
-// does. Each patch leaves a record; k_bullet_body sums the records the way the reference's
-// workgroup of 256 does (lane li takes patches li, li + 256, ... in order, then the tree) and
-// steps the body.
+// latency-bound (each samples the grid at about 250 stencil cells, two dependent loads deep), so
+// k_bullet_patches gives every patch a wave: its 64 lanes load the 4^3 cells that hold all the
+// patch's stencils at once, and each stencil's sums are taken from the lanes in the sequential
+// loop's order (readlane), so they round as the loop does. Each patch leaves a record;
+// k_bullet_body sums the records the way the reference's workgroup of 256 does (lane li takes
+// patches li, li + 256, ... in order, then the tree) and steps the body.
 
 // The 27 cells of a stencil around x (in cells, on a lattice), lane l < 27 taking cell
 // (l / 9, l / 3 % 3, l % 3): its B-spline weights and its index (NONE where there is none).
@@ -3063,18 +3063,6 @@ __device__ __forceinline__ float2 ordered_sum27(float v, float w, bool present) 
     }
   return make_float2(sx, sy);
 }
-// sample_grid (membrane.wgsl:218-237) by one wave.
-__device__ __forceinline__ float2 wave_sample_grid(const Dev& d, const float x[3], uint32_t fu, uint32_t fw, uint32_t lane) {
-  const Stencil st = stencil_of(x);
-  int a, b, c;
-  const uint32_t cell = stencil_cell(d, st, lane, a, b, c);
-  float v = 0.0f, w = 0.0f;
-  if (cell != NONE) {
-    w = st.w[0][a] * st.w[1][b] * st.w[2][c] * from_fp(accr(d, fw, cell), NATRAY_FP_W);
-    v = w * cfr(d, fu, cell);
-  }
-  return ordered_sum27(v, w, cell != NONE);
-}
 // sample_pressure (bullet.wgsl:113-134) by one wave.
 __device__ __forceinline__ float2 wave_sample_pressure(const Dev& d, const float x[3], bool gas_only, uint32_t lane) {
   const Stencil st = stencil_of(x);
@@ -3092,19 +3080,6 @@ __device__ __forceinline__ float2 wave_sample_pressure(const Dev& d, const float
   }
   return ordered_sum27(v, w, present);
 }
-// push_water (bullet.wgsl:136-150) by one wave (integer atomics: any order gives the same sums).
-__device__ __forceinline__ void wave_push_water(const Dev& d, const float X[3], uint32_t dim, float j, float sw, uint32_t lane) {
-  if (sw <= 1e-6f || j == 0.0f) return;
-  const natray_params* U = d.U;
-  const float lx[3] = {X[0] - (dim == 0 ? 0.0f : 0.5f), X[1] - (dim == 1 ? 0.0f : 0.5f), X[2] - (dim == 2 ? 0.0f : 0.5f)};
-  const Stencil st = stencil_of(lx);
-  const float hs = U->fluid[3];
-  const float kk = j / (U->fluid[0] * hs * hs * hs * sw);
-  int a, b, c;
-  const uint32_t cell = stencil_cell(d, st, lane, a, b, c);
-  if (cell == NONE || accr(d, NATRAY_A_WT(dim), cell) <= 0) return;
-  atomicAdd(&accr(d, NATRAY_A_BU + dim, cell), to_fp(kk * st.w[0][a] * st.w[1][b] * st.w[2][c], NATRAY_FP_M));
-}
 // in_cavity (bullet.wgsl:156-170) by one wave: lane l < 32 tests section l.
 __device__ __forceinline__ bool wave_in_cavity(const Dev& d, uint32_t bi, b3 p, float t, uint32_t lane) {
   bool hit = false;
@@ -3120,6 +3095,47 @@ __device__ __forceinline__ bool wave_in_cavity(const Dev& d, uint32_t bi, b3 p, 
     }
   }
   return __ballot(hit) != 0ull;
+}
+
+// The 4^3 cells around a patch hold every stencil it samples: the three face lattices, the
+// cell centres and the pushes. Per axis those stencils' bases are floor(X - 1) or one more, so a
+// wave loads the neighbourhood once, lane l taking cell b0 + (l >> 4, (l >> 2) & 3, l & 3), and
+// each stencil is a 3^3 sub-block of it at offsets 0 or 1 per axis.
+struct Sub {
+  int off[3];
+  float w[3][3];
+};
+__device__ __forceinline__ Sub sub_of(const float xs[3], const int b0[3]) {
+  Sub st;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float bf = floorf(xs[k] - 0.5f);
+    st.off[k] = __builtin_amdgcn_readfirstlane((int)bf - b0[k]);
+    axis_weights(xs[k] - bf, st.w[k]);
+  }
+  return st;
+}
+__device__ __forceinline__ float pick3(const float w[3], int i) { return i == 0 ? w[0] : (i == 1 ? w[1] : w[2]); }
+// This lane's (a, b, c) in the sub-block, if it is in it.
+__device__ __forceinline__ bool in_sub(const Sub& st, uint32_t lane, int& a, int& b, int& c) {
+  a = (int)(lane >> 4) - st.off[0];
+  b = (int)((lane >> 2) & 3u) - st.off[1];
+  c = (int)(lane & 3u) - st.off[2];
+  return (unsigned)a < 3u && (unsigned)b < 3u && (unsigned)c < 3u;
+}
+// The sums over the sub-block's cells where `present`, in the sequential loop's (a, b, c) order.
+__device__ __forceinline__ float2 sub_sum(const Sub& st, float v, float w, bool present) {
+  const unsigned long long m = __ballot(present);
+  float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const int ln = ((i / 9 + st.off[0]) << 4) | (((i / 3) % 3 + st.off[1]) << 2) | (i % 3 + st.off[2]);
+    if ((m >> ln) & 1ull) {
+      sx = sx + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), ln));
+      sy = sy + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), ln));
+    }
+  }
+  return make_float2(sx, sy);
 }
 
 // Whether bullet b takes part in this step (it joins at the first step that starts once it has
@@ -3172,19 +3188,67 @@ __global__ __launch_bounds__(256) void k_bullet_patches(L l) {
   const float area = pp.w;
   const b3 Xv = (p - origin) / hh;
   const float X[3] = {Xv.x, Xv.y, Xv.z};
+  const float xc[3] = {X[0] - 0.5f, X[1] - 0.5f, X[2] - 0.5f};
+  // The neighbourhood: one cell per lane, and the fields every stencil reads there.
+  int b0[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) b0[k] = __builtin_amdgcn_readfirstlane((int)floorf(xc[k] - 0.5f) < (int)floorf(X[k] - 0.5f)
+                                                                      ? (int)floorf(xc[k] - 0.5f)
+                                                                      : (int)floorf(X[k] - 0.5f));
+  const uint32_t cell = cell_at(d, b0[0] + (int)(lane >> 4), b0[1] + (int)((lane >> 2) & 3u), b0[2] + (int)(lane & 3u));
+  const bool here = cell != NONE;
+  int32_t wt_i[3] = {0, 0, 0}, wc_i = 0;
+  float u_c[3] = {0.0f, 0.0f, 0.0f}, ty = 0.0f, p_c = 0.0f;
+  if (here) {
+#pragma unroll
+    for (uint32_t dim = 0; dim < 3; ++dim) {
+      wt_i[dim] = accr(d, NATRAY_A_WT(dim), cell);
+      u_c[dim] = cfr(d, NATRAY_F_U + dim, cell);
+    }
+    wc_i = accr(d, NATRAY_A_WC, cell);
+    ty = cfr(d, NATRAY_F_TYPE, cell);
+    p_c = cfr(d, NATRAY_F_P, cell);
+  }
+  // sample_grid (membrane.wgsl:218-237) on the three face lattices.
   float uw[3], sw[3];
+  Sub face[3];
 #pragma unroll
   for (uint32_t dim = 0; dim < 3; ++dim) {
     const float xs[3] = {X[0] - (dim == 0 ? 0.0f : 0.5f), X[1] - (dim == 1 ? 0.0f : 0.5f), X[2] - (dim == 2 ? 0.0f : 0.5f)};
-    const float2 sg = wave_sample_grid(d, xs, NATRAY_F_U + dim, NATRAY_A_WT(dim), lane);
+    face[dim] = sub_of(xs, b0);
+    int a, b, c;
+    float v = 0.0f, wgt = 0.0f;
+    if (here && in_sub(face[dim], lane, a, b, c)) {
+      wgt = pick3(face[dim].w[0], a) * pick3(face[dim].w[1], b) * pick3(face[dim].w[2], c) * from_fp(wt_i[dim], NATRAY_FP_W);
+      v = wgt * u_c[dim];
+    }
+    const float2 sg = sub_sum(face[dim], v, wgt, here);
     uw[dim] = sg.y > 1e-6f ? sg.x / sg.y : 0.0f;
     sw[dim] = sg.y;
   }
-  const float xc[3] = {X[0] - 0.5f, X[1] - 0.5f, X[2] - 0.5f};
-  float fill = fmin_(fmax_(wave_sample_grid(d, xc, NATRAY_F_TYPE, NATRAY_A_WC, lane).y / 8.0f, 0.0f), 1.0f);
+  // The water around it (sample_grid of F_TYPE weighted by A_WC) and the gas's pressure
+  // (sample_pressure over GAS cells), both on the cell centres.
+  const Sub cen_s = sub_of(xc, b0);
+  float fill, p_gas;
+  {
+    int a, b, c;
+    float wf = 0.0f, vf = 0.0f, wg = 0.0f, vg = 0.0f;
+    const bool in = in_sub(cen_s, lane, a, b, c);
+    const bool gas = here && !(ty == SOLID || ty == WATER);
+    if (here && in) {
+      const float w3 = pick3(cen_s.w[0], a) * pick3(cen_s.w[1], b) * pick3(cen_s.w[2], c);
+      wf = w3 * from_fp(wc_i, NATRAY_FP_W);
+      vf = wf * ty;
+      if (gas) {
+        wg = w3;
+        vg = w3 * p_c;
+      }
+    }
+    fill = fmin_(fmax_(sub_sum(cen_s, vf, wf, here).y / 8.0f, 0.0f), 1.0f);
+    const float2 gs = sub_sum(cen_s, vg, wg, gas);
+    p_gas = gs.y > 1e-6f ? gs.x / gs.y : 0.0f;
+  }
   if (wave_in_cavity(d, bi, p, t_end - step, lane)) fill = 0.0f;
-  const float2 gs = wave_sample_pressure(d, xc, true, lane);
-  const float p_gas = gs.y > 1e-6f ? gs.x / gs.y : 0.0f;
   const b3 vp = bv + cross3(w, r);
   const b3 rel = vp - b3{uw[0], uw[1], uw[2]};
   const float sp = len3(rel);
@@ -3197,9 +3261,21 @@ __global__ __launch_bounds__(256) void k_bullet_patches(L l) {
   const float rtl = len3(rt);
   if (rtl > 1e-9f) fw = fw - (rt * (fill * CF_SKIN * 0.5f * rho * rtl)) * area;
   const b3 fp = fw - n * ((1.0f - fill) * fmax_(p_gas, NATRAY_P_FLOOR) * area);
-  wave_push_water(d, X, 0, -fw.x * step, sw[0], lane);
-  wave_push_water(d, X, 1, -fw.y * step, sw[1], lane);
-  wave_push_water(d, X, 2, -fw.z * step, sw[2], lane);
+  // push_water (bullet.wgsl:136-150) on the three face lattices: integer atomics, any order.
+  {
+    const float hs = U->fluid[3];
+    const float fwd[3] = {fw.x, fw.y, fw.z};
+#pragma unroll
+    for (uint32_t dim = 0; dim < 3; ++dim) {
+      const float j = -fwd[dim] * step;
+      if (sw[dim] <= 1e-6f || j == 0.0f) continue;
+      const float kk = j / (U->fluid[0] * hs * hs * hs * sw[dim]);
+      int a, b, c;
+      if (here && wt_i[dim] > 0 && in_sub(face[dim], lane, a, b, c))
+        atomicAdd(&accr(d, NATRAY_A_BU + dim, cell),
+                  to_fp(kk * pick3(face[dim].w[0], a) * pick3(face[dim].w[1], b) * pick3(face[dim].w[2], c), NATRAY_FP_M));
+    }
+  }
   if (lane == 0) {
     const b3 tq = cross3(r, fp);
     const bool wet_face = wn > 0.0f && fill > 0.0f;
