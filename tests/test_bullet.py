@@ -78,3 +78,24 @@ def test_bullets_removed_leave_no_source():
         assert_same_float(f"after removal {step} particles", sim.particles(), orc.particles())
     sim.close()
     orc.close()
+
+
+def test_bad_bullets_are_refused_before_any_launch():
+    s = scenes.bullet_scene(side=32, n=3200)
+    say(f"[bullet] argument checks, N={s.n}")
+    sim = Sim.from_scene(s)
+    b, pt = scenes.bullet_shot()
+    five = np.stack([pt] * 5)
+    with pytest.raises(Exception):
+        abi.check(abi.lib().natray_sim_set_bullets(sim._h, 5, abi.ptr(np.frombuffer(bytes(b) * 5, np.uint8).copy()),
+                                                   abi.ptr(five)))
+    bad = abi.Bullet.from_buffer_copy(bytes(b))
+    bad.info[1] = abi.BULLET_PATCHES + 1
+    with pytest.raises(Exception):
+        sim.set_bullets([bad], pt[None])
+    # the scene still steps as one without bullets, bit for bit
+    orc = oracle.Oracle.from_scene(s, "f32")
+    compare_state("no bullets", sim.run(3, iters=200), orc.run(3, 200), keys=KEYS)
+    assert_same_float("no bullets particles", sim.particles(), orc.particles())
+    sim.close()
+    orc.close()
