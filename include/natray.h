@@ -400,10 +400,11 @@ int natray_sim_pack_keyframe(natray_sim* sim, uint32_t n_tris, const uint32_t* a
  * begin_step's choice of dt (fluid.wgsl:36-50). Not part of this library: the flight through
  * the air before the bullet arrives (src/flight.rs), the rubber's contact with it
  * (rubber_contact, bullet.wgsl:519-611, part of the membrane's own substeps; its impulses and
- * held rubber are zero here) and the renderer's pose log. */
+ * held rubber are zero here). The renderer's pose log is kept (natray_sim_read_pose_log). */
 #define NATRAY_MAX_BULLETS 4u       /* MAX_SHOTS (src/params.rs:49) */
 #define NATRAY_BULLET_PATCHES 576u  /* MAX_PATCHES (src/bullet.rs:9) */
 #define NATRAY_BULLET_SECTIONS 32u  /* N_SEC (src/bullet.rs:10) */
+#define NATRAY_BULLET_LOG 256u      /* LOG_CAP (src/bullet.rs:11): pose log entries per bullet */
 
 /* == Bullet (bullet.wgsl:16-34, GpuBullet src/bullet.rs:19-28), 112 bytes. */
 typedef struct natray_bullet {
@@ -428,6 +429,11 @@ int natray_sim_set_bullets(natray_sim* sim, uint32_t n, const natray_bullet* bul
  * sections[n][NATRAY_BULLET_SECTIONS][12] (the reference's Section, bullet.wgsl:42-50: a.xyz
  * and when made (s), b.xyz and d(R^2)/dt, R^2 at the start). Either array may be null. */
 int natray_sim_read_bullets(natray_sim* sim, natray_bullet* bullets, float* sections);
+/* The bullets' pose log (bullet.wgsl:52-57, 206-209, 355): log[n][NATRAY_BULLET_LOG][12], the
+ * entry of step s at s % NATRAY_BULLET_LOG: position and the f32 bits of t_ns, rotation,
+ * velocity and roll. A step a bullet is not in the grid zeroes its entry's first four words.
+ * The renderer reads it as Sim::read_poses does (src/sim.rs:650-667). */
+int natray_sim_read_pose_log(natray_sim* sim, float* log);
 
 /* With NATRAY_SCENE_AIR: each cell's gas after the last phase that wrote it, air[5][64 n] for
  * the n blocks in use (density kg/m^3, momentum x, y, z, total energy per volume J/m^3; 0 in

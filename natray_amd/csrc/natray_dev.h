@@ -112,6 +112,7 @@ struct Dev {
   const float* patches;
   float* sections;
   float* bullet_rec;  // [4][576][12] + [4][576]: each patch's share of bullet_step (k_bullet_patches)
+  float* pose_log;    // [4][256][12]: each bullet's pose per step (the renderer's log)
   uint32_t n_bullets;
   const Dev* self;  // a device copy of this struct, for device functions that take it by pointer
   uint32_t N;

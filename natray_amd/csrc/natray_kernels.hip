@@ -3310,8 +3310,12 @@ __global__ __launch_bounds__(256) void k_bullet_body(L l) {
   const natray_bullet b = *B;
   const uint32_t state = bullet_state(d, b);
   __syncthreads();  // every thread has read the bullet before thread 0 writes it
+  float4* pose = reinterpret_cast<float4*>(d.pose_log) + ((size_t)bi * NATRAY_BULLET_LOG + d.st->step % NATRAY_BULLET_LOG) * 3;
   if (state != 1u) {
-    if (li == 0) B->info[2] = state;
+    if (li == 0) {
+      B->info[2] = state;
+      pose[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     return;
   }
   const float step = dt_of(d), t_end = now_of(d), hh = U->origin[3];
@@ -3433,6 +3437,9 @@ __global__ __launch_bounds__(256) void k_bullet_body(L l) {
     B->l[0] = lnew.x, B->l[1] = lnew.y, B->l[2] = lnew.z, B->l[3] = pace;
     B->reach[0] = x.x, B->reach[1] = x.y, B->reach[2] = x.z, B->reach[3] = reach > 0.0f ? reach + R_HAND * hh : 0.0f;
     for (int k = 0; k < 4; ++k) B->info[k] = info[k];
+    pose[0] = make_float4(x.x, x.y, x.z, __uint_as_float(d.st->t_ns));
+    pose[1] = qn;
+    pose[2] = make_float4(v.x, v.y, v.z, roll);
   }
   // Sections thinner than the grid push the water out of the way as they open: one (section,
   // stencil cell) pair per thread.

@@ -870,7 +870,8 @@ int natray_sim_set_bullets(natray_sim* s, uint32_t n, const natray_bullet* bulle
     float* pt = nullptr;
     if ((e = dalloc(s, &D.bullets, NATRAY_MAX_BULLETS)) || (e = dalloc(s, &pt, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 8)) ||
         (e = dalloc(s, &D.sections, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 12)) ||
-        (e = dalloc(s, &D.bullet_rec, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 13)))
+        (e = dalloc(s, &D.bullet_rec, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 13)) ||
+        (e = dalloc(s, &D.pose_log, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_LOG * 12)))
       return e;
     D.patches = pt;
   }
@@ -879,6 +880,7 @@ int natray_sim_set_bullets(natray_sim* s, uint32_t n, const natray_bullet* bulle
     HIPCHK(hipMemcpyAsync(const_cast<float*>(D.patches), patches, (size_t)n * NATRAY_BULLET_PATCHES * 32,
                           hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipMemsetAsync(D.sections, 0, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 48, s->stream));
+    HIPCHK(hipMemsetAsync(D.pose_log, 0, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_LOG * 48, s->stream));
   }
   // Without bullets nothing clears their acc fields any more, and the solve reads A_SRC.
   if (!n && D.n_bullets)
@@ -903,6 +905,14 @@ int natray_sim_read_bullets(natray_sim* s, natray_bullet* bullets, float* sectio
   if (n && bullets) HIPCHK(hipMemcpyAsync(bullets, s->D.bullets, (size_t)n * sizeof(natray_bullet), hipMemcpyDeviceToHost, s->stream));
   if (n && sections)
     HIPCHK(hipMemcpyAsync(sections, s->D.sections, (size_t)n * NATRAY_BULLET_SECTIONS * 48, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
+int natray_sim_read_pose_log(natray_sim* s, float* log) {
+  if (!s || !log) return fail(NATRAY_E_ARG, "null argument");
+  const uint32_t n = s->D.n_bullets;
+  if (n) HIPCHK(hipMemcpyAsync(log, s->D.pose_log, (size_t)n * NATRAY_BULLET_LOG * 48, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return NATRAY_OK;
 }

@@ -228,6 +228,12 @@ class Sim:
         abi.check(abi.lib().natray_sim_read_bullets(self._h, C.cast(arr, C.c_void_p), abi.ptr(sec)))
         return list(arr)[:n], sec
 
+    def pose_log(self) -> np.ndarray:
+        """The bullets' pose log (natray_sim_read_pose_log): [n][256][12]."""
+        out = np.zeros((getattr(self, "_bullets_n", 0), abi.BULLET_LOG, 12), np.float32)
+        abi.check(abi.lib().natray_sim_read_pose_log(self._h, abi.ptr(out)))
+        return out
+
     def pack_water(self) -> np.ndarray:
         """The water's keyframe words (natray_sim_pack_water): [N][2] uint32."""
         out = np.zeros((self.n, 2), np.uint32)

@@ -66,6 +66,7 @@ def lib() -> C.CDLL:
                 "pack_keyframe": (C.c_uint32, [P, C.c_uint32, P, P, C.c_uint32, P]),
                 "set_bullets": (C.c_int, [P, C.c_uint32, P, P]),
                 "read_bullets": (None, [P, P, P]),
+                "read_pose_log": (None, [P, P]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -257,6 +258,12 @@ class Oracle:
         sec = np.zeros((n, abi.BULLET_SECTIONS, 12), np.float64)
         self._f("read_bullets")(self._h, C.cast(arr, C.c_void_p), abi.ptr(sec))
         return list(arr)[:n], sec
+
+    def pose_log(self) -> np.ndarray:
+        """The bullets' pose log [n][256][12] (float32)."""
+        out = np.zeros((getattr(self, "_bullets_n", 0), abi.BULLET_LOG, 12), np.float32)
+        self._f("read_pose_log")(self._h, abi.ptr(out))
+        return out
 
     def pack_water(self) -> np.ndarray:
         out = np.zeros((self.n, 2), np.uint32)

@@ -124,6 +124,7 @@ struct osim {
   } bullets[NATRAY_MAX_BULLETS];
   REAL* patches;
   REAL* sections;
+  REAL* pose_log; /* [n][256][12]: the renderer's log of each bullet's pose per step */
   uint32_t n_bullets;
 };
 
@@ -1693,8 +1694,10 @@ static void bullet_step_one(osim* s, uint32_t bi) {
     sample_pressure(s, xs, 0, sp);
     pf = sp[1] > (REAL)1e-6 ? sp[0] / sp[1] : (REAL)0;
   }
+  REAL* pose = &s->pose_log[((size_t)bi * NATRAY_BULLET_LOG + s->st.step % NATRAY_BULLET_LOG) * 12];
   if (state != 1u) {
     B->info[2] = state;
+    pose[0] = pose[1] = pose[2] = pose[3] = 0;
     return;
   }
   const REAL conj[4] = {-b.q[0], -b.q[1], -b.q[2], b.q[3]};
@@ -1822,6 +1825,11 @@ static void bullet_step_one(osim* s, uint32_t bi) {
     B->l[0] = lnew.x, B->l[1] = lnew.y, B->l[2] = lnew.z, B->l[3] = pace;
     B->reach[0] = x.x, B->reach[1] = x.y, B->reach[2] = x.z, B->reach[3] = reach > (REAL)0 ? reach + R_HAND * hh : (REAL)0;
     for (int k = 0; k < 4; ++k) B->info[k] = info[k];
+    float tbits;
+    memcpy(&tbits, &s->st.t_ns, 4);
+    pose[0] = x.x, pose[1] = x.y, pose[2] = x.z, pose[3] = (REAL)tbits;
+    for (int k = 0; k < 4; ++k) pose[4 + k] = qn[k];
+    pose[8] = v.x, pose[9] = v.y, pose[10] = v.z, pose[11] = roll;
   }
   /* Sections thinner than the grid push the water out of the way as they open. */
   for (uint32_t li = 0; li < B_SEC; ++li) {
@@ -2206,6 +2214,7 @@ void F(destroy)(osim* s) {
   free(s->touch);
   free(s->patches);
   free(s->sections);
+  free(s->pose_log);
   free(s->air_tmp);
   free(s->prev_ptr);
   free(s->block_list_prev);
@@ -2453,7 +2462,8 @@ int F(set_bullets)(osim* s, uint32_t n, const natray_bullet* bullets, const floa
   if (n && !s->patches) {
     s->patches = (REAL*)calloc((size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 8, sizeof(REAL));
     s->sections = (REAL*)calloc((size_t)NATRAY_MAX_BULLETS * B_SEC * 12, sizeof(REAL));
-    if (!s->patches || !s->sections) return NATRAY_E_NOMEM;
+    s->pose_log = (REAL*)calloc((size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_LOG * 12, sizeof(REAL));
+    if (!s->patches || !s->sections || !s->pose_log) return NATRAY_E_NOMEM;
   }
   for (uint32_t k = 0; k < n; ++k) {
     const natray_bullet* b = &bullets[k];
@@ -2466,6 +2476,7 @@ int F(set_bullets)(osim* s, uint32_t n, const natray_bullet* bullets, const floa
   if (n) {
     for (size_t i = 0; i < (size_t)n * NATRAY_BULLET_PATCHES * 8; ++i) s->patches[i] = (REAL)patches[i];
     memset(s->sections, 0, (size_t)NATRAY_MAX_BULLETS * B_SEC * 12 * sizeof(REAL));
+    memset(s->pose_log, 0, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_LOG * 12 * sizeof(REAL));
   }
   if (!n && s->n_bullets)
     memset(&s->acc[(size_t)NATRAY_A_BU * cap_cells(s)], 0, (size_t)4 * cap_cells(s) * 4);
@@ -2485,6 +2496,10 @@ void F(read_bullets)(const osim* s, natray_bullet* bullets, double* sections) {
   }
   if (sections)
     for (size_t i = 0; i < (size_t)s->n_bullets * B_SEC * 12; ++i) sections[i] = (double)s->sections[i];
+}
+
+void F(read_pose_log)(const osim* s, float* log) {
+  for (size_t i = 0; i < (size_t)s->n_bullets * NATRAY_BULLET_LOG * 12; ++i) log[i] = (float)s->pose_log[i];
 }
 
 void F(read_wet)(const osim* s, uint32_t* touch) {

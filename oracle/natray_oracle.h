@@ -52,6 +52,7 @@ typedef struct osim osim;
                             uint32_t max_words, float* spray_w);                                    \
   int P##set_bullets(osim* s, uint32_t n, const natray_bullet* bullets, const float* patches);      \
   void P##read_bullets(const osim* s, natray_bullet* bullets, double* sections);                    \
+  void P##read_pose_log(const osim* s, float* log);                                                 \
   void P##elementary(int which, const float* a, const float* b, float* out, uint32_t n);           \
   void P##set_threads(int n);
 

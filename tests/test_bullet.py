@@ -1,7 +1,7 @@
 """Bullets on the GPU (natray_sim_set_bullets) against the oracle, step by step: two bullets
 fired into a water sphere, one arriving later and at an angle, through their flight in the
-air, their entry and the cavity they open. Particles, State, the bullets and their cavity
-sections bit for bit, with and without the step graphs."""
+air, their entry and the cavity they open. Particles, State, the bullets, their cavity
+sections and their pose log bit for bit, with and without the step graphs."""
 import sys
 
 import numpy as np
@@ -49,6 +49,8 @@ def test_bullets_bit_exact(graph):
         for k in range(2):
             assert_same_float(f"step {step} bullet {k}", gb[k].as_array(), ob[k].as_array())
         assert_same_float(f"step {step} sections", gs, osec.astype(np.float32))
+        np.testing.assert_array_equal(sim.pose_log().view(np.uint32), orc.pose_log().view(np.uint32),
+                                      err_msg=f"step {step} pose log")
         assert_same_float(f"step {step} particles", sim.particles(), orc.particles())
     # both joined, the first is in the water with its cavity open
     assert gb[0].info[2] == 1 and gb[1].info[2] == 1

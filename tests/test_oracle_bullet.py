@@ -10,6 +10,7 @@ what the mathematics and the physics fix:
   form of a torque-free symmetric top; the roll advances at L.axis / Ix; no cavity, no push;
 - the clock: a step is cut to end at the arrival, and the bullet joins in the next step;
   one that leaves the domain is gone;
+- the pose log holds each step's pose under step % 256, a zero time where the bullet waited;
 - in the water: the cavity source the sections splat sums to their volume rate
   (pi d(R^2)/dt |b - a| each, the B-spline weights summing to 1), and bullet_apply adds
   exactly the fixed-point pushes to the face velocities a twin without bullets computes.
@@ -225,3 +226,21 @@ def test_the_water_slows_the_bullet_and_the_cavity_opens():
     assert B.info[3] >= 10 and B.reach[3] > 0
     made = sec[0][sec[0, :, 7] > 0]
     assert (made[:, 8] > 0).all() and (np.diff(np.sort(made[:, 3])) > 0).all()
+
+
+def test_the_pose_log_records_each_step_in_the_grid():
+    s = scenes.bullet_scene()
+    b, pt = scenes.bullet_shot(arrive_ns=30000)
+    o = oracle.Oracle.from_scene(s, "f32")
+    o.set_bullets([b], pt[None])
+    for k in range(6):
+        st = o.run(1, 200)
+        (B,), _ = o.bullets()
+        e = o.pose_log()[0, st.step % abi.BULLET_LOG]
+        if B.info[2] == 1:
+            assert list(e[:3]) == list(B.x[:3]) and e[3:4].view(np.uint32)[0] == st.t_ns
+            assert list(e[4:8]) == list(B.q) and list(e[8:11]) == list(B.v[:3]) and e[11] == B.v[3]
+        else:  # waiting for its arrival: the entry's position and time are zero
+            assert k == 0 and not e[:4].any()
+    log = o.pose_log()[0]
+    assert (log[1:st.step + 1, 3].view(np.uint32) > 0).sum() == st.step - 1
