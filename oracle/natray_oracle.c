@@ -16,9 +16,11 @@
  *     blocks are folded left to right in fixed chunks of K = ceil(n/256) (at most 256
  *     chunks), then the chunk sums go through the reference's total() (fluid.wgsl:426-439).
  *     For n <= 256 this is exactly the reference's total() over blocks in slot order.
- *   - out-of-scope hooks are stubbed as the reference behaves with no membrane, air or
- *     bullet: GAS pressure 0, acc fields 9..18 zero, carve/break_up/ground_touch no-ops
- *     (death only clears vel.w).
+ *   - the membrane coupling, the air boundary, the bullets and the spray are restated below
+ *     and are off unless set (set_membrane, NATRAY_SCENE_AIR, set_bullets, NATRAY_SCENE_SPRAY):
+ *     off, they behave as the reference without them (GAS pressure 0, their acc fields zero,
+ *     carve, break_up and ground_touch no-ops, death only clearing vel.w). The rubber's contact
+ *     with a bullet (bullet_hit) belongs to the membrane's own substeps and is always zero.
  *   - scene extensions (box walls, moving capsule solid, spherical mask, fixed dt, Jacobi
  *     preconditioner, mixed-precision CG) are defined here first and the HIP kernels follow.
  *
