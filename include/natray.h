@@ -237,8 +237,11 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
 void natray_sim_destroy(natray_sim* sim);
 
 /* == Sim::run (src/sim.rs:585-648): n_steps steps with a CG budget of `iters` iterations per
- * step, stopping at `stop_ns`; reads the State back into `out` (may be NULL). Returns
- * NATRAY_E_OVERFLOW where the reference asserts. */
+ * step (0: the budget the last batches adapted), stopping at `stop_ns`; reads the State back
+ * into `out` (may be NULL). Returns NATRAY_E_OVERFLOW where the reference asserts. With
+ * bullets set, n_steps may not exceed NATRAY_BULLET_LOG: the pose log holds one entry per
+ * step at step % NATRAY_BULLET_LOG and the reference reads it after every batch of at most
+ * that many steps. */
 int natray_sim_run(natray_sim* sim, uint32_t n_steps, uint32_t iters, uint32_t stop_ns, natray_state* out);
 
 /* == Sim::advance (src/sim.rs:498-522), with the reference's batch sizing and CG budget
@@ -254,6 +257,18 @@ int natray_sim_advance(natray_sim* sim, double t_target, uint32_t budget, uint32
  * [8, 4096], else unchanged. */
 uint32_t natray_plan_batch(uint32_t stop_ns, uint32_t t_ns, uint32_t free_dt_ns, double step_ms, uint32_t budget_left);
 uint32_t natray_next_iters(uint32_t iters, uint32_t iters_max, uint32_t unconverged);
+
+/* One natray_sim_run call, as natray_sim_advance issues them (the batches NATRAY_PROF would
+ * show, src/sim.rs:585-648): the steps asked for, the CG budget and stop time they ran with,
+ * the steps taken, and the State's iters_max, unconverged and t_ns after them. */
+typedef struct natray_run_record {
+  uint32_t n_steps, iters, stop_ns, taken, iters_max, unconverged, t_ns, pad;
+} natray_run_record;
+#define NATRAY_RUN_LOG 256u
+/* The last min(runs, max_records) records, oldest first; *count gets how many are kept (at
+ * most NATRAY_RUN_LOG). out may be NULL when max_records is 0. A run that overflowed is
+ * recorded too. */
+int natray_sim_run_log(natray_sim* sim, uint32_t max_records, natray_run_record* out, uint32_t* count);
 
 /* One step with the current budget and no stop time. */
 int natray_sim_step(natray_sim* sim, natray_state* out);

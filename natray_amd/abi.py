@@ -178,6 +178,7 @@ _SIGNATURES = {
     "natray_sim_phase_times": (C.c_int, [_P, _P, _P]),
     "natray_sim_set_graph": (C.c_int, [_P, C.c_int]),
     "natray_sim_counters": (C.c_int, [_P, _P]),
+    "natray_sim_run_log": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(C.c_uint32)]),
     "natray_sim_set_warm_start": (C.c_int, [_P, _P, C.c_int]),
     "natray_sim_set_membrane": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "natray_sim_pack_water": (C.c_int, [_P, _P]),

@@ -79,6 +79,9 @@ struct natray_sim {
   double step_ms = 0.0;
   bool started = false;
   double t = 0.0;
+  // The last NATRAY_RUN_LOG natray_sim_run calls (natray_sim_run_log).
+  natray_run_record run_log[NATRAY_RUN_LOG] = {};
+  uint64_t runs = 0;
   // Step graphs: the phases before the solve and those after it, captured on `stream`.
   bool use_graph = true;
   bool capturing = false;  // no timing events go into a captured graph
@@ -569,6 +572,10 @@ int natray_sim_phase(natray_sim* s, int phase) {
 
 int natray_sim_run(natray_sim* s, uint32_t n_steps, uint32_t iters, uint32_t stop_ns, natray_state* out) {
   if (!s) return fail(NATRAY_E_ARG, "null handle");
+  // The pose log keeps one entry per step at step % NATRAY_BULLET_LOG; the reference reads it
+  // after every batch of at most MAX_BATCH = LOG_CAP steps (src/sim.rs:22,514,628).
+  if (s->D.n_bullets && n_steps > NATRAY_BULLET_LOG)
+    return fail(NATRAY_E_ARG, "with bullets set, at most NATRAY_BULLET_LOG steps per run (the pose log would wrap)");
   // src/sim.rs:586-593
   s->U.limits[2] = iters ? iters : s->iters;
   s->U.limits[3] = stop_ns;
@@ -577,6 +584,7 @@ int natray_sim_run(natray_sim* s, uint32_t n_steps, uint32_t iters, uint32_t sto
   s->h_words[4] = 0;
   s->h_words[5] = 0;
   HIPCHK(hipMemcpyAsync(&s->D.st->iters_max, &s->h_words[4], 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
+  const uint32_t step_before = s->last.step;
   auto t0 = std::chrono::steady_clock::now();
   for (uint32_t k = 0; k < n_steps; ++k) {
     e = enqueue_step(s);
@@ -587,10 +595,24 @@ int natray_sim_run(natray_sim* s, uint32_t n_steps, uint32_t iters, uint32_t sto
   if (n_steps) s->step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n_steps;
   if (out) *out = s->last;
   const natray_state& st = s->last;
+  natray_run_record& rec = s->run_log[s->runs % NATRAY_RUN_LOG];
+  rec = natray_run_record{n_steps, s->U.limits[2], stop_ns, st.step - step_before, st.iters_max, st.unconverged, st.t_ns, 0};
+  ++s->runs;
   if (st.overflow) return fail(NATRAY_E_OVERFLOW, "the water needs more grid blocks than the capacity");
   // src/sim.rs:636-645
   s->unconverged += st.unconverged;
   s->iters = natray_next_iters(s->iters, st.iters_max, st.unconverged);
+  return NATRAY_OK;
+}
+
+int natray_sim_run_log(natray_sim* s, uint32_t max_records, natray_run_record* out, uint32_t* count) {
+  if (!s || !count) return fail(NATRAY_E_ARG, "null argument");
+  const uint64_t kept = std::min<uint64_t>(s->runs, NATRAY_RUN_LOG);
+  const uint32_t n = (uint32_t)std::min<uint64_t>(kept, max_records);
+  *count = (uint32_t)kept;
+  if (n && !out) return fail(NATRAY_E_ARG, "null argument");
+  // The last n records, oldest first.
+  for (uint32_t k = 0; k < n; ++k) out[k] = s->run_log[(s->runs - n + k) % NATRAY_RUN_LOG];
   return NATRAY_OK;
 }
 
@@ -865,15 +887,22 @@ int natray_sim_set_bullets(natray_sim* s, uint32_t n, const natray_bullet* bulle
     if (bullets[k].info[1] > NATRAY_BULLET_PATCHES) return fail(NATRAY_E_ARG, "a bullet has more than NATRAY_BULLET_PATCHES patches");
   HIPCHK(hipStreamSynchronize(s->stream));
   natray::Dev& D = s->D;
-  if (n && !D.bullets) {
+  if (n && !(D.bullets && D.patches && D.sections && D.bullet_rec && D.pose_log)) {
+    // All five buffers or none: a failed allocation leaves the handle without bullets, and a
+    // later call allocates them again (what did succeed is freed with the handle).
     int e;
-    float* pt = nullptr;
-    if ((e = dalloc(s, &D.bullets, NATRAY_MAX_BULLETS)) || (e = dalloc(s, &pt, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 8)) ||
-        (e = dalloc(s, &D.sections, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 12)) ||
-        (e = dalloc(s, &D.bullet_rec, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 13)) ||
-        (e = dalloc(s, &D.pose_log, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_LOG * 12)))
+    natray_bullet* b = nullptr;
+    float *pt = nullptr, *sec = nullptr, *rec = nullptr, *log = nullptr;
+    if ((e = dalloc(s, &b, NATRAY_MAX_BULLETS)) || (e = dalloc(s, &pt, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 8)) ||
+        (e = dalloc(s, &sec, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_SECTIONS * 12)) ||
+        (e = dalloc(s, &rec, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_PATCHES * 13)) ||
+        (e = dalloc(s, &log, (size_t)NATRAY_MAX_BULLETS * NATRAY_BULLET_LOG * 12)))
       return e;
+    D.bullets = b;
     D.patches = pt;
+    D.sections = sec;
+    D.bullet_rec = rec;
+    D.pose_log = log;
   }
   if (n) {
     HIPCHK(hipMemcpyAsync(D.bullets, bullets, (size_t)n * sizeof(natray_bullet), hipMemcpyHostToDevice, s->stream));

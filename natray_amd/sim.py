@@ -157,6 +157,17 @@ class Sim:
         keys = ["steps", "cg_iters", "cg_cell_iters", "blocks", "water_cells", "particles", "blocks_cleared"]
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
+    RUN_LOG_KEYS = ("n_steps", "iters", "stop_ns", "taken", "iters_max", "unconverged", "t_ns")
+
+    def run_log(self) -> list[dict]:
+        """The natray_sim_run calls made so far (the last 256), oldest first: the batches that
+        advance() issued, with their CG budget and stop time (natray_sim_run_log)."""
+        cnt = C.c_uint32()
+        abi.check(abi.lib().natray_sim_run_log(self._h, 0, None, C.byref(cnt)))
+        rows = np.zeros((max(int(cnt.value), 1), 8), np.uint32)
+        abi.check(abi.lib().natray_sim_run_log(self._h, int(cnt.value), abi.ptr(rows), C.byref(cnt)))
+        return [dict(zip(self.RUN_LOG_KEYS, (int(v) for v in r[:7]))) for r in rows[: int(cnt.value)]]
+
     def set_membrane(self, x: np.ndarray, v: np.ndarray, mass: np.ndarray, push: np.ndarray) -> None:
         """The membrane's vertices for the coming steps (natray_sim_set_membrane): positions and
         velocities [n][4], masses [n], push (N) and area (m^2) [n][2]."""
