@@ -29,7 +29,10 @@
  * partition of unity, vanishing first moment, APIC affine reproduction, free-fall grid
  * velocity), binning invariants against brute force, the CG solution against a direct
  * scipy solve of the same matrix, symmetry and definiteness of the block-SGS
- * preconditioner probed with unit vectors, and the begin_step clock against a table.
+ * preconditioner probed with unit vectors, and the begin_step clock against a table; the
+ * velocity extension by hand-worked means on built face patterns (test_oracle_extend.py);
+ * the capsule's solid cells, its face velocity in `bounded`, its `carve` and the spherical
+ * mask by float64 geometry and closed forms (test_oracle_capsule.py).
  */
 #include <float.h>
 #include <math.h>
