@@ -405,6 +405,25 @@ int natray_sim_read_wet(natray_sim* sim, uint32_t* touch);
 int natray_sim_pack_keyframe(natray_sim* sim, uint32_t n_tris, const uint32_t* alive, uint32_t* words,
                              uint32_t max_words, uint32_t* n_words, float* spray_w);
 
+/* The reference's spray capacity SPRAY_CAP (src/sim.rs:13), and the words of the fixed spray
+ * index natray_sim_pack_keyframe_static keeps on each GPU: 4 + 2 x ceil(SPRAY_CAP / 32). */
+#define NATRAY_SPRAY_CAP 1200000u
+#define NATRAY_KF_SCRATCH_WORDS (4u + 2u * ((NATRAY_SPRAY_CAP + 31u) / 32u))
+
+/* No counterpart in the reference. natray_sim_pack_keyframe for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same keyframe, word for word, with no hipMalloc, hipFree or allocating
+ * library call. Differences from natray_sim_pack_keyframe:
+ * - alive (n_tris flags), words (max_words) and spray_w (n x 2 floats, may be null) are DEVICE
+ *   memory the caller owns; the call returns once the keyframe is written;
+ * - the spray index (live bits and their prefix) lives in a fixed device array of
+ *   NATRAY_KF_SCRATCH_WORDS words (300 KB) that each GPU holds until the process ends. A scene
+ *   whose spray capacity (params.limits.y) exceeds NATRAY_SPRAY_CAP slots is NATRAY_E_ARG.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (from several
+ * host threads or handles at once) race, and must be serialised by the caller. */
+int natray_sim_pack_keyframe_static(natray_sim* sim, uint32_t n_tris, const uint32_t* alive, uint32_t* words,
+                                    uint32_t max_words, uint32_t* n_words, float* spray_w);
+
 /* ---- bullets -------------------------------------------------------------------------------
  * The water's side of the reference's bullet model (shaders/bullet.wgsl): each step, after the
  * grid is prepared, every bullet in the grid takes the pressure of the water or gas at each

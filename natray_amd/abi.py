@@ -186,6 +186,7 @@ _SIGNATURES = {
     "natray_sim_read_spray": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "natray_sim_read_wet": (C.c_int, [_P, _P]),
     "natray_sim_pack_keyframe": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
+    "natray_sim_pack_keyframe_static": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
     "natray_sim_set_bullets": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "natray_sim_read_bullets": (C.c_int, [_P, _P, _P]),
     "natray_sim_read_pose_log": (C.c_int, [_P, _P]),

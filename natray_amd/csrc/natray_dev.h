@@ -195,9 +195,14 @@ struct KeyframeArgs {
   uint32_t n_verts, n_tris, n_spray, cw;
   const uint32_t* alive;  // n_tris flags (device memory)
   const uint32_t* index;  // the spray index launch_spray_index built
-  float* spray_w;         // n_spray x 2: each slot's weight and time made
+  float* spray_w;         // n_spray x 2: each slot's weight and time made (may be null)
+  int write_header;       // write the 4 header words from `index` (zeros without spray)
 };
 int launch_pack_keyframe(const Target& t, void* stream, const KeyframeArgs& a);
+// The device address of this GPU's fixed spray-index array (NATRAY_KF_SCRATCH_WORDS words),
+// looked up once per device with that device current.
+#define NATRAY_MAX_DEVICES 64u
+int kf_scratch(int device, uint32_t** out);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also
 // into the network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has
 // run on them (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a

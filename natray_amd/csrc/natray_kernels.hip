@@ -26,6 +26,7 @@
 // contraction, so the kernels reproduce the CPU oracle bit for bit given the same inputs.
 #include <hip/hip_runtime.h>
 #include <float.h>
+#include <mutex>
 #include <type_traits>
 
 #include "natray_dev.h"
@@ -3694,7 +3695,7 @@ __global__ __launch_bounds__(256) void k_pack_spray(const float4* __restrict__ s
     key[spray_word + nw + w] = before;
   }
   const float4 pos = spray[(size_t)i * 3], vel = spray[(size_t)i * 3 + 1], air = spray[(size_t)i * 3 + 2];
-  spray_w[i] = make_float2(vel.w, air.w);
+  if (spray_w) spray_w[i] = make_float2(vel.w, air.w);
   const uint32_t bit = 1u << (i % 32u);
   if ((bits & bit) == 0u) return;
   const uint32_t base = spray_word + 2u * nw + 3u * (before + (uint32_t)__popc(bits & (bit - 1u)));
@@ -3702,6 +3703,46 @@ __global__ __launch_bounds__(256) void k_pack_spray(const float4* __restrict__ s
   key[base] = q.x;
   key[base + 1u] = q.y;
   key[base + 2u] = __float_as_uint(pos.w);
+}
+
+// The 4 header words (keys.wgsl:1-8): spray slots and live slots from the spray index, or zeros
+// without spray.
+__global__ __launch_bounds__(64) void k_key_header(const uint32_t* __restrict__ index, uint32_t* __restrict__ key) {
+  const uint32_t l = threadIdx.x;
+  if (l < 4u) key[l] = (index && l < 2u) ? index[l] : 0u;
+}
+
+// ---- the spray index in a fixed device array (natray_sim_pack_keyframe_static) --------------------
+//
+// For callers that cannot allocate, the keyframe's spray index lives in this array, sized for
+// the reference's spray capacity (NATRAY_SPRAY_CAP slots: 4 + 2 x 37,500 words, 300 KB). Each
+// GPU holds its own copy until the process ends. Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (kf_scratch below); the array's name in
+// host code is not a device address.
+static __device__ __align__(256) uint32_t scratch[NATRAY_KF_SCRATCH_WORDS];
+
+int kf_scratch(int device, uint32_t** out) {
+  static std::mutex lock;
+  static uint32_t* addr[NATRAY_MAX_DEVICES] = {};
+  if (device < 0 || device >= (int)NATRAY_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(lock);
+  if (!addr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return (int)e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return (int)e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) {
+      const hipError_t e2 = hipSetDevice(cur);
+      if (e == hipSuccess) e = e2;
+    }
+    if (e != hipSuccess) return (int)e;
+    if (!p) return (int)hipErrorInvalidSymbol;
+    addr[device] = static_cast<uint32_t*>(p);
+  }
+  *out = addr[device];
+  return 0;
 }
 
 // ---- launch wrappers -------------------------------------------------------------------------------
@@ -3847,6 +3888,7 @@ int launch_pack_keyframe(const Target& t, void* stream, const KeyframeArgs& a) {
   if (!t.d) return (int)hipErrorInvalidValue;
   const Dev& d = *t.d;
   uint32_t* key = static_cast<uint32_t*>(a.key);
+  if (a.write_header) hipLaunchKernelGGL(k_key_header, dim3(1), dim3(64), 0, S(stream), a.index, key);
   NATRAY_LAUNCH(k_pack_water, LK_PARTS, 256, t, S(stream), reinterpret_cast<uint2*>(key + 4));
   const uint32_t mem_word = 4u + 2u * d.N, alive_word = mem_word + 2u * a.n_verts;
   if (a.n_verts)

@@ -1048,6 +1048,42 @@ int natray_sim_pack_keyframe(natray_sim* s, uint32_t n_tris, const uint32_t* ali
   return NATRAY_OK;
 }
 
+int natray_sim_pack_keyframe_static(natray_sim* s, uint32_t n_tris, const uint32_t* alive, uint32_t* words,
+                                    uint32_t max_words, uint32_t* n_words, float* spray_w) {
+  if (!s || !n_words) return fail(NATRAY_E_ARG, "null argument");
+  if (n_tris && !alive) return fail(NATRAY_E_ARG, "null alive flags");
+  natray::Dev& D = s->D;
+  const uint32_t cw = (D.spray_cap + 31u) / 32u;
+  if (D.spray && 4ull + 2ull * cw > NATRAY_KF_SCRATCH_WORDS)
+    return fail(NATRAY_E_ARG, "the scene's spray capacity is larger than the fixed spray index holds (NATRAY_SPRAY_CAP slots)");
+  uint32_t* index = nullptr;
+  int e = natray::kf_scratch(s->device, &index);
+  if (e) return fail(NATRAY_E_HIP, std::string("spray index array: ") + hipGetErrorString((hipError_t)e));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  e = sync_state(s);
+  if (e) return e;
+  const uint32_t n_verts = std::min(s->U.mem[0], D.mem_cap);
+  const uint32_t n_spray = D.spray ? std::min(s->last.spray_count, D.spray_cap) : 0u;
+  uint32_t live = 0;
+  if (D.spray) {
+    // The index is rebuilt on every call: the array keeps whatever the last call on this GPU left.
+    LAUNCHCHK(natray::launch_spray_index(s->tgt, s->stream, n_spray, cw, index));
+    HIPCHK(hipMemcpyAsync(s->h_words + 12, index, 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    live = s->h_words[13];
+  }
+  const uint64_t total = 4ull + 2ull * ((uint64_t)D.N + n_verts) + (n_tris + 31ull) / 32ull + 2ull * ((n_spray + 31ull) / 32ull) +
+                         3ull * live;
+  if (total > 0xffffffffull) return fail(NATRAY_E_ARG, "keyframe larger than 2^32 words");
+  *n_words = (uint32_t)total;
+  if (!words) return NATRAY_OK;
+  if (max_words < total) return fail(NATRAY_E_ARG, "the keyframe needs " + std::to_string(total) + " words");
+  natray::KeyframeArgs a{words, n_verts, n_tris, n_spray, cw, alive, D.spray ? index : nullptr, spray_w, 1};
+  LAUNCHCHK(natray::launch_pack_keyframe(s->tgt, s->stream, a));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NATRAY_OK;
+}
+
 int natray_sim_read_membrane(natray_sim* s, float* water) {
   if (!s || !water) return fail(NATRAY_E_ARG, "null argument");
   const uint32_t n = std::min(s->U.mem[0], s->D.mem_cap);
