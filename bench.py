@@ -38,6 +38,11 @@ PER_CELL = {"bin": 76, "p2g": 36, "grid_prep": 64, "solve": 36, "project": 32, "
 PER_CG_CELL_ITER = 60
 
 
+def note(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def phase_bytes(c: dict) -> dict:
     """Algorithmic bytes of each phase summed over the steps in counter deltas `c`."""
     N, C, C_prev, WI = c["particles"], 64 * c["blocks"], 64 * c["blocks_cleared"], c["cg_cell_iters"]
@@ -163,22 +168,28 @@ def compare_precision(scene, args, plain: dict) -> dict:
     keep = (sc.precision, sc.refresh_every)
     out = {"tol": scene.params.solve[0], "plain_fp32": plain}
     try:
-        for label, prec, refresh in (("mixed_f32_f64", abi.CG_MIXED, 20), ("pipelined_f32_f64", abi.CG_PIPELINED, 10)):
+        for label, prec, refresh, zero in (("plain_fp32_from_zero", abi.CG_FP32, 0, True),
+                                           ("mixed_f32_f64", abi.CG_MIXED, 20, False),
+                                           ("pipelined_f32_f64", abi.CG_PIPELINED, 10, False)):
             sc.precision, sc.refresh_every = prec, refresh
             sim = Sim.from_scene(scene)
+            if zero:
+                sim.set_warm_start(None, 3)
             wall, c, times, st = timed_run(sim, args.warmup, args.steps, args.iters)
             sim.close()
             steps = max(c["steps"], 1)
             out[label] = {"cg_iters_per_step": c["cg_iters"] / steps, "solve_ms_per_step": times["solve"][0] / steps,
                           "ms_per_step": wall * 1e3 / max(args.steps, 1), "unconverged_steps": st.unconverged,
-                          "refresh_every": refresh}
+                          "refresh_every": refresh, "p0": "0" if zero else "particle-carried"}
     finally:
         sc.precision, sc.refresh_every = keep
     for label, key in (("mixed_f32_f64", "solve_speedup_mixed_over_plain"),
                        ("pipelined_f32_f64", "solve_speedup_pipelined_over_plain")):
         m = out[label]["solve_ms_per_step"]
         out[key] = plain["solve_ms_per_step"] / m if m > 0 else None
-    out["warm_start"] = "particle-carried pressure (the reference's p0) in every run"
+        out[key + "_from_zero"] = out["plain_fp32_from_zero"]["solve_ms_per_step"] / m if m > 0 else None
+    out["warm_start"] = ("particle-carried pressure (the reference's p0) in every run but plain_fp32_from_zero, "
+                         "which starts each solve from p = 0 with the same b")
     return out
 
 
@@ -223,7 +234,7 @@ def warm_start_report(scene, args) -> dict:
     val_ratio = ws.residual_ratio(net, val, 64)
     del val
 
-    def run(with_net: bool, pipelined: bool = False) -> dict:
+    def run(with_net: bool, pipelined: bool = False, zero: bool = False) -> dict:
         sc = scene.scene
         keep = (sc.precision, sc.refresh_every)
         if pipelined:
@@ -234,6 +245,8 @@ def warm_start_report(scene, args) -> dict:
             sc.precision, sc.refresh_every = keep
         if with_net:
             sim.set_warm_start(net, 2)
+        elif zero:
+            sim.set_warm_start(None, 3)
         wall, c, times, st = timed_run(sim, args.warmup, args.ws_steps, args.iters)
         stats = sim.ws_stats() if with_net else None
         sim.close()
@@ -248,7 +261,7 @@ def warm_start_report(scene, args) -> dict:
                                                            if stats["carried_rr"] > 0 else None)
         return r
 
-    plain, learned, combined = run(False), run(True), run(True, pipelined=True)
+    plain, learned, combined, zero = run(False), run(True), run(True, pipelined=True), run(False, zero=True)
     return {
         "training": {"tiles": T, "seeds": seeds, "steps_per_seed": args.ws_collect_steps,
                      "collect_s": t_collect, "adam_steps": args.ws_train_steps, "batch_tiles": 32,
@@ -262,8 +275,32 @@ def warm_start_report(scene, args) -> dict:
         # Both added techniques together: the pipelined CG at mixed precision, warm-started.
         "warm_started_pipelined_mixed": combined,
         "solve_speedup_pipelined_mixed_warm_over_plain": plain["solve_ms_per_step"] / combined["solve_ms_per_step"],
+        # Plain FP32 CG from p = 0 (config 5's fallback guess), the same b and tolerance.
+        "plain_from_zero": zero,
+        "solve_speedup_warm_over_plain_from_zero": zero["solve_ms_per_step"] / learned["solve_ms_per_step"],
+        "solve_speedup_pipelined_mixed_warm_over_plain_from_zero": zero["solve_ms_per_step"] / combined["solve_ms_per_step"],
         "note": "solve time includes the warm start (inputs, network, guess check); tol and b are the same",
     }
+
+
+def stated_radius_report(args) -> dict:
+    """The workload's stated geometry: the radius-0.3 sphere filled at 8 particles per cell on
+    256^3 (15.2M particles) instead of the stated 2,097,152 particles, with the same capsule,
+    CFL and CG. Timed like the main run (graph steps, particles resident)."""
+    from natray_amd import Sim, scenes
+
+    s = scenes.stated_radius_impact(iters=args.iters)
+    sim = Sim.from_scene(s)
+    wall, c, times, st = timed_run(sim, args.warmup, args.stated_steps, args.iters)
+    sim.close()
+    steps = max(c["steps"], 1)
+    t_cg = times["solve"][0] * 1e-3
+    return {"workload": s.meta["workload"], "n_particles": s.n, "steps": args.stated_steps,
+            "particle_substeps_per_s": c["particles"] / wall if wall > 0 else 0.0,
+            "ms_per_step": wall * 1e3 / max(args.stated_steps, 1),
+            "cg_iters_per_step": c["cg_iters"] / steps, "blocks_per_step": c["blocks"] / steps,
+            "cg_cell_iters_per_s": c["cg_cell_iters"] / t_cg if t_cg > 0 else 0.0,
+            "solve_ms_per_step": times["solve"][0] / steps, "unconverged_steps": st.unconverged}
 
 
 def batched_report(args) -> dict:
@@ -452,24 +489,65 @@ def accuracy_vs_fp64(scene, checkpoints, iters: int, one_from: int = 20) -> dict
     return out
 
 
-def cpu_baseline(scene, iters: int, budget_s: float, max_steps: int) -> dict:
-    """The FP32 oracle (OpenMP over the host's cores) on the same workload from the same
-    initial state: one untimed step, then steps until `budget_s` of wall time or max_steps."""
+def host_cpu() -> dict:
+    """The host's CPU model and counts, from /proc/cpuinfo and os.cpu_count()."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
+def cpu_threads() -> int:
+    """The host cores this job is given: OMP_NUM_THREADS where the environment sets it (the
+    GPU box allots a share of its host to each GPU and sets it), else every logical CPU."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return n if n > 0 else (os.cpu_count() or 1)
+
+
+def cpu_baseline(scene, iters: int, budget_s: float, runs: int = 5, one_thread_steps: int = 2) -> dict:
+    """The FP32 C restatement of the reference kernels (oracle/; the reference itself has no
+    CPU path and its WGSL-on-lavapipe route cannot run here) on the same workload from the same
+    initial state: one untimed step, then `runs` timed runs of consecutive steps on all the
+    job's host cores, each until budget_s / runs of wall time; the value is the median run.
+    Then `one_thread_steps` steps on one thread."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     oracle.set_threads(threads)
     o = oracle.Oracle.from_scene(scene, "f32")
     o.run(1, iters)
-    steps, t0 = 0, time.perf_counter()
-    while steps < max_steps and time.perf_counter() - t0 < budget_s:
-        o.run(1, iters)
-        steps += 1
-    dt = time.perf_counter() - t0
+    rates, steps_all, t_all = [], 0, 0.0
+    for _ in range(runs):
+        steps, t0 = 0, time.perf_counter()
+        while steps < 1 or time.perf_counter() - t0 < budget_s / runs:
+            o.run(1, iters)
+            steps += 1
+        dt = time.perf_counter() - t0
+        rates.append(scene.n * steps / dt)
+        steps_all += steps
+        t_all += dt
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    o.run(one_thread_steps, iters)
+    dt1 = time.perf_counter() - t0
+    oracle.set_threads(threads)
     o.close()
-    return {"value": scene.n * steps / dt, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (steps 2..{steps + 1}) of the same {scene.n}-particle workload, "
-                      f"FP32 C restatement of the reference kernels (oracle/), OpenMP, {dt:.1f} s"}
+    rates.sort()
+    return {"value": float(np.median(rates)), "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "label": "CPU restatement, not the lavapipe reference",
+            "runs": [float(r) for r in rates], "one_thread_value": scene.n * one_thread_steps / dt1,
+            "host": host_cpu(),
+            "sample": f"median of {runs} runs ({steps_all} steps in all, {t_all:.1f} s) of the same {scene.n}-particle "
+                      f"workload after one untimed step, FP32 C restatement of the reference kernels (oracle/), "
+                      f"OpenMP on {threads} threads; then {one_thread_steps} steps on 1 thread ({dt1:.1f} s)"}
 
 
 def main() -> int:
@@ -484,9 +562,11 @@ def main() -> int:
     ap.add_argument("--no-compare", action="store_true", help="skip the mixed-precision and pipelined comparison runs")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error report against the FP64 oracle")
     ap.add_argument("--accuracy-steps", type=int, default=5)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall time of the CPU baseline's timed runs")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start report")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes report")
+    ap.add_argument("--no-stated", action="store_true", help="skip the stated-radius (15.2M particles) report")
+    ap.add_argument("--stated-steps", type=int, default=40, help="timed steps of the stated-radius workload")
     ap.add_argument("--batch-steps", type=int, default=30, help="timed steps of the 64-scene batch")
     ap.add_argument("--batch-tune-steps", type=int, default=4, help="autotuner rounds (one timed step per candidate each)")
     ap.add_argument("--ws-tiles", type=int, default=50000, help="training tiles to log for the warm start")
@@ -520,9 +600,11 @@ def main() -> int:
         scenes.write_scene_file(scene, args.write_scene)
         print(f"wrote {args.write_scene}: {cfg['workload']}")
         return 0
+    note(f"main run: {cfg['workload']}, {args.warmup} warmup + {args.steps} timed steps")
     sim = Sim.from_scene(scene)
     wall, c, times, st = timed_run(sim, args.warmup, args.steps, args.iters)
     sim.close()
+    note(f"main run: {wall * 1e3 / max(args.steps, 1):.3f} ms per step")
     # The per-phase breakdown: the same steps from the same state, every kernel launched and
     # timed on its own (the timed run above replays graphs and times the solve only).
     sim = Sim.from_scene(scene)
@@ -587,20 +669,29 @@ def main() -> int:
         },
     }
     if args.workload == "impact" and args.precision == "fp32" and not args.no_compare:
+        note("CG forms: plain from p = 0, mixed, pipelined")
         out["precision_compare"] = compare_precision(
             scene, args, {"cg_iters_per_step": out["cg_iters_per_step"],
                           "solve_ms_per_step": times["solve"][0] / max(steps_done, 1),
                           "ms_per_step": out["ms_per_step"], "unconverged_steps": st.unconverged})
+    if args.workload == "impact" and not args.no_stated:
+        note("stated radius: 15.2M particles")
+        out["stated_radius"] = stated_radius_report(args)
     if args.workload == "impact" and not args.no_batch:
+        note("batch of 64 scenes")
         out["batched_scenes"] = batched_report(args)
     if args.workload == "impact" and not args.no_warm_start:
+        note("learned warm start: logging, training, timed runs")
         out["learned_warm_start"] = warm_start_report(scene, args)
     if args.workload == "impact" and not args.no_bullets:
+        note("bullets")
         out["bullets"] = bullet_report(args)
     if args.workload == "impact" and not args.no_accuracy:
+        note("accuracy against the FP64 oracle")
         out["accuracy_vs_fp64_oracle"] = accuracy_vs_fp64(scene, (1, args.accuracy_steps), args.iters)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, args.iters, args.cpu_seconds, 200)
+        note("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(scene, args.iters, args.cpu_seconds)
     print(json.dumps(out), flush=True)
     return 0
 

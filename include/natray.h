@@ -536,7 +536,8 @@ int natray_batch_plan(const uint64_t* weights, const uint32_t* max_each, uint32_
  * o, guess = p_carried + s o on the tiles' water cells (p_carried elsewhere), and the solve
  * starts from the guess when |b - A guess|^2 < |r0|^2 (FP64 sums), else from p_carried. The
  * right-hand side b and the tolerance are the same either way. The network must stay alive
- * while the mode uses it. Resets the statistics of natray_sim_ws_stats. */
+ * while the mode uses it. mode 3: every solve starts from p = 0 (the zero guess), b unchanged;
+ * `net` is not used. Resets the statistics of natray_sim_ws_stats. */
 int natray_sim_set_warm_start(natray_sim* sim, natray_net* net, int mode);
 /* Training samples of the last step's listed tiles (mode 1 or 2), at most max_tiles of them:
  * inputs[tiles][4096][2] as the network got them, targets[tiles][4096] = (p_solved -

@@ -210,4 +210,6 @@ int kf_scratch(int device, uint32_t** out);
 int launch_ws_inputs(const Target& t, void* stream, uint32_t cap, void* net_in);
 int launch_ws_guess(const Target& t, void* stream, const float* pred);
 int launch_ws_samples(const Target& t, void* stream);
+// Warm-start mode 3: the solve starts from the guess buffer (held at zero by the host).
+int launch_ws_zero_start(const Target& t, void* stream);
 }  // namespace natray

@@ -2285,6 +2285,14 @@ __global__ __launch_bounds__(1024) void k_ws_decide(L l) {
 
 // Training samples of the last solve on the listed tiles, [slot][voxel]: target
 // (p - p_carried) / s, the water mask, and the operator's diagonal and water-neighbour bits.
+// The zero guess (warm-start mode 3): this step's solve starts from the guess buffer, which the
+// host holds at zero.
+template <class L>
+__global__ __launch_bounds__(64) void k_ws_zero_start(L l) {
+  const View v = view(l);
+  if (v.wg == 0u && threadIdx.x == 0u) v.d.bar[11] = 1u;
+}
+
 template <class L>
 __global__ __launch_bounds__(256) void k_ws_samples(L l) {
   const View v = view(l);
@@ -3836,6 +3844,10 @@ int launch_ws_guess(const Target& t, void* stream, const float* pred) {
   NATRAY_LAUNCH(k_ws_scatter, LK_WAVES, 256, t, S(stream), pred);
   NATRAY_LAUNCH(k_ws_check, LK_WAVES, 256, t, S(stream));
   NATRAY_LAUNCH(k_ws_decide, LK_ONE, 1024, t, S(stream));
+  return (int)hipGetLastError();
+}
+int launch_ws_zero_start(const Target& t, void* stream) {
+  NATRAY_LAUNCH(k_ws_zero_start, LK_ONE, 64, t, S(stream));
   return (int)hipGetLastError();
 }
 int launch_ws_samples(const Target& t, void* stream) {

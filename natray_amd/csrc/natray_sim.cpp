@@ -176,6 +176,10 @@ uint64_t solve_tag_bound(const natray_sim* s) {
 int enqueue_warm_start(natray_sim* s) {
   const natray::Dev& D = s->D;
   void* st = s->stream;
+  if (s->ws_mode == 3) {
+    LAUNCHCHK(natray::launch_ws_zero_start(s->tgt, st));
+    return NATRAY_OK;
+  }
   const bool guess = s->ws_mode == 2;
   const uint32_t cap = guess ? std::min(D.ws_max, natray::net::max_tiles(s->ws_net)) : D.ws_max;
   LAUNCHCHK(natray::launch_ws_inputs(s->tgt, st, cap, guess ? natray::net::input_buffer(s->ws_net) : nullptr));
@@ -803,7 +807,7 @@ int natray_sim_phase_times(natray_sim* s, double* ms, uint32_t* counts) {
 
 int natray_sim_set_warm_start(natray_sim* s, natray_net* net, int mode) {
   if (!s) return fail(NATRAY_E_ARG, "null handle");
-  if (mode < 0 || mode > 2) return fail(NATRAY_E_ARG, "warm-start mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 3) return fail(NATRAY_E_ARG, "warm-start mode must be 0, 1, 2 or 3");
   if (mode == 2 && !net) return fail(NATRAY_E_ARG, "warm-start mode 2 needs a network");
   HIPCHK(hipStreamSynchronize(s->stream));
   if (mode && !s->ws_ready) {
@@ -812,6 +816,8 @@ int natray_sim_set_warm_start(natray_sim* s, natray_net* net, int mode) {
   }
   s->ws_mode = mode;
   s->ws_net = mode == 2 ? net : nullptr;
+  // Mode 3 starts every solve from the guess buffer held at zero (nothing else writes it then).
+  if (mode == 3) HIPCHK(hipMemsetAsync(s->D.guess, 0, (size_t)s->D.capc * 4, s->stream));
   if (s->ws_ready) {
     HIPCHK(hipMemsetAsync(s->D.ws_stat, 0, 8 * 8, s->stream));
     HIPCHK(hipMemsetAsync(s->D.bar + 10, 0, 2 * 4, s->stream));

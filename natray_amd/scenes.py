@@ -206,6 +206,16 @@ def nearest_cells(side: int, n_cells: int, centre=(0.5, 0.5, 0.5)) -> np.ndarray
     return np.stack([X[order], Y[order], Z[order]], axis=1)
 
 
+def cells_within(side: int, radius: float, centre=(0.5, 0.5, 0.5)) -> int:
+    """How many cells of a side^3 unit-box grid have their centre closer than `radius` (in
+    box units) to `centre`: the cells a sphere of that radius fills at 8 particles per cell."""
+    c2 = np.round(2.0 * np.asarray(centre) * side).astype(np.int64)
+    i = 2 * np.arange(side, dtype=np.int64) + 1
+    dx2, dy2, dz2 = (i - c2[0]) ** 2, (i - c2[1]) ** 2, (i - c2[2]) ** 2
+    lim = (2.0 * radius * side) ** 2  # squared distances times 4, as in nearest_cells
+    return int(sum(int(np.count_nonzero(dx2[x] + dy2[:, None] + dz2[None, :] < lim)) for x in range(side)))
+
+
 def block_order(cells: np.ndarray, side: int) -> np.ndarray:
     """Sorts cells by 4^3 block (slot order), then by cell within the block, so particles of
     one block sit together in memory."""
@@ -368,6 +378,27 @@ def balloon_impact(side: int = 256, n: int = 2_097_152, seed: int = 2, speed: fl
     sc.mask_p = 0.0
     return Scene(U, P, sc, no_limits_state(),
                  {"workload": f"balloon impact {n} particles {side}^3", "r_eff": r_eff, "dx": dx})
+
+
+def stated_radius_impact(side: int = 256, radius: float = 0.3, **kw) -> Scene:
+    """The balloon impact with the sphere of the radius the workload states (0.3 on 256^3)
+    filled at 8 particles per cell, instead of the stated particle count: every cell whose
+    centre lies within the radius holds 8 particles (15.2M at the defaults)."""
+    n = 8 * cells_within(side, radius)
+    s = balloon_impact(side=side, n=n, **kw)
+    s.meta["workload"] = f"balloon impact, stated radius {radius} at 8 particles per cell: {n} particles {side}^3"
+    s.meta["radius"] = radius
+    return s
+
+
+def stated_radius_sphere(side: int = 64, radius: float = 0.25, **kw) -> Scene:
+    """The water sphere with the radius the workload states (0.25 on 64^3) filled at 8 particles
+    per cell (137k particles at the defaults) instead of the stated count."""
+    n = 8 * cells_within(side, radius)
+    s = water_sphere(side=side, n=n, **kw)
+    s.meta["workload"] = f"water sphere, stated radius {radius} at 8 particles per cell: {n} particles {side}^3"
+    s.meta["radius"] = radius
+    return s
 
 
 def batch_impacts(count: int = 64, seed: int = 4, n_lo: int = 16384, n_hi: int = 524288, side_lo: int = 64,

@@ -253,7 +253,8 @@ class Sim:
 
     def set_warm_start(self, net=None, mode: int = 2) -> None:
         """The learned warm start of the solve (natray_sim_set_warm_start): mode 0 off, 1 inputs
-        only (for ws_samples), 2 the network's guess with fallback to the carried pressure."""
+        only (for ws_samples), 2 the network's guess with fallback to the carried pressure, 3
+        every solve from p = 0 (the zero guess; b unchanged)."""
         self._ws_net = net if mode == 2 else None  # the network must outlive its use
         h = net.handle if (net is not None and mode == 2) else None
         abi.check(abi.lib().natray_sim_set_warm_start(self._h, h, mode))

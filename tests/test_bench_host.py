@@ -58,3 +58,36 @@ def test_solve_traffic_is_read_from_the_profile():
     t = bench.measured_traffic("k_solve<false,One>")
     assert t is not None and 1e8 < t < 1e9
     assert bench.measured_traffic("no_such_kernel") is None
+
+
+def test_cpu_baseline_reports_median_threads_and_host(monkeypatch):
+    """The CPU baseline: the median of its timed runs on the job's cores (OMP_NUM_THREADS), a
+    one-thread figure, and the host's CPU model, on a small scene."""
+    import numpy as np
+
+    import bench
+    from natray_amd import scenes
+
+    monkeypatch.setenv("OMP_NUM_THREADS", "2")
+    s = scenes.water_sphere(side=32, n=4096)
+    r = bench.cpu_baseline(s, 200, budget_s=0.5, runs=5, one_thread_steps=1)
+    assert r["cores"] == 2 and r["kind"] == "port" and len(r["runs"]) == 5
+    assert r["value"] == float(np.median(r["runs"])) and r["value"] > 0 and r["one_thread_value"] > 0
+    assert "not the lavapipe" in r["label"] and r["host"]["logical_cpus"] == os.cpu_count()
+
+
+def test_stated_radius_scenes():
+    """The stated geometry: every cell whose centre lies inside the radius holds 8 particles."""
+    import numpy as np
+
+    from natray_amd import scenes
+
+    s = scenes.stated_radius_sphere()
+    side = 64
+    cells = np.unique(np.floor(s.particles[:, :3].astype(np.float64) * side).astype(int), axis=0)
+    d = np.linalg.norm((cells + 0.5) / side - 0.5, axis=1)
+    assert s.n == 8 * len(cells) == 138048 and d.max() < 0.25
+    # Brute force over the whole grid: exactly those cells lie inside.
+    g = (np.indices((side, side, side)).reshape(3, -1).T + 0.5) / side
+    assert int((np.linalg.norm(g - 0.5, axis=1) < 0.25).sum()) == len(cells)
+    assert scenes.cells_within(256, 0.3) * 8 == 15179264

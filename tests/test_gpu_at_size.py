@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 import oracle
-from natray_amd import abi, scenes
+from natray_amd import Sim, abi, scenes
 from tests.test_batch import alone, batched, check_same
 from tests.test_gpu_parity import assert_same_float, compare_state, phase_parity, say
 from tests.test_warm_start import warm_start_parity
@@ -43,10 +43,12 @@ AT_SIZE = {
     "impact_2M_mixed_mem": lambda: solve_groups(scenes.balloon_impact(precision=abi.CG_MIXED), 64),
     "impact_2M_pipelined": lambda: scenes.balloon_impact(precision=abi.CG_PIPELINED),
     "impact_2M_pipelined_mem": lambda: solve_groups(scenes.balloon_impact(precision=abi.CG_PIPELINED), 64),
+    # The stated geometry of the water sphere: radius 0.25 on 64^3 at 8 particles per cell.
+    "sphere_stated_r025_138k": lambda: scenes.stated_radius_sphere(),
 }
 
 # Blocks the scenes hold: more than 256 (several blocks per chunk) in every case.
-MIN_BLOCKS = {"impact_262k_128": 600}
+MIN_BLOCKS = {"impact_262k_128": 600, "sphere_stated_r025_138k": 500}
 
 
 @pytest.mark.parametrize("name", list(AT_SIZE))
@@ -85,3 +87,20 @@ def test_warm_start_at_size(name, c):
     taken = warm_start_parity(name, s, c, steps=STEPS, max_tiles=1024)
     # The first step is at rest (r0 = 0, nothing to take); a c > 0 guess is taken after that.
     assert taken == (STEPS - 1 if c > 0 else 0), (name, c, taken)
+
+
+def test_stated_radius_impact_runs_like_the_oracle():
+    """The balloon impact's stated geometry: radius 0.3 on 256^3 at 8 particles per cell,
+    15,179,264 particles in ~35,000 blocks (K = 138 blocks per chunk, the solve's memory form
+    at 256 workgroups). Two whole steps through natray_sim_run against the oracle's run."""
+    s = scenes.stated_radius_impact()
+    say(f"[at size] stated radius impact N={s.n}")
+    sim = Sim.from_scene(s)
+    orc = oracle.Oracle.from_scene(s, "f32")
+    a = sim.run(2, iters=int(s.params.limits[2]))
+    b = orc.run(2, iters=int(s.params.limits[2]))
+    compare_state("stated radius", a, b)
+    assert_same_float("stated radius particles", sim.particles(), orc.particles())
+    assert a.n_blocks > 30000 and a.solve_iters > 10, (a.n_blocks, a.solve_iters)
+    sim.close()
+    orc.close()

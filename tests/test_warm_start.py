@@ -259,3 +259,31 @@ def test_trained_network_gain():
     say(f"[warm] CG iterations per step: plain {iters['plain']:.2f}, warm-started {iters['net']:.2f}")
     assert iters["net"] <= iters["plain"]
     n.close()
+
+
+@pytest.mark.parametrize("name", ["impact_sgs_32k", "impact_mixed_32k_mem", "impact_pipelined_32k"])
+def test_zero_start_matches_the_oracle(name):
+    """Warm-start mode 3 (every solve from p = 0, b unchanged) against the oracle's solve from a
+    zero guess, phase by phase."""
+    s = SCENES[name]()
+    say(f"[warm] zero start {name}")
+    sim = Sim.from_scene(s)
+    sim.set_warm_start(None, 3)
+    orc = oracle.Oracle.from_scene(s, "f32")
+    zeros = np.zeros(int(s.params.bdims[3]) * 64, np.float32)
+    iters = 0
+    for step in range(4):
+        for ph in range(abi.N_PHASES):
+            if ph == abi.PHASE_SOLVE:
+                orc.set_guess(zeros)
+            sim.phase(ph)
+            orc.phase(ph)
+            tag = f"zero start {name} step {step} after {abi.PHASE_NAMES[ph]}"
+            compare_state(tag, sim.state(), orc.state())
+            if ph == abi.PHASE_SOLVE:
+                compare_grid(tag, sim.grid(), orc.grid())
+        assert_same_float(f"zero start {name} step {step} particles", sim.particles(), orc.particles())
+        iters += int(orc.state().solve_iters)
+    assert iters > 0
+    sim.close()
+    orc.close()
