@@ -853,11 +853,21 @@ constexpr uint32_t SOLVE_THREADS = 1024;
 constexpr uint32_t SOLVE_WAVES = SOLVE_THREADS / 64;
 // Blocks a wave of the solve keeps in registers.
 constexpr uint32_t SOLVE_RMAX = 2;
-// Words of one reduction buffer per chunk: 4 components x 2 halves.
+// Words of one reduction buffer per chunk: 4 components x 2 halves. Word w is component w / H,
+// half w % H (H = 1 for f32, 2 for f64); words 2j and 2j + 1 travel together as one 16-byte
+// access, so the words of a chunk arrive in as few loads as they take.
 constexpr uint32_t RED_SLOTS = 8;
-// Word `slot` of buffer `buf` in chunk c's record.
-__device__ __forceinline__ unsigned long long* red_word(const Dev& d, uint32_t c, uint32_t buf, uint32_t slot) {
-  return d.rwords + (size_t)c * RED_RECORD + buf * RED_SLOTS + slot;
+constexpr uint32_t RED_PAIRS = RED_SLOTS / 2;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// Cache policy of the chunk records' 16-byte loads and stores: sc1, the bits of the agent-scope
+// atomics the other tagged words use.
+constexpr int kRedPolicy = 16;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t red_rsrc(const Dev& d) {
+  return __builtin_amdgcn_make_buffer_rsrc(d.rwords, 0, (int)(RED_WORDS * 8u), 0x00020000);
+}
+// Byte offset of pair j of buffer `buf` in chunk c's record.
+__device__ __forceinline__ uint32_t red_pair(uint32_t c, uint32_t buf, uint32_t j) {
+  return ((c * RED_RECORD + buf * RED_SLOTS) + 2u * j) * 8u;
 }
 
 __device__ __forceinline__ unsigned long long tag_word(uint32_t tag, uint32_t payload) {
@@ -967,12 +977,52 @@ struct NoWait {
   __device__ __forceinline__ bool operator()(uint32_t) const { return true; }
 };
 
+// The register form's block partials of one workgroup in LDS, in the reduction's precision and
+// component-major: v[k][b - b0]. A block's four components: the reduction's values, or (at the
+// start) r.z, r.r, b.b and its water cells (a count, exact in either precision).
+template <class R>
+struct LPart {
+  R v[4][SOLVE_WAVES * SOLVE_RMAX];
+};
+
+// Components k0 and k1 of chunk [lo, hi)'s block partials (`lp` in the register form, Dev::part
+// in the memory form), each summed left to right as total()'s 256-stride sums add blocks. Eight
+// blocks' partials are loaded (from a clamped row, so without branches) before any is added,
+// and the two components' sums run side by side.
+template <class R>
+__device__ __forceinline__ void chunk_fold(const Dev& d, const SolveSpan& sp, const LPart<R>* lp, uint32_t lo, uint32_t hi,
+                                           int k0, int k1, R& a0, R& a1) {
+  a0 = (R)0;
+  a1 = (R)0;
+  for (uint32_t b = lo; b < hi; b += 8u) {
+    R x0[8], x1[8];
+#pragma unroll
+    for (uint32_t i = 0; i < 8u; ++i) {
+      const uint32_t row = min(b + i, hi - 1u);
+      if (sp.in_regs) {
+        x0[i] = lp->v[k0][row - sp.b0];
+        x1[i] = lp->v[k1][row - sp.b0];
+      } else {
+        x0[i] = (R)d.part[row * 4u + (uint32_t)k0];
+        x1[i] = (R)d.part[row * 4u + (uint32_t)k1];
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < 8u; ++i) {
+      if (b + i < hi) {
+        a0 += x0[i];
+        a1 += x1[i];
+      }
+    }
+  }
+}
+
 // Reduction `tag` over components [0, nc) of the block partials (nc = 0: a barrier only).
-// This workgroup's block partials are in `lpart` (blocks b0.., register form) or Dev::part.
+// This workgroup's block partials are in `lp` (blocks b0.., register form) or Dev::part.
 // Every workgroup returns the same totals; false when the solve aborted. `wait` runs in every
 // wave alongside the polling (see NoWait).
 template <class R, class Wait = NoWait>
-__device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const double (*lpart)[4], R* out,
+__device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const LPart<R>* lp, R* out,
                        Wait wait = Wait()) {
   constexpr int H = Halves<R>::N;
   // The chunk sums, then the totals and the abort flag, the last two per tag parity: a wave can
@@ -987,18 +1037,23 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
   uint32_t* abort = d.bar + 2;
   if (t == 0) s_fail[buf] = 0u;
   wg_sync(!sp.in_regs);
-  // total()'s chunk fold, then publish: one word per chunk, component and half. One thread
-  // per chunk and component, each summing its component over the chunk's blocks in order.
-  if (t < (sp.c1 - sp.c0) * (uint32_t)nw) {
-    const uint32_t k = t % (uint32_t)nw, c = sp.c0 + t / (uint32_t)nw, lo = c * sp.K, hi = min(sp.n, lo + sp.K);
+  // total()'s chunk fold, then publish: one thread per chunk and pair of words folds the
+  // components of its two words over the chunk's blocks, left to right, and stores both words
+  // (tag above payload each) as one 16-byte record.
+  const __amdgpu_buffer_rsrc_t rs = red_rsrc(d);
+  const uint32_t words = (uint32_t)(nw * H), P = (words + 1u) / 2u;
+  if (t < (sp.c1 - sp.c0) * P) {
+    // A thread per chunk and pair of words.
+    const uint32_t j = t % P, c = sp.c0 + t / P, lo = c * sp.K, hi = min(sp.n, lo + sp.K);
     if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    R acc = (R)0;
-    if ((int)k < nc) {
-#pragma unroll 8
-      for (uint32_t b = lo; b < hi; ++b) acc += (R)(sp.in_regs ? lpart[b - sp.b0][k] : d.part[b * 4u + k]);
-    }
-#pragma unroll
-    for (int h = 0; h < H; ++h) store_word(red_word(d, c, buf, k * 2u + (uint32_t)h), tag_word(tag, Halves<R>::part(acc, h)));
+    // The components of words 2j and 2j + 1 (the same one for f64, whose halves they are).
+    const int k0 = (int)(2u * j) / H, k1 = (int)(2u * j + 1u) / H;
+    R a0, a1;
+    chunk_fold<R>(d, sp, lp, lo, hi, k0, k1, a0, a1);
+    if (k0 >= nc) a0 = (R)0;
+    if (k1 >= nc) a1 = (R)0;
+    const u32x4 rec = {Halves<R>::part(a0, (int)(2u * j) % H), tag, Halves<R>::part(a1, (int)(2u * j + 1u) % H), tag};
+    __builtin_amdgcn_raw_buffer_store_b128(rec, rs, red_pair(c, buf, j), 0, kRedPolicy);
   }
   // Every chunk's words, summed as total() does (fluid.wgsl:426-439): thread t holds chunk t,
   // then the tree over the 256 threads.
@@ -1006,34 +1061,30 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
   {
     const bool poller = t < 256u && t < sp.m;
     uint32_t pay[RED_SLOTS];
-    const uint32_t full = (1u << (uint32_t)(nw * 2)) - 1u;
-    // Slots of absent halves count as seen; a thread without a chunk has seen everything.
-    uint32_t have = full, spins = 0;
+    const uint32_t full = (1u << P) - 1u;
+    // A thread without a chunk has every pair.
+    uint32_t have = poller ? 0u : full, spins = 0;
 #pragma unroll
-    for (uint32_t s = 0; s < RED_SLOTS; ++s) {
-      pay[s] = 0u;
-      if (poller && (s & 1u) < (uint32_t)H) have &= ~(1u << s);
-    }
-    // A chunk's words are stored together (the last slot's with the last component), so until
-    // the last slot's word has come, only that one is polled: every workgroup polls every
-    // chunk, and polling all of a chunk's words each round multiplies the uncached loads that
-    // compete with the stores being waited for. Each word's own tag still decides whether it
-    // has come.
-    const uint32_t last = (uint32_t)((nw - 1) * 2 + (H - 1));
+    for (uint32_t w = 0; w < RED_SLOTS; ++w) pay[w] = 0u;
     while (true) {
       // Issue every load of the round before looking at any, so one round trip serves both
-      // the chunk words and the waves' own waits.
-      const uint32_t want = ((have >> last) & 1u) ? full & ~have : full & ~have & (1u << last);
-      unsigned long long w[RED_SLOTS];
+      // the chunk words and the waves' own waits. The compiler fence keeps each round's loads
+      // in the round (a record changes under the loop, from other workgroups).
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      u32x4 rec[RED_PAIRS];
 #pragma unroll
-      for (uint32_t s = 0; s < RED_SLOTS; ++s) w[s] = ((want >> s) & 1u) ? load_word(red_word(d, t, buf, s)) : 0ull;
+      for (uint32_t j = 0; j < RED_PAIRS; ++j)
+        rec[j] = (j < P && !((have >> j) & 1u)) ? __builtin_amdgcn_raw_buffer_load_b128(rs, red_pair(t, buf, j), 0, kRedPolicy)
+                                                  : u32x4{0u, 0u, 0u, 0u};
       const bool waited = wait(tag);
 #pragma unroll
-      for (uint32_t s = 0; s < RED_SLOTS; ++s) {
-        if (!((want >> s) & 1u)) continue;
-        if ((uint32_t)(w[s] >> 32) == tag) {
-          pay[s] = (uint32_t)w[s];
-          have |= 1u << s;
+      for (uint32_t j = 0; j < RED_PAIRS; ++j) {
+        if (j >= P || ((have >> j) & 1u)) continue;
+        // Each word's own tag decides; a record counts once both of its words carry the tag.
+        if (rec[j].y == tag && rec[j].w == tag) {
+          pay[2u * j] = rec[j].x;
+          pay[2u * j + 1u] = rec[j].z;
+          have |= 1u << j;
         }
       }
       if (have == full && waited) break;
@@ -1048,7 +1099,7 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
     if (poller) {
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        if (k < nc) v[k] = (R)0 + Halves<R>::join(pay[k * 2], pay[k * 2 + 1]);
+        if (k < nc) v[k] = (R)0 + Halves<R>::join(pay[k * H], pay[k * H + H - 1]);
     }
   }
   if (t < 256u) {
@@ -1132,7 +1183,7 @@ __device__ __forceinline__ Row cell_row(const Dev& d, uint32_t cell, int lx, int
 // from lane 0.
 template <class R>
 __device__ __forceinline__ void init_cell(const Dev& d, uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk,
-                                          unsigned long long* zw, uint32_t ztag, double* prow, const float* p0) {
+                                          unsigned long long* zw, uint32_t ztag, R* prow, const float* p0) {
   const uint32_t capc = d.capc;
   const uint32_t cell = b * 64u + lane;
   const Row w = cell_row(d, cell, lx, ly, lz, lk, p0);
@@ -1151,10 +1202,10 @@ __device__ __forceinline__ void init_cell(const Dev& d, uint32_t b, uint32_t lan
   const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r)), v2 = wave_tree<R>(prod<R>(bv, bv));
   const uint32_t nw = (uint32_t)__popcll(__ballot(w.water));
   if (lane == 0) {
-    prow[0] = (double)v0;
-    prow[1] = (double)v1;
-    prow[2] = (double)v2;
-    prow[3] = (double)nw;
+    prow[0] = v0;
+    prow[1] = v1;
+    prow[2] = v2;
+    prow[3] = (R)nw;
   }
 }
 
@@ -1250,7 +1301,10 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
   float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
   const float* const p0 = start_pressure(d);
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    init_cell<R>(d, b, lane, lx, ly, lz, lk, nullptr, 0u, d.part + b * 4u, p0);
+    R v[4];
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, nullptr, 0u, v, p0);
+    if (lane == 0)
+      for (uint32_t k = 0; k < 4u; ++k) d.part[b * 4u + k] = (double)v[k];
   });
   R tt[4];
   SolveScalars<R> s;
@@ -1394,13 +1448,16 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
   float* const cP = d.cf + (size_t)NATRAY_F_P * capc;
   const float* const p0 = start_pressure(d);
   unsigned long long* const hz = d.hz;
-  __shared__ double lpart[SOLVE_WAVES * SOLVE_RMAX][4];
+  __shared__ LPart<R> lp;
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    init_cell<R>(d, b, lane, lx, ly, lz, lk, d.hz, tag + 1u, lpart[b - b0], p0);
+    R v[4];
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, d.hz, tag + 1u, v, p0);
+    if (lane == 0)
+      for (uint32_t k = 0; k < 4u; ++k) lp.v[k][b - b0] = v[k];
   });
   R tt[4];
   SolveScalars<R> s;
-  bool go = solve_start<R>(d, tt, reduce<R>(d, sp, ++tag, 4, lpart, tt), s);
+  bool go = solve_start<R>(d, tt, reduce<R>(d, sp, ++tag, 4, &lp, tt), s);
   const bool refresh_on = MIXED && d.sc->precision == NATRAY_CG_MIXED && d.sc->refresh_every > 0u;
 
   __shared__ uint32_t lkr[SOLVE_WAVES][SOLVE_RMAX][27];
@@ -1499,9 +1556,9 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
         v = prod<R>(dn, q);
       }
       v = wave_tree<R>(v);
-      if (ln == 0) lpart[b - b0][0] = (double)v;
+      if (ln == 0) lp.v[0][b - b0] = v;
     }
-    if (!reduce<R>(d, sp, ++tag, 1, lpart, tt) || !ok) break;
+    if (!reduce<R>(d, sp, ++tag, 1, &lp, tt) || !ok) break;
     // solve_alpha (fluid.wgsl:536-541)
     s.alpha = tt[0] > (R)0 ? s.rz / tt[0] : (R)0;
     const float fa = (float)s.alpha;
@@ -1548,8 +1605,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
       }
       const R v0 = wave_tree<R>(prod<R>(rr, z)), v1 = wave_tree<R>(prod<R>(rr, rr));
       if (ln == 0) {
-        lpart[b - b0][0] = (double)v0;
-        lpart[b - b0][1] = (double)v1;
+        lp.v[0][b - b0] = v0;
+        lp.v[1][b - b0] = v1;
       }
     }
     // Halo z for the next solve_q: neighbours publish it before their part of the reduction
@@ -1568,7 +1625,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
       }
       return fresh;
     };
-    if (!reduce<R>(d, sp, ++tag, 2, lpart, tt, halo_poll)) break;
+    if (!reduce<R>(d, sp, ++tag, 2, &lp, tt, halo_poll)) break;
     go = solve_beta<R>(d, tt, s);
   }
   // The fields the other kernels and the host read: p, r, z and q of this wave's water cells
@@ -1861,15 +1918,18 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
   const uint32_t capc = d.capc;
   const float* const p0 = start_pressure(d);
   unsigned long long* const hz = d.hz;
-  __shared__ double lpart[SOLVE_WAVES * SOLVE_RMAX][4];
+  __shared__ LPart<double> lp;
   const uint32_t t0 = tag + 1u;  // the start's reduction
   unsigned long long* const hz0 = hz + (size_t)(t0 & 1u) * capc;
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-    init_cell<double>(d, b, lane, lx, ly, lz, lk, hz0, 8u * t0 + 5u, lpart[b - b0], p0);
+    double v[4];
+    init_cell<double>(d, b, lane, lx, ly, lz, lk, hz0, 8u * t0 + 5u, v, p0);
+    if (lane == 0)
+      for (uint32_t k = 0; k < 4u; ++k) lp.v[k][b - b0] = v[k];
   });
   double tt[4];
   SolveScalars<double> s;
-  bool go = solve_start<double>(d, tt, reduce<double>(d, sp, ++tag, 4, lpart, tt), s);
+  bool go = solve_start<double>(d, tt, reduce<double>(d, sp, ++tag, 4, &lp, tt), s);
   const bool refresh_on = d.sc->refresh_every > 0u;
   const double replace_above = PIPE_REPLACE * (double)d.U->solve[0] * (s.b_norm < 1.0 ? 1.0 : s.b_norm);
 
@@ -1956,9 +2016,9 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
       const double v0 = wave_tree<double>(prod<double>(rR[r], rU[r])), v1 = wave_tree<double>(prod<double>(rW[r], rU[r])),
                    v2 = wave_tree<double>(prod<double>(rR[r], rR[r]));
       if (ln == 0) {
-        lpart[b - b0][0] = v0;
-        lpart[b - b0][1] = v1;
-        lpart[b - b0][2] = v2;
+        lp.v[0][b - b0] = v0;
+        lp.v[1][b - b0] = v1;
+        lp.v[2][b - b0] = v2;
       }
     }
     publish(rMv, hzT, 8u * T);
@@ -1977,7 +2037,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
       }
       return fresh;
     };
-    if (!reduce<double>(d, sp, ++tag, 3, lpart, tt, halo_poll) || !ok) break;
+    if (!reduce<double>(d, sp, ++tag, 3, &lp, tt, halo_poll) || !ok) break;
     if (!pipe_scalars(d, tt, s)) break;
     const float fa = (float)s.alpha, fb = (float)s.beta;
     const bool refresh = refresh_on && (s.iters + 1u) % d.sc->refresh_every == 0u && s.r_norm > replace_above;
