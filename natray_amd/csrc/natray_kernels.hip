@@ -985,76 +985,111 @@ struct LPart {
   R v[4][SOLVE_WAVES * SOLVE_RMAX];
 };
 
-// Components k0 and k1 of chunk [lo, hi)'s block partials (`lp` in the register form, Dev::part
-// in the memory form), each summed left to right as total()'s 256-stride sums add blocks. Eight
-// blocks' partials are loaded (from a clamped row, so without branches) before any is added,
-// and the two components' sums run side by side.
+// The chunk fold's tree width: the next power of two >= K (at most MAX_CHUNKS).
+__host__ __device__ __forceinline__ uint32_t chunk_width(uint32_t K) {
+  uint32_t w = 1u;
+  while (w < K) w <<= 1;
+  return w;
+}
+
+// v[l] += v[l + off] for off = off0, off0 / 2, .., 1 (off0 <= 32, a power of two or 0) across
+// the wave's lanes; lane 0 ends with the tree's sum of lanes [0, 2 off0).
 template <class R>
-__device__ __forceinline__ void chunk_fold(const Dev& d, const SolveSpan& sp, const LPart<R>* lp, uint32_t lo, uint32_t hi,
-                                           int k0, int k1, R& a0, R& a1) {
-  a0 = (R)0;
-  a1 = (R)0;
-  for (uint32_t b = lo; b < hi; b += 8u) {
-    R x0[8], x1[8];
-#pragma unroll
-    for (uint32_t i = 0; i < 8u; ++i) {
-      const uint32_t row = min(b + i, hi - 1u);
-      if (sp.in_regs) {
-        x0[i] = lp->v[k0][row - sp.b0];
-        x1[i] = lp->v[k1][row - sp.b0];
-      } else {
-        x0[i] = (R)d.part[row * 4u + (uint32_t)k0];
-        x1[i] = (R)d.part[row * 4u + (uint32_t)k1];
-      }
-    }
-#pragma unroll
-    for (uint32_t i = 0; i < 8u; ++i) {
-      if (b + i < hi) {
-        a0 += x0[i];
-        a1 += x1[i];
+__device__ __forceinline__ R lane_tree_from(R v, uint32_t off0) {
+  if (off0 >= 32u) v += lane_xor32(v);
+  if (off0 >= 16u) v += lane_xor16(v);
+  if (off0 >= 8u) v += lane_up<8>(v);
+  if (off0 >= 4u) v += lane_up<4>(v);
+  if (off0 >= 2u) v += lane_up<2>(v);
+  if (off0 >= 1u) v += lane_up<1>(v);
+  return v;
+}
+
+// Component k of chunk [lo, hi)'s tree sum (the chunk fold: a binary tree over W slots, slot i
+// = block lo + i's partial, zero past the chunk's last block), in lane 0 of the calling wave (all
+// lanes active). The partials come from `lp` (register form) or Dev::part (memory form). Lane l
+// holds slots l, l + 64, .. and combines them in the tree's order first.
+template <class R>
+__device__ __forceinline__ R chunk_tree(const Dev& d, const SolveSpan& sp, const LPart<R>* lp, uint32_t lo, uint32_t hi,
+                                        uint32_t W, int k, uint32_t ln) {
+  auto slot = [&](uint32_t i) -> R {
+    const uint32_t b = min(lo + i, hi - 1u);
+    const R x = sp.in_regs ? lp->v[k][b - sp.b0] : (R)d.part[b * 4u + (uint32_t)k];
+    return lo + i < hi ? x : (R)0;
+  };
+  if (W == 256u) return lane_tree_from<R>((slot(ln) + slot(ln + 128u)) + (slot(ln + 64u) + slot(ln + 192u)), 32u);
+  if (W == 128u) return lane_tree_from<R>(slot(ln) + slot(ln + 64u), 32u);
+  return lane_tree_from<R>(slot(ln), W / 2u);
+}
+
+// Lane 0's value in every lane (all lanes active).
+__device__ __forceinline__ float first_lane(float x) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_readfirstlane((int)__float_as_uint(x)));
+}
+__device__ __forceinline__ double first_lane(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// A reduction's LDS: the chunk sums, and the abort flag per tag parity (a wave can leave once it
+// has read it; it is rewritten two reductions later, after barriers every wave has passed).
+template <class R>
+struct RedLds {
+  R red[4][256];
+  uint32_t fail[2];
+};
+
+// Reduction `tag` over components [0, nc) of the block partials (nc = 0: a barrier only), in two
+// halves. reduce_publish: once every wave has its block partials in `lp` (blocks b0.., register
+// form) or Dev::part, the chunk folds are published. reduce_wait: every workgroup waits for
+// every chunk and returns the same totals (false when the solve aborted); `wait` runs in every
+// wave alongside the polling (see NoWait). Work that does not depend on the totals can run
+// between the two halves, while the other workgroups' chunks arrive.
+template <class R>
+__device__ void reduce_publish(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const LPart<R>* lp, RedLds<R>& L) {
+  constexpr int H = Halves<R>::N;
+  const uint32_t t = threadIdx.x;
+  const int nw = nc > 0 ? nc : 1;
+  const uint32_t buf = tag & 1u;
+  if (t == 0) L.fail[buf] = 0u;
+  wg_sync(!sp.in_regs);
+  // The chunk fold, then publish: per chunk and pair of words, the components of both words
+  // summed over the chunk's blocks, and both words (tag above payload each) stored as one
+  // 16-byte record.
+  const __amdgpu_buffer_rsrc_t rs = red_rsrc(d);
+  const uint32_t words = (uint32_t)(nw * H), P = (words + 1u) / 2u;
+  {
+    // A wave per chunk and pair of words, its lanes over the chunk's blocks.
+    const uint32_t wv = t >> 6, ln = t & 63u, tasks = (sp.c1 - sp.c0) * P, W = chunk_width(sp.K);
+    for (uint32_t task = wv; task < tasks; task += SOLVE_WAVES) {
+      const uint32_t j = task % P, c = sp.c0 + task / P, lo = c * sp.K, hi = min(sp.n, lo + sp.K);
+      // The components of words 2j and 2j + 1 (the same one for f64, whose halves they are).
+      const int k0 = (int)(2u * j) / H, k1 = (int)(2u * j + 1u) / H;
+      R a0 = chunk_tree<R>(d, sp, lp, lo, hi, W, k0, ln);
+      R a1 = k1 == k0 ? a0 : chunk_tree<R>(d, sp, lp, lo, hi, W, k1, ln);
+      if (k0 >= nc) a0 = (R)0;
+      if (k1 >= nc) a1 = (R)0;
+      if (ln == 0) {
+        if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const u32x4 rec = {Halves<R>::part(a0, (int)(2u * j) % H), tag, Halves<R>::part(a1, (int)(2u * j + 1u) % H), tag};
+        __builtin_amdgcn_raw_buffer_store_b128(rec, rs, red_pair(c, buf, j), 0, kRedPolicy);
       }
     }
   }
 }
 
-// Reduction `tag` over components [0, nc) of the block partials (nc = 0: a barrier only).
-// This workgroup's block partials are in `lp` (blocks b0.., register form) or Dev::part.
-// Every workgroup returns the same totals; false when the solve aborted. `wait` runs in every
-// wave alongside the polling (see NoWait).
 template <class R, class Wait = NoWait>
-__device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const LPart<R>* lp, R* out,
-                       Wait wait = Wait()) {
+__device__ bool reduce_wait(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, R* out, RedLds<R>& L,
+                            Wait wait = Wait()) {
   constexpr int H = Halves<R>::N;
-  // The chunk sums, then the totals and the abort flag, the last two per tag parity: a wave can
-  // leave once it has read them, and they are rewritten two reductions later, after barriers
-  // every wave has passed.
-  __shared__ R red[4][256];
-  __shared__ R red_out[2][4];
-  __shared__ uint32_t s_fail[2];
   const uint32_t t = threadIdx.x;
   const int nw = nc > 0 ? nc : 1;
   const uint32_t buf = tag & 1u;
   uint32_t* abort = d.bar + 2;
-  if (t == 0) s_fail[buf] = 0u;
-  wg_sync(!sp.in_regs);
-  // total()'s chunk fold, then publish: one thread per chunk and pair of words folds the
-  // components of its two words over the chunk's blocks, left to right, and stores both words
-  // (tag above payload each) as one 16-byte record.
   const __amdgpu_buffer_rsrc_t rs = red_rsrc(d);
-  const uint32_t words = (uint32_t)(nw * H), P = (words + 1u) / 2u;
-  if (t < (sp.c1 - sp.c0) * P) {
-    // A thread per chunk and pair of words.
-    const uint32_t j = t % P, c = sp.c0 + t / P, lo = c * sp.K, hi = min(sp.n, lo + sp.K);
-    if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    // The components of words 2j and 2j + 1 (the same one for f64, whose halves they are).
-    const int k0 = (int)(2u * j) / H, k1 = (int)(2u * j + 1u) / H;
-    R a0, a1;
-    chunk_fold<R>(d, sp, lp, lo, hi, k0, k1, a0, a1);
-    if (k0 >= nc) a0 = (R)0;
-    if (k1 >= nc) a1 = (R)0;
-    const u32x4 rec = {Halves<R>::part(a0, (int)(2u * j) % H), tag, Halves<R>::part(a1, (int)(2u * j + 1u) % H), tag};
-    __builtin_amdgcn_raw_buffer_store_b128(rec, rs, red_pair(c, buf, j), 0, kRedPolicy);
-  }
+  const uint32_t P = ((uint32_t)(nw * H) + 1u) / 2u;
   // Every chunk's words, summed as total() does (fluid.wgsl:426-439): thread t holds chunk t,
   // then the tree over the 256 threads.
   R v[4] = {(R)0, (R)0, (R)0, (R)0};
@@ -1092,7 +1127,7 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
       if ((++spins & 255u) == 0u &&
           (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
         __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_fail[buf] = 1u;
+        L.fail[buf] = 1u;
         break;
       }
     }
@@ -1106,28 +1141,33 @@ __device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, 
     if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (k < nc) red[k][t] = v[k];
+      if (k < nc) L.red[k][t] = v[k];
   }
   wg_sync(!sp.in_regs);
   if (nc > 0) {
-    // total()'s tree in one wave: lane l sums (c_l + c_l+128) + (c_l+64 + c_l+192), the tree's
-    // first two levels, then the 64 lanes.
-    if (t < 64u) {
+    // total()'s tree, in every wave alike: lane l sums (c_l + c_l+128) + (c_l+64 + c_l+192), the
+    // tree's first two levels, then the 64 lanes, and lane 0's sum goes to the whole wave. `red`
+    // is next written after the next reduction's first barrier, which each wave reaches only
+    // after these reads.
+    const uint32_t ln = t & 63u;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (k >= nc) continue;
-        const R x = lane_tree<R>((red[k][t] + red[k][t + 128u]) + (red[k][t + 64u] + red[k][t + 192u]));
-        if (t == 0) red_out[buf][k] = x;
-      }
+    for (int k = 0; k < 4; ++k) {
+      if (k >= nc) continue;
+      const R x = lane_tree<R>((L.red[k][ln] + L.red[k][ln + 128u]) + (L.red[k][ln + 64u] + L.red[k][ln + 192u]));
+      out[k] = first_lane(x);
     }
-    wg_sync(!sp.in_regs);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (k < nc) out[k] = red_out[buf][k];
   }
-  const bool ok = s_fail[buf] == 0u;
+  const bool ok = L.fail[buf] == 0u;
   if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   return ok;
+}
+
+template <class R, class Wait = NoWait>
+__device__ bool reduce(const Dev& d, const SolveSpan& sp, uint32_t tag, int nc, const LPart<R>* lp, R* out,
+                       Wait wait = Wait()) {
+  __shared__ RedLds<R> L;
+  reduce_publish<R>(d, sp, tag, nc, lp, L);
+  return reduce_wait<R, Wait>(d, sp, tag, nc, out, L, wait);
 }
 
 // The solve's row of one cell (solve_init, fluid.wgsl:450-489): whether it is water, the
@@ -2002,17 +2042,15 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
       if (rM[r] & 64u) rW[r] = w;
     }
   }
+  __shared__ RedLds<double> rl;
   while (go) {
     const uint32_t T = tag + 1u;  // this iteration's reduction
     unsigned long long* const hzT = hz + (size_t)(T & 1u) * capc;
-    // m = M^-1 w, published for the neighbours, and the block partials.
+    // The block partials of (r, u), (w, u) and (r, r), published at once.
 #pragma unroll
     for (uint32_t r = 0; r < RM; ++r) {
       const uint32_t b = b0 + wv + SOLVE_WAVES * r;
       if (b >= b1) continue;
-      const bool water = (rM[r] & 64u) != 0u;
-      const float m = precondition(rW[r], rDg[r], water, lx, ly, lz, mode);
-      rMv[r] = water ? m : 0.0f;
       const double v0 = wave_tree<double>(prod<double>(rR[r], rU[r])), v1 = wave_tree<double>(prod<double>(rW[r], rU[r])),
                    v2 = wave_tree<double>(prod<double>(rR[r], rR[r]));
       if (ln == 0) {
@@ -2020,6 +2058,16 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
         lp.v[1][b - b0] = v1;
         lp.v[2][b - b0] = v2;
       }
+    }
+    reduce_publish<double>(d, sp, ++tag, 3, &lp, rl);
+    // While the other workgroups' chunks arrive: m = M^-1 w, published for the neighbours.
+#pragma unroll
+    for (uint32_t r = 0; r < RM; ++r) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * r;
+      if (b >= b1) continue;
+      const bool water = (rM[r] & 64u) != 0u;
+      const float m = precondition(rW[r], rDg[r], water, lx, ly, lz, mode);
+      rMv[r] = water ? m : 0.0f;
     }
     publish(rMv, hzT, 8u * T);
     // The halo m words, polled while the reduction completes.
@@ -2037,7 +2085,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
       }
       return fresh;
     };
-    if (!reduce<double>(d, sp, ++tag, 3, &lp, tt, halo_poll) || !ok) break;
+    if (!reduce_wait<double>(d, sp, tag, 3, tt, rl, halo_poll) || !ok) break;
     if (!pipe_scalars(d, tt, s)) break;
     const float fa = (float)s.alpha, fb = (float)s.beta;
     const bool refresh = refresh_on && (s.iters + 1u) % d.sc->refresh_every == 0u && s.r_norm > replace_above;

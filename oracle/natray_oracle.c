@@ -13,9 +13,10 @@
  *   - ranks inside a block: particle index order (the reference's atomicAdd order is
  *     nondeterministic; nothing downstream depends on it).
  *   - solver reductions over blocks: per-block tree as block_sum (fluid.wgsl:414-422), then
- *     blocks are folded left to right in fixed chunks of K = ceil(n/256) (at most 256
- *     chunks), then the chunk sums go through the reference's total() (fluid.wgsl:426-439).
- *     For n <= 256 this is exactly the reference's total() over blocks in slot order.
+ *     fixed chunks of K = ceil(n/256) consecutive blocks (at most 256 chunks), each summed as a
+ *     binary tree over the next power of two >= K slots, then the chunk sums go through the
+ *     reference's total() (fluid.wgsl:426-439). For n <= 256 this is exactly the reference's
+ *     total() over blocks in slot order.
  *   - the membrane coupling, the air boundary, the bullets and the spray are restated below
  *     and are off unless set (set_membrane, NATRAY_SCENE_AIR, set_bullets, NATRAY_SCENE_SPRAY):
  *     off, they behave as the reference without them (GAS pressure 0, their acc fields zero,
@@ -802,19 +803,27 @@ static double tree64(double v[64], int dbl) {
   return (double)r[0];
 }
 
-/* Sum of component k of the n block partials: fold in chunks of K = ceil(n/CHUNKS_MAX),
- * then the reference's total() (fluid.wgsl:426-439) over the chunk sums. */
+/* Sum of component k of the n block partials: chunks of K = ceil(n/CHUNKS_MAX) consecutive
+ * blocks, each summed as a binary tree over W = the next power of two >= K slots (slot i = block
+ * lo + i, zero past the chunk's last block; v[i] += v[i + off] for off = W/2 .. 1), then the
+ * reference's total() (fluid.wgsl:426-439) over the chunk sums. For n <= CHUNKS_MAX (K = 1)
+ * this is the reference's total() over the blocks in slot order. */
 #define CHUNKS_MAX 256u
+static uint32_t chunk_width(uint32_t K) {
+  uint32_t w = 1u;
+  while (w < K) w <<= 1;
+  return w;
+}
 static double total(const osim* s, uint32_t n, int k, int dbl) {
   if (n == 0) return 0.0;
-  uint32_t K = (n + CHUNKS_MAX - 1u) / CHUNKS_MAX, m = (n + K - 1u) / K;
-  double chunk[CHUNKS_MAX];
-  double red[256];
+  uint32_t K = (n + CHUNKS_MAX - 1u) / CHUNKS_MAX, m = (n + K - 1u) / K, W = chunk_width(K);
   if (dbl) {
+    double chunk[CHUNKS_MAX], red[256], v[CHUNKS_MAX];
     for (uint32_t j = 0; j < m; ++j) {
-      double acc = 0;
-      for (uint32_t b = j * K; b < umin(n, j * K + K); ++b) acc += s->part[b * 4 + k];
-      chunk[j] = acc;
+      for (uint32_t i = 0; i < W; ++i) v[i] = j * K + i < umin(n, j * K + K) ? s->part[(j * K + i) * 4 + k] : 0.0;
+      for (uint32_t off = W / 2u; off > 0; off >>= 1)
+        for (uint32_t i = 0; i < off; ++i) v[i] += v[i + off];
+      chunk[j] = v[0];
     }
     for (uint32_t li = 0; li < 256; ++li) {
       double acc = 0;
@@ -825,11 +834,12 @@ static double total(const osim* s, uint32_t n, int k, int dbl) {
       for (uint32_t li = 0; li < off; ++li) red[li] += red[li + off];
     return red[0];
   }
-  REAL rc[CHUNKS_MAX], rr[256];
+  REAL rc[CHUNKS_MAX], rr[256], v[CHUNKS_MAX];
   for (uint32_t j = 0; j < m; ++j) {
-    REAL acc = 0;
-    for (uint32_t b = j * K; b < umin(n, j * K + K); ++b) acc += (REAL)s->part[b * 4 + k];
-    rc[j] = acc;
+    for (uint32_t i = 0; i < W; ++i) v[i] = j * K + i < umin(n, j * K + K) ? (REAL)s->part[(j * K + i) * 4 + k] : (REAL)0;
+    for (uint32_t off = W / 2u; off > 0; off >>= 1)
+      for (uint32_t i = 0; i < off; ++i) v[i] += v[i + off];
+    rc[j] = v[0];
   }
   for (uint32_t li = 0; li < 256; ++li) {
     REAL acc = 0;
@@ -838,8 +848,6 @@ static double total(const osim* s, uint32_t n, int k, int dbl) {
   }
   for (uint32_t off = 128; off > 0; off >>= 1)
     for (uint32_t li = 0; li < off; ++li) rr[li] += rr[li + off];
-  (void)chunk;
-  (void)red;
   return (double)rr[0];
 }
 
