@@ -54,8 +54,10 @@ struct Dev {
                            // solve starts from the warm start's guess, [12] particles that
                            // broke up this step (reset by k_begin)
   unsigned long long* rwords;  // RED_WORDS: tagged chunk sums of the solve's reductions
-  unsigned long long* hz;      // 2 x capc: tagged halo words of each water cell (register forms of
-                               // the solve; the pipelined form alternates between the two halves)
+  unsigned long long* hz;      // 3 x capc: tagged halo words of each water cell (the register
+                               // forms and the memory form's streaming passes: z in the first
+                               // third, the streaming passes' directions alternating between the
+                               // other two; the pipelined form alternates between the first two)
   float* pcg;                  // 4 x capc: z, q and two m buffers of the pipelined solve's memory
                                // form (NATRAY_CG_PIPELINED only; else null)
   float* vmax_part;        // fastest particle of each g2p workgroup (<= cap workgroups)

@@ -853,6 +853,10 @@ constexpr uint32_t SOLVE_THREADS = 1024;
 constexpr uint32_t SOLVE_WAVES = SOLVE_THREADS / 64;
 // Blocks a wave of the solve keeps in registers.
 constexpr uint32_t SOLVE_RMAX = 2;
+// The memory form's streaming passes: at most this many blocks per wave, the loads of this
+// many blocks issued together.
+constexpr uint32_t SOLVE_JMAX = 32;
+constexpr uint32_t SOLVE_GROUP = 2;
 // Words of one reduction buffer per chunk: 4 components x 2 halves. Word w is component w / H,
 // half w % H (H = 1 for f32, 2 for f64); words 2j and 2j + 1 travel together as one 16-byte
 // access, so the words of a chunk arrive in as few loads as they take.
@@ -969,6 +973,9 @@ struct SolveSpan {
   uint32_t c0, c1;  // chunks of this workgroup
   uint32_t b0, b1;  // their blocks
   bool in_regs;     // every workgroup's blocks fit in registers (the same for all)
+  bool stream;      // the memory form's streaming passes (the same for all)
+  bool coherent;    // a reduction also publishes the plain stores before it to the other
+                    // workgroups (agent-scope release and acquire)
 };
 
 // Work for the waves of a workgroup while a reduction completes: `wait(tag)` is called once
@@ -1072,7 +1079,7 @@ __device__ void reduce_publish(const Dev& d, const SolveSpan& sp, uint32_t tag, 
       if (k0 >= nc) a0 = (R)0;
       if (k1 >= nc) a1 = (R)0;
       if (ln == 0) {
-        if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (sp.coherent) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const u32x4 rec = {Halves<R>::part(a0, (int)(2u * j) % H), tag, Halves<R>::part(a1, (int)(2u * j + 1u) % H), tag};
         __builtin_amdgcn_raw_buffer_store_b128(rec, rs, red_pair(c, buf, j), 0, kRedPolicy);
       }
@@ -1138,7 +1145,7 @@ __device__ bool reduce_wait(const Dev& d, const SolveSpan& sp, uint32_t tag, int
     }
   }
   if (t < 256u) {
-    if (!sp.in_regs) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (sp.coherent) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       if (k < nc) L.red[k][t] = v[k];
@@ -1325,8 +1332,30 @@ __device__ __forceinline__ void solve_finish(const Dev& d, const SolveSpan& sp, 
   d.stats[6] += st->n_blocks_prev;
 }
 
+// Cell i of a field through a 32-bit byte offset over the field's base (a field holds fewer
+// than 2^30 cells): one address register per cell instead of a 64-bit pointer.
+__device__ __forceinline__ float ld_cell(const float* f, uint32_t i) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (i << 2));
+}
+__device__ __forceinline__ void st_cell(float* f, uint32_t i, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(f) + (i << 2)) = v;
+}
+
 // The memory form of the solve: the same iteration on the cf fields in memory, for launches
 // whose workgroups cannot keep their blocks in registers.
+//
+// Streaming passes (sp.stream: every workgroup holds at most SOLVE_WAVES x SOLVE_JMAX blocks):
+// wave wv walks its own blocks b0 + wv + SOLVE_WAVES j with no workgroup barrier, the same
+// blocks in every pass, so a cell's own fields are only ever read back by the wave that wrote
+// them. Values a cell's neighbours in other workgroups need travel as tagged words (tag above
+// the float), as in the register form: z in Dev::hz[cell] with the tag of the reduction after
+// the pass that made it, the new direction in hz[capc (1 + k % 2) + cell] with the tag of
+// iteration k's first reduction, and before a residual refresh p in the other direction buffer
+// (whose last words every neighbour has read) with the tag of the refresh's reduction. A reader
+// waits for the tag it expects, so the reductions need no agent-scope fences. Each wave keeps its blocks' neighbour masks and their six
+// face-neighbour blocks (-x, +x, -y, +y, -z, +z) in LDS, issues the loads of SOLVE_GROUP blocks before it uses any, and takes a cell's neighbours
+// inside the block from the wave's lanes. The arithmetic is that of the per-block passes,
+// expression for expression.
 template <bool MIXED>
 __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t tag) {
   using R = typename std::conditional<MIXED, double, float>::type;
@@ -1334,15 +1363,17 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
   const uint32_t mode = d.sc->precond;
   const uint32_t b0 = sp.b0, b1 = sp.b1;
   const uint32_t capc = d.capc;
+  const bool stream = sp.stream;
   float* const cP = d.cf + (size_t)NATRAY_F_P * capc;
   float* const cR = d.cf + (size_t)NATRAY_F_R * capc;
   float* const cZ = d.cf + (size_t)NATRAY_F_Z * capc;
   float* const cQ = d.cf + (size_t)NATRAY_F_Q * capc;
   float* const cDiag = d.cf + (size_t)NATRAY_F_DIAG * capc;
+  unsigned long long* const zw = d.hz;
   const float* const p0 = start_pressure(d);
   for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
     R v[4];
-    init_cell<R>(d, b, lane, lx, ly, lz, lk, nullptr, 0u, v, p0);
+    init_cell<R>(d, b, lane, lx, ly, lz, lk, stream ? zw : nullptr, tag + 1u, v, p0);
     if (lane == 0)
       for (uint32_t k = 0; k < 4u; ++k) d.part[b * 4u + k] = (double)v[k];
   });
@@ -1356,6 +1387,61 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
     __syncthreads();
   }
   const bool refresh_on = MIXED && d.sc->precision == NATRAY_CG_MIXED && d.sc->refresh_every > 0u;
+
+  const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
+  const int lx = (int)(ln >> 4), ly = (int)((ln >> 2) & 3u), lz = (int)(ln & 3u);
+  const uint32_t face_bit[3] = {lx == 0 ? 1u : (lx == 3 ? 2u : 0u), ly == 0 ? 4u : (ly == 3 ? 8u : 0u),
+                                lz == 0 ? 16u : (lz == 3 ? 32u : 0u)};
+  // The face neighbour this lane needs on each axis (its index among the six), and the cell of
+  // that block the face touches (the coordinate on the axis 0 <-> 3).
+  const uint32_t nb_dir[3] = {lx == 3 ? 1u : 0u, ly == 3 ? 3u : 2u, lz == 3 ? 5u : 4u};
+  const uint32_t nb_cell[3] = {ln ^ 48u, ln ^ 12u, ln ^ 3u};
+  __shared__ uint32_t nbl[SOLVE_WAVES][6][SOLVE_JMAX];
+  __shared__ uint8_t mk[SOLVE_WAVES][SOLVE_JMAX][64];
+  // Each pass reads its wave's first block from LDS, so that the blocks' addresses are formed
+  // inside the iteration rather than held in registers across it.
+  __shared__ uint32_t wave_b0[SOLVE_WAVES];
+  if (stream) {
+    for (uint32_t j = 0; j < SOLVE_JMAX; ++j) {
+      const uint32_t b = b0 + wv + SOLVE_WAVES * j;
+      if (b >= b1) break;
+      mk[wv][j][ln] = (uint8_t)(d.nbm[b * 64u + ln] & 127u);
+      if (ln < 6u) {
+        // block_nb's index of face neighbour ln: 13 -+ 9, 3 or 1.
+        const uint32_t a = ln >> 1, step = a == 0 ? 9u : (a == 1 ? 3u : 1u);
+        nbl[wv][ln][j] = d.block_nb[b * 27u + ((ln & 1u) ? 13u + step : 13u - step)];
+      }
+    }
+    if (ln == 0u) wave_b0[wv] = b0 + wv;
+    // Written and read by the same wave only.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  }
+  // The neighbour mask of this lane's cell in the wave's block j; w as in halo_of.
+  auto mask_of = [&](uint32_t w, uint32_t j) -> uint32_t { return mk[w][j][ln]; };
+  // The cell across face `a` of this lane in the wave's block j (used only where the mask
+  // has the face's bit, so the neighbour block exists); w is the wave's index as a pass read
+  // it from LDS.
+  auto halo_of = [&](uint32_t w, uint32_t j, int a) -> uint32_t { return nbl[w][nb_dir[a]][j] * 64u + nb_cell[a]; };
+  // Waits until word `w` (reloaded from p) carries `want`; false once the solve aborts.
+  auto wait_word = [&](unsigned long long& w, const unsigned long long* p, uint32_t want) -> bool {
+    uint32_t spins = 0;
+    while ((uint32_t)(w >> 32) != want) {
+      __builtin_amdgcn_s_sleep(1);
+      w = load_word(p);
+      if ((++spins & 255u) == 0u &&
+          (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kSpinLimit)) {
+        __hip_atomic_store(d.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+    }
+    return true;
+  };
+  // The tag of the last iteration's first reduction: its directions' words carry it.
+  uint32_t tag_d = 0u;
+  bool ok = true;
+
   while (go) {
     // solve_q (fluid.wgsl:508-534): the new direction and q = A d; neighbours' directions are
     // rebuilt from z and the old direction.
@@ -1363,53 +1449,244 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
     const float* cDo = d.cf + (size_t)(NATRAY_F_D + old) * capc;
     float* cDn = d.cf + (size_t)(NATRAY_F_D + 1u - old) * capc;
     const float fb = (float)s.beta;
-    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
-      const uint32_t cell = b * 64u + lane;
-      const uint32_t mask = d.nbm[cell];
-      R v = (R)0;
-      if (mask & 64u) {
-        const float dn = cZ[cell] + fb * cDo[cell];
-        float q = cDiag[cell] * dn;
-        if (mask & 1u) { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 2u) { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 4u) { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 8u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 16u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; }
-        if (mask & 32u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; }
-        cDn[cell] = dn;
-        cQ[cell] = q;
-        v = prod<R>(dn, q);
+    if (stream) {
+      const uint32_t wb = wave_b0[wv], wl = wb - b0;
+      // Directions of the neighbours: the last iteration's words, or before the first
+      // iteration the direction field as the solve found it.
+      const bool first = s.iters == 0u;
+      const unsigned long long* const dw_old = zw + (size_t)capc * (1u + (s.iters + 1u) % 2u);
+      unsigned long long* const dw_new = zw + (size_t)capc * (1u + s.iters % 2u);
+      // This wave's blocks; a rolled loop, so that nothing per block lives across it.
+      const uint32_t nbw = b1 > wb ? (b1 - wb + SOLVE_WAVES - 1u) / SOLVE_WAVES : 0u;
+#pragma unroll 1
+      for (uint32_t j0 = 0; j0 < nbw; j0 += SOLVE_GROUP) {
+        float z[SOLVE_GROUP], dold[SOLVE_GROUP], dg[SOLVE_GROUP];
+        unsigned long long hzw[SOLVE_GROUP][3], hdw[SOLVE_GROUP][3];
+#pragma unroll
+        for (uint32_t u = 0; u < SOLVE_GROUP; ++u) {
+          const uint32_t j = j0 + u, b = wb + SOLVE_WAVES * j;
+          z[u] = dold[u] = 0.0f;
+          dg[u] = 1.0f;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) hzw[u][a] = hdw[u][a] = 0ull;
+          const uint32_t mask = mask_of(wl, j);
+          if (b >= b1 || !(mask & 64u)) continue;
+          const uint32_t cell = b * 64u + ln;
+          z[u] = ld_cell(cZ, cell);
+          dold[u] = ld_cell(cDo, cell);
+          dg[u] = ld_cell(cDiag, cell);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            if (!(mask & face_bit[a])) continue;
+            const uint32_t o = halo_of(wl, j, a);
+            hzw[u][a] = load_word(&zw[o]);
+            hdw[u][a] = first ? tag_word(tag_d, __float_as_uint(ld_cell(cDo, o))) : load_word(&dw_old[o]);
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < SOLVE_GROUP; ++u) {
+          const uint32_t j = j0 + u, b = wb + SOLVE_WAVES * j;
+          if (b >= b1) continue;
+          const uint32_t cell = b * 64u + ln, mask = mask_of(wl, j);
+          float hd[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            if (!(mask & 64u) || !(mask & face_bit[a])) continue;
+            const uint32_t o = halo_of(wl, j, a);
+            ok = ok && wait_word(hzw[u][a], &zw[o], tag) && wait_word(hdw[u][a], &dw_old[o], tag_d);
+            hd[a] = __uint_as_float((uint32_t)hzw[u][a]) + fb * __uint_as_float((uint32_t)hdw[u][a]);
+          }
+          const float dn = z[u] + fb * dold[u];
+          const Faces f = block_faces(dn);
+          R v = (R)0;
+          if (mask & 64u) {
+            float q = dg[u] * dn;
+            if (mask & 1u) q -= lx > 0 ? f.xm : hd[0];
+            if (mask & 2u) q -= lx < 3 ? f.xp : hd[0];
+            if (mask & 4u) q -= ly > 0 ? f.ym : hd[1];
+            if (mask & 8u) q -= ly < 3 ? f.yp : hd[1];
+            if (mask & 16u) q -= lz > 0 ? f.zm : hd[2];
+            if (mask & 32u) q -= lz < 3 ? f.zp : hd[2];
+            st_cell(cDn, cell, dn);
+            st_cell(cQ, cell, q);
+            store_word(&dw_new[cell], tag_word(tag + 1u, __float_as_uint(dn)));
+            v = prod<R>(dn, q);
+          }
+          v = wave_tree<R>(v);
+          if (ln == 0) d.part[b * 4u] = (double)v;
+        }
       }
-      v = wave_tree<R>(v);
-      if (lane == 0) d.part[b * 4u] = (double)v;
-    });
-    if (!reduce<R>(d, sp, ++tag, 1, nullptr, tt)) break;
+    } else {
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
+        const uint32_t cell = b * 64u + lane;
+        const uint32_t mask = d.nbm[cell];
+        R v = (R)0;
+        if (mask & 64u) {
+          const float dn = cZ[cell] + fb * cDo[cell];
+          float q = cDiag[cell] * dn;
+          if (mask & 1u) { uint32_t o = nbr(lk, lx, ly, lz, -1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 2u) { uint32_t o = nbr(lk, lx, ly, lz, 1, 0, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 4u) { uint32_t o = nbr(lk, lx, ly, lz, 0, -1, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 8u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 1, 0); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 16u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, -1); q -= cZ[o] + fb * cDo[o]; }
+          if (mask & 32u) { uint32_t o = nbr(lk, lx, ly, lz, 0, 0, 1); q -= cZ[o] + fb * cDo[o]; }
+          cDn[cell] = dn;
+          cQ[cell] = q;
+          v = prod<R>(dn, q);
+        }
+        v = wave_tree<R>(v);
+        if (lane == 0) d.part[b * 4u] = (double)v;
+      });
+    }
+    if (!reduce<R>(d, sp, ++tag, 1, nullptr, tt) || !ok) break;
+    tag_d = tag;
     // solve_alpha (fluid.wgsl:536-541)
     s.alpha = tt[0] > (R)0 ? s.rz / tt[0] : (R)0;
     const float fa = (float)s.alpha;
     const bool refresh = refresh_on && (s.iters + 1u) % d.sc->refresh_every == 0u;
     // solve_update (fluid.wgsl:543-559): p += alpha d, r -= alpha q, z = M^-1 r.
-    for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t*) {
-      const uint32_t cell = b * 64u + lane;
-      const bool water = (d.nbm[cell] & 64u) != 0u;
-      float r = 0.0f, diag = 1.0f;
-      if (water) {
-        cP[cell] += fa * cDn[cell];
-        r = cR[cell] - fa * cQ[cell];
-        cR[cell] = r;
-        diag = cDiag[cell];
-      }
-      if (!refresh) {
-        const float z = precondition(r, diag, water, lx, ly, lz, mode);
-        if (water) cZ[cell] = z;
-        const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
-        if (lane == 0) {
-          d.part[b * 4u + 0] = (double)v0;
-          d.part[b * 4u + 1] = (double)v1;
+    unsigned long long* const pw = zw + (size_t)capc * (1u + (s.iters + 1u) % 2u);
+    if (stream) {
+      const uint32_t wb = wave_b0[wv], wl = wb - b0;
+      // This wave's blocks; a rolled loop, so that nothing per block lives across it.
+      const uint32_t nbw = b1 > wb ? (b1 - wb + SOLVE_WAVES - 1u) / SOLVE_WAVES : 0u;
+#pragma unroll 1
+      for (uint32_t j0 = 0; j0 < nbw; j0 += SOLVE_GROUP) {
+        float p[SOLVE_GROUP], dn[SOLVE_GROUP], rr[SOLVE_GROUP], q[SOLVE_GROUP], dg[SOLVE_GROUP];
+#pragma unroll
+        for (uint32_t u = 0; u < SOLVE_GROUP; ++u) {
+          const uint32_t j = j0 + u, b = wb + SOLVE_WAVES * j;
+          p[u] = dn[u] = rr[u] = q[u] = 0.0f;
+          dg[u] = 1.0f;
+          if (b >= b1 || !(mask_of(wl, j) & 64u)) continue;
+          const uint32_t cell = b * 64u + ln;
+          p[u] = ld_cell(cP, cell);
+          dn[u] = ld_cell(cDn, cell);
+          rr[u] = ld_cell(cR, cell);
+          q[u] = ld_cell(cQ, cell);
+          dg[u] = ld_cell(cDiag, cell);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < SOLVE_GROUP; ++u) {
+          const uint32_t j = j0 + u, b = wb + SOLVE_WAVES * j;
+          if (b >= b1) continue;
+          const uint32_t cell = b * 64u + ln;
+          const bool water = (mask_of(wl, j) & 64u) != 0u;
+          float r = 0.0f, diag = 1.0f;
+          if (water) {
+            const float pn = p[u] + fa * dn[u];
+            st_cell(cP, cell, pn);
+            // Before a refresh, p for the neighbours' A p, in the direction buffer the last
+            // iteration used (every neighbour has read it by now).
+            if (refresh) store_word(&pw[cell], tag_word(tag + 1u, __float_as_uint(pn)));
+            r = rr[u] - fa * q[u];
+            st_cell(cR, cell, r);
+            diag = dg[u];
+          }
+          if (!refresh) {
+            const float z = precondition(r, diag, water, lx, ly, lz, mode);
+            if (water) {
+              st_cell(cZ, cell, z);
+              store_word(&zw[cell], tag_word(tag + 1u, __float_as_uint(z)));
+            }
+            const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+            if (ln == 0) {
+              d.part[b * 4u + 0] = (double)v0;
+              d.part[b * 4u + 1] = (double)v1;
+            }
+          }
         }
       }
-    });
-    if (refresh) {
+    } else {
+      for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t*) {
+        const uint32_t cell = b * 64u + lane;
+        const bool water = (d.nbm[cell] & 64u) != 0u;
+        float r = 0.0f, diag = 1.0f;
+        if (water) {
+          cP[cell] += fa * cDn[cell];
+          r = cR[cell] - fa * cQ[cell];
+          cR[cell] = r;
+          diag = cDiag[cell];
+        }
+        if (!refresh) {
+          const float z = precondition(r, diag, water, lx, ly, lz, mode);
+          if (water) cZ[cell] = z;
+          const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+          if (lane == 0) {
+            d.part[b * 4u + 0] = (double)v0;
+            d.part[b * 4u + 1] = (double)v1;
+          }
+        }
+      });
+    }
+    if (refresh && stream) {
+      // True residual r = b - A p in double once every p is updated, then the preconditioner;
+      // the neighbours' p across block faces from their words.
+      if (!reduce<R>(d, sp, ++tag, 0, nullptr, tt)) break;
+      const uint32_t wb = wave_b0[wv], wl = wb - b0;
+      // This wave's blocks; a rolled loop, so that nothing per block lives across it.
+      const uint32_t nbw = b1 > wb ? (b1 - wb + SOLVE_WAVES - 1u) / SOLVE_WAVES : 0u;
+#pragma unroll 1
+      for (uint32_t j0 = 0; j0 < nbw; j0 += SOLVE_GROUP) {
+        float p[SOLVE_GROUP], dg[SOLVE_GROUP], bv[SOLVE_GROUP];
+        unsigned long long hpw[SOLVE_GROUP][3];
+#pragma unroll
+        for (uint32_t u = 0; u < SOLVE_GROUP; ++u) {
+          const uint32_t j = j0 + u, b = wb + SOLVE_WAVES * j;
+          p[u] = bv[u] = 0.0f;
+          dg[u] = 1.0f;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) hpw[u][a] = 0ull;
+          const uint32_t mask = mask_of(wl, j);
+          if (b >= b1 || !(mask & 64u)) continue;
+          const uint32_t cell = b * 64u + ln;
+          p[u] = ld_cell(cP, cell);
+          dg[u] = ld_cell(cDiag, cell);
+          bv[u] = ld_cell(d.bvec, cell);
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            if (mask & face_bit[a]) hpw[u][a] = load_word(&pw[halo_of(wl, j, a)]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < SOLVE_GROUP; ++u) {
+          const uint32_t j = j0 + u, b = wb + SOLVE_WAVES * j;
+          if (b >= b1) continue;
+          const uint32_t cell = b * 64u + ln, mask = mask_of(wl, j);
+          const bool water = (mask & 64u) != 0u;
+          float hp[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            if (!water || !(mask & face_bit[a])) continue;
+            ok = ok && wait_word(hpw[u][a], &pw[halo_of(wl, j, a)], tag);
+            hp[a] = __uint_as_float((uint32_t)hpw[u][a]);
+          }
+          const Faces f = block_faces(p[u]);
+          float r = 0.0f, diag = 1.0f;
+          if (water) {
+            diag = dg[u];
+            double ap = (double)diag * (double)p[u];
+            if (mask & 1u) ap -= (double)(lx > 0 ? f.xm : hp[0]);
+            if (mask & 2u) ap -= (double)(lx < 3 ? f.xp : hp[0]);
+            if (mask & 4u) ap -= (double)(ly > 0 ? f.ym : hp[1]);
+            if (mask & 8u) ap -= (double)(ly < 3 ? f.yp : hp[1]);
+            if (mask & 16u) ap -= (double)(lz > 0 ? f.zm : hp[2]);
+            if (mask & 32u) ap -= (double)(lz < 3 ? f.zp : hp[2]);
+            r = (float)((double)bv[u] - ap);
+            st_cell(cR, cell, r);
+          }
+          const float z = precondition(r, diag, water, lx, ly, lz, mode);
+          if (water) {
+            st_cell(cZ, cell, z);
+            store_word(&zw[cell], tag_word(tag + 1u, __float_as_uint(z)));
+          }
+          const R v0 = wave_tree<R>(prod<R>(r, z)), v1 = wave_tree<R>(prod<R>(r, r));
+          if (ln == 0) {
+            d.part[b * 4u + 0] = (double)v0;
+            d.part[b * 4u + 1] = (double)v1;
+          }
+        }
+      }
+    } else if (refresh) {
       // True residual r = b - A p in double once every p is updated, then the preconditioner.
       if (!reduce<R>(d, sp, ++tag, 0, nullptr, tt)) break;
       for_blocks<SOLVE_WAVES>(d, b0, b1, SOLVE_WAVES, [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
@@ -1446,7 +1723,7 @@ __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t 
 
 // This workgroup's chunks and blocks in its scene's solve, and whether every workgroup keeps
 // its blocks in registers.
-__device__ __forceinline__ SolveSpan solve_span(const View& v) {
+__device__ __forceinline__ SolveSpan solve_span(const View& v, bool streams) {
   SolveSpan sp;
   sp.n = n_blocks(v.d);
   sp.K = sp.n ? (sp.n + MAX_CHUNKS - 1u) / MAX_CHUNKS : 1u;
@@ -1457,7 +1734,11 @@ __device__ __forceinline__ SolveSpan solve_span(const View& v) {
   sp.c1 = (uint32_t)((uint64_t)(v.wg + 1) * sp.m / sp.G);
   sp.b0 = sp.c0 * sp.K;
   sp.b1 = min(sp.n, sp.c1 * sp.K);
-  sp.in_regs = (sp.m + sp.G - 1u) / sp.G * sp.K <= SOLVE_WAVES * SOLVE_RMAX;
+  // The most blocks a workgroup holds: its chunks are at most ceil(m / G).
+  const uint32_t most = (sp.m + sp.G - 1u) / sp.G * sp.K;
+  sp.in_regs = most <= SOLVE_WAVES * SOLVE_RMAX;
+  sp.stream = streams && !sp.in_regs && most <= SOLVE_WAVES * SOLVE_JMAX;
+  sp.coherent = !sp.in_regs && !sp.stream;
   return sp;
 }
 
@@ -1470,7 +1751,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
   if (d.st->dt_ns == 0u) return;
   // An earlier solve of this batch lost its barrier: the host reports it, nothing more runs.
   if (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-  const SolveSpan sp = solve_span(v);
+  const SolveSpan sp = solve_span(v, true);
   uint32_t tag = __hip_atomic_load(d.bar + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!sp.in_regs) {
     // Called with the device copy of Dev, which the callee reads with scalar loads.
@@ -1943,7 +2224,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
   constexpr uint32_t RM = SOLVE_RMAX;
   if (d.st->dt_ns == 0u) return;
   if (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-  const SolveSpan sp = solve_span(v);
+  const SolveSpan sp = solve_span(v, false);
   uint32_t tag = __hip_atomic_load(d.bar + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!sp.in_regs) {
     solve_pipe_memory(d.self, sp, tag);

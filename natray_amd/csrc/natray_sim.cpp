@@ -156,7 +156,7 @@ int solve_grid_for(const natray_sim* s) {
 // Zeroes every tagged word of the solve and its tag counter, so tags start again from 1.
 int clear_tags(natray_sim* s) {
   HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, s->stream));
-  HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 16, s->stream));
+  HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 24, s->stream));
   HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, s->stream));
   s->tags_used = 0;
   return NATRAY_OK;
@@ -454,7 +454,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   TRY(dalloc(s, &D.nbm, capc));
   TRY(dalloc(s, &D.part, (size_t)D.cap * 4));
   TRY(dalloc(s, &D.rwords, natray::RED_WORDS));
-  TRY(dalloc(s, &D.hz, 2 * capc));
+  TRY(dalloc(s, &D.hz, 3 * capc));
   if (s->sc.precision == NATRAY_CG_PIPELINED) TRY(dalloc(s, &D.pcg, 4 * capc));
   TRY(dalloc(s, &D.tile_sum, (size_t)D.ntiles * 2));
   TRY(dalloc(s, &D.tile_off, (size_t)D.ntiles * 2));
@@ -526,7 +526,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
   HIPTRY(hipMemsetAsync(D.part, 0, (size_t)D.cap * 4 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.bar, 0, 1024 * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.rwords, 0, natray::RED_WORDS * 8, s->stream));
-  HIPTRY(hipMemsetAsync(D.hz, 0, capc * 16, s->stream));
+  HIPTRY(hipMemsetAsync(D.hz, 0, capc * 24, s->stream));
   HIPTRY(hipMemsetAsync(D.vmax_part, 0, (size_t)D.cap * 4, s->stream));
   HIPTRY(hipMemsetAsync(D.solver, 0, 8 * 8, s->stream));
   HIPTRY(hipMemsetAsync(D.stats, 0, 8 * 8, s->stream));
@@ -1341,7 +1341,7 @@ int batch_phase_launch(natray_batch* b, int phase) {
         const uint64_t bound = solve_tag_bound(s);
         if (s->tags_used + bound >= kTagLimit) {
           HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, b->stream));
-          HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 16, b->stream));
+          HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 24, b->stream));
           HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, b->stream));
           s->tags_used = 0;
         }
