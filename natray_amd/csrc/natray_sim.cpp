@@ -1282,29 +1282,54 @@ int batch_layout(natray_batch* b) {
 
 int batch_phase_launch(natray_batch* b, int phase);
 
-// The solve's workgroups shared by the work each scene's last solve had, blocks times CG
-// iterations (the slowest scene sets the batch's solve time), once every scene has run a
-// step; the solve is launched outside the step graphs, so only its map changes.
+// The solve's workgroups from each scene's last solve, once every scene has run a step; the
+// solve is launched outside the step graphs, so only its map changes. A scene's solve lasts
+// about (iterations + 1) x (kSolveFixed + the most blocks one of its workgroups holds): every
+// iteration pays its two reductions whatever the workgroup holds (kSolveFixed, in blocks of
+// pass time: about 15 us against 0.3 us per block, measured), and a workgroup holds whole
+// chunks of K blocks. Starting from one workgroup each, the scene that would take longest gets
+// the workgroups that take one chunk off each of its workgroups, until they do not fit in the
+// co-resident workgroups.
+constexpr uint64_t kSolveFixed = 48;
 int batch_plan_solve(natray_batch* b) {
   const size_t n = b->sims.size();
-  std::vector<uint64_t> w(n);
+  std::vector<uint64_t> it(n);
+  std::vector<uint32_t> K(n), m(n), shares(n, 1u);
   for (size_t k = 0; k < n; ++k) {
     const natray_state& st = b->sims[k]->last;
     if (st.step == 0 || st.n_blocks == 0) return NATRAY_OK;
-    w[k] = (uint64_t)std::min(st.n_blocks, b->sims[k]->D.cap) * (uint64_t)(st.solve_iters + 1u);
+    const uint32_t nb = std::min(st.n_blocks, b->sims[k]->D.cap);
+    K[k] = (nb + natray::MAX_CHUNKS - 1u) / natray::MAX_CHUNKS;
+    m[k] = (nb + K[k] - 1u) / K[k];
+    it[k] = (uint64_t)st.solve_iters + 1u;
   }
-  std::vector<uint32_t> shares;
-  plan_shares(w, std::vector<uint32_t>(n, 0xffffffffu), (uint32_t)b->solve_budget, 1, shares);
+  auto cost = [&](size_t k, uint32_t G) {
+    return it[k] * (kSolveFixed + (uint64_t)((m[k] + G - 1u) / G) * K[k]);
+  };
+  uint32_t used = (uint32_t)n;
+  while (true) {
+    size_t worst = 0;
+    for (size_t k = 1; k < n; ++k)
+      if (cost(k, shares[k]) > cost(worst, shares[worst])) worst = k;
+    // The fewest workgroups that hold one chunk fewer each (a workgroup holds whole chunks);
+    // none once every workgroup holds one, or when they do not fit.
+    const uint32_t most = (m[worst] + shares[worst] - 1u) / shares[worst];
+    if (most <= 1u) break;
+    const uint32_t G = (m[worst] + most - 2u) / (most - 1u);
+    if (used + (G - shares[worst]) > (uint32_t)b->solve_budget) break;
+    used += G - shares[worst];
+    shares[worst] = G;
+  }
   if (shares == b->counts[natray::LK_SOLVE]) return NATRAY_OK;
-  std::vector<natray::WgMap> m;
+  std::vector<natray::WgMap> map;
   for (size_t k = 0; k < n; ++k)
-    for (uint32_t g = 0; g < shares[k]; ++g) m.push_back({(uint32_t)k, g, shares[k], 0u});
-  if (m.size() > b->map_cap[natray::LK_SOLVE]) return NATRAY_OK;  // keeps the current plan
-  HIPCHK(hipMemcpyAsync(b->maps[natray::LK_SOLVE], m.data(), m.size() * sizeof(natray::WgMap), hipMemcpyHostToDevice,
+    for (uint32_t g = 0; g < shares[k]; ++g) map.push_back({(uint32_t)k, g, shares[k], 0u});
+  if (map.size() > b->map_cap[natray::LK_SOLVE]) return NATRAY_OK;  // keeps the current plan
+  HIPCHK(hipMemcpyAsync(b->maps[natray::LK_SOLVE], map.data(), map.size() * sizeof(natray::WgMap), hipMemcpyHostToDevice,
                         b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->counts[natray::LK_SOLVE] = shares;
-  b->tgt.grid[natray::LK_SOLVE] = (uint32_t)m.size();
+  b->tgt.grid[natray::LK_SOLVE] = (uint32_t)map.size();
   return NATRAY_OK;
 }
 
