@@ -542,6 +542,11 @@ __device__ __forceinline__ uint32_t tile_cell(const uint32_t* links, uint32_t t)
   return b * 64u + (uint32_t)(rx - nx * 4) * 16u + (uint32_t)(ry - ny * 4) * 4u + (uint32_t)(rz - nz * 4);
 }
 
+// P2G's LDS tile: the 7^3 nodes a block's particles reach, x-major with padded strides, so
+// that the atomics of a half-wave's stencils spread over more LDS banks.
+constexpr uint32_t P2G_SX = 60, P2G_SY = 8;
+constexpr uint32_t P2G_TILE = 6u * P2G_SX + 6u * P2G_SY + 7u;
+
 // splat (fluid.wgsl:233-252) of one lattice (offset o = (OX, OY, OZ) / 2 cells) into the tile.
 template <int OX, int OY, int OZ>
 __device__ __forceinline__ void splat(int* tile, const float x[3], const int corner[3], uint32_t fa, float val,
@@ -559,9 +564,9 @@ __device__ __forceinline__ void splat(int* tile, const float x[3], const int cor
   axis_weights(fx[0], wx);
   axis_weights(fx[1], wy);
   axis_weights(fx[2], wz);
-  int t0 = (base[0] - corner[0]) * 49 + (base[1] - corner[1]) * 7 + (base[2] - corner[2]);
-  int* ta = tile + fa * TILE + t0;
-  int* tw = tile + fw * TILE + t0;
+  int t0 = (base[0] - corner[0]) * (int)P2G_SX + (base[1] - corner[1]) * (int)P2G_SY + (base[2] - corner[2]);
+  int* ta = tile + fa * P2G_TILE + t0;
+  int* tw = tile + fw * P2G_TILE + t0;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -570,8 +575,8 @@ __device__ __forceinline__ void splat(int* tile, const float x[3], const int cor
       for (int c = 0; c < 3; ++c) {
         float weight = wx[a] * wy[b] * wz[c];
         float node = val + ((hg[0] * ((float)a - fx[0]) + hg[1] * ((float)b - fx[1])) + hg[2] * ((float)c - fx[2]));
-        atomicAdd(ta + a * 49 + b * 7 + c, to_fp(weight * node, scale));
-        atomicAdd(tw + a * 49 + b * 7 + c, to_fp_weight(weight));
+        atomicAdd(ta + a * (int)P2G_SX + b * (int)P2G_SY + c, to_fp(weight * node, scale));
+        atomicAdd(tw + a * (int)P2G_SX + b * (int)P2G_SY + c, to_fp_weight(weight));
       }
 }
 
@@ -636,7 +641,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const View v = view(l);
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
-  __shared__ int tile[8 * TILE];
+  __shared__ int tile[8 * P2G_TILE];
   __shared__ uint32_t cnt[64], drop[64];
   __shared__ uint32_t order[P2G_MAXP];
   __shared__ uint8_t ocell[P2G_MAXP];
@@ -653,7 +658,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
       cnt[tid] = 0;
       drop[tid] = 0;
     }
-    for (uint32_t t = tid; t < 8 * TILE; t += 256) tile[t] = 0;
+    for (uint32_t t = tid; t < 8 * P2G_TILE; t += 256) tile[t] = 0;
     const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
     __syncthreads();
     if (count == 0u) {
@@ -737,9 +742,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     for (uint32_t t = tid; t < TILE; t += 256) {
       uint32_t cell = tile_cell(links, t);
       if (cell == NONE) continue;
+      const uint32_t pt = t / 49u * P2G_SX + t / 7u % 7u * P2G_SY + t % 7u;
 #pragma unroll
       for (uint32_t f = 0; f < 8u; ++f) {
-        int v = tile[f * TILE + t];
+        int v = tile[f * P2G_TILE + pt];
         if (v != 0) atomicAdd(&accr(d, f, cell), v);
       }
     }
