@@ -356,6 +356,10 @@ def batched_report(args) -> dict:
         "ms_per_batch_step": wall * 1e3 / max(args.batch_steps, 1),
         "particle_substeps_per_s": parts / wall if wall > 0 else 0.0,
         "cg_iters_per_scene_step": (c1["cg_iters"] - c0["cg_iters"]) / max(c1["steps"] - c0["steps"], 1),
+        # Blocks per batch step (all scenes) and particles per block: the batch's small scenes
+        # hold more surface blocks per particle than one large scene.
+        "blocks_per_batch_step": (c1["blocks"] - c0["blocks"]) / max(args.batch_steps, 1),
+        "particles_per_block": parts / max(c1["blocks"] - c0["blocks"], 1),
         "phases_ms_per_batch_step_kernel_by_kernel": {k: v[0] / max(v[1], 1) for k, v in times.items()},
         "scenes_one_after_another": {
             "ms_per_step_of_all_scenes": wall_alone * 1e3 / max(args.batch_steps, 1),
@@ -645,6 +649,7 @@ def main() -> int:
         "particle_substeps_per_s_outside_cg": particles / max(wall - t_cg, 1e-12),
         "cg_iters_per_step": c["cg_iters"] / max(steps_done, 1),
         "blocks_per_step": c["blocks"] / max(steps_done, 1),
+        "particles_per_block": particles / max(c["blocks"], 1),
         "water_cells_per_step": c["water_cells"] / max(steps_done, 1),
         "final_state": {"t_ns": st.t_ns, "step": st.step, "dt_ns": st.dt_ns, "unconverged": st.unconverged,
                         "max_speed": st.max_speed},
