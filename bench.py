@@ -94,6 +94,30 @@ def measured_traffic(kernel: str) -> float | None:
     return None
 
 
+# The kernels of each phase, as the counter summary names them (single scene, FP32 CG).
+PHASE_KERNELS = {
+    "bin": ["k_clear<One>", "k_mark<One>", "k_scan_reduce<One>", "k_scan_tiles<One>", "k_scan_scatter<One>", "k_sort<One>"],
+    "p2g": ["k_p2g<One>"],
+    "grid_prep": ["k_grid_prep<One>"],
+    "solve": ["k_solve<false,One>"],
+    "project": ["k_project<One>"],
+    "extend": ["k_extend1<One>", "k_extend2<One>"],
+    "g2p": ["k_g2p<One,false>", "k_vmax<One>"],
+}
+
+
+def measured_phase_traffic() -> dict | None:
+    """Counter-measured HBM bytes per step of each phase (its kernels' fetch + write per launch,
+    one launch each per step), or None when the summary lacks a kernel."""
+    out = {}
+    for ph, ks in PHASE_KERNELS.items():
+        t = [measured_traffic(k) for k in ks]
+        if any(x is None for x in t):
+            return None
+        out[ph] = float(sum(t))
+    return out
+
+
 def timed_run(sim, warmup: int, steps: int, iters: int, graph: bool = True):
     """Warmup steps, then `steps` timed steps with events on the library's stream; wall time,
     counter deltas, phase times and the final State. With `graph` (the library's default)
@@ -619,6 +643,7 @@ def main() -> int:
     value = particles / wall if wall > 0 else 0.0
 
     pb = phase_bytes(c)
+    ptr = measured_phase_traffic() if args.workload == "impact" and args.precision == "fp32" else None
     single = ("p2g", "grid_prep", "solve", "project", "g2p")  # phases that are one kernel
     dominant = max(single, key=lambda k: times_eager[k][0])
     # The dominant kernel's launches are timed in the graph run when it is the solve (launched
@@ -658,6 +683,12 @@ def main() -> int:
         "phases_algorithmic_GBps": {k: (pb[k] / (times_eager[k][0] * 1e-3) / 1e9 if times_eager[k][0] > 0 else 0.0)
                                     for k in pb},
         "kernel_time_fraction_of_wall_kernel_by_kernel": t_all_kernels / wall_eager if wall_eager > 0 else 0.0,
+        # Counter-side figures per phase (the profile's HBM bytes per launch, this run's phase
+        # times), and the counter bytes over the algorithmic ones: above 1 means re-reads.
+        "phases_counter_GBps": ({k: v / (times_eager[k][0] * 1e-3 / max(steps_done, 1)) / 1e9
+                                 for k, v in ptr.items() if times_eager[k][0] > 0} if ptr else None),
+        "phases_counter_over_algorithmic": ({k: v * max(steps_done, 1) / pb[k] for k, v in ptr.items() if pb[k] > 0}
+                                            if ptr else None),
         "roofline": {
             "kernel": kernel_names[dominant],
             "bound": "hbm",

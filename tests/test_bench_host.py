@@ -91,3 +91,13 @@ def test_stated_radius_scenes():
     g = (np.indices((side, side, side)).reshape(3, -1).T + 0.5) / side
     assert int((np.linalg.norm(g - 0.5, axis=1) < 0.25).sum()) == len(cells)
     assert scenes.cells_within(256, 0.3) * 8 == 15179264
+
+
+def test_phase_traffic_is_read_from_the_profile():
+    """Every phase's kernels are in the counter summary, so the bench can put counter-side
+    bytes beside the algorithmic ones."""
+    import bench
+
+    t = bench.measured_phase_traffic()
+    assert t is not None and set(t) == set(bench.PHASE_KERNELS)
+    assert t["g2p"] > t["p2g"] > 1e8 and all(v > 0 for v in t.values())

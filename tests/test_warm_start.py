@@ -215,7 +215,8 @@ def test_trained_network_gain():
     on: training tiles from two seeded runs of the 32k-particle impact, Adam on the residual
     loss with the network's own GPU kernels; then 20 steps with the warm start against 20
     without. Every guess must lower the starting residual (all are taken), by at least half in
-    squared norm, and the solve must not need more iterations than without it."""
+    squared norm, and the solve must save at least 5 of the ~470 CG iterations of the 20 steps
+    (measured: 23.50 iterations per step without it, 22.95 with it)."""
     from natray_amd import scenes
     from natray_amd import warm_start as ws
 
@@ -257,7 +258,7 @@ def test_trained_network_gain():
             assert np.mean(ratios) < 0.5
         sim.close()
     say(f"[warm] CG iterations per step: plain {iters['plain']:.2f}, warm-started {iters['net']:.2f}")
-    assert iters["net"] <= iters["plain"]
+    assert iters["net"] <= iters["plain"] - 0.25, iters
     n.close()
 
 
