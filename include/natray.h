@@ -315,6 +315,31 @@ int natray_sim_set_graph(natray_sim* sim, int on);
  * binned, [6] blocks cleared, [7] 0. They give the exact algorithmic bytes of a run. */
 int natray_sim_counters(natray_sim* sim, uint64_t out[8]);
 
+/* The water line of debug_stats (src/sim.rs:691-716) for the particles as they are now: over
+ * the alive particles (record float 7 > 0.5), their count; their bounding box; mean position
+ * and velocity; sum of 0.5 |v|^2 (per unit particle mass: times the particle mass
+ * RHO_WATER spacing^3 for joules, src/sim.rs:853); the fastest one's speed and position (the
+ * first in particle order among equal speeds, as the reference's strict comparison keeps; 0
+ * with none); and the grid cells floor((x - origin) / dx) holding at least 4 of them, with
+ * their volume in litres (cells x dx^3 x 1e3). The reference sums in f32 one particle after
+ * another; here the sums are f64 in a fixed order that does not depend on timing. With no
+ * alive particle, lo is FLT_MAX and hi -FLT_MAX, as the reference's fold leaves them.
+ * Allocates a temporary cell table of 12 bytes x the next power of two >= 2 n (freed before
+ * returning). */
+typedef struct natray_water_stats {
+  double mean[3];
+  double vmean[3];
+  double ke;
+  double volume_l;
+  float lo[3];
+  float hi[3];
+  float fastest[3];
+  float vmax;
+  uint32_t alive;
+  uint32_t full_cells;
+} natray_water_stats;
+int natray_sim_water_stats(natray_sim* sim, natray_water_stats* out);
+
 const char* natray_last_error(void);
 
 /* ---- the CG warm-start network ------------------------------------------------------------

@@ -144,6 +144,23 @@ class Scene(C.Structure):
     ]
 
 
+class WaterStats(C.Structure):
+    """natray_water_stats: the water line of debug_stats (src/sim.rs:691-716)."""
+
+    _fields_ = [
+        ("mean", C.c_double * 3),
+        ("vmean", C.c_double * 3),
+        ("ke", C.c_double),
+        ("volume_l", C.c_double),
+        ("lo", C.c_float * 3),
+        ("hi", C.c_float * 3),
+        ("fastest", C.c_float * 3),
+        ("vmax", C.c_float),
+        ("alive", C.c_uint32),
+        ("full_cells", C.c_uint32),
+    ]
+
+
 PARTICLE_FLOATS = 20  # 5 x vec4f, 80 bytes (shaders/sim_defs.wgsl:61)
 
 
@@ -179,6 +196,7 @@ _SIGNATURES = {
     "natray_sim_set_graph": (C.c_int, [_P, C.c_int]),
     "natray_sim_counters": (C.c_int, [_P, _P]),
     "natray_sim_run_log": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(C.c_uint32)]),
+    "natray_sim_water_stats": (C.c_int, [_P, _P]),
     "natray_sim_set_warm_start": (C.c_int, [_P, _P, C.c_int]),
     "natray_sim_set_membrane": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "natray_sim_pack_water": (C.c_int, [_P, _P]),

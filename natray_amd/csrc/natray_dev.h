@@ -205,6 +205,21 @@ int launch_pack_keyframe(const Target& t, void* stream, const KeyframeArgs& a);
 // looked up once per device with that device current.
 #define NATRAY_MAX_DEVICES 64u
 int kf_scratch(int device, uint32_t** out);
+
+// One workgroup's share of natray_sim_water_stats (the host folds them in workgroup order).
+struct WaterPart {
+  double sum[7];  // x, y, z, vx, vy, vz, 0.5 |v|^2 of the alive particles
+  float lo[3], hi[3];
+  float vmax;
+  uint32_t fastest;  // particle index of the fastest (the first among equal speeds), NONE with none
+  uint32_t alive;
+  uint32_t pad;
+};
+// The water diagnostics' kernels: particles[20 n] on `groups` workgroups, the cell table of
+// `slots` keys and counts (a power of two >= 2 n, keys cleared to all ones and counts to 0
+// beforehand), parts[groups], and *full (cleared beforehand) gets the cells with >= 4.
+int launch_water_stats(const float* particles, uint32_t n, const float origin[3], float h, unsigned long long* keys,
+                       uint32_t* counts, uint32_t slots, WaterPart* parts, uint32_t groups, uint32_t* full, void* stream);
 // Warm start (one scene), before the solve: the network's inputs for at most `cap` tiles (also
 // into the network's BF16 input buffer `net_in`, [cap][4096][8]), then, after the network has
 // run on them (its output `pred`, [cap][4096][16] FP32), the guess and its acceptance. After a

@@ -4148,6 +4148,135 @@ int kf_scratch(int device, uint32_t** out) {
   return 0;
 }
 
+// ---- water diagnostics (debug_stats' water line, src/sim.rs:691-716) ----------------------------
+//
+// natray_sim_water_stats: the alive particles' count, box, mean position and velocity, kinetic
+// energy per unit mass, fastest particle, and the grid cells that hold at least 4 of them now.
+// Each workgroup sums its particles (a fixed grid-stride share) in double, in index order per
+// thread, then over its threads as a fixed tree; the host folds the workgroups in order, so
+// the sums do not depend on timing. The cells are counted in an open-addressing hash table of
+// cell keys (integer counts, exact in any order).
+
+constexpr unsigned long long CELL_EMPTY = ~0ull;
+
+// The key of the cell holding (x, y, z): floor((p - origin) / h) per axis as the reference
+// computes it in f32, each coordinate offset by 2^20 into 21 bits (coordinates beyond +-2^20
+// cells are clamped).
+__device__ __forceinline__ unsigned long long water_cell_key(float x, float y, float z, float ox, float oy, float oz,
+                                                             float h) {
+  const float c[3] = {floorf((x - ox) / h), floorf((y - oy) / h), floorf((z - oz) / h)};
+  unsigned long long k = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float f = fminf(fmaxf(c[a], -1048576.0f), 1048575.0f);
+    k = (k << 21) | (unsigned long long)((int)f + 1048576);
+  }
+  return k;
+}
+
+__global__ __launch_bounds__(256) void k_water_stats(const float4* P, uint32_t n, float ox, float oy, float oz, float h,
+                                                     unsigned long long* keys, uint32_t* counts, uint32_t mask,
+                                                     WaterPart* out) {
+  double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  float vmax = 0.0f;
+  uint32_t fast = NONE, alive = 0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const float4 pos = P[(size_t)i * 5u], vel = P[(size_t)i * 5u + 1u];
+    if (!(vel.w > 0.5f)) continue;
+    alive += 1u;
+    lo[0] = fminf(lo[0], pos.x);
+    lo[1] = fminf(lo[1], pos.y);
+    lo[2] = fminf(lo[2], pos.z);
+    hi[0] = fmaxf(hi[0], pos.x);
+    hi[1] = fmaxf(hi[1], pos.y);
+    hi[2] = fmaxf(hi[2], pos.z);
+    s[0] += (double)pos.x;
+    s[1] += (double)pos.y;
+    s[2] += (double)pos.z;
+    s[3] += (double)vel.x;
+    s[4] += (double)vel.y;
+    s[5] += (double)vel.z;
+    const float v2 = vel.x * vel.x + vel.y * vel.y + vel.z * vel.z;
+    s[6] += 0.5 * (double)v2;
+    const float sp = __builtin_sqrtf(v2);
+    if (sp > vmax) {
+      vmax = sp;
+      fast = i;
+    }
+    // The table has at least twice as many slots as particles, so a free slot is always found.
+    const unsigned long long key = water_cell_key(pos.x, pos.y, pos.z, ox, oy, oz, h);
+    uint32_t slot = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+      const unsigned long long prev = atomicCAS(&keys[slot], CELL_EMPTY, key);
+      if (prev == CELL_EMPTY || prev == key) {
+        atomicAdd(&counts[slot], 1u);
+        break;
+      }
+      slot = (slot + 1u) & mask;
+    }
+  }
+  // The workgroup's 256 threads as a fixed tree: lanes within each wave, then the 4 waves.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) s[k] += __shfl_xor(s[k], off, 64);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+    const float v2 = __shfl_xor(vmax, off, 64);
+    const uint32_t f2 = (uint32_t)__shfl_xor((int)fast, off, 64);
+    if (v2 > vmax || (v2 == vmax && f2 < fast)) {
+      vmax = v2;
+      fast = f2;
+    }
+    alive += (uint32_t)__shfl_xor((int)alive, off, 64);
+  }
+  __shared__ WaterPart w[4];
+  const uint32_t wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0u) {
+    WaterPart& o = w[wv];
+    for (int k = 0; k < 7; ++k) o.sum[k] = s[k];
+    for (int a = 0; a < 3; ++a) {
+      o.lo[a] = lo[a];
+      o.hi[a] = hi[a];
+    }
+    o.vmax = vmax;
+    o.fastest = fast;
+    o.alive = alive;
+    o.pad = 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    WaterPart o = w[0];
+    for (int q = 1; q < 4; ++q) {
+      const WaterPart& b = w[q];
+      for (int k = 0; k < 7; ++k) o.sum[k] += b.sum[k];
+      for (int a = 0; a < 3; ++a) {
+        o.lo[a] = fminf(o.lo[a], b.lo[a]);
+        o.hi[a] = fmaxf(o.hi[a], b.hi[a]);
+      }
+      if (b.vmax > o.vmax || (b.vmax == o.vmax && b.fastest < o.fastest)) {
+        o.vmax = b.vmax;
+        o.fastest = b.fastest;
+      }
+      o.alive += b.alive;
+    }
+    out[blockIdx.x] = o;
+  }
+}
+
+// Table slots whose cell holds at least 4 particles, added into *full.
+__global__ __launch_bounds__(256) void k_water_cells(const uint32_t* counts, uint32_t slots, uint32_t* full) {
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < slots; i += gridDim.x * 256u) c += counts[i] >= 4u ? 1u : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += (uint32_t)__shfl_xor((int)c, off, 64);
+  if ((threadIdx.x & 63u) == 0u && c) atomicAdd(full, c);
+}
+
 // ---- launch wrappers -------------------------------------------------------------------------------
 
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -4319,6 +4448,16 @@ int launch_spray_index(const Target& t, void* stream, uint32_t n, uint32_t cw, u
 }
 int launch_pack_water(const Target& t, void* stream, void* out) {
   NATRAY_LAUNCH(k_pack_water, LK_PARTS, 256, t, S(stream), static_cast<uint2*>(out));
+  return (int)hipGetLastError();
+}
+
+int launch_water_stats(const float* particles, uint32_t n, const float origin[3], float h, unsigned long long* keys,
+                       uint32_t* counts, uint32_t slots, WaterPart* parts, uint32_t groups, uint32_t* full, void* stream) {
+  if (!n || !groups || slots < 2u * n || (slots & (slots - 1u))) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_water_stats, dim3(groups), dim3(256), 0, S(stream), reinterpret_cast<const float4*>(particles), n,
+                     origin[0], origin[1], origin[2], h, keys, counts, slots - 1u, parts);
+  hipLaunchKernelGGL(k_water_cells, dim3(std::min<uint32_t>(1024u, (slots + 255u) / 256u)), dim3(256), 0, S(stream), counts,
+                     slots, full);
   return (int)hipGetLastError();
 }
 

@@ -1126,6 +1126,87 @@ int natray_sim_counters(natray_sim* s, uint64_t out[8]) {
   return NATRAY_OK;
 }
 
+int natray_sim_water_stats(natray_sim* s, natray_water_stats* out) {
+  if (!s || !out) return fail(NATRAY_E_ARG, "null argument");
+  natray_water_stats r{};
+  for (int a = 0; a < 3; ++a) {
+    r.lo[a] = FLT_MAX;
+    r.hi[a] = -FLT_MAX;
+  }
+  const uint32_t n = s->D.N;
+  if (n == 0) {
+    *out = r;
+    return NATRAY_OK;
+  }
+  uint32_t slots = 1024;
+  while (slots < 2u * n) slots <<= 1;
+  const uint32_t groups = std::min<uint32_t>(1024u, (n + 255u) / 256u);
+  // One allocation: the keys, the counts, the workgroups' parts and the full-cell counter.
+  const size_t off_counts = (size_t)slots * 8, off_parts = off_counts + (size_t)slots * 4,
+               off_full = off_parts + (size_t)groups * sizeof(natray::WaterPart), bytes = off_full + 256;
+  char* buf = nullptr;
+  hipError_t e = hipMalloc(&buf, bytes);
+  if (e != hipSuccess) return fail(NATRAY_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  std::vector<natray::WaterPart> parts(groups);
+  uint32_t full = 0;
+  int rc = NATRAY_OK;
+  auto step = [&](hipError_t x) {
+    if (x != hipSuccess && rc == NATRAY_OK) rc = fail(NATRAY_E_HIP, std::string("HIP: ") + hipGetErrorString(x));
+    return rc == NATRAY_OK;
+  };
+  auto* keys = reinterpret_cast<unsigned long long*>(buf);
+  auto* counts = reinterpret_cast<uint32_t*>(buf + off_counts);
+  auto* dparts = reinterpret_cast<natray::WaterPart*>(buf + off_parts);
+  auto* dfull = reinterpret_cast<uint32_t*>(buf + off_full);
+  const float origin[3] = {s->U.origin[0], s->U.origin[1], s->U.origin[2]};
+  if (step(hipMemsetAsync(keys, 0xff, (size_t)slots * 8, s->stream)) &&
+      step(hipMemsetAsync(counts, 0, (size_t)slots * 4, s->stream)) && step(hipMemsetAsync(dfull, 0, 4, s->stream)) &&
+      step((hipError_t)natray::launch_water_stats(s->D.P, n, origin, s->U.origin[3], keys, counts, slots, dparts, groups,
+                                                  dfull, s->stream)) &&
+      step(hipMemcpyAsync(parts.data(), dparts, (size_t)groups * sizeof(natray::WaterPart), hipMemcpyDeviceToHost,
+                          s->stream)) &&
+      step(hipMemcpyAsync(&full, dfull, 4, hipMemcpyDeviceToHost, s->stream)))
+    step(hipStreamSynchronize(s->stream));
+  const hipError_t ef = hipFree(buf);
+  if (rc != NATRAY_OK) return rc;
+  if (ef != hipSuccess) return fail(NATRAY_E_HIP, std::string("hipFree: ") + hipGetErrorString(ef));
+  // The workgroups in order: sums, box, and the fastest (the lowest index among equal speeds).
+  double sum[7] = {};
+  uint32_t fast = 0xffffffffu;
+  for (const natray::WaterPart& p : parts) {
+    for (int k = 0; k < 7; ++k) sum[k] += p.sum[k];
+    for (int a = 0; a < 3; ++a) {
+      r.lo[a] = std::fmin(r.lo[a], p.lo[a]);
+      r.hi[a] = std::fmax(r.hi[a], p.hi[a]);
+    }
+    if (p.vmax > r.vmax || (p.vmax == r.vmax && p.fastest < fast)) {
+      r.vmax = p.vmax;
+      fast = p.fastest;
+    }
+    r.alive += p.alive;
+  }
+  if (r.alive) {
+    for (int a = 0; a < 3; ++a) {
+      r.mean[a] = sum[a] / r.alive;
+      r.vmean[a] = sum[3 + a] / r.alive;
+    }
+  }
+  r.ke = sum[6];
+  if (fast != 0xffffffffu && r.vmax > 0.0f) {
+    float rec[4];
+    HIPCHK(hipMemcpyAsync(rec, s->D.P + (size_t)fast * 20, 16, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int a = 0; a < 3; ++a) r.fastest[a] = rec[a];
+  } else {
+    r.vmax = 0.0f;
+  }
+  r.full_cells = full;
+  const double dx = s->U.origin[3];
+  r.volume_l = (double)full * dx * dx * dx * 1e3;
+  *out = r;
+  return NATRAY_OK;
+}
+
 }  // extern "C"
 
 // ---- batches of independent scenes --------------------------------------------------------------

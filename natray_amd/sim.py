@@ -168,6 +168,28 @@ class Sim:
         abi.check(abi.lib().natray_sim_run_log(self._h, int(cnt.value), abi.ptr(rows), C.byref(cnt)))
         return [dict(zip(self.RUN_LOG_KEYS, (int(v) for v in r[:7]))) for r in rows[: int(cnt.value)]]
 
+    def water_stats(self) -> dict:
+        """The water line of the reference's debug_stats (src/sim.rs:691-716) for the particles
+        as they are now (natray_sim_water_stats): alive count, box, mean position and velocity,
+        sum of 0.5 |v|^2 per unit mass, the fastest particle, and the cells with >= 4 particles
+        with their volume in litres."""
+        w = abi.WaterStats()
+        abi.check(abi.lib().natray_sim_water_stats(self._h, C.byref(w)))
+        return {"alive": int(w.alive), "lo": list(w.lo), "hi": list(w.hi), "mean": list(w.mean),
+                "vmean": list(w.vmean), "ke": float(w.ke), "vmax": float(w.vmax), "fastest": list(w.fastest),
+                "full_cells": int(w.full_cells), "volume_l": float(w.volume_l)}
+
+    def debug_stats(self) -> str:
+        """The reference's debug_stats water line (src/sim.rs:718-740), from water_stats and
+        the State: time, dt, step, water, box, means, fastest, volume, blocks and the solver."""
+        w, st = self.water_stats(), self.state()
+        f = lambda v, d: "[" + ", ".join(f"{x:.{d}f}" for x in v) + "]"
+        return (f"t={st.t_ns * 1e-9:.6f} dt={st.dt_ns}ns steps {st.step} | water {w['alive']}/{self.n} "
+                f"bbox {f(w['lo'], 3)}..{f(w['hi'], 3)} mean {f(w['mean'], 4)} v {f(w['vmean'], 3)} "
+                f"vmax {w['vmax']:.2f} at {f(w['fastest'], 4)} volume {w['volume_l']:.3f} L | blocks {st.n_blocks} "
+                f"solver {st.solve_iters} iters r {st.r_norm:.2e} of {st.b_norm:.2e}, {st.unconverged} unconverged "
+                f"steps | spray {st.spray_count}")
+
     def set_membrane(self, x: np.ndarray, v: np.ndarray, mass: np.ndarray, push: np.ndarray) -> None:
         """The membrane's vertices for the coming steps (natray_sim_set_membrane): positions and
         velocities [n][4], masses [n], push (N) and area (m^2) [n][2]."""

@@ -102,6 +102,45 @@ def elementary(which: str, a: np.ndarray, b: np.ndarray | None = None) -> np.nda
     return out
 
 
+def water_stats(particles: np.ndarray, origin, h: float) -> dict:
+    """The water line of debug_stats (src/sim.rs:691-716) over particle records [n][20]: the
+    alive ones (float 7 > 0.5, :693); their box (:702-703); mean position and velocity (:704-705,
+    here summed in float64); the sum of 0.5 |v|^2 in float64 of the f32 |v|^2 (:706); the first
+    particle whose f32 speed sqrt((x x + y y) + z z) is the strict maximum above 0 (:707-710);
+    and the cells floor((x - origin) / h), computed in f32, that hold at least 4 (:711-716), with
+    their volume in litres (:735)."""
+    P = np.asarray(particles, np.float32).reshape(-1, 20)
+    A = P[P[:, 7] > np.float32(0.5)]
+    x, v = A[:, 0:3], A[:, 4:7]
+    n = len(A)
+    out = {"alive": n}
+    if n:
+        out["lo"], out["hi"] = [float(t) for t in x.min(0)], [float(t) for t in x.max(0)]
+        out["mean"] = [float(t) for t in x.astype(np.float64).sum(0) / n]
+        out["vmean"] = [float(t) for t in v.astype(np.float64).sum(0) / n]
+    else:
+        big = float(np.finfo(np.float32).max)
+        out["lo"], out["hi"] = [big] * 3, [-big] * 3
+        out["mean"] = out["vmean"] = [0.0] * 3
+    v2 = (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]
+    out["ke"] = float((0.5 * v2.astype(np.float64)).sum())
+    sp = np.sqrt(v2)
+    if n and float(sp.max()) > 0.0:
+        k = int(np.argmax(sp))  # the first of the maxima
+        out["vmax"], out["fastest"] = float(sp[k]), [float(t) for t in x[k]]
+    else:
+        out["vmax"], out["fastest"] = 0.0, [0.0, 0.0, 0.0]
+    o = np.asarray(origin, np.float32)[:3]
+    cells = np.floor((x - o) / np.float32(h)).astype(np.int64)
+    full = 0
+    if n:
+        _, counts = np.unique(cells, axis=0, return_counts=True)
+        full = int((counts >= 4).sum())
+    out["full_cells"] = full
+    out["volume_l"] = full * float(h) ** 3 * 1e3
+    return out
+
+
 class Oracle:
     def __init__(self, params: abi.Params, particles: np.ndarray, scene: abi.Scene | None = None,
                  init: abi.State | None = None, precision: str = "f32"):
