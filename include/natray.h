@@ -304,10 +304,11 @@ int natray_sim_set_profiling(natray_sim* sim, int on);
 int natray_sim_phase_times(natray_sim* sim, double* ms, uint32_t* counts);
 
 /* How natray_sim_run launches a step (no counterpart in the reference, whose step is one
- * encoded command buffer, src/sim.rs:602-627). On (the default): the kernels before the
- * solve and those after it are two HIP graphs, captured once and replayed every step, with
- * the cooperative solve launched between them. Off: every kernel is launched on its own.
- * Both give the same bits. With graphs on, profiling times only the solve phase. */
+ * encoded command buffer, src/sim.rs:602-627). On (the default): the step is one HIP graph,
+ * captured once and replayed every step, whose solve is a cooperative kernel node; with
+ * profiling or the warm start on, the kernels before the solve and those after it are two
+ * graphs with the solve launched between them. Off: every kernel is launched on its own. All
+ * give the same bits. With graphs on, profiling times only the solve phase. */
 int natray_sim_set_graph(natray_sim* sim, int on);
 
 /* Work counters since creation, summed over the steps that ran: [0] steps, [1] CG

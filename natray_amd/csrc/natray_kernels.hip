@@ -4347,7 +4347,19 @@ int launch_solve(const Target& t, void* stream, int kind) {
     fn = t.d ? (const void*)k_solve<true, One> : (const void*)k_solve<true, Many>;
   else
     fn = t.d ? (const void*)k_solve<false, One> : (const void*)k_solve<false, Many>;
-  return (int)hipLaunchCooperativeKernel(fn, grid, block, t.d ? args_one : args_many, 0, S(stream));
+  // A cooperative launch (every workgroup co-resident, or an error), through the launch
+  // attribute so that stream capture can record it as a graph node.
+  hipLaunchAttribute attr{};
+  attr.id = hipLaunchAttributeCooperative;
+  attr.val.cooperative = 1;
+  hipLaunchConfig_t cfg{};
+  cfg.gridDim = grid;
+  cfg.blockDim = block;
+  cfg.dynamicSmemBytes = 0;
+  cfg.stream = S(stream);
+  cfg.attrs = &attr;
+  cfg.numAttrs = 1;
+  return (int)hipLaunchKernelExC(&cfg, fn, t.d ? args_one : args_many);
 }
 int launch_project(const Target& t, void* stream) {
   NATRAY_LAUNCH(k_project, LK_WAVES, 256, t, S(stream));

@@ -85,7 +85,8 @@ struct natray_sim {
   // Step graphs: the phases before the solve and those after it, captured on `stream`.
   bool use_graph = true;
   bool capturing = false;  // no timing events go into a captured graph
-  hipGraphExec_t graph_pre = nullptr, graph_post = nullptr;
+  hipGraphExec_t graph_pre = nullptr, graph_post = nullptr, graph_all = nullptr;
+  bool skip_tags = false;  // the step's caller did the solve's tag bookkeeping (graph_all)
   // Learned warm start of the solve (natray_sim_set_warm_start).
   int ws_mode = 0;
   natray_net* ws_net = nullptr;
@@ -117,6 +118,8 @@ void free_all(natray_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
   if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+  if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
+  s->graph_all = nullptr;
   s->graph_pre = s->graph_post = nullptr;
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
   s->events.clear();
@@ -221,6 +224,8 @@ int alloc_warm_start(natray_sim* s) {
   HIPCHK(hipStreamSynchronize(s->stream));
   if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
   if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+  if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
+  s->graph_all = nullptr;
   s->graph_pre = s->graph_post = nullptr;
   s->ws_ready = true;
   return NATRAY_OK;
@@ -251,12 +256,14 @@ int enqueue_phase(natray_sim* s, int phase) {
       break;
     case NATRAY_PHASE_SOLVE: {
       // Tags are 32-bit: clear the tagged words well before a tag could repeat.
-      const uint64_t bound = solve_tag_bound(s);
-      if (s->tags_used + bound >= kTagLimit) {
-        int e = clear_tags(s);
-        if (e) return e;
+      if (!s->skip_tags) {
+        const uint64_t bound = solve_tag_bound(s);
+        if (s->tags_used + bound >= kTagLimit) {
+          int e = clear_tags(s);
+          if (e) return e;
+        }
+        s->tags_used += bound;
       }
-      s->tags_used += bound;
       if (s->ws_mode) {
         int e = enqueue_warm_start(s);
         if (e) return e;
@@ -301,14 +308,33 @@ int capture_phases(natray_sim* s, int lo, int hi, hipGraphExec_t* out) {
   return NATRAY_OK;
 }
 
-// One step: the two captured graphs around the solve, or every phase launched on its own.
-// With profiling on, the graphs' phases are not timed; the solve between them is.
+// One step: one captured graph of every phase, its solve a cooperative kernel node; or, with
+// profiling or the warm start on, two graphs around the solve launched between them (its
+// events, and the warm start's host work, stay outside the graphs; profiling then times only
+// the solve); or every phase launched on its own. The solve's tag bookkeeping is host work, so
+// with the single graph it is done here before each replay.
 int enqueue_step(natray_sim* s) {
   if (!s->use_graph) {
     for (int ph = 0; ph < NATRAY_N_PHASES; ++ph) {
       int e = enqueue_phase(s, ph);
       if (e) return e;
     }
+    return NATRAY_OK;
+  }
+  if (!s->ws_mode && !s->prof) {
+    const uint64_t bound = solve_tag_bound(s);
+    if (s->tags_used + bound >= kTagLimit) {
+      int e = clear_tags(s);
+      if (e) return e;
+    }
+    s->tags_used += bound;
+    if (!s->graph_all) {
+      s->skip_tags = true;
+      int e = capture_phases(s, NATRAY_PHASE_BEGIN, NATRAY_N_PHASES, &s->graph_all);
+      s->skip_tags = false;
+      if (e) return e;
+    }
+    HIPCHK(hipGraphLaunch(s->graph_all, s->stream));
     return NATRAY_OK;
   }
   if (!s->graph_pre) {
@@ -870,6 +896,8 @@ int natray_sim_set_membrane(natray_sim* s, uint32_t n, const float* x, const flo
     HIPCHK(hipMemcpyAsync(const_cast<natray::Dev*>(D.self), &D, sizeof(natray::Dev), hipMemcpyHostToDevice, s->stream));
     if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
     if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+    if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
+    s->graph_all = nullptr;
     s->graph_pre = s->graph_post = nullptr;
   }
   if (n) {
@@ -926,6 +954,8 @@ int natray_sim_set_bullets(natray_sim* s, uint32_t n, const natray_bullet* bulle
   HIPCHK(hipMemcpyAsync(const_cast<natray::Dev*>(D.self), &D, sizeof(natray::Dev), hipMemcpyHostToDevice, s->stream));
   if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
   if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+  if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
+  s->graph_all = nullptr;
   s->graph_pre = s->graph_post = nullptr;
   s->U.mem[3] = n;
   int e = upload_params(s);
