@@ -34,6 +34,7 @@ SCENE_GROUND, SCENE_KNOT, SCENE_WALLS, SCENE_CAPSULE, SCENE_MASK, SCENE_AIR, SCE
 SCENE_REFERENCE = SCENE_GROUND | SCENE_KNOT | SCENE_AIR | SCENE_SPRAY
 PRECOND_BLOCK_SGS, PRECOND_JACOBI = 0, 1
 CG_FP32, CG_MIXED, CG_PIPELINED = 0, 1, 2
+SOLVE_PLAIN, SOLVE_COOPERATIVE, SOLVE_PLAIN_TEST_CHECK = 0, 1, 2
 MAX_BULLETS, BULLET_PATCHES, BULLET_SECTIONS, BULLET_LOG = 4, 576, 32, 256
 
 OK, E_OVERFLOW, E_NOMEM, E_HIP, E_ARG, E_SOLVER = 0, -1, -2, -3, -4, -5
@@ -197,6 +198,7 @@ _SIGNATURES = {
     "natray_sim_counters": (C.c_int, [_P, _P]),
     "natray_sim_run_log": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(C.c_uint32)]),
     "natray_sim_water_stats": (C.c_int, [_P, _P]),
+    "natray_sim_set_solve_launch": (C.c_int, [_P, C.c_int]),
     "natray_sim_set_warm_start": (C.c_int, [_P, _P, C.c_int]),
     "natray_sim_set_membrane": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "natray_sim_pack_water": (C.c_int, [_P, _P]),

@@ -125,6 +125,9 @@ struct Dev {
   uint32_t capc;   // cap x 64 cells per field
   uint32_t fixed_ns;  // fixed step in ns, or 0
   uint32_t g2p_wgs;   // workgroups of the G2P launch (entries of vmax_part it writes)
+  // The single scene's solve: 0 a cooperative launch, 1 a plain one, 2 a plain one whose last
+  // workgroup stays out (the test of a workgroup that is not resident).
+  uint32_t solve_plain;
 };
 
 // How a kernel finds its scene. One: a single scene, its Dev passed by value. Many: a batch of

@@ -1755,6 +1755,11 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(L l) {
   using R = typename std::conditional<MIXED, double, float>::type;
   constexpr uint32_t RM = SOLVE_RMAX;
   if (d.st->dt_ns == 0u) return;
+  // A test of the plain launch's failure path: the last workgroup stays out, as one that is not
+  // resident would, and the others' bounded waits abort the solve.
+  if constexpr (std::is_same<L, One>::value) {
+    if (d.solve_plain == 2u && v.wg + 1u == v.wgs) return;
+  }
   // An earlier solve of this batch lost its barrier: the host reports it, nothing more runs.
   if (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
   const SolveSpan sp = solve_span(v, true);
@@ -2229,6 +2234,11 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_pipe(L l) {
   const Dev& d = v.d;
   constexpr uint32_t RM = SOLVE_RMAX;
   if (d.st->dt_ns == 0u) return;
+  // A test of the plain launch's failure path: the last workgroup stays out, as one that is not
+  // resident would, and the others' bounded waits abort the solve.
+  if constexpr (std::is_same<L, One>::value) {
+    if (d.solve_plain == 2u && v.wg + 1u == v.wgs) return;
+  }
   if (__hip_atomic_load(d.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
   const SolveSpan sp = solve_span(v, false);
   uint32_t tag = __hip_atomic_load(d.bar + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -4347,8 +4357,14 @@ int launch_solve(const Target& t, void* stream, int kind) {
     fn = t.d ? (const void*)k_solve<true, One> : (const void*)k_solve<true, Many>;
   else
     fn = t.d ? (const void*)k_solve<false, One> : (const void*)k_solve<false, Many>;
-  // A cooperative launch (every workgroup co-resident, or an error), through the launch
-  // attribute so that stream capture can record it as a graph node.
+  // A single scene's solve is a plain launch unless its Dev says otherwise. Nothing promises
+  // that a plain launch's workgroups run at the same time, but every wait of the solve is
+  // bounded: were one not resident, the others would abort and the step would fail with
+  // NATRAY_E_SOLVER rather than hang. A cooperative launch (every workgroup resident, or an
+  // error) goes through the launch attribute so that stream capture can record it as a graph
+  // node.
+  if (t.d && t.d->solve_plain)
+    return (int)hipLaunchKernel(fn, grid, block, t.d ? args_one : args_many, 0, S(stream));
   hipLaunchAttribute attr{};
   attr.id = hipLaunchAttributeCooperative;
   attr.val.cooperative = 1;

@@ -372,7 +372,11 @@ int sync_state(natray_sim* s) {
     int e = clear_tags(s);
     if (e) return e;
     HIPCHK(hipStreamSynchronize(s->stream));
-    return fail(NATRAY_E_SOLVER, "the pressure solve lost its grid barrier (not all workgroups were resident)");
+    return fail(NATRAY_E_SOLVER,
+                s->D.solve_plain ? "the pressure solve lost its grid barrier: not all of its workgroups were resident "
+                                   "(another program on the GPU?); natray_sim_set_solve_launch(sim, "
+                                   "NATRAY_SOLVE_COOPERATIVE) launches it cooperatively"
+                                 : "the pressure solve lost its grid barrier (not all workgroups were resident)");
   }
   return NATRAY_OK;
 }
@@ -538,6 +542,7 @@ int natray_sim_create(const natray_params* params, const float* particles, uint3
     t.grid[natray::LK_WAVES] = (uint32_t)s->grid_waves;
     t.grid[natray::LK_SOLVE] = (uint32_t)solve_grid_for(s);
     t.grid[natray::LK_VERTS] = 1;
+    D.solve_plain = 1u;  // natray_sim_set_solve_launch
   }
 
   HIPTRY(hipMemsetAsync(D.block_ptr, 0xff, (size_t)D.Spad * 4, s->stream));
@@ -808,6 +813,22 @@ int natray_sim_set_profiling(natray_sim* s, int on) {
     s->ms[k] = 0.0;
     s->cnt[k] = 0;
   }
+  return NATRAY_OK;
+}
+
+int natray_sim_set_solve_launch(natray_sim* s, int mode) {
+  if (!s) return fail(NATRAY_E_ARG, "null handle");
+  if (mode != NATRAY_SOLVE_PLAIN && mode != NATRAY_SOLVE_COOPERATIVE && mode != NATRAY_SOLVE_PLAIN_TEST_CHECK)
+    return fail(NATRAY_E_ARG, "unknown solve launch mode");
+  s->D.solve_plain = mode == NATRAY_SOLVE_COOPERATIVE ? 0u : (mode == NATRAY_SOLVE_PLAIN ? 1u : 2u);
+  // The graphs hold the Dev by value: capture them again.
+  HIPCHK(hipMemcpyAsync(const_cast<natray::Dev*>(s->D.self), &s->D, sizeof(natray::Dev), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
+  if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
+  if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
+  s->graph_all = nullptr;
+  s->graph_pre = s->graph_post = nullptr;
   return NATRAY_OK;
 }
 

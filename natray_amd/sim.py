@@ -168,6 +168,12 @@ class Sim:
         abi.check(abi.lib().natray_sim_run_log(self._h, int(cnt.value), abi.ptr(rows), C.byref(cnt)))
         return [dict(zip(self.RUN_LOG_KEYS, (int(v) for v in r[:7]))) for r in rows[: int(cnt.value)]]
 
+    def set_solve_launch(self, mode: int) -> None:
+        """How the solve is launched (natray_sim_set_solve_launch): abi.SOLVE_PLAIN (the
+        default, with a residency check), abi.SOLVE_COOPERATIVE, or abi.SOLVE_PLAIN_TEST_CHECK
+        (the check always fails)."""
+        abi.check(abi.lib().natray_sim_set_solve_launch(self._h, int(mode)))
+
     def water_stats(self) -> dict:
         """The water line of the reference's debug_stats (src/sim.rs:691-716) for the particles
         as they are now (natray_sim_water_stats): alive count, box, mean position and velocity,

@@ -69,3 +69,43 @@ def test_stop_time_truncates_then_steps_are_dead():
     assert_same_float("dead particles", sim.particles(), P)
     assert_same_float("dead particles oracle", orc.particles(), P)
     sim.close()
+
+
+@pytest.mark.parametrize("name", ["impact_sgs_32k", "impact_pipelined_32k"])
+def test_solve_launches_give_the_same_bits(name):
+    """The plain solve launch (the default, one graph per step) and the cooperative one, over
+    the same steps from the same state, with graphs and kernel by kernel."""
+    print(f"[failures] plain and cooperative solve launches, {name}", flush=True)
+    s = SCENES[name]()
+    out = []
+    for mode, graph in ((abi.SOLVE_PLAIN, True), (abi.SOLVE_COOPERATIVE, True), (abi.SOLVE_PLAIN, False)):
+        sim = Sim.from_scene(s)
+        sim.set_solve_launch(mode)
+        sim.set_graph(graph)
+        st = sim.run(6, iters=400)
+        out.append((sim.particles(), st))
+        sim.close()
+    for P, st in out[1:]:
+        compare_state("launch", st, out[0][1])
+        assert_same_float("launch particles", P, out[0][0])
+    assert out[0][1].solve_iters > 5
+
+
+def test_plain_solve_reports_workgroups_not_resident():
+    """A plain launch with a workgroup that is not resident, forced from the host: the last
+    workgroup stays out, the others' bounded waits for its chunk sums and halo words run out and
+    abort the solve, and the step reports NATRAY_E_SOLVER with a message naming the cooperative
+    launch. The handle then runs again in either mode."""
+    print("[failures] plain solve residency check", flush=True)
+    s = SCENES["impact_sgs_32k"]()
+    sim = Sim.from_scene(s)
+    sim.run(2, iters=400)
+    sim.set_solve_launch(abi.SOLVE_PLAIN_TEST_CHECK)
+    with pytest.raises(abi.NatrayError) as e:
+        sim.run(1, iters=400)
+    assert e.value.code == abi.E_SOLVER and "COOPERATIVE" in str(e.value)
+    for mode in (abi.SOLVE_COOPERATIVE, abi.SOLVE_PLAIN):
+        sim.set_solve_launch(mode)
+        st = sim.run(2, iters=400)
+        assert st.solve_iters > 5 and st.unconverged == 0
+    sim.close()
