@@ -86,6 +86,7 @@ struct natray_sim {
   bool use_graph = true;
   bool capturing = false;  // no timing events go into a captured graph
   hipGraphExec_t graph_pre = nullptr, graph_post = nullptr, graph_all = nullptr;
+  hipGraphExec_t graph_many = nullptr;  // kGraphSteps steps of graph_all's kernels
   bool skip_tags = false;  // the step's caller did the solve's tag bookkeeping (graph_all)
   // Learned warm start of the solve (natray_sim_set_warm_start).
   int ws_mode = 0;
@@ -119,7 +120,8 @@ void free_all(natray_sim* s) {
   if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
   if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
   if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
-  s->graph_all = nullptr;
+  if (s->graph_many) (void)hipGraphExecDestroy(s->graph_many);
+  s->graph_all = s->graph_many = nullptr;
   s->graph_pre = s->graph_post = nullptr;
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
   s->events.clear();
@@ -225,7 +227,8 @@ int alloc_warm_start(natray_sim* s) {
   if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
   if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
   if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
-  s->graph_all = nullptr;
+  if (s->graph_many) (void)hipGraphExecDestroy(s->graph_many);
+  s->graph_all = s->graph_many = nullptr;
   s->graph_pre = s->graph_post = nullptr;
   s->ws_ready = true;
   return NATRAY_OK;
@@ -289,12 +292,13 @@ int enqueue_phase(natray_sim* s, int phase) {
 }
 
 // Captures phases [lo, hi) as a graph on the handle's stream.
-int capture_phases(natray_sim* s, int lo, int hi, hipGraphExec_t* out) {
+int capture_phases(natray_sim* s, int lo, int hi, hipGraphExec_t* out, uint32_t reps = 1) {
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int e = NATRAY_OK;
   s->capturing = true;
-  for (int ph = lo; ph < hi && e == NATRAY_OK; ++ph) e = enqueue_phase(s, ph);
+  for (uint32_t r = 0; r < reps && e == NATRAY_OK; ++r)
+    for (int ph = lo; ph < hi && e == NATRAY_OK; ++ph) e = enqueue_phase(s, ph);
   s->capturing = false;
   hipError_t ec = hipStreamEndCapture(s->stream, &g);
   if (e) {
@@ -307,6 +311,20 @@ int capture_phases(natray_sim* s, int lo, int hi, hipGraphExec_t* out) {
   if (ec != hipSuccess) return fail(NATRAY_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
   return NATRAY_OK;
 }
+
+// The tags of `steps` solves, cleared first if a tag could repeat within them.
+int reserve_tags(natray_sim* s, uint32_t steps) {
+  const uint64_t bound = solve_tag_bound(s) * steps;
+  if (s->tags_used + bound >= kTagLimit) {
+    int e = clear_tags(s);
+    if (e) return e;
+  }
+  s->tags_used += bound;
+  return NATRAY_OK;
+}
+
+// Steps replay one graph of every phase (no events, no warm start).
+bool one_graph(const natray_sim* s) { return s->use_graph && !s->ws_mode && !s->prof; }
 
 // One step: one captured graph of every phase, its solve a cooperative kernel node; or, with
 // profiling or the warm start on, two graphs around the solve launched between them (its
@@ -321,13 +339,9 @@ int enqueue_step(natray_sim* s) {
     }
     return NATRAY_OK;
   }
-  if (!s->ws_mode && !s->prof) {
-    const uint64_t bound = solve_tag_bound(s);
-    if (s->tags_used + bound >= kTagLimit) {
-      int e = clear_tags(s);
-      if (e) return e;
-    }
-    s->tags_used += bound;
+  if (one_graph(s)) {
+    int e = reserve_tags(s, 1);
+    if (e) return e;
     if (!s->graph_all) {
       s->skip_tags = true;
       int e = capture_phases(s, NATRAY_PHASE_BEGIN, NATRAY_N_PHASES, &s->graph_all);
@@ -347,6 +361,30 @@ int enqueue_step(natray_sim* s) {
   int e = enqueue_phase(s, NATRAY_PHASE_SOLVE);
   if (e) return e;
   HIPCHK(hipGraphLaunch(s->graph_post, s->stream));
+  return NATRAY_OK;
+}
+
+// n steps: with the single-graph launch, runs of kGraphSteps steps replay one graph that holds
+// them all (no launch gap between its steps), the rest one step at a time.
+constexpr uint32_t kGraphSteps = 8;
+int enqueue_steps(natray_sim* s, uint32_t n) {
+  if (one_graph(s) && n >= kGraphSteps) {
+    if (!s->graph_many) {
+      s->skip_tags = true;
+      int e = capture_phases(s, NATRAY_PHASE_BEGIN, NATRAY_N_PHASES, &s->graph_many, kGraphSteps);
+      s->skip_tags = false;
+      if (e) return e;
+    }
+    for (; n >= kGraphSteps; n -= kGraphSteps) {
+      int e = reserve_tags(s, kGraphSteps);
+      if (e) return e;
+      HIPCHK(hipGraphLaunch(s->graph_many, s->stream));
+    }
+  }
+  for (; n > 0; --n) {
+    int e = enqueue_step(s);
+    if (e) return e;
+  }
   return NATRAY_OK;
 }
 
@@ -621,10 +659,8 @@ int natray_sim_run(natray_sim* s, uint32_t n_steps, uint32_t iters, uint32_t sto
   HIPCHK(hipMemcpyAsync(&s->D.st->iters_max, &s->h_words[4], 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
   const uint32_t step_before = s->last.step;
   auto t0 = std::chrono::steady_clock::now();
-  for (uint32_t k = 0; k < n_steps; ++k) {
-    e = enqueue_step(s);
-    if (e) return e;
-  }
+  e = enqueue_steps(s, n_steps);
+  if (e) return e;
   e = sync_state(s);
   if (e) return e;
   if (n_steps) s->step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n_steps;
@@ -827,7 +863,8 @@ int natray_sim_set_solve_launch(natray_sim* s, int mode) {
   if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
   if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
   if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
-  s->graph_all = nullptr;
+  if (s->graph_many) (void)hipGraphExecDestroy(s->graph_many);
+  s->graph_all = s->graph_many = nullptr;
   s->graph_pre = s->graph_post = nullptr;
   return NATRAY_OK;
 }
@@ -918,7 +955,8 @@ int natray_sim_set_membrane(natray_sim* s, uint32_t n, const float* x, const flo
     if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
     if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
     if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
-    s->graph_all = nullptr;
+    if (s->graph_many) (void)hipGraphExecDestroy(s->graph_many);
+    s->graph_all = s->graph_many = nullptr;
     s->graph_pre = s->graph_post = nullptr;
   }
   if (n) {
@@ -976,7 +1014,8 @@ int natray_sim_set_bullets(natray_sim* s, uint32_t n, const natray_bullet* bulle
   if (s->graph_pre) (void)hipGraphExecDestroy(s->graph_pre);
   if (s->graph_post) (void)hipGraphExecDestroy(s->graph_post);
   if (s->graph_all) (void)hipGraphExecDestroy(s->graph_all);
-  s->graph_all = nullptr;
+  if (s->graph_many) (void)hipGraphExecDestroy(s->graph_many);
+  s->graph_all = s->graph_many = nullptr;
   s->graph_pre = s->graph_post = nullptr;
   s->U.mem[3] = n;
   int e = upload_params(s);

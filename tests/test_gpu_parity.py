@@ -172,17 +172,17 @@ def test_repeat_runs_identical():
 
 @pytest.mark.parametrize("name", ["impact_sgs_32k", "impact_mixed_32k"])
 def test_graph_replays_match_kernel_by_kernel(name):
-    """natray_sim_run replays two captured HIP graphs around the cooperative solve; every
-    replay gives the bits of the same steps launched kernel by kernel, with other work and
-    readbacks between the replays."""
+    """natray_sim_run replays captured HIP graphs: one of 8 steps for each 8 of a run, one of a
+    step for the rest. Every replay gives the bits of the same steps launched kernel by kernel,
+    with other work and readbacks between the replays."""
     s = SCENES[name]()
     say(f"[parity] graph replays vs eager launches: {name}")
     g = Sim.from_scene(s)
     e = Sim.from_scene(s)
     e.set_graph(False)
-    for k in range(4):
-        a = g.run(3 + k, iters=400)
-        b = e.run(3 + k, iters=400)
+    for k, n in enumerate((3, 8, 11, 17)):
+        a = g.run(n, iters=400)
+        b = e.run(n, iters=400)
         compare_state(f"{name} run {k}", a, b)
         assert_same_float(f"{name} run {k} particles", g.particles(), e.particles())
         gg, ge = g.grid(), e.grid()
