@@ -5,8 +5,8 @@ chunk fold runs with one block per chunk and each workgroup holds at most one bl
 scenes are the workloads themselves:
 
 - the 2,097,152-particle balloon impact on 256^3 (~5,630 blocks): chunks of K = 22 blocks, 1-2
-  blocks per wave in the register form, and the memory form with 64 solve workgroups (88
-  blocks each);
+  blocks per wave in the register form, the memory form's streaming passes with 64 solve
+  workgroups (88 blocks each), and its per-block passes with 8 (~700 each);
 - a 262,144-particle impact on 128^3 (~700 blocks, K = 3);
 - the same 2M impact with the mixed-precision and the pipelined CG, in both forms;
 - the 64-scene batch of balloon impacts (16k-512k particles on 64^3-192^3), two steps against
@@ -43,6 +43,10 @@ AT_SIZE = {
     "impact_2M_mixed_mem": lambda: solve_groups(scenes.balloon_impact(precision=abi.CG_MIXED), 64),
     "impact_2M_pipelined": lambda: scenes.balloon_impact(precision=abi.CG_PIPELINED),
     "impact_2M_pipelined_mem": lambda: solve_groups(scenes.balloon_impact(precision=abi.CG_PIPELINED), 64),
+    # 8 workgroups of ~700 blocks: the memory form's per-block passes (beyond the streaming
+    # passes' 512 blocks per workgroup), with its fenced reductions, in FP32 and mixed.
+    "impact_2M_mem8": lambda: solve_groups(scenes.balloon_impact(), 8),
+    "impact_2M_mixed_mem8": lambda: solve_groups(scenes.balloon_impact(precision=abi.CG_MIXED), 8),
     # The stated geometry of the water sphere: radius 0.25 on 64^3 at 8 particles per cell.
     "sphere_stated_r025_138k": lambda: scenes.stated_radius_sphere(),
 }
