@@ -1358,10 +1358,15 @@ __device__ __forceinline__ void st_cell(float* f, uint32_t i, float v) {
 // the pass that made it, the new direction in hz[capc (1 + k % 2) + cell] with the tag of
 // iteration k's first reduction, and before a residual refresh p in the other direction buffer
 // (whose last words every neighbour has read) with the tag of the refresh's reduction. A reader
-// waits for the tag it expects, so the reductions need no agent-scope fences. Each wave keeps its blocks' neighbour masks and their six
-// face-neighbour blocks (-x, +x, -y, +y, -z, +z) in LDS, issues the loads of SOLVE_GROUP blocks before it uses any, and takes a cell's neighbours
-// inside the block from the wave's lanes. The arithmetic is that of the per-block passes,
-// expression for expression.
+// waits for the tag it expects, so the reductions need no agent-scope fences. Each wave keeps
+// its blocks' neighbour masks and their six face-neighbour blocks (-x, +x, -y, +y, -z, +z) in
+// LDS, walks its blocks in a rolled loop, issues the loads of SOLVE_GROUP blocks before it uses
+// any, and takes a cell's neighbours inside the block from the wave's lanes. The arithmetic is
+// that of the per-block passes, expression for expression.
+//
+// Per-block passes (larger workgroups): for_blocks, the workgroup's waves in lock step, with
+// agent-scope release and acquire fences around each reduction (sp.coherent), since the
+// neighbours' fields are read from memory.
 template <bool MIXED>
 __device__ __noinline__ void solve_memory(const Dev* dp, SolveSpan sp, uint32_t tag) {
   using R = typename std::conditional<MIXED, double, float>::type;
