@@ -320,7 +320,7 @@ int natray_sim_set_graph(natray_sim* sim, int on);
  * NATRAY_SOLVE_COOPERATIVE: a cooperative launch (all resident, or the launch fails); it costs
  * about 27 us more per step on the 2M impact. NATRAY_SOLVE_PLAIN_TEST_CHECK: the plain launch
  * with its last workgroup staying out, so the solve aborts (for tests of that path). Both
- * launches give the same bits. Batches always launch cooperatively. */
+ * launches give the same bits. A batch's solve: natray_batch_set_solve_launch. */
 #define NATRAY_SOLVE_PLAIN 0
 #define NATRAY_SOLVE_COOPERATIVE 1
 #define NATRAY_SOLVE_PLAIN_TEST_CHECK 2
@@ -545,8 +545,14 @@ void natray_batch_destroy(natray_batch* batch);
  * scene has stepped, the solve's workgroups are shared by each scene's blocks times its last
  * CG iteration count rather than by particles. */
 int natray_batch_run(natray_batch* batch, uint32_t n_steps, uint32_t iters);
-/* Steps as two replayed graphs around the solve (on, the default) or kernel by kernel. */
+/* Steps as replayed graphs (on, the default): one graph of every phase of a step, and one of 8
+ * steps for each run of 8 in natray_batch_run, captured again when the solve's share of
+ * workgroups changes; or kernel by kernel. */
 int natray_batch_set_graph(natray_batch* batch, int on);
+/* How a batch's solve is launched (no counterpart in the reference): NATRAY_SOLVE_PLAIN (the
+ * default), a node of the step graphs whose every wait is bounded, as for a single scene; or
+ * NATRAY_SOLVE_COOPERATIVE. Both give the same bits. */
+int natray_batch_set_solve_launch(natray_batch* batch, int mode);
 int natray_batch_set_shape(natray_batch* batch, const natray_batch_shape* shape);
 /* Per-phase GPU time of the batch's kernel-by-kernel steps (graphs off), summed over the runs
  * since profiling was switched on, as natray_sim_phase_times. */

@@ -218,6 +218,7 @@ _SIGNATURES = {
     "natray_batch_destroy": (None, [_P]),
     "natray_batch_run": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "natray_batch_set_graph": (C.c_int, [_P, C.c_int]),
+    "natray_batch_set_solve_launch": (C.c_int, [_P, C.c_int]),
     "natray_batch_set_shape": (C.c_int, [_P, _P]),
     "natray_batch_set_profiling": (C.c_int, [_P, C.c_int]),
     "natray_batch_phase_times": (C.c_int, [_P, _P, _P]),

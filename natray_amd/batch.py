@@ -56,6 +56,10 @@ class Batch:
     def set_graph(self, on: bool) -> None:
         abi.check(abi.lib().natray_batch_set_graph(self._h, 1 if on else 0))
 
+    def set_solve_launch(self, mode: int) -> None:
+        """abi.SOLVE_PLAIN (the default) or abi.SOLVE_COOPERATIVE (natray_batch_set_solve_launch)."""
+        abi.check(abi.lib().natray_batch_set_solve_launch(self._h, int(mode)))
+
     def set_profiling(self, on: bool) -> None:
         abi.check(abi.lib().natray_batch_set_profiling(self._h, 1 if on else 0))
 

@@ -157,6 +157,7 @@ struct Target {
   const Dev* scenes = nullptr;
   const WgMap* map[LK_KINDS] = {};
   uint32_t grid[LK_KINDS] = {};
+  bool plain_solve = false;  // a batch's solve as a plain launch (a single scene: Dev::solve_plain)
 };
 
 }  // namespace natray

@@ -4362,13 +4362,14 @@ int launch_solve(const Target& t, void* stream, int kind) {
     fn = t.d ? (const void*)k_solve<true, One> : (const void*)k_solve<true, Many>;
   else
     fn = t.d ? (const void*)k_solve<false, One> : (const void*)k_solve<false, Many>;
-  // A single scene's solve is a plain launch unless its Dev says otherwise. Nothing promises
+  // The solve is a plain launch unless its Dev (a single scene) or the Target (a batch) says
+  // otherwise. Nothing promises
   // that a plain launch's workgroups run at the same time, but every wait of the solve is
   // bounded: were one not resident, the others would abort and the step would fail with
   // NATRAY_E_SOLVER rather than hang. A cooperative launch (every workgroup resident, or an
   // error) goes through the launch attribute so that stream capture can record it as a graph
   // node.
-  if (t.d && t.d->solve_plain)
+  if (t.d ? t.d->solve_plain != 0u : t.plain_solve)
     return (int)hipLaunchKernel(fn, grid, block, t.d ? args_one : args_many, 0, S(stream));
   hipLaunchAttribute attr{};
   attr.id = hipLaunchAttributeCooperative;

@@ -1315,7 +1315,10 @@ struct natray_batch {
   std::vector<uint32_t> counts[natray::LK_KINDS];  // workgroups per scene, per kind
   natray::Target tgt;
   bool use_graph = true;
-  hipGraphExec_t graph_pre = nullptr, graph_post = nullptr;
+  // The step's graph (every phase) and one of kGraphSteps steps; the host does the solve's tag
+  // bookkeeping before each replay (skip_tags while capturing).
+  hipGraphExec_t graph_all = nullptr, graph_many = nullptr;
+  bool skip_tags = false;
   std::vector<void*> allocs;
   // Per-phase timing (natray_batch_phase_times): events around each phase of kernel-by-kernel
   // steps, summed when the run ends.
@@ -1329,13 +1332,17 @@ struct natray_batch {
 
 namespace {
 
+void batch_drop_graphs(natray_batch* b) {
+  if (b->graph_all) (void)hipGraphExecDestroy(b->graph_all);
+  if (b->graph_many) (void)hipGraphExecDestroy(b->graph_many);
+  b->graph_all = b->graph_many = nullptr;
+}
+
 void batch_free(natray_batch* b) {
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   b->events.clear();
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->graph_pre) (void)hipGraphExecDestroy(b->graph_pre);
-  if (b->graph_post) (void)hipGraphExecDestroy(b->graph_post);
-  b->graph_pre = b->graph_post = nullptr;
+  batch_drop_graphs(b);
   for (void* p : b->allocs) (void)hipFree(p);
   b->allocs.clear();
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1445,9 +1452,7 @@ int batch_layout(natray_batch* b) {
   b->tgt.d = nullptr;
   b->tgt.scenes = b->scenes;
   HIPCHK(hipStreamSynchronize(b->stream));
-  if (b->graph_pre) (void)hipGraphExecDestroy(b->graph_pre);
-  if (b->graph_post) (void)hipGraphExecDestroy(b->graph_post);
-  b->graph_pre = b->graph_post = nullptr;
+  batch_drop_graphs(b);
   return NATRAY_OK;
 }
 
@@ -1501,6 +1506,7 @@ int batch_plan_solve(natray_batch* b) {
   HIPCHK(hipStreamSynchronize(b->stream));
   b->counts[natray::LK_SOLVE] = shares;
   b->tgt.grid[natray::LK_SOLVE] = (uint32_t)map.size();
+  batch_drop_graphs(b);  // they hold the solve's grid
   return NATRAY_OK;
 }
 
@@ -1515,6 +1521,21 @@ int batch_phase(natray_batch* b, int phase) {
   int e = batch_phase_launch(b, phase);
   if (e) return e;
   if (timed) HIPCHK(hipEventRecord(b->events[2 * slot + 1], b->stream));
+  return NATRAY_OK;
+}
+
+// Every scene's tags for `steps` solves, its tagged words cleared first if a tag could repeat.
+int batch_reserve_tags(natray_batch* b, uint32_t steps) {
+  for (natray_sim* s : b->sims) {
+    const uint64_t bound = solve_tag_bound(s) * steps;
+    if (s->tags_used + bound >= kTagLimit) {
+      HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, b->stream));
+      HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 24, b->stream));
+      HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, b->stream));
+      s->tags_used = 0;
+    }
+    s->tags_used += bound;
+  }
   return NATRAY_OK;
 }
 
@@ -1533,15 +1554,9 @@ int batch_phase_launch(natray_batch* b, int phase) {
       if (b->air) LAUNCHCHK(natray::launch_air_fill(t, st));
       break;
     case NATRAY_PHASE_SOLVE: {
-      for (natray_sim* s : b->sims) {
-        const uint64_t bound = solve_tag_bound(s);
-        if (s->tags_used + bound >= kTagLimit) {
-          HIPCHK(hipMemsetAsync(s->D.rwords, 0, natray::RED_WORDS * 8, b->stream));
-          HIPCHK(hipMemsetAsync(s->D.hz, 0, (size_t)s->D.capc * 24, b->stream));
-          HIPCHK(hipMemsetAsync(s->D.bar + 4, 0, 4, b->stream));
-          s->tags_used = 0;
-        }
-        s->tags_used += bound;
+      if (!b->skip_tags) {
+        int e = batch_reserve_tags(b, 1);
+        if (e) return e;
       }
       LAUNCHCHK(natray::launch_solve(t, st, b->mixed));
       break;
@@ -1554,11 +1569,14 @@ int batch_phase_launch(natray_batch* b, int phase) {
   return NATRAY_OK;
 }
 
-int batch_capture(natray_batch* b, int lo, int hi, hipGraphExec_t* out) {
+int batch_capture(natray_batch* b, hipGraphExec_t* out, uint32_t steps) {
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int e = NATRAY_OK;
-  for (int ph = lo; ph < hi && e == NATRAY_OK; ++ph) e = batch_phase(b, ph);
+  b->skip_tags = true;
+  for (uint32_t r = 0; r < steps && e == NATRAY_OK; ++r)
+    for (int ph = 0; ph < NATRAY_N_PHASES && e == NATRAY_OK; ++ph) e = batch_phase(b, ph);
+  b->skip_tags = false;
   hipError_t ec = hipStreamEndCapture(b->stream, &g);
   if (e) {
     if (g) (void)hipGraphDestroy(g);
@@ -1571,12 +1589,10 @@ int batch_capture(natray_batch* b, int lo, int hi, hipGraphExec_t* out) {
   return NATRAY_OK;
 }
 
-// Captures the step graphs of the current shape if they are missing (recording runs nothing).
+// Captures the step graph of the current shape if it is missing (recording runs nothing).
 int batch_graphs(natray_batch* b) {
-  if (!b->use_graph || b->graph_pre) return NATRAY_OK;
-  int e = batch_capture(b, NATRAY_PHASE_BEGIN, NATRAY_PHASE_SOLVE, &b->graph_pre);
-  if (e) return e;
-  return batch_capture(b, NATRAY_PHASE_PROJECT, NATRAY_N_PHASES, &b->graph_post);
+  if (!b->use_graph || b->graph_all) return NATRAY_OK;
+  return batch_capture(b, &b->graph_all, 1);
 }
 
 int batch_step(natray_batch* b) {
@@ -1589,10 +1605,30 @@ int batch_step(natray_batch* b) {
   }
   int e = batch_graphs(b);
   if (e) return e;
-  HIPCHK(hipGraphLaunch(b->graph_pre, b->stream));
-  e = batch_phase(b, NATRAY_PHASE_SOLVE);
+  e = batch_reserve_tags(b, 1);
   if (e) return e;
-  HIPCHK(hipGraphLaunch(b->graph_post, b->stream));
+  HIPCHK(hipGraphLaunch(b->graph_all, b->stream));
+  return NATRAY_OK;
+}
+
+// n steps: with graphs, runs of kGraphSteps steps replay one graph of them all, the rest one
+// step at a time.
+int batch_steps(natray_batch* b, uint32_t n) {
+  if (b->use_graph && n >= kGraphSteps) {
+    if (!b->graph_many) {
+      int e = batch_capture(b, &b->graph_many, kGraphSteps);
+      if (e) return e;
+    }
+    for (; n >= kGraphSteps; n -= kGraphSteps) {
+      int e = batch_reserve_tags(b, kGraphSteps);
+      if (e) return e;
+      HIPCHK(hipGraphLaunch(b->graph_many, b->stream));
+    }
+  }
+  for (; n > 0; --n) {
+    int e = batch_step(b);
+    if (e) return e;
+  }
   return NATRAY_OK;
 }
 
@@ -1612,10 +1648,8 @@ int batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters, hipEvent_t t0 =
     HIPCHK(hipMemcpyAsync(&s->D.st->iters_max, &s->h_words[4], 2 * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
   }
   if (t0) HIPCHK(hipEventRecord(t0, b->stream));
-  for (uint32_t k = 0; k < n_steps; ++k) {
-    int e = batch_step(b);
-    if (e) return e;
-  }
+  int e = batch_steps(b, n_steps);
+  if (e) return e;
   if (t1) HIPCHK(hipEventRecord(t1, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   for (size_t k = 0; k < b->ev_used; ++k) {
@@ -1669,6 +1703,7 @@ int natray_batch_create(natray_sim* const* sims, uint32_t n, natray_batch** out)
   b->mixed = sims[0]->mixed;
   for (uint32_t k = 0; k < n; ++k) b->air = b->air || sims[k]->D.air_x != nullptr;
   b->solve_budget = sims[0]->solve_grid;
+  b->tgt.plain_solve = true;  // natray_batch_set_solve_launch
   b->shape = {8u, 8u, 4u, 0u};
   auto bail = [&](int code) {
     batch_free(b);
@@ -1703,6 +1738,16 @@ int natray_batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters) {
   if (!b) return fail(NATRAY_E_ARG, "null handle");
   for (natray_sim* s : b->sims) HIPCHK(hipStreamSynchronize(s->stream));
   return batch_run(b, n_steps, iters);
+}
+
+int natray_batch_set_solve_launch(natray_batch* b, int mode) {
+  if (!b) return fail(NATRAY_E_ARG, "null handle");
+  if (mode != NATRAY_SOLVE_PLAIN && mode != NATRAY_SOLVE_COOPERATIVE)
+    return fail(NATRAY_E_ARG, "a batch's solve launch is NATRAY_SOLVE_PLAIN or NATRAY_SOLVE_COOPERATIVE");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->tgt.plain_solve = mode == NATRAY_SOLVE_PLAIN;
+  batch_drop_graphs(b);
+  return NATRAY_OK;
 }
 
 int natray_batch_set_graph(natray_batch* b, int on) {

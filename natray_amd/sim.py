@@ -170,8 +170,9 @@ class Sim:
 
     def set_solve_launch(self, mode: int) -> None:
         """How the solve is launched (natray_sim_set_solve_launch): abi.SOLVE_PLAIN (the
-        default, with a residency check), abi.SOLVE_COOPERATIVE, or abi.SOLVE_PLAIN_TEST_CHECK
-        (the check always fails)."""
+        default; its waits are bounded, so a workgroup that never becomes resident fails the
+        step), abi.SOLVE_COOPERATIVE, or abi.SOLVE_PLAIN_TEST_CHECK (the plain launch with its
+        last workgroup staying out, so the solve aborts)."""
         abi.check(abi.lib().natray_sim_set_solve_launch(self._h, int(mode)))
 
     def water_stats(self) -> dict:

@@ -2,7 +2,9 @@
 stepped alone (which tests/test_gpu_parity.py holds bit-exact against the oracle), with the
 steps replayed as graphs or launched kernel by kernel, with the solve in register form (few
 scenes) and in memory form (many scenes sharing the solve's workgroups), for both CG
-precisions and the pipelined CG; and the autotuner's choice leaves the results unchanged."""
+precisions and the pipelined CG, with the solve launched plain or cooperatively, over step
+counts that use the 8-step graph and the one-step graph; and the autotuner's choice leaves the
+results unchanged."""
 import sys
 
 import numpy as np
@@ -45,10 +47,11 @@ def alone(scene_list, steps):
     return out
 
 
-def batched(scene_list, steps, graph=True, tune=False):
+def batched(scene_list, steps, graph=True, tune=False, launch=abi.SOLVE_PLAIN):
     sims = [Sim.from_scene(s) for s in scene_list]
     b = Batch(sims)
     b.set_graph(graph)
+    b.set_solve_launch(launch)
     chosen = None
     if tune:
         chosen = b.tune(1, iters=400)
@@ -76,9 +79,18 @@ def check_same(tag, got, want, keys=STATE_KEYS):
 def test_batch_matches_scenes_alone(precision, graph):
     say(f"[batch] 4 scenes, precision {precision}, graph {graph}")
     sl = small_set(precision)
-    want = alone(sl, 6)
-    got, _ = batched(sl, 6, graph=graph)
+    want = alone(sl, 11)
+    got, _ = batched(sl, 11, graph=graph)
     check_same(f"precision {precision} graph {graph}", got, want)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_cooperative_solve_launch(graph):
+    say(f"[batch] 4 scenes, cooperative solve launch, graph {graph}")
+    sl = small_set(abi.CG_MIXED)
+    want = alone(sl, 9)
+    got, _ = batched(sl, 9, graph=graph, launch=abi.SOLVE_COOPERATIVE)
+    check_same(f"cooperative graph {graph}", got, want)
 
 
 @pytest.mark.parametrize("precision", [abi.CG_MIXED, abi.CG_PIPELINED])
