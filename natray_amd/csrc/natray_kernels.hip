@@ -89,8 +89,6 @@ __device__ __forceinline__ void cell_coord(const Dev& d, uint32_t cell, int& x, 
 __device__ __forceinline__ int32_t to_fp(float x, float scale) {
   return (int32_t)__builtin_rintf(__builtin_amdgcn_fmed3f(x * scale, -2e9f, 2e9f));
 }
-// to_fp(w, FP_W) of a B-spline weight product: w lies in [0, 1], so the clamp cannot bind.
-__device__ __forceinline__ int32_t to_fp_weight(float w) { return (int32_t)__builtin_rintf(w * NATRAY_FP_W); }
 __device__ __forceinline__ float from_fp(int32_t x, float scale) { return (float)x / scale; }
 
 __device__ __forceinline__ float& cfr(const Dev& d, uint32_t f, uint32_t cell) { return d.cf[(size_t)f * d.capc + cell]; }
@@ -543,14 +541,34 @@ __device__ __forceinline__ uint32_t tile_cell(const uint32_t* links, uint32_t t)
 }
 
 // P2G's LDS tile: the 7^3 nodes a block's particles reach, x-major with padded strides, so
-// that the atomics of a half-wave's stencils spread over more LDS banks.
+// that the atomics of a half-wave's stencils spread over more LDS banks. Each lattice's node
+// holds a pair of 32-bit sums, the weight's first and then the momentum's (or the pressure's).
 constexpr uint32_t P2G_SX = 60, P2G_SY = 8;
 constexpr uint32_t P2G_TILE = 6u * P2G_SX + 6u * P2G_SY + 7u;
+// The largest weight one particle adds to a node is rint(0.75^3 * FP_W) = 7077888, so the
+// weight sums of a block of at most 606 particles stay below 2^32. Such a block adds each
+// node's pair with one 64-bit atomic: the weight's sum in the low word never carries into the
+// momentum's in the high word, which wraps like the 32-bit atomic it replaces.
+constexpr uint32_t P2G_PACK_MAX = 606;
+static_assert(NATRAY_A_MOM(1) == 2u && NATRAY_A_WT(1) == 3u && NATRAY_A_PC == NATRAY_A_WC + 1u,
+              "P2G's flush maps each lattice's pair onto these fields");
 
 // splat (fluid.wgsl:233-252) of one lattice (offset o = (OX, OY, OZ) / 2 cells) into the tile.
-template <int OX, int OY, int OZ>
-__device__ __forceinline__ void splat(int* tile, const float x[3], const int corner[3], uint32_t fa, float val,
-                                      const float grad[3], uint32_t fw, float scale, float h) {
+// The reference adds to_fp(weight * node, scale) and to_fp(weight, FP_W) per node. Here the
+// powers of two move onto the factors: w24 = (wx * wy) * (wz * FP_W) and node' = node *
+// (scale / FP_W), built from val and the gradient terms each scaled by scale / FP_W = 2^-8.
+// Scaling by a power of two commutes with rounding in the normal range, so w24 is
+// weight * FP_W and w24 * node' rounds to RN(weight * node) * scale, bit for bit. Where a
+// subnormal occurs instead (a weight below 2^-126, a scaled term below 2^-126), both products
+// are far below 0.5 and convert to 0: such a term changes a sum only while the sum stays within
+// 2^25 of it, so after node's three additions it reaches at most 2^-51 and w24 * node' < 2^-27
+// (and |node| < 2^100 bounds the other side). This saves two multiplications per node. A
+// lattice without gradient (GRAD false: the pressure's, grad = 0) has node = val + (+-0) = val
+// for every nonzero val, and +-0 converts to 0 like val = 0, so it splats val alone.
+template <int OX, int OY, int OZ, bool GRAD, bool PACK>
+__device__ __forceinline__ void splat(int* tile, const float x[3], const int corner[3], uint32_t lat, float val,
+                                      const float grad[3], float scale, float h) {
+  const float down = scale / NATRAY_FP_W;
   const float o[3] = {0.5f * OX, 0.5f * OY, 0.5f * OZ};
   int base[3];
   float fx[3], hg[3];
@@ -558,26 +576,45 @@ __device__ __forceinline__ void splat(int* tile, const float x[3], const int cor
   for (int k = 0; k < 3; ++k) {
     base[k] = (int)floorf(x[k] - o[k] - 0.5f);
     fx[k] = x[k] - o[k] - (float)base[k];
-    hg[k] = grad[k] * h;
+    hg[k] = (grad[k] * h) * down;
   }
   float wx[3], wy[3], wz[3];
   axis_weights(fx[0], wx);
   axis_weights(fx[1], wy);
   axis_weights(fx[2], wz);
-  int t0 = (base[0] - corner[0]) * (int)P2G_SX + (base[1] - corner[1]) * (int)P2G_SY + (base[2] - corner[2]);
-  int* ta = tile + fa * P2G_TILE + t0;
-  int* tw = tile + fw * P2G_TILE + t0;
+  float wz24[3], gx[3], gy[3], gz[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    wz24[k] = wz[k] * NATRAY_FP_W;
+    gx[k] = hg[0] * ((float)k - fx[0]);
+    gy[k] = hg[1] * ((float)k - fx[1]);
+    gz[k] = hg[2] * ((float)k - fx[2]);
+  }
+  const float val8 = val * down;
+  const int t0 = (base[0] - corner[0]) * (int)P2G_SX + (base[1] - corner[1]) * (int)P2G_SY + (base[2] - corner[2]);
+  int* tp = tile + 2 * ((int)(lat * P2G_TILE) + t0);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
+    for (int b = 0; b < 3; ++b) {
+      const float wxy = wx[a] * wy[b];
+      const float gxy = gx[a] + gy[b];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        float weight = wx[a] * wy[b] * wz[c];
-        float node = val + ((hg[0] * ((float)a - fx[0]) + hg[1] * ((float)b - fx[1])) + hg[2] * ((float)c - fx[2]));
-        atomicAdd(ta + a * (int)P2G_SX + b * (int)P2G_SY + c, to_fp(weight * node, scale));
-        atomicAdd(tw + a * (int)P2G_SX + b * (int)P2G_SY + c, to_fp_weight(weight));
+        const float w24 = wxy * wz24[c];
+        const float node = GRAD ? val8 + (gxy + gz[c]) : val8;
+        const int32_t wq = (int32_t)__builtin_rintf(w24);
+        const int32_t mq = (int32_t)__builtin_rintf(__builtin_amdgcn_fmed3f(w24 * node, -2e9f, 2e9f));
+        int* q = tp + 2 * (a * (int)P2G_SX + b * (int)P2G_SY + c);
+        if (PACK) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(q),
+                    (unsigned long long)(uint32_t)mq << 32 | (unsigned long long)(uint32_t)wq);
+        } else {
+          atomicAdd(q, wq);
+          atomicAdd(q + 1, mq);
+        }
       }
+    }
 }
 
 // Neighbour links of block b (fluid.wgsl:261-270), also stored for the later kernels.
@@ -599,6 +636,7 @@ __device__ __forceinline__ void block_links(const Dev& d, uint32_t b, uint32_t l
 // tile: the four lattices of fluid.wgsl:286-289. Returns false (and adds nothing) for a
 // particle that is not in that cell, which only a position written behind the library's back
 // can cause.
+template <bool PACK>
 __device__ __forceinline__ bool splat_particle(const Dev& d, int* tile, uint32_t i, uint32_t l, int bx, int by, int bz,
                                                const int corner[3], float h) {
   const natray_params* U = d.U;
@@ -611,10 +649,10 @@ __device__ __forceinline__ bool splat_particle(const Dev& d, int* tile, uint32_t
     return false;
   const float g0[3] = {c0.x, c0.y, c0.z}, g1[3] = {c1.x, c1.y, c1.z}, g2[3] = {c2.x, c2.y, c2.z};
   const float zero[3] = {0.0f, 0.0f, 0.0f};
-  splat<0, 1, 1>(tile, x, corner, NATRAY_A_MOM(0), vel.x, g0, NATRAY_A_WT(0), NATRAY_FP_M, h);
-  splat<1, 0, 1>(tile, x, corner, NATRAY_A_MOM(1), vel.y, g1, NATRAY_A_WT(1), NATRAY_FP_M, h);
-  splat<1, 1, 0>(tile, x, corner, NATRAY_A_MOM(2), vel.z, g2, NATRAY_A_WT(2), NATRAY_FP_M, h);
-  splat<1, 1, 1>(tile, x, corner, NATRAY_A_PC, c0.w * 1e-6f, zero, NATRAY_A_WC, NATRAY_FP_P, h);
+  splat<0, 1, 1, true, PACK>(tile, x, corner, 0u, vel.x, g0, NATRAY_FP_M, h);
+  splat<1, 0, 1, true, PACK>(tile, x, corner, 1u, vel.y, g1, NATRAY_FP_M, h);
+  splat<1, 1, 0, true, PACK>(tile, x, corner, 2u, vel.z, g2, NATRAY_FP_M, h);
+  splat<1, 1, 1, false, PACK>(tile, x, corner, 3u, c0.w * 1e-6f, zero, NATRAY_FP_P, h);
   return true;
 }
 
@@ -641,7 +679,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const View v = view(l);
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
-  __shared__ int tile[8 * P2G_TILE];
+  __shared__ __align__(8) int tile[8 * P2G_TILE];
   __shared__ uint32_t cnt[64], drop[64];
   __shared__ uint32_t order[P2G_MAXP];
   __shared__ uint8_t ocell[P2G_MAXP];
@@ -719,23 +757,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
         __syncthreads();
         uint32_t placed = 0;
         for (uint32_t r = 0; r < maxc; ++r) placed += (uint32_t)__popcll(rmask[r]);
-        for (uint32_t k = tid; k < placed; k += 256) {
-          const uint32_t l = ocell[k];
-          if (!splat_particle(d, tile, order[k], l, bx, by, bz, corner, h)) atomicAdd(&drop[l], 1u);
+        if (count <= P2G_PACK_MAX) {
+          for (uint32_t k = tid; k < placed; k += 256) {
+            const uint32_t l = ocell[k];
+            if (!splat_particle<true>(d, tile, order[k], l, bx, by, bz, corner, h)) atomicAdd(&drop[l], 1u);
+          }
+        } else {
+          for (uint32_t k = tid; k < placed; k += 256) {
+            const uint32_t l = ocell[k];
+            if (!splat_particle<false>(d, tile, order[k], l, bx, by, bz, corner, h)) atomicAdd(&drop[l], 1u);
+          }
         }
       } else {
         // A cell with more than P2G_MAXR particles: splat in stored order (counted above).
         for (uint32_t k = tid; k < count; k += 256) {
           const uint32_t i = d.sorted[first + k];
           const uint32_t l = key_local(d.key[i], bx, by, bz);
-          if (l < 64u && !splat_particle(d, tile, i, l, bx, by, bz, corner, h)) atomicAdd(&drop[l], 1u);
+          if (l < 64u && !splat_particle<false>(d, tile, i, l, bx, by, bz, corner, h)) atomicAdd(&drop[l], 1u);
         }
       }
     } else {
       for (uint32_t k = tid; k < count; k += 256) {
         const uint32_t i = d.sorted[first + k];
         const uint32_t l = key_local(d.key[i], bx, by, bz);
-        if (l < 64u && splat_particle(d, tile, i, l, bx, by, bz, corner, h)) atomicAdd(&cnt[l], 1u);
+        if (l < 64u && splat_particle<false>(d, tile, i, l, bx, by, bz, corner, h)) atomicAdd(&cnt[l], 1u);
       }
     }
     __syncthreads();
@@ -744,10 +789,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
       if (cell == NONE) continue;
       const uint32_t pt = t / 49u * P2G_SX + t / 7u % 7u * P2G_SY + t % 7u;
 #pragma unroll
-      for (uint32_t f = 0; f < 8u; ++f) {
-        int v = tile[f * P2G_TILE + pt];
-        if (v != 0) atomicAdd(&accr(d, f, cell), v);
-      }
+      for (uint32_t lat = 0; lat < 4u; ++lat)
+#pragma unroll
+        for (uint32_t j = 0; j < 2u; ++j) {
+          // j = 0: the weight (A_WT(lat), or A_WC); j = 1: the momentum (A_MOM(lat)), or A_PC.
+          const uint32_t f = lat < 3u ? 2u * lat + 1u - j : NATRAY_A_WC + j;
+          const int v = tile[2u * (lat * P2G_TILE + pt) + j];
+          if (v != 0) atomicAdd(&accr(d, f, cell), v);
+        }
     }
     if (tid < 64) accr(d, NATRAY_A_N, b * 64u + tid) = (int32_t)(cnt[tid] - drop[tid]);
   }
