@@ -49,10 +49,10 @@ struct Dev {
   uint32_t* tile_sum;      // 2 x ntiles: scan tile totals (slots claimed, particles)
   uint32_t* tile_off;      // 2 x ntiles: their exclusive prefix
   uint32_t* bar;           // control words: [2] the solve's abort flag a stuck wait raises,
-                           // [4] its last reduction tag, [8] / [9] the next block P2G / G2P
-                           // hands out, [10] warm-start tiles this step, [11] 1 when the
-                           // solve starts from the warm start's guess, [12] particles that
-                           // broke up this step (reset by k_begin)
+                           // [4] its last reduction tag, [10] warm-start tiles this step, [11] 1
+                           // when the solve starts from the warm start's guess, [12] particles
+                           // that broke up this step; [256 + 32 x] and [512 + 32 x], x < 8: the
+                           // counters that deal P2G's and G2P's blocks (reset by k_begin)
   unsigned long long* rwords;  // RED_WORDS: tagged chunk sums of the solve's reductions
   unsigned long long* hz;      // 3 x capc: tagged halo words of each water cell (the register
                                // forms and the memory form's streaming passes: z in the first

@@ -203,6 +203,76 @@ __device__ __forceinline__ uint32_t pack_key(const Dev& d, float px, float py, f
   return (uint32_t)cx | ((uint32_t)cy << KEY_BITS) | ((uint32_t)cz << (2 * KEY_BITS));
 }
 
+// ---- handing out blocks ---------------------------------------------------------------------
+
+// P2G and G2P hand out blocks to their workgroups as each finishes one, since blocks cost what
+// their particles cost. The dealer has eight counters per kernel, Dev::bar[base + 32 x] for
+// x = 0..7, 128 bytes apart, and counter x deals range x, the blocks [x n / 8, (x + 1) n / 8).
+// Workgroup g (of G) starts on range x = g mod 8: its first block is that range's (g / 8)-th,
+// with no atomic, and its later ones are the range's (G_x + i)-th for draws i = 0, 1, ... of
+// counter x, where G_x workgroups start on range x. Once a range is dealt, wave 0 of the
+// workgroup checks the other counters at once and draws from the next open range. Each draw
+// happens one block ahead: lane 0 issues the atomic for the next block and reads its result
+// only when that block starts, so its latency overlaps the current block's work. One counter
+// for every draw serialised them on one address: at 5,700 blocks that took most of the
+// kernels' time. k_begin resets the counters.
+constexpr uint32_t DEAL_P2G = 256, DEAL_G2P = 512;
+
+struct Dealer {
+  uint32_t x;      // the range drawn from (the same in every lane of wave 0)
+  uint32_t spent;  // bit x: range x is dealt (the same in every lane of wave 0)
+  uint32_t ahead;  // lane 0: the result of the draw for the next block
+};
+
+__device__ __forceinline__ uint32_t deal_lo(uint32_t n, uint32_t x) { return (uint32_t)((uint64_t)n * x / 8u); }
+__device__ __forceinline__ uint32_t deal_size(uint32_t n, uint32_t x) { return deal_lo(n, x + 1u) - deal_lo(n, x); }
+// Workgroups (of wgs) that start on range x.
+__device__ __forceinline__ uint32_t deal_starts(uint32_t wgs, uint32_t x) { return wgs > x ? (wgs - x + 7u) / 8u : 0u; }
+
+// The workgroup's first block (FIRST), or its next one. Call from all threads with the
+// workgroup's index wg among wgs; returns the same block to all, NONE once all are dealt.
+template <bool FIRST>
+__device__ __forceinline__ uint32_t deal(const Dev& d, uint32_t base, uint32_t n, uint32_t wg, uint32_t wgs, Dealer& s) {
+  __shared__ uint32_t s_next;
+  if (!FIRST) __syncthreads();
+  if (threadIdx.x < 64u) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t x = FIRST ? (wg & 7u) : s.x;
+    uint32_t spent = FIRST ? 0u : s.spent;
+    uint32_t j = FIRST ? (wg >> 3) : __builtin_amdgcn_readfirstlane(s.ahead) + deal_starts(wgs, x);
+    uint32_t b = NONE;
+    for (;;) {
+      if (j < deal_size(n, x)) {
+        b = deal_lo(n, x) + j;
+        break;
+      }
+      spent |= 1u << x;
+      bool open = false;
+      if (lane < 8u && !(spent >> lane & 1u))
+        open = __hip_atomic_load(d.bar + base + 32u * lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+                   deal_starts(wgs, lane) <
+               deal_size(n, lane);
+      const uint32_t m = (uint32_t)__ballot(open) & 0xffu;
+      spent |= ~m & 0xffu;
+      if (m == 0u) break;
+      const uint32_t sh = (x + 1u) & 7u;  // bit i of rot: range (x + 1 + i) mod 8
+      const uint32_t rot = ((m >> sh) | (m << (8u - sh))) & 0xffu;
+      x = (x + 1u + (uint32_t)__builtin_ctz(rot)) & 7u;
+      uint32_t i = 0;
+      if (lane == 0u) i = atomicAdd(d.bar + base + 32u * x, 1u);
+      j = __builtin_amdgcn_readfirstlane(i) + deal_starts(wgs, x);
+    }
+    s.x = x;
+    s.spent = spent;
+    if (lane == 0u) {
+      if (b != NONE) s.ahead = atomicAdd(d.bar + base + 32u * x, 1u);
+      s_next = b;
+    }
+  }
+  __syncthreads();
+  return s_next;
+}
+
 // ---- begin_step (shaders/fluid.wgsl:31-89) -----------------------------------------------------
 
 template <class L>
@@ -260,23 +330,15 @@ __global__ void k_begin(L l) {
     st->n_blocks = 0;
   }
   st->dt_ns = dt_ns;
-  d.bar[8] = 0u;
-  d.bar[9] = 0u;
+  for (uint32_t x = 0; x < 8u; ++x) {
+    d.bar[DEAL_P2G + 32u * x] = 0u;
+    d.bar[DEAL_G2P + 32u * x] = 0u;
+  }
   d.bar[10] = 0u;
   d.bar[11] = 0u;
   d.bar[12] = 0u;
 }
 
-// The next block for a workgroup of a kernel that hands blocks out through counter `word` of
-// Dev::bar: blocks cost what their particles cost, so a workgroup takes a new one when it is
-// done rather than every gridDim-th. Call from all threads; returns the same block to all.
-__device__ __forceinline__ uint32_t next_block(const Dev& d, uint32_t word) {
-  __shared__ uint32_t s_next;
-  __syncthreads();
-  if (threadIdx.x == 0) s_next = atomicAdd(&d.bar[word], 1u);
-  __syncthreads();
-  return s_next;
-}
 
 // Keys of particles written from the host.
 template <class L>
@@ -690,7 +752,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const uint32_t n = n_blocks(d), tid = threadIdx.x;
   const natray_params* U = d.U;
   const float h = U->origin[3];
-  for (uint32_t b = next_block(d, 8); b < n; b = next_block(d, 8)) {
+  Dealer deal_s;
+  for (uint32_t b = deal<true>(d, DEAL_P2G, n, v.wg, v.wgs, deal_s); b < n;
+       b = deal<false>(d, DEAL_P2G, n, v.wg, v.wgs, deal_s)) {
     if (tid < 27) block_links(d, b, tid, links);
     if (tid < 64) {
       cnt[tid] = 0;
@@ -3262,7 +3326,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ uint32_t links[27];
   const uint32_t n = n_blocks(d), tid = threadIdx.x;
   float fastest = 0.0f;
-  for (uint32_t b = next_block(d, 9); b < n; b = next_block(d, 9)) {
+  Dealer deal_s;
+  for (uint32_t b = deal<true>(d, DEAL_G2P, n, v.wg, v.wgs, deal_s); b < n;
+       b = deal<false>(d, DEAL_G2P, n, v.wg, v.wgs, deal_s)) {
     if (tid < 27) links[tid] = d.block_nb[b * 27u + tid];
     const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
     __syncthreads();
