@@ -382,50 +382,75 @@ __global__ void k_clear(L l) {
 }
 
 // mark_blocks + claim (fluid.wgsl:109-150): mark the blocks of cells c-1..c+2 as claimed and
-// take a rank in the own block. Lanes of a wave with the same own block share one atomic.
-template <class L>
+// take a rank in the own block. Each thread takes PER particles, 256 apart, so that their loads
+// and atomics are in flight together: the kernel is a chain of dependent round trips (key,
+// block slots, the claim), and fewer, fuller waves wait through fewer of them. Lanes of a wave
+// with the same own block share one atomic; every group's atomic is issued before any result
+// is used.
+template <class L, uint32_t PER>
 __global__ __launch_bounds__(256) void k_mark(L l) {
   const View v = view(l);
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
   const natray_params* U = d.U;
-  uint32_t i = v.wg * blockDim.x + threadIdx.x;
-  uint32_t key = i < d.N ? d.key[i] : NONE;
-  bool active = key != NONE;
-  uint32_t own = 0;
-  if (active) {
-    int cx = (int)(key & KEY_MASK), cy = (int)((key >> KEY_BITS) & KEY_MASK), cz = (int)(key >> (2 * KEY_BITS));
-    int bmx = (int)U->bdims[0] - 1, bmy = (int)U->bdims[1] - 1, bmz = (int)U->bdims[2] - 1;
-    int lx = max(0, min((cx - 1) >> 2, bmx)), ly = max(0, min((cy - 1) >> 2, bmy)), lz = max(0, min((cz - 1) >> 2, bmz));
-    int hx = max(0, min((cx + 2) >> 2, bmx)), hy = max(0, min((cy + 2) >> 2, bmy)), hz = max(0, min((cz + 2) >> 2, bmz));
+  const uint32_t lane = lane_id();
+  const uint32_t i0 = v.wg * 256u * PER + threadIdx.x;
+  uint32_t key[PER], own[PER];
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint32_t i = i0 + 256u * k;
+    key[k] = i < d.N ? d.key[i] : NONE;
+  }
+  const int bmx = (int)U->bdims[0] - 1, bmy = (int)U->bdims[1] - 1, bmz = (int)U->bdims[2] - 1;
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) {
+    own[k] = 0;
+    if (key[k] == NONE) continue;
+    const int cx = (int)(key[k] & KEY_MASK), cy = (int)((key[k] >> KEY_BITS) & KEY_MASK), cz = (int)(key[k] >> (2 * KEY_BITS));
+    const int lx = max(0, min((cx - 1) >> 2, bmx)), ly = max(0, min((cy - 1) >> 2, bmy)), lz = max(0, min((cz - 1) >> 2, bmz));
+    const int hx = max(0, min((cx + 2) >> 2, bmx)), hy = max(0, min((cy + 2) >> 2, bmy)), hz = max(0, min((cz + 2) >> 2, bmz));
     // Cells c-1..c+2 span one or two blocks per axis. All of their slots are read before any
     // is written, so the reads go out together instead of one round trip per slot.
     uint32_t slot[8], cur[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int x = lx + (k >> 2), y = ly + ((k >> 1) & 1), z = lz + (k & 1);
-      slot[k] = (x <= hx && y <= hy && z <= hz) ? block_slot(U, x, y, z) : NONE;
-      cur[k] = slot[k] != NONE ? d.block_ptr[slot[k]] : PENDING;
+    for (int q = 0; q < 8; ++q) {
+      const int x = lx + (q >> 2), y = ly + ((q >> 1) & 1), z = lz + (q & 1);
+      slot[q] = (x <= hx && y <= hy && z <= hz) ? block_slot(U, x, y, z) : NONE;
+      cur[q] = slot[q] != NONE ? d.block_ptr[slot[q]] : PENDING;
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (cur[k] != PENDING) d.block_ptr[slot[k]] = PENDING;
-    own = block_slot(U, cx >> 2, cy >> 2, cz >> 2);
+    for (int q = 0; q < 8; ++q)
+      if (cur[q] != PENDING) d.block_ptr[slot[q]] = PENDING;
+    own[k] = block_slot(U, cx >> 2, cy >> 2, cz >> 2);
   }
-  const uint32_t lane = lane_id();
-  unsigned long long todo = __ballot(active);
-  uint32_t rank = 0;
-  while (todo) {
-    int leader = __ffsll((long long)todo) - 1;
-    uint32_t lslot = __shfl(own, leader);
-    unsigned long long peers = __ballot(active && own == lslot);
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(&d.slot_count[lslot], (uint32_t)__popcll(peers));
-    base = __shfl(base, leader);
-    if ((peers >> lane) & 1ull) rank = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-    todo &= ~peers;
+  // Groups of lanes with the same own block: each lane's leader and its place in the group.
+  uint32_t leader[PER], place[PER], base[PER];
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) {
+    const bool active = key[k] != NONE;
+    unsigned long long todo = __ballot(active);
+    leader[k] = 0;
+    place[k] = 0;
+    base[k] = 0;
+    uint32_t size = 0;
+    while (todo) {
+      const int first = __ffsll((long long)todo) - 1;
+      const uint32_t lslot = __shfl(own[k], first);
+      const unsigned long long peers = __ballot(active && own[k] == lslot);
+      if ((peers >> lane) & 1ull) {
+        leader[k] = (uint32_t)first;
+        place[k] = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+      }
+      if ((int)lane == first) size = (uint32_t)__popcll(peers);
+      todo &= ~peers;
+    }
+    if (size) base[k] = atomicAdd(&d.slot_count[own[k]], size);
   }
-  if (active) d.rank[i] = rank;
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint32_t got = __shfl(base[k], (int)leader[k]);
+    if (key[k] != NONE) d.rank[i0 + 256u * k] = got + place[k];
+  }
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -3281,9 +3306,15 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
   const bool on_ground = x[1] < 1.5f * spacing;
   const float r_thin = 0.75f * spacing;
   const float t_thin = 3.0f * __builtin_sqrtf(U->fluid[0] * r_thin * r_thin * r_thin / NATRAY_SIGMA_W);
-  const uint32_t own_cell = tile_cell(links, (uint32_t)((own[0] - corner[0]) * 49 + (own[1] - corner[1]) * 7 + (own[2] - corner[2])));
-  bool covered = own_cell != NONE && accr(d, NATRAY_A_MA, own_cell) > 0;
-  if ((d.sc->flags & NATRAY_SCENE_MASK) && in_mask(d, own[0], own[1], own[2])) covered = true;
+  // `covered` only matters below fill 0.45 (the thin test and, below 0.2, the removal), so the
+  // cell's mark is read only there.
+  bool covered = false;
+  if (fill < 0.45f) {
+    const uint32_t own_cell =
+        tile_cell(links, (uint32_t)((own[0] - corner[0]) * 49 + (own[1] - corner[1]) * 7 + (own[2] - corner[2])));
+    covered = own_cell != NONE && accr(d, NATRAY_A_MA, own_cell) > 0;
+    if ((d.sc->flags & NATRAY_SCENE_MASK) && in_mask(d, own[0], own[1], own[2])) covered = true;
+  }
   const float thin_age = (fill < 0.45f && !on_ground && !covered) ? c1w + step : 0.0f;
   if (escaped || (!covered && fill < 0.2f) || thin_age > t_thin) {
     if (d.spray) {
@@ -4432,7 +4463,13 @@ int launch_begin(const Target& t, void* stream) {
 int launch_bin(const Target& t, void* stream, bool rubber, bool air) {
   NATRAY_LAUNCH(k_clear, LK_CELLS, 256, t, S(stream));
   if (air) NATRAY_LAUNCH(k_keep_blocks, LK_ONE, 256, t, S(stream));
-  NATRAY_LAUNCH(k_mark, LK_PARTS, 256, t, S(stream));
+  // A single scene's marks take four particles per thread; a batch's scenes share the particle
+  // workgroup map, one particle per thread.
+  if (t.d)
+    hipLaunchKernelGGL((k_mark<One, 4>), dim3((t.grid[LK_PARTS] + 3u) / 4u), dim3(256), 0, S(stream), One{*t.d});
+  else
+    hipLaunchKernelGGL((k_mark<Many, 1>), dim3(t.grid[LK_PARTS]), dim3(256), 0, S(stream),
+                       Many{t.scenes, t.map[LK_PARTS]});
   if (rubber) NATRAY_LAUNCH(k_mark_rubber, LK_VERTS, 256, t, S(stream));
   NATRAY_LAUNCH(k_scan_reduce, LK_TILES, 256, t, S(stream));
   NATRAY_LAUNCH(k_scan_tiles, LK_ONE, 1024, t, S(stream));
