@@ -594,19 +594,26 @@ __global__ __launch_bounds__(256) void k_scan_scatter(L l) {
 }
 
 // sort_particles (fluid.wgsl:189-196)
-template <class L>
+template <class L, uint32_t PER>
 __global__ __launch_bounds__(256) void k_sort(L l) {
   const View v = view(l);
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
-  uint32_t i = v.wg * blockDim.x + threadIdx.x;
-  if (i >= d.N) return;
-  uint32_t key = d.key[i];
-  if (key == NONE) return;
-  int cx = (int)(key & KEY_MASK), cy = (int)((key >> KEY_BITS) & KEY_MASK), cz = (int)(key >> (2 * KEY_BITS));
-  uint32_t b = d.block_ptr[block_slot(d.U, cx >> 2, cy >> 2, cz >> 2)];
-  if (b >= PENDING) return;
-  d.sorted[d.block_start[b] + d.rank[i]] = i;
+  // PER particles per thread, 256 apart, each step's loads in flight together (as k_mark).
+  const uint32_t i0 = v.wg * 256u * PER + threadIdx.x;
+  uint32_t key[PER], b[PER];
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) key[k] = i0 + 256u * k < d.N ? d.key[i0 + 256u * k] : NONE;
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) {
+    b[k] = PENDING;
+    if (key[k] == NONE) continue;
+    const int cx = (int)(key[k] & KEY_MASK), cy = (int)((key[k] >> KEY_BITS) & KEY_MASK), cz = (int)(key[k] >> (2 * KEY_BITS));
+    b[k] = d.block_ptr[block_slot(d.U, cx >> 2, cy >> 2, cz >> 2)];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k)
+    if (b[k] < PENDING) d.sorted[d.block_start[b[k]] + d.rank[i0 + 256u * k]] = i0 + 256u * k;
 }
 
 // ---- p2g (shaders/fluid.wgsl:198-303) --------------------------------------------------------------
@@ -1078,9 +1085,12 @@ __device__ __forceinline__ float precondition(float r, float diag, bool water, i
   const bool wym = ly > 0 && ((wm >> (lane - 4u)) & 1ull), wyp = ly < 3 && ((wm >> (lane + 4u)) & 1ull);
   const bool wzm = lz > 0 && ((wm >> (lane - 1u)) & 1ull), wzp = lz < 3 && ((wm >> (lane + 1u)) & 1ull);
   const uint32_t colour = (uint32_t)(lx + ly + lz) & 1u;
+  // The first half sweep starts from z = +0, so each wet neighbour adds +0: s stays r, except
+  // that r = -0 becomes +0 once any neighbour is added.
   float z = 0.0f;
+  if (water && colour == 0u) z = ((wxm || wxp || wym || wyp || wzm || wzp) ? r + 0.0f : r) / diag;
 #pragma unroll
-  for (uint32_t k = 0; k < 5u; ++k) {
+  for (uint32_t k = 1; k < 5u; ++k) {
     const Faces f = block_faces(z);
     if (water && colour == (k & 1u)) {
       float s = r;
@@ -4474,7 +4484,11 @@ int launch_bin(const Target& t, void* stream, bool rubber, bool air) {
   NATRAY_LAUNCH(k_scan_reduce, LK_TILES, 256, t, S(stream));
   NATRAY_LAUNCH(k_scan_tiles, LK_ONE, 1024, t, S(stream));
   NATRAY_LAUNCH(k_scan_scatter, LK_TILES, 256, t, S(stream));
-  NATRAY_LAUNCH(k_sort, LK_PARTS, 256, t, S(stream));
+  if (t.d)
+    hipLaunchKernelGGL((k_sort<One, 4>), dim3((t.grid[LK_PARTS] + 3u) / 4u), dim3(256), 0, S(stream), One{*t.d});
+  else
+    hipLaunchKernelGGL((k_sort<Many, 1>), dim3(t.grid[LK_PARTS]), dim3(256), 0, S(stream),
+                       Many{t.scenes, t.map[LK_PARTS]});
   return (int)hipGetLastError();
 }
 int launch_p2g(const Target& t, void* stream) {
