@@ -2894,15 +2894,18 @@ __global__ __launch_bounds__(256) void k_project(L l) {
              });
 }
 
-// face_set (fluid.wgsl:607-612) of the cell at offset (ox, oy, oz), for face dim.
-__device__ __forceinline__ bool face_set_at(const Dev& d, const uint32_t* lk, int lx, int ly, int lz, int ox, int oy, int oz,
-                                            uint32_t dim) {
-  const uint32_t c = nbr(lk, lx, ly, lz, ox, oy, oz);
-  if (c == NONE) return false;
-  if (cfr(d, NATRAY_F_TYPE, c) == WATER) return true;
-  return cell_type(d, nbr(lk, lx, ly, lz, ox - (dim == 0), oy - (dim == 1), oz - (dim == 2))) == WATER;
+// The six face neighbours, k = 2 axis + (0: -1, 1: +1).
+__device__ __forceinline__ void face_offsets(int k, int& ox, int& oy, int& oz) {
+  const int sgn = (k & 1) ? 1 : -1;
+  ox = (k >> 1) == 0 ? sgn : 0;
+  oy = (k >> 1) == 1 ? sgn : 0;
+  oz = (k >> 1) == 2 ? sgn : 0;
 }
 
+// extend_1 (fluid.wgsl:607-640). A wave whose cells are all water has every face set and keeps
+// its velocities. Any other wave loads, before it tests anything, every cell type and velocity
+// the tests may need, so their loads are in flight together rather than one dependent round
+// trip per test; the faces and sums are those of the reference, in its order.
 template <class L>
 __global__ __launch_bounds__(256) void k_extend1(L l) {
   const View v = view(l);
@@ -2911,38 +2914,66 @@ __global__ __launch_bounds__(256) void k_extend1(L l) {
   for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u,
              [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
                const uint32_t cell = b * 64u + lane;
+               const float tc = cfr(d, NATRAY_F_TYPE, cell);
+               float u[3];
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim) u[dim] = cfr(d, NATRAY_F_U + dim, cell);
+               if (__ballot(tc != WATER) == 0ull) {
+#pragma unroll
+                 for (uint32_t dim = 0; dim < 3; ++dim) cfr(d, NATRAY_F_EXT + dim, cell) = u[dim];
+                 cfr(d, NATRAY_F_EXT_SET, cell) = 7.0f;
+                 return;
+               }
+               uint32_t nb[6];
+               float tn[6];
+#pragma unroll
+               for (int k = 0; k < 6; ++k) {
+                 int ox, oy, oz;
+                 face_offsets(k, ox, oy, oz);
+                 nb[k] = nbr(lk, lx, ly, lz, ox, oy, oz);
+                 tn[k] = cell_type(d, nb[k]);
+               }
+               // face_set (fluid.wgsl:607-612) of neighbour k for component dim: the neighbour is
+               // water, or the cell below it along dim is.
+               float tb[3][6], un[3][6];
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim)
+#pragma unroll
+                 for (int k = 0; k < 6; ++k) {
+                   int ox, oy, oz;
+                   face_offsets(k, ox, oy, oz);
+                   tb[dim][k] = nb[k] == NONE ? GAS
+                                              : cell_type(d, nbr(lk, lx, ly, lz, ox - (dim == 0), oy - (dim == 1), oz - (dim == 2)));
+                   un[dim][k] = nb[k] == NONE ? 0.0f : cfr(d, NATRAY_F_U + dim, nb[k]);
+                 }
                uint32_t done = 0;
 #pragma unroll
                for (uint32_t dim = 0; dim < 3; ++dim) {
-                 float u = cfr(d, NATRAY_F_U + dim, cell);
-                 if (face_set_at(d, lk, lx, ly, lz, 0, 0, 0, dim)) {
+                 float ud = u[dim];
+                 if (tc == WATER || tn[2 * dim] == WATER) {
                    done |= 1u << dim;
                  } else {
                    float sum = 0.0f, cnt = 0.0f;
 #pragma unroll
                    for (int k = 0; k < 6; ++k) {
-                     const int s = (k & 1) ? 1 : -1;
-                     const int ox = (k >> 1) == 0 ? s : 0, oy = (k >> 1) == 1 ? s : 0, oz = (k >> 1) == 2 ? s : 0;
-                     if (face_set_at(d, lk, lx, ly, lz, ox, oy, oz, dim)) {
-                       sum += cfr(d, NATRAY_F_U + dim, nbr(lk, lx, ly, lz, ox, oy, oz));
+                     if (nb[k] != NONE && (tn[k] == WATER || tb[dim][k] == WATER)) {
+                       sum += un[dim][k];
                        cnt += 1.0f;
                      }
                    }
                    if (cnt > 0.0f) {
-                     u = sum / cnt;
+                     ud = sum / cnt;
                      done |= 1u << dim;
                    }
                  }
-                 cfr(d, NATRAY_F_EXT + dim, cell) = u;
+                 cfr(d, NATRAY_F_EXT + dim, cell) = ud;
                }
                cfr(d, NATRAY_F_EXT_SET, cell) = (float)done;
              });
 }
 
-__device__ __forceinline__ bool ext_set(const Dev& d, uint32_t cell, uint32_t dim) {
-  return cell != NONE && ((((uint32_t)cfr(d, NATRAY_F_EXT_SET, cell)) >> dim) & 1u) != 0u;
-}
-
+// extend_2 (fluid.wgsl:642-674), then bounded: as extend_1, a wave whose faces are all set
+// skips the neighbours, and any other loads every neighbour's mark and value up front.
 template <class L>
 __global__ __launch_bounds__(256) void k_extend2(L l) {
   const View v = view(l);
@@ -2952,26 +2983,45 @@ __global__ __launch_bounds__(256) void k_extend2(L l) {
              [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
                const uint32_t cell = b * 64u + lane;
                const float ty = cfr(d, NATRAY_F_TYPE, cell);
+               const uint32_t set = (uint32_t)cfr(d, NATRAY_F_EXT_SET, cell);
+               uint32_t nb[6];
 #pragma unroll
-               for (uint32_t dim = 0; dim < 3; ++dim) {
-                 float u = cfr(d, NATRAY_F_U + dim, cell);
-                 if (ext_set(d, cell, dim)) {
-                   u = cfr(d, NATRAY_F_EXT + dim, cell);
-                 } else {
+               for (int k = 0; k < 6; ++k) {
+                 int ox, oy, oz;
+                 face_offsets(k, ox, oy, oz);
+                 nb[k] = nbr(lk, lx, ly, lz, ox, oy, oz);
+               }
+               float u[3];
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim)
+                 u[dim] = (set >> dim & 1u) ? cfr(d, NATRAY_F_EXT + dim, cell) : cfr(d, NATRAY_F_U + dim, cell);
+               if (__ballot(set != 7u) != 0ull) {
+                 uint32_t sn[6];
+                 float en[3][6];
+#pragma unroll
+                 for (int k = 0; k < 6; ++k) sn[k] = nb[k] == NONE ? 0u : (uint32_t)cfr(d, NATRAY_F_EXT_SET, nb[k]);
+#pragma unroll
+                 for (uint32_t dim = 0; dim < 3; ++dim)
+#pragma unroll
+                   for (int k = 0; k < 6; ++k) en[dim][k] = nb[k] == NONE ? 0.0f : cfr(d, NATRAY_F_EXT + dim, nb[k]);
+#pragma unroll
+                 for (uint32_t dim = 0; dim < 3; ++dim) {
+                   if (set >> dim & 1u) continue;
                    float sum = 0.0f, cnt = 0.0f;
 #pragma unroll
                    for (int k = 0; k < 6; ++k) {
-                     const int s = (k & 1) ? 1 : -1;
-                     const uint32_t nb = nbr(lk, lx, ly, lz, (k >> 1) == 0 ? s : 0, (k >> 1) == 1 ? s : 0, (k >> 1) == 2 ? s : 0);
-                     if (ext_set(d, nb, dim)) {
-                       sum += cfr(d, NATRAY_F_EXT + dim, nb);
+                     if (nb[k] != NONE && (sn[k] >> dim & 1u)) {
+                       sum += en[dim][k];
                        cnt += 1.0f;
                      }
                    }
-                   if (cnt > 0.0f) u = sum / cnt;
+                   if (cnt > 0.0f) u[dim] = sum / cnt;
                  }
-                 const uint32_t lo = nbr(lk, lx, ly, lz, -(dim == 0), -(dim == 1), -(dim == 2));
-                 cfr(d, NATRAY_F_U + dim, cell) = bounded(d, u, ty, cell_type(d, lo), cell, lo, dim);
+               }
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim) {
+                 const uint32_t lo = nb[2 * dim];
+                 cfr(d, NATRAY_F_U + dim, cell) = bounded(d, u[dim], ty, cell_type(d, lo), cell, lo, dim);
                }
              });
 }
