@@ -928,35 +928,44 @@ __global__ __launch_bounds__(256) void k_grid_prep(L l) {
   const natray_params* U = d.U;
   for_blocks(d, v.wg * 4u, n_blocks(d), v.wgs * 4u, [&](uint32_t b, uint32_t lane, int, int, int, const uint32_t*) {
     const uint32_t cell = b * 64u + lane;
+    // Every sum the cell may use is read up front, so the loads go out together rather than
+    // behind the tests that pick among them.
+    static_assert(NATRAY_A_N < 11u && NATRAY_A_PC < 11u && NATRAY_A_WC < 11u && NATRAY_A_MF < 11u && NATRAY_A_MA < 11u &&
+                      NATRAY_A_WT(2) < 11u && NATRAY_A_MOM(2) < 11u,
+                  "grid_prep's sums are the first 11 fields and A_MW");
+    int32_t a[11];
+#pragma unroll
+    for (uint32_t f = 0; f < 11u; ++f) a[f] = accr(d, f, cell);
+    const int32_t a_mw = accr(d, NATRAY_A_MW, cell);
     int cx, cy, cz;
     cell_coord(d, cell, cx, cy, cz);
     float ty = GAS;
     if (is_solid_cell(d, cx, cy, cz)) {
       ty = SOLID;
-    } else if (accr(d, NATRAY_A_N, cell) > 0) {
+    } else if (a[NATRAY_A_N] > 0) {
       ty = WATER;
     }
     cfr(d, NATRAY_F_TYPE, cell) = ty;
 #pragma unroll
     for (uint32_t k = 0; k < 3; ++k) {
-      float w = from_fp(accr(d, NATRAY_A_WT(k), cell), NATRAY_FP_W);
+      float w = from_fp(a[NATRAY_A_WT(k)], NATRAY_FP_W);
       float u = 0.0f;
-      if (w > 1e-6f) u = from_fp(accr(d, NATRAY_A_MOM(k), cell), NATRAY_FP_M) / w + dt * U->gravity[k];
+      if (w > 1e-6f) u = from_fp(a[NATRAY_A_MOM(k)], NATRAY_FP_M) / w + dt * U->gravity[k];
       cfr(d, NATRAY_F_U + k, cell) = u;
     }
-    float wc = from_fp(accr(d, NATRAY_A_WC, cell), NATRAY_FP_W);
-    float ma = from_fp(accr(d, NATRAY_A_MA, cell), NATRAY_FP_A);
-    if (ty == GAS && accr(d, NATRAY_A_MW, cell) > 0) {
+    float wc = from_fp(a[NATRAY_A_WC], NATRAY_FP_W);
+    float ma = from_fp(a[NATRAY_A_MA], NATRAY_FP_A);
+    if (ty == GAS && a_mw > 0) {
       ty = RUBBER;
       cfr(d, NATRAY_F_TYPE, cell) = ty;
     }
     float p = 0.0f;
     if (ty == WATER && wc > 1e-6f) {
-      p = from_fp(accr(d, NATRAY_A_PC, cell), NATRAY_FP_P) / wc * 1e6f;
+      p = from_fp(a[NATRAY_A_PC], NATRAY_FP_P) / wc * 1e6f;
     } else if (ty != WATER && ty != SOLID && ma > 0.0f) {
       ty = COVERED;
       cfr(d, NATRAY_F_TYPE, cell) = ty;
-      p = from_fp(accr(d, NATRAY_A_MF, cell), NATRAY_FP_F) / ma * 1e6f;
+      p = from_fp(a[NATRAY_A_MF], NATRAY_FP_F) / ma * 1e6f;
     } else if (ty != WATER && ty != SOLID && (d.sc->flags & NATRAY_SCENE_MASK) && in_mask(d, cx, cy, cz)) {
       ty = COVERED;
       cfr(d, NATRAY_F_TYPE, cell) = ty;
