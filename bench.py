@@ -96,7 +96,8 @@ def measured_traffic(kernel: str) -> float | None:
 
 # The kernels of each phase, as the counter summary names them (single scene, FP32 CG).
 PHASE_KERNELS = {
-    "bin": ["k_clear<One>", "k_mark<One>", "k_scan_reduce<One>", "k_scan_tiles<One>", "k_scan_scatter<One>", "k_sort<One>"],
+    "bin": ["k_clear<One>", "k_mark<One,4u>", "k_scan_reduce<One>", "k_scan_tiles<One>", "k_scan_scatter<One>",
+            "k_sort<One,4u>"],
     "p2g": ["k_p2g<One>"],
     "grid_prep": ["k_grid_prep<One>"],
     "solve": ["k_solve<false,One>"],
