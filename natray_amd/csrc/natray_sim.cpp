@@ -1458,8 +1458,8 @@ int batch_layout(natray_batch* b) {
 
 int batch_phase_launch(natray_batch* b, int phase);
 
-// The solve's workgroups from each scene's last solve, once every scene has run a step; the
-// solve is launched outside the step graphs, so only its map changes. A scene's solve lasts
+// The solve's workgroups from each scene's last solve, once every scene has run a step (a new
+// plan drops the step graphs, which hold the solve's grid). A scene's solve lasts
 // about (iterations + 1) x (kSolveFixed + the most blocks one of its workgroups holds): every
 // iteration pays its two reductions whatever the workgroup holds (kSolveFixed, in blocks of
 // pass time: about 15 us against 0.3 us per block, measured), and a workgroup holds whole
