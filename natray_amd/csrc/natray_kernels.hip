@@ -212,16 +212,17 @@ __device__ __forceinline__ uint32_t pack_key(const Dev& d, float px, float py, f
 // with no atomic, and its later ones are the range's (G_x + i)-th for draws i = 0, 1, ... of
 // counter x, where G_x workgroups start on range x. Once a range is dealt, wave 0 of the
 // workgroup checks the other counters at once and draws from the next open range. Each draw
-// happens one block ahead: lane 0 issues the atomic for the next block and reads its result
-// only when that block starts, so its latency overlaps the current block's work. One counter
-// for every draw serialised them on one address: at 5,700 blocks that took most of the
-// kernels' time. k_begin resets the counters.
+// happens one block ahead: lane 0 issues the atomic for the block after next, and wave 0
+// reads its result during the next block, past that block's first barrier, so neither the
+// atomic's latency nor a barrier of its own falls between two blocks. One counter for every
+// draw serialised them on one address: at 5,700 blocks that took most of the kernels' time.
+// k_begin resets the counters.
 constexpr uint32_t DEAL_P2G = 256, DEAL_G2P = 512;
 
 struct Dealer {
   uint32_t x;      // the range drawn from (the same in every lane of wave 0)
   uint32_t spent;  // bit x: range x is dealt (the same in every lane of wave 0)
-  uint32_t ahead;  // lane 0: the result of the draw for the next block
+  uint32_t ahead;  // lane 0: the result of the last draw, resolved during the next block
 };
 
 __device__ __forceinline__ uint32_t deal_lo(uint32_t n, uint32_t x) { return (uint32_t)((uint64_t)n * x / 8u); }
@@ -229,48 +230,74 @@ __device__ __forceinline__ uint32_t deal_size(uint32_t n, uint32_t x) { return d
 // Workgroups (of wgs) that start on range x.
 __device__ __forceinline__ uint32_t deal_starts(uint32_t wgs, uint32_t x) { return wgs > x ? (wgs - x + 7u) / 8u : 0u; }
 
-// The workgroup's first block (FIRST), or its next one. Call from all threads with the
-// workgroup's index wg among wgs; returns the same block to all, NONE once all are dealt.
-template <bool FIRST>
-__device__ __forceinline__ uint32_t deal(const Dev& d, uint32_t base, uint32_t n, uint32_t wg, uint32_t wgs, Dealer& s) {
-  __shared__ uint32_t s_next;
-  if (!FIRST) __syncthreads();
-  if (threadIdx.x < 64u) {
-    const uint32_t lane = threadIdx.x;
-    uint32_t x = FIRST ? (wg & 7u) : s.x;
-    uint32_t spent = FIRST ? 0u : s.spent;
-    uint32_t j = FIRST ? (wg >> 3) : __builtin_amdgcn_readfirstlane(s.ahead) + deal_starts(wgs, x);
-    uint32_t b = NONE;
-    for (;;) {
-      if (j < deal_size(n, x)) {
-        b = deal_lo(n, x) + j;
-        break;
-      }
-      spent |= 1u << x;
-      bool open = false;
-      if (lane < 8u && !(spent >> lane & 1u))
-        open = __hip_atomic_load(d.bar + base + 32u * lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
-                   deal_starts(wgs, lane) <
-               deal_size(n, lane);
-      const uint32_t m = (uint32_t)__ballot(open) & 0xffu;
-      spent |= ~m & 0xffu;
-      if (m == 0u) break;
-      const uint32_t sh = (x + 1u) & 7u;  // bit i of rot: range (x + 1 + i) mod 8
-      const uint32_t rot = ((m >> sh) | (m << (8u - sh))) & 0xffu;
-      x = (x + 1u + (uint32_t)__builtin_ctz(rot)) & 7u;
-      uint32_t i = 0;
-      if (lane == 0u) i = atomicAdd(d.bar + base + 32u * x, 1u);
-      j = __builtin_amdgcn_readfirstlane(i) + deal_starts(wgs, x);
+// The block after draw `j` of range s.x (wave 0, every lane): that range's block, else the
+// next open range's, whose counters wave 0 checks at once; NONE once every block is dealt.
+__device__ __forceinline__ uint32_t deal_resolve(const Dev& d, uint32_t base, uint32_t n, uint32_t wgs, Dealer& s,
+                                                 uint32_t j) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t x = s.x, spent = s.spent, b = NONE;
+  for (;;) {
+    if (j < deal_size(n, x)) {
+      b = deal_lo(n, x) + j;
+      break;
     }
-    s.x = x;
-    s.spent = spent;
-    if (lane == 0u) {
-      if (b != NONE) s.ahead = atomicAdd(d.bar + base + 32u * x, 1u);
-      s_next = b;
+    spent |= 1u << x;
+    bool open = false;
+    if (lane < 8u && !(spent >> lane & 1u))
+      open = __hip_atomic_load(d.bar + base + 32u * lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+                 deal_starts(wgs, lane) <
+             deal_size(n, lane);
+    const uint32_t m = (uint32_t)__ballot(open) & 0xffu;
+    spent |= ~m & 0xffu;
+    if (m == 0u) break;
+    const uint32_t sh = (x + 1u) & 7u;  // bit i of rot: range (x + 1 + i) mod 8
+    const uint32_t rot = ((m >> sh) | (m << (8u - sh))) & 0xffu;
+    x = (x + 1u + (uint32_t)__builtin_ctz(rot)) & 7u;
+    uint32_t i = 0;
+    if (lane == 0u) i = atomicAdd(d.bar + base + 32u * x, 1u);
+    j = __builtin_amdgcn_readfirstlane(i) + deal_starts(wgs, x);
+  }
+  s.x = x;
+  s.spent = spent;
+  return b;
+}
+
+// The workgroup's first block: range wg mod 8's (wg / 8)-th, with no atomic. Call from all
+// threads with the workgroup's index wg among wgs; returns the same block to all.
+__device__ __forceinline__ uint32_t deal_first(const Dev& d, uint32_t base, uint32_t n, uint32_t wg, uint32_t wgs,
+                                               Dealer& s, uint32_t* s_next) {
+  if (threadIdx.x < 64u) {
+    s.x = wg & 7u;
+    s.spent = 0u;
+    const uint32_t b = deal_resolve(d, base, n, wgs, s, wg >> 3);
+    if (threadIdx.x == 0u) {
+      if (b != NONE) s.ahead = atomicAdd(d.bar + base + 32u * s.x, 1u);
+      *s_next = NONE;
+      s_next[1] = b;
     }
   }
   __syncthreads();
-  return s_next;
+  return s_next[1];
+}
+
+// During a block, after its first barrier (all threads call it): wave 0 resolves the draw
+// made one block ago into the block after this one, and lane 0 draws again. The result waits
+// in s_next[0] for deal_next.
+__device__ __forceinline__ void deal_ahead(const Dev& d, uint32_t base, uint32_t n, uint32_t wgs, Dealer& s,
+                                           uint32_t* s_next) {
+  if (threadIdx.x < 64u) {
+    const uint32_t b = deal_resolve(d, base, n, wgs, s, __builtin_amdgcn_readfirstlane(s.ahead) + deal_starts(wgs, s.x));
+    if (threadIdx.x == 0u) {
+      if (b != NONE) s.ahead = atomicAdd(d.bar + base + 32u * s.x, 1u);
+      s_next[0] = b;
+    }
+  }
+}
+
+// The next block, once every thread is done with the current one.
+__device__ __forceinline__ uint32_t deal_next(const uint32_t* s_next) {
+  __syncthreads();
+  return s_next[0];
 }
 
 // ---- begin_step (shaders/fluid.wgsl:31-89) -----------------------------------------------------
@@ -785,8 +812,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const natray_params* U = d.U;
   const float h = U->origin[3];
   Dealer deal_s;
-  for (uint32_t b = deal<true>(d, DEAL_P2G, n, v.wg, v.wgs, deal_s); b < n;
-       b = deal<false>(d, DEAL_P2G, n, v.wg, v.wgs, deal_s)) {
+  __shared__ uint32_t deal_w[2];
+  for (uint32_t b = deal_first(d, DEAL_P2G, n, v.wg, v.wgs, deal_s, deal_w); b < n; b = deal_next(deal_w)) {
     if (tid < 27) block_links(d, b, tid, links);
     if (tid < 64) {
       cnt[tid] = 0;
@@ -795,6 +822,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     for (uint32_t t = tid; t < 8 * P2G_TILE; t += 256) tile[t] = 0;
     const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
     __syncthreads();
+    deal_ahead(d, DEAL_P2G, n, v.wgs, deal_s, deal_w);
     if (count == 0u) {
       if (tid < 64) accr(d, NATRAY_A_N, b * 64u + tid) = 0;
       continue;
@@ -3427,11 +3455,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const uint32_t n = n_blocks(d), tid = threadIdx.x;
   float fastest = 0.0f;
   Dealer deal_s;
-  for (uint32_t b = deal<true>(d, DEAL_G2P, n, v.wg, v.wgs, deal_s); b < n;
-       b = deal<false>(d, DEAL_G2P, n, v.wg, v.wgs, deal_s)) {
+  __shared__ uint32_t deal_w[2];
+  for (uint32_t b = deal_first(d, DEAL_G2P, n, v.wg, v.wgs, deal_s, deal_w); b < n; b = deal_next(deal_w)) {
     if (tid < 27) links[tid] = d.block_nb[b * 27u + tid];
     const uint32_t first = d.block_start[b], count = d.block_start[b + 1] - first;
     __syncthreads();
+    deal_ahead(d, DEAL_G2P, n, v.wgs, deal_s, deal_w);
     if (count == 0u) continue;
     for (uint32_t t = tid; t < TILE; t += 256) {
       const uint32_t cell = tile_cell(links, t);
