@@ -394,8 +394,10 @@ int upload_params(natray_sim* s) {
   return NATRAY_OK;
 }
 
-// Reads the State back and checks the solver's barrier; the stream is idle afterwards.
-int sync_state(natray_sim* s) {
+// Reads the State back and checks the solver's barrier; the stream is idle afterwards. A lost
+// barrier's message names `launch_call`, the call that switches the launch, when the solve was
+// a plain launch (`plain`; < 0: the scene's own setting).
+int sync_state(natray_sim* s, const char* launch_call = "natray_sim_set_solve_launch(sim, ", int plain = -1) {
   HIPCHK(hipMemcpyAsync(s->h_state, s->D.st, sizeof(natray_state), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipMemcpyAsync(s->h_words, s->D.bar, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -410,11 +412,12 @@ int sync_state(natray_sim* s) {
     int e = clear_tags(s);
     if (e) return e;
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (plain < 0) plain = s->D.solve_plain != 0u;
     return fail(NATRAY_E_SOLVER,
-                s->D.solve_plain ? "the pressure solve lost its grid barrier: not all of its workgroups were resident "
-                                   "(another program on the GPU?); natray_sim_set_solve_launch(sim, "
-                                   "NATRAY_SOLVE_COOPERATIVE) launches it cooperatively"
-                                 : "the pressure solve lost its grid barrier (not all workgroups were resident)");
+                plain ? std::string("the pressure solve lost its grid barrier: not all of its workgroups were resident "
+                                    "(another program on the GPU?); ") +
+                            launch_call + "NATRAY_SOLVE_COOPERATIVE) launches it cooperatively"
+                      : std::string("the pressure solve lost its grid barrier (not all workgroups were resident)"));
   }
   return NATRAY_OK;
 }
@@ -1660,7 +1663,7 @@ int batch_run(natray_batch* b, uint32_t n_steps, uint32_t iters, hipEvent_t t0 =
   }
   b->ev_used = 0;
   for (size_t k = 0; k < b->sims.size(); ++k) {
-    int e = sync_state(b->sims[k]);
+    int e = sync_state(b->sims[k], "natray_batch_set_solve_launch(batch, ", b->tgt.plain_solve ? 1 : 0);
     if (e) return e;
     if (b->sims[k]->last.overflow)
       return fail(NATRAY_E_OVERFLOW, "scene " + std::to_string(k) + " needs more grid blocks than its capacity");

@@ -93,6 +93,24 @@ def test_batch_cooperative_solve_launch(graph):
     check_same(f"cooperative graph {graph}", got, want)
 
 
+def test_batch_solve_launch_modes():
+    """A batch takes the plain and the cooperative launch; the test-only mode of a single
+    scene's solve is refused with NATRAY_E_ARG, and the batch still steps afterwards."""
+    say("[batch] solve launch modes")
+    sims = [Sim.from_scene(s) for s in small_set(abi.CG_FP32)[:2]]
+    b = Batch(sims)
+    with pytest.raises(abi.NatrayError) as e:
+        b.set_solve_launch(abi.SOLVE_PLAIN_TEST_CHECK)
+    assert e.value.code == abi.E_ARG
+    b.set_solve_launch(abi.SOLVE_COOPERATIVE)
+    b.set_solve_launch(abi.SOLVE_PLAIN)
+    b.run(2, iters=400)
+    assert all(s.state().solve_iters > 0 for s in sims)
+    b.close()
+    for s in sims:
+        s.close()
+
+
 @pytest.mark.parametrize("precision", [abi.CG_MIXED, abi.CG_PIPELINED])
 def test_batch_of_many_scenes_memory_form(precision):
     say(f"[batch] 64 scenes of 32k particles, precision {precision}")
