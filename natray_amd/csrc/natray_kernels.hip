@@ -1382,35 +1382,57 @@ __device__ __forceinline__ Row cell_row(const Dev& d, uint32_t cell, int lx, int
   const natray_params* U = d.U;
   const float h = U->origin[3], dt = dt_of(d);
   const float* cP = d.cf + (size_t)NATRAY_F_P * d.capc;
-  Row w = {cfr(d, NATRAY_F_TYPE, cell) == WATER, 1.0f, 0.0f, 0.0f, 0u};
+  const float tc = cfr(d, NATRAY_F_TYPE, cell);
+  Row w = {tc == WATER, 1.0f, 0.0f, 0.0f, 0u};
   if (!w.water) return w;
+  // Every value the row may use is loaded before any test picks among them, so the loads go
+  // out together; the arithmetic below is the reference's, in its order.
+  uint32_t nb[6];  // k = 2 dim + (0: below, 1: above)
+  float tn[6], pn[6], qn[6], uu[3], uc[3];
+#pragma unroll
+  for (uint32_t k = 0; k < 6u; ++k) {
+    const int sg = (k & 1u) ? 1 : -1, dim = (int)(k >> 1);
+    nb[k] = nbr(lk, lx, ly, lz, dim == 0 ? sg : 0, dim == 1 ? sg : 0, dim == 2 ? sg : 0);
+    tn[k] = cell_type(d, nb[k]);
+    pn[k] = nb[k] != NONE ? cP[nb[k]] : 0.0f;
+    qn[k] = nb[k] != NONE ? p0[nb[k]] : 0.0f;
+  }
+#pragma unroll
+  for (uint32_t dim = 0; dim < 3u; ++dim) {
+    uc[dim] = cfr(d, NATRAY_F_U + dim, cell);
+    uu[dim] = nb[2 * dim + 1] != NONE ? cfr(d, NATRAY_F_U + dim, nb[2 * dim + 1]) : 0.0f;
+  }
+  const int32_t src = accr(d, NATRAY_A_SRC, cell);
+  const float pc = cP[cell], p0c = p0[cell];
   float cs = U->gravity[3] * dt;
   float kc = h * h / (cs * cs);
   float div = 0.0f, off = 0.0f;
   w.diag = kc;
-  w.bv = kc * cP[cell];
+  w.bv = kc * pc;
 #pragma unroll
   for (uint32_t dim = 0; dim < 3; ++dim) {
-    const int ex = dim == 0, ey = dim == 1, ez = dim == 2;
-    const uint32_t up = nbr(lk, lx, ly, lz, ex, ey, ez), dn = nbr(lk, lx, ly, lz, -ex, -ey, -ez);
-    div += face_u(d, up, cell, dim) - face_u(d, cell, dn, dim);
+    const uint32_t up = nb[2 * dim + 1], dn = nb[2 * dim];
+    // face_u(up, cell) - face_u(cell, dn) (fluid.wgsl:368-371)
+    const float fu = up == NONE ? 0.0f : bounded(d, uu[dim], tn[2 * dim + 1], tc, up, cell, dim);
+    div += fu - bounded(d, uc[dim], tc, tn[2 * dim], cell, dn, dim);
 #pragma unroll
-    for (int sg = -1; sg <= 1; sg += 2) {
-      const uint32_t nb = sg < 0 ? dn : up;
-      const float nt = cell_type(d, nb);
+    for (uint32_t e = 0; e < 2u; ++e) {
+      const uint32_t k = 2 * dim + e;
+      const float nt = tn[k];
       if (nt == SOLID) continue;
       w.diag += 1.0f;
       if (nt == WATER) {
-        off -= p0[nb];
-        w.mask |= 1u << (dim * 2u + (sg > 0 ? 1u : 0u));
+        off -= qn[k];
+        w.mask |= 1u << k;
       } else {
-        w.bv += pressure_of(d, nb);
+        // pressure_of (fluid.wgsl:375-379) of a cell that is not water, 0 past the grid
+        w.bv += nb[k] == NONE ? 0.0f : pn[k];
       }
     }
   }
-  div -= from_fp(accr(d, NATRAY_A_SRC, cell), NATRAY_FP_Q) / (h * h);
+  div -= from_fp(src, NATRAY_FP_Q) / (h * h);
   w.bv -= U->fluid[0] * h / dt * div;
-  w.r = w.bv - w.diag * p0[cell] - off;
+  w.r = w.bv - w.diag * p0c - off;
   w.mask |= 64u;
   return w;
 }
