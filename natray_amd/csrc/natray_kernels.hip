@@ -2941,13 +2941,25 @@ __global__ __launch_bounds__(256) void k_project(L l) {
              [&](uint32_t b, uint32_t lane, int lx, int ly, int lz, const uint32_t* lk) {
                const uint32_t cell = b * 64u + lane;
                const float ty = cfr(d, NATRAY_F_TYPE, cell);
+               // Both pressures are loaded before the face test that decides whether they are
+               // used (pressure_of, fluid.wgsl:375-379, on the loaded values).
+               const float pc = cfr(d, NATRAY_F_P, cell), qc = ty == WATER ? fmax_(pc, NATRAY_P_FLOOR) : pc;
+               uint32_t nb[3];
+               float nt[3], pn[3], un[3];
 #pragma unroll
                for (uint32_t dim = 0; dim < 3; ++dim) {
-                 const uint32_t nb = nbr(lk, lx, ly, lz, -(dim == 0), -(dim == 1), -(dim == 2));
-                 const float nt = cell_type(d, nb);
-                 float u = bounded(d, cfr(d, NATRAY_F_U + dim, cell), ty, nt, cell, nb, dim);
-                 if (ty != SOLID && nt != SOLID && (ty == WATER || nt == WATER))
-                   u -= k * (pressure_of(d, cell) - pressure_of(d, nb));
+                 nb[dim] = nbr(lk, lx, ly, lz, -(dim == 0), -(dim == 1), -(dim == 2));
+                 nt[dim] = cell_type(d, nb[dim]);
+                 pn[dim] = nb[dim] != NONE ? cfr(d, NATRAY_F_P, nb[dim]) : 0.0f;
+                 un[dim] = cfr(d, NATRAY_F_U + dim, cell);
+               }
+#pragma unroll
+               for (uint32_t dim = 0; dim < 3; ++dim) {
+                 float u = bounded(d, un[dim], ty, nt[dim], cell, nb[dim], dim);
+                 if (ty != SOLID && nt[dim] != SOLID && (ty == WATER || nt[dim] == WATER)) {
+                   const float qn = nb[dim] == NONE ? 0.0f : (nt[dim] == WATER ? fmax_(pn[dim], NATRAY_P_FLOOR) : pn[dim]);
+                   u -= k * (qc - qn);
+                 }
                  cfr(d, NATRAY_F_U + dim, cell) = u;
                }
              });
