@@ -52,6 +52,30 @@ def few_solve_groups(name, groups=2):
     return make
 
 
+def dense_sphere(per_cell, extra_cell=0):
+    """The water sphere's 4,096 cells on 64^3 with `per_cell` particles each (copies of the
+    seeded fill at other seeds), plus `extra_cell` more in one cell. Full blocks of more than
+    606 particles take P2G's 32-bit atomics, of more than 2,048 its unordered splat, and a cell
+    of more than 64 particles its splat in stored order."""
+    def make():
+        s = scenes.water_sphere()
+        parts = [s.particles] + [scenes.sphere_particles(64, 32768, seed=100 + k) for k in range(1, per_cell // 8)]
+        if extra_cell:
+            P = np.zeros((extra_cell, abi.PARTICLE_FLOATS), np.float32)
+            u = scenes.uniform01(77, 3 * extra_cell).reshape(-1, 3)
+            P[:, :3] = ((np.array([32.0, 32.0, 32.0]) + u) / 64.0).astype(np.float32)
+            P[:, 3] = 1.0
+            P[:, 7] = 1.0
+            parts.append(P)
+        s.particles = np.concatenate(parts)
+        s.params.dims[3] = s.n
+        return s
+    return make
+
+
+SCENES["sphere_16ppc"] = dense_sphere(16)
+SCENES["sphere_40ppc"] = dense_sphere(40)
+SCENES["sphere_crowded_cell"] = dense_sphere(8, extra_cell=72)
 SCENES["sphere_jacobi_32k_mem"] = few_solve_groups("sphere_jacobi_32k")
 SCENES["impact_mixed_32k_mem"] = few_solve_groups("impact_mixed_32k")
 SCENES["impact_pipelined_32k_mem"] = few_solve_groups("impact_pipelined_32k")
