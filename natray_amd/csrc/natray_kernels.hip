@@ -3480,7 +3480,7 @@ __device__ __forceinline__ float g2p_one(const Dev& d, const float* tg, const ui
 }
 
 template <class L, bool CAV = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(L l) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_g2p(L l) {
   const View v = view(l);
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
