@@ -796,7 +796,7 @@ constexpr uint32_t P2G_MAXR = 64;
 // instruction hold particles of different cells (their stencils, not the same words), and no
 // lane idles while a crowded cell still has particles. The per-cell counts are A_N.
 template <class L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_p2g(L l) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_p2g(L l) {
   const View v = view(l);
   const Dev& d = v.d;
   if (d.st->dt_ns == 0u) return;
